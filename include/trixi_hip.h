@@ -1,0 +1,188 @@
+/*
+ * trixi_hip.h -- C ABI of the MI355X DGSEM right-hand side for the compressible
+ * Euler equations on a conforming periodic TreeMesh (2D and 3D).
+ *
+ * The library replaces the CPU methods that Trixi.jl dispatches from
+ *   rhs_hyperbolic!(du_ode, u_ode, semi, t)
+ *     src/semidiscretization/semidiscretization_hyperbolic.jl:578-597
+ *   -> rhs_hyperbolic!(backend, du, u, t, mesh::TreeMesh{2,3}, equations,
+ *                      boundary_conditions, source_terms, dg, cache)
+ *     src/solvers/dgsem_tree/dg_2d.jl:113-186
+ * A Julia method for a marker backend ccalls these entry points; everything
+ * above it (ODEProblem, the 2N Runge-Kutta integrators, Stepsize/Analysis
+ * callbacks) is unchanged.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative TRIXI_HIP_E* code on
+ *    failure; trixi_hip_last_error(ctx) describes the last failure.
+ *  - State vectors use the reference layout of u_ode (dg.jl:1169-1212):
+ *    column-major [nvars, nnodes^ndims, nelements], Float64 at the boundary.
+ *  - Element ids, interface ids and orientations are 0-based here (the Julia
+ *    shim subtracts 1).
+ *  - One context per host thread; a context owns one HIP stream (or uses the
+ *    stream set by trixi_hip_set_stream) and its device memory. Not
+ *    re-entrant. The library never calls exit().
+ */
+#ifndef TRIXI_HIP_H
+#define TRIXI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define TRIXI_HIP_OK 0
+#define TRIXI_HIP_EINVAL (-1)   /* bad argument / unsupported configuration */
+#define TRIXI_HIP_EHIP (-2)     /* a HIP runtime call failed */
+#define TRIXI_HIP_ENOMEM (-3)   /* device allocation failed */
+#define TRIXI_HIP_ESTATE (-4)   /* call not valid in the context's state */
+
+/* precision of the bulk flux arithmetic */
+#define TRIXI_HIP_FP64 0 /* plain FP64 everywhere */
+#define TRIXI_HIP_FP32 1 /* FP32 fluxes, FP64 RK registers and reductions */
+
+/* volume integral (src/solvers/dg.jl:135-221, 574-582) */
+#define TRIXI_HIP_VI_WEAK 0               /* VolumeIntegralWeakForm */
+#define TRIXI_HIP_VI_FLUX_DIFFERENCING 1  /* VolumeIntegralFluxDifferencing(volume_flux) */
+#define TRIXI_HIP_VI_SHOCK_CAPTURING_HG 2 /* VolumeIntegralShockCapturingHG(indicator; volume_flux_dg, volume_flux_fv) */
+#define TRIXI_HIP_VI_PURE_FV 3            /* VolumeIntegralPureLGLFiniteVolume(volume_flux_fv) */
+
+/* numerical flux (src/equations/numerical_fluxes.jl, compressible_euler_{2,3}d.jl) */
+#define TRIXI_HIP_FLUX_LLF 0       /* FluxLaxFriedrichs() = flux_lax_friedrichs (max_abs_speed) */
+#define TRIXI_HIP_FLUX_LLF_NAIVE 1 /* FluxLaxFriedrichs(max_abs_speed_naive) */
+#define TRIXI_HIP_FLUX_RANOCHA 2   /* flux_ranocha */
+#define TRIXI_HIP_FLUX_CENTRAL 3   /* flux_central */
+
+/* source terms */
+#define TRIXI_HIP_SRC_NONE 0
+#define TRIXI_HIP_SRC_CONVERGENCE_TEST 1 /* source_terms_convergence_test */
+
+/* initial conditions known to the device (for error norms, exact solutions) */
+#define TRIXI_HIP_IC_CONSTANT 0
+#define TRIXI_HIP_IC_CONVERGENCE_TEST 1
+#define TRIXI_HIP_IC_WEAK_BLAST_WAVE 2
+#define TRIXI_HIP_IC_ISENTROPIC_VORTEX 3
+#define TRIXI_HIP_IC_KELVIN_HELMHOLTZ 4
+#define TRIXI_HIP_IC_FOURIER_MODES 5
+
+/* indicator modes for trixi_hip_indicator */
+#define TRIXI_HIP_IND_HG 0 /* IndicatorHennemannGassner */
+
+/* where the pointers passed to trixi_hip_rhs live */
+#define TRIXI_HIP_HOST 0
+#define TRIXI_HIP_DEVICE 1
+
+typedef struct trixi_hip_config {
+  int32_t ndims;           /* 2 or 3 */
+  int32_t polydeg;         /* N, nnodes = N + 1 (supported: 1..7) */
+  int32_t nelements;       /* E */
+  int32_t ninterfaces;     /* I */
+  double gamma;            /* CompressibleEulerEquations{2,3}D(gamma) */
+  int32_t precision;       /* TRIXI_HIP_FP64 | TRIXI_HIP_FP32 */
+  int32_t volume_integral; /* TRIXI_HIP_VI_* */
+  int32_t volume_flux;     /* TRIXI_HIP_FLUX_RANOCHA | TRIXI_HIP_FLUX_CENTRAL (FD and HG high order part) */
+  int32_t surface_flux;    /* TRIXI_HIP_FLUX_* used at interfaces */
+  int32_t fv_flux;         /* TRIXI_HIP_FLUX_* used for the FV subcells (HG, pure FV) */
+  double alpha_max;        /* IndicatorHennemannGassner(alpha_max, alpha_min, alpha_smooth) */
+  double alpha_min;
+  int32_t alpha_smooth;
+  int32_t source_terms;    /* TRIXI_HIP_SRC_* */
+  int32_t elements_per_block; /* 0 = autotuned default; else workgroup tile (elements) */
+  int32_t threads_per_block;  /* 0 = autotuned default */
+} trixi_hip_config;
+
+/* Mesh data as held in the reference cache (containers_{2,3}d.jl). */
+typedef struct trixi_hip_mesh {
+  const int32_t *neighbor_ids;     /* [2, I] interfaces.neighbor_ids (left, right), 0-based */
+  const int32_t *orientations;     /* [I] interfaces.orientations, 0-based (x=0, y=1, z=2) */
+  const double *inverse_jacobian;  /* [E] elements.inverse_jacobian */
+  const double *element_centers;   /* [ndims, E] tree.coordinates[:, elements.cell_ids] */
+} trixi_hip_mesh;
+
+/* LobattoLegendreBasis fields (basis_lobatto_legendre.jl:17-86), row-major
+ * matrices M[r*n + c] = M[r, c] (r, c 0-based). The analysis basis is the
+ * SolutionAnalyzer (basis_lobatto_legendre.jl:274-290). */
+typedef struct trixi_hip_basis {
+  const double *nodes;                        /* [n] */
+  const double *weights;                      /* [n] */
+  const double *derivative_split;             /* [n, n] Dsplit */
+  const double *derivative_hat;               /* [n, n] Dhat */
+  const double *inverse_vandermonde_legendre; /* [n, n] */
+  int32_t analysis_nnodes;                    /* na (2N + 1) */
+  const double *analysis_weights;             /* [na] */
+  const double *analysis_vandermonde;         /* [na, n] interpolation matrix */
+} trixi_hip_basis;
+
+typedef struct trixi_hip_ctx trixi_hip_ctx;
+
+/* Build a context: copies mesh and basis to the device and allocates the
+ * resident state. Replaces create_cache(mesh::TreeMesh, ...) (dg_2d.jl:14-37)
+ * for the device. E and I are fixed for the context's lifetime. */
+int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
+                     const trixi_hip_basis *basis, trixi_hip_ctx **out);
+int trixi_hip_destroy(trixi_hip_ctx *ctx);
+const char *trixi_hip_last_error(const trixi_hip_ctx *ctx);
+
+/* Use an external HIP stream (e.g. torch.cuda.current_stream().cuda_stream);
+ * NULL restores the context's own stream. */
+int trixi_hip_set_stream(trixi_hip_ctx *ctx, void *hip_stream);
+int trixi_hip_synchronize(trixi_hip_ctx *ctx);
+
+/* Resident state u (FP64 RK register; in FP32 mode also its FP32 shadow). */
+int trixi_hip_upload_u(trixi_hip_ctx *ctx, const double *u_host);
+int trixi_hip_download_u(trixi_hip_ctx *ctx, double *u_host);
+int trixi_hip_set_u_device(trixi_hip_ctx *ctx, const double *u_dev);
+int trixi_hip_get_u_device(trixi_hip_ctx *ctx, double *u_dev);
+
+/* rhs_hyperbolic!(du, u, t) -- semidiscretization_hyperbolic.jl:578-597 and
+ * dg_2d.jl:113-186. FP64 arrays in the reference layout, host or device
+ * (where = TRIXI_HIP_HOST | TRIXI_HIP_DEVICE). In FP32 mode u is rounded to
+ * FP32, the fluxes run in FP32 and du is widened to FP64. */
+int trixi_hip_rhs(trixi_hip_ctx *ctx, double *du, const double *u, double t, int where);
+
+/* The same operator on device arrays in the context's native precision
+ * (float in FP32 mode, double in FP64 mode); no conversion, no allocation. */
+int trixi_hip_rhs_native(trixi_hip_ctx *ctx, void *du_dev, const void *u_dev, double t);
+
+/* One 2N low-storage stage on the resident state (methods_2N.jl:146-159):
+ *   du = rhs(u, t_stage); u_tmp = du - a_stage*u_tmp; u += b_stage_dt*u_tmp
+ * u and u_tmp are FP64. first_stage != 0 zeroes u_tmp first (methods_2N.jl:143). */
+int trixi_hip_rk_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage_dt,
+                       double t_stage, int first_stage);
+
+/* One full step of a 2N scheme on the resident state (step!, methods_2N.jl:131-168),
+ * replayed from a captured hipGraph when use_graph != 0. */
+int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, const double *c,
+                      int nstages, double t, double dt, int use_graph);
+
+/* max_dt (stepsize_dg{2,3}d.jl:8-32) of the resident state times cfl
+ * (calculate_dt, stepsize.jl:146-155). FP64 reduction. */
+int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt);
+
+/* Blending factors alpha[E] of IndicatorHennemannGassner (indicators.jl:119-156,
+ * indicators_{2,3}d.jl) for the resident state, after smoothing. */
+int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host);
+
+/* Exact solution used by calc_error_norms (analysis_dg{2,3}d.jl). params:
+ * IC_ISENTROPIC_VORTEX {p_background}; IC_FOURIER_MODES {A_rho, A_v} and
+ * modes[4*8*4] = (kx, ky, kz, phase) per field (rho, v1, v2, v3) and mode. */
+int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *params,
+                                 int nparams, const double *modes);
+
+/* calc_error_norms(cons2cons, ...) on the resident state (analysis_dg3d.jl:123-161). */
+int trixi_hip_error_norms(trixi_hip_ctx *ctx, double t, double *l2, double *linf);
+
+/* integrate_via_indices of the conserved variables and of entropy_math,
+ * normalised by the domain volume (analysis_dg3d.jl:364-386). */
+int trixi_hip_integrals(trixi_hip_ctx *ctx, double *cons_integrals, double *entropy_integral);
+
+/* Workgroup tile used by the element kernel: elements per block and threads. */
+int trixi_hip_get_tile(const trixi_hip_ctx *ctx, int *elements_per_block, int *threads_per_block);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRIXI_HIP_H */

@@ -1,0 +1,195 @@
+"""ctypes wrapper of oracle/libtrixi_oracle.so (the C++ FP64 restatement of
+Trixi.jl's TreeMesh DGSEM Euler right-hand side). Test infrastructure only."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtrixi_oracle.so")
+
+VI = {"weak": 0, "fd": 1, "hg": 2, "fv": 3}
+FLUX = {"llf": 0, "llf_naive": 1, "ranocha": 2, "central": 3}
+IC = {"constant": 0, "convergence_test": 1, "weak_blast_wave": 2, "isentropic_vortex": 3,
+      "kelvin_helmholtz": 4, "fourier_modes": 5}
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("ndims", ctypes.c_int), ("polydeg", ctypes.c_int), ("initial_refinement_level", ctypes.c_int),
+                ("coordinates_min", ctypes.c_double * 3), ("coordinates_max", ctypes.c_double * 3),
+                ("gamma", ctypes.c_double), ("volume_integral", ctypes.c_int), ("volume_flux", ctypes.c_int),
+                ("surface_flux", ctypes.c_int), ("fv_flux", ctypes.c_int), ("alpha_max", ctypes.c_double),
+                ("alpha_min", ctypes.c_double), ("alpha_smooth", ctypes.c_int), ("source_terms", ctypes.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        L.oracle_create.restype = P
+        L.oracle_create.argtypes = [ctypes.POINTER(_Config)]
+        L.oracle_destroy.argtypes = [P]
+        for nm in ("oracle_nelements", "oracle_ninterfaces", "oracle_nnodes"):
+            getattr(L, nm).restype = I
+            getattr(L, nm).argtypes = [P]
+        L.oracle_ndofs.restype = ctypes.c_long
+        L.oracle_ndofs.argtypes = [P]
+        L.oracle_get_mesh.argtypes = [P] + [P] * 6
+        L.oracle_get_basis.argtypes = [P] + [P] * 6
+        L.oracle_set_initial_condition.argtypes = [P, I, P, I, P]
+        L.oracle_compute_coefficients.argtypes = [P, D, P]
+        L.oracle_rhs.argtypes = [P, P, P, D]
+        L.oracle_indicator_hg.argtypes = [P, P, P]
+        L.oracle_max_dt.restype = D
+        L.oracle_max_dt.argtypes = [P, P]
+        L.oracle_error_norms.argtypes = [P, P, D, P, P]
+        L.oracle_integrals.argtypes = [P, P, P, P]
+        L.oracle_rk_step.argtypes = [P, P, P, P, D, D]
+        L.oracle_solve.restype = I
+        L.oracle_solve.argtypes = [P, P, D, D, D, I, P]
+        L.oracle_set_threads.argtypes = [I]
+        L.oracle_get_threads.restype = I
+        L.oracle_ck2n54_coefficients.argtypes = [P, P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    """One oracle discretisation: Oracle(ndims=3, polydeg=3, level=2, cmin=..., cmax=...,
+    volume_integral='fd', volume_flux='ranocha', surface_flux='llf', fv_flux='llf',
+    alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, source='none'|'convergence_test',
+    gamma=1.4)."""
+
+    def __init__(self, ndims, polydeg, level, cmin, cmax, volume_integral="fd", volume_flux="ranocha",
+                 surface_flux="llf", fv_flux="llf", alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
+                 source="none", gamma=1.4):
+        L = _load()
+        c = _Config()
+        c.ndims, c.polydeg, c.initial_refinement_level = ndims, polydeg, level
+        for d in range(ndims):
+            c.coordinates_min[d] = cmin[d]
+            c.coordinates_max[d] = cmax[d]
+        c.gamma = gamma
+        c.volume_integral = VI[volume_integral]
+        c.volume_flux = FLUX[volume_flux]
+        c.surface_flux = FLUX[surface_flux]
+        c.fv_flux = FLUX[fv_flux]
+        c.alpha_max, c.alpha_min, c.alpha_smooth = alpha_max, alpha_min, int(alpha_smooth)
+        c.source_terms = 1 if source == "convergence_test" else 0
+        self._cfg = c
+        self._c = L.oracle_create(ctypes.byref(c))
+        if not self._c:
+            raise ValueError("oracle_create rejected the configuration")
+        self.L = L
+        self.ndims, self.polydeg = ndims, polydeg
+        self.n = polydeg + 1
+        self.nvars = ndims + 2
+        self.nelements = L.oracle_nelements(self._c)
+        self.ninterfaces = L.oracle_ninterfaces(self._c)
+        self.ndofs = L.oracle_ndofs(self._c)
+        self.nvalues = self.ndofs * self.nvars
+
+    def __del__(self):
+        try:
+            if self._c:
+                self.L.oracle_destroy(self._c)
+                self._c = None
+        except Exception:
+            pass
+
+    def mesh(self):
+        E, I, D, NN = self.nelements, self.ninterfaces, self.ndims, self.n ** self.ndims
+        nid = np.empty(2 * I, np.int32)
+        ori = np.empty(I, np.int32)
+        ij = np.empty(E)
+        cen = np.empty(D * E)
+        xc = np.empty(D * NN * E)
+        en = np.empty(2 * D * E, np.int32)
+        self.L.oracle_get_mesh(self._c, _p(nid), _p(ori), _p(ij), _p(cen), _p(xc), _p(en))
+        return dict(neighbor_ids=nid.reshape(I, 2), orientations=ori, inverse_jacobian=ij,
+                    centers=cen.reshape(E, D), node_coordinates=xc.reshape(E, NN, D),
+                    element_neighbors=en.reshape(E, 2 * D))
+
+    def basis(self):
+        n = self.n
+        out = [np.empty(n), np.empty(n), np.empty(n * n), np.empty(n * n), np.empty(n * n), np.empty(n * n)]
+        self.L.oracle_get_basis(self._c, *[_p(a) for a in out])
+        return dict(nodes=out[0], weights=out[1], D=out[2].reshape(n, n), Dsplit=out[3].reshape(n, n),
+                    Dhat=out[4].reshape(n, n), Vinv=out[5].reshape(n, n))
+
+    def set_initial_condition(self, kind, params=(), modes=None):
+        pa = np.array(list(params) + [0.0] * 8, dtype=np.float64)[:8]
+        mo = np.ascontiguousarray(modes, dtype=np.float64) if modes is not None else np.zeros(128)
+        self._ic_keep = (pa, mo)
+        self.L.oracle_set_initial_condition(self._c, IC[kind], _p(pa), len(params), _p(mo))
+
+    def compute_coefficients(self, t=0.0):
+        u = np.empty(self.nvalues)
+        self.L.oracle_compute_coefficients(self._c, float(t), _p(u))
+        return u
+
+    def rhs(self, u, t=0.0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        du = np.empty_like(u)
+        self.L.oracle_rhs(self._c, _p(du), _p(u), float(t))
+        return du
+
+    def indicator_hg(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        a = np.empty(self.nelements)
+        self.L.oracle_indicator_hg(self._c, _p(u), _p(a))
+        return a
+
+    def max_dt(self, u, cfl=1.0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return cfl * self.L.oracle_max_dt(self._c, _p(u))
+
+    def error_norms(self, u, t):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        l2, li = np.empty(self.nvars), np.empty(self.nvars)
+        self.L.oracle_error_norms(self._c, _p(u), float(t), _p(l2), _p(li))
+        return l2, li
+
+    def integrals(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        cons = np.empty(self.nvars)
+        ent = np.empty(1)
+        self.L.oracle_integrals(self._c, _p(u), _p(cons), _p(ent))
+        return cons, float(ent[0])
+
+    def rk_step(self, u, t, dt):
+        du = np.empty_like(u)
+        ut = np.empty_like(u)
+        self.L.oracle_rk_step(self._c, _p(u), _p(du), _p(ut), float(t), float(dt))
+        return u
+
+    def solve(self, u, t0, t_end, cfl, maxiters=0):
+        u = np.array(u, dtype=np.float64, copy=True)
+        tout = np.empty(1)
+        nsteps = self.L.oracle_solve(self._c, _p(u), float(t0), float(t_end), float(cfl), int(maxiters), _p(tout))
+        if nsteps < 0:
+            raise FloatingPointError("time step size `dt` is NaN")
+        return u, float(tout[0]), nsteps
+
+    @staticmethod
+    def set_threads(n):
+        _load().oracle_set_threads(int(n))
+
+    @staticmethod
+    def get_threads():
+        return _load().oracle_get_threads()

@@ -1,0 +1,1238 @@
+// trixi_oracle.cpp -- FP64 CPU restatement of Trixi.jl's DGSEM right-hand side
+// for the compressible Euler equations on a uniform periodic TreeMesh (2D, 3D).
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP library: the
+// tests, the smoke check and the CPU-baseline leg of bench.py call it; the
+// product path never links, loads or calls it.
+//
+// Each routine follows the Julia reference loop by loop (file:line cited at
+// each function, paths relative to the Trixi.jl source tree). Arithmetic is
+// IEEE double throughout; `@muladd` contraction in the reference means results
+// agree with Trixi to round-off (~1e-15 relative), not bitwise.
+//
+// Layout (Julia column-major, first index fastest), 0-based here:
+//   u[v + V*(node + NN*e)],  node = i + n*j (+ n*n*k)
+//   surface_flux_values sfv[v + V*(face_node + nf*(dir + 2d*e))]
+//   face_node = l + n*m with (l,m) the two in-face indices in reference order.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// LGL basis -- src/solvers/dgsem/basis_lobatto_legendre.jl
+// ---------------------------------------------------------------------------
+
+// calc_q_and_l, basis_lobatto_legendre.jl:642-667 (Kopriva alg. 24)
+void calc_q_and_l(int N, double x, double &q, double &qder, double &L) {
+    double L_Nm2 = 1.0, L_Nm1 = x, Lder_Nm2 = 0.0, Lder_Nm1 = 1.0;
+    L = x;  // N == 1 never reaches here with an unset L in practice
+    for (int i = 2; i <= N; ++i) {
+        L = ((2 * i - 1) * x * L_Nm1 - (i - 1) * L_Nm2) / i;
+        double Lder = Lder_Nm2 + (2 * i - 1) * L_Nm1;
+        L_Nm2 = L_Nm1;
+        L_Nm1 = L;
+        Lder_Nm2 = Lder_Nm1;
+        Lder_Nm1 = Lder;
+    }
+    q = (2.0 * N + 1) / (N + 1) * (x * L - L_Nm2);
+    qder = (2.0 * N + 1) * L;
+}
+
+// gauss_lobatto_nodes_weights, basis_lobatto_legendre.jl:570-640 (Kopriva alg. 25)
+void gauss_lobatto_nodes_weights(int n_nodes, std::vector<double> &nodes,
+                                 std::vector<double> &weights) {
+    const int n_iterations = 20;
+    const double tolerance = 2 * 2.220446049250313e-16;
+    nodes.assign(n_nodes, 0.0);
+    weights.assign(n_nodes, 0.0);
+    if (n_nodes == 1) {
+        nodes[0] = 0;
+        weights[0] = 2;
+        return;
+    }
+    int N = n_nodes - 1;
+    nodes[0] = -1;
+    nodes[N] = 1;
+    weights[0] = 2.0 / (N * (N + 1));
+    weights[N] = weights[0];
+    if (N > 1) {
+        double cont1 = M_PI / N;
+        double cont2 = 3.0 / (8 * N * M_PI);
+        for (int i = 1; i <= (N + 1) / 2 - 1; ++i) {
+            nodes[i] = -std::cos(cont1 * (i + 0.25) - cont2 / (i + 0.25));
+            for (int k = 0; k <= n_iterations; ++k) {
+                double q, qder, L;
+                calc_q_and_l(N, nodes[i], q, qder, L);
+                double dx = -q / qder;
+                nodes[i] += dx;
+                if (std::fabs(dx) < tolerance * std::fabs(nodes[i])) break;
+            }
+            double q, qder, L;
+            calc_q_and_l(N, nodes[i], q, qder, L);
+            weights[i] = weights[0] / (L * L);
+            nodes[N - i] = -nodes[i];
+            weights[N - i] = weights[i];
+        }
+    }
+    if (n_nodes % 2 == 1) {
+        double q, qder, L;
+        calc_q_and_l(N, 0.0, q, qder, L);
+        nodes[N / 2] = 0;
+        weights[N / 2] = weights[0] / (L * L);
+    }
+}
+
+// barycentric_weights, basis_lobatto_legendre.jl:500-513
+std::vector<double> barycentric_weights(const std::vector<double> &nodes) {
+    int n = (int)nodes.size();
+    std::vector<double> w(n, 1.0);
+    for (int j = 1; j < n; ++j)
+        for (int k = 0; k < j; ++k) {
+            w[k] *= nodes[k] - nodes[j];
+            w[j] *= nodes[j] - nodes[k];
+        }
+    for (int j = 0; j < n; ++j) w[j] = 1.0 / w[j];
+    return w;
+}
+
+// polynomial_derivative_matrix, basis_lobatto_legendre.jl:427-440 (Kopriva alg. 37)
+// Row-major: D[i*n + j] = D[i, j].
+std::vector<double> polynomial_derivative_matrix(const std::vector<double> &nodes) {
+    int n = (int)nodes.size();
+    std::vector<double> D(n * n, 0.0);
+    auto wbary = barycentric_weights(nodes);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if (j != i) {
+                D[i * n + j] = (wbary[j] / wbary[i]) * 1.0 / (nodes[i] - nodes[j]);
+                D[i * n + i] -= D[i * n + j];
+            }
+    return D;
+}
+
+// legendre_polynomial_and_derivative (normalized), basis_lobatto_legendre.jl:677-708
+double legendre_normalized(int N, double x) {
+    double poly;
+    if (N == 0) poly = 1.0;
+    else if (N == 1) poly = x;
+    else {
+        double p2 = 1.0, p1 = x;
+        poly = 0.0;
+        for (int i = 2; i <= N; ++i) {
+            poly = ((2 * i - 1) * x * p1 - (i - 1) * p2) / i;
+            p2 = p1;
+            p1 = poly;
+        }
+    }
+    return poly * std::sqrt(N + 0.5);
+}
+
+// Dense inverse by Gauss-Jordan with partial pivoting (the reference calls
+// Julia's `inv`, basis_lobatto_legendre.jl:711-727).
+std::vector<double> invert(const std::vector<double> &A, int n) {
+    std::vector<double> M(A), I(n * n, 0.0);
+    for (int i = 0; i < n; ++i) I[i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(M[r * n + c]) > std::fabs(M[p * n + c])) p = r;
+        if (p != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(M[c * n + k], M[p * n + k]);
+                std::swap(I[c * n + k], I[p * n + k]);
+            }
+        double piv = M[c * n + c];
+        for (int k = 0; k < n; ++k) {
+            M[c * n + k] /= piv;
+            I[c * n + k] /= piv;
+        }
+        for (int r = 0; r < n; ++r)
+            if (r != c) {
+                double f = M[r * n + c];
+                if (f != 0.0)
+                    for (int k = 0; k < n; ++k) {
+                        M[r * n + k] -= f * M[c * n + k];
+                        I[r * n + k] -= f * I[c * n + k];
+                    }
+            }
+    }
+    return I;
+}
+
+// polynomial_interpolation_matrix, basis_lobatto_legendre.jl:456-484 (Kopriva alg. 32)
+std::vector<double> interpolation_matrix(const std::vector<double> &nodes_in,
+                                         const std::vector<double> &nodes_out) {
+    int nin = (int)nodes_in.size(), nout = (int)nodes_out.size();
+    auto wb = barycentric_weights(nodes_in);
+    std::vector<double> V(nout * nin, 0.0);
+    const double rtol = 1.4901161193847656e-08;  // isapprox default sqrt(eps)
+    for (int k = 0; k < nout; ++k) {
+        bool match = false;
+        for (int j = 0; j < nin; ++j) {
+            double a = nodes_out[k], b = nodes_in[j];
+            if (a == b || std::fabs(a - b) <= rtol * std::max(std::fabs(a), std::fabs(b))) {
+                match = true;
+                V[k * nin + j] = 1;
+            }
+        }
+        if (!match) {
+            double s = 0;
+            for (int j = 0; j < nin; ++j) {
+                double t = wb[j] / (nodes_out[k] - nodes_in[j]);
+                V[k * nin + j] = t;
+                s += t;
+            }
+            for (int j = 0; j < nin; ++j) V[k * nin + j] /= s;
+        }
+    }
+    return V;
+}
+
+struct Basis {
+    int n = 0;
+    std::vector<double> nodes, weights, inv_weights, D, Dsplit, Dhat, Vinv;
+    // analysis basis (analysis_polydeg = 2*polydeg, basis_lobatto_legendre.jl:274-290)
+    int na = 0;
+    std::vector<double> a_nodes, a_weights, a_vdm;  // a_vdm[k*n + j], na x n
+
+    void init(int polydeg) {
+        n = polydeg + 1;
+        gauss_lobatto_nodes_weights(n, nodes, weights);
+        inv_weights.resize(n);
+        for (int i = 0; i < n; ++i) inv_weights[i] = 1.0 / weights[i];
+        std::vector<double> V(n * n);
+        for (int i = 0; i < n; ++i)
+            for (int m = 0; m < n; ++m) V[i * n + m] = legendre_normalized(m, nodes[i]);
+        Vinv = invert(V, n);
+        D = polynomial_derivative_matrix(nodes);
+        // calc_Dsplit, basis_lobatto_legendre.jl:414-424
+        Dsplit.resize(n * n);
+        for (int k = 0; k < n * n; ++k) Dsplit[k] = 2.0 * D[k];
+        Dsplit[0] += 1.0 / weights[0];
+        Dsplit[n * n - 1] -= 1.0 / weights[n - 1];
+        // calc_Dhat, basis_lobatto_legendre.jl:399-411: Dhat = D^T, scaled by -w[c]/w[r]
+        Dhat.resize(n * n);
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) Dhat[r * n + c] = D[c * n + r];
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) Dhat[r * n + c] *= -weights[c] / weights[r];
+        na = 2 * polydeg + 1;
+        gauss_lobatto_nodes_weights(na, a_nodes, a_weights);
+        a_vdm = interpolation_matrix(nodes, a_nodes);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Equations -- src/equations/compressible_euler_{2,3}d.jl, numerical_fluxes.jl,
+// src/auxiliary/math.jl. Templated on the number of space dimensions D
+// (V = D + 2 conserved variables: rho, rho v_1..rho v_D, rho e_total).
+// ---------------------------------------------------------------------------
+
+// ln_mean, math.jl:198-210
+inline double ln_mean(double x, double y) {
+    const double epsilon_f2 = 1.0e-4;
+    double f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y);
+    if (f2 < epsilon_f2)
+        return (x + y) / (2 + f2 * (2.0 / 3 + f2 * (2.0 / 5 + f2 * (2.0 / 7))));
+    return (y - x) / std::log(y / x);
+}
+
+// inv_ln_mean, math.jl:238-250
+inline double inv_ln_mean(double x, double y) {
+    const double epsilon_f2 = 1.0e-4;
+    double f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y);
+    if (f2 < epsilon_f2)
+        return (2 + f2 * (2.0 / 3 + f2 * (2.0 / 5 + f2 * (2.0 / 7)))) / (x + y);
+    return std::log(y / x) / (y - x);
+}
+
+template <int D> struct Euler {
+    static constexpr int V = D + 2;
+    double gamma, inv_gamma_minus_one;
+    explicit Euler(double g = 1.4) : gamma(g), inv_gamma_minus_one(1.0 / (g - 1)) {}
+
+    // cons2prim: euler_3d.jl:1783-1794, euler_2d.jl:2038-2046
+    void cons2prim(const double *u, double *w) const {
+        double rho = u[0];
+        double ke2 = 0;
+        w[0] = rho;
+        for (int d = 0; d < D; ++d) {
+            w[1 + d] = u[1 + d] / rho;
+            ke2 += u[1 + d] * w[1 + d];
+        }
+        w[V - 1] = (gamma - 1) * (u[V - 1] - 0.5 * ke2);
+    }
+
+    // flux: euler_3d.jl:420-447, euler_2d.jl:415-434
+    void flux(const double *u, int o, double *f) const {
+        double w[V];
+        cons2prim(u, w);
+        double p = w[V - 1];
+        f[0] = u[1 + o];
+        for (int d = 0; d < D; ++d) f[1 + d] = u[1 + o] * w[1 + d];
+        f[1 + o] += p;
+        f[V - 1] = (u[V - 1] + p) * w[1 + o];
+    }
+
+    // flux_central: numerical_fluxes.jl:17-25
+    void flux_central(const double *ul, const double *ur, int o, double *f) const {
+        double fl[V], fr[V];
+        flux(ul, o, fl);
+        flux(ur, o, fr);
+        for (int v = 0; v < V; ++v) f[v] = 0.5 * (fl[v] + fr[v]);
+    }
+
+    // flux_ranocha: euler_3d.jl:746-793, euler_2d.jl:686-722
+    void flux_ranocha(const double *ul, const double *ur, int o, double *f) const {
+        double wl[V], wr[V];
+        cons2prim(ul, wl);
+        cons2prim(ur, wr);
+        double rho_ll = wl[0], rho_rr = wr[0], p_ll = wl[V - 1], p_rr = wr[V - 1];
+        double rho_mean = ln_mean(rho_ll, rho_rr);
+        double inv_rho_p_mean = p_ll * p_rr * inv_ln_mean(rho_ll * p_rr, rho_rr * p_ll);
+        double vavg[D];
+        double velocity_square_avg_sum = 0;
+        for (int d = 0; d < D; ++d) {
+            vavg[d] = 0.5 * (wl[1 + d] + wr[1 + d]);
+            velocity_square_avg_sum += wl[1 + d] * wr[1 + d];
+        }
+        double p_avg = 0.5 * (p_ll + p_rr);
+        double velocity_square_avg = 0.5 * velocity_square_avg_sum;
+        double f1 = rho_mean * vavg[o];
+        f[0] = f1;
+        for (int d = 0; d < D; ++d) f[1 + d] = f1 * vavg[d];
+        f[1 + o] += p_avg;
+        f[V - 1] = f1 * (velocity_square_avg + inv_rho_p_mean * inv_gamma_minus_one) +
+                   0.5 * (p_ll * wr[1 + o] + p_rr * wl[1 + o]);
+    }
+
+    // max_abs_speed_naive: euler_3d.jl:1112-1133, euler_2d.jl:1423-1440
+    double max_abs_speed_naive(const double *ul, const double *ur, int o) const {
+        double wl[V], wr[V];
+        cons2prim(ul, wl);
+        cons2prim(ur, wr);
+        double c_ll = std::sqrt(gamma * wl[V - 1] / wl[0]);
+        double c_rr = std::sqrt(gamma * wr[V - 1] / wr[0]);
+        return std::max(std::fabs(wl[1 + o]), std::fabs(wr[1 + o])) + std::max(c_ll, c_rr);
+    }
+
+    // max_abs_speed: euler_3d.jl:1156-1177, euler_2d.jl:1464-1482
+    double max_abs_speed(const double *ul, const double *ur, int o) const {
+        double wl[V], wr[V];
+        cons2prim(ul, wl);
+        cons2prim(ur, wr);
+        double c_ll = std::sqrt(gamma * wl[V - 1] / wl[0]);
+        double c_rr = std::sqrt(gamma * wr[V - 1] / wr[0]);
+        return std::max(std::fabs(wl[1 + o]) + c_ll, std::fabs(wr[1 + o]) + c_rr);
+    }
+
+    // FluxLaxFriedrichs = FluxPlusDissipation(flux_central, DissipationLocalLaxFriedrichs)
+    // numerical_fluxes.jl:37-45, 166-178, 229-253
+    void flux_llf(const double *ul, const double *ur, int o, bool naive, double *f) const {
+        double fc[V];
+        flux_central(ul, ur, o, fc);
+        double lambda = naive ? max_abs_speed_naive(ul, ur, o) : max_abs_speed(ul, ur, o);
+        for (int v = 0; v < V; ++v) f[v] = fc[v] + (-0.5 * lambda * (ur[v] - ul[v]));
+    }
+
+    // max_abs_speeds: euler_3d.jl:1770-1775, euler_2d.jl:2025-2030
+    void max_abs_speeds(const double *u, double *lam) const {
+        double w[V];
+        cons2prim(u, w);
+        double c = std::sqrt(gamma * w[V - 1] / w[0]);
+        for (int d = 0; d < D; ++d) lam[d] = std::fabs(w[1 + d]) + c;
+    }
+
+    // density_pressure: euler_3d.jl:1951-1956, euler_2d.jl:2205-2210
+    double density_pressure(const double *u) const {
+        double m2 = 0;
+        for (int d = 0; d < D; ++d) m2 += u[1 + d] * u[1 + d];
+        return (gamma - 1) * (u[0] * u[V - 1] - 0.5 * m2);
+    }
+
+    // entropy_math: euler_3d.jl:1970-1976 (S = -rho s / (gamma-1), s = log p - gamma log rho)
+    double entropy_math(const double *u) const {
+        double m2 = 0;
+        for (int d = 0; d < D; ++d) m2 += u[1 + d] * u[1 + d];
+        double p = (gamma - 1) * (u[V - 1] - 0.5 * m2 / u[0]);
+        double s = std::log(p) - gamma * std::log(u[0]);
+        return -s * u[0] * inv_gamma_minus_one;
+    }
+
+    // cons2entropy: euler_3d.jl (w = ((gamma - s)/(gamma-1) - 0.5 rho/p |v|^2, rho/p v, -rho/p))
+    void cons2entropy(const double *u, double *wout) const {
+        double w[V];
+        cons2prim(u, w);
+        double v2 = 0;
+        for (int d = 0; d < D; ++d) v2 += w[1 + d] * w[1 + d];
+        double p = w[V - 1];
+        double s = std::log(p) - gamma * std::log(w[0]);
+        double rho_p = w[0] / p;
+        wout[0] = (gamma - s) * inv_gamma_minus_one - 0.5 * rho_p * v2;
+        for (int d = 0; d < D; ++d) wout[1 + d] = rho_p * w[1 + d];
+        wout[V - 1] = -rho_p;
+    }
+
+    // prim2cons: euler_2d.jl:2120-2127 (and the 3D twin)
+    void prim2cons(const double *w, double *u) const {
+        u[0] = w[0];
+        double ke2 = 0;
+        for (int d = 0; d < D; ++d) {
+            u[1 + d] = w[0] * w[1 + d];
+            ke2 += u[1 + d] * w[1 + d];
+        }
+        u[V - 1] = w[V - 1] * inv_gamma_minus_one + 0.5 * ke2;
+    }
+};
+
+// Numerical-flux selector shared by the volume (two-point), interface and FV paths.
+enum FluxKind { FLUX_LLF = 0, FLUX_LLF_NAIVE = 1, FLUX_RANOCHA = 2, FLUX_CENTRAL = 3 };
+
+template <int D>
+inline void num_flux(const Euler<D> &eq, int kind, const double *ul, const double *ur, int o,
+                     double *f) {
+    switch (kind) {
+        case FLUX_LLF: eq.flux_llf(ul, ur, o, false, f); break;
+        case FLUX_LLF_NAIVE: eq.flux_llf(ul, ur, o, true, f); break;
+        case FLUX_RANOCHA: eq.flux_ranocha(ul, ur, o, f); break;
+        default: eq.flux_central(ul, ur, o, f); break;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Public C API
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+// Volume-integral selector (src/solvers/dg.jl:135-221, 574-582).
+enum { VI_WEAK = 0, VI_FLUX_DIFFERENCING = 1, VI_SHOCK_CAPTURING_HG = 2, VI_PURE_FV = 3 };
+// Source-term selector.
+enum { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1 };
+// Initial-condition selector.
+enum {
+    IC_CONSTANT = 0,
+    IC_CONVERGENCE_TEST = 1,
+    IC_WEAK_BLAST_WAVE = 2,
+    IC_ISENTROPIC_VORTEX = 3,
+    IC_KELVIN_HELMHOLTZ = 4,
+    IC_FOURIER_MODES = 5
+};
+
+struct oracle_config {
+    int ndims;
+    int polydeg;
+    int initial_refinement_level;
+    double coordinates_min[3];
+    double coordinates_max[3];
+    double gamma;
+    int volume_integral;  // VI_*
+    int volume_flux;      // FluxKind for flux differencing / HG high-order part
+    int surface_flux;     // FluxKind for interfaces
+    int fv_flux;          // FluxKind for the FV subcell fluxes (HG / pure FV)
+    double alpha_max;
+    double alpha_min;
+    int alpha_smooth;
+    int source_terms;     // SRC_*
+};
+
+struct oracle_ctx;
+
+}  // extern "C"
+
+struct oracle_ctx {
+    oracle_config cfg;
+    int D, n, NN, nf, V, E, I;
+    Basis basis;
+    // mesh (src/meshes/abstract_tree.jl, containers_{2,3}d.jl)
+    std::vector<double> centers;        // [D, E]
+    std::vector<double> lengths;        // [E]
+    std::vector<double> inverse_jacobian;  // [E]
+    std::vector<double> node_coords;    // [D, NN, E]
+    std::vector<int32_t> neighbor_ids;  // [2, I] 0-based element ids
+    std::vector<int32_t> orientations;  // [I] 0-based orientation
+    std::vector<int32_t> elem_nbr;      // [2D, E] neighbour element per direction (0-based dir)
+    std::vector<double> sfv;            // surface_flux_values
+    std::vector<double> alpha, alpha_tmp;
+    // analysis exact-solution parameters
+    int ic_kind = IC_CONSTANT;
+    double ic_params[8] = {0};
+    std::vector<double> ic_modes;       // IC_FOURIER_MODES: [4 fields][8 modes][kx,ky,kz,phase]
+    double domain_length = 0;
+};
+
+namespace {
+
+// Uniform refinement in depth-first order: refine_uniformly!, abstract_tree.jl:206-250;
+// child_coordinates :668-674; child_sign table :118-128; leaves in id order :154-166.
+void build_leaves(int D, int level, int max_level, const double *center, double length,
+                  const int *icoord, std::vector<double> &centers, std::vector<double> &lengths,
+                  std::vector<int> &icoords) {
+    if (level >= max_level) {
+        for (int d = 0; d < D; ++d) centers.push_back(center[d]);
+        lengths.push_back(length);
+        for (int d = 0; d < D; ++d) icoords.push_back(icoord[d]);
+        return;
+    }
+    int nchild = 1 << D;
+    for (int child = 0; child < nchild; ++child) {
+        double cc[3];
+        int ic[3];
+        double child_length = length / 2;
+        for (int d = 0; d < D; ++d) {
+            int sgn = ((child >> d) & 1) ? 1 : -1;  // child_sign(child, d)
+            cc[d] = center[d] + sgn * child_length / 2;
+            ic[d] = 2 * icoord[d] + (sgn > 0 ? 1 : 0);
+        }
+        build_leaves(D, level + 1, max_level, cc, child_length, ic, centers, lengths, icoords);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Initial conditions (host-side setup) -- cited per case.
+// ---------------------------------------------------------------------------
+template <int D>
+void initial_condition(const oracle_ctx &c, const Euler<D> &eq, const double *x, double t,
+                       double *u) {
+    const int V = D + 2;
+    switch (c.ic_kind) {
+        case IC_CONSTANT: {
+            // euler_3d.jl:72-80, euler_2d.jl:71-78
+            u[0] = 1;
+            u[1] = 0.1;
+            u[2] = -0.2;
+            if (D == 3) u[3] = 0.7;
+            u[V - 1] = 10;
+            break;
+        }
+        case IC_CONVERGENCE_TEST: {
+            // euler_3d.jl:94-111, euler_2d.jl:87-106
+            double cst = 2, A = 0.1, L = 2, f = 1.0 / L, om = 2 * M_PI * f;
+            double s = 0;
+            for (int d = 0; d < D; ++d) s += x[d];
+            double ini = cst + A * std::sin(om * (s - t));
+            for (int v = 0; v < V - 1; ++v) u[v] = ini;
+            u[V - 1] = ini * ini;
+            break;
+        }
+        case IC_WEAK_BLAST_WAVE: {
+            // euler_3d.jl:163-183 (3D only)
+            double xn = x[0], yn = x[1], zn = D == 3 ? x[2] : 0.0;
+            double r = std::sqrt(xn * xn + yn * yn + zn * zn);
+            double phi = std::atan2(yn, xn);
+            double theta = r == 0 ? 0.0 : std::acos(zn / r);
+            double w[5];
+            w[0] = r > 0.5 ? 1.0 : 1.1691;
+            w[1] = r > 0.5 ? 0.0 : 0.1882 * std::cos(phi) * std::sin(theta);
+            w[2] = r > 0.5 ? 0.0 : 0.1882 * std::sin(phi) * std::sin(theta);
+            w[3] = r > 0.5 ? 0.0 : 0.1882 * std::cos(theta);
+            w[4] = r > 0.5 ? 1.0 : 1.245;
+            eq.prim2cons(w, u);
+            break;
+        }
+        case IC_ISENTROPIC_VORTEX: {
+            // examples/tree_2d_dgsem/elixir_euler_vortex.jl:18-50; background
+            // pressure taken from ic_params[0] (25 in the elixir).
+            double rho = 1, v1 = 1, v2 = 1, p = c.ic_params[0];
+            double iniamplitude = 5;
+            double rt = p / rho;
+            double cx = x[0] - 0.0, cy = x[1] - 0.0;
+            double c0 = -cy, c1 = cx;
+            double r2 = c0 * c0 + c1 * c1;
+            double du = iniamplitude / (2 * M_PI) * std::exp(0.5 * (1 - r2));
+            double dtemp = -(eq.gamma - 1) / (2 * eq.gamma * rt) * du * du;
+            rho = rho * std::pow(1 + dtemp, 1 / (eq.gamma - 1));
+            v1 = v1 + du * c0;
+            v2 = v2 + du * c1;
+            p = p * std::pow(1 + dtemp, eq.gamma / (eq.gamma - 1));
+            double w[4] = {rho, v1, v2, p};
+            eq.prim2cons(w, u);
+            break;
+        }
+        case IC_KELVIN_HELMHOLTZ: {
+            // examples/tree_2d_dgsem/elixir_euler_kelvin_helmholtz_instability.jl:18-31
+            double slope = 15;
+            double B = std::tanh(slope * x[1] + 7.5) - std::tanh(slope * x[1] - 7.5);
+            double rho = 0.5 + 0.75 * B;
+            double v1 = 0.5 * (B - 1);
+            double v2 = 0.1 * std::sin(M_PI * 2 * x[0]);
+            double w[4] = {rho, v1, v2, 1.0};
+            eq.prim2cons(w, u);
+            break;
+        }
+        case IC_FOURIER_MODES: {
+            // Seeded smooth state: rho = 1 + A_rho sum_m sin(pi k_m.(x+1) + phi_m),
+            // v_d = A_v sum_m sin(...), p = 1 (modes passed by the caller).
+            double w[5];
+            for (int fld = 0; fld < 1 + D; ++fld) {
+                double s = 0;
+                for (int m = 0; m < 8; ++m) {
+                    const double *md = &c.ic_modes[(fld * 8 + m) * 4];
+                    double arg = 0;
+                    for (int d = 0; d < D; ++d) arg += md[d] * (x[d] + 1.0);
+                    s += std::sin(M_PI * arg + md[3]);
+                }
+                w[fld] = (fld == 0 ? 1.0 + c.ic_params[0] * s : c.ic_params[1] * s);
+            }
+            w[D + 1] = 1.0;
+            eq.prim2cons(w, u);
+            break;
+        }
+    }
+}
+
+// source_terms_convergence_test: euler_3d.jl:127-153, euler_2d.jl:120-146
+template <int D> void source_convergence(const Euler<D> &eq, const double *x, double t, double *s) {
+    double cst = 2, A = 0.1, L = 2, f = 1.0 / L, om = 2 * M_PI * f, g = eq.gamma;
+    double xs = 0;
+    for (int d = 0; d < D; ++d) xs += x[d];
+    double si = std::sin(om * (xs - t)), co = std::cos(om * (xs - t));
+    double rho = cst + A * si;
+    double rho_x = om * A * co;
+    if (D == 3) {
+        double tmp = (2 * rho - 1.5) * (g - 1);
+        s[0] = 2 * rho_x;
+        s[1] = rho_x * (2 + tmp);
+        s[2] = s[1];
+        s[3] = s[1];
+        s[4] = rho_x * (4 * rho + 3 * tmp);
+    } else {
+        double tmp = (2 * rho - 1) * (g - 1);
+        s[0] = rho_x;
+        s[1] = rho_x * (1 + tmp);
+        s[2] = s[1];
+        s[3] = 2 * rho_x * (rho + tmp);
+    }
+}
+
+inline int ipow(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+}
+
+// Node multi-index helpers: node = i + n*j (+ n*n*k).
+template <int D> inline int node_index(int n, const int *idx) {
+    int r = 0, s = 1;
+    for (int d = 0; d < D; ++d) {
+        r += idx[d] * s;
+        s *= n;
+    }
+    return r;
+}
+
+// Face-node mapping: for orientation o (0-based) the face of nodes with index_o
+// fixed; in-face indices are the remaining indices in increasing dimension order
+// (prolong2interfaces!, dg_3d.jl:530-567, dg_2d.jl:515-541).
+template <int D> inline int face_to_node(int n, int o, int side_index, int fnode) {
+    int idx[3];
+    int rest[2] = {fnode % n, fnode / n};
+    int r = 0;
+    for (int d = 0; d < D; ++d) {
+        if (d == o) idx[d] = side_index;
+        else idx[d] = rest[r++];
+    }
+    return node_index<D>(n, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Volume kernels
+// ---------------------------------------------------------------------------
+
+// weak_form_kernel!: dg_3d.jl:133-164, dg_2d.jl:195-221
+template <int D>
+void weak_form_kernel(const oracle_ctx &c, const Euler<D> &eq, double *du, const double *u,
+                      int e, double alpha) {
+    const int V = D + 2, n = c.n, NN = c.NN;
+    const double *Dhat = c.basis.Dhat.data();
+    for (int node = 0; node < NN; ++node) {
+        int idx[3] = {node % n, (node / n) % n, node / (n * n)};
+        const double *un = &u[V * (node + NN * e)];
+        for (int o = 0; o < D; ++o) {
+            double f[V];
+            eq.flux(un, o, f);
+            for (int ii = 0; ii < n; ++ii) {
+                int jdx[3] = {idx[0], idx[1], idx[2]};
+                jdx[o] = ii;
+                int tnode = node_index<D>(n, jdx);
+                double fac = alpha * Dhat[ii * n + idx[o]];
+                for (int v = 0; v < V; ++v) du[V * (tnode + NN * e) + v] += fac * f[v];
+            }
+        }
+    }
+}
+
+// flux_differencing_kernel!: dg_3d.jl:166-214, dg_2d.jl:223-259
+template <int D>
+void flux_differencing_kernel(const oracle_ctx &c, const Euler<D> &eq, int fluxkind, double *du,
+                              const double *u, int e, double alpha) {
+    const int V = D + 2, n = c.n, NN = c.NN;
+    const double *Ds = c.basis.Dsplit.data();
+    for (int node = 0; node < NN; ++node) {
+        int idx[3] = {node % n, (node / n) % n, node / (n * n)};
+        const double *un = &u[V * (node + NN * e)];
+        for (int o = 0; o < D; ++o) {
+            int i = idx[o];
+            for (int ii = i + 1; ii < n; ++ii) {
+                int jdx[3] = {idx[0], idx[1], idx[2]};
+                jdx[o] = ii;
+                int onode = node_index<D>(n, jdx);
+                const double *uo = &u[V * (onode + NN * e)];
+                double f[V];
+                num_flux<D>(eq, fluxkind, un, uo, o, f);
+                double fi = alpha * Ds[i * n + ii], fii = alpha * Ds[ii * n + i];
+                for (int v = 0; v < V; ++v) {
+                    du[V * (node + NN * e) + v] += fi * f[v];
+                    du[V * (onode + NN * e) + v] += fii * f[v];
+                }
+            }
+        }
+    }
+}
+
+// fv_kernel! + calcflux_fv!: dg_3d.jl:268-306, 359-389; dg_2d.jl:407-464.
+// Subcell fluxes at element faces (index 0 and n) are zero (create_f_threaded,
+// dg_3d.jl:11-54).
+template <int D>
+void fv_kernel(const oracle_ctx &c, const Euler<D> &eq, int fluxkind, double *du,
+               const double *u, int e, double alpha) {
+    const int V = D + 2, n = c.n, NN = c.NN;
+    const double *iw = c.basis.inv_weights.data();
+    // fstar[o][v + V*(node index with index_o in 0..n)] -- store per orientation
+    // as [n+1 along o] x [n^(D-1)] x V
+    std::vector<double> fstar(D * (n + 1) * ipow(n, D - 1) * V, 0.0);
+    auto fidx = [&](int o, int io, int rest) { return ((o * (n + 1) + io) * ipow(n, D - 1) + rest) * V; };
+    for (int o = 0; o < D; ++o) {
+        for (int rest = 0; rest < ipow(n, D - 1); ++rest) {
+            for (int io = 1; io < n; ++io) {
+                int nl = face_to_node<D>(n, o, io - 1, rest);
+                int nr = face_to_node<D>(n, o, io, rest);
+                double f[V];
+                num_flux<D>(eq, fluxkind, &u[V * (nl + NN * e)], &u[V * (nr + NN * e)], o, f);
+                for (int v = 0; v < V; ++v) fstar[fidx(o, io, rest) + v] = f[v];
+            }
+        }
+    }
+    for (int node = 0; node < NN; ++node) {
+        int idx[3] = {node % n, (node / n) % n, node / (n * n)};
+        for (int v = 0; v < V; ++v) {
+            double acc = 0;
+            for (int o = 0; o < D; ++o) {
+                int rest = 0, s = 1;
+                for (int d = 0; d < D; ++d)
+                    if (d != o) {
+                        rest += idx[d] * s;
+                        s *= n;
+                    }
+                int io = idx[o];
+                acc += iw[io] * (fstar[fidx(o, io + 1, rest) + v] - fstar[fidx(o, io, rest) + v]);
+            }
+            du[V * (node + NN * e) + v] += alpha * acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hennemann-Gassner indicator: indicators.jl:119-156, indicators_2d.jl:30-135,
+// indicators_3d.jl:39-173, interpolation.jl:207-235, 348-386.
+// ---------------------------------------------------------------------------
+template <int D>
+void hg_indicator(oracle_ctx &c, const Euler<D> &eq, const double *u) {
+    const int V = D + 2, n = c.n, NN = c.NN, E = c.E;
+    const double threshold = 0.5 * std::pow(10.0, -1.8 * std::pow((double)n, 0.25));
+    const double o_0001 = 0.0001;
+    const double parameter_s = std::log((1 - o_0001) / o_0001);
+    const double *Vi = c.basis.Vinv.data();
+    c.alpha.assign(E, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < E; ++e) {
+        std::vector<double> ind(NN), t1(NN), t2(NN), modal(NN);
+        for (int node = 0; node < NN; ++node) ind[node] = eq.density_pressure(&u[V * (node + NN * e)]);
+        // dimensionwise modal transform
+        std::vector<double> *src = &ind;
+        std::vector<double> *bufs[3] = {&t1, &t2, &modal};
+        if (D == 2) bufs[1] = &modal;
+        for (int o = 0; o < D; ++o) {
+            std::vector<double> &dst = *bufs[o];
+            for (int node = 0; node < NN; ++node) {
+                int idx[3] = {node % n, (node / n) % n, node / (n * n)};
+                double res = 0;
+                for (int ii = 0; ii < n; ++ii) {
+                    int jdx[3] = {idx[0], idx[1], idx[2]};
+                    jdx[o] = ii;
+                    res += Vi[idx[o] * n + ii] * (*src)[node_index<D>(n, jdx)];
+                }
+                dst[node] = res;
+            }
+            src = &dst;
+        }
+        auto m2 = [&](int i, int j, int k) {
+            int idx[3] = {i, j, k};
+            double m = modal[node_index<D>(n, idx)];
+            return m * m;
+        };
+        double te2 = 0, te1, te;
+        if (D == 2) {
+            for (int j = 0; j < n - 2; ++j)
+                for (int i = 0; i < n - 2; ++i) te2 += m2(i, j, 0);
+            te1 = te2;
+            for (int i = 0; i < n - 1; ++i) te1 += m2(i, n - 2, 0);
+            for (int j = 0; j < n - 2; ++j) te1 += m2(n - 2, j, 0);
+            te = te1;
+            for (int i = 0; i < n; ++i) te += m2(i, n - 1, 0);
+            for (int j = 0; j < n - 1; ++j) te += m2(n - 1, j, 0);
+        } else {
+            for (int k = 0; k < n - 2; ++k)
+                for (int j = 0; j < n - 2; ++j)
+                    for (int i = 0; i < n - 2; ++i) te2 += m2(i, j, k);
+            te1 = te2;
+            for (int j = 0; j < n - 1; ++j)
+                for (int i = 0; i < n - 1; ++i) te1 += m2(i, j, n - 2);
+            for (int k = 0; k < n - 2; ++k)
+                for (int i = 0; i < n - 1; ++i) te1 += m2(i, n - 2, k);
+            for (int k = 0; k < n - 2; ++k)
+                for (int j = 0; j < n - 2; ++j) te1 += m2(n - 2, j, k);
+            te = te1;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) te += m2(i, j, n - 1);
+            for (int k = 0; k < n - 1; ++k)
+                for (int i = 0; i < n; ++i) te += m2(i, n - 1, k);
+            for (int k = 0; k < n - 1; ++k)
+                for (int j = 0; j < n - 1; ++j) te += m2(n - 1, j, k);
+        }
+        double ef1 = te != 0 ? (te - te1) / te : 0.0;
+        double ef2 = te1 != 0 ? (te1 - te2) / te1 : 0.0;
+        double energy = std::max(ef1, ef2);
+        double a = 1 / (1 + std::exp(-parameter_s / threshold * (energy - threshold)));
+        if (a < c.cfg.alpha_min) a = 0;
+        if (a > 1 - c.cfg.alpha_min) a = 1;
+        c.alpha[e] = std::min(c.cfg.alpha_max, a);
+    }
+    if (c.cfg.alpha_smooth) {
+        // apply_smoothing!, indicators_2d.jl:104-135 / indicators_3d.jl:125-173
+        c.alpha_tmp = c.alpha;
+        for (int i = 0; i < c.I; ++i) {
+            int l = c.neighbor_ids[2 * i], r = c.neighbor_ids[2 * i + 1];
+            c.alpha[l] = std::max(std::max(c.alpha_tmp[l], 0.5 * c.alpha_tmp[r]), c.alpha[l]);
+            c.alpha[r] = std::max(std::max(c.alpha_tmp[r], 0.5 * c.alpha_tmp[l]), c.alpha[r]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// rhs_hyperbolic!: dg_2d.jl:113-186 (shared by 2D/3D TreeMesh)
+// ---------------------------------------------------------------------------
+template <int D> void rhs(oracle_ctx &c, double *du, const double *u, double t) {
+    const int V = D + 2, n = c.n, NN = c.NN, nf = c.nf, E = c.E, I = c.I;
+    Euler<D> eq(c.cfg.gamma);
+    const size_t total = (size_t)V * NN * E;
+    // set_zero!, solvers.jl:8-29
+    std::memset(du, 0, total * sizeof(double));
+
+    // calc_volume_integral!, calc_volume_integral.jl:180-272
+    const int vi = c.cfg.volume_integral;
+    if (vi == VI_SHOCK_CAPTURING_HG) hg_indicator<D>(c, eq, u);
+    const double atol = 1.8189894035458565e-12;  // max(100 eps, eps^0.75) for Float64
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < E; ++e) {
+        if (vi == VI_WEAK) weak_form_kernel<D>(c, eq, du, u, e, 1.0);
+        else if (vi == VI_FLUX_DIFFERENCING) flux_differencing_kernel<D>(c, eq, c.cfg.volume_flux, du, u, e, 1.0);
+        else if (vi == VI_PURE_FV) fv_kernel<D>(c, eq, c.cfg.fv_flux, du, u, e, 1.0);
+        else {
+            double a = c.alpha[e];
+            bool dg_only = std::fabs(a) <= atol;
+            if (dg_only) flux_differencing_kernel<D>(c, eq, c.cfg.volume_flux, du, u, e, 1.0);
+            else {
+                flux_differencing_kernel<D>(c, eq, c.cfg.volume_flux, du, u, e, 1 - a);
+                fv_kernel<D>(c, eq, c.cfg.fv_flux, du, u, e, a);
+            }
+        }
+    }
+
+    // prolong2interfaces! + calc_interface_flux!: dg_3d.jl:530-602, dg_2d.jl:515-632
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < I; ++i) {
+        int left = c.neighbor_ids[2 * i], right = c.neighbor_ids[2 * i + 1];
+        int o = c.orientations[i];
+        int left_dir = 2 * o + 1, right_dir = 2 * o;  // 0-based: 2o (+side) and 2o-1 (-side)
+        for (int fn = 0; fn < nf; ++fn) {
+            int nl = face_to_node<D>(n, o, n - 1, fn);
+            int nr = face_to_node<D>(n, o, 0, fn);
+            double f[V];
+            num_flux<D>(eq, c.cfg.surface_flux, &u[V * (nl + NN * left)], &u[V * (nr + NN * right)], o, f);
+            for (int v = 0; v < V; ++v) {
+                c.sfv[V * (fn + nf * (left_dir + 2 * D * left)) + v] = f[v];
+                c.sfv[V * (fn + nf * (right_dir + 2 * D * right)) + v] = f[v];
+            }
+        }
+    }
+
+    // calc_surface_integral!: dg_3d.jl:1337-1394, dg_2d.jl:1245-1287
+    const double factor = c.basis.inv_weights[0];
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < E; ++e) {
+        for (int o = 0; o < D; ++o)
+            for (int fn = 0; fn < nf; ++fn) {
+                int nm = face_to_node<D>(n, o, 0, fn), np = face_to_node<D>(n, o, n - 1, fn);
+                for (int v = 0; v < V; ++v) {
+                    du[V * (nm + NN * e) + v] = du[V * (nm + NN * e) + v] - c.sfv[V * (fn + nf * (2 * o + 2 * D * e)) + v] * factor;
+                    du[V * (np + NN * e) + v] = du[V * (np + NN * e) + v] + c.sfv[V * (fn + nf * (2 * o + 1 + 2 * D * e)) + v] * factor;
+                }
+            }
+        // apply_jacobian!: dg_3d.jl:1396-1415
+        double fac = -c.inverse_jacobian[e];
+        for (int k = 0; k < V * NN; ++k) du[(size_t)V * NN * e + k] *= fac;
+        // calc_sources!: dg_3d.jl:1417-1438
+        if (c.cfg.source_terms == SRC_CONVERGENCE_TEST) {
+            for (int node = 0; node < NN; ++node) {
+                double x[3];
+                for (int d = 0; d < D; ++d) x[d] = c.node_coords[d + D * (node + NN * e)];
+                double s[V];
+                source_convergence<D>(eq, x, t, s);
+                for (int v = 0; v < V; ++v) du[V * (node + NN * e) + v] += s[v];
+            }
+        }
+    }
+}
+
+// max_dt: stepsize_dg3d.jl:8-32, stepsize_dg2d.jl:8-30
+template <int D> double max_dt(const oracle_ctx &c, const double *u) {
+    const int V = D + 2, NN = c.NN;
+    Euler<D> eq(c.cfg.gamma);
+    double max_scaled_speed = 4.9406564584124654e-324;  // nextfloat(0.0)
+    for (int e = 0; e < c.E; ++e) {
+        double ml[3] = {0, 0, 0};
+        for (int node = 0; node < NN; ++node) {
+            double lam[3];
+            eq.max_abs_speeds(&u[V * (node + NN * e)], lam);
+            for (int d = 0; d < D; ++d) ml[d] = std::max(ml[d], lam[d]);
+        }
+        double s = 0;
+        for (int d = 0; d < D; ++d) s += ml[d];
+        max_scaled_speed = std::max(max_scaled_speed, c.inverse_jacobian[e] * s);
+    }
+    return 2 / (c.n * max_scaled_speed);
+}
+
+// calc_error_norms: analysis_dg3d.jl:123-161 (and analysis_dg2d.jl twin), func = cons2cons
+template <int D> void error_norms(const oracle_ctx &c, const double *u, double t, double *l2, double *linf) {
+    const int V = D + 2, n = c.n, NN = c.NN, na = c.basis.na;
+    Euler<D> eq(c.cfg.gamma);
+    const double *vdm = c.basis.a_vdm.data();
+    const int NA = ipow(na, D);
+    for (int v = 0; v < V; ++v) l2[v] = linf[v] = 0;
+    std::vector<double> ul(NA * V), xl(NA * D), t1(NA * (V > D ? V : D)), t2(NA * (V > D ? V : D));
+    for (int e = 0; e < c.E; ++e) {
+        // multiply_dimensionwise! for u and x: interpolate along dims one at a time
+        auto interp = [&](const double *in, int nv, double *out) {
+            // in: [nv, n^D]; out: [nv, na^D]; stage through t1/t2
+            std::vector<int> cur = {n, n, n};
+            std::vector<double> a(in, in + (size_t)nv * NN), b;
+            for (int o = 0; o < D; ++o) {
+                std::vector<int> nxt = cur;
+                nxt[o] = na;
+                int cnt = 1;
+                for (int d = 0; d < D; ++d) cnt *= nxt[d];
+                b.assign((size_t)cnt * nv, 0.0);
+                for (int idx = 0; idx < cnt; ++idx) {
+                    int id[3] = {0, 0, 0};
+                    int rem = idx;
+                    for (int d = 0; d < D; ++d) {
+                        id[d] = rem % nxt[d];
+                        rem /= nxt[d];
+                    }
+                    for (int vv = 0; vv < nv; ++vv) {
+                        double res = 0;
+                        for (int ii = 0; ii < n; ++ii) {
+                            int jd[3] = {id[0], id[1], id[2]};
+                            jd[o] = ii;
+                            int src = 0, s = 1;
+                            for (int d = 0; d < D; ++d) {
+                                src += jd[d] * s;
+                                s *= cur[d];
+                            }
+                            res += vdm[id[o] * n + ii] * a[(size_t)src * nv + vv];
+                        }
+                        b[(size_t)idx * nv + vv] = res;
+                    }
+                }
+                a.swap(b);
+                cur = nxt;
+            }
+            std::copy(a.begin(), a.end(), out);
+        };
+        interp(&u[(size_t)V * NN * e], V, ul.data());
+        interp(&c.node_coords[(size_t)D * NN * e], D, xl.data());
+        double vol = std::pow(1.0 / c.inverse_jacobian[e], D);
+        for (int q = 0; q < NA; ++q) {
+            int id[3] = {q % na, (q / na) % na, q / (na * na)};
+            double w = 1;
+            for (int d = 0; d < D; ++d) w *= c.basis.a_weights[id[d]];
+            double ue[V];
+            initial_condition<D>(c, eq, &xl[D * q], t, ue);
+            for (int v = 0; v < V; ++v) {
+                double diff = ue[v] - ul[V * q + v];
+                l2[v] += diff * diff * (w * vol);
+                linf[v] = std::max(linf[v], std::fabs(diff));
+            }
+        }
+    }
+    double tv = std::pow(c.domain_length, D);
+    for (int v = 0; v < V; ++v) l2[v] = std::sqrt(l2[v] / tv);
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+oracle_ctx *oracle_create(const oracle_config *cfg) {
+    if (!cfg || (cfg->ndims != 2 && cfg->ndims != 3) || cfg->polydeg < 1 || cfg->polydeg > 15)
+        return nullptr;
+    oracle_ctx *c = new oracle_ctx();
+    c->cfg = *cfg;
+    c->D = cfg->ndims;
+    c->n = cfg->polydeg + 1;
+    c->NN = ipow(c->n, c->D);
+    c->nf = ipow(c->n, c->D - 1);
+    c->V = c->D + 2;
+    c->basis.init(cfg->polydeg);
+    // TreeMesh(coordinates_min, coordinates_max; initial_refinement_level, periodicity=true)
+    // tree_mesh.jl:128-185
+    double center[3];
+    for (int d = 0; d < c->D; ++d) center[d] = (cfg->coordinates_min[d] + cfg->coordinates_max[d]) / 2;
+    double length = cfg->coordinates_max[0] - cfg->coordinates_min[0];
+    c->domain_length = length;
+    std::vector<int> icoords;
+    int ic0[3] = {0, 0, 0};
+    build_leaves(c->D, 0, cfg->initial_refinement_level, center, length, ic0, c->centers, c->lengths, icoords);
+    c->E = (int)c->lengths.size();
+    const int L = cfg->initial_refinement_level, nside = 1 << L;
+    // integer coordinate -> element id
+    std::vector<int> lookup(ipow(nside, c->D), -1);
+    auto lin = [&](const int *ic) {
+        int r = 0, s = 1;
+        for (int d = 0; d < c->D; ++d) {
+            r += ic[d] * s;
+            s *= nside;
+        }
+        return r;
+    };
+    for (int e = 0; e < c->E; ++e) lookup[lin(&icoords[c->D * e])] = e;
+    // periodic neighbours (init_neighbors!, abstract_tree.jl:252-313, with the
+    // level-0 cell its own neighbour, serial_tree.jl:119-123)
+    c->elem_nbr.assign(2 * c->D * c->E, -1);
+    for (int e = 0; e < c->E; ++e)
+        for (int dir = 0; dir < 2 * c->D; ++dir) {
+            int ic[3] = {0, 0, 0};
+            for (int d = 0; d < c->D; ++d) ic[d] = icoords[c->D * e + d];
+            int o = dir / 2;
+            ic[o] = (ic[o] + (dir % 2 ? 1 : -1) + nside) % nside;
+            c->elem_nbr[dir + 2 * c->D * e] = lookup[lin(ic)];
+        }
+    // init_elements!: containers_3d.jl:87-128
+    c->inverse_jacobian.resize(c->E);
+    c->node_coords.resize((size_t)c->D * c->NN * c->E);
+    double reference_length = 0;
+    for (double w : c->basis.weights) reference_length += w;
+    double reference_offset = (c->basis.nodes.front() + c->basis.nodes.back()) / 2;
+    for (int e = 0; e < c->E; ++e) {
+        double dx = c->lengths[e];
+        double jacobian = dx / reference_length;
+        c->inverse_jacobian[e] = 1.0 / jacobian;
+        for (int node = 0; node < c->NN; ++node) {
+            int idx[3] = {node % c->n, (node / c->n) % c->n, node / (c->n * c->n)};
+            for (int d = 0; d < c->D; ++d)
+                c->node_coords[d + c->D * (node + c->NN * e)] =
+                    c->centers[c->D * e + d] + jacobian * (c->basis.nodes[idx[d]] - reference_offset);
+        }
+    }
+    // init_interfaces!: containers_3d.jl:229-279 (positive directions, element order)
+    c->neighbor_ids.clear();
+    c->orientations.clear();
+    for (int e = 0; e < c->E; ++e)
+        for (int o = 0; o < c->D; ++o) {
+            c->neighbor_ids.push_back(e);
+            c->neighbor_ids.push_back(c->elem_nbr[2 * o + 1 + 2 * c->D * e]);
+            c->orientations.push_back(o);
+        }
+    c->I = (int)c->orientations.size();
+    c->sfv.assign((size_t)c->V * c->nf * 2 * c->D * c->E, 0.0);
+    c->alpha.assign(c->E, 0.0);
+    return c;
+}
+
+void oracle_destroy(oracle_ctx *c) { delete c; }
+
+int oracle_nelements(const oracle_ctx *c) { return c->E; }
+int oracle_ninterfaces(const oracle_ctx *c) { return c->I; }
+int oracle_nnodes(const oracle_ctx *c) { return c->n; }
+long oracle_ndofs(const oracle_ctx *c) { return (long)c->NN * c->E; }
+
+// Mesh / geometry export (0-based ids/orientations).
+void oracle_get_mesh(const oracle_ctx *c, int32_t *neighbor_ids, int32_t *orientations,
+                     double *inverse_jacobian, double *centers, double *node_coords,
+                     int32_t *elem_nbr) {
+    if (neighbor_ids) std::copy(c->neighbor_ids.begin(), c->neighbor_ids.end(), neighbor_ids);
+    if (orientations) std::copy(c->orientations.begin(), c->orientations.end(), orientations);
+    if (inverse_jacobian) std::copy(c->inverse_jacobian.begin(), c->inverse_jacobian.end(), inverse_jacobian);
+    if (centers) std::copy(c->centers.begin(), c->centers.end(), centers);
+    if (node_coords) std::copy(c->node_coords.begin(), c->node_coords.end(), node_coords);
+    if (elem_nbr) std::copy(c->elem_nbr.begin(), c->elem_nbr.end(), elem_nbr);
+}
+
+// Basis export: matrices row-major [r*n + c] = M[r, c].
+void oracle_get_basis(const oracle_ctx *c, double *nodes, double *weights, double *D,
+                      double *Dsplit, double *Dhat, double *Vinv) {
+    const Basis &b = c->basis;
+    if (nodes) std::copy(b.nodes.begin(), b.nodes.end(), nodes);
+    if (weights) std::copy(b.weights.begin(), b.weights.end(), weights);
+    if (D) std::copy(b.D.begin(), b.D.end(), D);
+    if (Dsplit) std::copy(b.Dsplit.begin(), b.Dsplit.end(), Dsplit);
+    if (Dhat) std::copy(b.Dhat.begin(), b.Dhat.end(), Dhat);
+    if (Vinv) std::copy(b.Vinv.begin(), b.Vinv.end(), Vinv);
+}
+
+void oracle_set_initial_condition(oracle_ctx *c, int kind, const double *params, int nparams,
+                                  const double *modes) {
+    c->ic_kind = kind;
+    for (int i = 0; i < 8; ++i) c->ic_params[i] = i < nparams ? params[i] : 0.0;
+    if (kind == IC_FOURIER_MODES && modes) c->ic_modes.assign(modes, modes + 4 * 8 * 4);
+}
+
+// compute_coefficients!: dg.jl:1280-1325 (pointwise initial condition at nodes)
+void oracle_compute_coefficients(const oracle_ctx *c, double t, double *u) {
+    for (int e = 0; e < c->E; ++e)
+        for (int node = 0; node < c->NN; ++node) {
+            const double *x = &c->node_coords[c->D * (node + c->NN * e)];
+            double *un = &u[c->V * (node + c->NN * e)];
+            if (c->D == 2) initial_condition<2>(*c, Euler<2>(c->cfg.gamma), x, t, un);
+            else initial_condition<3>(*c, Euler<3>(c->cfg.gamma), x, t, un);
+        }
+}
+
+void oracle_rhs(oracle_ctx *c, double *du, const double *u, double t) {
+    if (c->D == 2) rhs<2>(*c, du, u, t);
+    else rhs<3>(*c, du, u, t);
+}
+
+// Indicator only (alpha[E]); runs the HG indicator regardless of volume integral.
+void oracle_indicator_hg(oracle_ctx *c, const double *u, double *alpha) {
+    if (c->D == 2) hg_indicator<2>(*c, Euler<2>(c->cfg.gamma), u);
+    else hg_indicator<3>(*c, Euler<3>(c->cfg.gamma), u);
+    std::copy(c->alpha.begin(), c->alpha.end(), alpha);
+}
+
+double oracle_max_dt(const oracle_ctx *c, const double *u) {
+    return c->D == 2 ? max_dt<2>(*c, u) : max_dt<3>(*c, u);
+}
+
+void oracle_error_norms(const oracle_ctx *c, const double *u, double t, double *l2, double *linf) {
+    if (c->D == 2) error_norms<2>(*c, u, t, l2, linf);
+    else error_norms<3>(*c, u, t, l2, linf);
+}
+
+// integrate_via_indices with normalize (analysis_dg3d.jl:364-386): returns the
+// volume-normalised integral of each conserved variable and of entropy_math.
+void oracle_integrals(const oracle_ctx *c, const double *u, double *cons_int, double *entropy_int) {
+    const int V = c->V, n = c->n, NN = c->NN, D = c->D;
+    std::vector<double> acc(V, 0.0);
+    double S = 0;
+    Euler<2> e2(c->cfg.gamma);
+    Euler<3> e3(c->cfg.gamma);
+    for (int e = 0; e < c->E; ++e) {
+        double vol = std::pow(1.0 / c->inverse_jacobian[e], D);
+        for (int node = 0; node < NN; ++node) {
+            int idx[3] = {node % n, (node / n) % n, node / (n * n)};
+            double w = vol;
+            for (int d = 0; d < D; ++d) w *= c->basis.weights[idx[d]];
+            const double *un = &u[V * (node + NN * e)];
+            for (int v = 0; v < V; ++v) acc[v] += w * un[v];
+            S += w * (D == 2 ? e2.entropy_math(un) : e3.entropy_math(un));
+        }
+    }
+    double tv = std::pow(c->domain_length, D);
+    for (int v = 0; v < V; ++v) cons_int[v] = acc[v] / tv;
+    *entropy_int = S / tv;
+}
+
+// CarpenterKennedy2N54 coefficients: methods_2N.jl:24-48
+static const double CK_A[5] = {0.0, 567301805773.0 / 1357537059087.0,
+                               2404267990393.0 / 2016746695238.0,
+                               3550918686646.0 / 2091501179385.0,
+                               1275806237668.0 / 842570457699.0};
+static const double CK_B[5] = {1432997174477.0 / 9575080441755.0,
+                               5161836677717.0 / 13612068292357.0,
+                               1720146321549.0 / 2090206949498.0,
+                               3134564353537.0 / 4481467310338.0,
+                               2277821191437.0 / 14882151754819.0};
+static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0,
+                               2526269341429.0 / 6820363962896.0,
+                               2006345519317.0 / 3224310063776.0,
+                               2802321613138.0 / 2924317926251.0};
+
+void oracle_ck2n54_coefficients(double *a, double *b, double *cc) {
+    for (int i = 0; i < 5; ++i) {
+        a[i] = CK_A[i];
+        b[i] = CK_B[i];
+        cc[i] = CK_C[i];
+    }
+}
+
+// One 2N low-storage step (step!, methods_2N.jl:131-168).
+void oracle_rk_step(oracle_ctx *c, double *u, double *du, double *u_tmp, double t, double dt) {
+    const size_t total = (size_t)c->V * c->NN * c->E;
+    std::memset(u_tmp, 0, total * sizeof(double));
+    for (int s = 0; s < 5; ++s) {
+        double t_stage = t + dt * CK_C[s];
+        oracle_rhs(c, du, u, t_stage);
+        double a_stage = CK_A[s], b_stage_dt = CK_B[s] * dt;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < (long)total; ++i) {
+            u_tmp[i] = du[i] - u_tmp[i] * a_stage;
+            u[i] += u_tmp[i] * b_stage_dt;
+        }
+    }
+}
+
+// solve(ode, CarpenterKennedy2N54; callback = StepsizeCallback(cfl)) from t0 to t_end:
+// stepsize callback at initialisation and after each step (stepsize.jl:71-124),
+// limit_dt! at each step (time_integration.jl:46-55). Returns the number of steps,
+// or -1 if dt became NaN. maxiters <= 0 means unlimited.
+int oracle_solve(oracle_ctx *c, double *u, double t0, double t_end, double cfl, int maxiters,
+                 double *t_out) {
+    const size_t total = (size_t)c->V * c->NN * c->E;
+    std::vector<double> du(total), u_tmp(total);
+    double t = t0;
+    double dt = cfl * oracle_max_dt(c, u);
+    int iter = 0;
+    while (true) {
+        if (std::isnan(dt)) return -1;
+        bool final_step = false;
+        double te = t + dt;
+        if (te > t_end || te == t_end || std::fabs(te - t_end) <= 1.4901161193847656e-08 * std::max(std::fabs(te), std::fabs(t_end))) {
+            dt = t_end - t;
+            final_step = true;
+        }
+        oracle_rk_step(c, u, du.data(), u_tmp.data(), t, dt);
+        ++iter;
+        t += dt;
+        dt = cfl * oracle_max_dt(c, u);
+        if (final_step || (maxiters > 0 && iter >= maxiters)) break;
+    }
+    if (t_out) *t_out = t;
+    return iter;
+}
+
+}  // extern "C"

@@ -1,0 +1,455 @@
+// dg_aux.hpp -- the kernels around the element kernel: Hennemann-Gassner
+// indicator, 2N Runge-Kutta stage update, precision conversion, max_dt,
+// error norms and integrals. All reductions run in FP64 with a fixed
+// two-pass order (per-block partials, then one block), so results are
+// deterministic and the kernels can live inside a captured graph.
+#pragma once
+#include "dg_common.hpp"
+
+namespace trixi_hip {
+
+// ---------------------------------------------------------------------------
+// IndicatorHennemannGassner, raw (before smoothing):
+//   indicators.jl:119-156, indicators_2d.jl:30-102, indicators_3d.jl:39-122,
+//   multiply_scalar_dimensionwise! interpolation.jl:207-235 / 348-386,
+//   density_pressure compressible_euler_3d.jl:1951-1956.
+// FP64 arithmetic in both precision modes (the clipping at alpha_min is
+// threshold-sensitive). One workgroup handles IPB elements, one thread per
+// node for the modal transform, one thread per element for the energies
+// (reference loop order).
+// ---------------------------------------------------------------------------
+struct HGParams {
+  double gamma, alpha_max, alpha_min, threshold, parameter_s;
+};
+
+template <int N> struct VinvConst {
+  double Vinv[(N + 1) * (N + 1)];
+};
+
+template <typename T, int D, int N, int IPB, int NT>
+__global__ __launch_bounds__(NT) void hg_indicator_kernel(const T *__restrict__ u,
+                                                          double *__restrict__ alpha_raw, int E,
+                                                          HGParams hp, VinvConst<N> vc) {
+  constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
+  __shared__ double s_a[IPB * NN], s_b[IPB * NN];
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * IPB;
+  if (e0 >= E) return;
+  const int ne = min(IPB, E - e0);
+  for (int q = tid; q < ne * NN; q += NT) {
+    const T *un = u + (size_t)(e0 * NN + q) * V;
+    double rho = un[0], m2 = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) m2 += double(un[1 + d]) * double(un[1 + d]);
+    s_a[q] = (hp.gamma - 1) * (rho * double(un[D + 1]) - 0.5 * m2);
+  }
+  __syncthreads();
+  double *src = s_a, *dst = s_b;
+#pragma unroll
+  for (int o = 0; o < D; ++o) {
+    const int st = o == 0 ? 1 : (o == 1 ? n : n * n);
+    for (int q = tid; q < ne * NN; q += NT) {
+      const int el = q / NN, node = q - el * NN;
+      const int io = (node / st) % n;
+      const int b0 = el * NN + node - io * st;
+      double res = 0;
+#pragma unroll
+      for (int ii = 0; ii < n; ++ii) res += vc.Vinv[io * n + ii] * src[b0 + ii * st];
+      dst[q] = res;
+    }
+    __syncthreads();
+    double *t = src;
+    src = dst;
+    dst = t;
+  }
+  if (tid < ne) {
+    const double *m = src + tid * NN;
+    auto m2 = [&](int i, int j, int kk) {
+      double x = m[i + n * j + n * n * kk];
+      return x * x;
+    };
+    double te2 = 0, te1, te;
+    if constexpr (D == 2) {
+      for (int j = 0; j < n - 2; ++j)
+        for (int i = 0; i < n - 2; ++i) te2 += m2(i, j, 0);
+      te1 = te2;
+      for (int i = 0; i < n - 1; ++i) te1 += m2(i, n - 2, 0);
+      for (int j = 0; j < n - 2; ++j) te1 += m2(n - 2, j, 0);
+      te = te1;
+      for (int i = 0; i < n; ++i) te += m2(i, n - 1, 0);
+      for (int j = 0; j < n - 1; ++j) te += m2(n - 1, j, 0);
+    } else {
+      for (int kk = 0; kk < n - 2; ++kk)
+        for (int j = 0; j < n - 2; ++j)
+          for (int i = 0; i < n - 2; ++i) te2 += m2(i, j, kk);
+      te1 = te2;
+      for (int j = 0; j < n - 1; ++j)
+        for (int i = 0; i < n - 1; ++i) te1 += m2(i, j, n - 2);
+      for (int kk = 0; kk < n - 2; ++kk)
+        for (int i = 0; i < n - 1; ++i) te1 += m2(i, n - 2, kk);
+      for (int kk = 0; kk < n - 2; ++kk)
+        for (int j = 0; j < n - 2; ++j) te1 += m2(n - 2, j, kk);
+      te = te1;
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) te += m2(i, j, n - 1);
+      for (int kk = 0; kk < n - 1; ++kk)
+        for (int i = 0; i < n; ++i) te += m2(i, n - 1, kk);
+      for (int kk = 0; kk < n - 1; ++kk)
+        for (int j = 0; j < n - 1; ++j) te += m2(n - 1, j, kk);
+    }
+    double ef1 = te != 0 ? (te - te1) / te : 0.0;
+    double ef2 = te1 != 0 ? (te1 - te2) / te1 : 0.0;
+    double energy = ef1 > ef2 ? ef1 : ef2;
+    double al = 1.0 / (1.0 + exp(-hp.parameter_s / hp.threshold * (energy - hp.threshold)));
+    if (al < hp.alpha_min) al = 0.0;
+    if (al > 1 - hp.alpha_min) al = 1.0;
+    alpha_raw[e0 + tid] = al < hp.alpha_max ? al : hp.alpha_max;
+  }
+}
+
+// Stand-alone smoothing (for trixi_hip_indicator; the element kernel smooths inline).
+template <int D>
+__global__ void hg_smooth_kernel(const double *__restrict__ alpha_raw,
+                                 const int32_t *__restrict__ nbr, double *__restrict__ alpha,
+                                 int E, int smooth) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double al = alpha_raw[e];
+  if (smooth) {
+    double m = al;
+    for (int dir = 0; dir < 2 * D; ++dir) {
+      double an = 0.5 * alpha_raw[nbr[dir + 2 * D * e]];
+      m = an > m ? an : m;
+    }
+    al = m;
+  }
+  alpha[e] = al;
+}
+
+// ---------------------------------------------------------------------------
+// 2N low-storage stage update (methods_2N.jl:153-157):
+//   u_tmp = du - u_tmp * a_stage;  u += u_tmp * b_stage_dt
+// u, u_tmp FP64 registers; du in the RHS precision; optional FP32 shadow of u.
+// b_stage_dt = b_stage * dt with dt taken from tdev[1] when tdev != null.
+// ---------------------------------------------------------------------------
+template <typename TD>
+__global__ void rk2n_update_kernel(const TD *__restrict__ du, double *__restrict__ u_tmp,
+                                   double *__restrict__ u, float *__restrict__ u32, size_t count,
+                                   double a_stage, double b_stage, const double *__restrict__ tdev,
+                                   int first_stage) {
+  const double b_dt = tdev ? b_stage * tdev[1] : b_stage;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) {
+    double ut = first_stage ? 0.0 : u_tmp[i];
+    ut = double(du[i]) - ut * a_stage;
+    double un = __builtin_fma(ut, b_dt, u[i]);
+    u_tmp[i] = ut;
+    u[i] = un;
+    if (u32) u32[i] = float(un);
+  }
+}
+
+template <typename TO, typename TI>
+__global__ void convert_kernel(TO *__restrict__ out, const TI *__restrict__ in, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) out[i] = TO(in[i]);
+}
+
+// ---------------------------------------------------------------------------
+// max_dt (stepsize_dg3d.jl:8-32, stepsize_dg2d.jl:8-30), max_abs_speeds
+// (compressible_euler_3d.jl:1770-1775). Pass 1: one thread per element, block
+// max into partials. Pass 2: one block reduces and writes
+//   out[0] = cfl * 2 / (nnodes * max_scaled_speed)
+// NaN propagates (fmax would drop it; the reference uses Base.max).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double nanmax(double a, double b) {
+  return (a != a || b != b) ? __builtin_nan("") : (a > b ? a : b);
+}
+
+template <typename T, int D, int N>
+__global__ __launch_bounds__(256) void max_speed_kernel(const T *__restrict__ u,
+                                                        const double *__restrict__ inv_jac, int E,
+                                                        double gamma, double *__restrict__ partial) {
+  constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
+  __shared__ double s_red[256];
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  double val = 4.9406564584124654e-324;
+  if (e < E) {
+    double ml[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) ml[d] = 0.0;
+    const T *ue = u + (size_t)e * NN * V;
+    for (int node = 0; node < NN; ++node) {
+      const T *un = ue + node * V;
+      double rho = un[0];
+      double ke2 = 0, v[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        v[d] = double(un[1 + d]) / rho;
+        ke2 += double(un[1 + d]) * v[d];
+      }
+      double p = (gamma - 1) * (double(un[D + 1]) - 0.5 * ke2);
+      double c = sqrt(gamma * p / rho);
+#pragma unroll
+      for (int d = 0; d < D; ++d) ml[d] = nanmax(ml[d], fabs(v[d]) + c);
+    }
+    double s = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) s += ml[d];
+    val = nanmax(val, inv_jac[e] * s);
+  }
+  s_red[threadIdx.x] = val;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s_red[threadIdx.x] = nanmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s_red[0];
+}
+
+__global__ __launch_bounds__(256) void max_dt_finalize_kernel(const double *__restrict__ partial,
+                                                              int np, int nnodes, double cfl,
+                                                              double *__restrict__ out) {
+  __shared__ double s_red[256];
+  double m = 4.9406564584124654e-324;
+  for (int i = threadIdx.x; i < np; i += 256) m = nanmax(m, partial[i]);
+  s_red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s_red[threadIdx.x] = nanmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = cfl * (2.0 / (nnodes * s_red[0]));
+}
+
+// ---------------------------------------------------------------------------
+// Exact solutions for calc_error_norms (initial conditions, host-side setup
+// in the reference; the device needs them to compare at analysis nodes).
+// ---------------------------------------------------------------------------
+struct ExactParams {
+  int kind;
+  double gamma;
+  double params[4];
+  double modes[4 * 8 * 4];
+};
+
+template <int D>
+__device__ void exact_solution(const ExactParams &ep, const double *x, double t, double *u) {
+  const int V = D + 2;
+  const double g = ep.gamma, igm1 = 1.0 / (g - 1);
+  auto prim2cons = [&](const double *w) {
+    u[0] = w[0];
+    double ke2 = 0;
+    for (int d = 0; d < D; ++d) {
+      u[1 + d] = w[0] * w[1 + d];
+      ke2 += u[1 + d] * w[1 + d];
+    }
+    u[V - 1] = w[V - 1] * igm1 + 0.5 * ke2;
+  };
+  switch (ep.kind) {
+    case 0: {  // initial_condition_constant
+      u[0] = 1;
+      u[1] = 0.1;
+      u[2] = -0.2;
+      if (D == 3) u[3] = 0.7;
+      u[V - 1] = 10;
+      break;
+    }
+    case 1: {  // initial_condition_convergence_test
+      double s = 0;
+      for (int d = 0; d < D; ++d) s += x[d];
+      double ini = 2 + 0.1 * sin(M_PI * (s - t));
+      for (int v = 0; v < V - 1; ++v) u[v] = ini;
+      u[V - 1] = ini * ini;
+      break;
+    }
+    case 2: {  // initial_condition_weak_blast_wave (3D)
+      double xn = x[0], yn = x[1], zn = D == 3 ? x[2] : 0.0;
+      double r = sqrt(xn * xn + yn * yn + zn * zn);
+      double phi = atan2(yn, xn);
+      double theta = r == 0 ? 0.0 : acos(zn / r);
+      double w[5];
+      w[0] = r > 0.5 ? 1.0 : 1.1691;
+      w[1] = r > 0.5 ? 0.0 : 0.1882 * cos(phi) * sin(theta);
+      w[2] = r > 0.5 ? 0.0 : 0.1882 * sin(phi) * sin(theta);
+      w[3] = r > 0.5 ? 0.0 : 0.1882 * cos(theta);
+      w[4] = r > 0.5 ? 1.0 : 1.245;
+      if (D == 3) prim2cons(w);
+      break;
+    }
+    case 3: {  // initial_condition_isentropic_vortex (elixir_euler_vortex.jl)
+      double rho = 1, v1 = 1, v2 = 1, p = ep.params[0];
+      double rt = p / rho;
+      double c0 = -x[1], c1 = x[0];
+      double r2 = c0 * c0 + c1 * c1;
+      double du = 5.0 / (2 * M_PI) * exp(0.5 * (1 - r2));
+      double dtemp = -(g - 1) / (2 * g * rt) * du * du;
+      rho = rho * pow(1 + dtemp, 1 / (g - 1));
+      v1 = v1 + du * c0;
+      v2 = v2 + du * c1;
+      p = p * pow(1 + dtemp, g / (g - 1));
+      double w[4] = {rho, v1, v2, p};
+      if (D == 2) prim2cons(w);
+      break;
+    }
+    case 4: {  // initial_condition_kelvin_helmholtz_instability (noise-free)
+      double B = tanh(15 * x[1] + 7.5) - tanh(15 * x[1] - 7.5);
+      double w[4] = {0.5 + 0.75 * B, 0.5 * (B - 1), 0.1 * sin(M_PI * 2 * x[0]), 1.0};
+      if (D == 2) prim2cons(w);
+      break;
+    }
+    default: {  // seeded Fourier modes
+      double w[5];
+      for (int fld = 0; fld < 1 + D; ++fld) {
+        double s = 0;
+        for (int m = 0; m < 8; ++m) {
+          const double *md = &ep.modes[(fld * 8 + m) * 4];
+          double arg = 0;
+          for (int d = 0; d < D; ++d) arg += md[d] * (x[d] + 1.0);
+          s += sin(M_PI * arg + md[3]);
+        }
+        w[fld] = fld == 0 ? 1.0 + ep.params[0] * s : ep.params[1] * s;
+      }
+      w[D + 1] = 1.0;
+      prim2cons(w);
+      break;
+    }
+  }
+}
+
+template <int N, int NA> struct AnalysisConst {
+  double vdm[NA * (N + 1)];  // [a*n + j]
+  double aw[NA];
+  double nodes[N + 1];
+};
+
+// calc_error_norms (analysis_dg3d.jl:123-161): one block per element. The
+// element's solution and node coordinates are staged in LDS; each thread
+// interpolates to its analysis points with the tensor-product Vandermonde
+// (the reference applies it dimension by dimension; same sums, other order).
+// partial[e] = {l2 contributions [V], linf [V]}.
+template <typename T, int D, int N, int NA>
+__global__ __launch_bounds__(256) void error_norms_kernel(const T *__restrict__ u,
+                                                          const double *__restrict__ inv_jac,
+                                                          const double *__restrict__ centers,
+                                                          int E, double t, ExactParams ep,
+                                                          AnalysisConst<N, NA> ac,
+                                                          double *__restrict__ partial) {
+  constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2, NQ = ipow_c(NA, D);
+  constexpr int W = V + D;  // interpolate u and x together
+  __shared__ double s0[NN * W];
+  __shared__ double s_red[256];
+  const int e = blockIdx.x;
+  if (e >= E) return;
+  const int tid = threadIdx.x;
+  const double jac = 1.0 / inv_jac[e];
+  for (int q = tid; q < NN; q += 256) {
+    int idx[3] = {q % n, (q / n) % n, q / (n * n)};
+    for (int v = 0; v < V; ++v) s0[q * W + v] = double(u[((size_t)e * NN + q) * V + v]);
+    for (int d = 0; d < D; ++d) s0[q * W + V + d] = centers[d + D * e] + jac * ac.nodes[idx[d]];
+  }
+  __syncthreads();
+  const double vol = D == 3 ? jac * jac * jac : jac * jac;
+  double l2[V], li[V];
+  for (int v = 0; v < V; ++v) l2[v] = li[v] = 0;
+  for (int q = tid; q < NQ; q += 256) {
+    int id[3] = {q % NA, (q / NA) % NA, q / (NA * NA)};
+    double w = 1;
+    for (int d = 0; d < D; ++d) w *= ac.aw[id[d]];
+    double val[W];
+    for (int c = 0; c < W; ++c) val[c] = 0;
+    for (int node = 0; node < NN; ++node) {
+      int jd[3] = {node % n, (node / n) % n, node / (n * n)};
+      double c = 1;
+      for (int d = 0; d < D; ++d) c *= ac.vdm[id[d] * n + jd[d]];
+      for (int cc = 0; cc < W; ++cc) val[cc] += c * s0[node * W + cc];
+    }
+    double ue[V];
+    exact_solution<D>(ep, &val[V], t, ue);
+    for (int v = 0; v < V; ++v) {
+      double diff = ue[v] - val[v];
+      l2[v] += diff * diff * (w * vol);
+      double ad = fabs(diff);
+      li[v] = (ad > li[v] || ad != ad) ? ad : li[v];
+    }
+  }
+  for (int v = 0; v < V; ++v) {
+    s_red[tid] = l2[v];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) s_red[tid] += s_red[tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) partial[(size_t)e * 2 * V + v] = s_red[0];
+    __syncthreads();
+    s_red[tid] = li[v];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) s_red[tid] = nanmax(s_red[tid], s_red[tid + w]);
+      __syncthreads();
+    }
+    if (tid == 0) partial[(size_t)e * 2 * V + V + v] = s_red[0];
+    __syncthreads();
+  }
+}
+
+// integrate_via_indices (analysis_dg3d.jl:364-386) of u and entropy_math
+// (compressible_euler_3d.jl:1970-1976): partial[e] = {int u [V], int S}.
+template <typename T, int D, int N>
+__global__ __launch_bounds__(256) void integrals_kernel(const T *__restrict__ u,
+                                                        const double *__restrict__ inv_jac,
+                                                        int E, double gamma,
+                                                        const double *__restrict__ weights,
+                                                        double *__restrict__ partial) {
+  constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const double jac = 1.0 / inv_jac[e];
+  const double vol = D == 3 ? jac * jac * jac : jac * jac;
+  double acc[V + 1];
+  for (int v = 0; v <= V; ++v) acc[v] = 0;
+  for (int node = 0; node < NN; ++node) {
+    int idx[3] = {node % n, (node / n) % n, node / (n * n)};
+    double w = vol;
+    for (int d = 0; d < D; ++d) w *= weights[idx[d]];
+    const T *un = u + ((size_t)e * NN + node) * V;
+    double m2 = 0;
+    for (int d = 0; d < D; ++d) m2 += double(un[1 + d]) * double(un[1 + d]);
+    double rho = un[0];
+    double p = (gamma - 1) * (double(un[V - 1]) - 0.5 * m2 / rho);
+    double s = log(p) - gamma * log(rho);
+    double S = -s * rho / (gamma - 1);
+    for (int v = 0; v < V; ++v) acc[v] += w * double(un[v]);
+    acc[V] += w * S;
+  }
+  for (int v = 0; v <= V; ++v) partial[(size_t)e * (V + 1) + v] = acc[v];
+}
+
+// Column sums / maxima of a partial table [rows, cols] -> out[cols]; one block.
+__global__ __launch_bounds__(256) void reduce_columns_kernel(const double *__restrict__ partial,
+                                                             int rows, int cols, int nsum,
+                                                             double *__restrict__ out) {
+  __shared__ double s_red[256];
+  for (int c = 0; c < cols; ++c) {
+    const bool is_sum = c < nsum;
+    double acc = is_sum ? 0.0 : 0.0;
+    for (int r = threadIdx.x; r < rows; r += 256) {
+      double x = partial[(size_t)r * cols + c];
+      acc = is_sum ? acc + x : nanmax(acc, x);
+    }
+    s_red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w)
+        s_red[threadIdx.x] = is_sum ? s_red[threadIdx.x] + s_red[threadIdx.x + w]
+                                    : nanmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = s_red[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace trixi_hip

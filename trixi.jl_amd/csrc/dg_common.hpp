@@ -1,0 +1,267 @@
+// dg_common.hpp -- device-side physics for the DGSEM compressible-Euler RHS.
+//
+// Pointwise functions of Trixi.jl restated for CDNA4 (gfx950), templated on
+// the arithmetic type T (float for the FP32 bulk path, double for the plain
+// FP64 path) and the number of space dimensions D (V = D + 2 variables).
+//
+//   cons2prim              src/equations/compressible_euler_3d.jl:1783-1794 (2D :2038-2046)
+//   flux                   compressible_euler_3d.jl:420-447 (2D :415-434)
+//   flux_ranocha           compressible_euler_3d.jl:746-793 (2D :686-722), in the
+//                          per-node-logarithm form of the reference's own
+//                          flux_ranocha_turbo kernel (dg_3d_compressible_euler.jl:265-420)
+//   ln_mean / inv_ln_mean  src/auxiliary/math.jl:196-263
+//   flux_lax_friedrichs    src/equations/numerical_fluxes.jl:32-45, 166-178, 229-253
+//   max_abs_speed(_naive)  compressible_euler_3d.jl:1112-1177 (2D :1423-1482)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace trixi_hip {
+
+enum VolumeKind { VOL_WEAK = 0, VOL_FD_RANOCHA = 1, VOL_FD_CENTRAL = 2, VOL_HG_RANOCHA = 3,
+                  VOL_HG_CENTRAL = 4, VOL_PURE_FV = 5 };
+enum FluxKind { FLUX_LLF = 0, FLUX_LLF_NAIVE = 1, FLUX_RANOCHA = 2, FLUX_CENTRAL = 3 };
+
+constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
+
+// ---------------------------------------------------------------------------
+// Precision traits. The FP32 path uses the hardware reciprocal and base-2
+// logarithm (v_rcp_f32, v_log_f32, 1 ulp); the FP64 path refines v_rcp_f64
+// with two Newton steps and calls the correctly rounded double log.
+// ---------------------------------------------------------------------------
+template <typename T> struct Prec;
+
+template <> struct Prec<float> {
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float ln(float x) {
+    return __builtin_amdgcn_logf(x) * 0.69314718055994530942f;
+  }
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+  // Series branch of the logarithmic mean is taken for f^2 < 1e-2 in FP32:
+  // its truncation error (first omitted term ~ f^8 / 9 <= 1.2e-9) is below FP32
+  // round-off, and the cancellation-prone far branch is only used for
+  // |y - x| / (y + x) >= 0.1.  sqrt(1e-2) = 0.1.
+  static constexpr float sqrt_f2_threshold = 0.1f;
+};
+
+template <> struct Prec<double> {
+  static __device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return r;
+  }
+  static __device__ __forceinline__ double ln(double x) { return ::log(x); }
+  static __device__ __forceinline__ double sqrt_(double x) { return ::sqrt(x); }
+  // Reference threshold f^2 < 1e-4 (math.jl:199).
+  static constexpr double sqrt_f2_threshold = 0.01;
+};
+
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
+
+// ---------------------------------------------------------------------------
+// Logarithmic means with one reciprocal each.
+//
+// With s = x + y, d = y - x, f = d / s:  f^2 < threshold  <=>  |d| < sqrt(thr) * s.
+//   ln_mean near:  s / (2 + f2 (2/3 + f2 (2/5 + 2/7 f2)))
+//               =  s * 0.5 (1 - f2/3 - 4 f2^2/45 - 44 f2^3/945 - 428 f2^4/14175)
+//                  (series of f / atanh f; truncation < 1e-18 at f2 < 1e-4 and
+//                   < 4e-11 at f2 < 1e-2)
+//   ln_mean far:   d / (log y - log x)
+//   inv_ln_mean near: (2 + f2 (2/3 + f2 (2/5 + 2/7 f2))) / s   (math.jl:242-247)
+//   inv_ln_mean far:  (log y - log x) / d
+// The near/far selection happens before the single reciprocal.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T ln_mean_l(T x, T y, T lx, T ly) {
+  T s = x + y, d = y - x;
+  bool near = tabs(d) < Prec<T>::sqrt_f2_threshold * s;
+  T r = Prec<T>::rcp(near ? s : (ly - lx));
+  T t = d * r;
+  T z = t * t;
+  T q = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(T(-428.0 / 28350.0), z, T(-44.0 / 1890.0)),
+                                                  z, T(-4.0 / 90.0)), z, T(-1.0 / 6.0)), z, T(0.5));
+  return near ? s * q : t;
+}
+
+template <typename T>
+__device__ __forceinline__ T inv_ln_mean_l(T x, T y, T lx, T ly) {
+  T s = x + y, d = y - x;
+  bool near = tabs(d) < Prec<T>::sqrt_f2_threshold * s;
+  T r = Prec<T>::rcp(near ? s : d);
+  T t = d * r;
+  T z = t * t;
+  T p = __builtin_fma(__builtin_fma(__builtin_fma(T(2.0 / 7.0), z, T(2.0 / 5.0)), z, T(2.0 / 3.0)), z, T(2.0));
+  return near ? p * r : (ly - lx) * r;
+}
+
+// Per-node data of the flux_ranocha volume kernel: density, its log, the
+// inverse temperature beta = rho/p, its log, half velocities, half pressure.
+// flux_ranocha's inv_ln_mean(rho_ll p_rr, rho_rr p_ll) p_ll p_rr equals
+// inv_ln_mean(beta_ll, beta_rr) (compressible_euler_3d.jl:752-757).
+template <typename T, int D> struct NodeR {
+  T rho, lrho, beta, lbeta, h[D], hp;
+};
+constexpr int nodeR_count(int D) { return 5 + D; }
+
+template <typename T, int D>
+__device__ __forceinline__ void nodeR_from_cons(const T *u, T gamma_m1, NodeR<T, D> &w) {
+  T rho = u[0];
+  T irho = Prec<T>::rcp(rho);
+  T ke2 = 0;
+  T v[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    v[d] = u[1 + d] * irho;
+    ke2 = __builtin_fma(u[1 + d], v[d], ke2);
+  }
+  T p = gamma_m1 * (u[D + 1] - T(0.5) * ke2);
+  w.rho = rho;
+  w.lrho = Prec<T>::ln(rho);
+  w.beta = rho * Prec<T>::rcp(p);
+  w.lbeta = w.lrho - Prec<T>::ln(p);
+#pragma unroll
+  for (int d = 0; d < D; ++d) w.h[d] = T(0.5) * v[d];
+  w.hp = T(0.5) * p;
+}
+
+// flux_ranocha from per-node data, orientation o (compile-time).
+template <typename T, int D, int O>
+__device__ __forceinline__ void flux_ranocha_r(const NodeR<T, D> &l, const NodeR<T, D> &r,
+                                               T inv_gamma_m1, T *f) {
+  T rho_mean = ln_mean_l(l.rho, r.rho, l.lrho, r.lrho);
+  T inv_rho_p_mean = inv_ln_mean_l(l.beta, r.beta, l.lbeta, r.lbeta);
+  T vavg[D];
+  T hh = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    vavg[d] = l.h[d] + r.h[d];
+    hh = __builtin_fma(l.h[d], r.h[d], hh);  // = velocity_square_avg / 2
+  }
+  T p_avg = l.hp + r.hp;
+  T cross = __builtin_fma(l.hp, r.h[O], r.hp * l.h[O]);  // = 0.5 (p_ll v_rr + p_rr v_ll) / 2
+  T f1 = rho_mean * vavg[O];
+  f[0] = f1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) f[1 + d] = f1 * vavg[d];
+  f[1 + O] += p_avg;
+  T inner = __builtin_fma(inv_rho_p_mean, inv_gamma_m1, hh + hh);
+  f[D + 1] = __builtin_fma(f1, inner, cross + cross);
+}
+
+// Physical flux of a conservative state in orientation o (runtime), plus the
+// primitive data LLF needs.
+template <typename T, int D> struct ConsP {
+  T u[D + 2];
+  T v[D];
+  T p;
+};
+
+template <typename T, int D>
+__device__ __forceinline__ void consp_from_cons(const T *u, T gamma_m1, ConsP<T, D> &c) {
+  T irho = Prec<T>::rcp(u[0]);
+  T ke2 = 0;
+#pragma unroll
+  for (int k = 0; k < D + 2; ++k) c.u[k] = u[k];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    c.v[d] = u[1 + d] * irho;
+    ke2 = __builtin_fma(u[1 + d], c.v[d], ke2);
+  }
+  c.p = gamma_m1 * (u[D + 1] - T(0.5) * ke2);
+}
+
+template <typename T, int D, int O>
+__device__ __forceinline__ void phys_flux(const ConsP<T, D> &c, T *f) {
+  f[0] = c.u[1 + O];
+#pragma unroll
+  for (int d = 0; d < D; ++d) f[1 + d] = c.u[1 + O] * c.v[d];
+  f[1 + O] += c.p;
+  f[D + 1] = (c.u[D + 1] + c.p) * c.v[O];
+}
+
+// flux_central (numerical_fluxes.jl:17-25)
+template <typename T, int D, int O>
+__device__ __forceinline__ void flux_central_c(const ConsP<T, D> &l, const ConsP<T, D> &r, T *f) {
+  T fl[D + 2], fr[D + 2];
+  phys_flux<T, D, O>(l, fl);
+  phys_flux<T, D, O>(r, fr);
+#pragma unroll
+  for (int k = 0; k < D + 2; ++k) f[k] = T(0.5) * (fl[k] + fr[k]);
+}
+
+// FluxLaxFriedrichs(max_abs_speed | max_abs_speed_naive)
+template <typename T, int D, int O>
+__device__ __forceinline__ void flux_llf_c(const ConsP<T, D> &l, const ConsP<T, D> &r, bool naive,
+                                           T gamma, T *f) {
+  T fl[D + 2], fr[D + 2];
+  phys_flux<T, D, O>(l, fl);
+  phys_flux<T, D, O>(r, fr);
+  T c_ll = Prec<T>::sqrt_(gamma * l.p * Prec<T>::rcp(l.u[0]));
+  T c_rr = Prec<T>::sqrt_(gamma * r.p * Prec<T>::rcp(r.u[0]));
+  T al = tabs(l.v[O]), ar = tabs(r.v[O]);
+  T lambda = naive ? tmax(al, ar) + tmax(c_ll, c_rr) : tmax(al + c_ll, ar + c_rr);
+  T hl = T(-0.5) * lambda;
+#pragma unroll
+  for (int k = 0; k < D + 2; ++k) f[k] = T(0.5) * (fl[k] + fr[k]) + hl * (r.u[k] - l.u[k]);
+}
+
+// flux_ranocha from conservative states (surface / FV use).
+template <typename T, int D, int O>
+__device__ __forceinline__ void flux_ranocha_c(const T *ul, const T *ur, T gamma_m1, T *f) {
+  NodeR<T, D> a, b;
+  nodeR_from_cons<T, D>(ul, gamma_m1, a);
+  nodeR_from_cons<T, D>(ur, gamma_m1, b);
+  flux_ranocha_r<T, D, O>(a, b, T(1) / gamma_m1, f);
+}
+
+// Runtime-selected two-point flux of conservative states (surface and FV subcells).
+template <typename T, int D, int O>
+__device__ __forceinline__ void num_flux_c(int kind, const T *ul, const T *ur, T gamma, T *f) {
+  if (kind == FLUX_RANOCHA) {
+    flux_ranocha_c<T, D, O>(ul, ur, gamma - T(1), f);
+  } else {
+    ConsP<T, D> l, r;
+    consp_from_cons<T, D>(ul, gamma - T(1), l);
+    consp_from_cons<T, D>(ur, gamma - T(1), r);
+    if (kind == FLUX_CENTRAL) flux_central_c<T, D, O>(l, r, f);
+    else flux_llf_c<T, D, O>(l, r, kind == FLUX_LLF_NAIVE, gamma, f);
+  }
+}
+
+// source_terms_convergence_test (compressible_euler_3d.jl:127-153, 2D :120-146).
+template <typename T, int D>
+__device__ __forceinline__ void source_convergence(const T *x, T t, T gamma, T *s) {
+  const T c = 2, A = T(0.1), omega = T(3.14159265358979323846);  // 2 pi / L, L = 2
+  T xs = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) xs += x[d];
+  T si, co;
+  if constexpr (sizeof(T) == 4) {
+    sincosf(omega * (xs - t), &si, &co);
+  } else {
+    sincos(omega * (xs - t), &si, &co);
+  }
+  T rho = c + A * si;
+  T rho_x = omega * A * co;
+  if constexpr (D == 3) {
+    T tmp = (2 * rho - T(1.5)) * (gamma - 1);
+    s[0] = 2 * rho_x;
+    s[1] = rho_x * (2 + tmp);
+    s[2] = s[1];
+    s[3] = s[1];
+    s[4] = rho_x * (4 * rho + 3 * tmp);
+  } else {
+    T tmp = (2 * rho - 1) * (gamma - 1);
+    s[0] = rho_x;
+    s[1] = rho_x * (1 + tmp);
+    s[2] = s[1];
+    s[3] = 2 * rho_x * (rho + tmp);
+  }
+}
+
+}  // namespace trixi_hip

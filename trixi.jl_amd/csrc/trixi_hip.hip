@@ -1,0 +1,703 @@
+// trixi_hip.hip -- C ABI of the MI355X DGSEM compressible-Euler RHS (see
+// include/trixi_hip.h): context management, kernel dispatch, the 2N RK stage
+// and step (hipGraph-captured), reductions.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/trixi_hip.h"
+#include "dg_aux.hpp"
+#include "dg_element.hpp"
+
+using namespace trixi_hip;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Element-kernel registry: one entry per instantiated (T, D, N, VOL, EPB, NT).
+// ---------------------------------------------------------------------------
+struct ElemKey {
+  int fp32, D, N, vol, epb, nt;
+  bool operator<(const ElemKey &o) const {
+    return std::tie(fp32, D, N, vol, epb, nt) < std::tie(o.fp32, o.D, o.N, o.vol, o.epb, o.nt);
+  }
+};
+// Type-erased launcher: args and constants are passed as raw blobs that the
+// instantiation reinterprets as ElemArgs<T> / DGConst<T, N>.
+using ElemLaunch = hipError_t (*)(const void *args, const void *kconst, int nblocks, hipStream_t s);
+
+std::map<ElemKey, ElemLaunch> &elem_registry() {
+  static std::map<ElemKey, ElemLaunch> r;
+  return r;
+}
+
+template <typename T, int D, int N, int VOL, int EPB, int NT>
+hipError_t launch_elem(const void *args, const void *kconst, int nblocks, hipStream_t s) {
+  const ElemArgs<T> &a = *static_cast<const ElemArgs<T> *>(args);
+  const DGConst<T, N> &k = *static_cast<const DGConst<T, N> *>(kconst);
+  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, NT>), dim3(nblocks), dim3(NT), 0, s, a, k);
+  return hipGetLastError();
+}
+
+template <typename T, int D, int N, int VOL, int EPB, int NT> struct Reg {
+  Reg() { elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, NT}] = &launch_elem<T, D, N, VOL, EPB, NT>; }
+};
+
+// Tile candidates: lines per direction L = n^(D-1); EPB = elements per
+// workgroup, NT = threads. The first entry of each list is the default; the
+// autotuner (trixi_hip_autotune via the Python host) may pick another.
+#define TRIXI_REG(T, D, N, VOL, EPB, NT) static Reg<T, D, N, VOL, EPB, NT> reg_##T##_##D##_##N##_##VOL##_##EPB##_##NT;
+
+#define TRIXI_REG_ALLVOL(T, D, N, EPB, NT)           \
+  TRIXI_REG(T, D, N, VOL_WEAK, EPB, NT)              \
+  TRIXI_REG(T, D, N, VOL_FD_RANOCHA, EPB, NT)        \
+  TRIXI_REG(T, D, N, VOL_FD_CENTRAL, EPB, NT)        \
+  TRIXI_REG(T, D, N, VOL_HG_RANOCHA, EPB, NT)        \
+  TRIXI_REG(T, D, N, VOL_HG_CENTRAL, EPB, NT)        \
+  TRIXI_REG(T, D, N, VOL_PURE_FV, EPB, NT)
+
+#define TRIXI_REG_BOTH(D, N, EPB, NT) \
+  TRIXI_REG_ALLVOL(float, D, N, EPB, NT) \
+  TRIXI_REG_ALLVOL(double, D, N, EPB, NT)
+
+// defaults (one wavefront of lines per direction where the line count allows)
+TRIXI_REG_BOTH(2, 2, 21, 64)
+TRIXI_REG_BOTH(2, 3, 16, 64)
+TRIXI_REG_BOTH(2, 4, 12, 64)
+TRIXI_REG_BOTH(3, 2, 7, 64)
+TRIXI_REG_BOTH(3, 3, 4, 64)
+TRIXI_REG_BOTH(3, 4, 5, 128)
+
+// autotuner candidates for the headline volume integrals
+TRIXI_REG(float, 3, 3, VOL_FD_RANOCHA, 8, 128)
+TRIXI_REG(float, 3, 3, VOL_FD_RANOCHA, 16, 256)
+TRIXI_REG(float, 3, 3, VOL_FD_RANOCHA, 8, 64)
+TRIXI_REG(double, 3, 3, VOL_FD_RANOCHA, 8, 128)
+TRIXI_REG(double, 3, 3, VOL_FD_RANOCHA, 2, 64)
+TRIXI_REG(float, 3, 4, VOL_FD_RANOCHA, 2, 64)
+TRIXI_REG(float, 3, 4, VOL_FD_RANOCHA, 10, 256)
+TRIXI_REG(double, 3, 4, VOL_FD_RANOCHA, 2, 64)
+TRIXI_REG(float, 2, 3, VOL_HG_RANOCHA, 32, 128)
+TRIXI_REG(float, 2, 3, VOL_HG_RANOCHA, 64, 256)
+
+struct DefaultTile {
+  int D, N, epb, nt;
+};
+const DefaultTile kDefaultTiles[] = {{2, 2, 21, 64}, {2, 3, 16, 64}, {2, 4, 12, 64},
+                                     {3, 2, 7, 64},  {3, 3, 4, 64},  {3, 4, 5, 128}};
+
+int vol_kind(const trixi_hip_config &c) {
+  switch (c.volume_integral) {
+    case TRIXI_HIP_VI_WEAK: return VOL_WEAK;
+    case TRIXI_HIP_VI_FLUX_DIFFERENCING:
+      return c.volume_flux == TRIXI_HIP_FLUX_RANOCHA ? VOL_FD_RANOCHA
+             : c.volume_flux == TRIXI_HIP_FLUX_CENTRAL ? VOL_FD_CENTRAL : -1;
+    case TRIXI_HIP_VI_SHOCK_CAPTURING_HG:
+      return c.volume_flux == TRIXI_HIP_FLUX_RANOCHA ? VOL_HG_RANOCHA
+             : c.volume_flux == TRIXI_HIP_FLUX_CENTRAL ? VOL_HG_CENTRAL : -1;
+    case TRIXI_HIP_VI_PURE_FV: return VOL_PURE_FV;
+    default: return -1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Aux-kernel dispatch on (T, D, N) via a compile-time list.
+// ---------------------------------------------------------------------------
+template <typename F> bool dispatch_dn(int D, int N, F &&f) {
+#define TRIXI_DN(DD, NN_) \
+  if (D == DD && N == NN_) { f(std::integral_constant<int, DD>{}, std::integral_constant<int, NN_>{}); return true; }
+  TRIXI_DN(2, 2) TRIXI_DN(2, 3) TRIXI_DN(2, 4) TRIXI_DN(3, 2) TRIXI_DN(3, 3) TRIXI_DN(3, 4)
+#undef TRIXI_DN
+  return false;
+}
+
+constexpr int kMaxPartials = 1 << 16;
+
+}  // namespace
+
+struct trixi_hip_ctx {
+  trixi_hip_config cfg{};
+  int D = 0, N = 0, n = 0, NN = 0, V = 0, E = 0, I = 0;
+  size_t nvals = 0;  // V * NN * E
+  bool fp32 = false;
+  int vol = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  std::string err;
+  int device = 0;
+
+  // mesh / basis on the device
+  int32_t *d_nbr = nullptr;
+  double *d_invj = nullptr, *d_centers = nullptr;
+  // resident state
+  double *d_u = nullptr, *d_utmp = nullptr;
+  float *d_u32 = nullptr;
+  void *d_du = nullptr;  // native precision
+  // staging for trixi_hip_rhs (FP64 interface)
+  double *d_hu = nullptr, *d_hdu = nullptr;
+  float *d_su32 = nullptr, *d_sdu32 = nullptr;
+  // HG
+  double *d_alpha_raw = nullptr, *d_alpha = nullptr;
+  // reductions
+  double *d_partial = nullptr, *d_scalars = nullptr;  // scalars: [0..1] {t, dt}, [2] max_dt, [3..] reductions
+  size_t partial_cap = 0;
+
+  // host copies of basis
+  std::vector<double> nodes, weights, Dsplit, Dhat, Vinv, a_weights, a_vdm;
+  int na = 0;
+  HGParams hg{};
+  ExactParams exact{};
+
+  // element-kernel launch
+  ElemLaunch elem = nullptr;
+  int epb = 0, nt = 0;
+  std::vector<unsigned char> kconst;  // DGConst<T, N> blob
+  int xcd_remap = 1;
+
+  // graph of one RK step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<double> graph_key;
+};
+
+namespace {
+
+int set_err(trixi_hip_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return set_err(ctx, TRIXI_HIP_EHIP, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T> void build_kconst(trixi_hip_ctx *ctx) {
+  dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int N = decltype(Nc)::value;
+    constexpr int n = N + 1;
+    DGConst<T, N> k{};
+    for (int i = 0; i < n * n; ++i) {
+      k.Dsplit[i] = T(ctx->Dsplit[i]);
+      k.Dhat[i] = T(ctx->Dhat[i]);
+    }
+    for (int i = 0; i < n; ++i) {
+      k.inv_w[i] = T(1.0 / ctx->weights[i]);
+      k.nodes[i] = ctx->nodes[i];
+    }
+    ctx->kconst.resize(sizeof(k));
+    std::memcpy(ctx->kconst.data(), &k, sizeof(k));
+    (void)Dc;
+  });
+}
+
+int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
+  if (epb <= 0 || nt <= 0) {
+    for (const auto &t : kDefaultTiles)
+      if (t.D == ctx->D && t.N == ctx->N) {
+        epb = t.epb;
+        nt = t.nt;
+      }
+  }
+  auto it = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt});
+  if (it == elem_registry().end())
+    return set_err(ctx, TRIXI_HIP_EINVAL,
+                   "no element kernel instantiated for this (precision, ndims, polydeg, volume "
+                   "integral, tile)");
+  ctx->elem = it->second;
+  ctx->epb = epb;
+  ctx->nt = nt;
+  return TRIXI_HIP_OK;
+}
+
+template <typename T>
+int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *tdev) {
+  // HG: raw indicator first (needs the whole state), smoothing happens inline
+  if (ctx->vol == VOL_HG_RANOCHA || ctx->vol == VOL_HG_CENTRAL) {
+    bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+      constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+      constexpr int NN = ipow_c(N + 1, D);
+      constexpr int IPB = NN >= 256 ? 1 : 256 / NN;
+      VinvConst<N> vc{};
+      for (int i = 0; i < (N + 1) * (N + 1); ++i) vc.Vinv[i] = ctx->Vinv[i];
+      int nb = (ctx->E + IPB - 1) / IPB;
+      hipLaunchKernelGGL((hg_indicator_kernel<T, D, N, IPB, 256>), dim3(nb), dim3(256), 0,
+                         ctx->stream, u, ctx->d_alpha_raw, ctx->E, ctx->hg, vc);
+    });
+    if (!ok) return set_err(ctx, TRIXI_HIP_EINVAL, "unsupported (ndims, polydeg) for the indicator");
+    HIP_TRY(hipGetLastError());
+  }
+  ElemArgs<T> a{};
+  a.u = u;
+  a.du = du;
+  a.nbr = ctx->d_nbr;
+  a.inv_jac = ctx->d_invj;
+  a.centers = ctx->d_centers;
+  a.alpha_raw = ctx->d_alpha_raw;
+  a.alpha_out = ctx->d_alpha;
+  a.tdev = tdev;
+  a.t = t;
+  a.E = ctx->E;
+  a.surface_flux = ctx->cfg.surface_flux;
+  a.fv_flux = ctx->cfg.fv_flux;
+  a.source = ctx->cfg.source_terms == TRIXI_HIP_SRC_CONVERGENCE_TEST;
+  a.alpha_smooth = ctx->cfg.alpha_smooth;
+  a.xcd_remap = ctx->xcd_remap;
+  a.gamma = T(ctx->cfg.gamma);
+  int nblocks = (ctx->E + ctx->epb - 1) / ctx->epb;
+  hipError_t e = ctx->elem(&a, ctx->kconst.data(), nblocks, ctx->stream);
+  if (e != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("element kernel launch: ") + hipGetErrorString(e));
+  return TRIXI_HIP_OK;
+}
+
+int launch_rhs_native(trixi_hip_ctx *ctx, void *du, const void *u, double t, const double *tdev) {
+  if (ctx->fp32) return launch_rhs_T<float>(ctx, (float *)du, (const float *)u, t, tdev);
+  return launch_rhs_T<double>(ctx, (double *)du, (const double *)u, t, tdev);
+}
+
+int grid_for(size_t count) {
+  size_t b = (count + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+template <typename TO, typename TI> int convert(trixi_hip_ctx *ctx, TO *out, const TI *in, size_t count) {
+  hipLaunchKernelGGL((convert_kernel<TO, TI>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, out, in, count);
+  HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
+
+int launch_update(trixi_hip_ctx *ctx, double a_stage, double b_stage, const double *tdev, int first) {
+  size_t cnt = ctx->nvals;
+  if (ctx->fp32)
+    hipLaunchKernelGGL((rk2n_update_kernel<float>), dim3(grid_for(cnt)), dim3(256), 0, ctx->stream,
+                       (const float *)ctx->d_du, ctx->d_utmp, ctx->d_u, ctx->d_u32, cnt, a_stage, b_stage,
+                       tdev, first);
+  else
+    hipLaunchKernelGGL((rk2n_update_kernel<double>), dim3(grid_for(cnt)), dim3(256), 0, ctx->stream,
+                       (const double *)ctx->d_du, ctx->d_utmp, ctx->d_u, (float *)nullptr, cnt, a_stage,
+                       b_stage, tdev, first);
+  HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
+
+__global__ void set_time_kernel(double *tdt, double t, double dt) {
+  if (threadIdx.x == 0) {
+    tdt[0] = t;
+    tdt[1] = dt;
+  }
+}
+
+const void *native_u(trixi_hip_ctx *ctx) { return ctx->fp32 ? (const void *)ctx->d_u32 : (const void *)ctx->d_u; }
+
+}  // namespace
+
+extern "C" {
+
+int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
+                     const trixi_hip_basis *basis, trixi_hip_ctx **out) {
+  if (!cfg || !mesh || !basis || !out) return TRIXI_HIP_EINVAL;
+  *out = nullptr;
+  trixi_hip_ctx *ctx = new trixi_hip_ctx();
+  ctx->cfg = *cfg;
+  ctx->D = cfg->ndims;
+  ctx->N = cfg->polydeg;
+  ctx->n = cfg->polydeg + 1;
+  ctx->V = ctx->D + 2;
+  ctx->E = cfg->nelements;
+  ctx->I = cfg->ninterfaces;
+  auto fail = [&](int code, const char *msg) {
+    fprintf(stderr, "trixi_hip_create: %s\n", msg);
+    delete ctx;
+    return code;
+  };
+  if (ctx->D != 2 && ctx->D != 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 2 or 3");
+  if (ctx->E <= 0 || ctx->I != ctx->D * ctx->E)
+    return fail(TRIXI_HIP_EINVAL, "a conforming periodic TreeMesh has ninterfaces == ndims * nelements");
+  bool dn_ok = dispatch_dn(ctx->D, ctx->N, [](auto, auto) {});
+  if (!dn_ok) return fail(TRIXI_HIP_EINVAL, "unsupported (ndims, polydeg): supported are 2D/3D with polydeg 2..4");
+  ctx->NN = ctx->D == 3 ? ctx->n * ctx->n * ctx->n : ctx->n * ctx->n;
+  ctx->nvals = (size_t)ctx->V * ctx->NN * ctx->E;
+  ctx->fp32 = cfg->precision == TRIXI_HIP_FP32;
+  ctx->vol = vol_kind(*cfg);
+  if (ctx->vol < 0) return fail(TRIXI_HIP_EINVAL, "unsupported volume integral / volume flux");
+  if (cfg->surface_flux < 0 || cfg->surface_flux > 3 || cfg->fv_flux < 0 || cfg->fv_flux > 3)
+    return fail(TRIXI_HIP_EINVAL, "unsupported surface or FV flux");
+  const int n = ctx->n, D = ctx->D, E = ctx->E, I = ctx->I;
+  // element neighbour table from the interface container (containers_3d.jl:229-279):
+  // interface i with orientation o joins left (direction 2o+1 of left) and right (2o of right)
+  std::vector<int32_t> nbr((size_t)2 * D * E, -1);
+  for (int i = 0; i < I; ++i) {
+    int l = mesh->neighbor_ids[2 * i], r = mesh->neighbor_ids[2 * i + 1], o = mesh->orientations[i];
+    if (l < 0 || l >= E || r < 0 || r >= E || o < 0 || o >= D) return fail(TRIXI_HIP_EINVAL, "bad interface data");
+    nbr[(2 * o + 1) + 2 * D * l] = r;
+    nbr[(2 * o) + 2 * D * r] = l;
+  }
+  for (size_t k = 0; k < nbr.size(); ++k)
+    if (nbr[k] < 0) return fail(TRIXI_HIP_EINVAL, "element without a conforming neighbour (mortars/boundaries unsupported)");
+  // basis
+  ctx->nodes.assign(basis->nodes, basis->nodes + n);
+  ctx->weights.assign(basis->weights, basis->weights + n);
+  ctx->Dsplit.assign(basis->derivative_split, basis->derivative_split + n * n);
+  ctx->Dhat.assign(basis->derivative_hat, basis->derivative_hat + n * n);
+  ctx->Vinv.assign(basis->inverse_vandermonde_legendre, basis->inverse_vandermonde_legendre + n * n);
+  ctx->na = basis->analysis_nnodes;
+  if (ctx->na != 2 * ctx->N + 1) return fail(TRIXI_HIP_EINVAL, "analysis basis must have 2*polydeg+1 nodes");
+  ctx->a_weights.assign(basis->analysis_weights, basis->analysis_weights + ctx->na);
+  ctx->a_vdm.assign(basis->analysis_vandermonde, basis->analysis_vandermonde + ctx->na * n);
+  // HG parameters (indicators.jl:130-135)
+  ctx->hg.gamma = cfg->gamma;
+  ctx->hg.alpha_max = cfg->alpha_max;
+  ctx->hg.alpha_min = cfg->alpha_min;
+  ctx->hg.threshold = 0.5 * std::pow(10.0, -1.8 * std::pow((double)n, 0.25));
+  ctx->hg.parameter_s = std::log((1 - 0.0001) / 0.0001);
+  ctx->exact.kind = TRIXI_HIP_IC_CONSTANT;
+  ctx->exact.gamma = cfg->gamma;
+
+  if (ctx->fp32) build_kconst<float>(ctx);
+  else build_kconst<double>(ctx);
+  if (select_tile(ctx, cfg->elements_per_block, cfg->threads_per_block) != TRIXI_HIP_OK) {
+    fprintf(stderr, "trixi_hip_create: %s\n", ctx->err.c_str());
+    delete ctx;
+    return TRIXI_HIP_EINVAL;
+  }
+
+  hipError_t he = hipGetDevice(&ctx->device);
+  if (he != hipSuccess) return fail(TRIXI_HIP_EHIP, "hipGetDevice failed");
+  if (hipStreamCreate(&ctx->own_stream) != hipSuccess)
+    return fail(TRIXI_HIP_EHIP, "hipStreamCreate failed");
+  ctx->stream = ctx->own_stream;
+  const size_t nv = ctx->nvals;
+  const size_t tsz = ctx->fp32 ? 4 : 8;
+  ctx->partial_cap = std::max<size_t>((size_t)E * 2 * ctx->V, kMaxPartials);
+  bool ok = hipMalloc(&ctx->d_nbr, nbr.size() * 4) == hipSuccess &&
+            hipMalloc(&ctx->d_invj, E * 8) == hipSuccess &&
+            hipMalloc(&ctx->d_centers, (size_t)D * E * 8) == hipSuccess &&
+            hipMalloc(&ctx->d_u, nv * 8) == hipSuccess && hipMalloc(&ctx->d_utmp, nv * 8) == hipSuccess &&
+            hipMalloc(&ctx->d_du, nv * tsz) == hipSuccess &&
+            hipMalloc(&ctx->d_alpha_raw, E * 8) == hipSuccess && hipMalloc(&ctx->d_alpha, E * 8) == hipSuccess &&
+            hipMalloc(&ctx->d_partial, ctx->partial_cap * 8) == hipSuccess &&
+            hipMalloc(&ctx->d_scalars, 64 * 8) == hipSuccess;
+  if (ok && ctx->fp32)
+    ok = hipMalloc(&ctx->d_u32, nv * 4) == hipSuccess && hipMalloc(&ctx->d_su32, nv * 4) == hipSuccess &&
+         hipMalloc(&ctx->d_sdu32, nv * 4) == hipSuccess;
+  if (!ok) {
+    trixi_hip_destroy(ctx);
+    fprintf(stderr, "trixi_hip_create: device allocation failed\n");
+    return TRIXI_HIP_ENOMEM;
+  }
+  std::vector<double> zeros(E, 0.0);
+  ok = hipMemcpy(ctx->d_nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(ctx->d_invj, mesh->inverse_jacobian, E * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(ctx->d_centers, mesh->element_centers, (size_t)D * E * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(ctx->d_alpha_raw, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) {
+    trixi_hip_destroy(ctx);
+    fprintf(stderr, "trixi_hip_create: upload failed\n");
+    return TRIXI_HIP_EHIP;
+  }
+  *out = ctx;
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_destroy(trixi_hip_ctx *ctx) {
+  if (!ctx) return TRIXI_HIP_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_u, ctx->d_utmp, ctx->d_u32, ctx->d_du,
+                  ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
+                  ctx->d_partial, ctx->d_scalars};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+  return TRIXI_HIP_OK;
+}
+
+const char *trixi_hip_last_error(const trixi_hip_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int trixi_hip_set_stream(trixi_hip_ctx *ctx, void *hip_stream) {
+  if (!ctx) return TRIXI_HIP_EINVAL;
+  ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_synchronize(trixi_hip_ctx *ctx) {
+  if (!ctx) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_upload_u(trixi_hip_ctx *ctx, const double *u_host) {
+  if (!ctx || !u_host) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipMemcpyAsync(ctx->d_u, u_host, ctx->nvals * 8, hipMemcpyHostToDevice, ctx->stream));
+  if (ctx->fp32) {
+    int r = convert<float, double>(ctx, ctx->d_u32, ctx->d_u, ctx->nvals);
+    if (r) return r;
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_download_u(trixi_hip_ctx *ctx, double *u_host) {
+  if (!ctx || !u_host) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipMemcpyAsync(u_host, ctx->d_u, ctx->nvals * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_set_u_device(trixi_hip_ctx *ctx, const double *u_dev) {
+  if (!ctx || !u_dev) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipMemcpyAsync(ctx->d_u, u_dev, ctx->nvals * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  if (ctx->fp32) {
+    int r = convert<float, double>(ctx, ctx->d_u32, ctx->d_u, ctx->nvals);
+    if (r) return r;
+  }
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_get_u_device(trixi_hip_ctx *ctx, double *u_dev) {
+  if (!ctx || !u_dev) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipMemcpyAsync(u_dev, ctx->d_u, ctx->nvals * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_rhs(trixi_hip_ctx *ctx, double *du, const double *u, double t, int where) {
+  if (!ctx || !du || !u) return TRIXI_HIP_EINVAL;
+  const size_t nv = ctx->nvals;
+  const double *du_in = u;
+  double *du_out = du;
+  if (where == TRIXI_HIP_HOST) {
+    if (!ctx->d_hu) {
+      HIP_TRY(hipMalloc(&ctx->d_hu, nv * 8));
+      HIP_TRY(hipMalloc(&ctx->d_hdu, nv * 8));
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->d_hu, u, nv * 8, hipMemcpyHostToDevice, ctx->stream));
+    du_in = ctx->d_hu;
+    du_out = ctx->d_hdu;
+  } else if (where != TRIXI_HIP_DEVICE) {
+    return set_err(ctx, TRIXI_HIP_EINVAL, "where must be TRIXI_HIP_HOST or TRIXI_HIP_DEVICE");
+  } else if (((uintptr_t)u % 16) || ((uintptr_t)du % 16)) {
+    return set_err(ctx, TRIXI_HIP_EINVAL, "device arrays must be 16-byte aligned");
+  }
+  int r;
+  if (ctx->fp32) {
+    if ((r = convert<float, double>(ctx, ctx->d_su32, du_in, nv))) return r;
+    if ((r = launch_rhs_T<float>(ctx, ctx->d_sdu32, ctx->d_su32, t, nullptr))) return r;
+    if ((r = convert<double, float>(ctx, du_out, ctx->d_sdu32, nv))) return r;
+  } else {
+    if ((r = launch_rhs_T<double>(ctx, du_out, du_in, t, nullptr))) return r;
+  }
+  if (where == TRIXI_HIP_HOST) {
+    HIP_TRY(hipMemcpyAsync(du, du_out, nv * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_rhs_native(trixi_hip_ctx *ctx, void *du_dev, const void *u_dev, double t) {
+  if (!ctx || !du_dev || !u_dev) return TRIXI_HIP_EINVAL;
+  if (((uintptr_t)u_dev % 16) || ((uintptr_t)du_dev % 16))
+    return set_err(ctx, TRIXI_HIP_EINVAL, "device arrays must be 16-byte aligned");
+  return launch_rhs_native(ctx, du_dev, u_dev, t, nullptr);
+}
+
+int trixi_hip_rk_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage_dt, double t_stage,
+                       int first_stage) {
+  if (!ctx) return TRIXI_HIP_EINVAL;
+  int r = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), t_stage, nullptr);
+  if (r) return r;
+  return launch_update(ctx, a_stage, b_stage_dt, nullptr, first_stage);
+}
+
+int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, const double *c,
+                      int nstages, double t, double dt, int use_graph) {
+  if (!ctx || !a || !b || !c || nstages <= 0 || nstages > 16) return TRIXI_HIP_EINVAL;
+  double *tdt = ctx->d_scalars;
+  hipLaunchKernelGGL(set_time_kernel, dim3(1), dim3(1), 0, ctx->stream, tdt, t, dt);
+  HIP_TRY(hipGetLastError());
+  auto record = [&]() -> int {
+    for (int s = 0; s < nstages; ++s) {
+      // stage kernels read t and dt from device memory; ElemArgs.t carries c_stage
+      int r = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), c[s], tdt);
+      if (r) return r;
+      r = launch_update(ctx, a[s], b[s], tdt, s == 0);
+      if (r) return r;
+    }
+    return TRIXI_HIP_OK;
+  };
+  if (!use_graph) return record();
+  std::vector<double> key;
+  key.push_back(nstages);
+  for (int s = 0; s < nstages; ++s) {
+    key.push_back(a[s]);
+    key.push_back(b[s]);
+    key.push_back(c[s]);
+  }
+  key.push_back((double)(uintptr_t)ctx->stream);
+  if (!ctx->graph_exec || key != ctx->graph_key) {
+    if (ctx->graph_exec) {
+      (void)hipGraphExecDestroy(ctx->graph_exec);
+      ctx->graph_exec = nullptr;
+    }
+    if (ctx->graph) {
+      (void)hipGraphDestroy(ctx->graph);
+      ctx->graph = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int r = record();
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (r) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    if (ec != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("end capture: ") + hipGetErrorString(ec));
+    ctx->graph = g;
+    HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+    ctx->graph_key = key;
+  }
+  HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt) {
+  if (!ctx || !dt) return TRIXI_HIP_EINVAL;
+  int nb = (ctx->E + 255) / 256;
+  if ((size_t)nb > ctx->partial_cap) return set_err(ctx, TRIXI_HIP_EINVAL, "too many elements");
+  dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    hipLaunchKernelGGL((max_speed_kernel<double, D, N>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d_u,
+                       ctx->d_invj, ctx->E, ctx->cfg.gamma, ctx->d_partial);
+  });
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(max_dt_finalize_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, nb, ctx->n,
+                     cfl, ctx->d_scalars + 2);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(dt, ctx->d_scalars + 2, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host) {
+  if (!ctx || mode != TRIXI_HIP_IND_HG) return TRIXI_HIP_EINVAL;
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    constexpr int NN = ipow_c(N + 1, D);
+    constexpr int IPB = NN >= 256 ? 1 : 256 / NN;
+    VinvConst<N> vc{};
+    for (int i = 0; i < (N + 1) * (N + 1); ++i) vc.Vinv[i] = ctx->Vinv[i];
+    int nb = (ctx->E + IPB - 1) / IPB;
+    if (ctx->fp32)
+      hipLaunchKernelGGL((hg_indicator_kernel<float, D, N, IPB, 256>), dim3(nb), dim3(256), 0, ctx->stream,
+                         ctx->d_u32, ctx->d_alpha_raw, ctx->E, ctx->hg, vc);
+    else
+      hipLaunchKernelGGL((hg_indicator_kernel<double, D, N, IPB, 256>), dim3(nb), dim3(256), 0, ctx->stream,
+                         ctx->d_u, ctx->d_alpha_raw, ctx->E, ctx->hg, vc);
+    hipLaunchKernelGGL((hg_smooth_kernel<D>), dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_alpha_raw, ctx->d_nbr, ctx->d_alpha, ctx->E, ctx->cfg.alpha_smooth);
+  });
+  if (!ok) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipGetLastError());
+  if (alpha_host) {
+    HIP_TRY(hipMemcpyAsync(alpha_host, ctx->d_alpha, ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *params, int nparams,
+                                 const double *modes) {
+  if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_FOURIER_MODES) return TRIXI_HIP_EINVAL;
+  ctx->exact.kind = ic_kind;
+  ctx->exact.gamma = ctx->cfg.gamma;
+  for (int i = 0; i < 4; ++i) ctx->exact.params[i] = (params && i < nparams) ? params[i] : 0.0;
+  if (ic_kind == TRIXI_HIP_IC_FOURIER_MODES) {
+    if (!modes) return set_err(ctx, TRIXI_HIP_EINVAL, "Fourier-mode exact solution needs modes");
+    std::memcpy(ctx->exact.modes, modes, sizeof(ctx->exact.modes));
+  }
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_error_norms(trixi_hip_ctx *ctx, double t, double *l2, double *linf) {
+  if (!ctx || !l2 || !linf) return TRIXI_HIP_EINVAL;
+  const int V = ctx->V, E = ctx->E;
+  if ((size_t)E * 2 * V > ctx->partial_cap) return set_err(ctx, TRIXI_HIP_EINVAL, "partials too small");
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    constexpr int NA = 2 * N + 1;
+    AnalysisConst<N, NA> ac{};
+    for (int i = 0; i < NA * (N + 1); ++i) ac.vdm[i] = ctx->a_vdm[i];
+    for (int i = 0; i < NA; ++i) ac.aw[i] = ctx->a_weights[i];
+    for (int i = 0; i <= N; ++i) ac.nodes[i] = ctx->nodes[i];
+    hipLaunchKernelGGL((error_norms_kernel<double, D, N, NA>), dim3(E), dim3(256), 0, ctx->stream, ctx->d_u,
+                       ctx->d_invj, ctx->d_centers, E, t, ctx->exact, ac, ctx->d_partial);
+  });
+  if (!ok) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(reduce_columns_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, E, 2 * V, V,
+                     ctx->d_scalars + 8);
+  HIP_TRY(hipGetLastError());
+  double res[16];
+  HIP_TRY(hipMemcpyAsync(res, ctx->d_scalars + 8, 2 * V * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  double tv = 0;
+  {
+    // total_volume(mesh) = length_level_0^ndims = sum of element volumes
+    std::vector<double> ij(E);
+    HIP_TRY(hipMemcpy(ij.data(), ctx->d_invj, E * 8, hipMemcpyDeviceToHost));
+    for (int e = 0; e < E; ++e) tv += std::pow(1.0 / ij[e], ctx->D) * std::pow(2.0, ctx->D);
+  }
+  for (int v = 0; v < V; ++v) {
+    l2[v] = std::sqrt(res[v] / tv);
+    linf[v] = res[V + v];
+  }
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_integrals(trixi_hip_ctx *ctx, double *cons_integrals, double *entropy_integral) {
+  if (!ctx || !cons_integrals || !entropy_integral) return TRIXI_HIP_EINVAL;
+  const int V = ctx->V, E = ctx->E;
+  if ((size_t)E * (V + 1) > ctx->partial_cap) return set_err(ctx, TRIXI_HIP_EINVAL, "partials too small");
+  double *d_w = ctx->d_scalars + 32;
+  HIP_TRY(hipMemcpyAsync(d_w, ctx->weights.data(), ctx->n * 8, hipMemcpyHostToDevice, ctx->stream));
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    hipLaunchKernelGGL((integrals_kernel<double, D, N>), dim3((E + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_u, ctx->d_invj, E, ctx->cfg.gamma, d_w, ctx->d_partial);
+  });
+  if (!ok) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(reduce_columns_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, E, V + 1, V + 1,
+                     ctx->d_scalars + 8);
+  HIP_TRY(hipGetLastError());
+  double res[8];
+  HIP_TRY(hipMemcpyAsync(res, ctx->d_scalars + 8, (V + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  std::vector<double> ij(E);
+  HIP_TRY(hipMemcpy(ij.data(), ctx->d_invj, E * 8, hipMemcpyDeviceToHost));
+  double tv = 0;
+  for (int e = 0; e < E; ++e) tv += std::pow(2.0 / ij[e], ctx->D);
+  for (int v = 0; v < V; ++v) cons_integrals[v] = res[v] / tv;
+  *entropy_integral = res[V] / tv;
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_get_tile(const trixi_hip_ctx *ctx, int *elements_per_block, int *threads_per_block) {
+  if (!ctx) return TRIXI_HIP_EINVAL;
+  if (elements_per_block) *elements_per_block = ctx->epb;
+  if (threads_per_block) *threads_per_block = ctx->nt;
+  return TRIXI_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+"""trixi_hip -- host-side mirror of Trixi.jl's interface for the MI355X DGSEM
+right-hand side of the compressible Euler equations on a periodic TreeMesh.
+
+    mesh   = TreeMesh((0, 0, 0), (2, 2, 2), initial_refinement_level=5)
+    solver = DGSEM(polydeg=3, surface_flux=flux_lax_friedrichs,
+                   volume_integral=VolumeIntegralFluxDifferencing(flux_ranocha))
+    semi   = SemidiscretizationHyperbolic(mesh, CompressibleEulerEquations3D(1.4),
+                                          initial_condition_convergence_test, solver,
+                                          source_terms=source_terms_convergence_test,
+                                          precision="fp32")
+    ode    = semidiscretize(semi, (0.0, 1.0))
+    sol    = solve(ode, CarpenterKennedy2N54(), callback=CallbackSet(StepsizeCallback(cfl=0.6)))
+
+All numerical work runs in libtrixi_hip.so (HIP, gfx950) through the C ABI of
+include/trixi_hip.h.
+"""
+from . import _lib
+from .basis import LobattoLegendreBasis
+from .equations import (CompressibleEulerEquations2D, CompressibleEulerEquations3D, FluxLaxFriedrichs,
+                        flux_central, flux_lax_friedrichs, flux_ranocha, fourier_modes,
+                        initial_condition_constant, initial_condition_convergence_test,
+                        initial_condition_fourier_modes, initial_condition_isentropic_vortex,
+                        initial_condition_kelvin_helmholtz_instability, initial_condition_weak_blast_wave,
+                        max_abs_speed, max_abs_speed_naive, source_terms_convergence_test)
+from .mesh import TreeMesh
+from .semidiscretization import ODEProblem, SemidiscretizationHyperbolic, rhs_hyperbolic_, semidiscretize
+from .solver import (DGSEM, IndicatorHennemannGassner, VolumeIntegralFluxDifferencing,
+                     VolumeIntegralPureLGLFiniteVolume, VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm,
+                     density_pressure)
+from .time_integration import AnalysisCallback, CallbackSet, CarpenterKennedy2N54, StepsizeCallback, solve
+
+__all__ = [n for n in dir() if not n.startswith("_")]

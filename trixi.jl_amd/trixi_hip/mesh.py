@@ -1,0 +1,118 @@
+"""Uniform periodic TreeMesh (host-side setup, no device work).
+
+Restates the parts of Trixi.jl that fix element order and geometry:
+  TreeMesh(coordinates_min, coordinates_max; initial_refinement_level, periodicity)
+      src/meshes/tree_mesh.jl:128-185
+  refine_uniformly! / init_children! (depth-first, children x fastest)
+      src/meshes/abstract_tree.jl:206-250, child_sign table :118-128,
+      child_coordinates :668-674; leaf order = ascending cell id :154-166
+  init_neighbors! with the periodic level-0 cell its own neighbour
+      abstract_tree.jl:252-313, serial_tree.jl:101-140
+  init_elements! (inverse_jacobian, node_coordinates)
+      src/solvers/dgsem_tree/containers_3d.jl:87-128 (2D: containers_2d.jl)
+  init_interfaces! (per element, positive directions, left = element)
+      containers_3d.jl:229-279
+"""
+import numpy as np
+
+
+class TreeMesh:
+    def __init__(self, coordinates_min, coordinates_max, initial_refinement_level,
+                 periodicity=True):
+        cmin = tuple(float(c) for c in coordinates_min)
+        cmax = tuple(float(c) for c in coordinates_max)
+        if len(cmin) not in (2, 3) or len(cmin) != len(cmax):
+            raise ValueError("TreeMesh supports 2D and 3D here")
+        if not (isinstance(initial_refinement_level, int) and initial_refinement_level >= 0):
+            raise ValueError("`initial_refinement_level` must be a non-negative integer")
+        if periodicity is not True and not (isinstance(periodicity, (tuple, list)) and all(periodicity)):
+            raise ValueError("only fully periodic TreeMesh is supported by the HIP RHS")
+        self.ndims = len(cmin)
+        self.coordinates_min = cmin
+        self.coordinates_max = cmax
+        self.initial_refinement_level = initial_refinement_level
+        length = cmax[0] - cmin[0]
+        for d in range(1, self.ndims):
+            if not np.isclose(cmax[d] - cmin[d], length):
+                raise ValueError("The TreeMesh domain must be a hypercube")
+        self.length_level_0 = length
+        self.center_level_0 = tuple((a + b) / 2 for a, b in zip(cmin, cmax))
+        self._build()
+
+    def _build(self):
+        D, L = self.ndims, self.initial_refinement_level
+        centers, icoords = [], []
+
+        def rec(level, center, length, ic):
+            if level >= L:
+                centers.append(center)
+                icoords.append(ic)
+                return
+            child_length = length / 2
+            for child in range(1 << D):
+                cc, ci = [], []
+                for d in range(D):
+                    sgn = 1 if (child >> d) & 1 else -1
+                    cc.append(center[d] + sgn * child_length / 2)
+                    ci.append(2 * ic[d] + (1 if sgn > 0 else 0))
+                rec(level + 1, tuple(cc), child_length, tuple(ci))
+
+        rec(0, self.center_level_0, self.length_level_0, (0,) * D)
+        self.nelements = len(centers)
+        self.cell_centers = np.array(centers, dtype=np.float64)  # [E, D]
+        self.cell_length = self.length_level_0 / (1 << L)
+        nside = 1 << L
+        ic = np.array(icoords, dtype=np.int64)
+        lin = np.zeros(self.nelements, dtype=np.int64)
+        s = 1
+        for d in range(D):
+            lin += ic[:, d] * s
+            s *= nside
+        lookup = np.empty(nside ** D, dtype=np.int64)
+        lookup[lin] = np.arange(self.nelements)
+        # element neighbour per direction (0-based: 2o = -o, 2o+1 = +o)
+        nbr = np.empty((self.nelements, 2 * D), dtype=np.int32)
+        for dirn in range(2 * D):
+            o = dirn // 2
+            shifted = ic.copy()
+            shifted[:, o] = (shifted[:, o] + (1 if dirn % 2 else -1)) % nside
+            l2 = np.zeros(self.nelements, dtype=np.int64)
+            s = 1
+            for d in range(D):
+                l2 += shifted[:, d] * s
+                s *= nside
+            nbr[:, dirn] = lookup[l2]
+        self.element_neighbors = nbr
+        # interfaces: element order, positive directions (left = element)
+        E = self.nelements
+        self.neighbor_ids = np.empty((E * D, 2), dtype=np.int32)
+        self.orientations = np.empty(E * D, dtype=np.int32)
+        for o in range(D):
+            self.neighbor_ids[o::D, 0] = np.arange(E)
+            self.neighbor_ids[o::D, 1] = nbr[:, 2 * o + 1]
+            self.orientations[o::D] = o
+        self.ninterfaces = E * D
+
+    def ncells(self):
+        return self.nelements
+
+    def __repr__(self):
+        return (f"TreeMesh{{{self.ndims}}} with {self.nelements} elements on "
+                f"{self.coordinates_min}..{self.coordinates_max}")
+
+
+def element_geometry(mesh, basis):
+    """inverse_jacobian[E] and node_coordinates[E, NN, D] as init_elements! computes them."""
+    D = mesh.ndims
+    n = basis.nnodes()
+    reference_length = float(np.sum(basis.weights))
+    reference_offset = (basis.nodes[0] + basis.nodes[-1]) / 2
+    jacobian = mesh.cell_length / reference_length
+    inv_jac = np.full(mesh.nelements, 1.0 / jacobian)
+    rel = jacobian * (basis.nodes - reference_offset)
+    NN = n ** D
+    idx = np.indices((n,) * D).reshape(D, -1)[::-1]  # idx[d, node], node = i + n j (+ n^2 k)
+    coords = np.empty((mesh.nelements, NN, D))
+    for d in range(D):
+        coords[:, :, d] = mesh.cell_centers[:, d][:, None] + rel[idx[d]][None, :]
+    return inv_jac, coords

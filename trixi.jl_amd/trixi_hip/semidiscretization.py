@@ -1,0 +1,211 @@
+"""SemidiscretizationHyperbolic, semidiscretize and rhs_hyperbolic_ (the drop-in).
+
+Mirrors src/semidiscretization/semidiscretization_hyperbolic.jl:14-76, 578-597
+and src/semidiscretization/semidiscretization.jl:102-153, 224-242. The
+right-hand side is evaluated by the HIP library through the C ABI; there is no
+CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .mesh import element_geometry
+from .solver import (VolumeIntegralFluxDifferencing, VolumeIntegralPureLGLFiniteVolume,
+                     VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class SemidiscretizationHyperbolic:
+    """`SemidiscretizationHyperbolic(mesh, equations, initial_condition, solver;
+    source_terms, boundary_conditions)` with a HIP backend.
+
+    precision: "fp64" (plain FP64) or "fp32" (FP32 fluxes, FP64 RK registers
+    and reductions). elements_per_block / threads_per_block override the
+    element-kernel tile (0 = default)."""
+
+    def __init__(self, mesh, equations, initial_condition, solver, source_terms=None,
+                 boundary_conditions="periodic", precision="fp64", elements_per_block=0,
+                 threads_per_block=0):
+        if boundary_conditions not in ("periodic", None):
+            raise ValueError("only boundary_condition_periodic is supported")
+        if mesh.ndims != equations.ndims:
+            raise ValueError("mesh and equations dimension differ")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.mesh = mesh
+        self.equations = equations
+        self.initial_condition = initial_condition
+        self.solver = solver
+        self.source_terms = source_terms
+        self.precision = precision
+        self.tile = (int(elements_per_block), int(threads_per_block))
+        self.inverse_jacobian, self.node_coordinates = element_geometry(mesh, solver.basis)
+        self._ctx = None
+        self._keep = []
+        self.performance_counter = {"ncalls": 0, "time": 0.0}
+
+    # -- sizes ---------------------------------------------------------------
+    def nvariables(self):
+        return self.equations.nvariables()
+
+    def nnodes(self):
+        return self.solver.nnodes()
+
+    def nelements(self):
+        return self.mesh.nelements
+
+    def ndofs(self):
+        """`ndofs(mesh, dg, cache)` = nelements * nnodes^ndims (dg.jl:969-971)."""
+        return self.mesh.nelements * self.nnodes() ** self.mesh.ndims
+
+    def nvalues(self):
+        return self.ndofs() * self.nvariables()
+
+    # -- C ABI configuration ---------------------------------------------------
+    def config(self):
+        vi = self.solver.volume_integral
+        cfg = _lib.Config()
+        cfg.ndims = self.mesh.ndims
+        cfg.polydeg = self.solver.polydeg()
+        cfg.nelements = self.mesh.nelements
+        cfg.ninterfaces = self.mesh.ninterfaces
+        cfg.gamma = self.equations.gamma
+        cfg.precision = _lib.FP32 if self.precision == "fp32" else _lib.FP64
+        cfg.volume_integral = vi.code
+        cfg.volume_flux = _lib.FLUX_RANOCHA
+        cfg.fv_flux = _lib.FLUX_LLF
+        cfg.alpha_max, cfg.alpha_min, cfg.alpha_smooth = 0.5, 0.001, 1
+        if isinstance(vi, VolumeIntegralFluxDifferencing):
+            cfg.volume_flux = vi.volume_flux.code
+        elif isinstance(vi, VolumeIntegralShockCapturingHG):
+            cfg.volume_flux = vi.volume_flux_dg.code
+            cfg.fv_flux = vi.volume_flux_fv.code
+            ind = vi.indicator
+            cfg.alpha_max, cfg.alpha_min, cfg.alpha_smooth = ind.alpha_max, ind.alpha_min, int(ind.alpha_smooth)
+        elif isinstance(vi, VolumeIntegralPureLGLFiniteVolume):
+            cfg.fv_flux = vi.volume_flux_fv.code
+        elif not isinstance(vi, VolumeIntegralWeakForm):
+            raise ValueError(f"unsupported volume integral {vi}")
+        cfg.surface_flux = self.solver.surface_flux.code
+        cfg.source_terms = self.source_terms.code if self.source_terms is not None else _lib.SRC_NONE
+        cfg.elements_per_block, cfg.threads_per_block = self.tile
+        return cfg
+
+    @property
+    def ctx(self):
+        """The HIP context (created on first use; needs a GPU)."""
+        if self._ctx is None:
+            lib = _lib.load()
+            b = self.solver.basis
+            arrays = dict(
+                nbr=np.ascontiguousarray(self.mesh.neighbor_ids, dtype=np.int32),
+                ori=np.ascontiguousarray(self.mesh.orientations, dtype=np.int32),
+                invj=np.ascontiguousarray(self.inverse_jacobian, dtype=np.float64),
+                cen=np.ascontiguousarray(self.mesh.cell_centers, dtype=np.float64),
+                nodes=np.ascontiguousarray(b.nodes), weights=np.ascontiguousarray(b.weights),
+                ds=np.ascontiguousarray(b.derivative_split), dh=np.ascontiguousarray(b.derivative_hat),
+                vinv=np.ascontiguousarray(b.inverse_vandermonde_legendre),
+                aw=np.ascontiguousarray(b.analysis_weights), av=np.ascontiguousarray(b.analysis_vandermonde))
+            mesh = _lib.Mesh(arrays["nbr"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                             arrays["ori"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                             _ptr(arrays["invj"]), _ptr(arrays["cen"]))
+            basis = _lib.Basis(_ptr(arrays["nodes"]), _ptr(arrays["weights"]), _ptr(arrays["ds"]),
+                               _ptr(arrays["dh"]), _ptr(arrays["vinv"]), len(b.analysis_nodes),
+                               _ptr(arrays["aw"]), _ptr(arrays["av"]))
+            cfg = self.config()
+            out = ctypes.c_void_p()
+            rc = lib.trixi_hip_create(ctypes.byref(cfg), ctypes.byref(mesh), ctypes.byref(basis),
+                                      ctypes.byref(out))
+            _lib.check(rc, None, "trixi_hip_create")
+            self._ctx = out
+            self._lib = lib
+            ic = self.initial_condition
+            params = np.array(list(ic.params) + [0.0] * 4, dtype=np.float64)[:4]
+            modes = np.ascontiguousarray(ic.modes if ic.modes is not None else np.zeros((4, 8, 4)),
+                                         dtype=np.float64)
+            _lib.check(lib.trixi_hip_set_exact_solution(out, ic.code, _ptr(params), len(ic.params),
+                                                        _ptr(modes)), out, "set_exact_solution")
+        return self._ctx
+
+    def tile_used(self):
+        e, t = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.trixi_hip_get_tile(self.ctx, ctypes.byref(e), ctypes.byref(t)), self.ctx, "get_tile")
+        return e.value, t.value
+
+    def close(self):
+        if self._ctx is not None:
+            self._lib.trixi_hip_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup -----------------------------------------------------------------
+    def compute_coefficients(self, t):
+        """`compute_coefficients(t, semi)` (semidiscretization.jl:224-242): the
+        initial condition at every node, as the flat u_ode vector [V, NN, E]."""
+        u = self.initial_condition(self.node_coordinates, t, self.equations)  # [E, NN, V]
+        return np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+
+    def __repr__(self):
+        return (f"SemidiscretizationHyperbolic({self.mesh}, {self.equations}, {self.initial_condition}, "
+                f"{self.solver}, precision={self.precision})")
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def rhs_hyperbolic_(du_ode, u_ode, semi, t):
+    """`rhs_hyperbolic!(du_ode, u_ode, semi, t)` (semidiscretization_hyperbolic.jl:578-597).
+
+    u_ode / du_ode: flat float64 arrays of length nvariables*ndofs, either numpy
+    (host memory) or torch CUDA tensors (device memory, torch's current stream)."""
+    ctx = semi.ctx
+    lib = semi._lib
+    n = semi.nvalues()
+    if _is_torch(u_ode):
+        import torch
+        if not (u_ode.is_cuda and du_ode.is_cuda and u_ode.dtype == torch.float64 and du_ode.dtype == torch.float64):
+            raise TypeError("device arrays must be float64 CUDA tensors")
+        if u_ode.numel() != n or du_ode.numel() != n or not (u_ode.is_contiguous() and du_ode.is_contiguous()):
+            raise ValueError("u_ode/du_ode must be contiguous with nvariables*ndofs entries")
+        _lib.check(lib.trixi_hip_set_stream(ctx, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   ctx, "set_stream")
+        rc = lib.trixi_hip_rhs(ctx, ctypes.c_void_p(du_ode.data_ptr()), ctypes.c_void_p(u_ode.data_ptr()),
+                               float(t), _lib.DEVICE)
+        _lib.check(rc, ctx, "trixi_hip_rhs")
+    else:
+        if u_ode.dtype != np.float64 or du_ode.dtype != np.float64 or u_ode.size != n or du_ode.size != n:
+            raise ValueError("u_ode/du_ode must be float64 with nvariables*ndofs entries")
+        if not (u_ode.flags.c_contiguous and du_ode.flags.c_contiguous):
+            raise ValueError("u_ode/du_ode must be contiguous")
+        _lib.check(lib.trixi_hip_set_stream(ctx, None), ctx, "set_stream")
+        rc = lib.trixi_hip_rhs(ctx, du_ode.ctypes.data_as(ctypes.c_void_p), u_ode.ctypes.data_as(ctypes.c_void_p),
+                               float(t), _lib.HOST)
+        _lib.check(rc, ctx, "trixi_hip_rhs")
+    semi.performance_counter["ncalls"] += 1
+    return None
+
+
+class ODEProblem:
+    """`ODEProblem(rhs_hyperbolic!, u0_ode, tspan, semi)` (semidiscretization.jl:151)."""
+
+    def __init__(self, f, u0, tspan, p):
+        self.f = f
+        self.u0 = u0
+        self.tspan = (float(tspan[0]), float(tspan[1]))
+        self.p = p
+
+
+def semidiscretize(semi, tspan):
+    """`semidiscretize(semi, tspan)` (semidiscretization.jl:102-153)."""
+    u0_ode = semi.compute_coefficients(tspan[0])
+    return ODEProblem(rhs_hyperbolic_, u0_ode, tspan, semi)

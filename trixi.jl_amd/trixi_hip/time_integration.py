@@ -1,0 +1,150 @@
+"""2N low-storage Runge-Kutta integration on the device-resident state, with
+the StepsizeCallback and AnalysisCallback semantics Trixi.jl uses.
+
+  CarpenterKennedy2N54        src/time_integration/methods_2N.jl:24-48
+  SimpleIntegrator2N / step!  methods_2N.jl:95-168 (one graph replay per step)
+  limit_dt!                   src/time_integration/time_integration.jl:46-55
+  StepsizeCallback            src/callbacks_step/stepsize.jl:31-155
+  AnalysisCallback (errors)   src/callbacks_step/analysis.jl, analysis_dg3d.jl:123-161
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+
+
+class CarpenterKennedy2N54:
+    def __init__(self, williamson_condition=False):
+        self.a = np.array([0.0,
+                           567301805773.0 / 1357537059087.0,
+                           2404267990393.0 / 2016746695238.0,
+                           3550918686646.0 / 2091501179385.0,
+                           1275806237668.0 / 842570457699.0])
+        self.b = np.array([1432997174477.0 / 9575080441755.0,
+                           5161836677717.0 / 13612068292357.0,
+                           1720146321549.0 / 2090206949498.0,
+                           3134564353537.0 / 4481467310338.0,
+                           2277821191437.0 / 14882151754819.0])
+        self.c = np.array([0.0,
+                           1432997174477.0 / 9575080441755.0,
+                           2526269341429.0 / 6820363962896.0,
+                           2006345519317.0 / 3224310063776.0,
+                           2802321613138.0 / 2924317926251.0])
+
+
+class StepsizeCallback:
+    """`StepsizeCallback(; cfl)`: dt = cfl * max_dt(u) at initialisation and after every step."""
+
+    def __init__(self, cfl=1.0, interval=1):
+        self.cfl = float(cfl)
+        self.interval = int(interval)
+
+    def calculate_dt(self, semi):
+        dt = ctypes.c_double()
+        _lib.check(semi._lib.trixi_hip_max_dt(semi.ctx, self.cfl, ctypes.byref(dt)), semi.ctx, "max_dt")
+        return dt.value
+
+
+class AnalysisCallback:
+    """Error norms (l2, linf against the initial condition at time t) every
+    `interval` steps and at the end (analysis.jl:46-236, 380-470)."""
+
+    def __init__(self, semi, interval=100):
+        self.semi = semi
+        self.interval = int(interval)
+        self.history = []
+
+    def analyze(self, t, it):
+        semi = self.semi
+        V = semi.nvariables()
+        l2 = (ctypes.c_double * V)()
+        linf = (ctypes.c_double * V)()
+        _lib.check(semi._lib.trixi_hip_error_norms(semi.ctx, float(t), l2, linf), semi.ctx, "error_norms")
+        rec = {"t": t, "iter": it, "l2": list(l2), "linf": list(linf)}
+        self.history.append(rec)
+        return rec
+
+    def integrals(self):
+        semi = self.semi
+        V = semi.nvariables()
+        cons = (ctypes.c_double * V)()
+        ent = ctypes.c_double()
+        _lib.check(semi._lib.trixi_hip_integrals(semi.ctx, cons, ctypes.byref(ent)), semi.ctx, "integrals")
+        return list(cons), ent.value
+
+
+class CallbackSet:
+    def __init__(self, *callbacks):
+        self.callbacks = [c for c in callbacks if c is not None]
+
+    def of(self, kind):
+        return [c for c in self.callbacks if isinstance(c, kind)]
+
+
+class Solution:
+    def __init__(self, u, t, nsteps, analysis, wall, semi):
+        self.u = u
+        self.t = t
+        self.nsteps = nsteps
+        self.analysis = analysis
+        self.wall_time = wall
+        self.semi = semi
+
+
+def _isapprox(a, b):
+    return a == b or abs(a - b) <= math.sqrt(np.finfo(np.float64).eps) * max(abs(a), abs(b))
+
+
+def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_final=True):
+    """`solve(ode, CarpenterKennedy2N54(); dt, callback)` on the device.
+
+    The state lives in the HIP context for the whole run; one call of
+    trixi_hip_rk_step (a hipGraph replay of all stages) advances one step."""
+    semi = ode.p
+    lib = _lib.load()
+    ctx = semi.ctx
+    _lib.check(lib.trixi_hip_set_stream(ctx, None), ctx, "set_stream")
+    _lib.check(lib.trixi_hip_upload_u(ctx, ode.u0.ctypes.data_as(ctypes.c_void_p)), ctx, "upload_u")
+    cbs = callback if isinstance(callback, CallbackSet) else CallbackSet(*(callback or []))
+    stepsize = cbs.of(StepsizeCallback)
+    analysis = cbs.of(AnalysisCallback)
+    t0, t_end = ode.tspan
+    t = t0
+    if stepsize:
+        dt = stepsize[0].calculate_dt(semi)
+    if dt is None:
+        raise ValueError("give dt or a StepsizeCallback")
+    na = len(alg.a)
+    a = (ctypes.c_double * na)(*alg.a)
+    b = (ctypes.c_double * na)(*alg.b)
+    c = (ctypes.c_double * na)(*alg.c)
+    it = 0
+    wall0 = time.perf_counter()
+    while True:
+        if math.isnan(dt):
+            raise FloatingPointError("time step size `dt` is NaN")
+        final = False
+        if t + dt > t_end or _isapprox(t + dt, t_end):
+            dt = t_end - t
+            final = True
+        _lib.check(lib.trixi_hip_rk_step(ctx, a, b, c, na, float(t), float(dt), int(use_graph)), ctx, "rk_step")
+        it += 1
+        t += dt
+        if stepsize and (it % stepsize[0].interval == 0):
+            dt = stepsize[0].calculate_dt(semi)
+        if analysis and analysis[0].interval > 0 and it % analysis[0].interval == 0 and not final:
+            analysis[0].analyze(t, it)
+        if final or (maxiters is not None and it >= maxiters):
+            break
+    _lib.check(lib.trixi_hip_synchronize(ctx), ctx, "synchronize")
+    wall = time.perf_counter() - wall0
+    if analysis:
+        analysis[0].analyze(t, it)
+    u = None
+    if save_final:
+        u = np.empty(semi.nvalues())
+        _lib.check(lib.trixi_hip_download_u(ctx, u.ctypes.data_as(ctypes.c_void_p)), ctx, "download_u")
+    return Solution(u, t, it, analysis[0].history if analysis else [], wall, semi)
