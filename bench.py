@@ -1,0 +1,282 @@
+"""Benchmark: DOF-updates/s of the DGSEM compressible-Euler RHS on one MI355X.
+
+Workload (the metric's configuration): 3D Euler, periodic TreeMesh on [0,2]^3
+at initial_refinement_level 5 (32^3 = 32768 elements), polydeg 3 (64 LGL
+nodes per element, 2,097,152 DOFs), VolumeIntegralFluxDifferencing(flux_ranocha),
+surface flux_lax_friedrichs, source_terms_convergence_test, state
+initial_condition_convergence_test at t = 0.
+
+One "step" = one right-hand-side evaluation over the whole mesh with inputs
+resident in HBM. The FP32 RHS (FP32 fluxes, graph-replayed) is the headline;
+the plain FP64 RHS is measured beside it, as is the equal-L2-error protocol
+(both precisions integrated to the same final time, density L2 error against
+the exact manufactured solution), and the FP64 CPU oracle on the host cores
+(a port of Trixi's rhs_hyperbolic!, not Trixi itself).
+
+To keep the 256 MB Infinity Cache from serving the state between calls, the
+timed loop rotates over R input/output buffer pairs totalling > 512 MB.
+
+Prints ONE JSON line.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "trixi.jl_amd"))
+
+METRIC = "DOF-updates/s per RHS (3D Euler p3, 32^3): FP32 vs FP64 plain at equal L2 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--level", type=int, default=5)
+    p.add_argument("--polydeg", type=int, default=3)
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-l2", action="store_true")
+    p.add_argument("--only", choices=["fp32", "fp64"], default=None,
+                   help="time one precision only (profiling runs)")
+    return p.parse_args()
+
+
+def make_semi(th, level, polydeg, precision, tile=(0, 0)):
+    mesh = th.TreeMesh((0.0, 0.0, 0.0), (2.0, 2.0, 2.0), initial_refinement_level=level)
+    solver = th.DGSEM(polydeg=polydeg, surface_flux=th.flux_lax_friedrichs,
+                      volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_ranocha))
+    return th.SemidiscretizationHyperbolic(mesh, th.CompressibleEulerEquations3D(1.4),
+                                           th.initial_condition_convergence_test, solver,
+                                           source_terms=th.source_terms_convergence_test,
+                                           precision=precision, elements_per_block=tile[0],
+                                           threads_per_block=tile[1])
+
+
+def time_rhs(th, torch, semi, steps, warmup, u0):
+    """Graph-replayed RHS throughput and per-launch kernel time (HIP events on
+    the launch stream)."""
+    ctx = semi.ctx
+    lib = semi._lib
+    dtype = torch.float32 if semi.precision == "fp32" else torch.float64
+    nbytes_pair = 2 * semi.nvalues() * (4 if dtype == torch.float32 else 8)
+    R = max(2, min(16, math.ceil(600e6 / nbytes_pair)))
+    us = [torch.from_numpy(u0).to(device="cuda", dtype=dtype) for _ in range(R)]
+    dus = [torch.empty_like(us[0]) for _ in range(R)]
+    stream = torch.cuda.Stream()
+    lib.trixi_hip_set_stream(ctx, ctypes.c_void_p(stream.cuda_stream))
+
+    def launch(k):
+        rc = lib.trixi_hip_rhs_native(ctx, ctypes.c_void_p(dus[k].data_ptr()), ctypes.c_void_p(us[k].data_ptr()), 0.0)
+        if rc != 0:
+            raise RuntimeError(lib.trixi_hip_last_error(ctx))
+
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        for k in range(warmup):
+            launch(k % R)
+    torch.cuda.synchronize()
+    # per-launch kernel duration (events bracket each launch on the same stream)
+    nev = min(steps, 64)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(nev)]
+    with torch.cuda.stream(stream):
+        for k in range(nev):
+            ev[k][0].record(stream)
+            launch(k % R)
+            ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    kern_ms = sorted(a.elapsed_time(b) for a, b in ev)
+    kern_avg_ms = sum(kern_ms) / len(kern_ms)
+    # graph of one pass over the R buffer pairs (plus a remainder graph)
+    g_full = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g_full, stream=stream):
+        for k in range(R):
+            launch(k)
+    rem = steps % R
+    g_rem = None
+    if rem:
+        g_rem = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g_rem, stream=stream):
+            for k in range(rem):
+                launch(k)
+    lib.trixi_hip_set_stream(ctx, ctypes.c_void_p(stream.cuda_stream))
+    with torch.cuda.stream(stream):
+        g_full.replay()
+    torch.cuda.synchronize()
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        t_start.record(stream)
+        for _ in range(steps // R):
+            g_full.replay()
+        if g_rem is not None:
+            g_rem.replay()
+        t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
+    # correctness guard on the timed buffers: finite output
+    finite = bool(torch.isfinite(dus[0]).all().item())
+    lib.trixi_hip_set_stream(ctx, None)
+    del g_full, g_rem
+    return dict(ms_per_step=total_ms / steps, kernel_avg_ms=kern_avg_ms, kernel_min_ms=kern_ms[0],
+                wall_s=wall, rotate=R, finite=finite)
+
+
+def equal_l2(th, semi64, semi32, nsteps=100, cfl=0.6):
+    """Integrate both precisions to the same t (about nsteps CK2N54 steps) and
+    compare the density L2 errors against the exact manufactured solution."""
+    out = {}
+    dt0 = th.StepsizeCallback(cfl=cfl)
+    u0 = semi64.compute_coefficients(0.0)
+    lib = semi64._lib
+    lib.trixi_hip_upload_u(semi64.ctx, u0.ctypes.data_as(ctypes.c_void_p))
+    t_end = nsteps * dt0.calculate_dt(semi64)
+    for name, semi in (("fp64", semi64), ("fp32", semi32)):
+        ode = th.semidiscretize(semi, (0.0, t_end))
+        ac = th.AnalysisCallback(semi, interval=0)
+        sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=cfl), ac))
+        out[name] = dict(l2_rho=sol.analysis[-1]["l2"][0], linf_rho=sol.analysis[-1]["linf"][0],
+                         nsteps=sol.nsteps, t=sol.t, u=sol.u, wall_s=sol.wall_time)
+    import numpy as np
+    d = out["fp32"]["u"] - out["fp64"]["u"]
+    nv = semi64.nvariables()
+    diff_l2 = float(np.sqrt(np.mean(d.reshape(-1, nv)[:, 0] ** 2)))
+    diff_linf = float(np.max(np.abs(d.reshape(-1, nv)[:, 0])))
+    return dict(t_end=t_end, steps_fp64=out["fp64"]["nsteps"], steps_fp32=out["fp32"]["nsteps"],
+                l2_rho_fp64=out["fp64"]["l2_rho"], l2_rho_fp32=out["fp32"]["l2_rho"],
+                l2_rho_abs_diff=abs(out["fp32"]["l2_rho"] - out["fp64"]["l2_rho"]),
+                rho_fp32_vs_fp64_rms=diff_l2, rho_fp32_vs_fp64_max=diff_linf,
+                step_wall_ms_fp64=1e3 * out["fp64"]["wall_s"] / out["fp64"]["nsteps"],
+                step_wall_ms_fp32=1e3 * out["fp32"]["wall_s"] / out["fp32"]["nsteps"])
+
+
+def cpu_baseline(level, polydeg, seconds, u0):
+    from oracle import Oracle
+    ncpu = os.cpu_count() or 1
+    threads = min(16, ncpu)
+    Oracle.set_threads(threads)
+    o = Oracle(3, polydeg, level, (0, 0, 0), (2, 2, 2), volume_integral="fd", volume_flux="ranocha",
+               surface_flux="llf", source="convergence_test")
+    o.rhs(u0, 0.0)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.rhs(u0, 0.0)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds:
+            break
+    t_rhs = el / n
+    return dict(value=o.ndofs / t_rhs, unit="DOF-updates/s", cores=threads, kind="port",
+                sample=f"{n} full RHS calls of the same 3D p3 32^3 workload (FP64 C++ OpenMP restatement "
+                       f"of Trixi rhs_hyperbolic!, not Trixi/Julia), {el:.1f} s",
+                time_per_dof_per_rhs_s=t_rhs * threads / o.ndofs)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built "
+              "in this round.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import trixi_hip as th
+    torch.cuda.set_device(0)
+
+    res = {}
+    semis = {}
+    u0 = None
+    for prec in ("fp32", "fp64"):
+        semi = make_semi(th, args.level, args.polydeg, prec)
+        semis[prec] = semi
+        if u0 is None:
+            u0 = semi.compute_coefficients(0.0)
+        if args.only and prec != args.only:
+            continue
+        res[prec] = time_rhs(th, torch, semi, args.steps, args.warmup, u0)
+        res[prec]["tile"] = semi.tile_used()
+        print(f"[bench] {prec}: {res[prec]}", file=sys.stderr, flush=True)
+
+    ndofs = semis["fp32"].ndofs()
+    nv = semis["fp32"].nvariables()
+    l2 = None
+    if not args.no_l2 and not args.only:
+        l2 = equal_l2(th, semis["fp64"], semis["fp32"])
+        print(f"[bench] equal-L2: {l2}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu and not args.only:
+        cpu = cpu_baseline(args.level, args.polydeg, args.cpu_seconds, u0)
+        print(f"[bench] cpu: {cpu}", file=sys.stderr, flush=True)
+
+    head = "fp32" if "fp32" in res else "fp64"
+    r = res[head]
+    bpv = 4 if head == "fp32" else 8
+    alg_bytes = 2 * nv * ndofs * bpv  # read u once + write du once
+    achieved = alg_bytes / (r["kernel_avg_ms"] * 1e-3) / 1e9
+    value = ndofs / (r["ms_per_step"] * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D compressible Euler MMS, TreeMesh level {args.level} "
+                        f"({(2 ** args.level) ** 3} elements), polydeg {args.polydeg}, flux_ranocha FD volume, "
+                        "flux_lax_friedrichs surface, source terms, FP32 fluxes, graph-replayed RHS",
+            "ndofs": ndofs,
+            "nelements": (2 ** args.level) ** 3,
+            "element_tile": list(r["tile"]),
+            "rotating_buffers": r["rotate"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "dg_element_kernel (fused volume + interface + surface + Jacobian + source)",
+            "kernel_avg_ms": r["kernel_avg_ms"],
+            "algorithmic_bytes": alg_bytes,
+        },
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    if "fp64" in res and "fp32" in res:
+        v64 = ndofs / (res["fp64"]["ms_per_step"] * 1e-3)
+        out["fp64_plain"] = {"value": v64, "ms_per_step": res["fp64"]["ms_per_step"],
+                             "kernel_avg_ms": res["fp64"]["kernel_avg_ms"],
+                             "tile": list(res["fp64"]["tile"])}
+        out["fp32_over_fp64"] = value / v64
+    if l2 is not None:
+        out["equal_l2"] = l2
+    if cpu is not None:
+        out["gpu_fp32_over_cpu"] = value / cpu["value"]
+        out["cpu_time_per_dof_per_rhs_s"] = cpu["time_per_dof_per_rhs_s"]
+    ok = all(v["finite"] for v in res.values())
+    out["outputs_finite"] = ok
+    for s in semis.values():
+        s.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

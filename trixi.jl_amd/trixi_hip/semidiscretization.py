@@ -45,7 +45,7 @@ class SemidiscretizationHyperbolic:
         self.tile = (int(elements_per_block), int(threads_per_block))
         self.inverse_jacobian, self.node_coordinates = element_geometry(mesh, solver.basis)
         self._ctx = None
-        self._keep = []
+        self._lib = _lib.load()
         self.performance_counter = {"ncalls": 0, "time": 0.0}
 
     # -- sizes ---------------------------------------------------------------
@@ -122,7 +122,6 @@ class SemidiscretizationHyperbolic:
                                       ctypes.byref(out))
             _lib.check(rc, None, "trixi_hip_create")
             self._ctx = out
-            self._lib = lib
             ic = self.initial_condition
             params = np.array(list(ic.params) + [0.0] * 4, dtype=np.float64)[:4]
             modes = np.ascontiguousarray(ic.modes if ic.modes is not None else np.zeros((4, 8, 4)),
