@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--no-l2", action="store_true")
     p.add_argument("--only", choices=["fp32", "fp64"], default=None,
                    help="time one precision only (profiling runs)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="also run one plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -207,6 +209,16 @@ def main():
         res[prec] = time_rhs(th, torch, semi, args.steps, args.warmup, u0)
         res[prec]["tile"] = semi.tile_used()
         print(f"[bench] {prec}: {res[prec]}", file=sys.stderr, flush=True)
+
+    if args.copy_calibration:
+        src = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written",
+              file=sys.stderr, flush=True)
+        del src, dst
 
     ndofs = semis["fp32"].ndofs()
     nv = semis["fp32"].nvariables()

@@ -143,7 +143,7 @@ __global__ void rk2n_update_kernel(const TD *__restrict__ du, double *__restrict
   for (; i < count; i += stride) {
     double ut = first_stage ? 0.0 : u_tmp[i];
     ut = double(du[i]) - ut * a_stage;
-    double un = __builtin_fma(ut, b_dt, u[i]);
+    double un = tfma(ut, b_dt, u[i]);
     u_tmp[i] = ut;
     u[i] = un;
     if (u32) u32[i] = float(un);

@@ -59,6 +59,11 @@ template <> struct Prec<double> {
   static constexpr double sqrt_f2_threshold = 0.01;
 };
 
+// fused multiply-add in the operand precision (__builtin_fma is the double
+// builtin and would promote float operands)
+__device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
@@ -83,7 +88,7 @@ __device__ __forceinline__ T ln_mean_l(T x, T y, T lx, T ly) {
   T r = Prec<T>::rcp(near ? s : (ly - lx));
   T t = d * r;
   T z = t * t;
-  T q = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(T(-428.0 / 28350.0), z, T(-44.0 / 1890.0)),
+  T q = tfma(tfma(tfma(tfma(T(-428.0 / 28350.0), z, T(-44.0 / 1890.0)),
                                                   z, T(-4.0 / 90.0)), z, T(-1.0 / 6.0)), z, T(0.5));
   return near ? s * q : t;
 }
@@ -95,7 +100,7 @@ __device__ __forceinline__ T inv_ln_mean_l(T x, T y, T lx, T ly) {
   T r = Prec<T>::rcp(near ? s : d);
   T t = d * r;
   T z = t * t;
-  T p = __builtin_fma(__builtin_fma(__builtin_fma(T(2.0 / 7.0), z, T(2.0 / 5.0)), z, T(2.0 / 3.0)), z, T(2.0));
+  T p = tfma(tfma(tfma(T(2.0 / 7.0), z, T(2.0 / 5.0)), z, T(2.0 / 3.0)), z, T(2.0));
   return near ? p * r : (ly - lx) * r;
 }
 
@@ -117,7 +122,7 @@ __device__ __forceinline__ void nodeR_from_cons(const T *u, T gamma_m1, NodeR<T,
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     v[d] = u[1 + d] * irho;
-    ke2 = __builtin_fma(u[1 + d], v[d], ke2);
+    ke2 = tfma(u[1 + d], v[d], ke2);
   }
   T p = gamma_m1 * (u[D + 1] - T(0.5) * ke2);
   w.rho = rho;
@@ -140,17 +145,17 @@ __device__ __forceinline__ void flux_ranocha_r(const NodeR<T, D> &l, const NodeR
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     vavg[d] = l.h[d] + r.h[d];
-    hh = __builtin_fma(l.h[d], r.h[d], hh);  // = velocity_square_avg / 2
+    hh = tfma(l.h[d], r.h[d], hh);  // = velocity_square_avg / 2
   }
   T p_avg = l.hp + r.hp;
-  T cross = __builtin_fma(l.hp, r.h[O], r.hp * l.h[O]);  // = 0.5 (p_ll v_rr + p_rr v_ll) / 2
+  T cross = tfma(l.hp, r.h[O], r.hp * l.h[O]);  // = 0.5 (p_ll v_rr + p_rr v_ll) / 2
   T f1 = rho_mean * vavg[O];
   f[0] = f1;
 #pragma unroll
   for (int d = 0; d < D; ++d) f[1 + d] = f1 * vavg[d];
   f[1 + O] += p_avg;
-  T inner = __builtin_fma(inv_rho_p_mean, inv_gamma_m1, hh + hh);
-  f[D + 1] = __builtin_fma(f1, inner, cross + cross);
+  T inner = tfma(inv_rho_p_mean, inv_gamma_m1, hh + hh);
+  f[D + 1] = tfma(f1, inner, cross + cross);
 }
 
 // Physical flux of a conservative state in orientation o (runtime), plus the
@@ -159,6 +164,7 @@ template <typename T, int D> struct ConsP {
   T u[D + 2];
   T v[D];
   T p;
+  T irho;
 };
 
 template <typename T, int D>
@@ -170,9 +176,10 @@ __device__ __forceinline__ void consp_from_cons(const T *u, T gamma_m1, ConsP<T,
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     c.v[d] = u[1 + d] * irho;
-    ke2 = __builtin_fma(u[1 + d], c.v[d], ke2);
+    ke2 = tfma(u[1 + d], c.v[d], ke2);
   }
   c.p = gamma_m1 * (u[D + 1] - T(0.5) * ke2);
+  c.irho = irho;
 }
 
 template <typename T, int D, int O>
@@ -201,8 +208,8 @@ __device__ __forceinline__ void flux_llf_c(const ConsP<T, D> &l, const ConsP<T, 
   T fl[D + 2], fr[D + 2];
   phys_flux<T, D, O>(l, fl);
   phys_flux<T, D, O>(r, fr);
-  T c_ll = Prec<T>::sqrt_(gamma * l.p * Prec<T>::rcp(l.u[0]));
-  T c_rr = Prec<T>::sqrt_(gamma * r.p * Prec<T>::rcp(r.u[0]));
+  T c_ll = Prec<T>::sqrt_(gamma * l.p * l.irho);
+  T c_rr = Prec<T>::sqrt_(gamma * r.p * r.irho);
   T al = tabs(l.v[O]), ar = tabs(r.v[O]);
   T lambda = naive ? tmax(al, ar) + tmax(c_ll, c_rr) : tmax(al + c_ll, ar + c_rr);
   T hl = T(-0.5) * lambda;

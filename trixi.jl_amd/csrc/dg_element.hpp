@@ -16,14 +16,22 @@
 //   apply_jacobian!              dg_3d.jl:1396-1415
 //   calc_sources!                dg_3d.jl:1417-1438
 //
-// Work decomposition: after the tile is staged in LDS, each thread owns one
-// node LINE (fixed direction o, the other indices fixed). For that line it
-// evaluates every two-point volume flux between line nodes exactly once
-// (n(n-1)/2 fluxes, the symmetric update of flux_differencing_kernel!), plus
-// the two surface fluxes at the line ends, keeps the line's du in registers and
-// adds it to the LDS accumulator. Directions are processed one after another
-// with a barrier between them, so every du entry has exactly one writer per
-// phase: no atomics, bitwise reproducible.
+// Work decomposition. The workgroup has D groups of WPD wavefronts, one group
+// per coordinate direction. Phase A stages the tile (16-byte loads) and
+// computes per-node primitive data into LDS. Phase B: every lane of group O
+// owns one node LINE in direction O; it evaluates each two-point volume flux
+// of the line exactly once (n(n-1)/2 fluxes, the symmetric update of
+// flux_differencing_kernel!) plus the two surface fluxes at the line ends, and
+// writes the line's du into the LDS buffer of direction O. Phase C sums the D
+// buffers per node, applies the Jacobian and sources, and stores with 16-byte
+// writes. Every LDS word has one writer per phase: no atomics, bitwise
+// reproducible results.
+//
+// LDS layout: structure-of-arrays per variable, node index q = el*NN + node.
+// For the 3D polydeg-3 tile of 4 elements the node index is XOR-swizzled
+// (i, j, k) -> (i^el, j^el, k^el) and each direction group uses its own lane
+// order, which makes every line access of every direction, and the sequential
+// accesses of phases A and C, free of bank conflicts for 32 and for 64 banks.
 #pragma once
 #include "dg_common.hpp"
 
@@ -82,215 +90,273 @@ __device__ __forceinline__ void tile_copy(T *__restrict__ dst, const T *__restri
   }
 }
 
-// One direction O of the line phase.
-template <typename T, int D, int N, int VOL, int EPB, int NT, int O>
-__device__ __forceinline__ void line_phase(const ElemArgs<T> &a, const DGConst<T, N> &k, int e0,
-                                           int ne, const T *__restrict__ s_u,
-                                           const T *__restrict__ s_w, T *__restrict__ s_du,
-                                           const double *__restrict__ s_alpha, int SP) {
-  constexpr int n = N + 1, NN = ipow_c(n, D), L = ipow_c(n, D - 1), V = D + 2;
+// LDS index map and lane -> line assignment (see the header comment).
+template <int D, int N, int EPB> struct Layout {
+  static constexpr int n = N + 1, NN = ipow_c(n, D), L = ipow_c(n, D - 1);
+  static constexpr bool SWZ = (D == 3 && N == 3 && EPB == 4);
+  static constexpr int SIZE = SWZ ? EPB * NN : EPB * NN + (EPB * NN) / 32 + 1;
+  static __device__ __forceinline__ int phys(int q) {
+    if constexpr (SWZ) {
+      const int el = q >> 6, i = q & 3, j = (q >> 2) & 3, k = (q >> 4) & 3;
+      return (el << 6) | (i ^ el) | ((j ^ el) << 2) | ((k ^ el) << 4);
+    } else {
+      return q + (q >> 5);
+    }
+  }
+  // line = r0 + n*r1 with (r0, r1) the two free indices in increasing dimension order
+  template <int O> static __device__ __forceinline__ void lane_line(int ll, int &el, int &line) {
+    if constexpr (SWZ) {
+      if constexpr (O == 0) {
+        el = ll & 3;
+        line = ((ll >> 2) & 3) + 4 * (ll >> 4);  // (j, k)
+      } else if constexpr (O == 1) {
+        el = ll & 3;
+        line = ((ll >> 2) & 3) + 4 * (ll >> 4);  // (i, k)
+      } else {
+        el = ll >> 4;
+        line = ll & 15;  // (i, j)
+      }
+    } else {
+      el = ll / L;
+      line = ll - el * L;
+    }
+  }
+};
+
+template <int VOL> struct VolTraits {
+  static constexpr bool RAN = (VOL == VOL_FD_RANOCHA || VOL == VOL_HG_RANOCHA);
+  static constexpr bool CEN = (VOL == VOL_FD_CENTRAL || VOL == VOL_HG_CENTRAL);
+  static constexpr bool HG = (VOL == VOL_HG_RANOCHA || VOL == VOL_HG_CENTRAL);
+  static constexpr bool FV = HG || VOL == VOL_PURE_FV;
+  // conservative state needed in LDS (weak form, central flux, FV subcells)
+  static constexpr bool CONS = !RAN || HG;
+};
+
+// One line of direction O: volume terms, the two surface terms, LDS write.
+template <typename T, int D, int N, int VOL, int EPB, int O>
+__device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T, N> &k, int e0, int ne,
+                                          int ll, const T *__restrict__ s_w, const T *__restrict__ s_c,
+                                          T *__restrict__ s_d, const double *__restrict__ s_alpha) {
+  using LY = Layout<D, N, EPB>;
+  using VT = VolTraits<VOL>;
+  constexpr int n = N + 1, NN = LY::NN, V = D + 2, SZ = LY::SIZE;
   constexpr int st = ipow_c(n, O);
-  constexpr bool RAN = (VOL == VOL_FD_RANOCHA || VOL == VOL_HG_RANOCHA);
-  constexpr bool CEN = (VOL == VOL_FD_CENTRAL || VOL == VOL_HG_CENTRAL);
-  constexpr bool HG = (VOL == VOL_HG_RANOCHA || VOL == VOL_HG_CENTRAL);
+  int el, line;
+  LY::template lane_line<O>(ll, el, line);
+  if (el >= ne) return;
+  const int base = (line % st) + (line / st) * st * n;
+  const int q0 = el * NN + base;
+  const int e = e0 + el;
   const T gamma = a.gamma;
   const T gm1 = gamma - T(1);
   const T igm1 = T(1) / gm1;
 
-  for (int l = threadIdx.x; l < EPB * L; l += NT) {
-    const int el = l / L;
-    if (el >= ne) continue;
-    const int line = l - el * L;
-    const int base = (line % st) + (line / st) * st * n;
-    const int q0 = el * NN + base;  // tile-local node index of line position 0
-    const int e = e0 + el;
+  int pq[n];
+#pragma unroll
+  for (int i = 0; i < n; ++i) pq[i] = LY::phys(q0 + i * st);
 
-    T acc[n][V];
+  // issue the face-state loads first: they overlap the volume arithmetic
+  const int nbm = a.nbr[(2 * O) + 2 * D * e];
+  const int nbp = a.nbr[(2 * O + 1) + 2 * D * e];
+  const T *un_m = a.u + ((size_t)nbm * NN + base + (n - 1) * st) * V;
+  const T *un_p = a.u + ((size_t)nbp * NN + base) * V;
+  T ul_m[V], ur_p[V], own0[V], own1[V];
 #pragma unroll
-    for (int i = 0; i < n; ++i)
+  for (int v = 0; v < V; ++v) {
+    ul_m[v] = un_m[v];
+    ur_p[v] = un_p[v];
+  }
+  if constexpr (VT::CONS) {
 #pragma unroll
-      for (int v = 0; v < V; ++v) acc[i][v] = T(0);
+    for (int v = 0; v < V; ++v) {
+      own0[v] = s_c[v * SZ + pq[0]];
+      own1[v] = s_c[v * SZ + pq[n - 1]];
+    }
+  } else {
+    const T *uo = a.u + ((size_t)e * NN + base) * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      own0[v] = uo[v];
+      own1[v] = uo[(size_t)(n - 1) * st * V + v];
+    }
+  }
 
-    // ---------------- volume integral ----------------
-    bool do_fd = (VOL != VOL_PURE_FV);
-    double alpha = 0.0;
-    bool blend = false;
-    if constexpr (HG) {
-      alpha = s_alpha[el];
-      const double atol = sizeof(T) == 8 ? 1.8189894035458565e-12 : 1.1920928955078125e-5;
-      blend = !(alpha <= atol && alpha >= -atol);
-    }
-    if constexpr (VOL == VOL_WEAK) {
+  T acc[n][V];
 #pragma unroll
-      for (int b = 0; b < n; ++b) {
-        ConsP<T, D> cb;
-        consp_from_cons<T, D>(&s_u[(q0 + b * st) * V], gm1, cb);
-        T fb[V];
-        phys_flux<T, D, O>(cb, fb);
+  for (int i = 0; i < n; ++i)
 #pragma unroll
-        for (int i = 0; i < n; ++i) {
-          const T dh = k.Dhat[i * n + b];
+    for (int v = 0; v < V; ++v) acc[i][v] = T(0);
+
+  double alpha = 0.0;
+  bool blend = false;
+  if constexpr (VT::HG) {
+    alpha = s_alpha[el];
+    const double atol = sizeof(T) == 8 ? 1.8189894035458565e-12 : 1.1920928955078125e-5;
+    blend = !(alpha <= atol && alpha >= -atol);
+  }
+
+  // ---------------- volume integral ----------------
+  if constexpr (VOL == VOL_WEAK) {
 #pragma unroll
-          for (int v = 0; v < V; ++v) acc[i][v] = __builtin_fma(dh, fb[v], acc[i][v]);
-        }
-      }
-    }
-    if constexpr (RAN) {
-      NodeR<T, D> w[n];
+    for (int b = 0; b < n; ++b) {
+      T cb_u[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) cb_u[v] = s_c[v * SZ + pq[b]];
+      ConsP<T, D> cb;
+      consp_from_cons<T, D>(cb_u, gm1, cb);
+      T fb[V];
+      phys_flux<T, D, O>(cb, fb);
 #pragma unroll
       for (int i = 0; i < n; ++i) {
-        const int q = q0 + i * st;
-        w[i].rho = s_w[0 * SP + q];
-        w[i].lrho = s_w[1 * SP + q];
-        w[i].beta = s_w[2 * SP + q];
-        w[i].lbeta = s_w[3 * SP + q];
+        const T dh = k.Dhat[i * n + b];
 #pragma unroll
-        for (int d = 0; d < D; ++d) w[i].h[d] = s_w[(4 + d) * SP + q];
-        w[i].hp = s_w[(4 + D) * SP + q];
-      }
-#pragma unroll
-      for (int i = 0; i < n; ++i)
-#pragma unroll
-        for (int j = i + 1; j < n; ++j) {
-          T f[V];
-          flux_ranocha_r<T, D, O>(w[i], w[j], igm1, f);
-          const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
-#pragma unroll
-          for (int v = 0; v < V; ++v) {
-            acc[i][v] = __builtin_fma(di, f[v], acc[i][v]);
-            acc[j][v] = __builtin_fma(dj, f[v], acc[j][v]);
-          }
-        }
-    }
-    if constexpr (CEN) {
-      T fph[n][V];
-#pragma unroll
-      for (int i = 0; i < n; ++i) {
-        ConsP<T, D> ci;
-        consp_from_cons<T, D>(&s_u[(q0 + i * st) * V], gm1, ci);
-        phys_flux<T, D, O>(ci, fph[i]);
-      }
-#pragma unroll
-      for (int i = 0; i < n; ++i)
-#pragma unroll
-        for (int j = i + 1; j < n; ++j) {
-          const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
-#pragma unroll
-          for (int v = 0; v < V; ++v) {
-            T f = T(0.5) * (fph[i][v] + fph[j][v]);
-            acc[i][v] = __builtin_fma(di, f, acc[i][v]);
-            acc[j][v] = __builtin_fma(dj, f, acc[j][v]);
-          }
-        }
-    }
-    (void)do_fd;
-    if constexpr (HG || VOL == VOL_PURE_FV) {
-      T afv = T(1);
-      bool run_fv = true;
-      if constexpr (HG) {
-        run_fv = blend;
-        if (blend) {
-          const T s = T(1.0 - alpha);
-#pragma unroll
-          for (int i = 0; i < n; ++i)
-#pragma unroll
-            for (int v = 0; v < V; ++v) acc[i][v] *= s;
-        }
-        afv = T(alpha);
-      }
-      if (run_fv) {
-        // first-order FV on the LGL subcells along this line; the subcell
-        // fluxes at the element faces are zero (create_f_threaded, dg_3d.jl:11-54)
-        T fprev[V];
-#pragma unroll
-        for (int v = 0; v < V; ++v) fprev[v] = T(0);
-#pragma unroll
-        for (int i = 0; i < n; ++i) {
-          T fnext[V];
-          if (i < n - 1) {
-            num_flux_c<T, D, O>(a.fv_flux, &s_u[(q0 + i * st) * V], &s_u[(q0 + (i + 1) * st) * V],
-                                gamma, fnext);
-          } else {
-#pragma unroll
-            for (int v = 0; v < V; ++v) fnext[v] = T(0);
-          }
-          const T c = afv * k.inv_w[i];
-#pragma unroll
-          for (int v = 0; v < V; ++v) {
-            acc[i][v] = __builtin_fma(c, fnext[v] - fprev[v], acc[i][v]);
-            fprev[v] = fnext[v];
-          }
-        }
-      }
-    }
-
-    // ---------------- interface fluxes + surface integral ----------------
-    const T factor = k.inv_w[0];  // inverse_weights[1] of the reference
-    {
-      // -O face: this element is the right state of the interface
-      const int nb = a.nbr[(2 * O) + 2 * D * e];
-      const T *un = a.u + ((size_t)nb * NN + base + (n - 1) * st) * V;
-      T ul[V], ur[V], F[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        ul[v] = un[v];
-        ur[v] = s_u[q0 * V + v];
-      }
-      num_flux_c<T, D, O>(a.surface_flux, ul, ur, gamma, F);
-#pragma unroll
-      for (int v = 0; v < V; ++v) acc[0][v] = __builtin_fma(-F[v], factor, acc[0][v]);
-    }
-    {
-      // +O face: this element is the left state
-      const int nb = a.nbr[(2 * O + 1) + 2 * D * e];
-      const T *un = a.u + ((size_t)nb * NN + base) * V;
-      T ul[V], ur[V], F[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        ul[v] = s_u[(q0 + (n - 1) * st) * V + v];
-        ur[v] = un[v];
-      }
-      num_flux_c<T, D, O>(a.surface_flux, ul, ur, gamma, F);
-#pragma unroll
-      for (int v = 0; v < V; ++v) acc[n - 1][v] = __builtin_fma(F[v], factor, acc[n - 1][v]);
-    }
-
-    // ---------------- accumulate into LDS ----------------
-#pragma unroll
-    for (int i = 0; i < n; ++i) {
-      T *dst = &s_du[(q0 + i * st) * V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        if constexpr (O == 0) dst[v] = acc[i][v];
-        else dst[v] += acc[i][v];
+        for (int v = 0; v < V; ++v) acc[i][v] = tfma(dh, fb[v], acc[i][v]);
       }
     }
   }
+  if constexpr (VT::RAN) {
+    NodeR<T, D> w[n];
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      w[i].rho = s_w[0 * SZ + pq[i]];
+      w[i].lrho = s_w[1 * SZ + pq[i]];
+      w[i].beta = s_w[2 * SZ + pq[i]];
+      w[i].lbeta = s_w[3 * SZ + pq[i]];
+#pragma unroll
+      for (int d = 0; d < D; ++d) w[i].h[d] = s_w[(4 + d) * SZ + pq[i]];
+      w[i].hp = s_w[(4 + D) * SZ + pq[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+#pragma unroll
+      for (int j = i + 1; j < n; ++j) {
+        T f[V];
+        flux_ranocha_r<T, D, O>(w[i], w[j], igm1, f);
+        const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          acc[i][v] = tfma(di, f[v], acc[i][v]);
+          acc[j][v] = tfma(dj, f[v], acc[j][v]);
+        }
+      }
+  }
+  if constexpr (VT::CEN) {
+    T fph[n][V];
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      T ci_u[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) ci_u[v] = s_c[v * SZ + pq[i]];
+      ConsP<T, D> ci;
+      consp_from_cons<T, D>(ci_u, gm1, ci);
+      phys_flux<T, D, O>(ci, fph[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+#pragma unroll
+      for (int j = i + 1; j < n; ++j) {
+        const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          T f = T(0.5) * (fph[i][v] + fph[j][v]);
+          acc[i][v] = tfma(di, f, acc[i][v]);
+          acc[j][v] = tfma(dj, f, acc[j][v]);
+        }
+      }
+  }
+  if constexpr (VT::FV) {
+    T afv = T(1);
+    bool run_fv = true;
+    if constexpr (VT::HG) {
+      run_fv = blend;
+      if (blend) {
+        const T s = T(1.0 - alpha);
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+#pragma unroll
+          for (int v = 0; v < V; ++v) acc[i][v] *= s;
+      }
+      afv = T(alpha);
+    }
+    if (run_fv) {
+      // first-order FV on the LGL subcells of this line; subcell fluxes at the
+      // element faces are zero (create_f_threaded, dg_3d.jl:11-54)
+      T fprev[V], cur[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        fprev[v] = T(0);
+        cur[v] = s_c[v * SZ + pq[0]];
+      }
+#pragma unroll
+      for (int i = 0; i < n; ++i) {
+        T fnext[V];
+        if (i < n - 1) {
+          T nxt[V];
+#pragma unroll
+          for (int v = 0; v < V; ++v) nxt[v] = s_c[v * SZ + pq[i + 1]];
+          num_flux_c<T, D, O>(a.fv_flux, cur, nxt, gamma, fnext);
+#pragma unroll
+          for (int v = 0; v < V; ++v) cur[v] = nxt[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v) fnext[v] = T(0);
+        }
+        const T c = afv * k.inv_w[i];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          acc[i][v] = tfma(c, fnext[v] - fprev[v], acc[i][v]);
+          fprev[v] = fnext[v];
+        }
+      }
+    }
+  }
+
+  // ---------------- interface fluxes + surface integral ----------------
+  const T factor = k.inv_w[0];  // inverse_weights[1] of the reference
+  {
+    T F[V];
+    num_flux_c<T, D, O>(a.surface_flux, ul_m, own0, gamma, F);  // -O face: element is the right state
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
+    num_flux_c<T, D, O>(a.surface_flux, own1, ur_p, gamma, F);  // +O face: element is the left state
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
+  }
+
+#pragma unroll
+  for (int i = 0; i < n; ++i)
+#pragma unroll
+    for (int v = 0; v < V; ++v) s_d[(O * V + v) * SZ + pq[i]] = acc[i][v];
 }
 
-template <typename T, int D, int N, int VOL, int EPB, int NT>
-__global__ __launch_bounds__(NT) void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
-  constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
-  constexpr bool RAN = (VOL == VOL_FD_RANOCHA || VOL == VOL_HG_RANOCHA);
-  constexpr bool HG = (VOL == VOL_HG_RANOCHA || VOL == VOL_HG_CENTRAL);
-  constexpr int NW = RAN ? nodeR_count(D) : 1;
-  // pad the SoA stride so successive variables start in different LDS banks
-  constexpr int SP = EPB * NN + 1;
-  __shared__ __attribute__((aligned(16))) T s_u[EPB * NN * V];
-  __shared__ __attribute__((aligned(16))) T s_du[EPB * NN * V];
-  __shared__ T s_w[NW * SP];
-  __shared__ double s_alpha[HG ? EPB : 1];
+template <typename T, int D, int N, int VOL, int EPB, int WPD>
+__global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
+  using LY = Layout<D, N, EPB>;
+  using VT = VolTraits<VOL>;
+  constexpr int NT = D * WPD * 64;
+  constexpr int n = N + 1, NN = LY::NN, V = D + 2, SZ = LY::SIZE, L = LY::L;
+  constexpr int NW = VT::RAN ? nodeR_count(D) : 0;
+  constexpr int NC = VT::CONS ? V : 0;
+  constexpr int SIZE_A = (NW + NC) * SZ;  // primitive / conservative data, later output staging
+  constexpr int SIZE_B = D * V * SZ;      // du per direction, earlier input staging
+  static_assert(SIZE_A >= EPB * NN * V && SIZE_B >= EPB * NN * V, "staging does not fit");
+  static_assert(EPB * L <= WPD * 64, "one line per lane");
+  __shared__ __attribute__((aligned(16))) T smem[SIZE_A + SIZE_B];
+  __shared__ double s_alpha[VT::HG ? EPB : 1];
+  T *s_w = smem;               // [NW][SZ]
+  T *s_c = smem + NW * SZ;     // [NC][SZ]
+  T *s_d = smem + SIZE_A;      // [D][V][SZ]
 
   const int tid = threadIdx.x;
   const int blk = a.xcd_remap ? xcd_remap_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int e0 = blk * EPB;
   if (e0 >= a.E) return;  // uniform per block
   const int ne = min(EPB, a.E - e0);
-  constexpr bool vec_ok_c = ((NN * V * (int)sizeof(T)) % 16) == 0;
+  constexpr bool vec_ok = ((NN * V * (int)sizeof(T)) % 16) == 0;
 
-  // phase A: stage the tile
-  tile_copy<T, NT>(s_u, a.u + (size_t)e0 * NN * V, ne * NN * V, vec_ok_c);
-  if constexpr (HG) {
+  // ---- phase A: stage the tile, per-node data into LDS ----
+  tile_copy<T, NT>(s_d, a.u + (size_t)e0 * NN * V, ne * NN * V, vec_ok);
+  if constexpr (VT::HG) {
     if (tid < ne) {
       const int e = e0 + tid;
       double al = a.alpha_raw[e];
@@ -310,44 +376,61 @@ __global__ __launch_bounds__(NT) void dg_element_kernel(ElemArgs<T> a, DGConst<T
     }
   }
   __syncthreads();
-
-  // phase A2: per-node primitive data for flux_ranocha
-  if constexpr (RAN) {
+  {
     const T gm1 = a.gamma - T(1);
     for (int q = tid; q < ne * NN; q += NT) {
-      NodeR<T, D> w;
-      nodeR_from_cons<T, D>(&s_u[q * V], gm1, w);
-      s_w[0 * SP + q] = w.rho;
-      s_w[1 * SP + q] = w.lrho;
-      s_w[2 * SP + q] = w.beta;
-      s_w[3 * SP + q] = w.lbeta;
+      const int p = LY::phys(q);
+      T uq[V];
 #pragma unroll
-      for (int d = 0; d < D; ++d) s_w[(4 + d) * SP + q] = w.h[d];
-      s_w[(4 + D) * SP + q] = w.hp;
+      for (int v = 0; v < V; ++v) uq[v] = s_d[q * V + v];
+      if constexpr (VT::CONS) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) s_c[v * SZ + p] = uq[v];
+      }
+      if constexpr (VT::RAN) {
+        NodeR<T, D> w;
+        nodeR_from_cons<T, D>(uq, gm1, w);
+        s_w[0 * SZ + p] = w.rho;
+        s_w[1 * SZ + p] = w.lrho;
+        s_w[2 * SZ + p] = w.beta;
+        s_w[3 * SZ + p] = w.lbeta;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s_w[(4 + d) * SZ + p] = w.h[d];
+        s_w[(4 + D) * SZ + p] = w.hp;
+      }
     }
-    __syncthreads();
   }
+  __syncthreads();
 
-  // phase B: lines, one direction at a time
-  line_phase<T, D, N, VOL, EPB, NT, 0>(a, k, e0, ne, s_u, s_w, s_du, s_alpha, SP);
-  __syncthreads();
-  line_phase<T, D, N, VOL, EPB, NT, 1>(a, k, e0, ne, s_u, s_w, s_du, s_alpha, SP);
-  __syncthreads();
-  if constexpr (D == 3) {
-    line_phase<T, D, N, VOL, EPB, NT, 2>(a, k, e0, ne, s_u, s_w, s_du, s_alpha, SP);
-    __syncthreads();
+  // ---- phase B: one direction per wave group, one line per lane ----
+  {
+    const int grp = tid / (WPD * 64);  // wave-uniform
+    const int ll = tid - grp * (WPD * 64);
+    if (ll < EPB * L) {
+      if (grp == 0) line_work<T, D, N, VOL, EPB, 0>(a, k, e0, ne, ll, s_w, s_c, s_d, s_alpha);
+      else if (grp == 1) line_work<T, D, N, VOL, EPB, 1>(a, k, e0, ne, ll, s_w, s_c, s_d, s_alpha);
+      else if constexpr (D == 3) line_work<T, D, N, VOL, EPB, 2>(a, k, e0, ne, ll, s_w, s_c, s_d, s_alpha);
+    }
   }
+  __syncthreads();
 
-  // phase C: Jacobian and sources, node-wise in LDS
+  // ---- phase C: sum directions, Jacobian, sources; stage the output ----
   const double tt = a.tdev ? (a.tdev[0] + a.t * a.tdev[1]) : a.t;
+  T *s_o = smem;  // [EPB*NN][V] staging (region A is free now)
   for (int q = tid; q < ne * NN; q += NT) {
     const int el = q / NN, node = q - el * NN;
     const int e = e0 + el;
+    const int p = LY::phys(q);
     const double ij = a.inv_jac[e];
     const T fac = T(-ij);
-    T *d = &s_du[q * V];
+    T d[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) d[v] *= fac;
+    for (int v = 0; v < V; ++v) {
+      T s = s_d[(0 * V + v) * SZ + p];
+#pragma unroll
+      for (int o = 1; o < D; ++o) s += s_d[(o * V + v) * SZ + p];
+      d[v] = s * fac;
+    }
     if (a.source) {
       const double jac = 1.0 / ij;
       T x[D];
@@ -363,9 +446,11 @@ __global__ __launch_bounds__(NT) void dg_element_kernel(ElemArgs<T> a, DGConst<T
 #pragma unroll
       for (int v = 0; v < V; ++v) d[v] += s[v];
     }
+#pragma unroll
+    for (int v = 0; v < V; ++v) s_o[q * V + v] = d[v];
   }
   __syncthreads();
-  tile_copy<T, NT>(a.du + (size_t)e0 * NN * V, s_du, ne * NN * V, vec_ok_c);
+  tile_copy<T, NT>(a.du + (size_t)e0 * NN * V, s_o, ne * NN * V, vec_ok);
 }
 
 }  // namespace trixi_hip

@@ -20,7 +20,8 @@ using namespace trixi_hip;
 namespace {
 
 // ---------------------------------------------------------------------------
-// Element-kernel registry: one entry per instantiated (T, D, N, VOL, EPB, NT).
+// Element-kernel registry: one entry per instantiated (T, D, N, VOL, EPB, WPD).
+// The workgroup has D * WPD wavefronts (one group of WPD waves per direction).
 // ---------------------------------------------------------------------------
 struct ElemKey {
   int fp32, D, N, vol, epb, nt;
@@ -37,60 +38,61 @@ std::map<ElemKey, ElemLaunch> &elem_registry() {
   return r;
 }
 
-template <typename T, int D, int N, int VOL, int EPB, int NT>
+template <typename T, int D, int N, int VOL, int EPB, int WPD>
 hipError_t launch_elem(const void *args, const void *kconst, int nblocks, hipStream_t s) {
   const ElemArgs<T> &a = *static_cast<const ElemArgs<T> *>(args);
   const DGConst<T, N> &k = *static_cast<const DGConst<T, N> *>(kconst);
-  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, NT>), dim3(nblocks), dim3(NT), 0, s, a, k);
+  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, WPD>), dim3(nblocks), dim3(D * WPD * 64), 0, s, a,
+                     k);
   return hipGetLastError();
 }
 
-template <typename T, int D, int N, int VOL, int EPB, int NT> struct Reg {
-  Reg() { elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, NT}] = &launch_elem<T, D, N, VOL, EPB, NT>; }
+template <typename T, int D, int N, int VOL, int EPB, int WPD> struct Reg {
+  Reg() {
+    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, D * WPD * 64}] = &launch_elem<T, D, N, VOL, EPB, WPD>;
+  }
 };
 
-// Tile candidates: lines per direction L = n^(D-1); EPB = elements per
-// workgroup, NT = threads. The first entry of each list is the default; the
-// autotuner (trixi_hip_autotune via the Python host) may pick another.
-#define TRIXI_REG(T, D, N, VOL, EPB, NT) static Reg<T, D, N, VOL, EPB, NT> reg_##T##_##D##_##N##_##VOL##_##EPB##_##NT;
+// Tile candidates: EPB = elements per workgroup, WPD = wavefronts per
+// direction (lines per direction EPB * n^(D-1) <= 64 * WPD). The defaults
+// (kDefaultTiles) put one wave per direction; the autotuner may pick another.
+#define TRIXI_REG(T, D, N, VOL, EPB, WPD) \
+  static Reg<T, D, N, VOL, EPB, WPD> reg_##T##_##D##_##N##_##VOL##_##EPB##_##WPD;
 
-#define TRIXI_REG_ALLVOL(T, D, N, EPB, NT)           \
-  TRIXI_REG(T, D, N, VOL_WEAK, EPB, NT)              \
-  TRIXI_REG(T, D, N, VOL_FD_RANOCHA, EPB, NT)        \
-  TRIXI_REG(T, D, N, VOL_FD_CENTRAL, EPB, NT)        \
-  TRIXI_REG(T, D, N, VOL_HG_RANOCHA, EPB, NT)        \
-  TRIXI_REG(T, D, N, VOL_HG_CENTRAL, EPB, NT)        \
-  TRIXI_REG(T, D, N, VOL_PURE_FV, EPB, NT)
+#define TRIXI_REG_ALLVOL(T, D, N, EPB, WPD)    \
+  TRIXI_REG(T, D, N, VOL_WEAK, EPB, WPD)       \
+  TRIXI_REG(T, D, N, VOL_FD_RANOCHA, EPB, WPD) \
+  TRIXI_REG(T, D, N, VOL_FD_CENTRAL, EPB, WPD) \
+  TRIXI_REG(T, D, N, VOL_HG_RANOCHA, EPB, WPD) \
+  TRIXI_REG(T, D, N, VOL_HG_CENTRAL, EPB, WPD) \
+  TRIXI_REG(T, D, N, VOL_PURE_FV, EPB, WPD)
 
-#define TRIXI_REG_BOTH(D, N, EPB, NT) \
-  TRIXI_REG_ALLVOL(float, D, N, EPB, NT) \
-  TRIXI_REG_ALLVOL(double, D, N, EPB, NT)
+#define TRIXI_REG_BOTH(D, N, EPB, WPD)    \
+  TRIXI_REG_ALLVOL(float, D, N, EPB, WPD) \
+  TRIXI_REG_ALLVOL(double, D, N, EPB, WPD)
 
-// defaults (one wavefront of lines per direction where the line count allows)
-TRIXI_REG_BOTH(2, 2, 21, 64)
-TRIXI_REG_BOTH(2, 3, 16, 64)
-TRIXI_REG_BOTH(2, 4, 12, 64)
-TRIXI_REG_BOTH(3, 2, 7, 64)
-TRIXI_REG_BOTH(3, 3, 4, 64)
-TRIXI_REG_BOTH(3, 4, 5, 128)
+// defaults
+TRIXI_REG_BOTH(2, 2, 21, 1)
+TRIXI_REG_BOTH(2, 3, 16, 1)
+TRIXI_REG_BOTH(2, 4, 12, 1)
+TRIXI_REG_BOTH(3, 2, 7, 1)
+TRIXI_REG_BOTH(3, 3, 4, 1)
+TRIXI_REG_BOTH(3, 4, 2, 1)
 
 // autotuner candidates for the headline volume integrals
-TRIXI_REG(float, 3, 3, VOL_FD_RANOCHA, 8, 128)
-TRIXI_REG(float, 3, 3, VOL_FD_RANOCHA, 16, 256)
-TRIXI_REG(float, 3, 3, VOL_FD_RANOCHA, 8, 64)
-TRIXI_REG(double, 3, 3, VOL_FD_RANOCHA, 8, 128)
-TRIXI_REG(double, 3, 3, VOL_FD_RANOCHA, 2, 64)
-TRIXI_REG(float, 3, 4, VOL_FD_RANOCHA, 2, 64)
-TRIXI_REG(float, 3, 4, VOL_FD_RANOCHA, 10, 256)
-TRIXI_REG(double, 3, 4, VOL_FD_RANOCHA, 2, 64)
-TRIXI_REG(float, 2, 3, VOL_HG_RANOCHA, 32, 128)
-TRIXI_REG(float, 2, 3, VOL_HG_RANOCHA, 64, 256)
+TRIXI_REG(float, 3, 3, VOL_FD_RANOCHA, 8, 2)
+TRIXI_REG(double, 3, 3, VOL_FD_RANOCHA, 8, 2)
+TRIXI_REG(float, 3, 4, VOL_FD_RANOCHA, 5, 2)
+TRIXI_REG(double, 3, 4, VOL_FD_RANOCHA, 5, 2)
+TRIXI_REG(float, 2, 3, VOL_HG_RANOCHA, 32, 2)
+TRIXI_REG(double, 2, 3, VOL_HG_RANOCHA, 32, 2)
+TRIXI_REG(float, 2, 3, VOL_FD_RANOCHA, 32, 2)
 
 struct DefaultTile {
   int D, N, epb, nt;
 };
-const DefaultTile kDefaultTiles[] = {{2, 2, 21, 64}, {2, 3, 16, 64}, {2, 4, 12, 64},
-                                     {3, 2, 7, 64},  {3, 3, 4, 64},  {3, 4, 5, 128}};
+const DefaultTile kDefaultTiles[] = {{2, 2, 21, 128}, {2, 3, 16, 128}, {2, 4, 12, 128},
+                                     {3, 2, 7, 192},  {3, 3, 4, 192},  {3, 4, 2, 192}};
 
 int vol_kind(const trixi_hip_config &c) {
   switch (c.volume_integral) {
