@@ -108,10 +108,22 @@ __device__ __forceinline__ T inv_ln_mean_l(T x, T y, T lx, T ly) {
 // inverse temperature beta = rho/p, its log, half velocities, half pressure.
 // flux_ranocha's inv_ln_mean(rho_ll p_rr, rho_rr p_ll) p_ll p_rr equals
 // inv_ln_mean(beta_ll, beta_rr) (compressible_euler_3d.jl:752-757).
+// The "log" fields hold ln(x) in FP64 and log2(x)/2 in FP32 (see
+// flux_ranocha_fast); lnv() returns the natural log in both cases.
 template <typename T, int D> struct NodeR {
   T rho, lrho, beta, lbeta, h[D], hp;
 };
 constexpr int nodeR_count(int D) { return 5 + D; }
+
+template <typename T> struct LogRep;
+template <> struct LogRep<double> {
+  static __device__ __forceinline__ double store(double x) { return ::log(x); }
+  static __device__ __forceinline__ double lnv(double a) { return a; }
+};
+template <> struct LogRep<float> {
+  static __device__ __forceinline__ float store(float x) { return 0.5f * __builtin_amdgcn_logf(x); }
+  static __device__ __forceinline__ float lnv(float a) { return a * 1.38629436111989061883f; }  // 2 ln 2
+};
 
 template <typename T, int D>
 __device__ __forceinline__ void nodeR_from_cons(const T *u, T gamma_m1, NodeR<T, D> &w) {
@@ -126,9 +138,9 @@ __device__ __forceinline__ void nodeR_from_cons(const T *u, T gamma_m1, NodeR<T,
   }
   T p = gamma_m1 * (u[D + 1] - T(0.5) * ke2);
   w.rho = rho;
-  w.lrho = Prec<T>::ln(rho);
+  w.lrho = LogRep<T>::store(rho);
   w.beta = rho * Prec<T>::rcp(p);
-  w.lbeta = w.lrho - Prec<T>::ln(p);
+  w.lbeta = w.lrho - LogRep<T>::store(p);
 #pragma unroll
   for (int d = 0; d < D; ++d) w.h[d] = T(0.5) * v[d];
   w.hp = T(0.5) * p;
@@ -138,8 +150,8 @@ __device__ __forceinline__ void nodeR_from_cons(const T *u, T gamma_m1, NodeR<T,
 template <typename T, int D, int O>
 __device__ __forceinline__ void flux_ranocha_r(const NodeR<T, D> &l, const NodeR<T, D> &r,
                                                T inv_gamma_m1, T *f) {
-  T rho_mean = ln_mean_l(l.rho, r.rho, l.lrho, r.lrho);
-  T inv_rho_p_mean = inv_ln_mean_l(l.beta, r.beta, l.lbeta, r.lbeta);
+  T rho_mean = ln_mean_l(l.rho, r.rho, LogRep<T>::lnv(l.lrho), LogRep<T>::lnv(r.lrho));
+  T inv_rho_p_mean = inv_ln_mean_l(l.beta, r.beta, LogRep<T>::lnv(l.lbeta), LogRep<T>::lnv(r.lbeta));
   T vavg[D];
   T hh = 0;
 #pragma unroll
@@ -157,6 +169,47 @@ __device__ __forceinline__ void flux_ranocha_r(const NodeR<T, D> &l, const NodeR
   T inner = tfma(inv_rho_p_mean, inv_gamma_m1, hh + hh);
   f[D + 1] = tfma(f1, inner, cross + cross);
 }
+
+// FP32 flux_ranocha without divisions. With a = log2(x)/2 stored per node and
+// w = ln(y/x)/2 = ln2 (a_y - a_x), the logarithmic mean satisfies exactly
+//   ln_mean(x, y) = sqrt(x y) sinh(w)/w = 2^(a_x + a_y) * P(w^2),
+//   1/ln_mean(x, y) = 2^-(a_x + a_y) * Q(w^2),
+// P(w^2) = sum_k w^(2k)/(2k+1)!,  Q = w/sinh(w) = 1 - w^2/6 + 7 w^4/360 - 31 w^6/15120 + 127 w^8/604800.
+// Truncated after w^8 the relative error is <= 2.2e-8 for |w| <= 0.5 (y/x <= e),
+// below FP32 round-off; callers use this only when every pair of the line is
+// within that range (flux_ranocha_r otherwise). Coefficients below are pre-
+// multiplied by ln2^(2k) so that the series variable is (a_y - a_x)^2.
+template <int D, int O>
+__device__ __forceinline__ void flux_ranocha_fast(const NodeR<float, D> &l, const NodeR<float, D> &r,
+                                                  float inv_gamma_m1, float *f) {
+  const float m1 = l.lrho + r.lrho, d1 = r.lrho - l.lrho, z1 = d1 * d1;
+  const float p1 = tfma(tfma(tfma(tfma(1.46838742113e-7f, z1, 2.20050434191e-5f), z1, 0.00192362582153f), z1,
+                            0.0800755023197f), z1, 1.0f);
+  const float rho_mean = __builtin_amdgcn_exp2f(m1) * p1;
+  const float m2 = l.lbeta + r.lbeta, d2 = r.lbeta - l.lbeta, z2 = d2 * d2;
+  const float q2 = tfma(tfma(tfma(tfma(1.1189112149e-5f, z2, -0.000227385448664f), z2, 0.00448846025023f), z2,
+                            -0.0800755023197f), z2, 1.0f);
+  const float inv_rho_p_mean = __builtin_amdgcn_exp2f(-m2) * q2;
+  float vavg[D];
+  float hh = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    vavg[d] = l.h[d] + r.h[d];
+    hh = tfma(l.h[d], r.h[d], hh);
+  }
+  const float p_avg = l.hp + r.hp;
+  const float cross = tfma(l.hp, r.h[O], r.hp * l.h[O]);
+  const float f1 = rho_mean * vavg[O];
+  f[0] = f1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) f[1 + d] = f1 * vavg[d];
+  f[1 + O] += p_avg;
+  const float inner = tfma(inv_rho_p_mean, inv_gamma_m1, hh + hh);
+  f[D + 1] = tfma(f1, inner, cross + cross);
+}
+
+// |a_y - a_x| bound of the FP32 fast path: |w| <= 0.5  <=>  |a_y - a_x| <= 0.5/ln2
+constexpr float kFastLogSpan = 0.7213475204f;
 
 // Physical flux of a conservative state in orientation o (runtime), plus the
 // primitive data LLF needs.
@@ -240,19 +293,11 @@ __device__ __forceinline__ void num_flux_c(int kind, const T *ul, const T *ur, T
   }
 }
 
-// source_terms_convergence_test (compressible_euler_3d.jl:127-153, 2D :120-146).
+// source_terms_convergence_test (compressible_euler_3d.jl:127-153, 2D :120-146),
+// given si, co = sincos(omega (x_1 + ... + x_D - t)), omega = pi.
 template <typename T, int D>
-__device__ __forceinline__ void source_convergence(const T *x, T t, T gamma, T *s) {
-  const T c = 2, A = T(0.1), omega = T(3.14159265358979323846);  // 2 pi / L, L = 2
-  T xs = 0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) xs += x[d];
-  T si, co;
-  if constexpr (sizeof(T) == 4) {
-    sincosf(omega * (xs - t), &si, &co);
-  } else {
-    sincos(omega * (xs - t), &si, &co);
-  }
+__device__ __forceinline__ void source_convergence_sc(T si, T co, T gamma, T *s) {
+  const T c = 2, A = T(0.1), omega = T(3.14159265358979323846);
   T rho = c + A * si;
   T rho_x = omega * A * co;
   if constexpr (D == 3) {
@@ -270,5 +315,8 @@ __device__ __forceinline__ void source_convergence(const T *x, T t, T gamma, T *
     s[3] = 2 * rho_x * (rho + tmp);
   }
 }
+
+__device__ __forceinline__ void tsincos(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ void tsincos(double x, double *s, double *c) { sincos(x, s, c); }
 
 }  // namespace trixi_hip

@@ -132,21 +132,64 @@ template <int VOL> struct VolTraits {
   static constexpr bool CONS = !RAN || HG;
 };
 
+// Per-lane line context: which line the lane owns and the four face states
+// (neighbour states loaded from global memory at kernel entry, own states read
+// from the staged tile) so that no load latency sits inside the line phase.
+template <typename T, int V> struct LineCtx {
+  int el, line, base;
+  bool valid;
+  T ul_m[V], ur_p[V], own0[V], own1[V];
+};
+
+template <typename T, int D, int N, int EPB, int O>
+__device__ __forceinline__ void line_setup(const ElemArgs<T> &a, int e0, int ne, int ll,
+                                           LineCtx<T, D + 2> &c) {
+  using LY = Layout<D, N, EPB>;
+  constexpr int n = N + 1, NN = LY::NN, V = D + 2;
+  constexpr int st = ipow_c(n, O);
+  LY::template lane_line<O>(ll, c.el, c.line);
+  c.valid = (ll < EPB * LY::L) && (c.el < ne);
+  if (!c.valid) return;
+  c.base = (c.line % st) + (c.line / st) * st * n;
+  const int e = e0 + c.el;
+  const int nbm = a.nbr[(2 * O) + 2 * D * e];
+  const int nbp = a.nbr[(2 * O + 1) + 2 * D * e];
+  const T *un_m = a.u + ((size_t)nbm * NN + c.base + (n - 1) * st) * V;
+  const T *un_p = a.u + ((size_t)nbp * NN + c.base) * V;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    c.ul_m[v] = un_m[v];
+    c.ur_p[v] = un_p[v];
+  }
+}
+
+// own face states from the staged tile (AoS [q][v])
+template <typename T, int D, int N, int EPB, int O>
+__device__ __forceinline__ void line_own(const T *__restrict__ staged, LineCtx<T, D + 2> &c) {
+  using LY = Layout<D, N, EPB>;
+  constexpr int n = N + 1, NN = LY::NN, V = D + 2;
+  constexpr int st = ipow_c(n, O);
+  if (!c.valid) return;
+  const int q0 = c.el * NN + c.base;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    c.own0[v] = staged[q0 * V + v];
+    c.own1[v] = staged[(q0 + (n - 1) * st) * V + v];
+  }
+}
+
 // One line of direction O: volume terms, the two surface terms, LDS write.
 template <typename T, int D, int N, int VOL, int EPB, int O>
-__device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T, N> &k, int e0, int ne,
-                                          int ll, const T *__restrict__ s_w, const T *__restrict__ s_c,
+__device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T, N> &k, const LineCtx<T, D + 2> &c,
+                                          const T *__restrict__ s_w, const T *__restrict__ s_c,
                                           T *__restrict__ s_d, const double *__restrict__ s_alpha) {
   using LY = Layout<D, N, EPB>;
   using VT = VolTraits<VOL>;
   constexpr int n = N + 1, NN = LY::NN, V = D + 2, SZ = LY::SIZE;
   constexpr int st = ipow_c(n, O);
-  int el, line;
-  LY::template lane_line<O>(ll, el, line);
-  if (el >= ne) return;
-  const int base = (line % st) + (line / st) * st * n;
-  const int q0 = el * NN + base;
-  const int e = e0 + el;
+  if (!c.valid) return;
+  const int el = c.el;
+  const int q0 = el * NN + c.base;
   const T gamma = a.gamma;
   const T gm1 = gamma - T(1);
   const T igm1 = T(1) / gm1;
@@ -154,32 +197,6 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
   int pq[n];
 #pragma unroll
   for (int i = 0; i < n; ++i) pq[i] = LY::phys(q0 + i * st);
-
-  // issue the face-state loads first: they overlap the volume arithmetic
-  const int nbm = a.nbr[(2 * O) + 2 * D * e];
-  const int nbp = a.nbr[(2 * O + 1) + 2 * D * e];
-  const T *un_m = a.u + ((size_t)nbm * NN + base + (n - 1) * st) * V;
-  const T *un_p = a.u + ((size_t)nbp * NN + base) * V;
-  T ul_m[V], ur_p[V], own0[V], own1[V];
-#pragma unroll
-  for (int v = 0; v < V; ++v) {
-    ul_m[v] = un_m[v];
-    ur_p[v] = un_p[v];
-  }
-  if constexpr (VT::CONS) {
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      own0[v] = s_c[v * SZ + pq[0]];
-      own1[v] = s_c[v * SZ + pq[n - 1]];
-    }
-  } else {
-    const T *uo = a.u + ((size_t)e * NN + base) * V;
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      own0[v] = uo[v];
-      own1[v] = uo[(size_t)(n - 1) * st * V + v];
-    }
-  }
 
   T acc[n][V];
 #pragma unroll
@@ -226,19 +243,51 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
       for (int d = 0; d < D; ++d) w[i].h[d] = s_w[(4 + d) * SZ + pq[i]];
       w[i].hp = s_w[(4 + D) * SZ + pq[i]];
     }
+    bool fast = false;
+    if constexpr (sizeof(T) == 4) {
+      // the division-free series applies when every log ratio on the line is in range
+      T lo_r = w[0].lrho, hi_r = w[0].lrho, lo_b = w[0].lbeta, hi_b = w[0].lbeta;
 #pragma unroll
-    for (int i = 0; i < n; ++i)
-#pragma unroll
-      for (int j = i + 1; j < n; ++j) {
-        T f[V];
-        flux_ranocha_r<T, D, O>(w[i], w[j], igm1, f);
-        const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          acc[i][v] = tfma(di, f[v], acc[i][v]);
-          acc[j][v] = tfma(dj, f[v], acc[j][v]);
-        }
+      for (int i = 1; i < n; ++i) {
+        lo_r = tmin(lo_r, w[i].lrho);
+        hi_r = tmax(hi_r, w[i].lrho);
+        lo_b = tmin(lo_b, w[i].lbeta);
+        hi_b = tmax(hi_b, w[i].lbeta);
       }
+      const bool in_range = (hi_r - lo_r <= T(kFastLogSpan)) && (hi_b - lo_b <= T(kFastLogSpan));
+      fast = __builtin_amdgcn_ballot_w64(!in_range) == 0;  // wave-uniform
+    }
+    if (fast) {
+      if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+#pragma unroll
+          for (int j = i + 1; j < n; ++j) {
+            T f[V];
+            flux_ranocha_fast<D, O>(w[i], w[j], igm1, f);
+            const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+              acc[i][v] = tfma(di, f[v], acc[i][v]);
+              acc[j][v] = tfma(dj, f[v], acc[j][v]);
+            }
+          }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < n; ++i)
+#pragma unroll
+        for (int j = i + 1; j < n; ++j) {
+          T f[V];
+          flux_ranocha_r<T, D, O>(w[i], w[j], igm1, f);
+          const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            acc[i][v] = tfma(di, f[v], acc[i][v]);
+            acc[j][v] = tfma(dj, f[v], acc[j][v]);
+          }
+        }
+    }
   }
   if constexpr (VT::CEN) {
     T fph[n][V];
@@ -301,10 +350,10 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
 #pragma unroll
           for (int v = 0; v < V; ++v) fnext[v] = T(0);
         }
-        const T c = afv * k.inv_w[i];
+        const T cw = afv * k.inv_w[i];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          acc[i][v] = tfma(c, fnext[v] - fprev[v], acc[i][v]);
+          acc[i][v] = tfma(cw, fnext[v] - fprev[v], acc[i][v]);
           fprev[v] = fnext[v];
         }
       }
@@ -312,13 +361,16 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
   }
 
   // ---------------- interface fluxes + surface integral ----------------
+  // keep the face work (which waits on the neighbour loads issued above)
+  // behind the volume arithmetic
+  __builtin_amdgcn_sched_barrier(0);
   const T factor = k.inv_w[0];  // inverse_weights[1] of the reference
   {
     T F[V];
-    num_flux_c<T, D, O>(a.surface_flux, ul_m, own0, gamma, F);  // -O face: element is the right state
+    num_flux_c<T, D, O>(a.surface_flux, c.ul_m, c.own0, gamma, F);  // -O face: element is the right state
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
-    num_flux_c<T, D, O>(a.surface_flux, own1, ur_p, gamma, F);  // +O face: element is the left state
+    num_flux_c<T, D, O>(a.surface_flux, c.own1, c.ur_p, gamma, F);  // +O face: element is the left state
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
   }
@@ -329,128 +381,212 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
     for (int v = 0; v < V; ++v) s_d[(O * V + v) * SZ + pq[i]] = acc[i][v];
 }
 
-template <typename T, int D, int N, int VOL, int EPB, int WPD>
+// Source-term phase tables of one element: with x_d = c_d + J xi_{i_d} the
+// phase theta = pi (x_1 + ... + x_D - t) splits into
+//   theta = alpha_{i_1} + gamma_{i_2} (+ gamma_{i_3}),
+//   alpha_i = pi (c_1 + ... + c_D - t) + pi J xi_i,  gamma_j = pi J xi_j,
+// so each element needs 2n sincos evaluations instead of one per node; the
+// node values follow by angle addition.
+template <typename T, int D, int N>
+__device__ __forceinline__ void source_tables(const ElemArgs<T> &a, const DGConst<T, N> &k, int e, int idx,
+                                              double tt, T *__restrict__ tab) {
+  constexpr int n = N + 1;
+  const double jac = 1.0 / a.inv_jac[e];
+  const double pi = 3.14159265358979323846;
+  double csum = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) csum += a.centers[d + D * e];
+  const int which = idx / n, i = idx - which * n;  // which: 0 = alpha, 1 = gamma
+  const double ph = (which == 0 ? pi * (csum - tt) : 0.0) + pi * jac * k.nodes[i];
+  T si, co;
+  tsincos(T(ph), &si, &co);
+  tab[2 * idx] = si;
+  tab[2 * idx + 1] = co;
+}
+
+// PERSIST: a resident grid loops over tiles with a one-tile register prefetch
+// of the next tile's state; otherwise one tile per workgroup.
+template <typename T, int D, int N, int VOL, int EPB, int WPD, bool PERSIST = false>
 __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
   using LY = Layout<D, N, EPB>;
   using VT = VolTraits<VOL>;
   constexpr int NT = D * WPD * 64;
-  constexpr int n = N + 1, NN = LY::NN, V = D + 2, SZ = LY::SIZE, L = LY::L;
+  constexpr int n = N + 1, NN = LY::NN, V = D + 2, SZ = LY::SIZE;
   constexpr int NW = VT::RAN ? nodeR_count(D) : 0;
   constexpr int NC = VT::CONS ? V : 0;
   constexpr int SIZE_A = (NW + NC) * SZ;  // primitive / conservative data, later output staging
   constexpr int SIZE_B = D * V * SZ;      // du per direction, earlier input staging
   static_assert(SIZE_A >= EPB * NN * V && SIZE_B >= EPB * NN * V, "staging does not fit");
-  static_assert(EPB * L <= WPD * 64, "one line per lane");
+  static_assert(EPB * LY::L <= WPD * 64, "one line per lane");
+  constexpr bool vec_ok = ((NN * V * (int)sizeof(T)) % 16) == 0;
+  constexpr int TILE4 = vec_ok ? EPB * NN * V * (int)sizeof(T) / 16 : 1;  // 16-byte chunks per tile
+  constexpr int PF = (vec_ok && PERSIST) ? (TILE4 + NT - 1) / NT : 1;      // chunks per thread
   __shared__ __attribute__((aligned(16))) T smem[SIZE_A + SIZE_B];
   __shared__ double s_alpha[VT::HG ? EPB : 1];
-  T *s_w = smem;               // [NW][SZ]
-  T *s_c = smem + NW * SZ;     // [NC][SZ]
-  T *s_d = smem + SIZE_A;      // [D][V][SZ]
+  __shared__ T s_src[EPB * 4 * n];  // per element: (sin, cos) of alpha_i, gamma_j
+  T *s_w = smem;            // [NW][SZ]
+  T *s_c = smem + NW * SZ;  // [NC][SZ]
+  T *s_d = smem + SIZE_A;   // [D][V][SZ]
+  T *s_o = smem;            // [EPB*NN][V] output staging (region A, free after phase B)
 
   const int tid = threadIdx.x;
-  const int blk = a.xcd_remap ? xcd_remap_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  const int e0 = blk * EPB;
-  if (e0 >= a.E) return;  // uniform per block
-  const int ne = min(EPB, a.E - e0);
-  constexpr bool vec_ok = ((NN * V * (int)sizeof(T)) % 16) == 0;
-
-  // ---- phase A: stage the tile, per-node data into LDS ----
-  tile_copy<T, NT>(s_d, a.u + (size_t)e0 * NN * V, ne * NN * V, vec_ok);
-  if constexpr (VT::HG) {
-    if (tid < ne) {
-      const int e = e0 + tid;
-      double al = a.alpha_raw[e];
-      if (a.alpha_smooth) {
-        // apply_smoothing! (indicators_2d.jl:104-135, indicators_3d.jl:125-173):
-        // alpha_e = max(alpha_e, 0.5 alpha_nbr) over the element's interfaces
-        double m = al;
-#pragma unroll
-        for (int dir = 0; dir < 2 * D; ++dir) {
-          double an = 0.5 * a.alpha_raw[a.nbr[dir + 2 * D * e]];
-          m = an > m ? an : m;
-        }
-        al = m;
-      }
-      s_alpha[tid] = al;
-      if (a.alpha_out) a.alpha_out[e] = al;
-    }
-  }
-  __syncthreads();
-  {
-    const T gm1 = a.gamma - T(1);
-    for (int q = tid; q < ne * NN; q += NT) {
-      const int p = LY::phys(q);
-      T uq[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) uq[v] = s_d[q * V + v];
-      if constexpr (VT::CONS) {
-#pragma unroll
-        for (int v = 0; v < V; ++v) s_c[v * SZ + p] = uq[v];
-      }
-      if constexpr (VT::RAN) {
-        NodeR<T, D> w;
-        nodeR_from_cons<T, D>(uq, gm1, w);
-        s_w[0 * SZ + p] = w.rho;
-        s_w[1 * SZ + p] = w.lrho;
-        s_w[2 * SZ + p] = w.beta;
-        s_w[3 * SZ + p] = w.lbeta;
-#pragma unroll
-        for (int d = 0; d < D; ++d) s_w[(4 + d) * SZ + p] = w.h[d];
-        s_w[(4 + D) * SZ + p] = w.hp;
-      }
-    }
-  }
-  __syncthreads();
-
-  // ---- phase B: one direction per wave group, one line per lane ----
-  {
-    const int grp = tid / (WPD * 64);  // wave-uniform
-    const int ll = tid - grp * (WPD * 64);
-    if (ll < EPB * L) {
-      if (grp == 0) line_work<T, D, N, VOL, EPB, 0>(a, k, e0, ne, ll, s_w, s_c, s_d, s_alpha);
-      else if (grp == 1) line_work<T, D, N, VOL, EPB, 1>(a, k, e0, ne, ll, s_w, s_c, s_d, s_alpha);
-      else if constexpr (D == 3) line_work<T, D, N, VOL, EPB, 2>(a, k, e0, ne, ll, s_w, s_c, s_d, s_alpha);
-    }
-  }
-  __syncthreads();
-
-  // ---- phase C: sum directions, Jacobian, sources; stage the output ----
+  const int grp = tid / (WPD * 64);  // direction of this wave group (wave-uniform)
+  const int ll = tid - grp * (WPD * 64);
+  const int ntiles = (a.E + EPB - 1) / EPB;
   const double tt = a.tdev ? (a.tdev[0] + a.t * a.tdev[1]) : a.t;
-  T *s_o = smem;  // [EPB*NN][V] staging (region A is free now)
-  for (int q = tid; q < ne * NN; q += NT) {
-    const int el = q / NN, node = q - el * NN;
-    const int e = e0 + el;
-    const int p = LY::phys(q);
-    const double ij = a.inv_jac[e];
-    const T fac = T(-ij);
-    T d[V];
+
+  // Persistent loop over tiles with a one-tile-deep register prefetch: the
+  // next tile's state and neighbour face states are in flight while the
+  // current tile is computed.
+  int tile = blockIdx.x;
+  if (tile >= ntiles) return;  // uniform per block
+  float4 pf[PF];
+  auto prefetch = [&](int t) {
+    if constexpr (vec_ok && PERSIST) {
+      const int e0 = t * EPB;
+      const int ne = min(EPB, a.E - e0);
+      const float4 *src = reinterpret_cast<const float4 *>(a.u + (size_t)e0 * NN * V);
+      const int c4 = ne * NN * V * (int)sizeof(T) / 16;
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      T s = s_d[(0 * V + v) * SZ + p];
-#pragma unroll
-      for (int o = 1; o < D; ++o) s += s_d[(o * V + v) * SZ + p];
-      d[v] = s * fac;
-    }
-    if (a.source) {
-      const double jac = 1.0 / ij;
-      T x[D];
-      int rem = node;
-#pragma unroll
-      for (int dd = 0; dd < D; ++dd) {
-        const int idx = rem % n;
-        rem /= n;
-        x[dd] = T(a.centers[dd + D * e] + jac * k.nodes[idx]);
+      for (int r = 0; r < PF; ++r) {
+        const int i = tid + r * NT;
+        if (i < c4) pf[r] = src[i];
       }
-      T s[V];
-      source_convergence<T, D>(x, T(tt), a.gamma, s);
-#pragma unroll
-      for (int v = 0; v < V; ++v) d[v] += s[v];
     }
+  };
+  prefetch(tile);
+
+  while (true) {
+    const int e0 = tile * EPB;
+    const int ne = min(EPB, a.E - e0);
+    // ---- phase A: stage the tile, per-node data into LDS ----
+    if constexpr (!PERSIST) {
+      tile_copy<T, NT>(s_d, a.u + (size_t)e0 * NN * V, ne * NN * V, vec_ok);
+    } else if constexpr (vec_ok) {
+      float4 *d4 = reinterpret_cast<float4 *>(s_d);
+      const int c4 = ne * NN * V * (int)sizeof(T) / 16;
 #pragma unroll
-    for (int v = 0; v < V; ++v) s_o[q * V + v] = d[v];
+      for (int r = 0; r < PF; ++r) {
+        const int i = tid + r * NT;
+        if (i < c4) d4[i] = pf[r];
+      }
+    } else {
+      const T *src = a.u + (size_t)e0 * NN * V;
+      for (int i = tid; i < ne * NN * V; i += NT) s_d[i] = src[i];
+    }
+    // neighbour face states of this tile (consumed in phase B), then the
+    // next tile's state (consumed in the next iteration)
+    LineCtx<T, V> lc;
+    if (grp == 0) line_setup<T, D, N, EPB, 0>(a, e0, ne, ll, lc);
+    else if (grp == 1) line_setup<T, D, N, EPB, 1>(a, e0, ne, ll, lc);
+    else if constexpr (D == 3) line_setup<T, D, N, EPB, 2>(a, e0, ne, ll, lc);
+    const int next = PERSIST ? tile + (int)gridDim.x : ntiles;
+    if constexpr (PERSIST) {
+      if (next < ntiles) prefetch(next);
+    }
+
+    if constexpr (VT::HG) {
+      if (tid < ne) {
+        const int e = e0 + tid;
+        double al = a.alpha_raw[e];
+        if (a.alpha_smooth) {
+          // apply_smoothing! (indicators_2d.jl:104-135, indicators_3d.jl:125-173):
+          // alpha_e = max(alpha_e, 0.5 alpha_nbr) over the element's interfaces
+          double m = al;
+#pragma unroll
+          for (int dir = 0; dir < 2 * D; ++dir) {
+            double an = 0.5 * a.alpha_raw[a.nbr[dir + 2 * D * e]];
+            m = an > m ? an : m;
+          }
+          al = m;
+        }
+        s_alpha[tid] = al;
+        if (a.alpha_out) a.alpha_out[e] = al;
+      }
+    }
+    if (a.source && tid < ne * 2 * n) source_tables<T, D, N>(a, k, e0 + tid / (2 * n), tid % (2 * n), tt, &s_src[(tid / (2 * n)) * 4 * n]);
+    __syncthreads();
+    if (grp == 0) line_own<T, D, N, EPB, 0>(s_d, lc);
+    else if (grp == 1) line_own<T, D, N, EPB, 1>(s_d, lc);
+    else if constexpr (D == 3) line_own<T, D, N, EPB, 2>(s_d, lc);
+    {
+      const T gm1 = a.gamma - T(1);
+      for (int q = tid; q < ne * NN; q += NT) {
+        const int p = LY::phys(q);
+        T uq[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) uq[v] = s_d[q * V + v];
+        if constexpr (VT::CONS) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) s_c[v * SZ + p] = uq[v];
+        }
+        if constexpr (VT::RAN) {
+          NodeR<T, D> w;
+          nodeR_from_cons<T, D>(uq, gm1, w);
+          s_w[0 * SZ + p] = w.rho;
+          s_w[1 * SZ + p] = w.lrho;
+          s_w[2 * SZ + p] = w.beta;
+          s_w[3 * SZ + p] = w.lbeta;
+#pragma unroll
+          for (int d = 0; d < D; ++d) s_w[(4 + d) * SZ + p] = w.h[d];
+          s_w[(4 + D) * SZ + p] = w.hp;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- phase B: one direction per wave group, one line per lane ----
+    if (grp == 0) line_work<T, D, N, VOL, EPB, 0>(a, k, lc, s_w, s_c, s_d, s_alpha);
+    else if (grp == 1) line_work<T, D, N, VOL, EPB, 1>(a, k, lc, s_w, s_c, s_d, s_alpha);
+    else if constexpr (D == 3) line_work<T, D, N, VOL, EPB, 2>(a, k, lc, s_w, s_c, s_d, s_alpha);
+    __syncthreads();
+
+    // ---- phase C: sum directions, Jacobian, sources; stage the output ----
+    for (int q = tid; q < ne * NN; q += NT) {
+      const int el = q / NN, node = q - el * NN;
+      const int e = e0 + el;
+      const int p = LY::phys(q);
+      const T fac = T(-a.inv_jac[e]);
+      T d[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        T sum = s_d[(0 * V + v) * SZ + p];
+#pragma unroll
+        for (int o = 1; o < D; ++o) sum += s_d[(o * V + v) * SZ + p];
+        d[v] = sum * fac;
+      }
+      if (a.source) {
+        // theta = alpha_i + gamma_j (+ gamma_k) by angle addition
+        const T *tb = &s_src[el * 4 * n];
+        const int i0 = node % n, i1 = (node / n) % n;
+        T si = tb[2 * i0], co = tb[2 * i0 + 1];
+        T sg = tb[2 * (n + i1)], cg = tb[2 * (n + i1) + 1];
+        T s2 = tfma(si, cg, co * sg), c2 = tfma(co, cg, -si * sg);
+        if constexpr (D == 3) {
+          const int i2 = node / (n * n);
+          sg = tb[2 * (n + i2)];
+          cg = tb[2 * (n + i2) + 1];
+          si = tfma(s2, cg, c2 * sg);
+          co = tfma(c2, cg, -s2 * sg);
+        } else {
+          si = s2;
+          co = c2;
+        }
+        T s[V];
+        source_convergence_sc<T, D>(si, co, a.gamma, s);
+#pragma unroll
+        for (int v = 0; v < V; ++v) d[v] += s[v];
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) s_o[q * V + v] = d[v];
+    }
+    __syncthreads();
+    tile_copy<T, NT>(a.du + (size_t)e0 * NN * V, s_o, ne * NN * V, vec_ok);
+    if (!PERSIST || next >= ntiles) break;
+    tile = next;
+    // region A (s_o) is read by the store above and written again by phase A of
+    // the next tile only after the next barrier; region B is free now.
   }
-  __syncthreads();
-  tile_copy<T, NT>(a.du + (size_t)e0 * NN * V, s_o, ne * NN * V, vec_ok);
 }
 
 }  // namespace trixi_hip

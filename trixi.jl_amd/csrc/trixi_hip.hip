@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -32,24 +33,42 @@ struct ElemKey {
 // Type-erased launcher: args and constants are passed as raw blobs that the
 // instantiation reinterprets as ElemArgs<T> / DGConst<T, N>.
 using ElemLaunch = hipError_t (*)(const void *args, const void *kconst, int nblocks, hipStream_t s);
+using ElemOccupancy = int (*)();  // resident workgroups per CU
 
-std::map<ElemKey, ElemLaunch> &elem_registry() {
-  static std::map<ElemKey, ElemLaunch> r;
+struct ElemEntry {
+  ElemLaunch launch;
+  ElemOccupancy occupancy;
+};
+
+std::map<ElemKey, ElemEntry> &elem_registry() {
+  static std::map<ElemKey, ElemEntry> r;
   return r;
 }
 
-template <typename T, int D, int N, int VOL, int EPB, int WPD>
+template <typename T, int D, int N, int VOL, int EPB, int WPD, bool P>
 hipError_t launch_elem(const void *args, const void *kconst, int nblocks, hipStream_t s) {
   const ElemArgs<T> &a = *static_cast<const ElemArgs<T> *>(args);
   const DGConst<T, N> &k = *static_cast<const DGConst<T, N> *>(kconst);
-  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, WPD>), dim3(nblocks), dim3(D * WPD * 64), 0, s, a,
+  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, WPD, P>), dim3(nblocks), dim3(D * WPD * 64), 0, s, a,
                      k);
   return hipGetLastError();
 }
 
-template <typename T, int D, int N, int VOL, int EPB, int WPD> struct Reg {
+// persistent variants run a grid of resident workgroups; the others one tile per workgroup
+template <typename T, int D, int N, int VOL, int EPB, int WPD, bool P> int occupancy_elem() {
+  if (!P) return 0;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dg_element_kernel<T, D, N, VOL, EPB, WPD, P>, D * WPD * 64,
+                                                   0) != hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
+}
+
+// registry key: a persistent variant is keyed with a negative epb
+template <typename T, int D, int N, int VOL, int EPB, int WPD, bool P = false> struct Reg {
   Reg() {
-    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, D * WPD * 64}] = &launch_elem<T, D, N, VOL, EPB, WPD>;
+    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, P ? -EPB : EPB, D * WPD * 64}] =
+        ElemEntry{&launch_elem<T, D, N, VOL, EPB, WPD, P>, &occupancy_elem<T, D, N, VOL, EPB, WPD, P>};
   }
 };
 
@@ -87,6 +106,8 @@ TRIXI_REG(double, 3, 4, VOL_FD_RANOCHA, 5, 2)
 TRIXI_REG(float, 2, 3, VOL_HG_RANOCHA, 32, 2)
 TRIXI_REG(double, 2, 3, VOL_HG_RANOCHA, 32, 2)
 TRIXI_REG(float, 2, 3, VOL_FD_RANOCHA, 32, 2)
+static Reg<float, 3, 3, VOL_FD_RANOCHA, 4, 1, true> reg_persist_f_3_3;
+static Reg<double, 3, 3, VOL_FD_RANOCHA, 4, 1, true> reg_persist_d_3_3;
 
 struct DefaultTile {
   int D, N, epb, nt;
@@ -158,8 +179,9 @@ struct trixi_hip_ctx {
   // element-kernel launch
   ElemLaunch elem = nullptr;
   int epb = 0, nt = 0;
+  int grid_cap = 0;  // resident workgroups on the device (persistent element kernel)
   std::vector<unsigned char> kconst;  // DGConst<T, N> blob
-  int xcd_remap = 1;
+  int xcd_remap = 0;  // measured: the round-robin XCD placement is faster here
 
   // graph of one RK step
   hipGraph_t graph = nullptr;
@@ -213,9 +235,16 @@ int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
     return set_err(ctx, TRIXI_HIP_EINVAL,
                    "no element kernel instantiated for this (precision, ndims, polydeg, volume "
                    "integral, tile)");
-  ctx->elem = it->second;
+  ctx->elem = it->second.launch;
   ctx->epb = epb;
   ctx->nt = nt;
+  // persistent grid: every workgroup resident at once, looping over tiles
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
+    cus = 256;
+  const int occ = it->second.occupancy();
+  ctx->grid_cap = occ > 0 ? cus * occ : 0;
+  ctx->epb = epb < 0 ? -epb : epb;
   return TRIXI_HIP_OK;
 }
 
@@ -254,6 +283,7 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   a.xcd_remap = ctx->xcd_remap;
   a.gamma = T(ctx->cfg.gamma);
   int nblocks = (ctx->E + ctx->epb - 1) / ctx->epb;
+  if (ctx->grid_cap > 0 && nblocks > ctx->grid_cap) nblocks = ctx->grid_cap;
   hipError_t e = ctx->elem(&a, ctx->kconst.data(), nblocks, ctx->stream);
   if (e != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("element kernel launch: ") + hipGetErrorString(e));
   return TRIXI_HIP_OK;
@@ -361,19 +391,19 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   ctx->hg.alpha_min = cfg->alpha_min;
   ctx->hg.threshold = 0.5 * std::pow(10.0, -1.8 * std::pow((double)n, 0.25));
   ctx->hg.parameter_s = std::log((1 - 0.0001) / 0.0001);
+  if (const char *env = std::getenv("TRIXI_HIP_XCD_REMAP")) ctx->xcd_remap = std::atoi(env) != 0;
   ctx->exact.kind = TRIXI_HIP_IC_CONSTANT;
   ctx->exact.gamma = cfg->gamma;
 
   if (ctx->fp32) build_kconst<float>(ctx);
   else build_kconst<double>(ctx);
+  if (hipGetDevice(&ctx->device) != hipSuccess) return fail(TRIXI_HIP_EHIP, "hipGetDevice failed");
   if (select_tile(ctx, cfg->elements_per_block, cfg->threads_per_block) != TRIXI_HIP_OK) {
     fprintf(stderr, "trixi_hip_create: %s\n", ctx->err.c_str());
     delete ctx;
     return TRIXI_HIP_EINVAL;
   }
 
-  hipError_t he = hipGetDevice(&ctx->device);
-  if (he != hipSuccess) return fail(TRIXI_HIP_EHIP, "hipGetDevice failed");
   if (hipStreamCreate(&ctx->own_stream) != hipSuccess)
     return fail(TRIXI_HIP_EHIP, "hipStreamCreate failed");
   ctx->stream = ctx->own_stream;
