@@ -50,6 +50,7 @@ def _load():
         L.oracle_set_initial_condition.argtypes = [P, I, P, I, P]
         L.oracle_compute_coefficients.argtypes = [P, D, P]
         L.oracle_rhs.argtypes = [P, P, P, D]
+        L.oracle_rhs_ld.argtypes = [P, P, P, D]
         L.oracle_indicator_hg.argtypes = [P, P, P]
         L.oracle_max_dt.restype = D
         L.oracle_max_dt.argtypes = [P, P]
@@ -61,6 +62,12 @@ def _load():
         L.oracle_set_threads.argtypes = [I]
         L.oracle_get_threads.restype = I
         L.oracle_ck2n54_coefficients.argtypes = [P, P, P]
+        L.oracle_ln_mean.restype = D
+        L.oracle_ln_mean.argtypes = [D, D]
+        L.oracle_inv_ln_mean.restype = D
+        L.oracle_inv_ln_mean.argtypes = [D, D]
+        L.oracle_flux.argtypes = [I, I, D, P, P, I, P]
+        L.oracle_cons2entropy.argtypes = [I, D, P, P]
         _lib = L
     return _lib
 
@@ -149,6 +156,13 @@ class Oracle:
         self.L.oracle_rhs(self._c, _p(du), _p(u), float(t))
         return du
 
+    def rhs_extended(self, u, t=0.0):
+        """The RHS evaluated in 80-bit long double on the same FP64 inputs."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        du = np.empty_like(u)
+        self.L.oracle_rhs_ld(self._c, _p(du), _p(u), float(t))
+        return du
+
     def indicator_hg(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         a = np.empty(self.nelements)
@@ -193,3 +207,34 @@ class Oracle:
     @staticmethod
     def get_threads():
         return _load().oracle_get_threads()
+
+
+def ln_mean(x, y):
+    return _load().oracle_ln_mean(float(x), float(y))
+
+
+def inv_ln_mean(x, y):
+    return _load().oracle_inv_ln_mean(float(x), float(y))
+
+
+def flux(ndims, kind, ul, ur, orientation, gamma=1.4):
+    """kind: 'llf' | 'llf_naive' | 'ranocha' | 'central' | 'physical' (orientation 0-based)."""
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    ur = np.ascontiguousarray(ur if ur is not None else ul, dtype=np.float64)
+    f = np.empty(ndims + 2)
+    k = -1 if kind == "physical" else FLUX[kind]
+    _load().oracle_flux(ndims, k, gamma, _p(ul), _p(ur), int(orientation), _p(f))
+    return f
+
+
+def cons2entropy(ndims, u, gamma=1.4):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    w = np.empty(ndims + 2)
+    _load().oracle_cons2entropy(ndims, gamma, _p(u), _p(w))
+    return w
+
+
+def ck2n54():
+    a, b, c = np.empty(5), np.empty(5), np.empty(5)
+    _load().oracle_ck2n54_coefficients(_p(a), _p(b), _p(c))
+    return a, b, c

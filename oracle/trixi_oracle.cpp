@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 #include <omp.h>
 
 namespace {
@@ -236,45 +237,45 @@ struct Basis {
 // ---------------------------------------------------------------------------
 
 // ln_mean, math.jl:198-210
-inline double ln_mean(double x, double y) {
-    const double epsilon_f2 = 1.0e-4;
-    double f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y);
+template <typename R> inline R ln_mean(R x, R y) {
+    const R epsilon_f2 = R(1.0e-4);
+    R f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y);
     if (f2 < epsilon_f2)
-        return (x + y) / (2 + f2 * (2.0 / 3 + f2 * (2.0 / 5 + f2 * (2.0 / 7))));
+        return (x + y) / (2 + f2 * (R(2) / 3 + f2 * (R(2) / 5 + f2 * (R(2) / 7))));
     return (y - x) / std::log(y / x);
 }
 
 // inv_ln_mean, math.jl:238-250
-inline double inv_ln_mean(double x, double y) {
-    const double epsilon_f2 = 1.0e-4;
-    double f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y);
+template <typename R> inline R inv_ln_mean(R x, R y) {
+    const R epsilon_f2 = R(1.0e-4);
+    R f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y);
     if (f2 < epsilon_f2)
-        return (2 + f2 * (2.0 / 3 + f2 * (2.0 / 5 + f2 * (2.0 / 7)))) / (x + y);
+        return (2 + f2 * (R(2) / 3 + f2 * (R(2) / 5 + f2 * (R(2) / 7)))) / (x + y);
     return std::log(y / x) / (y - x);
 }
 
-template <int D> struct Euler {
+template <int D, typename R = double> struct Euler {
     static constexpr int V = D + 2;
-    double gamma, inv_gamma_minus_one;
-    explicit Euler(double g = 1.4) : gamma(g), inv_gamma_minus_one(1.0 / (g - 1)) {}
+    R gamma, inv_gamma_minus_one;
+    explicit Euler(double g = 1.4) : gamma(R(g)), inv_gamma_minus_one(R(1) / (R(g) - 1)) {}
 
     // cons2prim: euler_3d.jl:1783-1794, euler_2d.jl:2038-2046
-    void cons2prim(const double *u, double *w) const {
-        double rho = u[0];
-        double ke2 = 0;
+    void cons2prim(const R *u, R *w) const {
+        R rho = u[0];
+        R ke2 = 0;
         w[0] = rho;
         for (int d = 0; d < D; ++d) {
             w[1 + d] = u[1 + d] / rho;
             ke2 += u[1 + d] * w[1 + d];
         }
-        w[V - 1] = (gamma - 1) * (u[V - 1] - 0.5 * ke2);
+        w[V - 1] = (gamma - 1) * (u[V - 1] - R(0.5) * ke2);
     }
 
     // flux: euler_3d.jl:420-447, euler_2d.jl:415-434
-    void flux(const double *u, int o, double *f) const {
-        double w[V];
+    void flux(const R *u, int o, R *f) const {
+        R w[V];
         cons2prim(u, w);
-        double p = w[V - 1];
+        R p = w[V - 1];
         f[0] = u[1 + o];
         for (int d = 0; d < D; ++d) f[1 + d] = u[1 + o] * w[1 + d];
         f[1 + o] += p;
@@ -282,122 +283,121 @@ template <int D> struct Euler {
     }
 
     // flux_central: numerical_fluxes.jl:17-25
-    void flux_central(const double *ul, const double *ur, int o, double *f) const {
-        double fl[V], fr[V];
+    void flux_central(const R *ul, const R *ur, int o, R *f) const {
+        R fl[V], fr[V];
         flux(ul, o, fl);
         flux(ur, o, fr);
-        for (int v = 0; v < V; ++v) f[v] = 0.5 * (fl[v] + fr[v]);
+        for (int v = 0; v < V; ++v) f[v] = R(0.5) * (fl[v] + fr[v]);
     }
 
     // flux_ranocha: euler_3d.jl:746-793, euler_2d.jl:686-722
-    void flux_ranocha(const double *ul, const double *ur, int o, double *f) const {
-        double wl[V], wr[V];
+    void flux_ranocha(const R *ul, const R *ur, int o, R *f) const {
+        R wl[V], wr[V];
         cons2prim(ul, wl);
         cons2prim(ur, wr);
-        double rho_ll = wl[0], rho_rr = wr[0], p_ll = wl[V - 1], p_rr = wr[V - 1];
-        double rho_mean = ln_mean(rho_ll, rho_rr);
-        double inv_rho_p_mean = p_ll * p_rr * inv_ln_mean(rho_ll * p_rr, rho_rr * p_ll);
-        double vavg[D];
-        double velocity_square_avg_sum = 0;
+        R rho_ll = wl[0], rho_rr = wr[0], p_ll = wl[V - 1], p_rr = wr[V - 1];
+        R rho_mean = ln_mean(rho_ll, rho_rr);
+        R inv_rho_p_mean = p_ll * p_rr * inv_ln_mean(rho_ll * p_rr, rho_rr * p_ll);
+        R vavg[D];
+        R velocity_square_avg_sum = 0;
         for (int d = 0; d < D; ++d) {
-            vavg[d] = 0.5 * (wl[1 + d] + wr[1 + d]);
+            vavg[d] = R(0.5) * (wl[1 + d] + wr[1 + d]);
             velocity_square_avg_sum += wl[1 + d] * wr[1 + d];
         }
-        double p_avg = 0.5 * (p_ll + p_rr);
-        double velocity_square_avg = 0.5 * velocity_square_avg_sum;
-        double f1 = rho_mean * vavg[o];
+        R p_avg = R(0.5) * (p_ll + p_rr);
+        R velocity_square_avg = R(0.5) * velocity_square_avg_sum;
+        R f1 = rho_mean * vavg[o];
         f[0] = f1;
         for (int d = 0; d < D; ++d) f[1 + d] = f1 * vavg[d];
         f[1 + o] += p_avg;
         f[V - 1] = f1 * (velocity_square_avg + inv_rho_p_mean * inv_gamma_minus_one) +
-                   0.5 * (p_ll * wr[1 + o] + p_rr * wl[1 + o]);
+                   R(0.5) * (p_ll * wr[1 + o] + p_rr * wl[1 + o]);
     }
 
     // max_abs_speed_naive: euler_3d.jl:1112-1133, euler_2d.jl:1423-1440
-    double max_abs_speed_naive(const double *ul, const double *ur, int o) const {
-        double wl[V], wr[V];
+    R max_abs_speed_naive(const R *ul, const R *ur, int o) const {
+        R wl[V], wr[V];
         cons2prim(ul, wl);
         cons2prim(ur, wr);
-        double c_ll = std::sqrt(gamma * wl[V - 1] / wl[0]);
-        double c_rr = std::sqrt(gamma * wr[V - 1] / wr[0]);
+        R c_ll = std::sqrt(gamma * wl[V - 1] / wl[0]);
+        R c_rr = std::sqrt(gamma * wr[V - 1] / wr[0]);
         return std::max(std::fabs(wl[1 + o]), std::fabs(wr[1 + o])) + std::max(c_ll, c_rr);
     }
 
     // max_abs_speed: euler_3d.jl:1156-1177, euler_2d.jl:1464-1482
-    double max_abs_speed(const double *ul, const double *ur, int o) const {
-        double wl[V], wr[V];
+    R max_abs_speed(const R *ul, const R *ur, int o) const {
+        R wl[V], wr[V];
         cons2prim(ul, wl);
         cons2prim(ur, wr);
-        double c_ll = std::sqrt(gamma * wl[V - 1] / wl[0]);
-        double c_rr = std::sqrt(gamma * wr[V - 1] / wr[0]);
+        R c_ll = std::sqrt(gamma * wl[V - 1] / wl[0]);
+        R c_rr = std::sqrt(gamma * wr[V - 1] / wr[0]);
         return std::max(std::fabs(wl[1 + o]) + c_ll, std::fabs(wr[1 + o]) + c_rr);
     }
 
     // FluxLaxFriedrichs = FluxPlusDissipation(flux_central, DissipationLocalLaxFriedrichs)
     // numerical_fluxes.jl:37-45, 166-178, 229-253
-    void flux_llf(const double *ul, const double *ur, int o, bool naive, double *f) const {
-        double fc[V];
+    void flux_llf(const R *ul, const R *ur, int o, bool naive, R *f) const {
+        R fc[V];
         flux_central(ul, ur, o, fc);
-        double lambda = naive ? max_abs_speed_naive(ul, ur, o) : max_abs_speed(ul, ur, o);
-        for (int v = 0; v < V; ++v) f[v] = fc[v] + (-0.5 * lambda * (ur[v] - ul[v]));
+        R lambda = naive ? max_abs_speed_naive(ul, ur, o) : max_abs_speed(ul, ur, o);
+        for (int v = 0; v < V; ++v) f[v] = fc[v] + (-R(0.5) * lambda * (ur[v] - ul[v]));
     }
 
     // max_abs_speeds: euler_3d.jl:1770-1775, euler_2d.jl:2025-2030
-    void max_abs_speeds(const double *u, double *lam) const {
-        double w[V];
+    void max_abs_speeds(const R *u, R *lam) const {
+        R w[V];
         cons2prim(u, w);
-        double c = std::sqrt(gamma * w[V - 1] / w[0]);
+        R c = std::sqrt(gamma * w[V - 1] / w[0]);
         for (int d = 0; d < D; ++d) lam[d] = std::fabs(w[1 + d]) + c;
     }
 
     // density_pressure: euler_3d.jl:1951-1956, euler_2d.jl:2205-2210
-    double density_pressure(const double *u) const {
-        double m2 = 0;
+    R density_pressure(const R *u) const {
+        R m2 = 0;
         for (int d = 0; d < D; ++d) m2 += u[1 + d] * u[1 + d];
-        return (gamma - 1) * (u[0] * u[V - 1] - 0.5 * m2);
+        return (gamma - 1) * (u[0] * u[V - 1] - R(0.5) * m2);
     }
 
     // entropy_math: euler_3d.jl:1970-1976 (S = -rho s / (gamma-1), s = log p - gamma log rho)
-    double entropy_math(const double *u) const {
-        double m2 = 0;
+    R entropy_math(const R *u) const {
+        R m2 = 0;
         for (int d = 0; d < D; ++d) m2 += u[1 + d] * u[1 + d];
-        double p = (gamma - 1) * (u[V - 1] - 0.5 * m2 / u[0]);
-        double s = std::log(p) - gamma * std::log(u[0]);
+        R p = (gamma - 1) * (u[V - 1] - R(0.5) * m2 / u[0]);
+        R s = std::log(p) - gamma * std::log(u[0]);
         return -s * u[0] * inv_gamma_minus_one;
     }
 
     // cons2entropy: euler_3d.jl (w = ((gamma - s)/(gamma-1) - 0.5 rho/p |v|^2, rho/p v, -rho/p))
-    void cons2entropy(const double *u, double *wout) const {
-        double w[V];
+    void cons2entropy(const R *u, R *wout) const {
+        R w[V];
         cons2prim(u, w);
-        double v2 = 0;
+        R v2 = 0;
         for (int d = 0; d < D; ++d) v2 += w[1 + d] * w[1 + d];
-        double p = w[V - 1];
-        double s = std::log(p) - gamma * std::log(w[0]);
-        double rho_p = w[0] / p;
-        wout[0] = (gamma - s) * inv_gamma_minus_one - 0.5 * rho_p * v2;
+        R p = w[V - 1];
+        R s = std::log(p) - gamma * std::log(w[0]);
+        R rho_p = w[0] / p;
+        wout[0] = (gamma - s) * inv_gamma_minus_one - R(0.5) * rho_p * v2;
         for (int d = 0; d < D; ++d) wout[1 + d] = rho_p * w[1 + d];
         wout[V - 1] = -rho_p;
     }
 
     // prim2cons: euler_2d.jl:2120-2127 (and the 3D twin)
-    void prim2cons(const double *w, double *u) const {
+    void prim2cons(const R *w, R *u) const {
         u[0] = w[0];
-        double ke2 = 0;
+        R ke2 = 0;
         for (int d = 0; d < D; ++d) {
             u[1 + d] = w[0] * w[1 + d];
             ke2 += u[1 + d] * w[1 + d];
         }
-        u[V - 1] = w[V - 1] * inv_gamma_minus_one + 0.5 * ke2;
+        u[V - 1] = w[V - 1] * inv_gamma_minus_one + R(0.5) * ke2;
     }
 };
 
 // Numerical-flux selector shared by the volume (two-point), interface and FV paths.
 enum FluxKind { FLUX_LLF = 0, FLUX_LLF_NAIVE = 1, FLUX_RANOCHA = 2, FLUX_CENTRAL = 3 };
 
-template <int D>
-inline void num_flux(const Euler<D> &eq, int kind, const double *ul, const double *ur, int o,
-                     double *f) {
+template <int D, typename R>
+inline void num_flux(const Euler<D, R> &eq, int kind, const R *ul, const R *ur, int o, R *f) {
     switch (kind) {
         case FLUX_LLF: eq.flux_llf(ul, ur, o, false, f); break;
         case FLUX_LLF_NAIVE: eq.flux_llf(ul, ur, o, true, f); break;
@@ -591,22 +591,22 @@ void initial_condition(const oracle_ctx &c, const Euler<D> &eq, const double *x,
 }
 
 // source_terms_convergence_test: euler_3d.jl:127-153, euler_2d.jl:120-146
-template <int D> void source_convergence(const Euler<D> &eq, const double *x, double t, double *s) {
-    double cst = 2, A = 0.1, L = 2, f = 1.0 / L, om = 2 * M_PI * f, g = eq.gamma;
-    double xs = 0;
-    for (int d = 0; d < D; ++d) xs += x[d];
-    double si = std::sin(om * (xs - t)), co = std::cos(om * (xs - t));
-    double rho = cst + A * si;
-    double rho_x = om * A * co;
+template <int D, typename R> void source_convergence(const Euler<D, R> &eq, const double *x, double t, R *s) {
+    const R cst = 2, A = R(0.1), L = 2, f = R(1) / L, om = 2 * R(M_PI) * f, g = eq.gamma;
+    R xs = 0;
+    for (int d = 0; d < D; ++d) xs += R(x[d]);
+    R si = std::sin(om * (xs - R(t))), co = std::cos(om * (xs - R(t)));
+    R rho = cst + A * si;
+    R rho_x = om * A * co;
     if (D == 3) {
-        double tmp = (2 * rho - 1.5) * (g - 1);
+        R tmp = (2 * rho - R(1.5)) * (g - 1);
         s[0] = 2 * rho_x;
         s[1] = rho_x * (2 + tmp);
         s[2] = s[1];
         s[3] = s[1];
         s[4] = rho_x * (4 * rho + 3 * tmp);
     } else {
-        double tmp = (2 * rho - 1) * (g - 1);
+        R tmp = (2 * rho - 1) * (g - 1);
         s[0] = rho_x;
         s[1] = rho_x * (1 + tmp);
         s[2] = s[1];
@@ -649,22 +649,22 @@ template <int D> inline int face_to_node(int n, int o, int side_index, int fnode
 // ---------------------------------------------------------------------------
 
 // weak_form_kernel!: dg_3d.jl:133-164, dg_2d.jl:195-221
-template <int D>
-void weak_form_kernel(const oracle_ctx &c, const Euler<D> &eq, double *du, const double *u,
-                      int e, double alpha) {
+template <int D, typename R>
+void weak_form_kernel(const oracle_ctx &c, const Euler<D, R> &eq, R *du, const R *u,
+                      int e, R alpha) {
     const int V = D + 2, n = c.n, NN = c.NN;
     const double *Dhat = c.basis.Dhat.data();
     for (int node = 0; node < NN; ++node) {
         int idx[3] = {node % n, (node / n) % n, node / (n * n)};
-        const double *un = &u[V * (node + NN * e)];
+        const R *un = &u[V * (node + NN * e)];
         for (int o = 0; o < D; ++o) {
-            double f[V];
+            R f[V];
             eq.flux(un, o, f);
             for (int ii = 0; ii < n; ++ii) {
                 int jdx[3] = {idx[0], idx[1], idx[2]};
                 jdx[o] = ii;
                 int tnode = node_index<D>(n, jdx);
-                double fac = alpha * Dhat[ii * n + idx[o]];
+                R fac = alpha * R(Dhat[ii * n + idx[o]]);
                 for (int v = 0; v < V; ++v) du[V * (tnode + NN * e) + v] += fac * f[v];
             }
         }
@@ -672,24 +672,24 @@ void weak_form_kernel(const oracle_ctx &c, const Euler<D> &eq, double *du, const
 }
 
 // flux_differencing_kernel!: dg_3d.jl:166-214, dg_2d.jl:223-259
-template <int D>
-void flux_differencing_kernel(const oracle_ctx &c, const Euler<D> &eq, int fluxkind, double *du,
-                              const double *u, int e, double alpha) {
+template <int D, typename R>
+void flux_differencing_kernel(const oracle_ctx &c, const Euler<D, R> &eq, int fluxkind, R *du,
+                              const R *u, int e, R alpha) {
     const int V = D + 2, n = c.n, NN = c.NN;
     const double *Ds = c.basis.Dsplit.data();
     for (int node = 0; node < NN; ++node) {
         int idx[3] = {node % n, (node / n) % n, node / (n * n)};
-        const double *un = &u[V * (node + NN * e)];
+        const R *un = &u[V * (node + NN * e)];
         for (int o = 0; o < D; ++o) {
             int i = idx[o];
             for (int ii = i + 1; ii < n; ++ii) {
                 int jdx[3] = {idx[0], idx[1], idx[2]};
                 jdx[o] = ii;
                 int onode = node_index<D>(n, jdx);
-                const double *uo = &u[V * (onode + NN * e)];
-                double f[V];
+                const R *uo = &u[V * (onode + NN * e)];
+                R f[V];
                 num_flux<D>(eq, fluxkind, un, uo, o, f);
-                double fi = alpha * Ds[i * n + ii], fii = alpha * Ds[ii * n + i];
+                R fi = alpha * R(Ds[i * n + ii]), fii = alpha * R(Ds[ii * n + i]);
                 for (int v = 0; v < V; ++v) {
                     du[V * (node + NN * e) + v] += fi * f[v];
                     du[V * (onode + NN * e) + v] += fii * f[v];
@@ -702,21 +702,21 @@ void flux_differencing_kernel(const oracle_ctx &c, const Euler<D> &eq, int fluxk
 // fv_kernel! + calcflux_fv!: dg_3d.jl:268-306, 359-389; dg_2d.jl:407-464.
 // Subcell fluxes at element faces (index 0 and n) are zero (create_f_threaded,
 // dg_3d.jl:11-54).
-template <int D>
-void fv_kernel(const oracle_ctx &c, const Euler<D> &eq, int fluxkind, double *du,
-               const double *u, int e, double alpha) {
+template <int D, typename R>
+void fv_kernel(const oracle_ctx &c, const Euler<D, R> &eq, int fluxkind, R *du,
+               const R *u, int e, R alpha) {
     const int V = D + 2, n = c.n, NN = c.NN;
     const double *iw = c.basis.inv_weights.data();
     // fstar[o][v + V*(node index with index_o in 0..n)] -- store per orientation
     // as [n+1 along o] x [n^(D-1)] x V
-    std::vector<double> fstar(D * (n + 1) * ipow(n, D - 1) * V, 0.0);
+    std::vector<R> fstar(D * (n + 1) * ipow(n, D - 1) * V, 0.0);
     auto fidx = [&](int o, int io, int rest) { return ((o * (n + 1) + io) * ipow(n, D - 1) + rest) * V; };
     for (int o = 0; o < D; ++o) {
         for (int rest = 0; rest < ipow(n, D - 1); ++rest) {
             for (int io = 1; io < n; ++io) {
                 int nl = face_to_node<D>(n, o, io - 1, rest);
                 int nr = face_to_node<D>(n, o, io, rest);
-                double f[V];
+                R f[V];
                 num_flux<D>(eq, fluxkind, &u[V * (nl + NN * e)], &u[V * (nr + NN * e)], o, f);
                 for (int v = 0; v < V; ++v) fstar[fidx(o, io, rest) + v] = f[v];
             }
@@ -725,7 +725,7 @@ void fv_kernel(const oracle_ctx &c, const Euler<D> &eq, int fluxkind, double *du
     for (int node = 0; node < NN; ++node) {
         int idx[3] = {node % n, (node / n) % n, node / (n * n)};
         for (int v = 0; v < V; ++v) {
-            double acc = 0;
+            R acc = 0;
             for (int o = 0; o < D; ++o) {
                 int rest = 0, s = 1;
                 for (int d = 0; d < D; ++d)
@@ -734,7 +734,7 @@ void fv_kernel(const oracle_ctx &c, const Euler<D> &eq, int fluxkind, double *du
                         s *= n;
                     }
                 int io = idx[o];
-                acc += iw[io] * (fstar[fidx(o, io + 1, rest) + v] - fstar[fidx(o, io, rest) + v]);
+                acc += R(iw[io]) * (fstar[fidx(o, io + 1, rest) + v] - fstar[fidx(o, io, rest) + v]);
             }
             du[V * (node + NN * e) + v] += alpha * acc;
         }
@@ -831,29 +831,42 @@ void hg_indicator(oracle_ctx &c, const Euler<D> &eq, const double *u) {
 // ---------------------------------------------------------------------------
 // rhs_hyperbolic!: dg_2d.jl:113-186 (shared by 2D/3D TreeMesh)
 // ---------------------------------------------------------------------------
-template <int D> void rhs(oracle_ctx &c, double *du, const double *u, double t) {
+template <int D, typename R> void rhs(oracle_ctx &c, R *du, const R *u, double t) {
     const int V = D + 2, n = c.n, NN = c.NN, nf = c.nf, E = c.E, I = c.I;
-    Euler<D> eq(c.cfg.gamma);
+    Euler<D, R> eq(c.cfg.gamma);
+    std::vector<R> sfv_local;
+    R *sfv;
+    if constexpr (std::is_same<R, double>::value) sfv = c.sfv.data();
+    else {
+        sfv_local.assign(c.sfv.size(), R(0));
+        sfv = sfv_local.data();
+    }
     const size_t total = (size_t)V * NN * E;
     // set_zero!, solvers.jl:8-29
-    std::memset(du, 0, total * sizeof(double));
+    for (size_t k = 0; k < total; ++k) du[k] = R(0);
 
     // calc_volume_integral!, calc_volume_integral.jl:180-272
     const int vi = c.cfg.volume_integral;
-    if (vi == VI_SHOCK_CAPTURING_HG) hg_indicator<D>(c, eq, u);
+    if (vi == VI_SHOCK_CAPTURING_HG) {
+        if constexpr (std::is_same<R, double>::value) hg_indicator<D>(c, Euler<D>(c.cfg.gamma), u);
+        else {
+            std::vector<double> ud(u, u + total);
+            hg_indicator<D>(c, Euler<D>(c.cfg.gamma), ud.data());
+        }
+    }
     const double atol = 1.8189894035458565e-12;  // max(100 eps, eps^0.75) for Float64
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < E; ++e) {
-        if (vi == VI_WEAK) weak_form_kernel<D>(c, eq, du, u, e, 1.0);
-        else if (vi == VI_FLUX_DIFFERENCING) flux_differencing_kernel<D>(c, eq, c.cfg.volume_flux, du, u, e, 1.0);
-        else if (vi == VI_PURE_FV) fv_kernel<D>(c, eq, c.cfg.fv_flux, du, u, e, 1.0);
+        if (vi == VI_WEAK) weak_form_kernel<D, R>(c, eq, du, u, e, 1.0);
+        else if (vi == VI_FLUX_DIFFERENCING) flux_differencing_kernel<D, R>(c, eq, c.cfg.volume_flux, du, u, e, 1.0);
+        else if (vi == VI_PURE_FV) fv_kernel<D, R>(c, eq, c.cfg.fv_flux, du, u, e, 1.0);
         else {
-            double a = c.alpha[e];
-            bool dg_only = std::fabs(a) <= atol;
-            if (dg_only) flux_differencing_kernel<D>(c, eq, c.cfg.volume_flux, du, u, e, 1.0);
+            R a = R(c.alpha[e]);
+            bool dg_only = std::fabs(c.alpha[e]) <= atol;
+            if (dg_only) flux_differencing_kernel<D, R>(c, eq, c.cfg.volume_flux, du, u, e, 1.0);
             else {
-                flux_differencing_kernel<D>(c, eq, c.cfg.volume_flux, du, u, e, 1 - a);
-                fv_kernel<D>(c, eq, c.cfg.fv_flux, du, u, e, a);
+                flux_differencing_kernel<D, R>(c, eq, c.cfg.volume_flux, du, u, e, 1 - a);
+                fv_kernel<D, R>(c, eq, c.cfg.fv_flux, du, u, e, a);
             }
         }
     }
@@ -867,37 +880,37 @@ template <int D> void rhs(oracle_ctx &c, double *du, const double *u, double t) 
         for (int fn = 0; fn < nf; ++fn) {
             int nl = face_to_node<D>(n, o, n - 1, fn);
             int nr = face_to_node<D>(n, o, 0, fn);
-            double f[V];
+            R f[V];
             num_flux<D>(eq, c.cfg.surface_flux, &u[V * (nl + NN * left)], &u[V * (nr + NN * right)], o, f);
             for (int v = 0; v < V; ++v) {
-                c.sfv[V * (fn + nf * (left_dir + 2 * D * left)) + v] = f[v];
-                c.sfv[V * (fn + nf * (right_dir + 2 * D * right)) + v] = f[v];
+                sfv[V * (fn + nf * (left_dir + 2 * D * left)) + v] = f[v];
+                sfv[V * (fn + nf * (right_dir + 2 * D * right)) + v] = f[v];
             }
         }
     }
 
     // calc_surface_integral!: dg_3d.jl:1337-1394, dg_2d.jl:1245-1287
-    const double factor = c.basis.inv_weights[0];
+    const R factor = R(c.basis.inv_weights[0]);
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < E; ++e) {
         for (int o = 0; o < D; ++o)
             for (int fn = 0; fn < nf; ++fn) {
                 int nm = face_to_node<D>(n, o, 0, fn), np = face_to_node<D>(n, o, n - 1, fn);
                 for (int v = 0; v < V; ++v) {
-                    du[V * (nm + NN * e) + v] = du[V * (nm + NN * e) + v] - c.sfv[V * (fn + nf * (2 * o + 2 * D * e)) + v] * factor;
-                    du[V * (np + NN * e) + v] = du[V * (np + NN * e) + v] + c.sfv[V * (fn + nf * (2 * o + 1 + 2 * D * e)) + v] * factor;
+                    du[V * (nm + NN * e) + v] = du[V * (nm + NN * e) + v] - sfv[V * (fn + nf * (2 * o + 2 * D * e)) + v] * factor;
+                    du[V * (np + NN * e) + v] = du[V * (np + NN * e) + v] + sfv[V * (fn + nf * (2 * o + 1 + 2 * D * e)) + v] * factor;
                 }
             }
         // apply_jacobian!: dg_3d.jl:1396-1415
-        double fac = -c.inverse_jacobian[e];
+        R fac = -R(c.inverse_jacobian[e]);
         for (int k = 0; k < V * NN; ++k) du[(size_t)V * NN * e + k] *= fac;
         // calc_sources!: dg_3d.jl:1417-1438
         if (c.cfg.source_terms == SRC_CONVERGENCE_TEST) {
             for (int node = 0; node < NN; ++node) {
                 double x[3];
                 for (int d = 0; d < D; ++d) x[d] = c.node_coords[d + D * (node + NN * e)];
-                double s[V];
-                source_convergence<D>(eq, x, t, s);
+                R s[V];
+                source_convergence<D, R>(eq, x, t, s);
                 for (int v = 0; v < V; ++v) du[V * (node + NN * e) + v] += s[v];
             }
         }
@@ -1123,8 +1136,19 @@ void oracle_compute_coefficients(const oracle_ctx *c, double t, double *u) {
 }
 
 void oracle_rhs(oracle_ctx *c, double *du, const double *u, double t) {
-    if (c->D == 2) rhs<2>(*c, du, u, t);
-    else rhs<3>(*c, du, u, t);
+    if (c->D == 2) rhs<2, double>(*c, du, u, t);
+    else rhs<3, double>(*c, du, u, t);
+}
+
+// The same operator evaluated in 80-bit extended precision (long double) on
+// the given FP64 inputs; the result is rounded to FP64. Used to measure the
+// round-off of FP64 evaluations (cancellation in du grows as the mesh is refined).
+void oracle_rhs_ld(oracle_ctx *c, double *du, const double *u, double t) {
+    const size_t total = (size_t)c->V * c->NN * c->E;
+    std::vector<long double> ul(u, u + total), dul(total);
+    if (c->D == 2) rhs<2, long double>(*c, dul.data(), ul.data(), t);
+    else rhs<3, long double>(*c, dul.data(), ul.data(), t);
+    for (size_t k = 0; k < total; ++k) du[k] = (double)dul[k];
 }
 
 // Indicator only (alpha[E]); runs the HG indicator regardless of volume integral.
@@ -1165,6 +1189,29 @@ void oracle_integrals(const oracle_ctx *c, const double *u, double *cons_int, do
     double tv = std::pow(c->domain_length, D);
     for (int v = 0; v < V; ++v) cons_int[v] = acc[v] / tv;
     *entropy_int = S / tv;
+}
+
+// Pointwise functions (for unit tests of the restatement itself).
+double oracle_ln_mean(double x, double y) { return ln_mean(x, y); }
+double oracle_inv_ln_mean(double x, double y) { return inv_ln_mean(x, y); }
+
+// Two-point numerical flux `kind` (0 LLF, 1 LLF naive, 2 ranocha, 3 central) or,
+// for kind < 0, the physical flux of ul, in orientation o (0-based).
+void oracle_flux(int ndims, int kind, double gamma, const double *ul, const double *ur, int o, double *f) {
+    if (ndims == 2) {
+        Euler<2> eq(gamma);
+        if (kind < 0) eq.flux(ul, o, f);
+        else num_flux<2>(eq, kind, ul, ur, o, f);
+    } else {
+        Euler<3> eq(gamma);
+        if (kind < 0) eq.flux(ul, o, f);
+        else num_flux<3>(eq, kind, ul, ur, o, f);
+    }
+}
+
+void oracle_cons2entropy(int ndims, double gamma, const double *u, double *w) {
+    if (ndims == 2) Euler<2>(gamma).cons2entropy(u, w);
+    else Euler<3>(gamma).cons2entropy(u, w);
 }
 
 // CarpenterKennedy2N54 coefficients: methods_2N.jl:24-48

@@ -1,0 +1,131 @@
+"""HIP rhs_hyperbolic! against the FP64 oracle on the same inputs.
+Bar: FP64 within 1e-12 (max-norm, relative to max|du|); FP32 bulk within the
+per-case tolerance written below (FP32 fluxes; measured 1e-7..7e-5)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from gpu_helpers import build, fp32_abs_bound, oracle_for, relerr
+
+torch = pytest.importorskip("torch")
+
+MMS3 = dict(ndims=3, polydeg=3, level=2, cmin=(0, 0, 0), cmax=(2, 2, 2), ic="convergence_test", source="convergence_test")
+BLAST = dict(ndims=3, polydeg=3, level=2, cmin=(-2, -2, -2), cmax=(2, 2, 2), ic="weak_blast_wave")
+CASES = {
+    "3d_p3_fd_ranocha_llf_mms": (dict(MMS3, volume_integral="fd", surface_flux="llf"), 1e-4),
+    "3d_p3_fd_ranocha_llfnaive_mms": (dict(MMS3, volume_integral="fd", surface_flux="llf_naive"), 1e-4),
+    "3d_p3_weak_mms": (dict(MMS3, volume_integral="weak", surface_flux="llf_naive"), 1e-4),
+    "3d_p3_fd_central_mms": (dict(MMS3, volume_integral="fd", volume_flux="central", surface_flux="llf_naive"), 1e-4),
+    "3d_p3_ec_blast": (dict(BLAST, volume_integral="fd", surface_flux="ranocha"), 1e-5),
+    "3d_p3_hg_blast": (dict(BLAST, volume_integral="hg", surface_flux="ranocha", fv_flux="ranocha"), 1e-5),
+    "3d_p4_fd_fourier": (dict(ndims=3, polydeg=4, level=2, cmin=(-1,) * 3, cmax=(1,) * 3, ic="fourier_modes",
+                              ic_params=(0,)), 1e-5),
+    "3d_p2_fd_fourier": (dict(ndims=3, polydeg=2, level=2, cmin=(-1,) * 3, cmax=(1,) * 3, ic="fourier_modes",
+                              ic_params=(1,)), 1e-5),
+    "2d_p3_hg_khi": (dict(ndims=2, polydeg=3, level=4, cmin=(-1, -1), cmax=(1, 1), ic="kelvin_helmholtz",
+                          ic_params=(1e-3, 0), volume_integral="hg", surface_flux="llf_naive", fv_flux="llf_naive",
+                          alpha_max=0.002, alpha_min=1e-4), 2e-4),
+    "2d_p3_fd_vortex_p1": (dict(ndims=2, polydeg=3, level=4, cmin=(-10, -10), cmax=(10, 10), ic="isentropic_vortex",
+                                ic_params=(1.0,)), 1e-4),
+    "2d_p3_weak_vortex": (dict(ndims=2, polydeg=3, level=4, cmin=(-10, -10), cmax=(10, 10), ic="isentropic_vortex",
+                               ic_params=(25.0,), volume_integral="weak", surface_flux="llf_naive"), 2e-4),
+    "2d_p4_fd_mms": (dict(ndims=2, polydeg=4, level=3, cmin=(0, 0), cmax=(2, 2), ic="convergence_test",
+                          source="convergence_test"), 2e-4),
+    "2d_p2_fd_mms": (dict(ndims=2, polydeg=2, level=3, cmin=(0, 0), cmax=(2, 2), ic="convergence_test",
+                          source="convergence_test"), 2e-4),
+    "2d_p3_pure_fv": (dict(ndims=2, polydeg=3, level=3, cmin=(0, 0), cmax=(2, 2), ic="convergence_test",
+                           source="convergence_test", volume_integral="fv"), 1e-4),
+}
+
+
+def perturbed(u, seed, eps=0.02):
+    rng = np.random.default_rng(seed)
+    return u * (1 + eps * rng.standard_normal(u.shape))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_rhs_parity_fp64_and_fp32(name):
+    import trixi_hip as th
+    spec, tol32 = CASES[name]
+    o = oracle_for(spec)
+    for variant in ("ic", "perturbed"):
+        semi64, ic = build(spec, "fp64")
+        u = semi64.compute_coefficients(0.0)
+        if variant == "perturbed" and spec.get("ic") != "constant":
+            u = perturbed(u, 11)
+        t = 0.37
+        du_ref = o.rhs(u, t)
+        # host pointers
+        du = np.empty_like(u)
+        th.rhs_hyperbolic_(du, u, semi64, t)
+        assert relerr(du, du_ref) < 1e-12, (name, variant, "fp64 host")
+        # device pointers (torch tensors on torch's current stream)
+        ud = torch.from_numpy(u).cuda()
+        dud = torch.empty_like(ud)
+        th.rhs_hyperbolic_(dud, ud, semi64, t)
+        torch.cuda.synchronize()
+        assert relerr(dud.cpu().numpy(), du_ref) < 1e-12, (name, variant, "fp64 device")
+        semi64.close()
+        semi32, _ = build(spec, "fp32")
+        du32 = np.empty_like(u)
+        th.rhs_hyperbolic_(du32, u, semi32, t)
+        assert relerr(du32, du_ref) < tol32, (name, variant, "fp32", relerr(du32, du_ref))
+        semi32.close()
+
+
+@pytest.mark.gpu
+def test_native_fp32_entry_point_and_determinism():
+    """trixi_hip_rhs_native on float32 device arrays (the benchmarked path) equals
+    trixi_hip_rhs in FP32 mode bitwise, and repeated calls are bitwise identical."""
+    import trixi_hip as th
+    spec = CASES["3d_p3_fd_ranocha_llf_mms"][0]
+    semi, _ = build(spec, "fp32")
+    u = perturbed(semi.compute_coefficients(0.0), 3)
+    du_a = np.empty_like(u)
+    th.rhs_hyperbolic_(du_a, u, semi, 0.2)
+    u32 = torch.from_numpy(u).cuda().float()
+    outs = []
+    for _ in range(3):
+        d32 = torch.empty_like(u32)
+        rc = semi._lib.trixi_hip_rhs_native(semi.ctx, ctypes.c_void_p(d32.data_ptr()), ctypes.c_void_p(u32.data_ptr()),
+                                            0.2)
+        assert rc == 0
+        semi._lib.trixi_hip_synchronize(semi.ctx)
+        outs.append(d32.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+    assert np.array_equal(outs[0].astype(np.float64), du_a)
+    semi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D", [2, 3])
+def test_free_stream_preservation(D):
+    import trixi_hip as th
+    spec = dict(ndims=D, polydeg=3, level=3, cmin=(-1,) * D, cmax=(1,) * D, ic="constant", surface_flux="llf")
+    for prec in ("fp64", "fp32"):
+        semi, _ = build(spec, prec)
+        u = semi.compute_coefficients(0.0)
+        du = np.empty_like(u)
+        th.rhs_hyperbolic_(du, u, semi, 0.0)
+        # round-off level: the FP32 bound scales as eps32 J^-1 max|f|; FP64 likewise with eps64
+        tol = 2e-13 if prec == "fp64" else fp32_abs_bound(semi, u)
+        assert np.max(np.abs(du)) < tol, (prec, np.max(np.abs(du)), tol)
+        semi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["3d_p3_hg_blast", "2d_p3_hg_khi"])
+def test_hg_indicator_matches_oracle(name):
+    spec = CASES[name][0]
+    o = oracle_for(spec)
+    semi, _ = build(spec, "fp64")
+    u = perturbed(semi.compute_coefficients(0.0), 5, 0.01)
+    lib, ctx = semi._lib, semi.ctx
+    lib.trixi_hip_upload_u(ctx, u.ctypes.data_as(ctypes.c_void_p))
+    a = np.empty(semi.nelements())
+    assert lib.trixi_hip_indicator(ctx, 0, a.ctypes.data_as(ctypes.c_void_p)) == 0
+    a_ref = o.indicator_hg(u)
+    np.testing.assert_allclose(a, a_ref, rtol=1e-12, atol=1e-15)
+    semi.close()

@@ -1,0 +1,130 @@
+"""Device-resident time integration and reductions: max_dt, error norms,
+integrals, the 2N RK stage and the hipGraph-captured step, and Trixi.jl's
+known-answer trajectories run entirely on the GPU."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from gpu_helpers import build, oracle_for, relerr
+
+torch = pytest.importorskip("torch")
+
+MMS3 = dict(ndims=3, polydeg=3, level=2, cmin=(0, 0, 0), cmax=(2, 2, 2), ic="convergence_test",
+            source="convergence_test", surface_flux="llf")
+
+
+def upload(semi, u):
+    assert semi._lib.trixi_hip_upload_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p)) == 0
+
+
+def download(semi):
+    u = np.empty(semi.nvalues())
+    assert semi._lib.trixi_hip_download_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p)) == 0
+    return u
+
+
+@pytest.mark.gpu
+def test_reductions_match_oracle():
+    import trixi_hip as th
+    for spec in (MMS3, dict(ndims=2, polydeg=3, level=4, cmin=(-10, -10), cmax=(10, 10), ic="isentropic_vortex",
+                            ic_params=(25.0,))):
+        o = oracle_for(spec)
+        o.set_initial_condition(spec["ic"], spec.get("ic_params", ()))
+        semi, ic = build(spec, "fp64")
+        u = semi.compute_coefficients(0.0)
+        u = u * (1 + 0.01 * np.random.default_rng(1).standard_normal(u.shape))
+        upload(semi, u)
+        dt = th.StepsizeCallback(cfl=0.7).calculate_dt(semi)
+        assert dt == pytest.approx(o.max_dt(u, 0.7), rel=1e-14)
+        ac = th.AnalysisCallback(semi)
+        rec = ac.analyze(0.25, 0)
+        l2, li = o.error_norms(u, 0.25)
+        np.testing.assert_allclose(rec["l2"], l2, rtol=1e-11)
+        np.testing.assert_allclose(rec["linf"], li, rtol=1e-12)
+        cons, ent = ac.integrals()
+        cons_ref, ent_ref = o.integrals(u)
+        np.testing.assert_allclose(cons, cons_ref, rtol=1e-12)
+        assert ent == pytest.approx(ent_ref, rel=1e-12)
+        semi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 5e-6)])
+def test_rk_step_graph_eager_and_oracle(precision, tol):
+    """One CK2N54 step: graph replay == eager launches (bitwise), and both equal
+    the oracle's step (FP64 to 1e-12; FP32 bulk within tol of the state)."""
+    import trixi_hip as th
+    o = oracle_for(MMS3)
+    semi, _ = build(MMS3, precision)
+    lib, ctx = semi._lib, semi.ctx
+    alg = th.CarpenterKennedy2N54()
+    na = 5
+    a = (ctypes.c_double * na)(*alg.a)
+    b = (ctypes.c_double * na)(*alg.b)
+    c = (ctypes.c_double * na)(*alg.c)
+    u0 = semi.compute_coefficients(0.0)
+    t, dt = 0.1, 2e-3
+    results = {}
+    for graph in (0, 1):
+        upload(semi, u0)
+        for rep in range(3):  # replay the graph several times with host work between
+            assert lib.trixi_hip_rk_step(ctx, a, b, c, na, t + rep * dt, dt, graph) == 0
+            assert lib.trixi_hip_synchronize(ctx) == 0
+            results[(graph, rep)] = download(semi)
+    for rep in range(3):
+        assert np.array_equal(results[(0, rep)], results[(1, rep)])
+    u_ref = o.rk_step(u0.copy(), t, dt)
+    assert relerr(results[(1, 0)], u_ref) < tol
+    semi.close()
+
+
+def run_kta(case, precision):
+    import trixi_hip as th
+    c = golden("kta.json")[case]
+    spec = dict(c["oracle"], ic=c["ic"], ic_params=tuple(c["ic_params"]))
+    semi, _ = build(spec, precision)
+    ode = th.semidiscretize(semi, tuple(c["tspan"]))
+    ac = th.AnalysisCallback(semi, interval=0)
+    sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=c["cfl"]), ac),
+                   maxiters=c.get("maxiters"))
+    semi.close()
+    return c, np.array(sol.analysis[-1]["l2"]), np.array(sol.analysis[-1]["linf"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d",
+                                  "khi_2d", "vortex_2d", "vortex_2d_constant_one_step"])
+def test_kta_trajectory_on_gpu_fp64(case):
+    """The whole run (RK stages, dt, analysis) on the device reproduces the
+    reference's stored L2/Linf errors."""
+    c, l2, linf = run_kta(case, "fp64")
+    if c.get("roundoff"):
+        assert np.all(l2 < 1e-13) and np.all(linf < 1e-12)
+    else:
+        np.testing.assert_allclose(l2, c["l2"], rtol=1e-9)
+        np.testing.assert_allclose(linf, c["linf"], rtol=1e-9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["mms_3d_weak", "ec_3d", "khi_2d"])
+def test_kta_trajectory_on_gpu_fp32(case):
+    c, l2, linf = run_kta(case, "fp32")
+    np.testing.assert_allclose(l2, c["l2"], rtol=2e-4)
+    np.testing.assert_allclose(linf, c["linf"], rtol=2e-3)
+
+
+@pytest.mark.gpu
+def test_nan_state_stops_the_run():
+    """A non-physical state (negative pressure) gives NaN wave speeds -> NaN dt ->
+    the integrator raises, as step! does on isnan(dt) (methods_2N.jl:138-140)."""
+    import trixi_hip as th
+    semi, _ = build(MMS3, "fp64")
+    u = semi.compute_coefficients(0.0)
+    u[4] = -10.0  # rho e of node 0 of element 0: pressure < 0
+    ode = th.semidiscretize(semi, (0.0, 1.0))
+    ode.u0 = u
+    with pytest.raises(FloatingPointError):
+        th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=0.5)))
+    semi.close()

@@ -1,0 +1,139 @@
+"""Host-side mirror (numpy) against the oracle: TreeMesh element order and
+geometry, interfaces, initial conditions; and the C ABI of the HIP library
+(loads and exports every declared symbol; argument validation needs no GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+import trixi_hip as th
+from trixi_hip import _lib
+from trixi_hip.mesh import element_geometry
+
+
+@pytest.mark.parametrize("D,N,level,cmin,cmax", [(3, 3, 2, (0, 0, 0), (2, 2, 2)), (2, 3, 4, (-10, -10), (10, 10)),
+                                                 (3, 4, 1, (-1, -1, -1), (1, 1, 1)), (2, 2, 3, (-1, -1), (1, 1))])
+def test_mesh_and_geometry_match_oracle(D, N, level, cmin, cmax):
+    from oracle import Oracle
+    o = Oracle(D, N, level, cmin, cmax)
+    m = o.mesh()
+    mesh = th.TreeMesh(cmin, cmax, initial_refinement_level=level)
+    basis = th.LobattoLegendreBasis(N)
+    ij, xc = element_geometry(mesh, basis)
+    assert mesh.nelements == o.nelements and mesh.ninterfaces == o.ninterfaces
+    np.testing.assert_array_equal(mesh.cell_centers, m["centers"])
+    np.testing.assert_array_equal(mesh.neighbor_ids, m["neighbor_ids"])
+    np.testing.assert_array_equal(mesh.orientations, m["orientations"])
+    np.testing.assert_array_equal(mesh.element_neighbors, m["element_neighbors"])
+    np.testing.assert_array_equal(ij, m["inverse_jacobian"])
+    np.testing.assert_array_equal(xc, m["node_coordinates"])
+
+
+def test_tree_order_follows_child_sign_table():
+    """Element 1 (0-based) of a level-1 tree is child 2 of abstract_tree.jl:118-128: (+, -, -)."""
+    mesh = th.TreeMesh((0, 0, 0), (2, 2, 2), initial_refinement_level=1)
+    np.testing.assert_array_equal(mesh.cell_centers[:4], [[0.5, 0.5, 0.5], [1.5, 0.5, 0.5], [0.5, 1.5, 0.5],
+                                                          [1.5, 1.5, 0.5]])
+    # periodic neighbours: +x neighbour centre is +dx modulo the domain
+    mesh = th.TreeMesh((0, 0, 0), (2, 2, 2), initial_refinement_level=3)
+    c = mesh.cell_centers
+    for d in range(3):
+        nb = mesh.element_neighbors[:, 2 * d + 1]
+        expect = c.copy()
+        expect[:, d] = (expect[:, d] + mesh.cell_length) % 2.0
+        np.testing.assert_allclose(c[nb], expect)
+
+
+@pytest.mark.parametrize("name,D,ic,params", [
+    ("constant", 3, th.initial_condition_constant, ()),
+    ("convergence_test", 3, th.initial_condition_convergence_test, ()),
+    ("convergence_test", 2, th.initial_condition_convergence_test, ()),
+    ("weak_blast_wave", 3, th.initial_condition_weak_blast_wave, ()),
+    ("isentropic_vortex", 2, th.initial_condition_isentropic_vortex(25.0), (25.0,)),
+    ("kelvin_helmholtz", 2, th.initial_condition_kelvin_helmholtz_instability(), ()),
+    ("fourier_modes", 3, th.initial_condition_fourier_modes(2), (0.1, 0.2)),
+])
+def test_initial_conditions_match_oracle(name, D, ic, params):
+    from oracle import Oracle
+    cmin, cmax = ((-2,) * D, (2,) * D)
+    o = Oracle(D, 3, 2, cmin, cmax)
+    o.set_initial_condition(name, params, ic.modes.reshape(-1) if ic.modes is not None else None)
+    eq = th.CompressibleEulerEquations3D() if D == 3 else th.CompressibleEulerEquations2D()
+    mesh = th.TreeMesh(cmin, cmax, initial_refinement_level=2)
+    _, xc = element_geometry(mesh, th.LobattoLegendreBasis(3))
+    u = ic(xc, 0.3, eq).reshape(-1)
+    np.testing.assert_allclose(u, o.compute_coefficients(0.3), rtol=1e-13, atol=1e-14)
+
+
+def test_khi_noise_is_seeded_and_shared_on_faces():
+    eq = th.CompressibleEulerEquations2D()
+    mesh = th.TreeMesh((-1, -1), (1, 1), initial_refinement_level=3)
+    _, xc = element_geometry(mesh, th.LobattoLegendreBasis(3))
+    a = th.initial_condition_kelvin_helmholtz_instability(1e-3, 5)(xc, 0, eq)
+    b = th.initial_condition_kelvin_helmholtz_instability(1e-3, 5)(xc, 0, eq)
+    c = th.initial_condition_kelvin_helmholtz_instability(1e-3, 6)(xc, 0, eq)
+    np.testing.assert_array_equal(a, b)
+    assert np.max(np.abs(a - c)) > 1e-5
+    # element 0's +x face equals element (+x neighbour)'s -x face
+    nb = mesh.element_neighbors[0, 1]
+    n = 4
+    right = a[0].reshape(n, n, 4)[:, n - 1]
+    left = a[nb].reshape(n, n, 4)[:, 0]
+    np.testing.assert_array_equal(right, left)
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "trixi_hip.h")).read()
+    return sorted(set(re.findall(r"\b(trixi_hip_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    # the Python binding declares every entry point of the header
+    assert sorted(name for name, _, _ in _lib.SIGNATURES) == syms
+
+
+def test_create_rejects_bad_configuration_without_gpu():
+    """Argument validation happens before any device call."""
+    lib = _lib.load()
+    cfg = _lib.Config()
+    cfg.ndims, cfg.polydeg, cfg.nelements, cfg.ninterfaces = 4, 3, 8, 32
+    out = ctypes.c_void_p()
+    rc = lib.trixi_hip_create(ctypes.byref(cfg), ctypes.byref(_lib.Mesh()), ctypes.byref(_lib.Basis()),
+                              ctypes.byref(out))
+    assert rc == _lib.EINVAL and not out.value
+    cfg.ndims, cfg.ninterfaces = 3, 5  # not a conforming periodic tree (I != D E)
+    rc = lib.trixi_hip_create(ctypes.byref(cfg), ctypes.byref(_lib.Mesh()), ctypes.byref(_lib.Basis()),
+                              ctypes.byref(out))
+    assert rc == _lib.EINVAL
+    assert lib.trixi_hip_create(None, None, None, None) == _lib.EINVAL
+
+
+def test_host_api_config_mapping():
+    eq = th.CompressibleEulerEquations2D(1.4)
+    basis = th.LobattoLegendreBasis(3)
+    ind = th.IndicatorHennemannGassner(eq, basis, alpha_max=0.002, alpha_min=1e-4, alpha_smooth=True,
+                                       variable=th.density_pressure)
+    vi = th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=th.flux_ranocha,
+                                           volume_flux_fv=th.FluxLaxFriedrichs(th.max_abs_speed_naive))
+    solver = th.DGSEM(basis, th.FluxLaxFriedrichs(th.max_abs_speed_naive), vi)
+    mesh = th.TreeMesh((-1, -1), (1, 1), initial_refinement_level=3)
+    semi = th.SemidiscretizationHyperbolic(mesh, eq, th.initial_condition_kelvin_helmholtz_instability(), solver,
+                                           precision="fp32")
+    c = semi.config()
+    assert (c.ndims, c.polydeg, c.nelements, c.ninterfaces) == (2, 3, 64, 128)
+    assert c.volume_integral == _lib.VI_SHOCK_CAPTURING_HG and c.volume_flux == _lib.FLUX_RANOCHA
+    assert c.fv_flux == _lib.FLUX_LLF_NAIVE and c.surface_flux == _lib.FLUX_LLF_NAIVE
+    assert c.precision == _lib.FP32 and c.alpha_max == 0.002 and c.alpha_min == 1e-4 and c.alpha_smooth == 1
+    assert semi.ndofs() == 64 * 16
+    with pytest.raises(ValueError):
+        th.TreeMesh((0, 0), (1, 2), initial_refinement_level=1)
+    with pytest.raises(ValueError):
+        th.TreeMesh((0, 0), (1, 1), initial_refinement_level=1, periodicity=False)

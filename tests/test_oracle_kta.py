@@ -1,0 +1,39 @@
+"""The FP64 oracle reproduces Trixi.jl's own known-answer tests (full
+trajectories: CarpenterKennedy2N54 + StepsizeCallback + AnalysisCallback
+errors). Values and parameters: tests/golden/kta.json (cited per case)."""
+import numpy as np
+import pytest
+
+from conftest import make_oracle
+
+CASES = ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d", "khi_2d",
+         "vortex_2d", "vortex_2d_constant_one_step"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_reproduces_trixi_kta(kta, name):
+    c = kta[name]
+    o = make_oracle(c["oracle"])
+    o.set_initial_condition(c["ic"], c["ic_params"])
+    u = o.compute_coefficients(c["tspan"][0])
+    u, t, nsteps = o.solve(u, c["tspan"][0], c["tspan"][1], c["cfl"], c.get("maxiters", 0))
+    l2, linf = o.error_norms(u, t)
+    if c.get("roundoff"):
+        # free-stream preservation: the reference errors are round-off (~1e-16..1e-14)
+        assert np.all(l2 < 1e-13) and np.all(linf < 1e-12)
+    else:
+        assert t == pytest.approx(c["tspan"][1], abs=0, rel=1e-15)
+        np.testing.assert_allclose(l2, c["l2"], rtol=1e-9, atol=0)
+        np.testing.assert_allclose(linf, c["linf"], rtol=1e-9, atol=0)
+
+
+def test_planted_mistake_is_caught(kta):
+    """A wrong LLF wave-speed variant (max_abs_speed instead of the elixir's
+    max_abs_speed_naive) moves the MMS errors far outside the KTA tolerance."""
+    c = kta["mms_3d_weak"]
+    spec = dict(c["oracle"], surface_flux="llf")
+    o = make_oracle(spec)
+    o.set_initial_condition(c["ic"], c["ic_params"])
+    u, t, _ = o.solve(o.compute_coefficients(0.0), 0.0, 5.0, c["cfl"])
+    l2, _ = o.error_norms(u, t)
+    assert np.max(np.abs(l2 / np.array(c["l2"]) - 1)) > 1e-6

@@ -1,0 +1,92 @@
+"""Size-independent properties of the oracle's rhs_hyperbolic!: free-stream
+preservation, discrete conservation, design order of the MMS residual, the
+HG indicator range, and the order of the CarpenterKennedy2N54 recurrence."""
+import numpy as np
+import pytest
+
+from conftest import make_oracle
+import oracle.oracle as O
+
+
+def spec(D, N, level, vi="fd", vflux="ranocha", sflux="llf", **kw):
+    cmin, cmax = ((0,) * D, (2,) * D)
+    return dict(ndims=D, polydeg=N, level=level, cmin=cmin, cmax=cmax, volume_integral=vi, volume_flux=vflux,
+                surface_flux=sflux, **kw)
+
+
+@pytest.mark.parametrize("D,vi,vflux,sflux", [(3, "fd", "ranocha", "llf"), (3, "weak", "ranocha", "llf_naive"),
+                                               (2, "fd", "ranocha", "llf"), (2, "hg", "ranocha", "llf_naive")])
+def test_free_stream(D, vi, vflux, sflux):
+    o = make_oracle(spec(D, 3, 2, vi, vflux, sflux))
+    o.set_initial_condition("constant")
+    du = o.rhs(o.compute_coefficients(0.0))
+    assert np.max(np.abs(du)) < 1e-12
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_conservation(D):
+    """sum_e sum_nodes w J du = 0 for every conserved variable (periodic, no sources)."""
+    o = make_oracle(spec(D, 3, 2))
+    o.set_initial_condition("fourier_modes" if D == 3 else "kelvin_helmholtz", (0.1, 0.2),
+                            np.random.default_rng(3).uniform(0.5, 3, 128) if D == 3 else None)
+    u = o.compute_coefficients(0.0)
+    du = o.rhs(u)
+    b = o.basis()
+    n = o.n
+    w = b["weights"]
+    wn = np.einsum("i,j,k->kji", w, w, w).reshape(-1) if D == 3 else np.outer(w, w).reshape(-1)
+    ij = o.mesh()["inverse_jacobian"]
+    tot = np.einsum("env,n,e->v", du.reshape(o.nelements, n ** D, D + 2), wn, 1.0 / ij ** D)
+    scale = np.max(np.abs(du)) * o.nelements
+    assert np.max(np.abs(tot)) < 1e-12 * scale
+
+
+def test_mms_residual_converges_at_design_order():
+    """|rhs(u_exact) - du_exact/dt| decays like h^N (here N = 3) on the MMS."""
+    errs = []
+    for level in (1, 2, 3):
+        o = make_oracle(spec(3, 3, level, "fd", "ranocha", "llf", source="convergence_test"))
+        o.set_initial_condition("convergence_test")
+        t = 0.3
+        u = o.compute_coefficients(t)
+        du = o.rhs(u, t)
+        h = 1e-6
+        dudt = (o.compute_coefficients(t + h) - o.compute_coefficients(t - h)) / (2 * h)
+        errs.append(np.max(np.abs(du - dudt)))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 2.5), rates
+
+
+def test_hg_indicator_range():
+    o = make_oracle(dict(ndims=2, polydeg=3, level=4, cmin=(-1, -1), cmax=(1, 1), volume_integral="hg",
+                         volume_flux="ranocha", surface_flux="llf_naive", fv_flux="llf_naive", alpha_max=0.002,
+                         alpha_min=1e-4))
+    o.set_initial_condition("kelvin_helmholtz")
+    a = o.indicator_hg(o.compute_coefficients(0.0))
+    assert np.all(a >= 0) and np.all(a <= 0.002)
+    # smooth elements sit at 1/(1 + exp(s)) = 1e-4 = alpha_min (not below it, so
+    # they stay blended); the shear layers reach alpha_max
+    assert np.min(a) == pytest.approx(1e-4, rel=1e-9) and np.max(a) == 0.002
+
+
+def test_ck2n54_fourth_order():
+    """The 2N recurrence u_tmp = du - a u_tmp; u += b dt u_tmp (methods_2N.jl:153-157)
+    with the oracle's coefficients is 4th order on y' = lambda y."""
+    a, b, c = O.ck2n54()
+    lam = -1.3 + 0.7j
+
+    def integrate(nsteps, T=1.0):
+        y, dt = 1.0 + 0j, T / nsteps
+        for _ in range(nsteps):
+            tmp = 0j
+            for s in range(5):
+                du = lam * y
+                tmp = du - a[s] * tmp
+                y = y + b[s] * dt * tmp
+        return y
+
+    errs = [abs(integrate(n) - np.exp(lam)) for n in (10, 20, 40)]
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 3.8), rates
+    # consistency: sum of the effective weights equals one (exact for y' = 1)
+    assert abs(integrate(1, 0.5) - np.exp(lam * 0.5)) < 1e-2

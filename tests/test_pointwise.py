@@ -1,0 +1,80 @@
+"""Pointwise physics of the oracle: logarithmic means against mpmath, and
+flux_ranocha / flux_lax_friedrichs against mpmath values and the properties
+that fix them (consistency, symmetry, Tadmor entropy conservation)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+import oracle.oracle as O
+
+
+def test_ln_mean_vs_mpmath():
+    for c in golden("ln_mean_mpmath.json"):
+        x, y = float(c["x"]), float(c["y"])
+        assert O.ln_mean(x, y) == pytest.approx(float(c["ln_mean"]), rel=2e-15)
+        assert O.inv_ln_mean(x, y) == pytest.approx(float(c["inv_ln_mean"]), rel=2e-15)
+
+
+def test_ln_mean_threshold_continuity():
+    # both branches agree across f^2 = 1e-4 (|y-x|/(y+x) = 0.01)
+    x = 1.0
+    for r in np.linspace(1.0199, 1.0203, 41):
+        y = x * r
+        exact = (y - x) / math.log(y / x)
+        assert O.ln_mean(x, y) == pytest.approx(exact, rel=1e-13)
+
+
+def rand_state(rng, D):
+    rho = 0.5 + rng.uniform(0, 1.5)
+    v = rng.uniform(-1, 1, D)
+    p = 0.3 + rng.uniform(0, 2)
+    return np.array([rho, *(rho * v), p / 0.4 + 0.5 * rho * v @ v])
+
+
+def test_flux_ranocha_vs_mpmath():
+    for c in golden("flux_ranocha_mpmath.json"):
+        f = O.flux(c["ndims"], "ranocha", c["u_ll"], c["u_rr"], c["orientation"])
+        np.testing.assert_allclose(f, [float(v) for v in c["flux"]], rtol=2e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_flux_properties(D):
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        ul, ur = rand_state(rng, D), rand_state(rng, D)
+        for o in range(D):
+            fu = O.flux(D, "physical", ul, None, o)
+            # consistency f#(u, u) = f(u)
+            for kind in ("ranocha", "llf", "llf_naive", "central"):
+                np.testing.assert_allclose(O.flux(D, kind, ul, ul, o), fu, rtol=1e-13, atol=1e-14)
+            # symmetry of the entropy-conservative flux
+            np.testing.assert_allclose(O.flux(D, "ranocha", ul, ur, o), O.flux(D, "ranocha", ur, ul, o), rtol=1e-13)
+            # Tadmor: (w_R - w_L) . f# = psi_R - psi_L, psi = rho v_o (entropy_potential)
+            f = O.flux(D, "ranocha", ul, ur, o)
+            wl, wr = O.cons2entropy(D, ul), O.cons2entropy(D, ur)
+            lhs = (wr - wl) @ f
+            rhs = ur[1 + o] - ul[1 + o]
+            assert lhs == pytest.approx(rhs, abs=1e-12 * max(1.0, abs(rhs)))
+
+
+def test_llf_closed_form():
+    D = 3
+    ul = np.array([1.0, 0.1, -0.2, 0.3, 2.5])
+    ur = np.array([1.2, 0.3, 0.1, -0.1, 2.8])
+    gamma = 1.4
+
+    def prim(u):
+        v = u[1:4] / u[0]
+        return v, (gamma - 1) * (u[4] - 0.5 * u[1:4] @ v)
+
+    for o in range(D):
+        vl, pl = prim(ul)
+        vr, pr = prim(ur)
+        cl, cr = math.sqrt(gamma * pl / ul[0]), math.sqrt(gamma * pr / ur[0])
+        lam = max(abs(vl[o]) + cl, abs(vr[o]) + cr)
+        lam_naive = max(abs(vl[o]), abs(vr[o])) + max(cl, cr)
+        fc = 0.5 * (O.flux(D, "physical", ul, None, o) + O.flux(D, "physical", ur, None, o))
+        np.testing.assert_allclose(O.flux(D, "llf", ul, ur, o), fc - 0.5 * lam * (ur - ul), rtol=1e-14)
+        np.testing.assert_allclose(O.flux(D, "llf_naive", ul, ur, o), fc - 0.5 * lam_naive * (ur - ul), rtol=1e-14)
