@@ -90,3 +90,24 @@ def test_ck2n54_fourth_order():
     assert np.all(rates > 3.8), rates
     # consistency: sum of the effective weights equals one (exact for y' = 1)
     assert abs(integrate(1, 0.5) - np.exp(lam * 0.5)) < 1e-2
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_integrals_closed_forms(D):
+    """integrate_via_indices (normalised): a constant state integrates to itself
+    and its entropy_math S = -rho s/(gamma-1), s = ln p - gamma ln rho; the MMS
+    density integrates to its mean c = 2 (LGL quadrature is exact for the
+    periodic sine up to the quadrature error, < 1e-6 at level 3)."""
+    import math
+    o = make_oracle(spec(D, 3, 3))
+    o.set_initial_condition("constant")
+    u = o.compute_coefficients(0.0)
+    cons, ent = o.integrals(u)
+    u0 = u[:D + 2]
+    np.testing.assert_allclose(cons, u0, rtol=1e-12)  # plain summation over all nodes
+    p = 0.4 * (u0[-1] - 0.5 * np.sum(u0[1:-1] ** 2) / u0[0])
+    s = math.log(p) - 1.4 * math.log(u0[0])
+    assert ent == pytest.approx(-s * u0[0] / 0.4, rel=1e-12)
+    o.set_initial_condition("convergence_test")
+    cons, _ = o.integrals(o.compute_coefficients(0.3))
+    assert cons[0] == pytest.approx(2.0, abs=1e-6)

@@ -1,0 +1,166 @@
+# TrixiHIP.jl -- Julia binding of libtrixi_hip.so (include/trixi_hip.h).
+#
+# Makes the MI355X right-hand side a drop-in for Trixi.jl's
+#   rhs_hyperbolic!(du_ode, u_ode, semi, t)   (src/semidiscretization/semidiscretization_hyperbolic.jl:578-597)
+# for TreeMesh{2,3} + CompressibleEulerEquations{2,3}D + DGSEM with periodic
+# boundaries. The ODEProblem keeps `semi` as its parameter, so the 2N
+# integrators and the Stepsize/Analysis callbacks run unchanged; only the RHS
+# function is replaced by a callable that ccalls the library.
+#
+# Julia is not installed in the build image, so this file is shipped but not
+# executed there; the C++/Python harness of the repository makes exactly these
+# calls.
+module TrixiHIP
+
+using Trixi
+using Trixi: TreeMesh, DGSEM, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
+             VolumeIntegralWeakForm, VolumeIntegralFluxDifferencing, VolumeIntegralShockCapturingHGType,
+             VolumeIntegralPureLGLFiniteVolume, FluxLaxFriedrichs, flux_ranocha, flux_central,
+             max_abs_speed, max_abs_speed_naive, source_terms_convergence_test, SolutionAnalyzer,
+             mesh_equations_solver_cache, nelements, ninterfaces
+using SciMLBase: ODEProblem
+
+const libtrixi_hip = get(ENV, "TRIXI_HIP_LIB", joinpath(@__DIR__, "..", "libtrixi_hip.so"))
+
+# ---- C structs (field order and types of include/trixi_hip.h) ----
+struct HipConfig
+    ndims::Int32
+    polydeg::Int32
+    nelements::Int32
+    ninterfaces::Int32
+    gamma::Float64
+    precision::Int32
+    volume_integral::Int32
+    volume_flux::Int32
+    surface_flux::Int32
+    fv_flux::Int32
+    alpha_max::Float64
+    alpha_min::Float64
+    alpha_smooth::Int32
+    source_terms::Int32
+    elements_per_block::Int32
+    threads_per_block::Int32
+end
+
+struct HipMesh
+    neighbor_ids::Ptr{Int32}
+    orientations::Ptr{Int32}
+    inverse_jacobian::Ptr{Float64}
+    element_centers::Ptr{Float64}
+end
+
+struct HipBasis
+    nodes::Ptr{Float64}
+    weights::Ptr{Float64}
+    derivative_split::Ptr{Float64}
+    derivative_hat::Ptr{Float64}
+    inverse_vandermonde_legendre::Ptr{Float64}
+    analysis_nnodes::Int32
+    analysis_weights::Ptr{Float64}
+    analysis_vandermonde::Ptr{Float64}
+end
+
+const HOST = Int32(0)
+const DEVICE = Int32(1)
+
+# ---- numerical-flux and volume-integral codes ----
+flux_code(f::FluxLaxFriedrichs) = f.dissipation.max_abs_speed === max_abs_speed_naive ? Int32(1) : Int32(0)
+flux_code(::typeof(flux_ranocha)) = Int32(2)
+flux_code(::typeof(flux_central)) = Int32(3)
+
+vi_codes(::VolumeIntegralWeakForm) = (Int32(0), Int32(2), Int32(0), nothing)
+vi_codes(v::VolumeIntegralFluxDifferencing) = (Int32(1), flux_code(v.volume_flux), Int32(0), nothing)
+function vi_codes(v::VolumeIntegralShockCapturingHGType)
+    (Int32(2), flux_code(v.volume_integral_blend_high_order.volume_flux),
+     flux_code(v.volume_integral_blend_low_order.volume_flux_fv), v.indicator)
+end
+vi_codes(v::VolumeIntegralPureLGLFiniteVolume) = (Int32(3), Int32(2), flux_code(v.volume_flux_fv), nothing)
+
+source_code(::Nothing) = Int32(0)
+source_code(::typeof(source_terms_convergence_test)) = Int32(1)
+
+check(rc, ctx) = rc == 0 || error("libtrixi_hip: ", unsafe_string(ccall((:trixi_hip_last_error, libtrixi_hip),
+                                                                        Cstring, (Ptr{Cvoid},), ctx)))
+
+# Row-major copy of a Julia (column-major) matrix, as the C ABI expects.
+rowmajor(M) = vec(permutedims(Matrix{Float64}(M)))
+
+"""
+    HipRHS(semi; precision = :fp64)
+
+Callable `f(du_ode, u_ode, semi, t)` evaluating `rhs_hyperbolic!` on the GPU.
+Owns one library context (device copies of mesh and basis).
+"""
+mutable struct HipRHS
+    ctx::Ptr{Cvoid}
+end
+
+function HipRHS(semi; precision::Symbol = :fp64)
+    mesh, equations, solver, cache = mesh_equations_solver_cache(semi)
+    mesh isa TreeMesh || throw(ArgumentError("TrixiHIP supports TreeMesh only"))
+    D = ndims(mesh)
+    E = nelements(solver, cache)
+    I = ninterfaces(solver, cache)
+    vi, vflux, fvflux, indicator = vi_codes(solver.volume_integral)
+    amax, amin, asmooth = indicator === nothing ? (0.5, 0.001, true) :
+                          (indicator.alpha_max, indicator.alpha_min, indicator.alpha_smooth)
+    cfg = HipConfig(D, Trixi.polydeg(solver), E, I, equations.gamma, precision === :fp32 ? 1 : 0,
+                    vi, vflux, flux_code(solver.surface_integral.surface_flux), fvflux, amax, amin,
+                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0)
+    # interface container: 1-based ids and orientations -> 0-based
+    nbr = Int32.(cache.interfaces.neighbor_ids .- 1)            # [2, I]
+    ori = Int32.(cache.interfaces.orientations .- 1)            # [I]
+    invj = Vector{Float64}(cache.elements.inverse_jacobian)     # [E]
+    centers = Matrix{Float64}(mesh.tree.coordinates[:, cache.elements.cell_ids])  # [D, E]
+    basis = solver.basis
+    analyzer = SolutionAnalyzer(basis)
+    nodes, weights = Vector{Float64}(basis.nodes), Vector{Float64}(basis.weights)
+    ds, dh = rowmajor(basis.derivative_split), rowmajor(basis.derivative_hat)
+    vinv = rowmajor(basis.inverse_vandermonde_legendre)
+    aw, av = Vector{Float64}(analyzer.weights), rowmajor(analyzer.vandermonde)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve nbr ori invj centers nodes weights ds dh vinv aw av begin
+        hm = Ref(HipMesh(pointer(nbr), pointer(ori), pointer(invj), pointer(centers)))
+        hb = Ref(HipBasis(pointer(nodes), pointer(weights), pointer(ds), pointer(dh), pointer(vinv),
+                          length(aw), pointer(aw), pointer(av)))
+        rc = ccall((:trixi_hip_create, libtrixi_hip), Cint,
+                   (Ref{HipConfig}, Ref{HipMesh}, Ref{HipBasis}, Ref{Ptr{Cvoid}}), Ref(cfg), hm, hb, out)
+        rc == 0 || error("trixi_hip_create failed with code $rc")
+    end
+    f = HipRHS(out[])
+    finalizer(f) do g
+        ccall((:trixi_hip_destroy, libtrixi_hip), Cint, (Ptr{Cvoid},), g.ctx)
+    end
+    return f
+end
+
+# Host vectors (Vector{Float64}): the library stages them through device memory.
+function (f::HipRHS)(du_ode::Vector{Float64}, u_ode::Vector{Float64}, semi, t)
+    GC.@preserve du_ode u_ode begin
+        check(ccall((:trixi_hip_rhs, libtrixi_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cint),
+                    f.ctx, du_ode, u_ode, t, HOST), f.ctx)
+    end
+    return nothing
+end
+
+# Device vectors (e.g. AMDGPU.ROCVector{Float64}): pass the device pointers.
+function (f::HipRHS)(du_ode::AbstractVector{Float64}, u_ode::AbstractVector{Float64}, semi, t)
+    check(ccall((:trixi_hip_rhs, libtrixi_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cint),
+                f.ctx, pointer(du_ode), pointer(u_ode), t, DEVICE), f.ctx)
+    return nothing
+end
+
+"""
+    semidiscretize_hip(semi, tspan; precision = :fp64)
+
+Like `Trixi.semidiscretize(semi, tspan)` (semidiscretization.jl:102-153) but with
+the RHS evaluated by libtrixi_hip. `p` stays `semi`, so callbacks are unchanged.
+"""
+function semidiscretize_hip(semi, tspan; precision::Symbol = :fp64)
+    ode = Trixi.semidiscretize(semi, tspan)
+    return ODEProblem{true}(HipRHS(semi; precision = precision), ode.u0, ode.tspan, semi)
+end
+
+export HipRHS, semidiscretize_hip
+
+end # module
