@@ -76,7 +76,7 @@ extern "C" {
 
 typedef struct trixi_hip_config {
   int32_t ndims;           /* 2 or 3 */
-  int32_t polydeg;         /* N, nnodes = N + 1 (supported: 1..7) */
+  int32_t polydeg;         /* N, nnodes = N + 1 (supported: 2..4) */
   int32_t nelements;       /* E */
   int32_t ninterfaces;     /* I */
   double gamma;            /* CompressibleEulerEquations{2,3}D(gamma) */

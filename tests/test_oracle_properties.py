@@ -111,3 +111,32 @@ def test_integrals_closed_forms(D):
     o.set_initial_condition("convergence_test")
     cons, _ = o.integrals(o.compute_coefficients(0.3))
     assert cons[0] == pytest.approx(2.0, abs=1e-6)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_entropy_gradient_is_entropy_variables(D):
+    """entropy_math S(u) and cons2entropy w(u) are tied by w = dS/du; w is pinned
+    on its own by Tadmor's entropy-conservation identity of flux_ranocha, so a
+    central difference of the integrated S of constant states pins S."""
+    from oracle.oracle import cons2entropy
+    o = make_oracle(spec(D, 2, 1))
+    o.set_initial_condition("constant")
+    nn = o.compute_coefficients(0.0).size // (D + 2)
+    rng = np.random.default_rng(7)
+
+    def S(state):
+        return o.integrals(np.tile(state, nn))[1]
+
+    for _ in range(4):
+        rho = rng.uniform(0.5, 2.0)
+        v = rng.uniform(-0.5, 0.5, D)
+        p = rng.uniform(0.5, 2.0)
+        u = np.concatenate([[rho], rho * v, [p / 0.4 + 0.5 * rho * v @ v]])
+        grad = np.empty(D + 2)
+        for k in range(D + 2):
+            h = 1e-5 * max(1.0, abs(u[k]))
+            up, um = u.copy(), u.copy()
+            up[k] += h
+            um[k] -= h
+            grad[k] = (S(up) - S(um)) / (2 * h)
+        np.testing.assert_allclose(grad, cons2entropy(D, u), rtol=1e-7, atol=1e-8)
