@@ -67,8 +67,17 @@ extern "C" {
 #define TRIXI_HIP_IC_KELVIN_HELMHOLTZ 4
 #define TRIXI_HIP_IC_FOURIER_MODES 5
 
-/* indicator modes for trixi_hip_indicator */
-#define TRIXI_HIP_IND_HG 0 /* IndicatorHennemannGassner */
+/* indicator modes for trixi_hip_indicator / trixi_hip_set_indicator */
+#define TRIXI_HIP_IND_HG 0                /* IndicatorHennemannGassner */
+#define TRIXI_HIP_IND_MLP 1               /* learned indicator (2D p3), network output only */
+#define TRIXI_HIP_IND_MLP_WITH_FALLBACK 2 /* learned indicator; HG where an input leaves the trained range */
+
+/* learned indicator: 19 -> 64 -> 64 -> 1 perceptron, FP32 parameters in the
+ * order W1[64][19], b1[64], W2[64][64], b2[64], w3[64], b3 (row-major, rows =
+ * output units); features [E][32] FP32, the first 19 used */
+#define TRIXI_HIP_MLP_NPARAMS 5505
+#define TRIXI_HIP_MLP_NFEATURES 19
+#define TRIXI_HIP_MLP_FEATURE_STRIDE 32
 
 /* where the pointers passed to trixi_hip_rhs live */
 #define TRIXI_HIP_HOST 0
@@ -164,6 +173,29 @@ int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt);
 /* Blending factors alpha[E] of IndicatorHennemannGassner (indicators.jl:119-156,
  * indicators_{2,3}d.jl) for the resident state, after smoothing. */
 int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host);
+
+/* Learned troubled-cell indicator (2D polydeg 3 contexts). No counterpart in
+ * Trixi.jl's source tree (its neural-network indicators were moved to a
+ * separate package, NEWS.md:495); it replaces the HG blending factor of
+ * VolumeIntegralShockCapturingHG (indicators.jl:119-156) when selected with
+ * trixi_hip_set_indicator. The network reads features computed from the HG
+ * indicator's own quantities (indicators_2d.jl:30-102), alpha = alpha_max *
+ * sigmoid(output); smoothing (indicators_2d.jl:104-135) is applied as for HG. */
+int trixi_hip_mlp_set_params(trixi_hip_ctx *ctx, const float *params, const float *feature_lo,
+                             const float *feature_hi); /* params [NPARAMS]; box [NFEATURES] each; resets Adam */
+int trixi_hip_mlp_get_params(trixi_hip_ctx *ctx, float *params);
+/* blending-factor source of the shock-capturing RHS (TRIXI_HIP_IND_*) */
+int trixi_hip_set_indicator(trixi_hip_ctx *ctx, int mode);
+/* features [E][32] and raw HG alpha [E] of the resident state (either may be NULL) */
+int trixi_hip_mlp_features(trixi_hip_ctx *ctx, float *features_host, double *alpha_hg_host);
+/* y[n] = sigmoid(network(x[n][32])), device arrays */
+int trixi_hip_mlp_forward(trixi_hip_ctx *ctx, const float *x_dev, int n, float *y_dev);
+/* loss = sum (y - target)^2 / n and its gradient [NPARAMS] (host; either may be NULL) */
+int trixi_hip_mlp_loss_grad(trixi_hip_ctx *ctx, const float *x_dev, const float *target_dev, int n, float *loss,
+                            float *grad_host);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the loss above; loss may be NULL (no sync) */
+int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float *target_dev, int n,
+                             float learning_rate, float *loss);
 
 /* Exact solution used by calc_error_norms (analysis_dg{2,3}d.jl). params:
  * IC_ISENTROPIC_VORTEX {p_background}; IC_FOURIER_MODES {A_rho, A_v} and

@@ -74,21 +74,21 @@ __device__ __forceinline__ int xcd_remap_block(int b, int nb) {
   return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-// Cooperative contiguous copy global <-> LDS with 16-byte accesses when the
-// tile size allows it.
-template <typename T, int NT>
-__device__ __forceinline__ void tile_copy(T *__restrict__ dst, const T *__restrict__ src, int cnt,
-                                          bool vec_ok) {
-  const int tid = threadIdx.x;
-  if (vec_ok) {
-    const int c4 = cnt * (int)sizeof(T) / 16;
-    const float4 *s4 = reinterpret_cast<const float4 *>(src);
-    float4 *d4 = reinterpret_cast<float4 *>(dst);
-    for (int i = tid; i < c4; i += NT) d4[i] = s4[i];
-  } else {
-    for (int i = tid; i < cnt; i += NT) dst[i] = src[i];
-  }
-}
+template <int BYTES> struct ChunkOf;
+template <> struct ChunkOf<16> { using type = float4; };
+template <> struct ChunkOf<8> { using type = float2; };
+template <> struct ChunkOf<4> { using type = float; };
+
+// A full tile of EPB elements moves between global memory and LDS in chunks of
+// CB bytes, the largest of 16/8/4 that divides the tile size (tile offsets are
+// multiples of the tile size and the arrays are 16-byte aligned).
+template <typename T, int D, int N, int EPB> struct TileIO {
+  static constexpr int NN = ipow_c(N + 1, D), V = D + 2;
+  static constexpr int TB = EPB * NN * V * (int)sizeof(T);
+  static constexpr int CB = TB % 16 == 0 ? 16 : TB % 8 == 0 ? 8 : 4;
+  using C = typename ChunkOf<CB>::type;
+  static constexpr int NCH = TB / CB;
+};
 
 // LDS index map and lane -> line assignment (see the header comment).
 template <int D, int N, int EPB> struct Layout {
@@ -141,19 +141,38 @@ template <typename T, int V> struct LineCtx {
   T ul_m[V], ur_p[V], own0[V], own1[V];
 };
 
-template <typename T, int D, int N, int EPB, int O>
-__device__ __forceinline__ void line_setup(const ElemArgs<T> &a, int e0, int ne, int ll,
-                                           LineCtx<T, D + 2> &c) {
+// Line of this lane in direction o (wave-uniform, runtime) and the ids of the
+// two neighbours across its end faces. Branch-free over o, so that these loads
+// and the tile loads are issued back to back. Lanes without a line (or on a
+// missing element of a partial tile) use the tile's last element, so every
+// lane issues the same in-bounds loads.
+template <typename T, int D, int N, int EPB>
+__device__ __forceinline__ void line_ids(const ElemArgs<T> &a, int e0, int ne, int ll, int o, LineCtx<T, D + 2> &c,
+                                         int &nbm, int &nbp) {
+  using LY = Layout<D, N, EPB>;
+  constexpr int n = N + 1;
+  int el0, line0, el2, line2;
+  LY::template lane_line<0>(ll, el0, line0);  // directions 0 and 1 share the lane map
+  LY::template lane_line<D - 1>(ll, el2, line2);
+  c.el = o == D - 1 ? el2 : el0;
+  c.line = o == D - 1 ? line2 : line0;
+  c.valid = (ll < EPB * LY::L) && (c.el < ne);
+  // line = r0 + n r1 over the two free indices; base = node of index 0 along o
+  const int b0 = c.line * n;                                // o = 0: (j, k) -> j n + k n^2
+  const int b1 = (c.line % n) + (c.line / n) * n * n;       // o = 1: (i, k) -> i + k n^2
+  const int b2 = c.line;                                    // o = 2: (i, j) -> i + j n
+  c.base = o == 0 ? b0 : (o == 1 ? b1 : b2);
+  const int e = e0 + min(c.el, ne - 1);
+  nbm = a.nbr[(2 * o) + 2 * D * e];
+  nbp = a.nbr[(2 * o + 1) + 2 * D * e];
+}
+
+// Neighbour face states of the line in direction o (consumed in phase B).
+template <typename T, int D, int N, int EPB>
+__device__ __forceinline__ void line_loads(const ElemArgs<T> &a, LineCtx<T, D + 2> &c, int o, int nbm, int nbp) {
   using LY = Layout<D, N, EPB>;
   constexpr int n = N + 1, NN = LY::NN, V = D + 2;
-  constexpr int st = ipow_c(n, O);
-  LY::template lane_line<O>(ll, c.el, c.line);
-  c.valid = (ll < EPB * LY::L) && (c.el < ne);
-  if (!c.valid) return;
-  c.base = (c.line % st) + (c.line / st) * st * n;
-  const int e = e0 + c.el;
-  const int nbm = a.nbr[(2 * O) + 2 * D * e];
-  const int nbp = a.nbr[(2 * O + 1) + 2 * D * e];
+  const int st = o == 0 ? 1 : (o == 1 ? n : n * n);
   const T *un_m = a.u + ((size_t)nbm * NN + c.base + (n - 1) * st) * V;
   const T *un_p = a.u + ((size_t)nbp * NN + c.base) * V;
 #pragma unroll
@@ -387,205 +406,220 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
 //   alpha_i = pi (c_1 + ... + c_D - t) + pi J xi_i,  gamma_j = pi J xi_j,
 // so each element needs 2n sincos evaluations instead of one per node; the
 // node values follow by angle addition.
-template <typename T, int D, int N>
-__device__ __forceinline__ void source_tables(const ElemArgs<T> &a, const DGConst<T, N> &k, int e, int idx,
-                                              double tt, T *__restrict__ tab) {
+// Entry idx of an element's table from its inverse Jacobian and the sum of its
+// centre coordinates.
+template <typename T, int N>
+__device__ __forceinline__ void source_table_entry(const DGConst<T, N> &k, int idx, double inv_jac, double csum,
+                                                   double tt, T *__restrict__ tab) {
   constexpr int n = N + 1;
-  const double jac = 1.0 / a.inv_jac[e];
+  const double jac = 1.0 / inv_jac;
   const double pi = 3.14159265358979323846;
-  double csum = 0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) csum += a.centers[d + D * e];
   const int which = idx / n, i = idx - which * n;  // which: 0 = alpha, 1 = gamma
-  const double ph = (which == 0 ? pi * (csum - tt) : 0.0) + pi * jac * k.nodes[i];
+  // Select the node from registers: a dynamically indexed kernel-argument load
+  // would be a vector-memory load whose wait also drains the neighbour loads
+  // in flight. The empty asm makes each value opaque, so the selects are not
+  // folded back into an indexed load.
+  double xi = 0.0;
+#pragma unroll
+  for (int m = 0; m < n; ++m) {
+    double nm = k.nodes[m];
+    asm volatile("" : "+v"(nm));
+    xi = i == m ? nm : xi;
+  }
+  const double ph = (which == 0 ? pi * (csum - tt) : 0.0) + pi * jac * xi;
   T si, co;
   tsincos(T(ph), &si, &co);
   tab[2 * idx] = si;
   tab[2 * idx + 1] = co;
 }
 
-// PERSIST: a resident grid loops over tiles with a one-tile register prefetch
-// of the next tile's state; otherwise one tile per workgroup.
-template <typename T, int D, int N, int VOL, int EPB, int WPD, bool PERSIST = false>
+// One tile of EPB elements per workgroup. Every global load of phase A is
+// issued before the first one is consumed: neighbour ids, the tile (in chunks,
+// into registers), the per-element scalars, then the neighbour face states,
+// which stay in flight until phase B.
+template <typename T, int D, int N, int VOL, int EPB, int WPD>
 __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
   using LY = Layout<D, N, EPB>;
   using VT = VolTraits<VOL>;
+  using TIO = TileIO<T, D, N, EPB>;
+  using C = typename TIO::C;
   constexpr int NT = D * WPD * 64;
   constexpr int n = N + 1, NN = LY::NN, V = D + 2, SZ = LY::SIZE;
   constexpr int NW = VT::RAN ? nodeR_count(D) : 0;
   constexpr int NC = VT::CONS ? V : 0;
-  constexpr int SIZE_A = (NW + NC) * SZ;  // primitive / conservative data, later output staging
-  constexpr int SIZE_B = D * V * SZ;      // du per direction, earlier input staging
+  constexpr int ALIGN = 16 / (int)sizeof(T);
+  // primitive / conservative data, later output staging; a multiple of 16 bytes
+  constexpr int SIZE_A = ((NW + NC) * SZ + ALIGN - 1) / ALIGN * ALIGN;
+  constexpr int SIZE_B = D * V * SZ;  // du per direction, earlier input staging
   static_assert(SIZE_A >= EPB * NN * V && SIZE_B >= EPB * NN * V, "staging does not fit");
   static_assert(EPB * LY::L <= WPD * 64, "one line per lane");
-  constexpr bool vec_ok = ((NN * V * (int)sizeof(T)) % 16) == 0;
-  constexpr int TILE4 = vec_ok ? EPB * NN * V * (int)sizeof(T) / 16 : 1;  // 16-byte chunks per tile
-  constexpr int PF = (vec_ok && PERSIST) ? (TILE4 + NT - 1) / NT : 1;      // chunks per thread
+  static_assert(EPB * 2 * n <= NT, "one source-table entry per thread");
+  constexpr int RPT = (TIO::NCH + NT - 1) / NT;  // tile chunks per thread
   __shared__ __attribute__((aligned(16))) T smem[SIZE_A + SIZE_B];
   __shared__ double s_alpha[VT::HG ? EPB : 1];
   __shared__ T s_src[EPB * 4 * n];  // per element: (sin, cos) of alpha_i, gamma_j
+  __shared__ T s_fac[EPB];          // per element: -inverse_jacobian
   T *s_w = smem;            // [NW][SZ]
   T *s_c = smem + NW * SZ;  // [NC][SZ]
   T *s_d = smem + SIZE_A;   // [D][V][SZ]
   T *s_o = smem;            // [EPB*NN][V] output staging (region A, free after phase B)
 
   const int tid = threadIdx.x;
-  const int grp = tid / (WPD * 64);  // direction of this wave group (wave-uniform)
+  // direction of this wave group (wave-uniform, kept in a scalar register)
+  const int grp = __builtin_amdgcn_readfirstlane(tid / (WPD * 64));
   const int ll = tid - grp * (WPD * 64);
-  const int ntiles = (a.E + EPB - 1) / EPB;
+  const int e0 = (a.xcd_remap ? xcd_remap_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * EPB;
+  if (e0 >= a.E) return;  // uniform per block
+  const int ne = min(EPB, a.E - e0);
+  const bool full = ne == EPB;
   const double tt = a.tdev ? (a.tdev[0] + a.t * a.tdev[1]) : a.t;
 
-  // Persistent loop over tiles with a one-tile-deep register prefetch: the
-  // next tile's state and neighbour face states are in flight while the
-  // current tile is computed.
-  int tile = blockIdx.x;
-  if (tile >= ntiles) return;  // uniform per block
-  float4 pf[PF];
-  auto prefetch = [&](int t) {
-    if constexpr (vec_ok && PERSIST) {
-      const int e0 = t * EPB;
-      const int ne = min(EPB, a.E - e0);
-      const float4 *src = reinterpret_cast<const float4 *>(a.u + (size_t)e0 * NN * V);
-      const int c4 = ne * NN * V * (int)sizeof(T) / 16;
+  // ---- phase A: issue the loads ----
+  LineCtx<T, V> lc;
+  int nbm, nbp;
+  line_ids<T, D, N, EPB>(a, e0, ne, ll, grp, lc, nbm, nbp);
+  // chunk indices are clamped to the tile's extent, so all lanes load (a
+  // partial last tile is staged by the scalar loop below instead)
+  const int nch = full ? TIO::NCH : max(1, ne * NN * V * (int)sizeof(T) / TIO::CB);
+  const C *csrc = reinterpret_cast<const C *>(a.u + (size_t)e0 * NN * V);
+  C rc[RPT];
 #pragma unroll
-      for (int r = 0; r < PF; ++r) {
-        const int i = tid + r * NT;
-        if (i < c4) pf[r] = src[i];
-      }
-    }
-  };
-  prefetch(tile);
+  for (int r = 0; r < RPT; ++r) rc[r] = csrc[min(tid + r * NT, nch - 1)];
+  // element of this thread's source-table entry (threads beyond the table repeat the last one)
+  const int sel = min(tid / (2 * n), ne - 1);
+  const double ij = a.inv_jac[e0 + sel];
+  double cen[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) cen[d] = a.centers[d + D * (e0 + sel)];
+  line_loads<T, D, N, EPB>(a, lc, grp, nbm, nbp);
 
-  while (true) {
-    const int e0 = tile * EPB;
-    const int ne = min(EPB, a.E - e0);
-    // ---- phase A: stage the tile, per-node data into LDS ----
-    if constexpr (!PERSIST) {
-      tile_copy<T, NT>(s_d, a.u + (size_t)e0 * NN * V, ne * NN * V, vec_ok);
-    } else if constexpr (vec_ok) {
-      float4 *d4 = reinterpret_cast<float4 *>(s_d);
-      const int c4 = ne * NN * V * (int)sizeof(T) / 16;
+  // ---- phase A: stage the tile and the per-element data into LDS ----
+  {
+    // lanes past the last chunk rewrite that chunk with the value it holds
+    // (an unconditional store keeps the loads above where they are issued)
+    C *dst = reinterpret_cast<C *>(s_d);
 #pragma unroll
-      for (int r = 0; r < PF; ++r) {
-        const int i = tid + r * NT;
-        if (i < c4) d4[i] = pf[r];
-      }
-    } else {
+    for (int r = 0; r < RPT; ++r) dst[min(tid + r * NT, nch - 1)] = rc[r];
+    if (!full) {  // the bytes of a partial tile past its last whole chunk
       const T *src = a.u + (size_t)e0 * NN * V;
-      for (int i = tid; i < ne * NN * V; i += NT) s_d[i] = src[i];
+      for (int i = nch * (TIO::CB / (int)sizeof(T)) + tid; i < ne * NN * V; i += NT) s_d[i] = src[i];
     }
-    // neighbour face states of this tile (consumed in phase B), then the
-    // next tile's state (consumed in the next iteration)
-    LineCtx<T, V> lc;
-    if (grp == 0) line_setup<T, D, N, EPB, 0>(a, e0, ne, ll, lc);
-    else if (grp == 1) line_setup<T, D, N, EPB, 1>(a, e0, ne, ll, lc);
-    else if constexpr (D == 3) line_setup<T, D, N, EPB, 2>(a, e0, ne, ll, lc);
-    const int next = PERSIST ? tile + (int)gridDim.x : ntiles;
-    if constexpr (PERSIST) {
-      if (next < ntiles) prefetch(next);
-    }
-
-    if constexpr (VT::HG) {
-      if (tid < ne) {
-        const int e = e0 + tid;
-        double al = a.alpha_raw[e];
-        if (a.alpha_smooth) {
-          // apply_smoothing! (indicators_2d.jl:104-135, indicators_3d.jl:125-173):
-          // alpha_e = max(alpha_e, 0.5 alpha_nbr) over the element's interfaces
-          double m = al;
+  }
+  if constexpr (VT::HG) {
+    if (tid < ne) {
+      const int e = e0 + tid;
+      double al = a.alpha_raw[e];
+      if (a.alpha_smooth) {
+        // apply_smoothing! (indicators_2d.jl:104-135, indicators_3d.jl:125-173):
+        // alpha_e = max(alpha_e, 0.5 alpha_nbr) over the element's interfaces
+        double m = al;
 #pragma unroll
-          for (int dir = 0; dir < 2 * D; ++dir) {
-            double an = 0.5 * a.alpha_raw[a.nbr[dir + 2 * D * e]];
-            m = an > m ? an : m;
-          }
-          al = m;
+        for (int dir = 0; dir < 2 * D; ++dir) {
+          double an = 0.5 * a.alpha_raw[a.nbr[dir + 2 * D * e]];
+          m = an > m ? an : m;
         }
-        s_alpha[tid] = al;
-        if (a.alpha_out) a.alpha_out[e] = al;
+        al = m;
       }
+      s_alpha[tid] = al;
+      if (a.alpha_out) a.alpha_out[e] = al;
     }
-    if (a.source && tid < ne * 2 * n) source_tables<T, D, N>(a, k, e0 + tid / (2 * n), tid % (2 * n), tt, &s_src[(tid / (2 * n)) * 4 * n]);
-    __syncthreads();
-    if (grp == 0) line_own<T, D, N, EPB, 0>(s_d, lc);
-    else if (grp == 1) line_own<T, D, N, EPB, 1>(s_d, lc);
-    else if constexpr (D == 3) line_own<T, D, N, EPB, 2>(s_d, lc);
-    {
-      const T gm1 = a.gamma - T(1);
-      for (int q = tid; q < ne * NN; q += NT) {
-        const int p = LY::phys(q);
-        T uq[V];
+  }
+  if (tid < ne * 2 * n) {
+    const int idx = tid - sel * 2 * n;
+    if (idx == 0) s_fac[sel] = T(-ij);
+    if (a.source) {
+      double csum = 0;
 #pragma unroll
-        for (int v = 0; v < V; ++v) uq[v] = s_d[q * V + v];
-        if constexpr (VT::CONS) {
-#pragma unroll
-          for (int v = 0; v < V; ++v) s_c[v * SZ + p] = uq[v];
-        }
-        if constexpr (VT::RAN) {
-          NodeR<T, D> w;
-          nodeR_from_cons<T, D>(uq, gm1, w);
-          s_w[0 * SZ + p] = w.rho;
-          s_w[1 * SZ + p] = w.lrho;
-          s_w[2 * SZ + p] = w.beta;
-          s_w[3 * SZ + p] = w.lbeta;
-#pragma unroll
-          for (int d = 0; d < D; ++d) s_w[(4 + d) * SZ + p] = w.h[d];
-          s_w[(4 + D) * SZ + p] = w.hp;
-        }
-      }
+      for (int d = 0; d < D; ++d) csum += cen[d];
+      source_table_entry<T, N>(k, idx, ij, csum, tt, &s_src[sel * 4 * n]);
     }
-    __syncthreads();
-
-    // ---- phase B: one direction per wave group, one line per lane ----
-    if (grp == 0) line_work<T, D, N, VOL, EPB, 0>(a, k, lc, s_w, s_c, s_d, s_alpha);
-    else if (grp == 1) line_work<T, D, N, VOL, EPB, 1>(a, k, lc, s_w, s_c, s_d, s_alpha);
-    else if constexpr (D == 3) line_work<T, D, N, VOL, EPB, 2>(a, k, lc, s_w, s_c, s_d, s_alpha);
-    __syncthreads();
-
-    // ---- phase C: sum directions, Jacobian, sources; stage the output ----
+  }
+  __syncthreads();
+  if (grp == 0) line_own<T, D, N, EPB, 0>(s_d, lc);
+  else if (grp == 1) line_own<T, D, N, EPB, 1>(s_d, lc);
+  else if constexpr (D == 3) line_own<T, D, N, EPB, 2>(s_d, lc);
+  {
+    const T gm1 = a.gamma - T(1);
     for (int q = tid; q < ne * NN; q += NT) {
-      const int el = q / NN, node = q - el * NN;
-      const int e = e0 + el;
       const int p = LY::phys(q);
-      const T fac = T(-a.inv_jac[e]);
-      T d[V];
+      T uq[V];
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        T sum = s_d[(0 * V + v) * SZ + p];
+      for (int v = 0; v < V; ++v) uq[v] = s_d[q * V + v];
+      if constexpr (VT::CONS) {
 #pragma unroll
-        for (int o = 1; o < D; ++o) sum += s_d[(o * V + v) * SZ + p];
-        d[v] = sum * fac;
+        for (int v = 0; v < V; ++v) s_c[v * SZ + p] = uq[v];
       }
-      if (a.source) {
-        // theta = alpha_i + gamma_j (+ gamma_k) by angle addition
-        const T *tb = &s_src[el * 4 * n];
-        const int i0 = node % n, i1 = (node / n) % n;
-        T si = tb[2 * i0], co = tb[2 * i0 + 1];
-        T sg = tb[2 * (n + i1)], cg = tb[2 * (n + i1) + 1];
-        T s2 = tfma(si, cg, co * sg), c2 = tfma(co, cg, -si * sg);
-        if constexpr (D == 3) {
-          const int i2 = node / (n * n);
-          sg = tb[2 * (n + i2)];
-          cg = tb[2 * (n + i2) + 1];
-          si = tfma(s2, cg, c2 * sg);
-          co = tfma(c2, cg, -s2 * sg);
-        } else {
-          si = s2;
-          co = c2;
-        }
-        T s[V];
-        source_convergence_sc<T, D>(si, co, a.gamma, s);
+      if constexpr (VT::RAN) {
+        NodeR<T, D> w;
+        nodeR_from_cons<T, D>(uq, gm1, w);
+        s_w[0 * SZ + p] = w.rho;
+        s_w[1 * SZ + p] = w.lrho;
+        s_w[2 * SZ + p] = w.beta;
+        s_w[3 * SZ + p] = w.lbeta;
 #pragma unroll
-        for (int v = 0; v < V; ++v) d[v] += s[v];
+        for (int d = 0; d < D; ++d) s_w[(4 + d) * SZ + p] = w.h[d];
+        s_w[(4 + D) * SZ + p] = w.hp;
       }
-#pragma unroll
-      for (int v = 0; v < V; ++v) s_o[q * V + v] = d[v];
     }
-    __syncthreads();
-    tile_copy<T, NT>(a.du + (size_t)e0 * NN * V, s_o, ne * NN * V, vec_ok);
-    if (!PERSIST || next >= ntiles) break;
-    tile = next;
-    // region A (s_o) is read by the store above and written again by phase A of
-    // the next tile only after the next barrier; region B is free now.
+  }
+  __syncthreads();
+
+  // ---- phase B: one direction per wave group, one line per lane ----
+  if (grp == 0) line_work<T, D, N, VOL, EPB, 0>(a, k, lc, s_w, s_c, s_d, s_alpha);
+  else if (grp == 1) line_work<T, D, N, VOL, EPB, 1>(a, k, lc, s_w, s_c, s_d, s_alpha);
+  else if constexpr (D == 3) line_work<T, D, N, VOL, EPB, 2>(a, k, lc, s_w, s_c, s_d, s_alpha);
+  __syncthreads();
+
+  // ---- phase C: sum directions, Jacobian, sources; stage the output ----
+  for (int q = tid; q < ne * NN; q += NT) {
+    const int el = q / NN, node = q - el * NN;
+    const int p = LY::phys(q);
+    const T fac = s_fac[el];
+    T d[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      T sum = s_d[(0 * V + v) * SZ + p];
+#pragma unroll
+      for (int o = 1; o < D; ++o) sum += s_d[(o * V + v) * SZ + p];
+      d[v] = sum * fac;
+    }
+    if (a.source) {
+      // theta = alpha_i + gamma_j (+ gamma_k) by angle addition
+      const T *tb = &s_src[el * 4 * n];
+      const int i0 = node % n, i1 = (node / n) % n;
+      T si = tb[2 * i0], co = tb[2 * i0 + 1];
+      T sg = tb[2 * (n + i1)], cg = tb[2 * (n + i1) + 1];
+      T s2 = tfma(si, cg, co * sg), c2 = tfma(co, cg, -si * sg);
+      if constexpr (D == 3) {
+        const int i2 = node / (n * n);
+        sg = tb[2 * (n + i2)];
+        cg = tb[2 * (n + i2) + 1];
+        si = tfma(s2, cg, c2 * sg);
+        co = tfma(c2, cg, -s2 * sg);
+      } else {
+        si = s2;
+        co = c2;
+      }
+      T s[V];
+      source_convergence_sc<T, D>(si, co, a.gamma, s);
+#pragma unroll
+      for (int v = 0; v < V; ++v) d[v] += s[v];
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) s_o[q * V + v] = d[v];
+  }
+  __syncthreads();
+  if (full) {
+    const C *src = reinterpret_cast<const C *>(s_o);
+    C *dst = reinterpret_cast<C *>(a.du + (size_t)e0 * NN * V);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int i = tid + r * NT;
+      if (i < TIO::NCH) dst[i] = src[i];
+    }
+  } else {
+    T *dst = a.du + (size_t)e0 * NN * V;
+    for (int i = tid; i < ne * NN * V; i += NT) dst[i] = s_o[i];
   }
 }
 

@@ -15,6 +15,7 @@
 #include "../../include/trixi_hip.h"
 #include "dg_aux.hpp"
 #include "dg_element.hpp"
+#include "mlp.hpp"
 
 using namespace trixi_hip;
 
@@ -33,11 +34,8 @@ struct ElemKey {
 // Type-erased launcher: args and constants are passed as raw blobs that the
 // instantiation reinterprets as ElemArgs<T> / DGConst<T, N>.
 using ElemLaunch = hipError_t (*)(const void *args, const void *kconst, int nblocks, hipStream_t s);
-using ElemOccupancy = int (*)();  // resident workgroups per CU
-
 struct ElemEntry {
   ElemLaunch launch;
-  ElemOccupancy occupancy;
 };
 
 std::map<ElemKey, ElemEntry> &elem_registry() {
@@ -45,30 +43,18 @@ std::map<ElemKey, ElemEntry> &elem_registry() {
   return r;
 }
 
-template <typename T, int D, int N, int VOL, int EPB, int WPD, bool P>
+template <typename T, int D, int N, int VOL, int EPB, int WPD>
 hipError_t launch_elem(const void *args, const void *kconst, int nblocks, hipStream_t s) {
   const ElemArgs<T> &a = *static_cast<const ElemArgs<T> *>(args);
   const DGConst<T, N> &k = *static_cast<const DGConst<T, N> *>(kconst);
-  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, WPD, P>), dim3(nblocks), dim3(D * WPD * 64), 0, s, a,
-                     k);
+  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, WPD>), dim3(nblocks), dim3(D * WPD * 64), 0, s, a, k);
   return hipGetLastError();
 }
 
-// persistent variants run a grid of resident workgroups; the others one tile per workgroup
-template <typename T, int D, int N, int VOL, int EPB, int WPD, bool P> int occupancy_elem() {
-  if (!P) return 0;
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dg_element_kernel<T, D, N, VOL, EPB, WPD, P>, D * WPD * 64,
-                                                   0) != hipSuccess)
-    return 1;
-  return nb > 0 ? nb : 1;
-}
-
-// registry key: a persistent variant is keyed with a negative epb
-template <typename T, int D, int N, int VOL, int EPB, int WPD, bool P = false> struct Reg {
+template <typename T, int D, int N, int VOL, int EPB, int WPD> struct Reg {
   Reg() {
-    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, P ? -EPB : EPB, D * WPD * 64}] =
-        ElemEntry{&launch_elem<T, D, N, VOL, EPB, WPD, P>, &occupancy_elem<T, D, N, VOL, EPB, WPD, P>};
+    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, D * WPD * 64}] =
+        ElemEntry{&launch_elem<T, D, N, VOL, EPB, WPD>};
   }
 };
 
@@ -106,8 +92,6 @@ TRIXI_REG(double, 3, 4, VOL_FD_RANOCHA, 5, 2)
 TRIXI_REG(float, 2, 3, VOL_HG_RANOCHA, 32, 2)
 TRIXI_REG(double, 2, 3, VOL_HG_RANOCHA, 32, 2)
 TRIXI_REG(float, 2, 3, VOL_FD_RANOCHA, 32, 2)
-static Reg<float, 3, 3, VOL_FD_RANOCHA, 4, 1, true> reg_persist_f_3_3;
-static Reg<double, 3, 3, VOL_FD_RANOCHA, 4, 1, true> reg_persist_d_3_3;
 
 struct DefaultTile {
   int D, N, epb, nt;
@@ -141,6 +125,7 @@ template <typename F> bool dispatch_dn(int D, int N, F &&f) {
 }
 
 constexpr int kMaxPartials = 1 << 16;
+constexpr int kMlpMaxWaves = 1024;  // partial gradient rows of one training step
 
 }  // namespace
 
@@ -179,7 +164,6 @@ struct trixi_hip_ctx {
   // element-kernel launch
   ElemLaunch elem = nullptr;
   int epb = 0, nt = 0;
-  int grid_cap = 0;  // resident workgroups on the device (persistent element kernel)
   std::vector<unsigned char> kconst;  // DGConst<T, N> blob
   int xcd_remap = 0;  // measured: the round-robin XCD placement is faster here
 
@@ -187,6 +171,15 @@ struct trixi_hip_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<double> graph_key;
+
+  // blending factor source for shock capturing (TRIXI_HIP_IND_*)
+  int ind_mode = TRIXI_HIP_IND_HG;
+  // learned indicator (2D p3): d_mlp = weights [NP] | Adam m [NP] | Adam v [NP]
+  // | feature box lo [NF] | hi [NF] | gradient + loss [NPL]
+  float *d_mlp = nullptr, *d_feat = nullptr, *d_mlp_partial = nullptr;
+  double *d_alpha_hg = nullptr;
+  bool mlp_ready = false;
+  int mlp_step = 0;
 };
 
 namespace {
@@ -238,32 +231,54 @@ int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
   ctx->elem = it->second.launch;
   ctx->epb = epb;
   ctx->nt = nt;
-  // persistent grid: every workgroup resident at once, looping over tiles
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
-    cus = 256;
-  const int occ = it->second.occupancy();
-  ctx->grid_cap = occ > 0 ? cus * occ : 0;
-  ctx->epb = epb < 0 ? -epb : epb;
+  return TRIXI_HIP_OK;
+}
+
+// HG indicator of state u: raw alpha into alpha_out; with feat != null (2D p3)
+// also the learned indicator's features.
+template <typename T> int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat) {
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    constexpr int NN = ipow_c(N + 1, D);
+    constexpr int IPB = NN >= 256 ? 1 : 256 / NN;
+    VinvConst<N> vc{};
+    for (int i = 0; i < (N + 1) * (N + 1); ++i) vc.Vinv[i] = ctx->Vinv[i];
+    int nb = (ctx->E + IPB - 1) / IPB;
+    hipLaunchKernelGGL((hg_indicator_kernel<T, D, N, IPB, 256>), dim3(nb), dim3(256), 0, ctx->stream, u,
+                       alpha_out, ctx->E, ctx->hg, vc, feat);
+  });
+  if (!ok) return set_err(ctx, TRIXI_HIP_EINVAL, "unsupported (ndims, polydeg) for the indicator");
+  HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
+
+int mlp_blocks(int n) {
+  const int ntiles = (n + 15) / 16;
+  const int waves = std::min(std::max(ntiles, 1), kMlpMaxWaves);
+  return (waves + mlp::WAVES - 1) / mlp::WAVES;
+}
+
+// Raw (unsmoothed) blending factors of state u into d_alpha_raw, from the
+// indicator selected by ind_mode.
+template <typename T> int launch_indicator_raw(trixi_hip_ctx *ctx, const T *u) {
+  if (ctx->ind_mode == TRIXI_HIP_IND_HG) return launch_hg<T>(ctx, u, ctx->d_alpha_raw, nullptr);
+  int r = launch_hg<T>(ctx, u, ctx->d_alpha_hg, ctx->d_feat);
+  if (r) return r;
+  const float *lo = ctx->d_mlp + 3 * mlp::NP, *hi = lo + mlp::NF;
+  hipLaunchKernelGGL(mlp::mlp_infer_kernel, dim3(mlp_blocks(ctx->E)), dim3(mlp::NTHR), 0, ctx->stream, ctx->d_mlp,
+                     ctx->d_feat, lo, hi, ctx->d_alpha_hg, ctx->cfg.alpha_max,
+                     ctx->ind_mode == TRIXI_HIP_IND_MLP_WITH_FALLBACK ? 1 : 0, ctx->E, ctx->d_alpha_raw,
+                     (float *)nullptr);
+  HIP_TRY(hipGetLastError());
   return TRIXI_HIP_OK;
 }
 
 template <typename T>
 int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *tdev) {
-  // HG: raw indicator first (needs the whole state), smoothing happens inline
+  // shock capturing: raw indicator first (needs the whole state), smoothing happens inline
   if (ctx->vol == VOL_HG_RANOCHA || ctx->vol == VOL_HG_CENTRAL) {
-    bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
-      constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
-      constexpr int NN = ipow_c(N + 1, D);
-      constexpr int IPB = NN >= 256 ? 1 : 256 / NN;
-      VinvConst<N> vc{};
-      for (int i = 0; i < (N + 1) * (N + 1); ++i) vc.Vinv[i] = ctx->Vinv[i];
-      int nb = (ctx->E + IPB - 1) / IPB;
-      hipLaunchKernelGGL((hg_indicator_kernel<T, D, N, IPB, 256>), dim3(nb), dim3(256), 0,
-                         ctx->stream, u, ctx->d_alpha_raw, ctx->E, ctx->hg, vc);
-    });
-    if (!ok) return set_err(ctx, TRIXI_HIP_EINVAL, "unsupported (ndims, polydeg) for the indicator");
-    HIP_TRY(hipGetLastError());
+    int r = launch_indicator_raw<T>(ctx, u);
+    if (r) return r;
   }
   ElemArgs<T> a{};
   a.u = u;
@@ -283,7 +298,6 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   a.xcd_remap = ctx->xcd_remap;
   a.gamma = T(ctx->cfg.gamma);
   int nblocks = (ctx->E + ctx->epb - 1) / ctx->epb;
-  if (ctx->grid_cap > 0 && nblocks > ctx->grid_cap) nblocks = ctx->grid_cap;
   hipError_t e = ctx->elem(&a, ctx->kconst.data(), nblocks, ctx->stream);
   if (e != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("element kernel launch: ") + hipGetErrorString(e));
   return TRIXI_HIP_OK;
@@ -448,7 +462,8 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_u, ctx->d_utmp, ctx->d_u32, ctx->d_du,
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
-                  ctx->d_partial, ctx->d_scalars};
+                  ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
+                  ctx->d_alpha_hg};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -622,20 +637,16 @@ int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt) {
 }
 
 int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host) {
-  if (!ctx || mode != TRIXI_HIP_IND_HG) return TRIXI_HIP_EINVAL;
-  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
-    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
-    constexpr int NN = ipow_c(N + 1, D);
-    constexpr int IPB = NN >= 256 ? 1 : 256 / NN;
-    VinvConst<N> vc{};
-    for (int i = 0; i < (N + 1) * (N + 1); ++i) vc.Vinv[i] = ctx->Vinv[i];
-    int nb = (ctx->E + IPB - 1) / IPB;
-    if (ctx->fp32)
-      hipLaunchKernelGGL((hg_indicator_kernel<float, D, N, IPB, 256>), dim3(nb), dim3(256), 0, ctx->stream,
-                         ctx->d_u32, ctx->d_alpha_raw, ctx->E, ctx->hg, vc);
-    else
-      hipLaunchKernelGGL((hg_indicator_kernel<double, D, N, IPB, 256>), dim3(nb), dim3(256), 0, ctx->stream,
-                         ctx->d_u, ctx->d_alpha_raw, ctx->E, ctx->hg, vc);
+  if (!ctx || mode < TRIXI_HIP_IND_HG || mode > TRIXI_HIP_IND_MLP_WITH_FALLBACK) return TRIXI_HIP_EINVAL;
+  if (mode != TRIXI_HIP_IND_HG && !ctx->mlp_ready)
+    return set_err(ctx, TRIXI_HIP_ESTATE, "the learned indicator has no parameters (trixi_hip_mlp_set_params)");
+  const int saved = ctx->ind_mode;
+  ctx->ind_mode = mode;
+  int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, ctx->d_u32) : launch_indicator_raw<double>(ctx, ctx->d_u);
+  ctx->ind_mode = saved;
+  if (r) return r;
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto) {
+    constexpr int D = decltype(Dc)::value;
     hipLaunchKernelGGL((hg_smooth_kernel<D>), dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
                        ctx->d_alpha_raw, ctx->d_nbr, ctx->d_alpha, ctx->E, ctx->cfg.alpha_smooth);
   });
@@ -643,6 +654,133 @@ int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host) {
   HIP_TRY(hipGetLastError());
   if (alpha_host) {
     HIP_TRY(hipMemcpyAsync(alpha_host, ctx->d_alpha, ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  return TRIXI_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned indicator
+// ---------------------------------------------------------------------------
+static int mlp_check(trixi_hip_ctx *ctx) {
+  if (ctx->D != 2 || ctx->N != 3)
+    return set_err(ctx, TRIXI_HIP_EINVAL, "the learned indicator is defined for 2D elements of polydeg 3");
+  if (ctx->d_mlp) return TRIXI_HIP_OK;
+  const size_t nmlp = 3 * (size_t)mlp::NP + 2 * mlp::NF + mlp::NPL;
+  HIP_TRY(hipMalloc(&ctx->d_mlp, nmlp * 4));
+  HIP_TRY(hipMalloc(&ctx->d_feat, (size_t)ctx->E * kMlpFeatureStride * 4));
+  HIP_TRY(hipMalloc(&ctx->d_alpha_hg, (size_t)ctx->E * 8));
+  HIP_TRY(hipMalloc(&ctx->d_mlp_partial, (size_t)kMlpMaxWaves * mlp::NPL * 4));
+  return TRIXI_HIP_OK;
+}
+
+static float *mlp_grad_buffer(trixi_hip_ctx *ctx) { return ctx->d_mlp + 3 * mlp::NP + 2 * mlp::NF; }
+
+int trixi_hip_mlp_set_params(trixi_hip_ctx *ctx, const float *params, const float *feature_lo,
+                             const float *feature_hi) {
+  if (!ctx || !params || !feature_lo || !feature_hi) return TRIXI_HIP_EINVAL;
+  int r = mlp_check(ctx);
+  if (r) return r;
+  std::vector<float> h(3 * (size_t)mlp::NP + 2 * mlp::NF, 0.f);
+  std::memcpy(h.data(), params, mlp::NP * 4);
+  std::memcpy(h.data() + 3 * mlp::NP, feature_lo, mlp::NF * 4);
+  std::memcpy(h.data() + 3 * mlp::NP + mlp::NF, feature_hi, mlp::NF * 4);
+  HIP_TRY(hipMemcpyAsync(ctx->d_mlp, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->mlp_ready = true;
+  ctx->mlp_step = 0;
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_mlp_get_params(trixi_hip_ctx *ctx, float *params) {
+  if (!ctx || !params) return TRIXI_HIP_EINVAL;
+  if (!ctx->mlp_ready) return set_err(ctx, TRIXI_HIP_ESTATE, "no parameters set");
+  HIP_TRY(hipMemcpyAsync(params, ctx->d_mlp, mlp::NP * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_set_indicator(trixi_hip_ctx *ctx, int mode) {
+  if (!ctx || mode < TRIXI_HIP_IND_HG || mode > TRIXI_HIP_IND_MLP_WITH_FALLBACK) return TRIXI_HIP_EINVAL;
+  if (mode != TRIXI_HIP_IND_HG && !ctx->mlp_ready)
+    return set_err(ctx, TRIXI_HIP_ESTATE, "the learned indicator has no parameters (trixi_hip_mlp_set_params)");
+  if (mode != ctx->ind_mode && ctx->graph_exec) {  // the captured step launches the indicator kernels
+    (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->graph_exec = nullptr;
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph = nullptr;
+    ctx->graph_key.clear();
+  }
+  ctx->ind_mode = mode;
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_mlp_features(trixi_hip_ctx *ctx, float *features_host, double *alpha_hg_host) {
+  if (!ctx) return TRIXI_HIP_EINVAL;
+  int r = mlp_check(ctx);
+  if (r) return r;
+  r = ctx->fp32 ? launch_hg<float>(ctx, ctx->d_u32, ctx->d_alpha_hg, ctx->d_feat)
+                : launch_hg<double>(ctx, ctx->d_u, ctx->d_alpha_hg, ctx->d_feat);
+  if (r) return r;
+  if (features_host)
+    HIP_TRY(hipMemcpyAsync(features_host, ctx->d_feat, (size_t)ctx->E * kMlpFeatureStride * 4,
+                           hipMemcpyDeviceToHost, ctx->stream));
+  if (alpha_hg_host)
+    HIP_TRY(hipMemcpyAsync(alpha_hg_host, ctx->d_alpha_hg, (size_t)ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_mlp_forward(trixi_hip_ctx *ctx, const float *x_dev, int n, float *y_dev) {
+  if (!ctx || !x_dev || !y_dev || n <= 0) return TRIXI_HIP_EINVAL;
+  if (!ctx->mlp_ready) return set_err(ctx, TRIXI_HIP_ESTATE, "no parameters set");
+  if (((uintptr_t)x_dev & 15) != 0) return set_err(ctx, TRIXI_HIP_EINVAL, "features must be 16-byte aligned");
+  const float *lo = ctx->d_mlp + 3 * mlp::NP, *hi = lo + mlp::NF;
+  hipLaunchKernelGGL(mlp::mlp_infer_kernel, dim3(mlp_blocks(n)), dim3(mlp::NTHR), 0, ctx->stream, ctx->d_mlp, x_dev,
+                     lo, hi, (const double *)nullptr, 0.0, 0, n, (double *)nullptr, y_dev);
+  HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
+
+static int mlp_gradient(trixi_hip_ctx *ctx, const float *x_dev, const float *target_dev, int n) {
+  const int nb = mlp_blocks(n);
+  hipLaunchKernelGGL(mlp::mlp_grad_kernel, dim3(nb), dim3(mlp::NTHR), 0, ctx->stream, ctx->d_mlp, x_dev, target_dev,
+                     n, ctx->d_mlp_partial);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(mlp::mlp_reduce_kernel, dim3((mlp::NPL + 255) / 256), dim3(256), 0, ctx->stream,
+                     ctx->d_mlp_partial, nb * mlp::WAVES, mlp_grad_buffer(ctx));
+  HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_mlp_loss_grad(trixi_hip_ctx *ctx, const float *x_dev, const float *target_dev, int n, float *loss,
+                            float *grad_host) {
+  if (!ctx || !x_dev || !target_dev || n <= 0) return TRIXI_HIP_EINVAL;
+  if (!ctx->mlp_ready) return set_err(ctx, TRIXI_HIP_ESTATE, "no parameters set");
+  if (((uintptr_t)x_dev & 15) != 0) return set_err(ctx, TRIXI_HIP_EINVAL, "features must be 16-byte aligned");
+  int r = mlp_gradient(ctx, x_dev, target_dev, n);
+  if (r) return r;
+  if (grad_host)
+    HIP_TRY(hipMemcpyAsync(grad_host, mlp_grad_buffer(ctx), mlp::NP * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (loss) HIP_TRY(hipMemcpyAsync(loss, mlp_grad_buffer(ctx) + mlp::NP, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float *target_dev, int n,
+                             float learning_rate, float *loss) {
+  if (!ctx || !x_dev || !target_dev || n <= 0 || !(learning_rate > 0.f)) return TRIXI_HIP_EINVAL;
+  if (!ctx->mlp_ready) return set_err(ctx, TRIXI_HIP_ESTATE, "no parameters set");
+  if (((uintptr_t)x_dev & 15) != 0) return set_err(ctx, TRIXI_HIP_EINVAL, "features must be 16-byte aligned");
+  int r = mlp_gradient(ctx, x_dev, target_dev, n);
+  if (r) return r;
+  ctx->mlp_step += 1;
+  hipLaunchKernelGGL(mlp::mlp_adam_kernel, dim3((mlp::NP + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_mlp,
+                     ctx->d_mlp + mlp::NP, ctx->d_mlp + 2 * mlp::NP, mlp_grad_buffer(ctx), learning_rate, 0.9f,
+                     0.999f, 1e-8f, ctx->mlp_step);
+  HIP_TRY(hipGetLastError());
+  if (loss) {
+    HIP_TRY(hipMemcpyAsync(loss, mlp_grad_buffer(ctx) + mlp::NP, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
   }
   return TRIXI_HIP_OK;

@@ -40,6 +40,12 @@ IC_KELVIN_HELMHOLTZ = 4
 IC_FOURIER_MODES = 5
 
 IND_HG = 0
+IND_MLP = 1
+IND_MLP_WITH_FALLBACK = 2
+
+MLP_NPARAMS = 5505
+MLP_NFEATURES = 19
+MLP_FEATURE_STRIDE = 32
 
 HOST = 0
 DEVICE = 1
@@ -94,6 +100,7 @@ _D = ctypes.c_double
 _I = ctypes.c_int
 _PD = ctypes.POINTER(ctypes.c_double)
 _PI = ctypes.POINTER(ctypes.c_int)
+_F = ctypes.c_float
 SIGNATURES = [
     ("trixi_hip_create", _I, [ctypes.POINTER(Config), ctypes.POINTER(Mesh), ctypes.POINTER(Basis), ctypes.POINTER(_P)]),
     ("trixi_hip_destroy", _I, [_P]),
@@ -114,6 +121,13 @@ SIGNATURES = [
     ("trixi_hip_error_norms", _I, [_P, _D, _PD, _PD]),
     ("trixi_hip_integrals", _I, [_P, _PD, _PD]),
     ("trixi_hip_get_tile", _I, [_P, _PI, _PI]),
+    ("trixi_hip_mlp_set_params", _I, [_P, _P, _P, _P]),
+    ("trixi_hip_mlp_get_params", _I, [_P, _P]),
+    ("trixi_hip_set_indicator", _I, [_P, _I]),
+    ("trixi_hip_mlp_features", _I, [_P, _P, _P]),
+    ("trixi_hip_mlp_forward", _I, [_P, _P, _I, _P]),
+    ("trixi_hip_mlp_loss_grad", _I, [_P, _P, _P, _I, _P, _P]),
+    ("trixi_hip_mlp_train_step", _I, [_P, _P, _P, _I, _F, _P]),
 ]
 
 _lib = None
