@@ -1,0 +1,210 @@
+"""Learned indicator on the GPU: MFMA forward and gradient against the FP64
+reference (with the kernel's BF16 rounding points emulated), features against
+the reference definition, fallback semantics, training, and the RHS with the
+learned indicator. Tolerances: the kernels accumulate in FP32 and use the
+hardware exp, the reference in FP64."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _semi(level=3, seed=0, precision="fp64"):
+    from trixi_hip.learned_indicator import khi_semidiscretization
+    return khi_semidiscretization(level, 1e-3, seed, precision)
+
+
+def _params(seed, scale=1.0):
+    from trixi_hip.learned_indicator import init_params
+    p = init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 100)
+    from oracle import mlp_ref
+    for k in ("b1", "b2"):
+        o, s = mlp_ref.OFF[k]
+        p[o:o + s[0]] = rng.normal(0, 0.2, s[0])
+    return (p * scale).astype(np.float32)
+
+
+def _features(rng, n):
+    x = np.zeros((n, 32), dtype=np.float32)
+    x[:, :16] = rng.uniform(-1, 1, (n, 16))
+    x[:, 16:18] = rng.uniform(-2, 2, (n, 2))
+    x[:, 18] = rng.uniform(-2, 0, n)
+    return x
+
+
+def test_forward_matches_reference():
+    import torch
+    from oracle import mlp_ref
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork, forward
+    semi = _semi()
+    rng = np.random.default_rng(0)
+    p = _params(1)
+    IndicatorNeuralNetwork(p).upload(semi)
+    n = 1000  # not a multiple of 16: the last tile is partial
+    x = _features(rng, n)
+    y = torch.empty(n, dtype=torch.float32, device="cuda")
+    forward(semi, torch.from_numpy(x).cuda(), y)
+    y = y.cpu().numpy().astype(np.float64)
+    ref_bf = mlp_ref.forward(p, x, bf=True)["y"]
+    ref = mlp_ref.forward(p.astype(np.float64), x)["y"]
+    err = np.abs(y - ref_bf)
+    # FP32 accumulation may move a hidden value across a BF16 rounding boundary
+    assert np.median(err) < 2e-6 and err.max() < 2e-3, (np.median(err), err.max())
+    assert np.abs(y - ref).max() < 3e-2  # BF16 operands vs exact FP64
+    semi.close()
+
+
+def test_gradient_matches_reference():
+    import torch
+    from oracle import mlp_ref
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork, loss_grad
+    semi = _semi()
+    rng = np.random.default_rng(2)
+    p = _params(3)
+    IndicatorNeuralNetwork(p).upload(semi)
+    for n in (16, 1000, 20000):  # one tile; partial tiles; more tiles than gradient rows
+        x = _features(rng, n)
+        t = rng.uniform(0, 1, n).astype(np.float32)
+        loss, g = loss_grad(semi, torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda())
+        lref, gref = mlp_ref.loss_grad(p, x, t, bf=True)
+        assert loss == pytest.approx(lref, rel=2e-4)
+        rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+        assert rel < 5e-3, (n, rel)
+        for k, (o, s) in mlp_ref.OFF.items():  # every block, not only the largest
+            sl = slice(o, o + int(np.prod(s)))
+            assert np.linalg.norm(g[sl] - gref[sl]) <= 1e-2 * np.linalg.norm(gref[sl]) + 1e-7, (n, k)
+    semi.close()
+
+
+def test_features_match_reference():
+    from oracle import mlp_ref
+    from trixi_hip.learned_indicator import features, run_segments
+    semi = _semi(level=4, seed=3)
+    got = {}
+
+    def grab(sm, t):
+        got[t] = features(sm) + (_download(sm),)
+
+    run_segments(semi, 0.4, 2, on_snapshot=grab)
+    b = semi.solver.basis
+    for t, (f, a, u) in got.items():
+        fr, ar = mlp_ref.features(u, 2, 3, b.inverse_vandermonde_legendre, 1.4, 0.002, 0.0001)
+        np.testing.assert_allclose(a, ar, rtol=1e-9, atol=1e-14)
+        np.testing.assert_allclose(f[:, :19], fr[:, :19], atol=2e-6)
+        assert np.all(f[:, 19:] == 0)
+    semi.close()
+
+
+def _download(semi):
+    u = np.empty(semi.nvalues())
+    semi._lib.trixi_hip_download_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p))
+    return u
+
+
+def test_modes_and_fallback():
+    from trixi_hip import _lib
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork, features, indicator_alpha, run_segments
+    semi = _semi(level=4, seed=5)
+    run_segments(semi, 0.3, 2)
+    f, a_hg_raw = features(semi)
+    p = _params(7)
+    # no fallback: alpha = alpha_max * y for every element
+    net = IndicatorNeuralNetwork(p, fallback=False)
+    net.attach(semi)
+    a_nn = indicator_alpha(semi, _lib.IND_MLP)
+    assert np.all(a_nn >= 0) and np.all(a_nn <= 0.002 + 1e-12)
+    # fallback with an empty box: every element takes the HG value
+    IndicatorNeuralNetwork(p, np.full(19, 1.0, np.float32), np.full(19, -1.0, np.float32)).upload(semi)
+    a_fb = indicator_alpha(semi, _lib.IND_MLP_WITH_FALLBACK)
+    a_hg = indicator_alpha(semi, _lib.IND_HG)
+    np.testing.assert_array_equal(a_fb, a_hg)
+    # fallback with an unbounded box: the network everywhere
+    IndicatorNeuralNetwork(p).upload(semi)
+    np.testing.assert_array_equal(indicator_alpha(semi, _lib.IND_MLP_WITH_FALLBACK),
+                                  indicator_alpha(semi, _lib.IND_MLP))
+    # without parameters the learned modes are refused
+    semi2 = _semi(level=2)
+    rc = semi2._lib.trixi_hip_set_indicator(semi2.ctx, _lib.IND_MLP)
+    assert rc == _lib.ESTATE
+    semi2.close()
+    semi.close()
+
+
+def test_rhs_with_learned_indicator():
+    """With every element falling back, the shock-capturing RHS is bitwise the
+    HG one; with the network, the RHS uses the network's alpha (finite, and
+    different where the alphas differ)."""
+    from trixi_hip import _lib
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork, run_segments, use_hg
+    from trixi_hip.semidiscretization import rhs_hyperbolic_
+    semi = _semi(level=4, seed=1)
+    u = run_segments(semi, 0.3, 2)
+    du_hg = np.empty_like(u)
+    rhs_hyperbolic_(du_hg, u, semi, 0.3)
+    p = _params(9)
+    IndicatorNeuralNetwork(p, np.full(19, 1.0, np.float32), np.full(19, -1.0, np.float32)).attach(semi)
+    du_fb = np.empty_like(u)
+    rhs_hyperbolic_(du_fb, u, semi, 0.3)
+    np.testing.assert_array_equal(du_fb, du_hg)
+    IndicatorNeuralNetwork(p, fallback=False).attach(semi)
+    du_nn = np.empty_like(u)
+    rhs_hyperbolic_(du_nn, u, semi, 0.3)
+    assert np.all(np.isfinite(du_nn)) and not np.array_equal(du_nn, du_hg)
+    use_hg(semi)
+    du_back = np.empty_like(u)
+    rhs_hyperbolic_(du_back, u, semi, 0.3)
+    np.testing.assert_array_equal(du_back, du_hg)
+    assert _lib.IND_HG == 0
+    semi.close()
+
+
+def test_training_reduces_loss_deterministically():
+    import torch
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork, init_params, train_step
+    rng = np.random.default_rng(11)
+    n = 8192
+    x = _features(rng, n)
+    t = (1 / (1 + np.exp(-4 * x[:, 16]))).astype(np.float32)  # a smooth target of one feature
+    xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    runs = []
+    for _ in range(2):
+        semi = _semi(level=2)
+        IndicatorNeuralNetwork(init_params(0)).upload(semi)
+        l0 = train_step(semi, xd, td, 1e-3, want_loss=True)
+        for _ in range(300):
+            train_step(semi, xd, td, 3e-3)
+        l1 = train_step(semi, xd, td, 1e-3, want_loss=True)
+        runs.append((l0, l1, IndicatorNeuralNetwork(init_params(0)).download(semi).params.copy()))
+        semi.close()
+    assert runs[0][1] < 0.05 * runs[0][0], runs[0][:2]
+    np.testing.assert_array_equal(runs[0][2], runs[1][2])  # deterministic reduction order
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "trixi.jl_amd", "trixi_hip",
+                                                    "data", "indicator_nn_khi_p3.npz")),
+                    reason="trained weights not present")
+def test_trained_indicator_tracks_hg_on_unseen_khi():
+    """The committed network, on a KHI seed not used in training: alpha fields
+    close to HG's along the run, and the solution close to the HG run."""
+    from trixi_hip import _lib
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork, indicator_alpha, run_segments
+    net = IndicatorNeuralNetwork.load()
+    res = {}
+    for name in ("hg", "nn"):
+        semi = _semi(level=5, seed=1000)
+        if name == "nn":
+            net.attach(semi)
+        alphas = []
+        u = run_segments(semi, 1.0, 5, on_snapshot=lambda sm, t: alphas.append(
+            indicator_alpha(sm, _lib.IND_HG if name == "hg" else net.mode)))
+        res[name] = (u, alphas)
+        semi.close()
+    for a_hg, a_nn in zip(res["hg"][1], res["nn"][1]):
+        assert np.mean(np.abs(a_nn - a_hg)) < 0.05 * 0.002
+    rho_hg = res["hg"][0].reshape(-1, 4)[:, 0]
+    rho_nn = res["nn"][0].reshape(-1, 4)[:, 0]
+    assert np.sqrt(np.mean((rho_nn - rho_hg) ** 2)) < 2e-2

@@ -128,3 +128,29 @@ def test_nan_state_stops_the_run():
     with pytest.raises(FloatingPointError):
         th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=0.5)))
     semi.close()
+
+
+@pytest.mark.gpu
+def test_torch_device_arrays_after_a_library_context():
+    """A process that creates a context before it touches torch can still use
+    torch device arrays with the library (both share one HIP runtime)."""
+    import subprocess
+    import sys
+    from conftest import PKG
+    code = (
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {PKG!r})\n"
+        "import trixi_hip as th\n"
+        "m = th.TreeMesh((0, 0), (2, 2), initial_refinement_level=2)\n"
+        "s = th.SemidiscretizationHyperbolic(m, th.CompressibleEulerEquations2D(1.4),\n"
+        "    th.initial_condition_convergence_test, th.DGSEM(polydeg=3, surface_flux=th.flux_lax_friedrichs))\n"
+        "s.ctx\n"
+        "import torch\n"
+        "u = torch.from_numpy(s.compute_coefficients(0.0)).cuda()\n"
+        "du = torch.empty_like(u)\n"
+        "th.rhs_hyperbolic_(du, u, s, 0.0)\n"
+        "torch.cuda.synchronize()\n"
+        "assert bool(torch.isfinite(du).all())\n"
+        "print('ok')\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]

@@ -1,0 +1,105 @@
+"""The learned indicator's FP64 reference (oracle/mlp_ref.py) pinned on the CPU:
+analytic backward against central differences, BF16 rounding against exact
+cases, features against closed forms, layout constants against the C ABI."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+from oracle import mlp_ref
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.3, mlp_ref.NP)
+    return p
+
+
+def test_layout_constants_match_header():
+    h = open(os.path.join(ROOT, "include", "trixi_hip.h")).read()
+    assert int(re.search(r"TRIXI_HIP_MLP_NPARAMS (\d+)", h).group(1)) == mlp_ref.NP == 5505
+    assert int(re.search(r"TRIXI_HIP_MLP_NFEATURES (\d+)", h).group(1)) == mlp_ref.NF
+    assert int(re.search(r"TRIXI_HIP_MLP_FEATURE_STRIDE (\d+)", h).group(1)) == mlp_ref.KF
+
+
+def test_backward_matches_central_differences():
+    rng = np.random.default_rng(1)
+    p = _params(2)
+    x = rng.uniform(-1, 1, (37, mlp_ref.KF))
+    t = rng.uniform(0, 1, 37)
+    _, g = mlp_ref.loss_grad(p, x, t)
+    idx = np.concatenate([rng.choice(mlp_ref.NP, 60, replace=False), [mlp_ref.NP - 1]])
+    for i in idx:
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (mlp_ref.loss_grad(pp, x, t)[0] - mlp_ref.loss_grad(pm, x, t)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_planted_sign_error_in_backward_is_caught():
+    """A wrong sign in dz2 (a plausible slip) breaks the central-difference check."""
+    rng = np.random.default_rng(3)
+    p = _params(4)
+    x = rng.uniform(-1, 1, (20, mlp_ref.KF))
+    t = rng.uniform(0, 1, 20)
+    _, g = mlp_ref.loss_grad(p, x, t)
+    o = mlp_ref.OFF["W2"][0]
+    h = 1e-6
+    pp, pm = p.copy(), p.copy()
+    pp[o] += h
+    pm[o] -= h
+    fd = (mlp_ref.loss_grad(pp, x, t)[0] - mlp_ref.loss_grad(pm, x, t)[0]) / (2 * h)
+    assert abs(fd - g[o]) < 1e-6 * max(1.0, abs(fd))
+    if abs(fd) > 1e-8:
+        assert abs(fd - (-g[o])) > 1e-6 * abs(fd)
+
+
+def test_bf16_rounding():
+    # exactly representable values are unchanged; ties go to even
+    assert mlp_ref.bf16(1.0) == 1.0
+    assert mlp_ref.bf16(1.0 + 2.0 ** -8) == 1.0           # tie between 1 and 1 + 2^-7 -> even (1.0)
+    assert mlp_ref.bf16(1.0 + 3 * 2.0 ** -8) == 1.0 + 2.0 ** -6  # tie -> even mantissa
+    assert mlp_ref.bf16(1.0 + 2.0 ** -7) == 1.0 + 2.0 ** -7
+    x = np.random.default_rng(5).normal(size=1000)
+    r = mlp_ref.bf16(x)
+    assert np.all(np.abs(r - x) <= 2.0 ** -8 * np.abs(x) * (1 + 1e-7))
+
+
+def test_forward_closed_form():
+    """Zero weights: y = sigmoid(b3) for every input."""
+    p = np.zeros(mlp_ref.NP)
+    p[mlp_ref.OFF["b3"][0]] = 0.7
+    y = mlp_ref.forward(p, np.ones((3, mlp_ref.KF)))["y"]
+    np.testing.assert_allclose(y, 1 / (1 + np.exp(-0.7)), rtol=1e-15)
+
+
+def test_features_closed_forms():
+    """Constant state: centred values 0, no high modes (HG alpha 0). A state
+    whose rho*p carries a large highest-degree mode in x: the first energy
+    ratio is far above the threshold (feature 16 clipped at 2, alpha =
+    alpha_max), and the centred values reach +-1."""
+    from trixi_hip.basis import LobattoLegendreBasis
+    b = LobattoLegendreBasis(3)
+    E = 2
+    u = np.zeros((E, 16, 4))
+    u[..., 0] = 1.2
+    u[..., 1] = 0.1
+    u[..., 2] = -0.2
+    u[..., 3] = 2.5
+    f, a = mlp_ref.features(u.ravel(), 2, 3, b.inverse_vandermonde_legendre, 1.4, 0.5, 0.001)
+    assert np.all(f[:, :16] == 0) and np.all(a == 0)
+    # rho = 1, v = 0, p = 1 + eps * L3(x): rho*p has a highest-mode component in x
+    from numpy.polynomial import legendre
+    L3 = legendre.legval(b.nodes, [0, 0, 0, 1])
+    p = 1.0 + 0.5 * np.tile(L3, 4)
+    u2 = np.zeros((1, 16, 4))
+    u2[..., 0] = 1.0
+    u2[..., 3] = p / 0.4
+    f2, a2 = mlp_ref.features(u2.ravel(), 2, 3, b.inverse_vandermonde_legendre, 1.4, 0.5, 0.001)
+    assert f2[0, 16] == 2.0 and a2[0] == 0.5
+    assert np.max(np.abs(f2[0, :16])) == pytest.approx(1.0, abs=1e-6)

@@ -143,6 +143,13 @@ def load(path=None):
         raise RuntimeError(
             f"libtrixi_hip.so not found at {p}: build it with __graft_entry__.build() "
             "(there is no CPU fallback for the HIP path)")
+    # PyTorch ships its own libamdhip64.so.7 with the same soname as the system
+    # one; whichever is loaded first serves the whole process, and torch's
+    # device arrays only work on its own copy. Load torch first when present.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(p)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)

@@ -1,0 +1,265 @@
+"""Learned troubled-cell indicator for 2D p3 shock capturing.
+
+A 19 -> 64 -> 64 -> 1 perceptron (ReLU, ReLU, sigmoid) whose output times
+alpha_max replaces the Hennemann-Gassner blending factor of
+VolumeIntegralShockCapturingHG (src/solvers/dgsem/indicators.jl:119-156).
+Trixi.jl's own neural-network indicators were moved out of its source tree
+(NEWS.md:495); this one is trained on the HG indicator itself, from seeded
+Kelvin-Helmholtz runs (examples/tree_2d_dgsem/elixir_euler_kelvin_helmholtz_instability.jl),
+so that it is a drop-in for the HG factor.
+
+Inputs (computed on the device from the state, see the HIP kernel):
+  0..15  nodal density*pressure, centred per element and scaled by the largest
+         deviation; 16, 17 the HG logits of the two modal energy ratios / 8,
+         clipped to [-2, 2]; 18 log10 of the smoothest-modes energy share / 8.
+Output: alpha = alpha_max * sigmoid(net). With fallback (the default) an
+element whose inputs leave the range seen in training uses the HG value.
+
+Everything numerical runs in libtrixi_hip.so (BF16 MFMA forward and backward,
+Adam on FP32 master weights). Device arrays are torch tensors (memory only).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+
+NPARAMS = _lib.MLP_NPARAMS
+NFEATURES = _lib.MLP_NFEATURES
+FEATURE_STRIDE = _lib.MLP_FEATURE_STRIDE
+NHIDDEN = 64
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "indicator_nn_khi_p3.npz")
+
+
+def init_params(seed=0):
+    """He-normal weights for the ReLU layers, zero biases (FP32 vector in the
+    C ABI order W1, b1, W2, b2, w3, b3)."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.normal(0.0, np.sqrt(2.0 / NFEATURES), (NHIDDEN, NFEATURES))
+    W2 = rng.normal(0.0, np.sqrt(2.0 / NHIDDEN), (NHIDDEN, NHIDDEN))
+    w3 = rng.normal(0.0, np.sqrt(1.0 / NHIDDEN), NHIDDEN)
+    p = np.concatenate([W1.ravel(), np.zeros(NHIDDEN), W2.ravel(), np.zeros(NHIDDEN), w3, [-2.0]])
+    assert p.size == NPARAMS
+    return p.astype(np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class IndicatorNeuralNetwork:
+    """Learned replacement of IndicatorHennemannGassner's raw blending factor.
+
+    params: FP32 [NPARAMS]; feature_lo/hi: FP32 [NFEATURES] training range;
+    fallback: use the HG factor where an input leaves that range."""
+
+    def __init__(self, params, feature_lo=None, feature_hi=None, fallback=True):
+        self.params = np.ascontiguousarray(params, dtype=np.float32)
+        if self.params.size != NPARAMS:
+            raise ValueError(f"expected {NPARAMS} parameters")
+        big = np.float32(3.0e38)
+        self.feature_lo = np.ascontiguousarray(feature_lo if feature_lo is not None else np.full(NFEATURES, -big),
+                                               dtype=np.float32)
+        self.feature_hi = np.ascontiguousarray(feature_hi if feature_hi is not None else np.full(NFEATURES, big),
+                                               dtype=np.float32)
+        self.fallback = bool(fallback)
+
+    @classmethod
+    def load(cls, path=DEFAULT_WEIGHTS, fallback=True):
+        with np.load(path, allow_pickle=False) as z:
+            return cls(z["params"], z["feature_lo"], z["feature_hi"], fallback)
+
+    def save(self, path, **meta):
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        extra = {k: np.asarray(v) for k, v in meta.items()}
+        np.savez(path, params=self.params, feature_lo=self.feature_lo, feature_hi=self.feature_hi, **extra)
+
+    @property
+    def mode(self):
+        return _lib.IND_MLP_WITH_FALLBACK if self.fallback else _lib.IND_MLP
+
+    def upload(self, semi):
+        """Copy the parameters into the context of `semi` (resets the Adam state)."""
+        lib, ctx = semi._lib, semi.ctx
+        _lib.check(lib.trixi_hip_mlp_set_params(ctx, _fptr(self.params), _fptr(self.feature_lo),
+                                                _fptr(self.feature_hi)), ctx, "mlp_set_params")
+
+    def attach(self, semi):
+        """Make the shock-capturing RHS of `semi` blend with this indicator."""
+        self.upload(semi)
+        _lib.check(semi._lib.trixi_hip_set_indicator(semi.ctx, self.mode), semi.ctx, "set_indicator")
+        return semi
+
+    def download(self, semi):
+        _lib.check(semi._lib.trixi_hip_mlp_get_params(semi.ctx, _fptr(self.params)), semi.ctx, "mlp_get_params")
+        return self
+
+
+def use_hg(semi):
+    _lib.check(semi._lib.trixi_hip_set_indicator(semi.ctx, _lib.IND_HG), semi.ctx, "set_indicator")
+
+
+def features(semi):
+    """Features [E, 32] (FP32) and raw HG alpha [E] of the resident state."""
+    E = semi.nelements()
+    f = np.empty((E, FEATURE_STRIDE), dtype=np.float32)
+    a = np.empty(E, dtype=np.float64)
+    _lib.check(semi._lib.trixi_hip_mlp_features(semi.ctx, _fptr(f), _fptr(a)), semi.ctx, "mlp_features")
+    return f, a
+
+
+def indicator_alpha(semi, mode):
+    """Smoothed blending factors [E] of the resident state for indicator `mode`."""
+    a = np.empty(semi.nelements(), dtype=np.float64)
+    _lib.check(semi._lib.trixi_hip_indicator(semi.ctx, int(mode), _fptr(a)), semi.ctx, "indicator")
+    return a
+
+
+def forward(semi, x_dev, y_dev):
+    """y = sigmoid(net(x)) on device tensors (x: float32 [n, 32], y: float32 [n])."""
+    n = x_dev.shape[0]
+    _lib.check(semi._lib.trixi_hip_mlp_forward(semi.ctx, ctypes.c_void_p(x_dev.data_ptr()), int(n),
+                                               ctypes.c_void_p(y_dev.data_ptr())), semi.ctx, "mlp_forward")
+
+
+def loss_grad(semi, x_dev, t_dev):
+    """(loss, gradient [NPARAMS]) of sum (y - t)^2 / n."""
+    g = np.empty(NPARAMS, dtype=np.float32)
+    loss = ctypes.c_float()
+    _lib.check(semi._lib.trixi_hip_mlp_loss_grad(semi.ctx, ctypes.c_void_p(x_dev.data_ptr()),
+                                                 ctypes.c_void_p(t_dev.data_ptr()), int(x_dev.shape[0]),
+                                                 ctypes.byref(loss), _fptr(g)), semi.ctx, "mlp_loss_grad")
+    return float(loss.value), g
+
+
+def train_step(semi, x_dev, t_dev, learning_rate, want_loss=False):
+    loss = ctypes.c_float()
+    _lib.check(semi._lib.trixi_hip_mlp_train_step(semi.ctx, ctypes.c_void_p(x_dev.data_ptr()),
+                                                  ctypes.c_void_p(t_dev.data_ptr()), int(x_dev.shape[0]),
+                                                  float(learning_rate), ctypes.byref(loss) if want_loss else None),
+               semi.ctx, "mlp_train_step")
+    return float(loss.value) if want_loss else None
+
+
+# ---------------------------------------------------------------------------
+# Kelvin-Helmholtz training data and training
+# ---------------------------------------------------------------------------
+def khi_semidiscretization(level=5, noise_amplitude=1e-3, seed=0, precision="fp64"):
+    """The setup of elixir_euler_kelvin_helmholtz_instability.jl (HG alpha_max
+    0.002, alpha_min 1e-4, flux_ranocha volume flux, LLF with the naive wave
+    speed), with a seeded perturbation of v2 of the given amplitude."""
+    from .equations import (CompressibleEulerEquations2D, FluxLaxFriedrichs, flux_ranocha,
+                            initial_condition_kelvin_helmholtz_instability, max_abs_speed_naive)
+    from .basis import LobattoLegendreBasis
+    from .mesh import TreeMesh
+    from .semidiscretization import SemidiscretizationHyperbolic
+    from .solver import DGSEM, IndicatorHennemannGassner, VolumeIntegralShockCapturingHG
+    eq = CompressibleEulerEquations2D(1.4)
+    mesh = TreeMesh((-1.0, -1.0), (1.0, 1.0), initial_refinement_level=level)
+    surface_flux = FluxLaxFriedrichs(max_abs_speed_naive)
+    basis = LobattoLegendreBasis(3)
+    ind = IndicatorHennemannGassner(eq, basis, alpha_max=0.002, alpha_min=0.0001, alpha_smooth=True)
+    solver = DGSEM(basis, surface_flux,
+                   VolumeIntegralShockCapturingHG(ind, volume_flux_dg=flux_ranocha, volume_flux_fv=surface_flux))
+    ic = initial_condition_kelvin_helmholtz_instability(noise_amplitude, seed)
+    return SemidiscretizationHyperbolic(mesh, eq, ic, solver, precision=precision)
+
+
+def run_segments(semi, t_end, nsnap, cfl=1.3, on_snapshot=None):
+    """Integrate from t = 0 to t_end with CarpenterKennedy2N54 and the CFL
+    StepsizeCallback, stopping at nsnap equally spaced times (including 0) to
+    call on_snapshot(semi, t) on the resident state. Returns the final u."""
+    from .semidiscretization import ODEProblem, rhs_hyperbolic_, semidiscretize
+    from .time_integration import CallbackSet, CarpenterKennedy2N54, StepsizeCallback, solve
+    u = semidiscretize(semi, (0.0, t_end)).u0
+    times = np.linspace(0.0, t_end, nsnap)
+    lib, ctx = semi._lib, semi.ctx
+    _lib.check(lib.trixi_hip_upload_u(ctx, u.ctypes.data_as(ctypes.c_void_p)), ctx, "upload_u")
+    if on_snapshot:
+        on_snapshot(semi, 0.0)
+    for t0, t1 in zip(times[:-1], times[1:]):
+        sol = solve(ODEProblem(rhs_hyperbolic_, u, (t0, t1), semi), CarpenterKennedy2N54(),
+                    callback=CallbackSet(StepsizeCallback(cfl=cfl)))
+        u = sol.u
+        if on_snapshot:
+            on_snapshot(semi, t1)
+    return u
+
+
+def collect_khi_data(seeds, level=5, t_end=3.0, nsnap=21, precision="fp64", log=None):
+    """Features and HG targets (raw alpha / alpha_max) of HG-driven KHI runs."""
+    X, Y = [], []
+    for s in seeds:
+        semi = khi_semidiscretization(level, 1e-3, s, precision)
+
+        def grab(sm, t):
+            f, a = features(sm)
+            X.append(f)
+            Y.append((a / 0.002).astype(np.float32))
+
+        run_segments(semi, t_end, nsnap, on_snapshot=grab)
+        semi.close()
+        if log:
+            log(f"seed {s}: {len(X) * X[-1].shape[0]} samples")
+    return np.concatenate(X), np.concatenate(Y)
+
+
+def train(semi, X, Y, steps=4000, batch=8192, lr=2e-3, seed=0, log=None, log_every=500):
+    """Adam on the GPU from init_params(seed); minibatches drawn with
+    numpy.random.default_rng(seed). Returns the trained IndicatorNeuralNetwork
+    and the loss history [(step, loss)]."""
+    import torch
+    rng = np.random.default_rng(seed)
+    lo = X[:, :NFEATURES].min(axis=0)
+    hi = X[:, :NFEATURES].max(axis=0)
+    net = IndicatorNeuralNetwork(init_params(seed), lo, hi, fallback=True)
+    net.upload(semi)
+    Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
+    Yd = torch.from_numpy(np.ascontiguousarray(Y, dtype=np.float32)).cuda()
+    n = Xd.shape[0]
+    hist = []
+    for it in range(1, steps + 1):
+        idx = torch.from_numpy(rng.integers(0, n, batch)).cuda()
+        xb = Xd.index_select(0, idx).contiguous()
+        yb = Yd.index_select(0, idx).contiguous()
+        rate = lr * (0.5 * (1 + np.cos(np.pi * (it - 1) / steps)))  # cosine decay
+        want = it % log_every == 0 or it == steps
+        loss = train_step(semi, xb, yb, rate, want_loss=want)
+        if want:
+            hist.append((it, loss))
+            if log:
+                log(f"step {it}: loss {loss:.3e}")
+    net.download(semi)
+    return net, hist
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+    p = argparse.ArgumentParser(description="train the learned indicator on seeded KHI runs (needs a GPU)")
+    p.add_argument("--out", default=DEFAULT_WEIGHTS)
+    p.add_argument("--seeds", type=int, default=20)
+    p.add_argument("--level", type=int, default=5)
+    p.add_argument("--t-end", type=float, default=3.0)
+    p.add_argument("--snapshots", type=int, default=21)
+    p.add_argument("--steps", type=int, default=4000)
+    p.add_argument("--batch", type=int, default=8192)
+    p.add_argument("--lr", type=float, default=2e-3)
+    a = p.parse_args(argv)
+
+    def log(msg):
+        print(f"[learned_indicator] {msg}", file=sys.stderr, flush=True)
+
+    X, Y = collect_khi_data(range(a.seeds), a.level, a.t_end, a.snapshots, log=log)
+    log(f"{X.shape[0]} samples, {np.mean(Y > 0):.3f} with alpha > 0")
+    semi = khi_semidiscretization(a.level, 1e-3, 0)
+    net, hist = train(semi, X, Y, a.steps, a.batch, a.lr, log=log)
+    net.save(a.out, loss_history=np.array(hist), seeds=np.arange(a.seeds), level=a.level, t_end=a.t_end)
+    print(json.dumps({"out": a.out, "samples": int(X.shape[0]), "final_loss": hist[-1][1]}))
+
+
+if __name__ == "__main__":
+    main()
