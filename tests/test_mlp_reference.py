@@ -103,3 +103,27 @@ def test_features_closed_forms():
     f2, a2 = mlp_ref.features(u2.ravel(), 2, 3, b.inverse_vandermonde_legendre, 1.4, 0.5, 0.001)
     assert f2[0, 16] == 2.0 and a2[0] == 0.5
     assert np.max(np.abs(f2[0, :16])) == pytest.approx(1.0, abs=1e-6)
+
+
+def test_committed_network_tracks_hg_on_khi_states():
+    """The committed weights, evaluated by the FP64 reference on Kelvin-Helmholtz
+    states of the oracle (level 4, t = 0 and t = 0.4; no noise, unlike the
+    training runs): alpha close to the HG indicator's raw alpha."""
+    from conftest import PKG
+    from oracle import Oracle
+    path = os.path.join(PKG, "trixi_hip", "data", "indicator_nn_khi_p3.npz")
+    if not os.path.exists(path):
+        pytest.skip("trained weights not present")
+    with np.load(path, allow_pickle=False) as z:
+        params = z["params"].astype(np.float64)
+        assert params.shape == (mlp_ref.NP,) and z["feature_lo"].shape == (mlp_ref.NF,)
+    o = Oracle(2, 3, 4, (-1, -1), (1, 1), volume_integral="hg", volume_flux="ranocha", surface_flux="llf_naive",
+               fv_flux="llf_naive", alpha_max=0.002, alpha_min=0.0001, alpha_smooth=True)
+    o.set_initial_condition("kelvin_helmholtz")
+    u = o.compute_coefficients(0.0)
+    vinv = o.basis()["Vinv"]
+    for state in (u, o.solve(u, 0.0, 0.4, 1.3)[0]):
+        f, a_hg = mlp_ref.features(state, 2, 3, vinv, 1.4, 0.002, 0.0001)
+        a_nn = 0.002 * mlp_ref.forward(params, f)["y"]
+        err = np.abs(a_nn - a_hg) / 0.002
+        assert err.mean() < 0.05 and np.quantile(err, 0.95) < 0.2, (err.mean(), np.quantile(err, 0.95))

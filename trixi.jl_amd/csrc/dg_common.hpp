@@ -25,9 +25,10 @@ enum FluxKind { FLUX_LLF = 0, FLUX_LLF_NAIVE = 1, FLUX_RANOCHA = 2, FLUX_CENTRAL
 constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
 
 // ---------------------------------------------------------------------------
-// Precision traits. The FP32 path uses the hardware reciprocal and base-2
-// logarithm (v_rcp_f32, v_log_f32, 1 ulp); the FP64 path refines v_rcp_f64
-// with two Newton steps and calls the correctly rounded double log.
+// Precision traits. The FP32 path uses the hardware reciprocal, base-2
+// logarithm and square root (v_rcp_f32, v_log_f32, v_sqrt_f32, 1 ulp); the
+// FP64 path refines v_rcp_f64 with two Newton steps and calls the correctly
+// rounded double log and sqrt.
 // ---------------------------------------------------------------------------
 template <typename T> struct Prec;
 
@@ -36,7 +37,7 @@ template <> struct Prec<float> {
   static __device__ __forceinline__ float ln(float x) {
     return __builtin_amdgcn_logf(x) * 0.69314718055994530942f;
   }
-  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
   // Series branch of the logarithmic mean is taken for f^2 < 1e-2 in FP32:
   // its truncation error (first omitted term ~ f^8 / 9 <= 1.2e-9) is below FP32
   // round-off, and the cancellation-prone far branch is only used for
@@ -109,11 +110,27 @@ __device__ __forceinline__ T inv_ln_mean_l(T x, T y, T lx, T ly) {
 // flux_ranocha's inv_ln_mean(rho_ll p_rr, rho_rr p_ll) p_ll p_rr equals
 // inv_ln_mean(beta_ll, beta_rr) (compressible_euler_3d.jl:752-757).
 // The "log" fields hold ln(x) in FP64 and log2(x)/2 in FP32 (see
-// flux_ranocha_fast); lnv() returns the natural log in both cases.
+// flux_ranocha_fast); lnv() returns the natural log in both cases. r and b
+// hold rho and beta, or, in the square-root representation SQ (FP32 with
+// polydeg <= 3), r = sqrt(rho) and b = sqrt(p/rho) = beta^(-1/2): the fast
+// FP32 flux then forms 2^(a_x + a_y) = r_x r_y by a product. SQ costs two
+// registers per node in the line loop, which polydeg 4 lines cannot afford
+// without losing occupancy. rho_of / beta_of recover rho and beta.
 template <typename T, int D> struct NodeR {
-  T rho, lrho, beta, lbeta, h[D], hp;
+  T r, lrho, b, lbeta, h[D], hp;
 };
 constexpr int nodeR_count(int D) { return 5 + D; }
+
+template <typename T, int N> constexpr bool node_sqrt_rep() { return sizeof(T) == 4 && N <= 3; }
+
+template <bool SQ, typename T, int D> __device__ __forceinline__ T rho_of(const NodeR<T, D> &w) {
+  if constexpr (SQ) return w.r * w.r;
+  else return w.r;
+}
+template <bool SQ, typename T, int D> __device__ __forceinline__ T beta_of(const NodeR<T, D> &w) {
+  if constexpr (SQ) return Prec<T>::rcp(w.b * w.b);
+  else return w.b;
+}
 
 template <typename T> struct LogRep;
 template <> struct LogRep<double> {
@@ -125,7 +142,7 @@ template <> struct LogRep<float> {
   static __device__ __forceinline__ float lnv(float a) { return a * 1.38629436111989061883f; }  // 2 ln 2
 };
 
-template <typename T, int D>
+template <typename T, int D, bool SQ = false>
 __device__ __forceinline__ void nodeR_from_cons(const T *u, T gamma_m1, NodeR<T, D> &w) {
   T rho = u[0];
   T irho = Prec<T>::rcp(rho);
@@ -137,21 +154,27 @@ __device__ __forceinline__ void nodeR_from_cons(const T *u, T gamma_m1, NodeR<T,
     ke2 = tfma(u[1 + d], v[d], ke2);
   }
   T p = gamma_m1 * (u[D + 1] - T(0.5) * ke2);
-  w.rho = rho;
   w.lrho = LogRep<T>::store(rho);
-  w.beta = rho * Prec<T>::rcp(p);
   w.lbeta = w.lrho - LogRep<T>::store(p);
+  if constexpr (SQ) {
+    w.r = __builtin_amdgcn_sqrtf(rho);  // v_sqrt_f32, 1 ulp
+    w.b = __builtin_amdgcn_sqrtf(p * irho);
+  } else {
+    w.r = rho;
+    w.b = rho * Prec<T>::rcp(p);
+  }
 #pragma unroll
   for (int d = 0; d < D; ++d) w.h[d] = T(0.5) * v[d];
   w.hp = T(0.5) * p;
 }
 
 // flux_ranocha from per-node data, orientation o (compile-time).
-template <typename T, int D, int O>
+template <typename T, int D, int O, bool SQ = false>
 __device__ __forceinline__ void flux_ranocha_r(const NodeR<T, D> &l, const NodeR<T, D> &r,
                                                T inv_gamma_m1, T *f) {
-  T rho_mean = ln_mean_l(l.rho, r.rho, LogRep<T>::lnv(l.lrho), LogRep<T>::lnv(r.lrho));
-  T inv_rho_p_mean = inv_ln_mean_l(l.beta, r.beta, LogRep<T>::lnv(l.lbeta), LogRep<T>::lnv(r.lbeta));
+  T rho_mean = ln_mean_l(rho_of<SQ>(l), rho_of<SQ>(r), LogRep<T>::lnv(l.lrho), LogRep<T>::lnv(r.lrho));
+  T inv_rho_p_mean =
+      inv_ln_mean_l(beta_of<SQ>(l), beta_of<SQ>(r), LogRep<T>::lnv(l.lbeta), LogRep<T>::lnv(r.lbeta));
   T vavg[D];
   T hh = 0;
 #pragma unroll
@@ -178,18 +201,26 @@ __device__ __forceinline__ void flux_ranocha_r(const NodeR<T, D> &l, const NodeR
 // Truncated after w^8 the relative error is <= 2.2e-8 for |w| <= 0.5 (y/x <= e),
 // below FP32 round-off; callers use this only when every pair of the line is
 // within that range (flux_ranocha_r otherwise). Coefficients below are pre-
-// multiplied by ln2^(2k) so that the series variable is (a_y - a_x)^2.
-template <int D, int O>
+// multiplied by ln2^(2k) so that the series variable is (a_y - a_x)^2. In the
+// square-root representation (SQ) the factors 2^(a_x + a_y) = sqrt(x y) are
+// products of the per-node roots, otherwise one exp2 each.
+template <int D, int O, bool SQ>
 __device__ __forceinline__ void flux_ranocha_fast(const NodeR<float, D> &l, const NodeR<float, D> &r,
                                                   float inv_gamma_m1, float *f) {
-  const float m1 = l.lrho + r.lrho, d1 = r.lrho - l.lrho, z1 = d1 * d1;
+  const float d1 = r.lrho - l.lrho, z1 = d1 * d1;
   const float p1 = tfma(tfma(tfma(tfma(1.46838742113e-7f, z1, 2.20050434191e-5f), z1, 0.00192362582153f), z1,
                             0.0800755023197f), z1, 1.0f);
-  const float rho_mean = __builtin_amdgcn_exp2f(m1) * p1;
-  const float m2 = l.lbeta + r.lbeta, d2 = r.lbeta - l.lbeta, z2 = d2 * d2;
+  const float d2 = r.lbeta - l.lbeta, z2 = d2 * d2;
   const float q2 = tfma(tfma(tfma(tfma(1.1189112149e-5f, z2, -0.000227385448664f), z2, 0.00448846025023f), z2,
                             -0.0800755023197f), z2, 1.0f);
-  const float inv_rho_p_mean = __builtin_amdgcn_exp2f(-m2) * q2;
+  float rho_mean, inv_rho_p_mean;
+  if constexpr (SQ) {
+    rho_mean = (l.r * r.r) * p1;
+    inv_rho_p_mean = (l.b * r.b) * q2;
+  } else {
+    rho_mean = __builtin_amdgcn_exp2f(l.lrho + r.lrho) * p1;
+    inv_rho_p_mean = __builtin_amdgcn_exp2f(-(l.lbeta + r.lbeta)) * q2;
+  }
   float vavg[D];
   float hh = 0;
 #pragma unroll
@@ -254,20 +285,43 @@ __device__ __forceinline__ void flux_central_c(const ConsP<T, D> &l, const ConsP
   for (int k = 0; k < D + 2; ++k) f[k] = T(0.5) * (fl[k] + fr[k]);
 }
 
-// FluxLaxFriedrichs(max_abs_speed | max_abs_speed_naive)
+// FluxLaxFriedrichs(max_abs_speed | max_abs_speed_naive) with the sound speeds given
 template <typename T, int D, int O>
-__device__ __forceinline__ void flux_llf_c(const ConsP<T, D> &l, const ConsP<T, D> &r, bool naive,
-                                           T gamma, T *f) {
+__device__ __forceinline__ void flux_llf_pc(const ConsP<T, D> &l, const ConsP<T, D> &r, T c_ll, T c_rr, bool naive,
+                                            T *f) {
   T fl[D + 2], fr[D + 2];
   phys_flux<T, D, O>(l, fl);
   phys_flux<T, D, O>(r, fr);
-  T c_ll = Prec<T>::sqrt_(gamma * l.p * l.irho);
-  T c_rr = Prec<T>::sqrt_(gamma * r.p * r.irho);
   T al = tabs(l.v[O]), ar = tabs(r.v[O]);
   T lambda = naive ? tmax(al, ar) + tmax(c_ll, c_rr) : tmax(al + c_ll, ar + c_rr);
   T hl = T(-0.5) * lambda;
 #pragma unroll
   for (int k = 0; k < D + 2; ++k) f[k] = T(0.5) * (fl[k] + fr[k]) + hl * (r.u[k] - l.u[k]);
+}
+
+// Own face state of an FP32 flux_ranocha line from its node record in the
+// square-root representation: the same velocity and pressure as
+// consp_from_cons (v = 2 h, p = 2 hp exactly), and the sound speed
+// sqrt(gamma p / rho) = sqrt(gamma) b.
+template <int D>
+__device__ __forceinline__ void consp_from_node(const float *u, const NodeR<float, D> &w, float sqrt_gamma,
+                                                ConsP<float, D> &c, float &cs) {
+#pragma unroll
+  for (int k = 0; k < D + 2; ++k) c.u[k] = u[k];
+#pragma unroll
+  for (int d = 0; d < D; ++d) c.v[d] = 2.0f * w.h[d];
+  c.p = 2.0f * w.hp;
+  c.irho = 0.0f;  // not needed by phys_flux / flux_llf_pc
+  cs = sqrt_gamma * w.b;
+}
+
+// FluxLaxFriedrichs(max_abs_speed | max_abs_speed_naive)
+template <typename T, int D, int O>
+__device__ __forceinline__ void flux_llf_c(const ConsP<T, D> &l, const ConsP<T, D> &r, bool naive,
+                                           T gamma, T *f) {
+  const T c_ll = Prec<T>::sqrt_(gamma * l.p * l.irho);
+  const T c_rr = Prec<T>::sqrt_(gamma * r.p * r.irho);
+  flux_llf_pc<T, D, O>(l, r, c_ll, c_rr, naive, f);
 }
 
 // flux_ranocha from conservative states (surface / FV use).

@@ -62,6 +62,8 @@ template <typename T> struct ElemArgs {
   int alpha_smooth;
   int xcd_remap;
   T gamma;
+  T inv_gamma_m1;  // 1 / (gamma - 1), rounded once in T
+  T sqrt_gamma;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs. Remap so that each
@@ -211,7 +213,13 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
   const int q0 = el * NN + c.base;
   const T gamma = a.gamma;
   const T gm1 = gamma - T(1);
-  const T igm1 = T(1) / gm1;
+  const T igm1 = a.inv_gamma_m1;
+  // node records in the square-root representation (FP32, polydeg <= 3); the
+  // own face states' primitives and sound speeds then come from the records of
+  // the two end nodes
+  constexpr bool SQ = node_sqrt_rep<T, N>();
+  constexpr bool OWN_FROM_W = VT::RAN && SQ;
+  NodeR<T, D> wend[2];
 
   int pq[n];
 #pragma unroll
@@ -254,13 +262,17 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
     NodeR<T, D> w[n];
 #pragma unroll
     for (int i = 0; i < n; ++i) {
-      w[i].rho = s_w[0 * SZ + pq[i]];
+      w[i].r = s_w[0 * SZ + pq[i]];
       w[i].lrho = s_w[1 * SZ + pq[i]];
-      w[i].beta = s_w[2 * SZ + pq[i]];
+      w[i].b = s_w[2 * SZ + pq[i]];
       w[i].lbeta = s_w[3 * SZ + pq[i]];
 #pragma unroll
       for (int d = 0; d < D; ++d) w[i].h[d] = s_w[(4 + d) * SZ + pq[i]];
       w[i].hp = s_w[(4 + D) * SZ + pq[i]];
+    }
+    if constexpr (OWN_FROM_W) {
+      wend[0] = w[0];
+      wend[1] = w[n - 1];
     }
     bool fast = false;
     if constexpr (sizeof(T) == 4) {
@@ -283,7 +295,7 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
 #pragma unroll
           for (int j = i + 1; j < n; ++j) {
             T f[V];
-            flux_ranocha_fast<D, O>(w[i], w[j], igm1, f);
+            flux_ranocha_fast<D, O, SQ>(w[i], w[j], igm1, f);
             const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
 #pragma unroll
             for (int v = 0; v < V; ++v) {
@@ -298,7 +310,7 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
 #pragma unroll
         for (int j = i + 1; j < n; ++j) {
           T f[V];
-          flux_ranocha_r<T, D, O>(w[i], w[j], igm1, f);
+          flux_ranocha_r<T, D, O, SQ>(w[i], w[j], igm1, f);
           const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
 #pragma unroll
           for (int v = 0; v < V; ++v) {
@@ -386,12 +398,34 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
   const T factor = k.inv_w[0];  // inverse_weights[1] of the reference
   {
     T F[V];
-    num_flux_c<T, D, O>(a.surface_flux, c.ul_m, c.own0, gamma, F);  // -O face: element is the right state
+    bool done = false;
+    if constexpr (OWN_FROM_W) {
+      if (a.surface_flux == FLUX_LLF || a.surface_flux == FLUX_LLF_NAIVE) {
+        const bool naive = a.surface_flux == FLUX_LLF_NAIVE;
+        ConsP<T, D> own_face[2];
+        T own_c[2];
+        consp_from_node<D>(c.own0, wend[0], a.sqrt_gamma, own_face[0], own_c[0]);
+        consp_from_node<D>(c.own1, wend[1], a.sqrt_gamma, own_face[1], own_c[1]);
+        ConsP<T, D> nb;
+        consp_from_cons<T, D>(c.ul_m, gm1, nb);  // -O face: element is the right state
+        flux_llf_pc<T, D, O>(nb, own_face[0], Prec<T>::sqrt_(gamma * nb.p * nb.irho), own_c[0], naive, F);
 #pragma unroll
-    for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
-    num_flux_c<T, D, O>(a.surface_flux, c.own1, c.ur_p, gamma, F);  // +O face: element is the left state
+        for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
+        consp_from_cons<T, D>(c.ur_p, gm1, nb);  // +O face: element is the left state
+        flux_llf_pc<T, D, O>(own_face[1], nb, own_c[1], Prec<T>::sqrt_(gamma * nb.p * nb.irho), naive, F);
 #pragma unroll
-    for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
+        for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
+        done = true;
+      }
+    }
+    if (!done) {
+      num_flux_c<T, D, O>(a.surface_flux, c.ul_m, c.own0, gamma, F);  // -O face: element is the right state
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
+      num_flux_c<T, D, O>(a.surface_flux, c.own1, c.ur_p, gamma, F);  // +O face: element is the left state
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
+    }
   }
 
 #pragma unroll
@@ -551,10 +585,10 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
       }
       if constexpr (VT::RAN) {
         NodeR<T, D> w;
-        nodeR_from_cons<T, D>(uq, gm1, w);
-        s_w[0 * SZ + p] = w.rho;
+        nodeR_from_cons<T, D, node_sqrt_rep<T, N>()>(uq, gm1, w);
+        s_w[0 * SZ + p] = w.r;
         s_w[1 * SZ + p] = w.lrho;
-        s_w[2 * SZ + p] = w.beta;
+        s_w[2 * SZ + p] = w.b;
         s_w[3 * SZ + p] = w.lbeta;
 #pragma unroll
         for (int d = 0; d < D; ++d) s_w[(4 + d) * SZ + p] = w.h[d];

@@ -297,6 +297,8 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   a.alpha_smooth = ctx->cfg.alpha_smooth;
   a.xcd_remap = ctx->xcd_remap;
   a.gamma = T(ctx->cfg.gamma);
+  a.inv_gamma_m1 = T(1) / (a.gamma - T(1));  // the device expression, evaluated once in T
+  a.sqrt_gamma = T(std::sqrt(ctx->cfg.gamma));
   int nblocks = (ctx->E + ctx->epb - 1) / ctx->epb;
   hipError_t e = ctx->elem(&a, ctx->kconst.data(), nblocks, ctx->stream);
   if (e != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("element kernel launch: ") + hipGetErrorString(e));

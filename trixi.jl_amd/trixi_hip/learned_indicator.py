@@ -235,6 +235,36 @@ def train(semi, X, Y, steps=4000, batch=8192, lr=2e-3, seed=0, log=None, log_eve
     return net, hist
 
 
+def evaluate(net, seed=1000, level=5, t_end=1.0, nsnap=5):
+    """Run the KHI setup twice from the same seeded state, blending with HG and
+    with `net`, and compare along the run: mean / max |alpha_net - alpha_HG|
+    (in units of alpha_max) per snapshot, the share of elements that fell back
+    to HG, and the final density difference (RMS, max)."""
+    alphas, fellback, finals = {}, [], {}
+    for name in ("hg", "nn"):
+        semi = khi_semidiscretization(level, 1e-3, seed)
+        if name == "nn":
+            net.attach(semi)
+        rec = []
+
+        def snap(sm, t):
+            rec.append(indicator_alpha(sm, _lib.IND_HG if name == "hg" else net.mode))
+            if name == "nn" and net.fallback:
+                f, _ = features(sm)
+                out = np.any((f[:, :NFEATURES] < net.feature_lo) | (f[:, :NFEATURES] > net.feature_hi), axis=1)
+                fellback.append(float(np.mean(out)))
+
+        finals[name] = run_segments(semi, t_end, nsnap, on_snapshot=snap)
+        alphas[name] = rec
+        semi.close()
+    d = [np.abs(a - b) / 0.002 for a, b in zip(alphas["nn"], alphas["hg"])]
+    drho = finals["nn"].reshape(-1, 4)[:, 0] - finals["hg"].reshape(-1, 4)[:, 0]
+    return dict(seed=seed, level=level, t_end=t_end,
+                alpha_mean_abs_diff=[float(x.mean()) for x in d], alpha_max_abs_diff=[float(x.max()) for x in d],
+                fallback_share=fellback, rho_rms_diff=float(np.sqrt(np.mean(drho ** 2))),
+                rho_max_diff=float(np.abs(drho).max()))
+
+
 def main(argv=None):
     import argparse
     import json
@@ -248,10 +278,15 @@ def main(argv=None):
     p.add_argument("--steps", type=int, default=4000)
     p.add_argument("--batch", type=int, default=8192)
     p.add_argument("--lr", type=float, default=2e-3)
+    p.add_argument("--evaluate", action="store_true", help="evaluate the weights at --out instead of training")
     a = p.parse_args(argv)
 
     def log(msg):
         print(f"[learned_indicator] {msg}", file=sys.stderr, flush=True)
+
+    if a.evaluate:
+        print(json.dumps(evaluate(IndicatorNeuralNetwork.load(a.out), level=a.level)))
+        return
 
     X, Y = collect_khi_data(range(a.seeds), a.level, a.t_end, a.snapshots, log=log)
     log(f"{X.shape[0]} samples, {np.mean(Y > 0):.3f} with alpha > 0")
