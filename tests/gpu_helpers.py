@@ -87,3 +87,8 @@ def flux_scale(semi, u):
 def fp32_abs_bound(semi, u, K=200.0):
     """Round-off bound of the FP32 bulk RHS: K * eps32 * J^-1 * max|f| (absolute)."""
     return K * float(np.finfo(np.float32).eps) * flux_scale(semi, u)
+
+
+def fp64_abs_bound(semi, u, K=200.0):
+    """The same round-off bound with eps64 (free-stream residuals of the FP64 RHS)."""
+    return K * float(np.finfo(np.float64).eps) * flux_scale(semi, u)

@@ -6,7 +6,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from gpu_helpers import build, fp32_abs_bound, oracle_for, relerr
+from gpu_helpers import build, fp32_abs_bound, fp64_abs_bound, oracle_for, relerr
 
 torch = pytest.importorskip("torch")
 
@@ -109,8 +109,8 @@ def test_free_stream_preservation(D):
         u = semi.compute_coefficients(0.0)
         du = np.empty_like(u)
         th.rhs_hyperbolic_(du, u, semi, 0.0)
-        # round-off level: the FP32 bound scales as eps32 J^-1 max|f|; FP64 likewise with eps64
-        tol = 2e-13 if prec == "fp64" else fp32_abs_bound(semi, u)
+        # round-off level: K eps J^-1 max|f| with the working precision's eps
+        tol = fp64_abs_bound(semi, u) if prec == "fp64" else fp32_abs_bound(semi, u)
         assert np.max(np.abs(du)) < tol, (prec, np.max(np.abs(du)), tol)
         semi.close()
 

@@ -239,6 +239,41 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
     blend = !(alpha <= atol && alpha >= -atol);
   }
 
+  // ---------------- interface fluxes + surface integral ----------------
+  // (evaluated after the volume integral, see the end of this function)
+  const T factor = k.inv_w[0];  // inverse_weights[1] of the reference
+  auto faces = [&]() {
+    T F[V];
+    bool done = false;
+    if constexpr (OWN_FROM_W) {
+      if (a.surface_flux == FLUX_LLF || a.surface_flux == FLUX_LLF_NAIVE) {
+        const bool naive = a.surface_flux == FLUX_LLF_NAIVE;
+        ConsP<T, D> own_face[2];
+        T own_c[2];
+        consp_from_node<D>(c.own0, wend[0], a.sqrt_gamma, own_face[0], own_c[0]);
+        consp_from_node<D>(c.own1, wend[1], a.sqrt_gamma, own_face[1], own_c[1]);
+        ConsP<T, D> nb;
+        consp_from_cons<T, D>(c.ul_m, gm1, nb);  // -O face: element is the right state
+        flux_llf_pc<T, D, O>(nb, own_face[0], Prec<T>::sqrt_(gamma * nb.p * nb.irho), own_c[0], naive, F);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
+        consp_from_cons<T, D>(c.ur_p, gm1, nb);  // +O face: element is the left state
+        flux_llf_pc<T, D, O>(own_face[1], nb, own_c[1], Prec<T>::sqrt_(gamma * nb.p * nb.irho), naive, F);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
+        done = true;
+      }
+    }
+    if (!done) {
+      num_flux_c<T, D, O>(a.surface_flux, c.ul_m, c.own0, gamma, F);  // -O face: element is the right state
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
+      num_flux_c<T, D, O>(a.surface_flux, c.own1, c.ur_p, gamma, F);  // +O face: element is the left state
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
+    }
+  };
+
   // ---------------- volume integral ----------------
   if constexpr (VOL == VOL_WEAK) {
 #pragma unroll
@@ -260,8 +295,7 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
   }
   if constexpr (VT::RAN) {
     NodeR<T, D> w[n];
-#pragma unroll
-    for (int i = 0; i < n; ++i) {
+    auto load_w = [&](int i) {
       w[i].r = s_w[0 * SZ + pq[i]];
       w[i].lrho = s_w[1 * SZ + pq[i]];
       w[i].b = s_w[2 * SZ + pq[i]];
@@ -269,8 +303,10 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
 #pragma unroll
       for (int d = 0; d < D; ++d) w[i].h[d] = s_w[(4 + d) * SZ + pq[i]];
       w[i].hp = s_w[(4 + D) * SZ + pq[i]];
-    }
-    if constexpr (OWN_FROM_W) {
+    };
+#pragma unroll
+    for (int i = 0; i < n; ++i) load_w(i);
+    if constexpr (OWN_FROM_W) {  // the end records, for the faces after the volume terms
       wend[0] = w[0];
       wend[1] = w[n - 1];
     }
@@ -310,7 +346,32 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
 #pragma unroll
         for (int j = i + 1; j < n; ++j) {
           T f[V];
-          flux_ranocha_r<T, D, O, SQ>(w[i], w[j], igm1, f);
+          if constexpr (sizeof(T) == 4) {
+            // Rare path (a log ratio of the line outside the series range):
+            // fresh copies of the two records from LDS. The opaque zero offset
+            // keeps them apart from the register copies used by the fast path,
+            // which then die at the branch; reusing them here would set the
+            // register allocation of the whole kernel (144 instead of 106
+            // VGPRs for 3D p3, one wave per SIMD less).
+            int z0 = 0;
+            asm volatile("" : "+v"(z0));
+            const T *sw = s_w + z0;
+            NodeR<T, D> wi, wj;
+            auto read = [&](NodeR<T, D> &x, int q) {
+              x.r = sw[0 * SZ + q];
+              x.lrho = sw[1 * SZ + q];
+              x.b = sw[2 * SZ + q];
+              x.lbeta = sw[3 * SZ + q];
+#pragma unroll
+              for (int d = 0; d < D; ++d) x.h[d] = sw[(4 + d) * SZ + q];
+              x.hp = sw[(4 + D) * SZ + q];
+            };
+            read(wi, pq[i]);
+            read(wj, pq[j]);
+            flux_ranocha_r<T, D, O, SQ>(wi, wj, igm1, f);
+          } else {
+            flux_ranocha_r<T, D, O, SQ>(w[i], w[j], igm1, f);
+          }
           const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
 #pragma unroll
           for (int v = 0; v < V; ++v) {
@@ -391,42 +452,10 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
     }
   }
 
-  // ---------------- interface fluxes + surface integral ----------------
-  // keep the face work (which waits on the neighbour loads issued above)
-  // behind the volume arithmetic
+  // keep the face work (which waits on the neighbour loads of phase A) behind
+  // the volume arithmetic
   __builtin_amdgcn_sched_barrier(0);
-  const T factor = k.inv_w[0];  // inverse_weights[1] of the reference
-  {
-    T F[V];
-    bool done = false;
-    if constexpr (OWN_FROM_W) {
-      if (a.surface_flux == FLUX_LLF || a.surface_flux == FLUX_LLF_NAIVE) {
-        const bool naive = a.surface_flux == FLUX_LLF_NAIVE;
-        ConsP<T, D> own_face[2];
-        T own_c[2];
-        consp_from_node<D>(c.own0, wend[0], a.sqrt_gamma, own_face[0], own_c[0]);
-        consp_from_node<D>(c.own1, wend[1], a.sqrt_gamma, own_face[1], own_c[1]);
-        ConsP<T, D> nb;
-        consp_from_cons<T, D>(c.ul_m, gm1, nb);  // -O face: element is the right state
-        flux_llf_pc<T, D, O>(nb, own_face[0], Prec<T>::sqrt_(gamma * nb.p * nb.irho), own_c[0], naive, F);
-#pragma unroll
-        for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
-        consp_from_cons<T, D>(c.ur_p, gm1, nb);  // +O face: element is the left state
-        flux_llf_pc<T, D, O>(own_face[1], nb, own_c[1], Prec<T>::sqrt_(gamma * nb.p * nb.irho), naive, F);
-#pragma unroll
-        for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
-        done = true;
-      }
-    }
-    if (!done) {
-      num_flux_c<T, D, O>(a.surface_flux, c.ul_m, c.own0, gamma, F);  // -O face: element is the right state
-#pragma unroll
-      for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
-      num_flux_c<T, D, O>(a.surface_flux, c.own1, c.ur_p, gamma, F);  // +O face: element is the left state
-#pragma unroll
-      for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
-    }
-  }
+  faces();
 
 #pragma unroll
   for (int i = 0; i < n; ++i)
