@@ -37,8 +37,10 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=200)
-    p.add_argument("--warmup", type=int, default=20)
+    # ~40 ms (FP32) / ~110 ms (FP64) of warm-up: enough for the clocks of a
+    # freshly started GPU to settle before the timed region
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=1000)
     p.add_argument("--level", type=int, default=5)
     p.add_argument("--polydeg", type=int, default=3)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
