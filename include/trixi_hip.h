@@ -166,6 +166,20 @@ int trixi_hip_rk_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage_dt,
 int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, const double *c,
                       int nstages, double t, double dt, int use_graph);
 
+/* A whole run of a 2N scheme with StepsizeCallback(cfl) semantics on the resident
+ * state, without host synchronisation per step: solve() with step!
+ * (methods_2N.jl:131-168), the step size callback (stepsize.jl:93-155) and
+ * limit_dt! (time_integration.jl:46-55). After every step dt = cfl * max_dt(u);
+ * the step that would pass t_end (or land within sqrt(eps) of it) is shortened to
+ * end there. One captured graph (stages + step control) is replayed; the host
+ * reads the device control block every poll_every steps, so up to poll_every - 1
+ * replays run after the end and leave the state unchanged. t_out and steps_out
+ * receive the final time and the number of steps taken. Returns TRIXI_HIP_ESTATE
+ * with "time step size `dt` is NaN" when dt becomes NaN (methods_2N.jl:138-140). */
+int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const double *c,
+                    int nstages, double t0, double t_end, double cfl, int max_steps,
+                    int poll_every, double *t_out, int *steps_out);
+
 /* max_dt (stepsize_dg{2,3}d.jl:8-32) of the resident state times cfl
  * (calculate_dt, stepsize.jl:146-155). FP64 reduction. */
 int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt);

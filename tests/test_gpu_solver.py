@@ -154,3 +154,39 @@ def test_torch_device_arrays_after_a_library_context():
         "print('ok')\n")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_device_step_control_equals_host_loop(precision):
+    """trixi_hip_solve (dt and limit_dt on the device, no host synchronisation
+    per step) takes the same steps as the host loop that reads max_dt after
+    every step (stepsize.jl:93-155, time_integration.jl:46-55): same number of
+    steps, the same final time and a bitwise-equal state, for any poll interval;
+    maxiters stops both after the same step."""
+    import trixi_hip as th
+    c = golden("kta.json")["khi_2d"]
+    spec = dict(c["oracle"], ic=c["ic"], ic_params=tuple(c["ic_params"]))
+    semi, _ = build(spec, precision)
+    runs = {}
+    for tspan, maxiters in (((0.0, 0.37), None), ((0.0, 5.0), 23)):
+        for dev, poll in ((False, 1), (True, 1), (True, 7), (True, 64)):
+            ode = th.semidiscretize(semi, tspan)
+            sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=0.9)),
+                           maxiters=maxiters, device_steps=dev, poll_every=poll)
+            runs[(tspan, dev, poll)] = sol
+        host = runs[(tspan, False, 1)]
+        assert host.nsteps == (maxiters if maxiters else host.nsteps) and host.nsteps > 5
+        if maxiters is None:
+            assert host.t == tspan[1]
+        for poll in (1, 7, 64):
+            d = runs[(tspan, True, poll)]
+            assert (d.nsteps, d.t) == (host.nsteps, host.t), (poll, d.nsteps, host.nsteps, d.t, host.t)
+            assert np.array_equal(d.u, host.u), poll
+    # an empty time span takes one step of size 0, as step! with limit_dt! does
+    for dev in (False, True):
+        ode = th.semidiscretize(semi, (0.2, 0.2))
+        sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=0.9)),
+                       device_steps=dev)
+        assert sol.nsteps == 1 and sol.t == 0.2 and np.array_equal(sol.u, ode.u0), dev
+    semi.close()

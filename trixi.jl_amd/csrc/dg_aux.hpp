@@ -198,45 +198,103 @@ __device__ __forceinline__ double nanmax(double a, double b) {
   return (a != a || b != b) ? __builtin_nan("") : (a > b ? a : b);
 }
 
+// max_dt (stepsize_dg3d.jl:8-32, stepsize_dg2d.jl:8-30) in two passes: per
+// block, max over its elements of inv_jac * sum_d max_nodes lambda_d with
+// lambda_d = |v_d| + c (max_abs_speeds, compressible_euler_3d.jl:1770-1775);
+// one thread per node (EB = 256 / NN elements per block, coalesced loads), the
+// node maxima reduced through LDS. NaN speeds propagate (nanmax).
 template <typename T, int D, int N>
-__global__ __launch_bounds__(256) void max_speed_kernel(const T *__restrict__ u,
-                                                        const double *__restrict__ inv_jac, int E,
-                                                        double gamma, double *__restrict__ partial) {
+__global__ __launch_bounds__(256) void max_speed_nodes_kernel(const T *__restrict__ u,
+                                                              const double *__restrict__ inv_jac, int E,
+                                                              double gamma, double *__restrict__ partial) {
   constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
-  __shared__ double s_red[256];
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  double val = 4.9406564584124654e-324;
-  if (e < E) {
-    double ml[D];
+  constexpr int EB = NN >= 256 ? 1 : 256 / NN;
+  static_assert(NN <= 256, "one thread per node");
+  __shared__ double s_l[D][EB * NN];
+  __shared__ double s_e[EB];
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * EB;
+  const int ne = min(EB, E - e0);
+  if (tid < ne * NN) {
+    const T *un = u + ((size_t)e0 * NN + tid) * V;
+    const double rho = un[0];
+    double ke2 = 0, v[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) ml[d] = 0.0;
-    const T *ue = u + (size_t)e * NN * V;
-    for (int node = 0; node < NN; ++node) {
-      const T *un = ue + node * V;
-      double rho = un[0];
-      double ke2 = 0, v[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        v[d] = double(un[1 + d]) / rho;
-        ke2 += double(un[1 + d]) * v[d];
-      }
-      double p = (gamma - 1) * (double(un[D + 1]) - 0.5 * ke2);
-      double c = sqrt(gamma * p / rho);
-#pragma unroll
-      for (int d = 0; d < D; ++d) ml[d] = nanmax(ml[d], fabs(v[d]) + c);
+    for (int d = 0; d < D; ++d) {
+      v[d] = double(un[1 + d]) / rho;
+      ke2 += double(un[1 + d]) * v[d];
     }
+    const double p = (gamma - 1) * (double(un[D + 1]) - 0.5 * ke2);
+    const double c = sqrt(gamma * p / rho);
+#pragma unroll
+    for (int d = 0; d < D; ++d) s_l[d][tid] = fabs(v[d]) + c;
+  }
+  __syncthreads();
+  if (tid < ne) {
     double s = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) s += ml[d];
-    val = nanmax(val, inv_jac[e] * s);
+    for (int d = 0; d < D; ++d) {
+      double m = 0.0;
+      for (int q = 0; q < NN; ++q) m = nanmax(m, s_l[d][tid * NN + q]);
+      s += m;
+    }
+    s_e[tid] = inv_jac[e0 + tid] * s;
   }
-  s_red[threadIdx.x] = val;
+  __syncthreads();
+  if (tid == 0) {
+    double m = 4.9406564584124654e-324;
+    for (int i = 0; i < ne; ++i) m = nanmax(m, s_e[i]);
+    partial[blockIdx.x] = m;
+  }
+}
+
+// Device step control of a 2N run with StepsizeCallback(cfl) and limit_dt
+// (stepsize.jl:93-124, 146-155; time_integration.jl:46-55; the step loop of
+// methods_2N.jl:131-168), after the stages of one step:
+//   t += dt; steps += 1; the step was the last one -> done
+//   dt = cfl * 2 / (nnodes * max_e inv_jac_e sum_d max lambda_d)
+//   if t + dt > t_end or isapprox(t + dt, t_end): dt = t_end - t (next step is the last)
+//   NaN dt -> error, done; steps == max_steps -> done; done -> dt = 0.
+// tdt = {t, dt}; ctl = {t_end, cfl, last, done, steps, error, max_steps}. A
+// step replayed after done leaves u and t unchanged (dt = 0).
+__global__ __launch_bounds__(256) void step_control_kernel(const double *__restrict__ partial, int np, int nnodes,
+                                                           double *__restrict__ tdt, double *__restrict__ ctl) {
+  __shared__ double s_red[256];
+  double m = 4.9406564584124654e-324;
+  for (int i = threadIdx.x; i < np; i += 256) m = nanmax(m, partial[i]);
+  s_red[threadIdx.x] = m;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w) s_red[threadIdx.x] = nanmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = s_red[0];
+  if (threadIdx.x != 0) return;
+  if (ctl[3] != 0.0) {  // done: this replay changed nothing
+    tdt[1] = 0.0;
+    return;
+  }
+  const double t = tdt[0] + tdt[1];
+  const double steps = ctl[4] + 1.0;
+  bool done = ctl[2] != 0.0 || steps >= ctl[6];
+  double dt = ctl[1] * (2.0 / (nnodes * s_red[0]));
+  if (!done && dt != dt) {  // the next step would start with a NaN dt
+    ctl[5] = 1.0;
+    done = true;
+  }
+  double last = 0.0;
+  if (!done) {
+    const double t_end = ctl[0], s = t + dt;
+    const double tol = 1.4901161193847656e-08 * fmax(fabs(s), fabs(t_end));  // isapprox rtol sqrt(eps)
+    if (s > t_end || s == t_end || fabs(s - t_end) <= tol) {
+      dt = t_end - t;
+      last = 1.0;
+    }
+  }
+  tdt[0] = t;
+  tdt[1] = done ? 0.0 : dt;
+  ctl[2] = last;
+  ctl[3] = done ? 1.0 : 0.0;
+  ctl[4] = steps;
 }
 
 __global__ __launch_bounds__(256) void max_dt_finalize_kernel(const double *__restrict__ partial,

@@ -3,6 +3,7 @@
 // and step (hipGraph-captured), reductions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -622,21 +623,145 @@ int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, cons
   return TRIXI_HIP_OK;
 }
 
-int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt) {
-  if (!ctx || !dt) return TRIXI_HIP_EINVAL;
-  int nb = (ctx->E + 255) / 256;
-  if ((size_t)nb > ctx->partial_cap) return set_err(ctx, TRIXI_HIP_EINVAL, "too many elements");
+}  // extern "C"
+
+namespace {
+
+// Partial maxima of the scaled wave speed of the resident FP64 state into
+// d_partial (one per block); returns the number of partials, or < 0.
+int launch_max_speed(trixi_hip_ctx *ctx) {
+  int nb = -1;
   dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
     constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
-    hipLaunchKernelGGL((max_speed_kernel<double, D, N>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d_u,
+    constexpr int NN = ipow_c(N + 1, D);
+    constexpr int EB = NN >= 256 ? 1 : 256 / NN;
+    nb = (ctx->E + EB - 1) / EB;
+    if ((size_t)nb > ctx->partial_cap) {
+      nb = -1;
+      return;
+    }
+    hipLaunchKernelGGL((max_speed_nodes_kernel<double, D, N>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d_u,
                        ctx->d_invj, ctx->E, ctx->cfg.gamma, ctx->d_partial);
   });
+  if (nb < 0) return set_err(ctx, TRIXI_HIP_EINVAL, "too many elements for the reduction buffer");
   HIP_TRY(hipGetLastError());
+  return nb;
+}
+
+// d_scalars layout: [0] t, [1] dt, [2] max_dt result, [3, 48) reductions,
+// [48, 55) step control {t_end, cfl, last, done, steps, error, max_steps}
+constexpr int kCtl = 48;
+
+__global__ void solve_init_kernel(double *tdt, double *ctl, double t, double dt, double t_end, double cfl,
+                                  double last, double max_steps) {
+  if (threadIdx.x == 0) {
+    tdt[0] = t;
+    tdt[1] = dt;
+    ctl[0] = t_end;
+    ctl[1] = cfl;
+    ctl[2] = last;
+    ctl[3] = 0.0;
+    ctl[4] = 0.0;
+    ctl[5] = 0.0;
+    ctl[6] = max_steps;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt) {
+  if (!ctx || !dt) return TRIXI_HIP_EINVAL;
+  const int nb = launch_max_speed(ctx);
+  if (nb < 0) return nb;
   hipLaunchKernelGGL(max_dt_finalize_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, nb, ctx->n,
                      cfl, ctx->d_scalars + 2);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dt, ctx->d_scalars + 2, 8, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const double *c, int nstages,
+                    double t0, double t_end, double cfl, int max_steps, int poll_every, double *t_out,
+                    int *steps_out) {
+  if (!ctx || !a || !b || !c || nstages <= 0 || nstages > 16 || !(cfl > 0) || max_steps <= 0 ||
+      poll_every <= 0 || !(t_end >= t0))
+    return TRIXI_HIP_EINVAL;
+  if (t_out) *t_out = t0;
+  if (steps_out) *steps_out = 0;
+  double *tdt = ctx->d_scalars, *ctl = ctx->d_scalars + kCtl;
+  // first step size on the host side of the stream (dt = cfl max_dt(u0)) and its limit_dt
+  double dt = 0;
+  int r = trixi_hip_max_dt(ctx, cfl, &dt);
+  if (r) return r;
+  if (dt != dt) return set_err(ctx, TRIXI_HIP_ESTATE, "time step size `dt` is NaN");
+  double last = 0;
+  {
+    const double s = t0 + dt;
+    if (s > t_end || s == t_end || std::fabs(s - t_end) <= 1.4901161193847656e-08 * std::fmax(std::fabs(s), std::fabs(t_end))) {
+      dt = t_end - t0;
+      last = 1;
+    }
+  }
+  hipLaunchKernelGGL(solve_init_kernel, dim3(1), dim3(1), 0, ctx->stream, tdt, ctl, t0, dt, t_end, cfl, last,
+                     (double)max_steps);
+  HIP_TRY(hipGetLastError());
+  // one step = the stages + the step control, captured once per (scheme, stream)
+  std::vector<double> key{-1.0, (double)nstages};
+  for (int s = 0; s < nstages; ++s) {
+    key.push_back(a[s]);
+    key.push_back(b[s]);
+    key.push_back(c[s]);
+  }
+  key.push_back((double)(uintptr_t)ctx->stream);
+  if (!ctx->graph_exec || key != ctx->graph_key) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rr = TRIXI_HIP_OK;
+    for (int s = 0; s < nstages && rr == TRIXI_HIP_OK; ++s) {
+      rr = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), c[s], tdt);
+      if (rr == TRIXI_HIP_OK) rr = launch_update(ctx, a[s], b[s], tdt, s == 0);
+    }
+    int nb = rr == TRIXI_HIP_OK ? launch_max_speed(ctx) : -1;
+    if (nb > 0)
+      hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, nb, ctx->n, tdt,
+                         ctl);
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (rr != TRIXI_HIP_OK || nb <= 0) {
+      if (g) (void)hipGraphDestroy(g);
+      return rr != TRIXI_HIP_OK ? rr : nb;
+    }
+    if (ec != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("end capture: ") + hipGetErrorString(ec));
+    ctx->graph = g;
+    HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+    ctx->graph_key = key;
+  }
+  // Replay without host synchronisation, reading the control block between
+  // batches. A batch is at most poll_every steps and, near the end, no more than
+  // the steps left at the current dt (floor), so few replays run after the end.
+  double h[7] = {0}, cur[2] = {t0, dt};
+  int launched = 0;
+  for (;;) {
+    const double left = (t_end - cur[0]) / cur[1];  // > 0: not done, dt > 0
+    int batch = left >= poll_every ? poll_every : (left >= 1.0 ? (int)left : 1);
+    batch = std::min(batch, std::max(1, max_steps - (int)h[4]));
+    for (int i = 0; i < batch; ++i) HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    launched += batch;
+    HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(cur, tdt, sizeof(cur), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (t_out) *t_out = cur[0];
+    if (steps_out) *steps_out = (int)h[4];
+    if (h[5] != 0.0) return set_err(ctx, TRIXI_HIP_ESTATE, "time step size `dt` is NaN");
+    if (h[3] != 0.0) break;
+    if (launched > max_steps) return set_err(ctx, TRIXI_HIP_ESTATE, "step control did not finish");
+  }
   return TRIXI_HIP_OK;
 }
 

@@ -98,11 +98,16 @@ def _isapprox(a, b):
     return a == b or abs(a - b) <= math.sqrt(np.finfo(np.float64).eps) * max(abs(a), abs(b))
 
 
-def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_final=True):
+def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_final=True, device_steps=True,
+          poll_every=16):
     """`solve(ode, CarpenterKennedy2N54(); dt, callback)` on the device.
 
-    The state lives in the HIP context for the whole run; one call of
-    trixi_hip_rk_step (a hipGraph replay of all stages) advances one step."""
+    The state lives in the HIP context for the whole run. With a
+    StepsizeCallback (interval 1) and no periodic analysis, the whole loop runs
+    on the device (trixi_hip_solve: step graph with device-side dt and
+    limit_dt, host reads the device every `poll_every` steps); otherwise one
+    call of trixi_hip_rk_step (a hipGraph replay of all stages) advances one
+    step and the host computes dt."""
     semi = ode.p
     lib = _lib.load()
     ctx = semi.ctx
@@ -113,17 +118,29 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
     analysis = cbs.of(AnalysisCallback)
     t0, t_end = ode.tspan
     t = t0
-    if stepsize:
-        dt = stepsize[0].calculate_dt(semi)
-    if dt is None:
-        raise ValueError("give dt or a StepsizeCallback")
     na = len(alg.a)
     a = (ctypes.c_double * na)(*alg.a)
     b = (ctypes.c_double * na)(*alg.b)
     c = (ctypes.c_double * na)(*alg.c)
     it = 0
     wall0 = time.perf_counter()
-    while True:
+    on_device = (device_steps and use_graph and stepsize and stepsize[0].interval == 1
+                 and not (analysis and analysis[0].interval > 0))
+    if on_device:
+        tout, nsteps = ctypes.c_double(), ctypes.c_int()
+        rc = lib.trixi_hip_solve(ctx, a, b, c, na, float(t0), float(t_end), stepsize[0].cfl,
+                                 int(maxiters) if maxiters is not None else 2 ** 31 - 1, int(poll_every),
+                                 ctypes.byref(tout), ctypes.byref(nsteps))
+        if rc == _lib.ESTATE and b"NaN" in lib.trixi_hip_last_error(ctx):
+            raise FloatingPointError("time step size `dt` is NaN")
+        _lib.check(rc, ctx, "solve")
+        t, it = tout.value, nsteps.value
+    else:
+        if stepsize:
+            dt = stepsize[0].calculate_dt(semi)
+        if dt is None:
+            raise ValueError("give dt or a StepsizeCallback")
+    while not on_device:
         if math.isnan(dt):
             raise FloatingPointError("time step size `dt` is NaN")
         final = False
