@@ -167,8 +167,8 @@ int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, cons
                       int nstages, double t, double dt, int use_graph);
 
 /* A whole run of a 2N scheme with StepsizeCallback(cfl) semantics on the resident
- * state, without host synchronisation per step: solve() with step!
- * (methods_2N.jl:131-168), the step size callback (stepsize.jl:93-155) and
+ * state, without host synchronisation per step: solve!/step! (time_integration.jl:102-123,
+ * methods_2N.jl:131-168), the step size callback (stepsize.jl:93-155) and
  * limit_dt! (time_integration.jl:46-55). After every step dt = cfl * max_dt(u);
  * the step that would pass t_end (or land within sqrt(eps) of it) is shortened to
  * end there. One captured graph (stages + step control) is replayed; the host
@@ -223,6 +223,16 @@ int trixi_hip_error_norms(trixi_hip_ctx *ctx, double t, double *l2, double *linf
 /* integrate_via_indices of the conserved variables and of entropy_math,
  * normalised by the domain volume (analysis_dg3d.jl:364-386). */
 int trixi_hip_integrals(trixi_hip_ctx *ctx, double *cons_integrals, double *entropy_integral);
+
+/* AnalysisCallback(semi; interval) inside trixi_hip_solve (analysis.jl:100-236):
+ * the solve analyses the initial state, every interval-th step and the final
+ * state on the device, without host synchronisation, into a ring of capacity
+ * records. interval 0 switches it off. */
+int trixi_hip_set_analysis(trixi_hip_ctx *ctx, int interval, int capacity);
+/* The newest min(made, capacity, max_records) records of the last solve, oldest
+ * first, each [3 + 3V] = {t, iter, l2 [V], linf [V], integrals of u [V],
+ * entropy integral} as trixi_hip_error_norms and trixi_hip_integrals give them. */
+int trixi_hip_get_analysis(trixi_hip_ctx *ctx, double *records, int max_records, int *count);
 
 /* Workgroup tile used by the element kernel: elements per block and threads. */
 int trixi_hip_get_tile(const trixi_hip_ctx *ctx, int *elements_per_block, int *threads_per_block);

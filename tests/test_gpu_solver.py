@@ -163,7 +163,9 @@ def test_device_step_control_equals_host_loop(precision):
     per step) takes the same steps as the host loop that reads max_dt after
     every step (stepsize.jl:93-155, time_integration.jl:46-55): same number of
     steps, the same final time and a bitwise-equal state, for any poll interval;
-    maxiters stops both after the same step."""
+    maxiters stops both after the same step. Analysis every 5 steps (run on the
+    device during the device-controlled solve) gives the same records: the
+    initial state, every 5th step and the final state (analysis.jl:121-122)."""
     import trixi_hip as th
     c = golden("kta.json")["khi_2d"]
     spec = dict(c["oracle"], ic=c["ic"], ic_params=tuple(c["ic_params"]))
@@ -172,17 +174,28 @@ def test_device_step_control_equals_host_loop(precision):
     for tspan, maxiters in (((0.0, 0.37), None), ((0.0, 5.0), 23)):
         for dev, poll in ((False, 1), (True, 1), (True, 7), (True, 64)):
             ode = th.semidiscretize(semi, tspan)
-            sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=0.9)),
+            ac = th.AnalysisCallback(semi, interval=5)
+            sol = th.solve(ode, th.CarpenterKennedy2N54(),
+                           callback=th.CallbackSet(th.StepsizeCallback(cfl=0.9), ac),
                            maxiters=maxiters, device_steps=dev, poll_every=poll)
             runs[(tspan, dev, poll)] = sol
         host = runs[(tspan, False, 1)]
         assert host.nsteps == (maxiters if maxiters else host.nsteps) and host.nsteps > 5
         if maxiters is None:
             assert host.t == tspan[1]
+        iters = [r["iter"] for r in host.analysis]
+        assert iters == sorted(set(list(range(0, host.nsteps, 5)) + [host.nsteps])), iters
+        assert host.analysis[-1]["t"] == host.t
         for poll in (1, 7, 64):
             d = runs[(tspan, True, poll)]
             assert (d.nsteps, d.t) == (host.nsteps, host.t), (poll, d.nsteps, host.nsteps, d.t, host.t)
             assert np.array_equal(d.u, host.u), poll
+            assert len(d.analysis) == len(host.analysis), (poll, [r["iter"] for r in d.analysis], iters)
+            for rd, rh in zip(d.analysis, host.analysis):
+                assert (rd["t"], rd["iter"]) == (rh["t"], rh["iter"])
+                for key in ("l2", "linf", "integrals"):
+                    assert np.array_equal(rd[key], rh[key]), (poll, rd["iter"], key)
+                assert rd["entropy"] == rh["entropy"]
     # an empty time span takes one step of size 0, as step! with limit_dt! does
     for dev in (False, True):
         ode = th.semidiscretize(semi, (0.2, 0.2))

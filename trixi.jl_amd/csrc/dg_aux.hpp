@@ -417,13 +417,14 @@ template <int N, int NA> struct AnalysisConst {
 // element's solution and node coordinates are staged in LDS; each thread
 // interpolates to its analysis points with the tensor-product Vandermonde
 // (the reference applies it dimension by dimension; same sums, other order).
-// partial[e] = {l2 contributions [V], linf [V]}.
+// partial[e] = {l2 contributions [V], linf [V]}. The time is tdev[0] when
+// tdev is given (the device clock of a running solve), else t.
 template <typename T, int D, int N, int NA>
 __global__ __launch_bounds__(256) void error_norms_kernel(const T *__restrict__ u,
                                                           const double *__restrict__ inv_jac,
                                                           const double *__restrict__ centers,
-                                                          int E, double t, ExactParams ep,
-                                                          AnalysisConst<N, NA> ac,
+                                                          int E, double t_host, const double *__restrict__ tdev,
+                                                          ExactParams ep, AnalysisConst<N, NA> ac,
                                                           double *__restrict__ partial) {
   constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2, NQ = ipow_c(NA, D);
   constexpr int W = V + D;  // interpolate u and x together
@@ -432,6 +433,7 @@ __global__ __launch_bounds__(256) void error_norms_kernel(const T *__restrict__ 
   const int e = blockIdx.x;
   if (e >= E) return;
   const int tid = threadIdx.x;
+  const double t = tdev ? tdev[0] : t_host;
   const double jac = 1.0 / inv_jac[e];
   for (int q = tid; q < NN; q += 256) {
     int idx[3] = {q % n, (q / n) % n, q / (n * n)};

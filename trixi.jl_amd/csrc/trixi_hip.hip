@@ -183,6 +183,11 @@ struct trixi_hip_ctx {
   double *d_alpha_hg = nullptr;
   bool mlp_ready = false;
   int mlp_step = 0;
+
+  // analysis records of trixi_hip_solve (ring of rec_cap records)
+  double *d_records = nullptr;
+  int rec_cap = 0, rec_interval = 0;
+  double tv_err = -1, tv_int = -1;  // total volume as the error norms / integrals compute it
 };
 
 namespace {
@@ -450,7 +455,8 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
        hipMemcpy(ctx->d_invj, mesh->inverse_jacobian, E * 8, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(ctx->d_centers, mesh->element_centers, (size_t)D * E * 8, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(ctx->d_alpha_raw, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess;
+       hipMemcpy(ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(ctx->d_scalars + 32, ctx->weights.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess;
   if (!ok) {
     trixi_hip_destroy(ctx);
     fprintf(stderr, "trixi_hip_create: upload failed\n");
@@ -468,7 +474,7 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
   void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_u, ctx->d_utmp, ctx->d_u32, ctx->d_du,
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
                   ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
-                  ctx->d_alpha_hg};
+                  ctx->d_alpha_hg, ctx->d_records};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -648,9 +654,92 @@ int launch_max_speed(trixi_hip_ctx *ctx) {
   return nb;
 }
 
-// d_scalars layout: [0] t, [1] dt, [2] max_dt result, [3, 48) reductions,
-// [48, 55) step control {t_end, cfl, last, done, steps, error, max_steps}
-constexpr int kCtl = 48;
+// d_scalars layout: [0] t, [1] dt, [2] max_dt result, [8, 8 + 2V) error-norm
+// sums, [20, 21 + V) integral sums, [32, 32 + n) LGL weights, [48, 56) step
+// control {t_end, cfl, last, done, steps, error, max_steps, analyses made}
+constexpr int kErrSums = 8, kIntSums = 20, kWeights = 32, kCtl = 48;
+
+// one analysis record: {t, iter, l2 sums [V], linf [V], integral sums [V], entropy sum}
+int rec_width(const trixi_hip_ctx *ctx) { return 3 + 3 * ctx->V; }
+
+// total_volume (analysis_dg3d.jl:123-161 normalisation, integrate with
+// normalize=true), evaluated once in the two forms the reductions use.
+int total_volumes(trixi_hip_ctx *ctx) {
+  if (ctx->tv_err > 0) return TRIXI_HIP_OK;
+  std::vector<double> ij(ctx->E);
+  HIP_TRY(hipMemcpy(ij.data(), ctx->d_invj, ctx->E * 8, hipMemcpyDeviceToHost));
+  double a = 0, b = 0;
+  for (int e = 0; e < ctx->E; ++e) {
+    a += std::pow(1.0 / ij[e], ctx->D) * std::pow(2.0, ctx->D);
+    b += std::pow(2.0 / ij[e], ctx->D);
+  }
+  ctx->tv_err = a;
+  ctx->tv_int = b;
+  return TRIXI_HIP_OK;
+}
+
+// calc_error_norms sums of the resident FP64 state at time t (tdev[0] if tdev)
+// -> out[0, 2V): {sum w J (u - u_exact)^2 [V], max |u - u_exact| [V]}.
+int enqueue_error_sums(trixi_hip_ctx *ctx, double t, const double *tdev, double *out) {
+  const int V = ctx->V, E = ctx->E;
+  if ((size_t)E * 2 * V > ctx->partial_cap) return set_err(ctx, TRIXI_HIP_EINVAL, "partials too small");
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    constexpr int NA = 2 * N + 1;
+    AnalysisConst<N, NA> ac{};
+    for (int i = 0; i < NA * (N + 1); ++i) ac.vdm[i] = ctx->a_vdm[i];
+    for (int i = 0; i < NA; ++i) ac.aw[i] = ctx->a_weights[i];
+    for (int i = 0; i <= N; ++i) ac.nodes[i] = ctx->nodes[i];
+    hipLaunchKernelGGL((error_norms_kernel<double, D, N, NA>), dim3(E), dim3(256), 0, ctx->stream, ctx->d_u,
+                       ctx->d_invj, ctx->d_centers, E, t, tdev, ctx->exact, ac, ctx->d_partial);
+  });
+  if (!ok) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(reduce_columns_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, E, 2 * V, V, out);
+  HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
+
+// integrate_via_indices sums of u and the mathematical entropy -> out[0, V + 1).
+int enqueue_integral_sums(trixi_hip_ctx *ctx, double *out) {
+  const int V = ctx->V, E = ctx->E;
+  if ((size_t)E * (V + 1) > ctx->partial_cap) return set_err(ctx, TRIXI_HIP_EINVAL, "partials too small");
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    hipLaunchKernelGGL((integrals_kernel<double, D, N>), dim3((E + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_u, ctx->d_invj, E, ctx->cfg.gamma, ctx->d_scalars + kWeights, ctx->d_partial);
+  });
+  if (!ok) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(reduce_columns_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, E, V + 1, V + 1,
+                     out);
+  HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
+
+// Appends {t, steps, sums} of the current device state to the record ring.
+__global__ void record_kernel(const double *__restrict__ scalars, double *__restrict__ ctl,
+                              double *__restrict__ records, int cap, int V) {
+  if (threadIdx.x != 0) return;
+  const long long k = (long long)ctl[7];
+  double *rec = records + (size_t)(k % cap) * (3 + 3 * V);
+  rec[0] = scalars[0];
+  rec[1] = ctl[4];
+  for (int i = 0; i < 2 * V; ++i) rec[2 + i] = scalars[kErrSums + i];
+  for (int i = 0; i <= V; ++i) rec[2 + 2 * V + i] = scalars[kIntSums + i];
+  ctl[7] = double(k + 1);
+}
+
+// One analysis of the running solve, at the device time, without host sync.
+int enqueue_record(trixi_hip_ctx *ctx) {
+  int r = enqueue_error_sums(ctx, 0.0, ctx->d_scalars, ctx->d_scalars + kErrSums);
+  if (!r) r = enqueue_integral_sums(ctx, ctx->d_scalars + kIntSums);
+  if (r) return r;
+  hipLaunchKernelGGL(record_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->d_scalars, ctx->d_scalars + kCtl,
+                     ctx->d_records, ctx->rec_cap, ctx->V);
+  HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
 
 __global__ void solve_init_kernel(double *tdt, double *ctl, double t, double dt, double t_end, double cfl,
                                   double last, double max_steps) {
@@ -664,6 +753,7 @@ __global__ void solve_init_kernel(double *tdt, double *ctl, double t, double dt,
     ctl[4] = 0.0;
     ctl[5] = 0.0;
     ctl[6] = max_steps;
+    ctl[7] = 0.0;
   }
 }
 
@@ -745,21 +835,37 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
   // Replay without host synchronisation, reading the control block between
   // batches. A batch is at most poll_every steps and, near the end, no more than
   // the steps left at the current dt (floor), so few replays run after the end.
-  double h[7] = {0}, cur[2] = {t0, dt};
+  // With analysis on (trixi_hip_set_analysis), batches end at multiples of the
+  // interval, where the analysis is enqueued behind them; the first record is
+  // the initial state and the last one the final state (analysis.jl:121-122,
+  // 143-230).
+  const int every = ctx->d_records ? ctx->rec_interval : 0;
+  if (every > 0 && (r = enqueue_record(ctx))) return r;
+  double h[8] = {0}, cur[2] = {t0, dt};
   int launched = 0;
+  bool recorded = false;  // a record was enqueued behind the last batch
   for (;;) {
     const double left = (t_end - cur[0]) / cur[1];  // > 0: not done, dt > 0
     int batch = left >= poll_every ? poll_every : (left >= 1.0 ? (int)left : 1);
     batch = std::min(batch, std::max(1, max_steps - (int)h[4]));
+    if (every > 0) batch = std::min(batch, every - (int)h[4] % every);
     for (int i = 0; i < batch; ++i) HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
     launched += batch;
+    recorded = every > 0 && ((int)h[4] + batch) % every == 0;
+    if (recorded && (r = enqueue_record(ctx))) return r;
     HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(cur, tdt, sizeof(cur), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (t_out) *t_out = cur[0];
     if (steps_out) *steps_out = (int)h[4];
     if (h[5] != 0.0) return set_err(ctx, TRIXI_HIP_ESTATE, "time step size `dt` is NaN");
-    if (h[3] != 0.0) break;
+    if (h[3] != 0.0) {
+      if (every > 0 && !recorded) {  // the final state, if the last batch did not record it
+        if ((r = enqueue_record(ctx))) return r;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+      }
+      break;
+    }
     if (launched > max_steps) return set_err(ctx, TRIXI_HIP_ESTATE, "step control did not finish");
   }
   return TRIXI_HIP_OK;
@@ -930,35 +1036,15 @@ int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *
 
 int trixi_hip_error_norms(trixi_hip_ctx *ctx, double t, double *l2, double *linf) {
   if (!ctx || !l2 || !linf) return TRIXI_HIP_EINVAL;
-  const int V = ctx->V, E = ctx->E;
-  if ((size_t)E * 2 * V > ctx->partial_cap) return set_err(ctx, TRIXI_HIP_EINVAL, "partials too small");
-  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
-    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
-    constexpr int NA = 2 * N + 1;
-    AnalysisConst<N, NA> ac{};
-    for (int i = 0; i < NA * (N + 1); ++i) ac.vdm[i] = ctx->a_vdm[i];
-    for (int i = 0; i < NA; ++i) ac.aw[i] = ctx->a_weights[i];
-    for (int i = 0; i <= N; ++i) ac.nodes[i] = ctx->nodes[i];
-    hipLaunchKernelGGL((error_norms_kernel<double, D, N, NA>), dim3(E), dim3(256), 0, ctx->stream, ctx->d_u,
-                       ctx->d_invj, ctx->d_centers, E, t, ctx->exact, ac, ctx->d_partial);
-  });
-  if (!ok) return TRIXI_HIP_EINVAL;
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(reduce_columns_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, E, 2 * V, V,
-                     ctx->d_scalars + 8);
-  HIP_TRY(hipGetLastError());
+  const int V = ctx->V;
+  int r = enqueue_error_sums(ctx, t, nullptr, ctx->d_scalars + kErrSums);
+  if (r) return r;
   double res[16];
-  HIP_TRY(hipMemcpyAsync(res, ctx->d_scalars + 8, 2 * V * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(res, ctx->d_scalars + kErrSums, 2 * V * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  double tv = 0;
-  {
-    // total_volume(mesh) = length_level_0^ndims = sum of element volumes
-    std::vector<double> ij(E);
-    HIP_TRY(hipMemcpy(ij.data(), ctx->d_invj, E * 8, hipMemcpyDeviceToHost));
-    for (int e = 0; e < E; ++e) tv += std::pow(1.0 / ij[e], ctx->D) * std::pow(2.0, ctx->D);
-  }
+  if ((r = total_volumes(ctx))) return r;
   for (int v = 0; v < V; ++v) {
-    l2[v] = std::sqrt(res[v] / tv);
+    l2[v] = std::sqrt(res[v] / ctx->tv_err);
     linf[v] = res[V + v];
   }
   return TRIXI_HIP_OK;
@@ -966,29 +1052,60 @@ int trixi_hip_error_norms(trixi_hip_ctx *ctx, double t, double *l2, double *linf
 
 int trixi_hip_integrals(trixi_hip_ctx *ctx, double *cons_integrals, double *entropy_integral) {
   if (!ctx || !cons_integrals || !entropy_integral) return TRIXI_HIP_EINVAL;
-  const int V = ctx->V, E = ctx->E;
-  if ((size_t)E * (V + 1) > ctx->partial_cap) return set_err(ctx, TRIXI_HIP_EINVAL, "partials too small");
-  double *d_w = ctx->d_scalars + 32;
-  HIP_TRY(hipMemcpyAsync(d_w, ctx->weights.data(), ctx->n * 8, hipMemcpyHostToDevice, ctx->stream));
-  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
-    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
-    hipLaunchKernelGGL((integrals_kernel<double, D, N>), dim3((E + 255) / 256), dim3(256), 0, ctx->stream,
-                       ctx->d_u, ctx->d_invj, E, ctx->cfg.gamma, d_w, ctx->d_partial);
-  });
-  if (!ok) return TRIXI_HIP_EINVAL;
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(reduce_columns_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, E, V + 1, V + 1,
-                     ctx->d_scalars + 8);
-  HIP_TRY(hipGetLastError());
+  const int V = ctx->V;
+  int r = enqueue_integral_sums(ctx, ctx->d_scalars + kIntSums);
+  if (r) return r;
   double res[8];
-  HIP_TRY(hipMemcpyAsync(res, ctx->d_scalars + 8, (V + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(res, ctx->d_scalars + kIntSums, (V + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  std::vector<double> ij(E);
-  HIP_TRY(hipMemcpy(ij.data(), ctx->d_invj, E * 8, hipMemcpyDeviceToHost));
-  double tv = 0;
-  for (int e = 0; e < E; ++e) tv += std::pow(2.0 / ij[e], ctx->D);
-  for (int v = 0; v < V; ++v) cons_integrals[v] = res[v] / tv;
-  *entropy_integral = res[V] / tv;
+  if ((r = total_volumes(ctx))) return r;
+  for (int v = 0; v < V; ++v) cons_integrals[v] = res[v] / ctx->tv_int;
+  *entropy_integral = res[V] / ctx->tv_int;
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_set_analysis(trixi_hip_ctx *ctx, int interval, int capacity) {
+  if (!ctx || interval < 0 || (interval > 0 && capacity <= 0)) return TRIXI_HIP_EINVAL;
+  if (ctx->d_records) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipFree(ctx->d_records));
+    ctx->d_records = nullptr;
+  }
+  ctx->rec_interval = interval;
+  ctx->rec_cap = interval > 0 ? capacity : 0;
+  if (interval > 0) HIP_TRY(hipMalloc(&ctx->d_records, (size_t)capacity * rec_width(ctx) * 8));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_get_analysis(trixi_hip_ctx *ctx, double *records, int max_records, int *count) {
+  if (!ctx || !count || max_records < 0 || (max_records > 0 && !records)) return TRIXI_HIP_EINVAL;
+  *count = 0;
+  if (!ctx->d_records) return TRIXI_HIP_OK;
+  double total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, ctx->d_scalars + kCtl + 7, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const int W = rec_width(ctx), V = ctx->V;
+  const long long made = (long long)total;
+  const int kept = (int)std::min<long long>(made, ctx->rec_cap);
+  std::vector<double> raw((size_t)ctx->rec_cap * W);
+  HIP_TRY(hipMemcpy(raw.data(), ctx->d_records, raw.size() * 8, hipMemcpyDeviceToHost));
+  int r = total_volumes(ctx);
+  if (r) return r;
+  const int nout = std::min(kept, max_records);
+  for (int i = 0; i < nout; ++i) {  // the newest nout records, oldest first
+    const long long k = made - nout + i;
+    const double *src = raw.data() + (size_t)(k % ctx->rec_cap) * W;
+    double *dst = records + (size_t)i * W;
+    dst[0] = src[0];
+    dst[1] = src[1];
+    for (int v = 0; v < V; ++v) {
+      dst[2 + v] = std::sqrt(src[2 + v] / ctx->tv_err);
+      dst[2 + V + v] = src[2 + V + v];
+      dst[2 + 2 * V + v] = src[2 + 2 * V + v] / ctx->tv_int;
+    }
+    dst[2 + 3 * V] = src[2 + 3 * V] / ctx->tv_int;
+  }
+  *count = nout;
   return TRIXI_HIP_OK;
 }
 

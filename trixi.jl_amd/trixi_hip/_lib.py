@@ -121,6 +121,8 @@ SIGNATURES = [
     ("trixi_hip_set_exact_solution", _I, [_P, _I, _PD, _I, _PD]),
     ("trixi_hip_error_norms", _I, [_P, _D, _PD, _PD]),
     ("trixi_hip_integrals", _I, [_P, _PD, _PD]),
+    ("trixi_hip_set_analysis", _I, [_P, _I, _I]),
+    ("trixi_hip_get_analysis", _I, [_P, _PD, _I, _PI]),
     ("trixi_hip_get_tile", _I, [_P, _PI, _PI]),
     ("trixi_hip_mlp_set_params", _I, [_P, _P, _P, _P]),
     ("trixi_hip_mlp_get_params", _I, [_P, _P]),

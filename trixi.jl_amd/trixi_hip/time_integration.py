@@ -52,9 +52,10 @@ class AnalysisCallback:
     """Error norms (l2, linf against the initial condition at time t) every
     `interval` steps and at the end (analysis.jl:46-236, 380-470)."""
 
-    def __init__(self, semi, interval=100):
+    def __init__(self, semi, interval=100, capacity=1 << 16):
         self.semi = semi
         self.interval = int(interval)
+        self.capacity = int(capacity)  # records kept on the device during a device-controlled solve
         self.history = []
 
     def analyze(self, t, it):
@@ -63,9 +64,25 @@ class AnalysisCallback:
         l2 = (ctypes.c_double * V)()
         linf = (ctypes.c_double * V)()
         _lib.check(semi._lib.trixi_hip_error_norms(semi.ctx, float(t), l2, linf), semi.ctx, "error_norms")
-        rec = {"t": t, "iter": it, "l2": list(l2), "linf": list(linf)}
+        cons, ent = self.integrals()
+        rec = {"t": t, "iter": it, "l2": list(l2), "linf": list(linf), "integrals": cons, "entropy": ent}
         self.history.append(rec)
         return rec
+
+    def device_records(self):
+        """Records the device made during the last trixi_hip_solve."""
+        semi = self.semi
+        V = semi.nvariables()
+        W = 3 + 3 * V
+        cap = self.capacity
+        buf = (ctypes.c_double * (cap * W))()
+        n = ctypes.c_int()
+        _lib.check(semi._lib.trixi_hip_get_analysis(semi.ctx, buf, cap, ctypes.byref(n)), semi.ctx, "get_analysis")
+        a = np.frombuffer(buf, dtype=np.float64).reshape(cap, W)[:n.value]
+        for r in a:
+            self.history.append({"t": float(r[0]), "iter": int(r[1]), "l2": list(r[2:2 + V]),
+                                 "linf": list(r[2 + V:2 + 2 * V]), "integrals": list(r[2 + 2 * V:2 + 3 * V]),
+                                 "entropy": float(r[2 + 3 * V])})
 
     def integrals(self):
         semi = self.semi
@@ -103,11 +120,13 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
     """`solve(ode, CarpenterKennedy2N54(); dt, callback)` on the device.
 
     The state lives in the HIP context for the whole run. With a
-    StepsizeCallback (interval 1) and no periodic analysis, the whole loop runs
-    on the device (trixi_hip_solve: step graph with device-side dt and
-    limit_dt, host reads the device every `poll_every` steps); otherwise one
-    call of trixi_hip_rk_step (a hipGraph replay of all stages) advances one
-    step and the host computes dt."""
+    StepsizeCallback (interval 1) the whole loop runs on the device
+    (trixi_hip_solve: step graph with device-side dt and limit_dt, periodic
+    analysis enqueued on the device, host reads the device every `poll_every`
+    steps); otherwise one call of trixi_hip_rk_step (a hipGraph replay of all
+    stages) advances one step and the host computes dt. An AnalysisCallback
+    records the initial state, every `interval`-th step and the final state
+    (analysis.jl:121-122, 143-230)."""
     semi = ode.p
     lib = _lib.load()
     ctx = semi.ctx
@@ -124,9 +143,13 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
     c = (ctypes.c_double * na)(*alg.c)
     it = 0
     wall0 = time.perf_counter()
-    on_device = (device_steps and use_graph and stepsize and stepsize[0].interval == 1
-                 and not (analysis and analysis[0].interval > 0))
+    on_device = bool(device_steps and use_graph and stepsize and stepsize[0].interval == 1)
+    records_on_device = bool(on_device and analysis and analysis[0].interval > 0)
+    if analysis and not records_on_device:
+        analysis[0].analyze(t0, 0)
     if on_device:
+        _lib.check(lib.trixi_hip_set_analysis(ctx, analysis[0].interval if records_on_device else 0,
+                                              analysis[0].capacity if records_on_device else 0), ctx, "set_analysis")
         tout, nsteps = ctypes.c_double(), ctypes.c_int()
         rc = lib.trixi_hip_solve(ctx, a, b, c, na, float(t0), float(t_end), stepsize[0].cfl,
                                  int(maxiters) if maxiters is not None else 2 ** 31 - 1, int(poll_every),
@@ -135,6 +158,8 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
             raise FloatingPointError("time step size `dt` is NaN")
         _lib.check(rc, ctx, "solve")
         t, it = tout.value, nsteps.value
+        if records_on_device:
+            analysis[0].device_records()
     else:
         if stepsize:
             dt = stepsize[0].calculate_dt(semi)
@@ -158,7 +183,7 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
             break
     _lib.check(lib.trixi_hip_synchronize(ctx), ctx, "synchronize")
     wall = time.perf_counter() - wall0
-    if analysis:
+    if analysis and not records_on_device:
         analysis[0].analyze(t, it)
     u = None
     if save_final:
