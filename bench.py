@@ -54,7 +54,7 @@ def parse():
 
 
 def make_semi(th, level, polydeg, precision, tile=(0, 0)):
-    mesh = th.TreeMesh((0.0, 0.0, 0.0), (2.0, 2.0, 2.0), initial_refinement_level=level)
+    mesh = th.TreeMesh((0.0, 0.0, 0.0), (2.0, 2.0, 2.0), initial_refinement_level=level, periodicity=True)
     solver = th.DGSEM(polydeg=polydeg, surface_flux=th.flux_lax_friedrichs,
                       volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_ranocha))
     return th.SemidiscretizationHyperbolic(mesh, th.CompressibleEulerEquations3D(1.4),

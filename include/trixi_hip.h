@@ -1,6 +1,6 @@
 /*
  * trixi_hip.h -- C ABI of the MI355X DGSEM right-hand side for the compressible
- * Euler equations on a conforming periodic TreeMesh (2D and 3D).
+ * Euler equations on a conforming TreeMesh (2D and 3D), periodic or with Dirichlet boundaries.
  *
  * The library replaces the CPU methods that Trixi.jl dispatches from
  *   rhs_hyperbolic!(du_ode, u_ode, semi, t)
@@ -55,6 +55,12 @@ extern "C" {
 #define TRIXI_HIP_FLUX_RANOCHA 2   /* flux_ranocha */
 #define TRIXI_HIP_FLUX_CENTRAL 3   /* flux_central */
 
+/* boundary conditions of a non-periodic TreeMesh (trixi_hip_config.boundary_condition) */
+#define TRIXI_HIP_BC_NONE 0      /* fully periodic mesh: no boundaries */
+#define TRIXI_HIP_BC_DIRICHLET 1 /* BoundaryConditionDirichlet(initial_condition) on every boundary
+                                    (equations.jl:159-182); the boundary value function is the
+                                    one set by trixi_hip_set_exact_solution */
+
 /* source terms */
 #define TRIXI_HIP_SRC_NONE 0
 #define TRIXI_HIP_SRC_CONVERGENCE_TEST 1 /* source_terms_convergence_test */
@@ -100,6 +106,8 @@ typedef struct trixi_hip_config {
   int32_t source_terms;    /* TRIXI_HIP_SRC_* */
   int32_t elements_per_block; /* 0 = autotuned default; else workgroup tile (elements) */
   int32_t threads_per_block;  /* 0 = autotuned default */
+  int32_t nboundaries;        /* B: boundary faces of a non-periodic mesh (0 if periodic) */
+  int32_t boundary_condition; /* TRIXI_HIP_BC_*; must be set when nboundaries > 0 */
 } trixi_hip_config;
 
 /* Mesh data as held in the reference cache (containers_{2,3}d.jl). */
@@ -108,6 +116,12 @@ typedef struct trixi_hip_mesh {
   const int32_t *orientations;     /* [I] interfaces.orientations, 0-based (x=0, y=1, z=2) */
   const double *inverse_jacobian;  /* [E] elements.inverse_jacobian */
   const double *element_centers;   /* [ndims, E] tree.coordinates[:, elements.cell_ids] */
+  /* boundaries container (containers_2d.jl:475-543, containers_3d.jl), B entries;
+   * every element face is either in an interface or a boundary */
+  const int32_t *boundary_neighbor_ids;   /* [B] boundaries.neighbor_ids, 0-based element */
+  const int32_t *boundary_orientations;   /* [B] boundaries.orientations, 0-based */
+  const int32_t *boundary_neighbor_sides; /* [B] boundaries.neighbor_sides: 1 = element left of
+                                             the boundary (its + face), 2 = right (its - face) */
 } trixi_hip_mesh;
 
 /* LobattoLegendreBasis fields (basis_lobatto_legendre.jl:17-86), row-major

@@ -20,7 +20,8 @@ class _Config(ctypes.Structure):
                 ("coordinates_min", ctypes.c_double * 3), ("coordinates_max", ctypes.c_double * 3),
                 ("gamma", ctypes.c_double), ("volume_integral", ctypes.c_int), ("volume_flux", ctypes.c_int),
                 ("surface_flux", ctypes.c_int), ("fv_flux", ctypes.c_int), ("alpha_max", ctypes.c_double),
-                ("alpha_min", ctypes.c_double), ("alpha_smooth", ctypes.c_int), ("source_terms", ctypes.c_int)]
+                ("alpha_min", ctypes.c_double), ("alpha_smooth", ctypes.c_int), ("source_terms", ctypes.c_int),
+                ("periodicity", ctypes.c_int * 3), ("boundary_condition", ctypes.c_int)]
 
 
 def build():
@@ -40,12 +41,13 @@ def _load():
         L.oracle_create.restype = P
         L.oracle_create.argtypes = [ctypes.POINTER(_Config)]
         L.oracle_destroy.argtypes = [P]
-        for nm in ("oracle_nelements", "oracle_ninterfaces", "oracle_nnodes"):
+        for nm in ("oracle_nelements", "oracle_ninterfaces", "oracle_nboundaries", "oracle_nnodes"):
             getattr(L, nm).restype = I
             getattr(L, nm).argtypes = [P]
         L.oracle_ndofs.restype = ctypes.c_long
         L.oracle_ndofs.argtypes = [P]
         L.oracle_get_mesh.argtypes = [P] + [P] * 6
+        L.oracle_get_boundaries.argtypes = [P] + [P] * 3
         L.oracle_get_basis.argtypes = [P] + [P] * 6
         L.oracle_set_initial_condition.argtypes = [P, I, P, I, P]
         L.oracle_compute_coefficients.argtypes = [P, D, P]
@@ -80,11 +82,13 @@ class Oracle:
     """One oracle discretisation: Oracle(ndims=3, polydeg=3, level=2, cmin=..., cmax=...,
     volume_integral='fd', volume_flux='ranocha', surface_flux='llf', fv_flux='llf',
     alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, source='none'|'convergence_test',
-    gamma=1.4)."""
+    gamma=1.4, periodicity=True). Non-periodic boundaries carry
+    BoundaryConditionDirichlet(initial_condition) with the initial condition set
+    by set_initial_condition."""
 
     def __init__(self, ndims, polydeg, level, cmin, cmax, volume_integral="fd", volume_flux="ranocha",
                  surface_flux="llf", fv_flux="llf", alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
-                 source="none", gamma=1.4):
+                 source="none", gamma=1.4, periodicity=True):
         L = _load()
         c = _Config()
         c.ndims, c.polydeg, c.initial_refinement_level = ndims, polydeg, level
@@ -98,6 +102,10 @@ class Oracle:
         c.fv_flux = FLUX[fv_flux]
         c.alpha_max, c.alpha_min, c.alpha_smooth = alpha_max, alpha_min, int(alpha_smooth)
         c.source_terms = 1 if source == "convergence_test" else 0
+        per = (periodicity,) * ndims if isinstance(periodicity, bool) else tuple(periodicity)
+        for d in range(ndims):
+            c.periodicity[d] = int(bool(per[d]))
+        c.boundary_condition = 0 if all(per) else 1
         self._cfg = c
         self._c = L.oracle_create(ctypes.byref(c))
         if not self._c:
@@ -108,6 +116,7 @@ class Oracle:
         self.nvars = ndims + 2
         self.nelements = L.oracle_nelements(self._c)
         self.ninterfaces = L.oracle_ninterfaces(self._c)
+        self.nboundaries = L.oracle_nboundaries(self._c)
         self.ndofs = L.oracle_ndofs(self._c)
         self.nvalues = self.ndofs * self.nvars
 
@@ -131,6 +140,13 @@ class Oracle:
         return dict(neighbor_ids=nid.reshape(I, 2), orientations=ori, inverse_jacobian=ij,
                     centers=cen.reshape(E, D), node_coordinates=xc.reshape(E, NN, D),
                     element_neighbors=en.reshape(E, 2 * D))
+
+    def boundaries(self):
+        """(neighbor_ids, orientations, neighbor_sides) of the boundaries container, 0-based."""
+        B = self.nboundaries
+        el, ori, side = np.empty(B, np.int32), np.empty(B, np.int32), np.empty(B, np.int32)
+        self.L.oracle_get_boundaries(self._c, _p(el), _p(ori), _p(side))
+        return el, ori, side
 
     def basis(self):
         n = self.n

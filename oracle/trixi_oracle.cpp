@@ -418,6 +418,8 @@ extern "C" {
 enum { VI_WEAK = 0, VI_FLUX_DIFFERENCING = 1, VI_SHOCK_CAPTURING_HG = 2, VI_PURE_FV = 3 };
 // Source-term selector.
 enum { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1 };
+// BC_DIRICHLET_IC: BoundaryConditionDirichlet(initial_condition), equations.jl:159-182
+enum { BC_NONE = 0, BC_DIRICHLET_IC = 1 };
 // Initial-condition selector.
 enum {
     IC_CONSTANT = 0,
@@ -443,6 +445,8 @@ struct oracle_config {
     double alpha_min;
     int alpha_smooth;
     int source_terms;     // SRC_*
+    int periodicity[3];   // TreeMesh(...; periodicity) per dimension (nonzero = periodic)
+    int boundary_condition;  // BC_* for every non-periodic boundary
 };
 
 struct oracle_ctx;
@@ -460,7 +464,8 @@ struct oracle_ctx {
     std::vector<double> node_coords;    // [D, NN, E]
     std::vector<int32_t> neighbor_ids;  // [2, I] 0-based element ids
     std::vector<int32_t> orientations;  // [I] 0-based orientation
-    std::vector<int32_t> elem_nbr;      // [2D, E] neighbour element per direction (0-based dir)
+    std::vector<int32_t> elem_nbr;      // [2D, E] neighbour element per direction (0-based dir), -1 at a boundary
+    std::vector<int32_t> bnd_elem, bnd_dir;  // boundaries: element and its 0-based direction (init_boundaries!)
     std::vector<double> sfv;            // surface_flux_values
     std::vector<double> alpha, alpha_tmp;
     // analysis exact-solution parameters
@@ -889,6 +894,31 @@ template <int D, typename R> void rhs(oracle_ctx &c, R *du, const R *u, double t
         }
     }
 
+    // prolong2boundaries! + calc_boundary_flux!: dg_2d.jl:680-860, dg_3d.jl:651-768.
+    // BoundaryConditionDirichlet(initial_condition) (equations.jl:164-182): the
+    // outer state is the initial condition at the boundary node and time t; the
+    // element is the left state at its + faces (even 1-based direction).
+    {
+        const Euler<D> eqd(c.cfg.gamma);
+        const int B = (int)c.bnd_elem.size();
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < B; ++b) {
+            const int e = c.bnd_elem[b], dir = c.bnd_dir[b], o = dir / 2;
+            const bool plus = dir % 2 == 1;
+            for (int fn = 0; fn < nf; ++fn) {
+                const int node = face_to_node<D>(n, o, plus ? n - 1 : 0, fn);
+                double ub[D + 2];
+                initial_condition<D>(c, eqd, &c.node_coords[D * (node + NN * e)], t, ub);
+                R ubr[V], f[V];
+                for (int v = 0; v < V; ++v) ubr[v] = R(ub[v]);
+                const R *ui = &u[V * (node + NN * e)];
+                if (plus) num_flux<D>(eq, c.cfg.surface_flux, ui, ubr, o, f);
+                else num_flux<D>(eq, c.cfg.surface_flux, ubr, ui, o, f);
+                for (int v = 0; v < V; ++v) sfv[V * (fn + nf * (dir + 2 * D * e)) + v] = f[v];
+            }
+        }
+    }
+
     // calc_surface_integral!: dg_3d.jl:1337-1394, dg_2d.jl:1245-1287
     const R factor = R(c.basis.inv_weights[0]);
 #pragma omp parallel for schedule(static)
@@ -1043,17 +1073,31 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
         return r;
     };
     for (int e = 0; e < c->E; ++e) lookup[lin(&icoords[c->D * e])] = e;
-    // periodic neighbours (init_neighbors!, abstract_tree.jl:252-313, with the
-    // level-0 cell its own neighbour, serial_tree.jl:119-123)
+    // neighbours (init_neighbors!, abstract_tree.jl:252-313): across a periodic
+    // boundary the cell on the opposite side (the level-0 cell is its own
+    // neighbour, serial_tree.jl:119-123); none (-1) across a non-periodic one
     c->elem_nbr.assign(2 * c->D * c->E, -1);
     for (int e = 0; e < c->E; ++e)
         for (int dir = 0; dir < 2 * c->D; ++dir) {
             int ic[3] = {0, 0, 0};
             for (int d = 0; d < c->D; ++d) ic[d] = icoords[c->D * e + d];
             int o = dir / 2;
-            ic[o] = (ic[o] + (dir % 2 ? 1 : -1) + nside) % nside;
+            int k = ic[o] + (dir % 2 ? 1 : -1);
+            if ((k < 0 || k >= nside) && !cfg->periodicity[o]) continue;
+            ic[o] = (k + nside) % nside;
             c->elem_nbr[dir + 2 * c->D * e] = lookup[lin(ic)];
         }
+    // init_boundaries!: containers_2d.jl:475-543 (direction-major, element order)
+    for (int dir = 0; dir < 2 * c->D; ++dir)
+        for (int e = 0; e < c->E; ++e)
+            if (c->elem_nbr[dir + 2 * c->D * e] < 0) {
+                c->bnd_elem.push_back(e);
+                c->bnd_dir.push_back(dir);
+            }
+    if (!c->bnd_elem.empty() && cfg->boundary_condition != BC_DIRICHLET_IC) {
+        delete c;
+        return nullptr;
+    }
     // init_elements!: containers_3d.jl:87-128
     c->inverse_jacobian.resize(c->E);
     c->node_coords.resize((size_t)c->D * c->NN * c->E);
@@ -1076,6 +1120,7 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
     c->orientations.clear();
     for (int e = 0; e < c->E; ++e)
         for (int o = 0; o < c->D; ++o) {
+            if (c->elem_nbr[2 * o + 1 + 2 * c->D * e] < 0) continue;
             c->neighbor_ids.push_back(e);
             c->neighbor_ids.push_back(c->elem_nbr[2 * o + 1 + 2 * c->D * e]);
             c->orientations.push_back(o);
@@ -1090,6 +1135,16 @@ void oracle_destroy(oracle_ctx *c) { delete c; }
 
 int oracle_nelements(const oracle_ctx *c) { return c->E; }
 int oracle_ninterfaces(const oracle_ctx *c) { return c->I; }
+int oracle_nboundaries(const oracle_ctx *c) { return (int)c->bnd_elem.size(); }
+// boundaries container (0-based): element, orientation, neighbor_side (1: element
+// left of the boundary, 2: right), as init_boundaries! fills them
+void oracle_get_boundaries(const oracle_ctx *c, int32_t *elem, int32_t *orientation, int32_t *side) {
+    for (size_t b = 0; b < c->bnd_elem.size(); ++b) {
+        if (elem) elem[b] = c->bnd_elem[b];
+        if (orientation) orientation[b] = c->bnd_dir[b] / 2;
+        if (side) side[b] = c->bnd_dir[b] % 2 == 1 ? 1 : 2;
+    }
+}
 int oracle_nnodes(const oracle_ctx *c) { return c->n; }
 long oracle_ndofs(const oracle_ctx *c) { return (long)c->NN * c->E; }
 

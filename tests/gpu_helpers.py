@@ -48,11 +48,14 @@ def build(spec, precision="fp64"):
     else:
         VI = th.VolumeIntegralPureLGLFiniteVolume(FLUX[fvflux])
     solver = th.DGSEM(basis, FLUX[spec.get("surface_flux", "llf")], VI)
-    mesh = th.TreeMesh(spec["cmin"], spec["cmax"], initial_refinement_level=spec["level"])
+    mesh = th.TreeMesh(spec["cmin"], spec["cmax"], initial_refinement_level=spec["level"],
+                       periodicity=spec.get("periodicity", True))
     ic = make_ic(spec["ic"], spec.get("ic_params", ()))
     src = th.source_terms_convergence_test if spec.get("source") == "convergence_test" else None
-    semi = th.SemidiscretizationHyperbolic(mesh, eq, ic, solver, source_terms=src, precision=precision,
-                                           elements_per_block=spec.get("epb", 0), threads_per_block=spec.get("nt", 0))
+    bc = th.boundary_condition_periodic if mesh.nboundaries == 0 else th.BoundaryConditionDirichlet(ic)
+    semi = th.SemidiscretizationHyperbolic(mesh, eq, ic, solver, source_terms=src, boundary_conditions=bc,
+                                           precision=precision, elements_per_block=spec.get("epb", 0),
+                                           threads_per_block=spec.get("nt", 0))
     return semi, ic
 
 
@@ -61,7 +64,10 @@ def oracle_for(spec):
                volume_integral=spec.get("volume_integral", "fd"), volume_flux=spec.get("volume_flux", "ranocha"),
                surface_flux=spec.get("surface_flux", "llf"), fv_flux=spec.get("fv_flux", "llf"),
                alpha_max=spec.get("alpha_max", 0.5), alpha_min=spec.get("alpha_min", 0.001),
-               source=spec.get("source", "none"))
+               source=spec.get("source", "none"), periodicity=spec.get("periodicity", True))
+    if spec.get("periodicity", True) is not True:  # the boundary values come from the initial condition
+        ic = make_ic(spec["ic"], spec.get("ic_params", ()))
+        o.set_initial_condition(spec["ic"], tuple(ic.params), ic.modes.reshape(-1) if ic.modes is not None else None)
     return o
 
 

@@ -36,6 +36,20 @@ CASES = {
                           source="convergence_test"), 2e-4),
     "2d_p3_pure_fv": (dict(ndims=2, polydeg=3, level=3, cmin=(0, 0), cmax=(2, 2), ic="convergence_test",
                            source="convergence_test", volume_integral="fv"), 1e-4),
+    # non-periodic meshes: BoundaryConditionDirichlet(initial_condition) at time t
+    "2d_p3_weak_mms_bounded": (dict(ndims=2, polydeg=3, level=3, cmin=(0, 0), cmax=(2, 2), ic="convergence_test",
+                                    source="convergence_test", volume_integral="weak", surface_flux="llf_naive",
+                                    periodicity=False), 2e-4),
+    "3d_p3_fd_mms_bounded": (dict(MMS3, volume_integral="fd", surface_flux="llf", periodicity=False), 1e-4),
+    "3d_p3_hg_blast_bounded_y": (dict(BLAST, volume_integral="hg", surface_flux="ranocha", fv_flux="ranocha",
+                                      periodicity=(True, False, True)), 1e-5),
+    "3d_p4_fd_mms_bounded": (dict(ndims=3, polydeg=4, level=1, cmin=(0,) * 3, cmax=(2,) * 3, ic="convergence_test",
+                                  source="convergence_test", periodicity=False), 1e-4),
+    "3d_p2_weak_mms_bounded": (dict(ndims=3, polydeg=2, level=2, cmin=(0,) * 3, cmax=(2,) * 3, ic="convergence_test",
+                                    source="convergence_test", volume_integral="weak", periodicity=False), 1e-4),
+    "2d_p3_hg_khi_bounded": (dict(ndims=2, polydeg=3, level=4, cmin=(-1, -1), cmax=(1, 1), ic="kelvin_helmholtz",
+                                  ic_params=(0.0, 0), volume_integral="hg", surface_flux="llf_naive",
+                                  fv_flux="llf_naive", alpha_max=0.002, alpha_min=1e-4, periodicity=False), 2e-4),
 }
 
 
@@ -100,10 +114,13 @@ def test_native_fp32_entry_point_and_determinism():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("D", [2, 3])
-def test_free_stream_preservation(D):
+@pytest.mark.parametrize("D,per", [(2, True), (3, True), (2, False), (3, False)])
+def test_free_stream_preservation(D, per):
+    """A constant state gives du = 0 to round-off, also with Dirichlet boundaries
+    that impose the same constant state."""
     import trixi_hip as th
-    spec = dict(ndims=D, polydeg=3, level=3, cmin=(-1,) * D, cmax=(1,) * D, ic="constant", surface_flux="llf")
+    spec = dict(ndims=D, polydeg=3, level=3, cmin=(-1,) * D, cmax=(1,) * D, ic="constant", surface_flux="llf",
+                periodicity=per)
     for prec in ("fp64", "fp32"):
         semi, _ = build(spec, prec)
         u = semi.compute_coefficients(0.0)
@@ -116,7 +133,7 @@ def test_free_stream_preservation(D):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["3d_p3_hg_blast", "2d_p3_hg_khi"])
+@pytest.mark.parametrize("name", ["3d_p3_hg_blast", "2d_p3_hg_khi", "2d_p3_hg_khi_bounded"])
 def test_hg_indicator_matches_oracle(name):
     spec = CASES[name][0]
     o = oracle_for(spec)

@@ -95,7 +95,7 @@ def run_kta(case, precision):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d",
-                                  "khi_2d", "vortex_2d", "vortex_2d_constant_one_step"])
+                                  "khi_2d", "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic"])
 def test_kta_trajectory_on_gpu_fp64(case):
     """The whole run (RK stages, dt, analysis) on the device reproduces the
     reference's stored L2/Linf errors."""
@@ -103,8 +103,8 @@ def test_kta_trajectory_on_gpu_fp64(case):
     if c.get("roundoff"):
         assert np.all(l2 < 1e-13) and np.all(linf < 1e-12)
     else:
-        np.testing.assert_allclose(l2, c["l2"], rtol=1e-9)
-        np.testing.assert_allclose(linf, c["linf"], rtol=1e-9)
+        np.testing.assert_allclose(l2, c["l2"], rtol=1e-9, atol=500 * np.finfo(float).eps)
+        np.testing.assert_allclose(linf, c["linf"], rtol=1e-9, atol=500 * np.finfo(float).eps)
 
 
 @pytest.mark.gpu
@@ -141,7 +141,7 @@ def test_torch_device_arrays_after_a_library_context():
         "import sys, numpy as np\n"
         f"sys.path.insert(0, {PKG!r})\n"
         "import trixi_hip as th\n"
-        "m = th.TreeMesh((0, 0), (2, 2), initial_refinement_level=2)\n"
+        "m = th.TreeMesh((0, 0), (2, 2), initial_refinement_level=2, periodicity=True)\n"
         "s = th.SemidiscretizationHyperbolic(m, th.CompressibleEulerEquations2D(1.4),\n"
         "    th.initial_condition_convergence_test, th.DGSEM(polydeg=3, surface_flux=th.flux_lax_friedrichs))\n"
         "s.ctx\n"

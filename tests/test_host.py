@@ -14,16 +14,25 @@ from trixi_hip import _lib
 from trixi_hip.mesh import element_geometry
 
 
-@pytest.mark.parametrize("D,N,level,cmin,cmax", [(3, 3, 2, (0, 0, 0), (2, 2, 2)), (2, 3, 4, (-10, -10), (10, 10)),
-                                                 (3, 4, 1, (-1, -1, -1), (1, 1, 1)), (2, 2, 3, (-1, -1), (1, 1))])
-def test_mesh_and_geometry_match_oracle(D, N, level, cmin, cmax):
+@pytest.mark.parametrize("D,N,level,cmin,cmax,per", [
+    (3, 3, 2, (0, 0, 0), (2, 2, 2), True), (2, 3, 4, (-10, -10), (10, 10), True),
+    (3, 4, 1, (-1, -1, -1), (1, 1, 1), True), (2, 2, 3, (-1, -1), (1, 1), True),
+    (2, 3, 3, (0, 0), (2, 2), False), (3, 3, 2, (0, 0, 0), (2, 2, 2), False),
+    (3, 2, 2, (0, 0, 0), (2, 2, 2), (True, False, True)), (2, 3, 0, (0, 0), (1, 1), False)])
+def test_mesh_and_geometry_match_oracle(D, N, level, cmin, cmax, per):
     from oracle import Oracle
-    o = Oracle(D, N, level, cmin, cmax)
+    o = Oracle(D, N, level, cmin, cmax, periodicity=per)
     m = o.mesh()
-    mesh = th.TreeMesh(cmin, cmax, initial_refinement_level=level)
+    mesh = th.TreeMesh(cmin, cmax, initial_refinement_level=level, periodicity=per)
     basis = th.LobattoLegendreBasis(N)
     ij, xc = element_geometry(mesh, basis)
     assert mesh.nelements == o.nelements and mesh.ninterfaces == o.ninterfaces
+    assert mesh.nboundaries == o.nboundaries
+    assert 2 * mesh.ninterfaces + mesh.nboundaries == 2 * D * mesh.nelements
+    b_el, b_ori, b_side = o.boundaries()
+    np.testing.assert_array_equal(mesh.boundary_neighbor_ids, b_el)
+    np.testing.assert_array_equal(mesh.boundary_orientations, b_ori)
+    np.testing.assert_array_equal(mesh.boundary_neighbor_sides, b_side)
     np.testing.assert_array_equal(mesh.cell_centers, m["centers"])
     np.testing.assert_array_equal(mesh.neighbor_ids, m["neighbor_ids"])
     np.testing.assert_array_equal(mesh.orientations, m["orientations"])
@@ -38,7 +47,7 @@ def test_tree_order_follows_child_sign_table():
     np.testing.assert_array_equal(mesh.cell_centers[:4], [[0.5, 0.5, 0.5], [1.5, 0.5, 0.5], [0.5, 1.5, 0.5],
                                                           [1.5, 1.5, 0.5]])
     # periodic neighbours: +x neighbour centre is +dx modulo the domain
-    mesh = th.TreeMesh((0, 0, 0), (2, 2, 2), initial_refinement_level=3)
+    mesh = th.TreeMesh((0, 0, 0), (2, 2, 2), initial_refinement_level=3, periodicity=True)
     c = mesh.cell_centers
     for d in range(3):
         nb = mesh.element_neighbors[:, 2 * d + 1]
@@ -124,7 +133,7 @@ def test_host_api_config_mapping():
     vi = th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=th.flux_ranocha,
                                            volume_flux_fv=th.FluxLaxFriedrichs(th.max_abs_speed_naive))
     solver = th.DGSEM(basis, th.FluxLaxFriedrichs(th.max_abs_speed_naive), vi)
-    mesh = th.TreeMesh((-1, -1), (1, 1), initial_refinement_level=3)
+    mesh = th.TreeMesh((-1, -1), (1, 1), initial_refinement_level=3, periodicity=True)
     semi = th.SemidiscretizationHyperbolic(mesh, eq, th.initial_condition_kelvin_helmholtz_instability(), solver,
                                            precision="fp32")
     c = semi.config()
@@ -135,5 +144,32 @@ def test_host_api_config_mapping():
     assert semi.ndofs() == 64 * 16
     with pytest.raises(ValueError):
         th.TreeMesh((0, 0), (1, 2), initial_refinement_level=1)
+    assert c.nboundaries == 0 and c.boundary_condition == _lib.BC_NONE
+
+
+def test_boundaries_of_a_bounded_mesh():
+    """TreeMesh defaults to periodicity=false (tree_mesh.jl:108-130): the domain
+    faces become boundaries (init_boundaries!, containers_2d.jl:475-543), which
+    need BoundaryConditionDirichlet(initial_condition)."""
+    mesh = th.TreeMesh((0, 0), (2, 2), initial_refinement_level=2)
+    assert mesh.periodicity == (False, False)
+    assert (mesh.nelements, mesh.ninterfaces, mesh.nboundaries) == (16, 24, 16)
+    # direction-major: 4 boundaries per direction; side 2 on -faces, 1 on +faces
+    np.testing.assert_array_equal(mesh.boundary_orientations, [0] * 8 + [1] * 8)
+    np.testing.assert_array_equal(mesh.boundary_neighbor_sides, [2] * 4 + [1] * 4 + [2] * 4 + [1] * 4)
+    xc = mesh.cell_centers
+    assert np.all(xc[mesh.boundary_neighbor_ids[:4], 0] == 0.25) and np.all(xc[mesh.boundary_neighbor_ids[4:8], 0] == 1.75)
+    eq = th.CompressibleEulerEquations2D(1.4)
+    solver = th.DGSEM(polydeg=3, surface_flux=th.flux_lax_friedrichs)
+    ic = th.initial_condition_convergence_test
     with pytest.raises(ValueError):
-        th.TreeMesh((0, 0), (1, 1), initial_refinement_level=1, periodicity=False)
+        th.SemidiscretizationHyperbolic(mesh, eq, ic, solver)
+    with pytest.raises(ValueError):  # the boundary function must be the initial condition
+        th.SemidiscretizationHyperbolic(mesh, eq, ic, solver,
+                                        boundary_conditions=th.BoundaryConditionDirichlet(th.initial_condition_constant))
+    semi = th.SemidiscretizationHyperbolic(mesh, eq, ic, solver, boundary_conditions=th.BoundaryConditionDirichlet(ic))
+    c = semi.config()
+    assert (c.ninterfaces, c.nboundaries, c.boundary_condition) == (24, 16, _lib.BC_DIRICHLET)
+    with pytest.raises(ValueError):
+        th.SemidiscretizationHyperbolic(th.TreeMesh((0, 0), (2, 2), initial_refinement_level=2, periodicity=True),
+                                        eq, ic, solver, boundary_conditions=th.BoundaryConditionDirichlet(ic))

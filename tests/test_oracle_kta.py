@@ -7,7 +7,7 @@ import pytest
 from conftest import make_oracle
 
 CASES = ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d", "khi_2d",
-         "vortex_2d", "vortex_2d_constant_one_step"]
+         "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -23,8 +23,10 @@ def test_oracle_reproduces_trixi_kta(kta, name):
         assert np.all(l2 < 1e-13) and np.all(linf < 1e-12)
     else:
         assert t == pytest.approx(c["tspan"][1], abs=0, rel=1e-15)
-        np.testing.assert_allclose(l2, c["l2"], rtol=1e-9, atol=0)
-        np.testing.assert_allclose(linf, c["linf"], rtol=1e-9, atol=0)
+        # rtol 1e-9 (tighter than the reference tests' sqrt(eps)) plus their
+        # absolute floor of 500 eps for errors near round-off
+        np.testing.assert_allclose(l2, c["l2"], rtol=1e-9, atol=500 * np.finfo(float).eps)
+        np.testing.assert_allclose(linf, c["linf"], rtol=1e-9, atol=500 * np.finfo(float).eps)
 
 
 def test_planted_mistake_is_caught(kta):
@@ -35,5 +37,17 @@ def test_planted_mistake_is_caught(kta):
     o = make_oracle(spec)
     o.set_initial_condition(c["ic"], c["ic_params"])
     u, t, _ = o.solve(o.compute_coefficients(0.0), 0.0, 5.0, c["cfl"])
+    l2, _ = o.error_norms(u, t)
+    assert np.max(np.abs(l2 / np.array(c["l2"]) - 1)) > 1e-6
+
+
+def test_planted_boundary_mistake_is_caught(kta):
+    """Periodic faces in place of the Dirichlet boundaries of the non-periodic
+    MMS run move its errors far outside the KTA tolerance."""
+    c = kta["mms_2d_nonperiodic"]
+    spec = dict(c["oracle"], periodicity=True)
+    o = make_oracle(spec)
+    o.set_initial_condition(c["ic"], c["ic_params"])
+    u, t, _ = o.solve(o.compute_coefficients(0.0), 0.0, 2.0, c["cfl"])
     l2, _ = o.error_norms(u, t)
     assert np.max(np.abs(l2 / np.array(c["l2"]) - 1)) > 1e-6

@@ -148,7 +148,8 @@ __global__ void hg_smooth_kernel(const double *__restrict__ alpha_raw,
   if (smooth) {
     double m = al;
     for (int dir = 0; dir < 2 * D; ++dir) {
-      double an = 0.5 * alpha_raw[nbr[dir + 2 * D * e]];
+      const int nb = nbr[dir + 2 * D * e];  // < 0: boundary, no interface
+      double an = nb >= 0 ? 0.5 * alpha_raw[nb] : 0.0;
       m = an > m ? an : m;
     }
     al = m;
@@ -540,6 +541,48 @@ __global__ __launch_bounds__(256) void reduce_columns_kernel(const double *__res
     if (threadIdx.x == 0) out[c] = s_red[0];
     __syncthreads();
   }
+}
+
+// Outer states of the boundaries for BoundaryConditionDirichlet(initial_condition)
+// (equations.jl:159-182; calc_boundary_flux_by_direction!, dg_2d.jl:826-860):
+// ghost[b][l] = u_exact(x, t) at face node l of boundary b, where x are the
+// element's face node coordinates (calc_boundary_node_coordinates!,
+// containers_2d.jl:394-413) and l orders the face's free indices with the
+// lower dimension fastest. bnd[2b] = element, bnd[2b + 1] = 0-based direction.
+// The time is tdev[0] + t * tdev[1] (a stage of a running step) or t.
+template <int N> struct NodesConst {
+  double nodes[N + 1];
+};
+
+template <typename T, int D, int N>
+__global__ __launch_bounds__(256) void boundary_state_kernel(const int32_t *__restrict__ bnd, int B,
+                                                             const double *__restrict__ centers,
+                                                             const double *__restrict__ inv_jac,
+                                                             NodesConst<N> nc, double t,
+                                                             const double *__restrict__ tdev, ExactParams ep,
+                                                             T *__restrict__ ghost) {
+  constexpr int n = N + 1, L = ipow_c(n, D - 1), V = D + 2;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= B * L) return;
+  const int b = k / L, l = k - b * L;
+  const int e = bnd[2 * b], dir = bnd[2 * b + 1], o = dir >> 1;
+  const double tt = tdev ? (tdev[0] + t * tdev[1]) : t;
+  const double jac = 1.0 / inv_jac[e];
+  int idx[3];
+  idx[o] = (dir & 1) ? n - 1 : 0;
+  int r = l;
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    if (d != o) {
+      idx[d] = r % n;
+      r /= n;
+    }
+  double x[3], ub[V];
+#pragma unroll
+  for (int d = 0; d < D; ++d) x[d] = centers[d + D * e] + jac * nc.nodes[idx[d]];
+  exact_solution<D>(ep, x, tt, ub);
+#pragma unroll
+  for (int v = 0; v < V; ++v) ghost[(size_t)k * V + v] = T(ub[v]);
 }
 
 }  // namespace trixi_hip

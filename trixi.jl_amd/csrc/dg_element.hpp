@@ -48,7 +48,8 @@ template <typename T, int N> struct DGConst {
 template <typename T> struct ElemArgs {
   const T *__restrict__ u;               // [V, NN, E]
   T *__restrict__ du;                    // [V, NN, E]
-  const int32_t *__restrict__ nbr;       // [2D, E] neighbour element per direction
+  const int32_t *__restrict__ nbr;       // [2D, E] neighbour element per direction; -(b + 1) at boundary b
+  const T *__restrict__ ghost;           // [B, n^(D-1), V] outer states of the boundaries (null if none)
   const double *__restrict__ inv_jac;    // [E]
   const double *__restrict__ centers;    // [D, E]
   const double *__restrict__ alpha_raw;  // [E] HG indicator before smoothing (HG only)
@@ -169,14 +170,23 @@ __device__ __forceinline__ void line_ids(const ElemArgs<T> &a, int e0, int ne, i
   nbp = a.nbr[(2 * o + 1) + 2 * D * e];
 }
 
-// Neighbour face states of the line in direction o (consumed in phase B).
+// Neighbour face states of the line in direction o (consumed in phase B). At a
+// boundary face the outer state comes from the ghost record of that boundary
+// (face node = the lane's line index).
 template <typename T, int D, int N, int EPB>
 __device__ __forceinline__ void line_loads(const ElemArgs<T> &a, LineCtx<T, D + 2> &c, int o, int nbm, int nbp) {
   using LY = Layout<D, N, EPB>;
-  constexpr int n = N + 1, NN = LY::NN, V = D + 2;
+  constexpr int n = N + 1, NN = LY::NN, V = D + 2, L = LY::L;
   const int st = o == 0 ? 1 : (o == 1 ? n : n * n);
-  const T *un_m = a.u + ((size_t)nbm * NN + c.base + (n - 1) * st) * V;
-  const T *un_p = a.u + ((size_t)nbp * NN + c.base) * V;
+  const T *un_m, *un_p;
+  if (a.ghost) {  // kernel-argument uniform: periodic meshes never take this path
+    un_m = nbm >= 0 ? a.u + ((size_t)nbm * NN + c.base + (n - 1) * st) * V
+                    : a.ghost + ((size_t)(-nbm - 1) * L + c.line) * V;
+    un_p = nbp >= 0 ? a.u + ((size_t)nbp * NN + c.base) * V : a.ghost + ((size_t)(-nbp - 1) * L + c.line) * V;
+  } else {
+    un_m = a.u + ((size_t)nbm * NN + c.base + (n - 1) * st) * V;
+    un_p = a.u + ((size_t)nbp * NN + c.base) * V;
+  }
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     c.ul_m[v] = un_m[v];
@@ -578,7 +588,8 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
         double m = al;
 #pragma unroll
         for (int dir = 0; dir < 2 * D; ++dir) {
-          double an = 0.5 * a.alpha_raw[a.nbr[dir + 2 * D * e]];
+          const int nb = a.nbr[dir + 2 * D * e];  // < 0: boundary, no interface
+          double an = nb >= 0 ? 0.5 * a.alpha_raw[nb] : 0.0;
           m = an > m ? an : m;
         }
         al = m;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -144,6 +145,9 @@ struct trixi_hip_ctx {
 
   // mesh / basis on the device
   int32_t *d_nbr = nullptr;
+  int B = 0;                     // boundaries (non-periodic mesh)
+  int32_t *d_bnd = nullptr;      // [B] {element, direction}
+  void *d_ghost = nullptr;       // [B, n^(D-1), V] outer boundary states, native precision
   double *d_invj = nullptr, *d_centers = nullptr;
   // resident state
   double *d_u = nullptr, *d_utmp = nullptr;
@@ -288,10 +292,24 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
     int r = launch_indicator_raw<T>(ctx, u);
     if (r) return r;
   }
+  if (ctx->B > 0) {  // outer states of the Dirichlet boundaries at this stage's time
+    bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+      constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+      NodesConst<N> nc{};
+      for (int i = 0; i <= N; ++i) nc.nodes[i] = ctx->nodes[i];
+      const int total = ctx->B * ipow_c(N + 1, D - 1);
+      hipLaunchKernelGGL((boundary_state_kernel<T, D, N>), dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
+                         ctx->d_bnd, ctx->B, ctx->d_centers, ctx->d_invj, nc, t, tdev, ctx->exact,
+                         (T *)ctx->d_ghost);
+    });
+    if (!ok) return TRIXI_HIP_EINVAL;
+    HIP_TRY(hipGetLastError());
+  }
   ElemArgs<T> a{};
   a.u = u;
   a.du = du;
   a.nbr = ctx->d_nbr;
+  a.ghost = (const T *)ctx->d_ghost;
   a.inv_jac = ctx->d_invj;
   a.centers = ctx->d_centers;
   a.alpha_raw = ctx->d_alpha_raw;
@@ -376,8 +394,13 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     return code;
   };
   if (ctx->D != 2 && ctx->D != 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 2 or 3");
-  if (ctx->E <= 0 || ctx->I != ctx->D * ctx->E)
-    return fail(TRIXI_HIP_EINVAL, "a conforming periodic TreeMesh has ninterfaces == ndims * nelements");
+  ctx->B = cfg->nboundaries;
+  if (ctx->E <= 0 || ctx->B < 0 || 2 * ctx->I + ctx->B != 2 * ctx->D * ctx->E)
+    return fail(TRIXI_HIP_EINVAL, "a conforming TreeMesh has 2 ninterfaces + nboundaries == 2 ndims nelements");
+  if (ctx->B > 0 && cfg->boundary_condition != TRIXI_HIP_BC_DIRICHLET)
+    return fail(TRIXI_HIP_EINVAL, "a mesh with boundaries needs boundary_condition = TRIXI_HIP_BC_DIRICHLET");
+  if (ctx->B > 0 && (!mesh->boundary_neighbor_ids || !mesh->boundary_orientations || !mesh->boundary_neighbor_sides))
+    return fail(TRIXI_HIP_EINVAL, "boundary arrays missing");
   bool dn_ok = dispatch_dn(ctx->D, ctx->N, [](auto, auto) {});
   if (!dn_ok) return fail(TRIXI_HIP_EINVAL, "unsupported (ndims, polydeg): supported are 2D/3D with polydeg 2..4");
   ctx->NN = ctx->D == 3 ? ctx->n * ctx->n * ctx->n : ctx->n * ctx->n;
@@ -390,15 +413,31 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   const int n = ctx->n, D = ctx->D, E = ctx->E, I = ctx->I;
   // element neighbour table from the interface container (containers_3d.jl:229-279):
   // interface i with orientation o joins left (direction 2o+1 of left) and right (2o of right)
-  std::vector<int32_t> nbr((size_t)2 * D * E, -1);
+  constexpr int32_t kUnset = INT32_MIN;
+  std::vector<int32_t> nbr((size_t)2 * D * E, kUnset);
   for (int i = 0; i < I; ++i) {
     int l = mesh->neighbor_ids[2 * i], r = mesh->neighbor_ids[2 * i + 1], o = mesh->orientations[i];
     if (l < 0 || l >= E || r < 0 || r >= E || o < 0 || o >= D) return fail(TRIXI_HIP_EINVAL, "bad interface data");
     nbr[(2 * o + 1) + 2 * D * l] = r;
     nbr[(2 * o) + 2 * D * r] = l;
   }
+  // boundaries (init_boundaries!, containers_2d.jl:475-543): boundary b at the
+  // face of its element in direction 2o+1 (neighbor_side 1) or 2o (side 2),
+  // stored as -(b + 1) in the neighbour table
+  std::vector<int32_t> bnd((size_t)2 * ctx->B);
+  for (int b = 0; b < ctx->B; ++b) {
+    const int e = mesh->boundary_neighbor_ids[b], o = mesh->boundary_orientations[b];
+    const int side = mesh->boundary_neighbor_sides[b];
+    if (e < 0 || e >= E || o < 0 || o >= D || (side != 1 && side != 2))
+      return fail(TRIXI_HIP_EINVAL, "bad boundary data");
+    const int dir = side == 1 ? 2 * o + 1 : 2 * o;
+    if (nbr[dir + 2 * D * e] != kUnset) return fail(TRIXI_HIP_EINVAL, "a face is both an interface and a boundary");
+    nbr[dir + 2 * D * e] = -(b + 1);
+    bnd[2 * b] = e;
+    bnd[2 * b + 1] = dir;
+  }
   for (size_t k = 0; k < nbr.size(); ++k)
-    if (nbr[k] < 0) return fail(TRIXI_HIP_EINVAL, "element without a conforming neighbour (mortars/boundaries unsupported)");
+    if (nbr[k] == kUnset) return fail(TRIXI_HIP_EINVAL, "element face without interface or boundary (mortars unsupported)");
   // basis
   ctx->nodes.assign(basis->nodes, basis->nodes + n);
   ctx->weights.assign(basis->weights, basis->weights + n);
@@ -442,6 +481,9 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
             hipMalloc(&ctx->d_alpha_raw, E * 8) == hipSuccess && hipMalloc(&ctx->d_alpha, E * 8) == hipSuccess &&
             hipMalloc(&ctx->d_partial, ctx->partial_cap * 8) == hipSuccess &&
             hipMalloc(&ctx->d_scalars, 64 * 8) == hipSuccess;
+  if (ok && ctx->B > 0)
+    ok = hipMalloc(&ctx->d_bnd, bnd.size() * 4) == hipSuccess &&
+         hipMalloc(&ctx->d_ghost, (size_t)ctx->B * ipow_c(n, D - 1) * ctx->V * tsz) == hipSuccess;
   if (ok && ctx->fp32)
     ok = hipMalloc(&ctx->d_u32, nv * 4) == hipSuccess && hipMalloc(&ctx->d_su32, nv * 4) == hipSuccess &&
          hipMalloc(&ctx->d_sdu32, nv * 4) == hipSuccess;
@@ -456,7 +498,8 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
        hipMemcpy(ctx->d_centers, mesh->element_centers, (size_t)D * E * 8, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(ctx->d_alpha_raw, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(ctx->d_scalars + 32, ctx->weights.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess;
+       hipMemcpy(ctx->d_scalars + 32, ctx->weights.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       (ctx->B == 0 || hipMemcpy(ctx->d_bnd, bnd.data(), bnd.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
   if (!ok) {
     trixi_hip_destroy(ctx);
     fprintf(stderr, "trixi_hip_create: upload failed\n");
@@ -474,7 +517,7 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
   void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_u, ctx->d_utmp, ctx->d_u32, ctx->d_du,
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
                   ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
-                  ctx->d_alpha_hg, ctx->d_records};
+                  ctx->d_alpha_hg, ctx->d_records, ctx->d_bnd, ctx->d_ghost};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -1024,6 +1067,14 @@ int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float
 int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *params, int nparams,
                                  const double *modes) {
   if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_FOURIER_MODES) return TRIXI_HIP_EINVAL;
+  if (ctx->graph_exec) {  // a captured step holds the boundary-state kernel with the old function
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->graph_exec = nullptr;
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph = nullptr;
+    ctx->graph_key.clear();
+  }
   ctx->exact.kind = ic_kind;
   ctx->exact.gamma = ctx->cfg.gamma;
   for (int i = 0; i < 4; ++i) ctx->exact.params[i] = (params && i < nparams) ? params[i] : 0.0;

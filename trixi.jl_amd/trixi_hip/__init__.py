@@ -1,7 +1,7 @@
 """trixi_hip -- host-side mirror of Trixi.jl's interface for the MI355X DGSEM
-right-hand side of the compressible Euler equations on a periodic TreeMesh.
+right-hand side of the compressible Euler equations on a TreeMesh.
 
-    mesh   = TreeMesh((0, 0, 0), (2, 2, 2), initial_refinement_level=5)
+    mesh   = TreeMesh((0, 0, 0), (2, 2, 2), initial_refinement_level=5, periodicity=True)
     solver = DGSEM(polydeg=3, surface_flux=flux_lax_friedrichs,
                    volume_integral=VolumeIntegralFluxDifferencing(flux_ranocha))
     semi   = SemidiscretizationHyperbolic(mesh, CompressibleEulerEquations3D(1.4),
@@ -16,7 +16,8 @@ include/trixi_hip.h.
 """
 from . import _lib
 from .basis import LobattoLegendreBasis
-from .equations import (CompressibleEulerEquations2D, CompressibleEulerEquations3D, FluxLaxFriedrichs,
+from .equations import (BoundaryConditionDirichlet, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
+                        FluxLaxFriedrichs, boundary_condition_periodic,
                         flux_central, flux_lax_friedrichs, flux_ranocha, fourier_modes,
                         initial_condition_constant, initial_condition_convergence_test,
                         initial_condition_fourier_modes, initial_condition_isentropic_vortex,

@@ -29,6 +29,9 @@ FLUX_LLF_NAIVE = 1
 FLUX_RANOCHA = 2
 FLUX_CENTRAL = 3
 
+BC_NONE = 0
+BC_DIRICHLET = 1
+
 SRC_NONE = 0
 SRC_CONVERGENCE_TEST = 1
 
@@ -69,6 +72,8 @@ class Config(ctypes.Structure):
         ("source_terms", ctypes.c_int32),
         ("elements_per_block", ctypes.c_int32),
         ("threads_per_block", ctypes.c_int32),
+        ("nboundaries", ctypes.c_int32),
+        ("boundary_condition", ctypes.c_int32),
     ]
 
 
@@ -78,6 +83,9 @@ class Mesh(ctypes.Structure):
         ("orientations", ctypes.POINTER(ctypes.c_int32)),
         ("inverse_jacobian", ctypes.POINTER(ctypes.c_double)),
         ("element_centers", ctypes.POINTER(ctypes.c_double)),
+        ("boundary_neighbor_ids", ctypes.POINTER(ctypes.c_int32)),
+        ("boundary_orientations", ctypes.POINTER(ctypes.c_int32)),
+        ("boundary_neighbor_sides", ctypes.POINTER(ctypes.c_int32)),
     ]
 
 

@@ -105,12 +105,18 @@ flux_central = _Flux("flux_central", _lib.FLUX_CENTRAL)
 # Initial conditions (callables f(x, t, equations) -> u, vectorised over x[..., D])
 # ---------------------------------------------------------------------------
 class InitialCondition:
-    def __init__(self, name, func, code, params=(), modes=None):
+    """A pointwise initial condition. `code`/`params`/`modes` name the same
+    function for the device (error norms, Dirichlet boundary values);
+    device_exact is False when the device evaluates only part of it (the seeded
+    noise of the KHI setup)."""
+
+    def __init__(self, name, func, code, params=(), modes=None, device_exact=True):
         self.name = name
         self.func = func
         self.code = code
         self.params = tuple(params)
         self.modes = modes
+        self.device_exact = device_exact
 
     def __call__(self, x, t, equations):
         return self.func(np.asarray(x, dtype=np.float64), float(t), equations)
@@ -244,7 +250,8 @@ def initial_condition_isentropic_vortex(p_background=25.0):
 
 def initial_condition_kelvin_helmholtz_instability(noise_amplitude=0.0, seed=0):
     return InitialCondition("initial_condition_kelvin_helmholtz_instability",
-                            _make_khi(float(noise_amplitude), int(seed)), _lib.IC_KELVIN_HELMHOLTZ)
+                            _make_khi(float(noise_amplitude), int(seed)), _lib.IC_KELVIN_HELMHOLTZ,
+                            device_exact=not noise_amplitude)
 
 
 def fourier_modes(seed, nmodes=8, kmax=4):
@@ -272,3 +279,33 @@ class SourceTerms:
 
 
 source_terms_convergence_test = SourceTerms("source_terms_convergence_test", _lib.SRC_CONVERGENCE_TEST)
+
+
+# ---------------------------------------------------------------------------
+# Boundary conditions (src/equations/equations.jl:120-182)
+# ---------------------------------------------------------------------------
+class _PeriodicBC:
+    def __repr__(self):
+        return "boundary_condition_periodic"
+
+
+boundary_condition_periodic = _PeriodicBC()
+
+
+class BoundaryConditionDirichlet:
+    """`BoundaryConditionDirichlet(boundary_value_function)` (equations.jl:159-182):
+    the outer state at a boundary node x is boundary_value_function(x, t). The
+    device evaluates it as the semidiscretization's initial condition, so the
+    function must be that initial condition."""
+
+    code = _lib.BC_DIRICHLET
+
+    def __init__(self, boundary_value_function):
+        if not isinstance(boundary_value_function, InitialCondition):
+            raise TypeError("the boundary value function must be an initial condition known to the device")
+        if not boundary_value_function.device_exact:
+            raise ValueError(f"{boundary_value_function} is not evaluated exactly on the device (seeded noise)")
+        self.boundary_value_function = boundary_value_function
+
+    def __repr__(self):
+        return f"BoundaryConditionDirichlet({self.boundary_value_function})"

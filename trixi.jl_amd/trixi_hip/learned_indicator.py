@@ -157,7 +157,7 @@ def khi_semidiscretization(level=5, noise_amplitude=1e-3, seed=0, precision="fp6
     from .semidiscretization import SemidiscretizationHyperbolic
     from .solver import DGSEM, IndicatorHennemannGassner, VolumeIntegralShockCapturingHG
     eq = CompressibleEulerEquations2D(1.4)
-    mesh = TreeMesh((-1.0, -1.0), (1.0, 1.0), initial_refinement_level=level)
+    mesh = TreeMesh((-1.0, -1.0), (1.0, 1.0), initial_refinement_level=level, periodicity=True)
     surface_flux = FluxLaxFriedrichs(max_abs_speed_naive)
     basis = LobattoLegendreBasis(3)
     ind = IndicatorHennemannGassner(eq, basis, alpha_max=0.002, alpha_min=0.0001, alpha_smooth=True)

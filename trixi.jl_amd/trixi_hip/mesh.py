@@ -1,4 +1,4 @@
-"""Uniform periodic TreeMesh (host-side setup, no device work).
+"""Uniform TreeMesh, periodic or bounded (host-side setup, no device work).
 
 Restates the parts of Trixi.jl that fix element order and geometry:
   TreeMesh(coordinates_min, coordinates_max; initial_refinement_level, periodicity)
@@ -6,27 +6,33 @@ Restates the parts of Trixi.jl that fix element order and geometry:
   refine_uniformly! / init_children! (depth-first, children x fastest)
       src/meshes/abstract_tree.jl:206-250, child_sign table :118-128,
       child_coordinates :668-674; leaf order = ascending cell id :154-166
-  init_neighbors! with the periodic level-0 cell its own neighbour
+  init_neighbors! with the periodic level-0 cell its own neighbour, and no
+      neighbour across a non-periodic domain face
       abstract_tree.jl:252-313, serial_tree.jl:101-140
   init_elements! (inverse_jacobian, node_coordinates)
       src/solvers/dgsem_tree/containers_3d.jl:87-128 (2D: containers_2d.jl)
   init_interfaces! (per element, positive directions, left = element)
-      containers_3d.jl:229-279
+      containers_3d.jl:229-279, containers_2d.jl:228-290
+  init_boundaries! (per direction, then element order; neighbor_sides 1 when
+      the element is left of the boundary) containers_2d.jl:475-543
 """
 import numpy as np
 
 
 class TreeMesh:
     def __init__(self, coordinates_min, coordinates_max, initial_refinement_level,
-                 periodicity=True):
+                 periodicity=False):
         cmin = tuple(float(c) for c in coordinates_min)
         cmax = tuple(float(c) for c in coordinates_max)
         if len(cmin) not in (2, 3) or len(cmin) != len(cmax):
             raise ValueError("TreeMesh supports 2D and 3D here")
         if not (isinstance(initial_refinement_level, int) and initial_refinement_level >= 0):
             raise ValueError("`initial_refinement_level` must be a non-negative integer")
-        if periodicity is not True and not (isinstance(periodicity, (tuple, list)) and all(periodicity)):
-            raise ValueError("only fully periodic TreeMesh is supported by the HIP RHS")
+        if isinstance(periodicity, bool):
+            periodicity = (periodicity,) * len(cmin)
+        if not (isinstance(periodicity, (tuple, list)) and len(periodicity) == len(cmin)):
+            raise ValueError("`periodicity` must be a Bool or a tuple of ndims Bools")
+        self.periodicity = tuple(bool(p) for p in periodicity)
         self.ndims = len(cmin)
         self.coordinates_min = cmin
         self.coordinates_max = cmax
@@ -70,35 +76,50 @@ class TreeMesh:
             s *= nside
         lookup = np.empty(nside ** D, dtype=np.int64)
         lookup[lin] = np.arange(self.nelements)
-        # element neighbour per direction (0-based: 2o = -o, 2o+1 = +o)
+        # element neighbour per direction (0-based: 2o = -o, 2o+1 = +o), -1
+        # across a non-periodic domain face
         nbr = np.empty((self.nelements, 2 * D), dtype=np.int32)
         for dirn in range(2 * D):
             o = dirn // 2
             shifted = ic.copy()
-            shifted[:, o] = (shifted[:, o] + (1 if dirn % 2 else -1)) % nside
+            shifted[:, o] += 1 if dirn % 2 else -1
+            outside = (shifted[:, o] < 0) | (shifted[:, o] >= nside)
+            shifted[:, o] %= nside
             l2 = np.zeros(self.nelements, dtype=np.int64)
             s = 1
             for d in range(D):
                 l2 += shifted[:, d] * s
                 s *= nside
             nbr[:, dirn] = lookup[l2]
+            if not self.periodicity[o]:
+                nbr[outside, dirn] = -1
         self.element_neighbors = nbr
         # interfaces: element order, positive directions (left = element)
         E = self.nelements
-        self.neighbor_ids = np.empty((E * D, 2), dtype=np.int32)
-        self.orientations = np.empty(E * D, dtype=np.int32)
-        for o in range(D):
-            self.neighbor_ids[o::D, 0] = np.arange(E)
-            self.neighbor_ids[o::D, 1] = nbr[:, 2 * o + 1]
-            self.orientations[o::D] = o
-        self.ninterfaces = E * D
+        has = nbr[:, 1::2] >= 0  # [E, D]: interface at the + face
+        el, o = np.nonzero(has)  # row-major: element order, then orientation
+        self.neighbor_ids = np.stack([el, nbr[el, 2 * o + 1]], axis=1).astype(np.int32)
+        self.orientations = o.astype(np.int32)
+        self.ninterfaces = len(o)
+        # boundaries: direction-major, then element order
+        b_el, b_dir = [], []
+        for dirn in range(2 * D):
+            ids = np.nonzero(nbr[:, dirn] < 0)[0]
+            b_el.append(ids)
+            b_dir.append(np.full(len(ids), dirn))
+        b_el = np.concatenate(b_el).astype(np.int32)
+        b_dir = np.concatenate(b_dir).astype(np.int32)
+        self.boundary_neighbor_ids = b_el
+        self.boundary_orientations = (b_dir // 2).astype(np.int32)
+        self.boundary_neighbor_sides = np.where(b_dir % 2 == 1, 1, 2).astype(np.int32)
+        self.nboundaries = len(b_el)
 
     def ncells(self):
         return self.nelements
 
     def __repr__(self):
         return (f"TreeMesh{{{self.ndims}}} with {self.nelements} elements on "
-                f"{self.coordinates_min}..{self.coordinates_max}")
+                f"{self.coordinates_min}..{self.coordinates_max}, periodicity={self.periodicity}")
 
 
 def element_geometry(mesh, basis):

@@ -10,6 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from .equations import BoundaryConditionDirichlet, boundary_condition_periodic
 from .mesh import element_geometry
 from .solver import (VolumeIntegralFluxDifferencing, VolumeIntegralPureLGLFiniteVolume,
                      VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm)
@@ -25,13 +26,24 @@ class SemidiscretizationHyperbolic:
 
     precision: "fp64" (plain FP64) or "fp32" (FP32 fluxes, FP64 RK registers
     and reductions). elements_per_block / threads_per_block override the
-    element-kernel tile (0 = default)."""
+    element-kernel tile (0 = default). boundary_conditions: periodic (for a
+    periodic mesh) or one BoundaryConditionDirichlet(initial_condition) for
+    every boundary of a non-periodic mesh."""
 
     def __init__(self, mesh, equations, initial_condition, solver, source_terms=None,
-                 boundary_conditions="periodic", precision="fp64", elements_per_block=0,
+                 boundary_conditions=boundary_condition_periodic, precision="fp64", elements_per_block=0,
                  threads_per_block=0):
-        if boundary_conditions not in ("periodic", None):
-            raise ValueError("only boundary_condition_periodic is supported")
+        periodic_bc = boundary_conditions in ("periodic", None) or boundary_conditions is boundary_condition_periodic
+        if mesh.nboundaries == 0 and not periodic_bc:
+            raise ValueError("a periodic mesh takes boundary_condition_periodic")
+        if mesh.nboundaries > 0:
+            if not isinstance(boundary_conditions, BoundaryConditionDirichlet):
+                raise ValueError("a non-periodic mesh needs BoundaryConditionDirichlet(initial_condition)")
+            f = boundary_conditions.boundary_value_function
+            if (f.code, tuple(f.params)) != (initial_condition.code, tuple(initial_condition.params)):
+                raise ValueError("the Dirichlet boundary value function must be the initial condition "
+                                 "(the device evaluates one function for both)")
+        self.boundary_conditions = boundary_conditions
         if mesh.ndims != equations.ndims:
             raise ValueError("mesh and equations dimension differ")
         if precision not in ("fp64", "fp32"):
@@ -93,6 +105,8 @@ class SemidiscretizationHyperbolic:
         cfg.surface_flux = self.solver.surface_flux.code
         cfg.source_terms = self.source_terms.code if self.source_terms is not None else _lib.SRC_NONE
         cfg.elements_per_block, cfg.threads_per_block = self.tile
+        cfg.nboundaries = self.mesh.nboundaries
+        cfg.boundary_condition = _lib.BC_DIRICHLET if self.mesh.nboundaries else _lib.BC_NONE
         return cfg
 
     @property
@@ -106,13 +120,17 @@ class SemidiscretizationHyperbolic:
                 ori=np.ascontiguousarray(self.mesh.orientations, dtype=np.int32),
                 invj=np.ascontiguousarray(self.inverse_jacobian, dtype=np.float64),
                 cen=np.ascontiguousarray(self.mesh.cell_centers, dtype=np.float64),
+                bel=np.ascontiguousarray(self.mesh.boundary_neighbor_ids, dtype=np.int32),
+                bor=np.ascontiguousarray(self.mesh.boundary_orientations, dtype=np.int32),
+                bsd=np.ascontiguousarray(self.mesh.boundary_neighbor_sides, dtype=np.int32),
                 nodes=np.ascontiguousarray(b.nodes), weights=np.ascontiguousarray(b.weights),
                 ds=np.ascontiguousarray(b.derivative_split), dh=np.ascontiguousarray(b.derivative_hat),
                 vinv=np.ascontiguousarray(b.inverse_vandermonde_legendre),
                 aw=np.ascontiguousarray(b.analysis_weights), av=np.ascontiguousarray(b.analysis_vandermonde))
-            mesh = _lib.Mesh(arrays["nbr"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                             arrays["ori"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                             _ptr(arrays["invj"]), _ptr(arrays["cen"]))
+            i32 = ctypes.POINTER(ctypes.c_int32)
+            mesh = _lib.Mesh(arrays["nbr"].ctypes.data_as(i32), arrays["ori"].ctypes.data_as(i32),
+                             _ptr(arrays["invj"]), _ptr(arrays["cen"]), arrays["bel"].ctypes.data_as(i32),
+                             arrays["bor"].ctypes.data_as(i32), arrays["bsd"].ctypes.data_as(i32))
             basis = _lib.Basis(_ptr(arrays["nodes"]), _ptr(arrays["weights"]), _ptr(arrays["ds"]),
                                _ptr(arrays["dh"]), _ptr(arrays["vinv"]), len(b.analysis_nodes),
                                _ptr(arrays["aw"]), _ptr(arrays["av"]))
