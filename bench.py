@@ -50,16 +50,19 @@ def parse():
                    help="time one precision only (profiling runs)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run one plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE)")
+    p.add_argument("--no-bounded", action="store_true",
+                   help="skip the same RHS on the non-periodic mesh with Dirichlet boundaries")
     return p.parse_args()
 
 
-def make_semi(th, level, polydeg, precision, tile=(0, 0)):
-    mesh = th.TreeMesh((0.0, 0.0, 0.0), (2.0, 2.0, 2.0), initial_refinement_level=level, periodicity=True)
+def make_semi(th, level, polydeg, precision, tile=(0, 0), periodic=True):
+    mesh = th.TreeMesh((0.0, 0.0, 0.0), (2.0, 2.0, 2.0), initial_refinement_level=level, periodicity=periodic)
     solver = th.DGSEM(polydeg=polydeg, surface_flux=th.flux_lax_friedrichs,
                       volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_ranocha))
-    return th.SemidiscretizationHyperbolic(mesh, th.CompressibleEulerEquations3D(1.4),
-                                           th.initial_condition_convergence_test, solver,
-                                           source_terms=th.source_terms_convergence_test,
+    ic = th.initial_condition_convergence_test
+    bc = th.boundary_condition_periodic if periodic else th.BoundaryConditionDirichlet(ic)
+    return th.SemidiscretizationHyperbolic(mesh, th.CompressibleEulerEquations3D(1.4), ic, solver,
+                                           source_terms=th.source_terms_convergence_test, boundary_conditions=bc,
                                            precision=precision, elements_per_block=tile[0],
                                            threads_per_block=tile[1])
 
@@ -245,6 +248,18 @@ def main():
               file=sys.stderr, flush=True)
         del src, dst
 
+    bounded = {}
+    if not args.no_bounded and not args.only:
+        # the same RHS on [0,2]^3 with periodicity=false and
+        # BoundaryConditionDirichlet(initial_condition) on all six sides
+        for prec in ("fp32", "fp64"):
+            sb = make_semi(th, args.level, args.polydeg, prec, periodic=False)
+            rb = time_rhs(th, torch, sb, args.steps, args.warmup, u0)
+            bounded[prec] = dict(value=sb.ndofs() / (rb["ms_per_step"] * 1e-3), ms_per_step=rb["ms_per_step"],
+                                 nboundaries=sb.mesh.nboundaries, finite=rb["finite"])
+            sb.close()
+        print(f"[bench] bounded mesh: {bounded}", file=sys.stderr, flush=True)
+
     ndofs = semis["fp32"].ndofs()
     nv = semis["fp32"].nvariables()
     l2 = None
@@ -305,12 +320,16 @@ def main():
                              "kernel_avg_ms": res["fp64"]["kernel_avg_ms"],
                              "tile": list(res["fp64"]["tile"])}
         out["fp32_over_fp64"] = value / v64
+    if bounded:
+        out["bounded_mesh"] = dict(workload="same RHS, periodicity=false, BoundaryConditionDirichlet("
+                                            "initial_condition) on all sides (boundary-state kernel included)",
+                                   **bounded)
     if l2 is not None:
         out["equal_l2"] = l2
     if cpu is not None:
         out["gpu_fp32_over_cpu"] = value / cpu["value"]
         out["cpu_time_per_dof_per_rhs_s"] = cpu["time_per_dof_per_rhs_s"]
-    ok = all(v["finite"] for v in res.values())
+    ok = all(v["finite"] for v in res.values()) and all(v["finite"] for v in bounded.values())
     out["outputs_finite"] = ok
     for s in semis.values():
         s.close()

@@ -202,16 +202,21 @@ __device__ __forceinline__ double nanmax(double a, double b) {
 // max_dt (stepsize_dg3d.jl:8-32, stepsize_dg2d.jl:8-30) in two passes: per
 // block, max over its elements of inv_jac * sum_d max_nodes lambda_d with
 // lambda_d = |v_d| + c (max_abs_speeds, compressible_euler_3d.jl:1770-1775);
-// one thread per node (EB = 256 / NN elements per block, coalesced loads), the
-// node maxima reduced through LDS. NaN speeds propagate (nanmax).
+// one thread per node (EB = 256 / NN elements per block, coalesced loads). The
+// node maxima are reduced through LDS in parallel: S threads per (element,
+// direction) each take every S-th node, then one thread per pair combines
+// the S partials. Maxima are exact, so the result does not depend on the
+// order; the direction sum runs in d order. NaN speeds propagate (nanmax).
 template <typename T, int D, int N>
 __global__ __launch_bounds__(256) void max_speed_nodes_kernel(const T *__restrict__ u,
                                                               const double *__restrict__ inv_jac, int E,
                                                               double gamma, double *__restrict__ partial) {
   constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
   constexpr int EB = NN >= 256 ? 1 : 256 / NN;
+  constexpr int S = 8, NPAIR = EB * D;
   static_assert(NN <= 256, "one thread per node");
   __shared__ double s_l[D][EB * NN];
+  __shared__ double s_p[NPAIR * S];
   __shared__ double s_e[EB];
   const int tid = threadIdx.x;
   const int e0 = blockIdx.x * EB;
@@ -231,12 +236,22 @@ __global__ __launch_bounds__(256) void max_speed_nodes_kernel(const T *__restric
     for (int d = 0; d < D; ++d) s_l[d][tid] = fabs(v[d]) + c;
   }
   __syncthreads();
+  for (int k = tid; k < NPAIR * S; k += 256) {
+    const int pair = k / S, sub = k - pair * S;
+    const int el = pair / D, d = pair - el * D;
+    double m = 0.0;
+    if (el < ne)
+      for (int q = sub; q < NN; q += S) m = nanmax(m, s_l[d][el * NN + q]);
+    s_p[k] = m;
+  }
+  __syncthreads();
   if (tid < ne) {
     double s = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       double m = 0.0;
-      for (int q = 0; q < NN; ++q) m = nanmax(m, s_l[d][tid * NN + q]);
+#pragma unroll
+      for (int j = 0; j < S; ++j) m = nanmax(m, s_p[(tid * D + d) * S + j]);
       s += m;
     }
     s_e[tid] = inv_jac[e0 + tid] * s;

@@ -170,16 +170,17 @@ __device__ __forceinline__ void line_ids(const ElemArgs<T> &a, int e0, int ne, i
   nbp = a.nbr[(2 * o + 1) + 2 * D * e];
 }
 
-// Neighbour face states of the line in direction o (consumed in phase B). At a
-// boundary face the outer state comes from the ghost record of that boundary
-// (face node = the lane's line index).
-template <typename T, int D, int N, int EPB>
+// Neighbour face states of the line in direction o (consumed in phase B). With
+// BND (a mesh with boundaries), the outer state at a boundary face comes from
+// the ghost record of that boundary (face node = the lane's line index);
+// periodic meshes use the instantiation without that select.
+template <typename T, int D, int N, int EPB, bool BND>
 __device__ __forceinline__ void line_loads(const ElemArgs<T> &a, LineCtx<T, D + 2> &c, int o, int nbm, int nbp) {
   using LY = Layout<D, N, EPB>;
   constexpr int n = N + 1, NN = LY::NN, V = D + 2, L = LY::L;
   const int st = o == 0 ? 1 : (o == 1 ? n : n * n);
   const T *un_m, *un_p;
-  if (a.ghost) {  // kernel-argument uniform: periodic meshes never take this path
+  if constexpr (BND) {
     un_m = nbm >= 0 ? a.u + ((size_t)nbm * NN + c.base + (n - 1) * st) * V
                     : a.ghost + ((size_t)(-nbm - 1) * L + c.line) * V;
     un_p = nbp >= 0 ? a.u + ((size_t)nbp * NN + c.base) * V : a.ghost + ((size_t)(-nbp - 1) * L + c.line) * V;
@@ -510,7 +511,7 @@ __device__ __forceinline__ void source_table_entry(const DGConst<T, N> &k, int i
 // issued before the first one is consumed: neighbour ids, the tile (in chunks,
 // into registers), the per-element scalars, then the neighbour face states,
 // which stay in flight until phase B.
-template <typename T, int D, int N, int VOL, int EPB, int WPD>
+template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND>
 __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
   using LY = Layout<D, N, EPB>;
   using VT = VolTraits<VOL>;
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
   double cen[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) cen[d] = a.centers[d + D * (e0 + sel)];
-  line_loads<T, D, N, EPB>(a, lc, grp, nbm, nbp);
+  line_loads<T, D, N, EPB, BND>(a, lc, grp, nbm, nbp);
 
   // ---- phase A: stage the tile and the per-element data into LDS ----
   {

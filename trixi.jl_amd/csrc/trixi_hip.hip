@@ -16,87 +16,23 @@
 
 #include "../../include/trixi_hip.h"
 #include "dg_aux.hpp"
-#include "dg_element.hpp"
+#include "elem_registry.hpp"
 #include "mlp.hpp"
 
 using namespace trixi_hip;
 
+namespace trixi_hip {
+std::map<ElemKey, ElemLaunch> &elem_registry() {
+  static std::map<ElemKey, ElemLaunch> r;
+  return r;
+}
+}  // namespace trixi_hip
+
 namespace {
 
 // ---------------------------------------------------------------------------
-// Element-kernel registry: one entry per instantiated (T, D, N, VOL, EPB, WPD).
-// The workgroup has D * WPD wavefronts (one group of WPD waves per direction).
+// Element kernels: instantiated in elem_*.hip and looked up here by key.
 // ---------------------------------------------------------------------------
-struct ElemKey {
-  int fp32, D, N, vol, epb, nt;
-  bool operator<(const ElemKey &o) const {
-    return std::tie(fp32, D, N, vol, epb, nt) < std::tie(o.fp32, o.D, o.N, o.vol, o.epb, o.nt);
-  }
-};
-// Type-erased launcher: args and constants are passed as raw blobs that the
-// instantiation reinterprets as ElemArgs<T> / DGConst<T, N>.
-using ElemLaunch = hipError_t (*)(const void *args, const void *kconst, int nblocks, hipStream_t s);
-struct ElemEntry {
-  ElemLaunch launch;
-};
-
-std::map<ElemKey, ElemEntry> &elem_registry() {
-  static std::map<ElemKey, ElemEntry> r;
-  return r;
-}
-
-template <typename T, int D, int N, int VOL, int EPB, int WPD>
-hipError_t launch_elem(const void *args, const void *kconst, int nblocks, hipStream_t s) {
-  const ElemArgs<T> &a = *static_cast<const ElemArgs<T> *>(args);
-  const DGConst<T, N> &k = *static_cast<const DGConst<T, N> *>(kconst);
-  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, WPD>), dim3(nblocks), dim3(D * WPD * 64), 0, s, a, k);
-  return hipGetLastError();
-}
-
-template <typename T, int D, int N, int VOL, int EPB, int WPD> struct Reg {
-  Reg() {
-    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, D * WPD * 64}] =
-        ElemEntry{&launch_elem<T, D, N, VOL, EPB, WPD>};
-  }
-};
-
-// Tiles: EPB = elements per workgroup, WPD = wavefronts per direction (lines
-// per direction EPB * n^(D-1) <= 64 * WPD). The defaults (kDefaultTiles) put
-// one wave per direction; the alternatives below are selectable through
-// elements_per_block / threads_per_block (on MI355X they measured slower:
-// 3D p3 8x384 47.3 us vs 41.9 us, 3D p4 5x384 155 us vs 92 us, FP32).
-#define TRIXI_REG(T, D, N, VOL, EPB, WPD) \
-  static Reg<T, D, N, VOL, EPB, WPD> reg_##T##_##D##_##N##_##VOL##_##EPB##_##WPD;
-
-#define TRIXI_REG_ALLVOL(T, D, N, EPB, WPD)    \
-  TRIXI_REG(T, D, N, VOL_WEAK, EPB, WPD)       \
-  TRIXI_REG(T, D, N, VOL_FD_RANOCHA, EPB, WPD) \
-  TRIXI_REG(T, D, N, VOL_FD_CENTRAL, EPB, WPD) \
-  TRIXI_REG(T, D, N, VOL_HG_RANOCHA, EPB, WPD) \
-  TRIXI_REG(T, D, N, VOL_HG_CENTRAL, EPB, WPD) \
-  TRIXI_REG(T, D, N, VOL_PURE_FV, EPB, WPD)
-
-#define TRIXI_REG_BOTH(D, N, EPB, WPD)    \
-  TRIXI_REG_ALLVOL(float, D, N, EPB, WPD) \
-  TRIXI_REG_ALLVOL(double, D, N, EPB, WPD)
-
-// defaults
-TRIXI_REG_BOTH(2, 2, 21, 1)
-TRIXI_REG_BOTH(2, 3, 16, 1)
-TRIXI_REG_BOTH(2, 4, 12, 1)
-TRIXI_REG_BOTH(3, 2, 7, 1)
-TRIXI_REG_BOTH(3, 3, 4, 1)
-TRIXI_REG_BOTH(3, 4, 2, 1)
-
-// alternative tiles for the headline volume integrals
-TRIXI_REG(float, 3, 3, VOL_FD_RANOCHA, 8, 2)
-TRIXI_REG(double, 3, 3, VOL_FD_RANOCHA, 8, 2)
-TRIXI_REG(float, 3, 4, VOL_FD_RANOCHA, 5, 2)
-TRIXI_REG(double, 3, 4, VOL_FD_RANOCHA, 5, 2)
-TRIXI_REG(float, 2, 3, VOL_HG_RANOCHA, 32, 2)
-TRIXI_REG(double, 2, 3, VOL_HG_RANOCHA, 32, 2)
-TRIXI_REG(float, 2, 3, VOL_FD_RANOCHA, 32, 2)
-
 struct DefaultTile {
   int D, N, epb, nt;
 };
@@ -235,12 +171,12 @@ int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
         nt = t.nt;
       }
   }
-  auto it = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt});
+  auto it = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B > 0 ? 1 : 0});
   if (it == elem_registry().end())
     return set_err(ctx, TRIXI_HIP_EINVAL,
                    "no element kernel instantiated for this (precision, ndims, polydeg, volume "
-                   "integral, tile)");
-  ctx->elem = it->second.launch;
+                   "integral, tile, boundaries)");
+  ctx->elem = it->second;
   ctx->epb = epb;
   ctx->nt = nt;
   return TRIXI_HIP_OK;

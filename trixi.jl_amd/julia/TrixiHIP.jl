@@ -3,7 +3,7 @@
 # Makes the MI355X right-hand side a drop-in for Trixi.jl's
 #   rhs_hyperbolic!(du_ode, u_ode, semi, t)   (src/semidiscretization/semidiscretization_hyperbolic.jl:578-597)
 # for TreeMesh{2,3} + CompressibleEulerEquations{2,3}D + DGSEM with periodic
-# boundaries. The ODEProblem keeps `semi` as its parameter, so the 2N
+# boundaries or BoundaryConditionDirichlet(initial_condition). The ODEProblem keeps `semi` as its parameter, so the 2N
 # integrators and the Stepsize/Analysis callbacks run unchanged; only the RHS
 # function is replaced by a callable that ccalls the library.
 #
@@ -17,7 +17,9 @@ using Trixi: TreeMesh, DGSEM, CompressibleEulerEquations2D, CompressibleEulerEqu
              VolumeIntegralWeakForm, VolumeIntegralFluxDifferencing, VolumeIntegralShockCapturingHGType,
              VolumeIntegralPureLGLFiniteVolume, FluxLaxFriedrichs, flux_ranocha, flux_central,
              max_abs_speed, max_abs_speed_naive, source_terms_convergence_test, SolutionAnalyzer,
-             mesh_equations_solver_cache, nelements, ninterfaces
+             mesh_equations_solver_cache, nelements, ninterfaces, nboundaries,
+             BoundaryConditionDirichlet, initial_condition_constant, initial_condition_convergence_test,
+             initial_condition_weak_blast_wave
 using SciMLBase: ODEProblem
 
 const libtrixi_hip = get(ENV, "TRIXI_HIP_LIB", joinpath(@__DIR__, "..", "libtrixi_hip.so"))
@@ -40,6 +42,8 @@ struct HipConfig
     source_terms::Int32
     elements_per_block::Int32
     threads_per_block::Int32
+    nboundaries::Int32
+    boundary_condition::Int32
 end
 
 struct HipMesh
@@ -47,6 +51,9 @@ struct HipMesh
     orientations::Ptr{Int32}
     inverse_jacobian::Ptr{Float64}
     element_centers::Ptr{Float64}
+    boundary_neighbor_ids::Ptr{Int32}
+    boundary_orientations::Ptr{Int32}
+    boundary_neighbor_sides::Ptr{Int32}
 end
 
 struct HipBasis
@@ -79,6 +86,22 @@ vi_codes(v::VolumeIntegralPureLGLFiniteVolume) = (Int32(3), Int32(2), flux_code(
 source_code(::Nothing) = Int32(0)
 source_code(::typeof(source_terms_convergence_test)) = Int32(1)
 
+# initial conditions the device can evaluate (TRIXI_HIP_IC_*), for Dirichlet boundary values
+ic_code(::typeof(initial_condition_constant)) = Int32(0)
+ic_code(::typeof(initial_condition_convergence_test)) = Int32(1)
+ic_code(::typeof(initial_condition_weak_blast_wave)) = Int32(2)
+ic_code(f) = throw(ArgumentError("TrixiHIP cannot evaluate $f on the device"))
+
+# Every boundary must carry BoundaryConditionDirichlet(semi.initial_condition):
+# the device evaluates the initial condition as the outer state.
+function check_boundary_conditions(semi)
+    for bc in values(semi.boundary_conditions)
+        bc isa Trixi.BoundaryConditionPeriodic && continue
+        bc isa BoundaryConditionDirichlet && bc.boundary_value_function === semi.initial_condition ||
+            throw(ArgumentError("TrixiHIP supports BoundaryConditionDirichlet(initial_condition) only"))
+    end
+end
+
 check(rc, ctx) = rc == 0 || error("libtrixi_hip: ", unsafe_string(ccall((:trixi_hip_last_error, libtrixi_hip),
                                                                         Cstring, (Ptr{Cvoid},), ctx)))
 
@@ -101,17 +124,23 @@ function HipRHS(semi; precision::Symbol = :fp64)
     D = ndims(mesh)
     E = nelements(solver, cache)
     I = ninterfaces(solver, cache)
+    B = nboundaries(solver, cache)
+    B > 0 && check_boundary_conditions(semi)
     vi, vflux, fvflux, indicator = vi_codes(solver.volume_integral)
     amax, amin, asmooth = indicator === nothing ? (0.5, 0.001, true) :
                           (indicator.alpha_max, indicator.alpha_min, indicator.alpha_smooth)
     cfg = HipConfig(D, Trixi.polydeg(solver), E, I, equations.gamma, precision === :fp32 ? 1 : 0,
                     vi, vflux, flux_code(solver.surface_integral.surface_flux), fvflux, amax, amin,
-                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0)
+                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0, B, B > 0 ? 1 : 0)
     # interface container: 1-based ids and orientations -> 0-based
     nbr = Int32.(cache.interfaces.neighbor_ids .- 1)            # [2, I]
     ori = Int32.(cache.interfaces.orientations .- 1)            # [I]
     invj = Vector{Float64}(cache.elements.inverse_jacobian)     # [E]
     centers = Matrix{Float64}(mesh.tree.coordinates[:, cache.elements.cell_ids])  # [D, E]
+    # boundaries container (containers_2d.jl:475-543): 0-based ids/orientations, sides 1/2
+    bel = Int32.(cache.boundaries.neighbor_ids[1:B] .- 1)
+    bor = Int32.(cache.boundaries.orientations[1:B] .- 1)
+    bsd = Int32.(cache.boundaries.neighbor_sides[1:B])
     basis = solver.basis
     analyzer = SolutionAnalyzer(basis)
     nodes, weights = Vector{Float64}(basis.nodes), Vector{Float64}(basis.weights)
@@ -119,13 +148,20 @@ function HipRHS(semi; precision::Symbol = :fp64)
     vinv = rowmajor(basis.inverse_vandermonde_legendre)
     aw, av = Vector{Float64}(analyzer.weights), rowmajor(analyzer.vandermonde)
     out = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve nbr ori invj centers nodes weights ds dh vinv aw av begin
-        hm = Ref(HipMesh(pointer(nbr), pointer(ori), pointer(invj), pointer(centers)))
+    GC.@preserve nbr ori invj centers bel bor bsd nodes weights ds dh vinv aw av begin
+        hm = Ref(HipMesh(pointer(nbr), pointer(ori), pointer(invj), pointer(centers),
+                         pointer(bel), pointer(bor), pointer(bsd)))
         hb = Ref(HipBasis(pointer(nodes), pointer(weights), pointer(ds), pointer(dh), pointer(vinv),
                           length(aw), pointer(aw), pointer(av)))
         rc = ccall((:trixi_hip_create, libtrixi_hip), Cint,
                    (Ref{HipConfig}, Ref{HipMesh}, Ref{HipBasis}, Ref{Ptr{Cvoid}}), Ref(cfg), hm, hb, out)
         rc == 0 || error("trixi_hip_create failed with code $rc")
+    end
+    if B > 0  # the boundary value function
+        rc = ccall((:trixi_hip_set_exact_solution, libtrixi_hip), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}), out[], ic_code(semi.initial_condition),
+                   C_NULL, 0, C_NULL)
+        check(rc, out[])
     end
     f = HipRHS(out[])
     finalizer(f) do g

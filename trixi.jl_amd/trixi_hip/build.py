@@ -1,29 +1,57 @@
-"""In-tree build of libtrixi_hip.so for gfx950 (hipcc, no JIT cache)."""
+"""In-tree build of libtrixi_hip.so for gfx950 (hipcc, no JIT cache).
+
+Every csrc/*.hip translation unit is compiled to an object in parallel (the
+element-kernel instantiations are spread over elem_*.hip), then linked into
+one shared library."""
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(PKG_ROOT, "csrc")
 LIB = os.path.join(PKG_ROOT, "libtrixi_hip.so")
+OBJ_DIR = os.path.join(PKG_ROOT, "_build")
 
 HIPCC_FLAGS = [
-    "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
+    "-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
     # packed FP32 (v_pk_*_f32) stays off on this hardware
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
 
-def build_library(verbose=False, extra_flags=()):
+def sources():
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def build_library(verbose=False, extra_flags=(), jobs=None):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(OBJ_DIR, os.path.basename(s)[:-4] + ".o") for s in srcs]
+
+    def compile_one(so):
+        src, obj = so
+        cmd = [hipcc, *HIPCC_FLAGS, *extra_flags, "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        return subprocess.run(cmd, capture_output=True, text=True)
+
+    jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1), 8)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(compile_one, zip(srcs, objs)))
+    for src, r in zip(srcs, results):
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed for {os.path.basename(src)}")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, *extra_flags, "-o", tmp, os.path.join(CSRC, "trixi_hip.hip")]
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed for libtrixi_hip.so")
+        raise RuntimeError("linking libtrixi_hip.so failed")
     os.replace(tmp, LIB)
     return LIB
 
