@@ -203,3 +203,46 @@ def test_device_step_control_equals_host_loop(precision):
                        device_steps=dev)
         assert sol.nsteps == 1 and sol.t == 0.2 and np.array_equal(sol.u, ode.u0), dev
     semi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", ["khi_2d", "mms_2d_nonperiodic", "shockcapturing_3d", "mms_3d_weak"])
+def test_fused_stage_equals_rhs_plus_update(case, precision, monkeypatch):
+    """The element-kernel variant that applies the 2N update in its epilogue
+    (the state alternates between two buffers) runs bitwise the same steps as
+    the RHS kernel followed by rk2n_update_kernel: device-controlled solve with
+    an odd step count, per-step graph replays and eager steps, and single stages."""
+    import trixi_hip as th
+    c = golden("kta.json")[case]
+    spec = dict(c["oracle"], ic=c["ic"], ic_params=tuple(c["ic_params"]))
+    alg = th.CarpenterKennedy2N54()
+    na = 5
+    a = (ctypes.c_double * na)(*alg.a)
+    b = (ctypes.c_double * na)(*alg.b)
+    cc = (ctypes.c_double * na)(*alg.c)
+    out = {}
+    for unfused in ("1", "0"):
+        monkeypatch.setenv("TRIXI_HIP_UNFUSED_STAGE", unfused)
+        semi, _ = build(spec, precision)
+        lib, ctx = semi._lib, semi.ctx
+        ode = th.semidiscretize(semi, (0.0, 5.0))
+        sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=0.9)),
+                       maxiters=13)
+        res = [sol.nsteps, sol.t, sol.u]
+        u0 = semi.compute_coefficients(0.0)
+        for graph in (1, 0):
+            upload(semi, u0)
+            for rep in range(3):
+                assert lib.trixi_hip_rk_step(ctx, a, b, cc, na, 0.1 + rep * 1e-3, 1e-3, graph) == 0
+            res.append(download(semi))
+        upload(semi, u0)
+        assert lib.trixi_hip_rk_stage(ctx, 0.0, 2e-3, 0.05, 1) == 0
+        assert lib.trixi_hip_rk_stage(ctx, alg.a[1], alg.b[1] * 1e-3, 0.06, 0) == 0
+        res.append(download(semi))
+        out[unfused] = res
+        semi.close()
+    ref, fused = out["1"], out["0"]
+    assert ref[:2] == fused[:2] and ref[0] == 13
+    for k in range(2, len(ref)):
+        assert np.array_equal(ref[k], fused[k]), k

@@ -62,6 +62,16 @@ template <typename T> struct ElemArgs {
   int source;
   int alpha_smooth;
   int xcd_remap;
+  // fused 2N stage (STAGE variants, methods_2N.jl:153-157): instead of storing
+  // du, u_tmp = du - a_stage u_tmp and u_reg_out = u_reg_in + b_dt u_tmp, with
+  // b_dt = b_stage * tdev[1] (or b_stage without tdev); u32_out receives the
+  // FP32 shadow of u_reg_out (FP32 mode). u_reg_in == u_reg_out is allowed.
+  double *u_tmp;
+  const double *u_reg_in;
+  double *u_reg_out;
+  float *u32_out;
+  double a_stage, b_stage;
+  int first_stage;
   T gamma;
   T inv_gamma_m1;  // 1 / (gamma - 1), rounded once in T
   T sqrt_gamma;
@@ -511,7 +521,7 @@ __device__ __forceinline__ void source_table_entry(const DGConst<T, N> &k, int i
 // issued before the first one is consumed: neighbour ids, the tile (in chunks,
 // into registers), the per-element scalars, then the neighbour face states,
 // which stay in flight until phase B.
-template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND>
+template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND, bool STAGE>
 __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
   using LY = Layout<D, N, EPB>;
   using VT = VolTraits<VOL>;
@@ -646,6 +656,26 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
   __syncthreads();
 
   // ---- phase C: sum directions, Jacobian, sources; stage the output ----
+  // The fused stage's register loads (u_tmp, u) of a full tile are issued
+  // here, all at once, so that their latency overlaps the rest of phase C.
+  constexpr int W = TIO::CB / (int)sizeof(T);  // values per chunk
+  constexpr int NPRE = STAGE ? RPT * W : 1;
+  double pre_t[NPRE], pre_u[NPRE];
+  const size_t base = (size_t)e0 * NN * V;
+  if constexpr (STAGE) {
+    if (full) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int ic = min(tid + r * NT, TIO::NCH - 1);  // clamped: every lane loads in bounds
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const size_t g = base + (size_t)ic * W + w;
+          pre_t[r * W + w] = a.first_stage ? 0.0 : a.u_tmp[g];
+          pre_u[r * W + w] = a.u_reg_in[g];
+        }
+      }
+    }
+  }
   for (int q = tid; q < ne * NN; q += NT) {
     const int el = q / NN, node = q - el * NN;
     const int p = LY::phys(q);
@@ -684,7 +714,34 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
     for (int v = 0; v < V; ++v) s_o[q * V + v] = d[v];
   }
   __syncthreads();
-  if (full) {
+  if constexpr (STAGE) {
+    // the 2N update of the tile's values, in the same arithmetic as
+    // rk2n_update_kernel (bitwise equal to the unfused stage)
+    const double b_dt = a.tdev ? a.b_stage * a.tdev[1] : a.b_stage;
+    auto update = [&](size_t g, T du_v, double ut, double ur) {
+      ut = double(du_v) - ut * a.a_stage;
+      const double un = tfma(ut, b_dt, ur);
+      a.u_tmp[g] = ut;
+      a.u_reg_out[g] = un;
+      if (a.u32_out) a.u32_out[g] = float(un);
+    };
+    if (full) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int i = tid + r * NT;
+        if (i < TIO::NCH) {
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            update(base + (size_t)i * W + w, s_o[i * W + w], pre_t[r * W + w], pre_u[r * W + w]);
+        }
+      }
+    } else {
+      for (int i = tid; i < ne * NN * V; i += NT) {
+        const size_t g = base + i;
+        update(g, s_o[i], a.first_stage ? 0.0 : a.u_tmp[g], a.u_reg_in[g]);
+      }
+    }
+  } else if (full) {
     const C *src = reinterpret_cast<const C *>(s_o);
     C *dst = reinterpret_cast<C *>(a.du + (size_t)e0 * NN * V);
 #pragma unroll

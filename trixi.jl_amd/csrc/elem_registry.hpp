@@ -13,12 +13,14 @@
 namespace trixi_hip {
 
 // (precision, ndims, polydeg, volume integral, elements per block, threads per
-// block, boundaries) -- the workgroup has D * WPD wavefronts, one group of WPD
-// waves per direction; bnd selects the variant that reads boundary states.
+// block, boundaries, stage) -- the workgroup has D * WPD wavefronts, one group
+// of WPD waves per direction; bnd selects the variant that reads boundary
+// states, stage the variant that applies the 2N update instead of storing du.
 struct ElemKey {
-  int fp32, D, N, vol, epb, nt, bnd;
+  int fp32, D, N, vol, epb, nt, bnd, stage;
   bool operator<(const ElemKey &o) const {
-    return std::tie(fp32, D, N, vol, epb, nt, bnd) < std::tie(o.fp32, o.D, o.N, o.vol, o.epb, o.nt, o.bnd);
+    return std::tie(fp32, D, N, vol, epb, nt, bnd, stage) <
+           std::tie(o.fp32, o.D, o.N, o.vol, o.epb, o.nt, o.bnd, o.stage);
   }
 };
 
@@ -28,19 +30,19 @@ using ElemLaunch = hipError_t (*)(const void *args, const void *kconst, int nblo
 
 std::map<ElemKey, ElemLaunch> &elem_registry();  // defined in trixi_hip.hip
 
-template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND>
+template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND, bool STAGE>
 hipError_t launch_elem(const void *args, const void *kconst, int nblocks, hipStream_t s) {
   const ElemArgs<T> &a = *static_cast<const ElemArgs<T> *>(args);
   const DGConst<T, N> &k = *static_cast<const DGConst<T, N> *>(kconst);
-  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, WPD, BND>), dim3(nblocks), dim3(D * WPD * 64), 0, s, a,
-                     k);
+  hipLaunchKernelGGL((dg_element_kernel<T, D, N, VOL, EPB, WPD, BND, STAGE>), dim3(nblocks), dim3(D * WPD * 64), 0,
+                     s, a, k);
   return hipGetLastError();
 }
 
-template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND> struct Reg {
+template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND, bool STAGE> struct Reg {
   Reg() {
-    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, D * WPD * 64, BND ? 1 : 0}] =
-        &launch_elem<T, D, N, VOL, EPB, WPD, BND>;
+    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, D * WPD * 64, BND ? 1 : 0, STAGE ? 1 : 0}] =
+        &launch_elem<T, D, N, VOL, EPB, WPD, BND, STAGE>;
   }
 };
 
@@ -50,22 +52,22 @@ template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND> struct 
 // per direction EPB * n^(D-1) <= 64 * WPD).
 #define TRIXI_CAT2(a, b) a##b
 #define TRIXI_CAT(a, b) TRIXI_CAT2(a, b)
-#define TRIXI_REG(T, D, N, VOL, EPB, WPD, BND) \
-  static trixi_hip::Reg<T, D, N, VOL, EPB, WPD, BND> TRIXI_CAT(trixi_reg_, __COUNTER__);
+#define TRIXI_REG(T, D, N, VOL, EPB, WPD, BND, STAGE) \
+  static trixi_hip::Reg<T, D, N, VOL, EPB, WPD, BND, STAGE> TRIXI_CAT(trixi_reg_, __COUNTER__);
 
-#define TRIXI_REG_ALLVOL(T, D, N, EPB, WPD, BND)          \
-  TRIXI_REG(T, D, N, trixi_hip::VOL_WEAK, EPB, WPD, BND)       \
-  TRIXI_REG(T, D, N, trixi_hip::VOL_FD_RANOCHA, EPB, WPD, BND) \
-  TRIXI_REG(T, D, N, trixi_hip::VOL_FD_CENTRAL, EPB, WPD, BND) \
-  TRIXI_REG(T, D, N, trixi_hip::VOL_HG_RANOCHA, EPB, WPD, BND) \
-  TRIXI_REG(T, D, N, trixi_hip::VOL_HG_CENTRAL, EPB, WPD, BND) \
-  TRIXI_REG(T, D, N, trixi_hip::VOL_PURE_FV, EPB, WPD, BND)
+#define TRIXI_REG_ALLVOL(T, D, N, EPB, WPD, BND, STAGE)          \
+  TRIXI_REG(T, D, N, trixi_hip::VOL_WEAK, EPB, WPD, BND, STAGE)       \
+  TRIXI_REG(T, D, N, trixi_hip::VOL_FD_RANOCHA, EPB, WPD, BND, STAGE) \
+  TRIXI_REG(T, D, N, trixi_hip::VOL_FD_CENTRAL, EPB, WPD, BND, STAGE) \
+  TRIXI_REG(T, D, N, trixi_hip::VOL_HG_RANOCHA, EPB, WPD, BND, STAGE) \
+  TRIXI_REG(T, D, N, trixi_hip::VOL_HG_CENTRAL, EPB, WPD, BND, STAGE) \
+  TRIXI_REG(T, D, N, trixi_hip::VOL_PURE_FV, EPB, WPD, BND, STAGE)
 
 // the default tiles (one wave per direction), also listed in kDefaultTiles
-#define TRIXI_REG_DEFAULTS(T, BND)          \
-  TRIXI_REG_ALLVOL(T, 2, 2, 21, 1, BND)     \
-  TRIXI_REG_ALLVOL(T, 2, 3, 16, 1, BND)     \
-  TRIXI_REG_ALLVOL(T, 2, 4, 12, 1, BND)     \
-  TRIXI_REG_ALLVOL(T, 3, 2, 7, 1, BND)      \
-  TRIXI_REG_ALLVOL(T, 3, 3, 4, 1, BND)      \
-  TRIXI_REG_ALLVOL(T, 3, 4, 2, 1, BND)
+#define TRIXI_REG_DEFAULTS(T, BND, STAGE)          \
+  TRIXI_REG_ALLVOL(T, 2, 2, 21, 1, BND, STAGE)     \
+  TRIXI_REG_ALLVOL(T, 2, 3, 16, 1, BND, STAGE)     \
+  TRIXI_REG_ALLVOL(T, 2, 4, 12, 1, BND, STAGE)     \
+  TRIXI_REG_ALLVOL(T, 3, 2, 7, 1, BND, STAGE)      \
+  TRIXI_REG_ALLVOL(T, 3, 3, 4, 1, BND, STAGE)      \
+  TRIXI_REG_ALLVOL(T, 3, 4, 2, 1, BND, STAGE)

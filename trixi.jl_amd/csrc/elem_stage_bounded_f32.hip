@@ -1,0 +1,4 @@
+// F32 element kernels that apply the 2N stage update (default tiles, bounded meshes).
+#include "elem_registry.hpp"
+
+TRIXI_REG_DEFAULTS(float, true, true)

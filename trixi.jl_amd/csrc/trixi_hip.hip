@@ -85,9 +85,13 @@ struct trixi_hip_ctx {
   int32_t *d_bnd = nullptr;      // [B] {element, direction}
   void *d_ghost = nullptr;       // [B, n^(D-1), V] outer boundary states, native precision
   double *d_invj = nullptr, *d_centers = nullptr;
-  // resident state
-  double *d_u = nullptr, *d_utmp = nullptr;
-  float *d_u32 = nullptr;
+  // resident state: FP64 RK registers u (a pair in FP64 mode, one buffer in
+  // FP32 mode) and u_tmp, the FP32 shadow of u (a pair in FP32 mode). The fused
+  // stage kernel reads the native state of buffer par and writes the other one
+  // (neighbour faces are read while other workgroups write), then par flips.
+  double *d_ub[2] = {nullptr, nullptr}, *d_utmp = nullptr;
+  float *d_u32b[2] = {nullptr, nullptr};
+  int par = 0;
   void *d_du = nullptr;  // native precision
   // staging for trixi_hip_rhs (FP64 interface)
   double *d_hu = nullptr, *d_hdu = nullptr;
@@ -104,16 +108,21 @@ struct trixi_hip_ctx {
   HGParams hg{};
   ExactParams exact{};
 
-  // element-kernel launch
-  ElemLaunch elem = nullptr;
+  // element-kernel launch (elem_stage: the variant with the fused 2N update,
+  // null when not instantiated for the tile or switched off)
+  ElemLaunch elem = nullptr, elem_stage = nullptr;
   int epb = 0, nt = 0;
   std::vector<unsigned char> kconst;  // DGConst<T, N> blob
   int xcd_remap = 0;  // measured: the round-robin XCD placement is faster here
 
-  // graph of one RK step
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  std::vector<double> graph_key;
+  // captured RK steps, keyed by scheme, stream, entry point and start parity
+  // (two slots: consecutive steps of a scheme with an odd stage count alternate)
+  struct GraphSlot {
+    std::vector<double> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } graphs[2];
+  int graph_next = 0;
 
   // blending factor source for shock capturing (TRIXI_HIP_IND_*)
   int ind_mode = TRIXI_HIP_IND_HG;
@@ -136,6 +145,21 @@ int set_err(trixi_hip_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
   return code;
 }
+
+// the current resident state: FP64 register u, FP32 shadow (FP32 mode), native
+double *cur_u64(trixi_hip_ctx *c) { return c->d_ub[c->fp32 ? 0 : c->par]; }
+float *cur_u32(trixi_hip_ctx *c) { return c->d_u32b[c->par]; }
+const void *native_u(trixi_hip_ctx *c) { return c->fp32 ? (const void *)cur_u32(c) : (const void *)cur_u64(c); }
+
+// outputs of the fused stage kernel (ElemArgs fields of the STAGE variants)
+struct StageOut {
+  double *u_tmp = nullptr;
+  const double *u_reg_in = nullptr;
+  double *u_reg_out = nullptr;
+  float *u32_out = nullptr;
+  double a_stage = 0, b_stage = 0;
+  int first_stage = 0;
+};
 
 #define HIP_TRY(call)                                                                            \
   do {                                                                                           \
@@ -171,12 +195,15 @@ int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
         nt = t.nt;
       }
   }
-  auto it = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B > 0 ? 1 : 0});
+  auto it = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B > 0 ? 1 : 0, 0});
   if (it == elem_registry().end())
     return set_err(ctx, TRIXI_HIP_EINVAL,
                    "no element kernel instantiated for this (precision, ndims, polydeg, volume "
                    "integral, tile, boundaries)");
   ctx->elem = it->second;
+  auto st = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B > 0 ? 1 : 0, 1});
+  const char *unfused = std::getenv("TRIXI_HIP_UNFUSED_STAGE");
+  ctx->elem_stage = (st == elem_registry().end() || (unfused && std::atoi(unfused) != 0)) ? nullptr : st->second;
   ctx->epb = epb;
   ctx->nt = nt;
   return TRIXI_HIP_OK;
@@ -222,7 +249,8 @@ template <typename T> int launch_indicator_raw(trixi_hip_ctx *ctx, const T *u) {
 }
 
 template <typename T>
-int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *tdev) {
+int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *tdev,
+                 const StageOut *so = nullptr) {
   // shock capturing: raw indicator first (needs the whole state), smoothing happens inline
   if (ctx->vol == VOL_HG_RANOCHA || ctx->vol == VOL_HG_CENTRAL) {
     int r = launch_indicator_raw<T>(ctx, u);
@@ -261,8 +289,17 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   a.gamma = T(ctx->cfg.gamma);
   a.inv_gamma_m1 = T(1) / (a.gamma - T(1));  // the device expression, evaluated once in T
   a.sqrt_gamma = T(std::sqrt(ctx->cfg.gamma));
+  if (so) {
+    a.u_tmp = so->u_tmp;
+    a.u_reg_in = so->u_reg_in;
+    a.u_reg_out = so->u_reg_out;
+    a.u32_out = so->u32_out;
+    a.a_stage = so->a_stage;
+    a.b_stage = so->b_stage;
+    a.first_stage = so->first_stage;
+  }
   int nblocks = (ctx->E + ctx->epb - 1) / ctx->epb;
-  hipError_t e = ctx->elem(&a, ctx->kconst.data(), nblocks, ctx->stream);
+  hipError_t e = (so ? ctx->elem_stage : ctx->elem)(&a, ctx->kconst.data(), nblocks, ctx->stream);
   if (e != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("element kernel launch: ") + hipGetErrorString(e));
   return TRIXI_HIP_OK;
 }
@@ -289,13 +326,39 @@ int launch_update(trixi_hip_ctx *ctx, double a_stage, double b_stage, const doub
   size_t cnt = ctx->nvals;
   if (ctx->fp32)
     hipLaunchKernelGGL((rk2n_update_kernel<float>), dim3(grid_for(cnt)), dim3(256), 0, ctx->stream,
-                       (const float *)ctx->d_du, ctx->d_utmp, ctx->d_u, ctx->d_u32, cnt, a_stage, b_stage,
+                       (const float *)ctx->d_du, ctx->d_utmp, cur_u64(ctx), cur_u32(ctx), cnt, a_stage, b_stage,
                        tdev, first);
   else
     hipLaunchKernelGGL((rk2n_update_kernel<double>), dim3(grid_for(cnt)), dim3(256), 0, ctx->stream,
-                       (const double *)ctx->d_du, ctx->d_utmp, ctx->d_u, (float *)nullptr, cnt, a_stage,
+                       (const double *)ctx->d_du, ctx->d_utmp, cur_u64(ctx), (float *)nullptr, cnt, a_stage,
                        b_stage, tdev, first);
   HIP_TRY(hipGetLastError());
+  return TRIXI_HIP_OK;
+}
+
+// One 2N stage (methods_2N.jl:146-159) of the resident state: the RHS at
+// t (tdev == null) or tdev[0] + t tdev[1], and the update with b_dt = b_stage
+// (tdev == null) or b_stage tdev[1]. With the fused kernel the update runs in
+// the element kernel's epilogue and the state moves to the other buffer of
+// its pair (par flips); otherwise the RHS writes du and rk2n_update_kernel
+// updates in place.
+int launch_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage, double t, const double *tdev, int first) {
+  if (!ctx->elem_stage) {
+    int r = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), t, tdev);
+    return r ? r : launch_update(ctx, a_stage, b_stage, tdev, first);
+  }
+  StageOut so;
+  so.u_tmp = ctx->d_utmp;
+  so.u_reg_in = cur_u64(ctx);
+  so.u_reg_out = ctx->fp32 ? ctx->d_ub[0] : ctx->d_ub[ctx->par ^ 1];
+  so.u32_out = ctx->fp32 ? ctx->d_u32b[ctx->par ^ 1] : nullptr;
+  so.a_stage = a_stage;
+  so.b_stage = b_stage;
+  so.first_stage = first;
+  int r = ctx->fp32 ? launch_rhs_T<float>(ctx, nullptr, (const float *)native_u(ctx), t, tdev, &so)
+                    : launch_rhs_T<double>(ctx, nullptr, (const double *)native_u(ctx), t, tdev, &so);
+  if (r) return r;
+  ctx->par ^= 1;
   return TRIXI_HIP_OK;
 }
 
@@ -306,7 +369,46 @@ __global__ void set_time_kernel(double *tdt, double t, double dt) {
   }
 }
 
-const void *native_u(trixi_hip_ctx *ctx) { return ctx->fp32 ? (const void *)ctx->d_u32 : (const void *)ctx->d_u; }
+void drop_graphs(trixi_hip_ctx *ctx) {
+  for (auto &g : ctx->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = trixi_hip_ctx::GraphSlot{};
+  }
+}
+
+hipGraphExec_t find_graph(trixi_hip_ctx *ctx, const std::vector<double> &key) {
+  for (auto &g : ctx->graphs)
+    if (g.exec && g.key == key) return g.exec;
+  return nullptr;
+}
+
+// Captures what record() launches on the context's stream into a slot; the
+// host-side parity is restored afterwards (capture executes nothing).
+template <typename F> int capture_graph(trixi_hip_ctx *ctx, const std::vector<double> &key, F &&record,
+                                        hipGraphExec_t *out) {
+  auto &g = ctx->graphs[ctx->graph_next];
+  ctx->graph_next ^= 1;
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g = trixi_hip_ctx::GraphSlot{};
+  const int par0 = ctx->par;
+  HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int r = record();
+  hipGraph_t graph = nullptr;
+  hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+  ctx->par = par0;
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (ec != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("end capture: ") + hipGetErrorString(ec));
+  g.graph = graph;
+  HIP_TRY(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  g.key = key;
+  *out = g.exec;
+  return TRIXI_HIP_OK;
+}
 
 }  // namespace
 
@@ -412,7 +514,8 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   bool ok = hipMalloc(&ctx->d_nbr, nbr.size() * 4) == hipSuccess &&
             hipMalloc(&ctx->d_invj, E * 8) == hipSuccess &&
             hipMalloc(&ctx->d_centers, (size_t)D * E * 8) == hipSuccess &&
-            hipMalloc(&ctx->d_u, nv * 8) == hipSuccess && hipMalloc(&ctx->d_utmp, nv * 8) == hipSuccess &&
+            hipMalloc(&ctx->d_ub[0], nv * 8) == hipSuccess && hipMalloc(&ctx->d_utmp, nv * 8) == hipSuccess &&
+            (ctx->fp32 || hipMalloc(&ctx->d_ub[1], nv * 8) == hipSuccess) &&
             hipMalloc(&ctx->d_du, nv * tsz) == hipSuccess &&
             hipMalloc(&ctx->d_alpha_raw, E * 8) == hipSuccess && hipMalloc(&ctx->d_alpha, E * 8) == hipSuccess &&
             hipMalloc(&ctx->d_partial, ctx->partial_cap * 8) == hipSuccess &&
@@ -421,7 +524,8 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     ok = hipMalloc(&ctx->d_bnd, bnd.size() * 4) == hipSuccess &&
          hipMalloc(&ctx->d_ghost, (size_t)ctx->B * ipow_c(n, D - 1) * ctx->V * tsz) == hipSuccess;
   if (ok && ctx->fp32)
-    ok = hipMalloc(&ctx->d_u32, nv * 4) == hipSuccess && hipMalloc(&ctx->d_su32, nv * 4) == hipSuccess &&
+    ok = hipMalloc(&ctx->d_u32b[0], nv * 4) == hipSuccess && hipMalloc(&ctx->d_u32b[1], nv * 4) == hipSuccess &&
+         hipMalloc(&ctx->d_su32, nv * 4) == hipSuccess &&
          hipMalloc(&ctx->d_sdu32, nv * 4) == hipSuccess;
   if (!ok) {
     trixi_hip_destroy(ctx);
@@ -448,9 +552,9 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
 int trixi_hip_destroy(trixi_hip_ctx *ctx) {
   if (!ctx) return TRIXI_HIP_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-  void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_u, ctx->d_utmp, ctx->d_u32, ctx->d_du,
+  drop_graphs(ctx);
+  void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_ub[0], ctx->d_ub[1], ctx->d_utmp,
+                  ctx->d_u32b[0], ctx->d_u32b[1], ctx->d_du,
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
                   ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
                   ctx->d_alpha_hg, ctx->d_records, ctx->d_bnd, ctx->d_ghost};
@@ -477,9 +581,9 @@ int trixi_hip_synchronize(trixi_hip_ctx *ctx) {
 
 int trixi_hip_upload_u(trixi_hip_ctx *ctx, const double *u_host) {
   if (!ctx || !u_host) return TRIXI_HIP_EINVAL;
-  HIP_TRY(hipMemcpyAsync(ctx->d_u, u_host, ctx->nvals * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(cur_u64(ctx), u_host, ctx->nvals * 8, hipMemcpyHostToDevice, ctx->stream));
   if (ctx->fp32) {
-    int r = convert<float, double>(ctx, ctx->d_u32, ctx->d_u, ctx->nvals);
+    int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
     if (r) return r;
   }
   HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -488,16 +592,16 @@ int trixi_hip_upload_u(trixi_hip_ctx *ctx, const double *u_host) {
 
 int trixi_hip_download_u(trixi_hip_ctx *ctx, double *u_host) {
   if (!ctx || !u_host) return TRIXI_HIP_EINVAL;
-  HIP_TRY(hipMemcpyAsync(u_host, ctx->d_u, ctx->nvals * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(u_host, cur_u64(ctx), ctx->nvals * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return TRIXI_HIP_OK;
 }
 
 int trixi_hip_set_u_device(trixi_hip_ctx *ctx, const double *u_dev) {
   if (!ctx || !u_dev) return TRIXI_HIP_EINVAL;
-  HIP_TRY(hipMemcpyAsync(ctx->d_u, u_dev, ctx->nvals * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(cur_u64(ctx), u_dev, ctx->nvals * 8, hipMemcpyDeviceToDevice, ctx->stream));
   if (ctx->fp32) {
-    int r = convert<float, double>(ctx, ctx->d_u32, ctx->d_u, ctx->nvals);
+    int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
     if (r) return r;
   }
   return TRIXI_HIP_OK;
@@ -505,7 +609,7 @@ int trixi_hip_set_u_device(trixi_hip_ctx *ctx, const double *u_dev) {
 
 int trixi_hip_get_u_device(trixi_hip_ctx *ctx, double *u_dev) {
   if (!ctx || !u_dev) return TRIXI_HIP_EINVAL;
-  HIP_TRY(hipMemcpyAsync(u_dev, ctx->d_u, ctx->nvals * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(u_dev, cur_u64(ctx), ctx->nvals * 8, hipMemcpyDeviceToDevice, ctx->stream));
   return TRIXI_HIP_OK;
 }
 
@@ -552,9 +656,7 @@ int trixi_hip_rhs_native(trixi_hip_ctx *ctx, void *du_dev, const void *u_dev, do
 int trixi_hip_rk_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage_dt, double t_stage,
                        int first_stage) {
   if (!ctx) return TRIXI_HIP_EINVAL;
-  int r = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), t_stage, nullptr);
-  if (r) return r;
-  return launch_update(ctx, a_stage, b_stage_dt, nullptr, first_stage);
+  return launch_stage(ctx, a_stage, b_stage_dt, t_stage, nullptr, first_stage);
 }
 
 int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, const double *c,
@@ -566,45 +668,27 @@ int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, cons
   auto record = [&]() -> int {
     for (int s = 0; s < nstages; ++s) {
       // stage kernels read t and dt from device memory; ElemArgs.t carries c_stage
-      int r = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), c[s], tdt);
-      if (r) return r;
-      r = launch_update(ctx, a[s], b[s], tdt, s == 0);
+      int r = launch_stage(ctx, a[s], b[s], c[s], tdt, s == 0);
       if (r) return r;
     }
     return TRIXI_HIP_OK;
   };
   if (!use_graph) return record();
-  std::vector<double> key;
-  key.push_back(nstages);
+  std::vector<double> key{-2.0, (double)nstages};
   for (int s = 0; s < nstages; ++s) {
     key.push_back(a[s]);
     key.push_back(b[s]);
     key.push_back(c[s]);
   }
   key.push_back((double)(uintptr_t)ctx->stream);
-  if (!ctx->graph_exec || key != ctx->graph_key) {
-    if (ctx->graph_exec) {
-      (void)hipGraphExecDestroy(ctx->graph_exec);
-      ctx->graph_exec = nullptr;
-    }
-    if (ctx->graph) {
-      (void)hipGraphDestroy(ctx->graph);
-      ctx->graph = nullptr;
-    }
-    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int r = record();
-    hipGraph_t g = nullptr;
-    hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
-    if (r) {
-      if (g) (void)hipGraphDestroy(g);
-      return r;
-    }
-    if (ec != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("end capture: ") + hipGetErrorString(ec));
-    ctx->graph = g;
-    HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
-    ctx->graph_key = key;
+  key.push_back((double)ctx->par);  // buffers the captured stages read and write
+  hipGraphExec_t x = find_graph(ctx, key);
+  if (!x) {
+    int r = capture_graph(ctx, key, record, &x);
+    if (r) return r;
   }
-  HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  HIP_TRY(hipGraphLaunch(x, ctx->stream));
+  if (ctx->elem_stage) ctx->par ^= (nstages & 1);
   return TRIXI_HIP_OK;
 }
 
@@ -625,7 +709,7 @@ int launch_max_speed(trixi_hip_ctx *ctx) {
       nb = -1;
       return;
     }
-    hipLaunchKernelGGL((max_speed_nodes_kernel<double, D, N>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d_u,
+    hipLaunchKernelGGL((max_speed_nodes_kernel<double, D, N>), dim3(nb), dim3(256), 0, ctx->stream, cur_u64(ctx),
                        ctx->d_invj, ctx->E, ctx->cfg.gamma, ctx->d_partial);
   });
   if (nb < 0) return set_err(ctx, TRIXI_HIP_EINVAL, "too many elements for the reduction buffer");
@@ -669,7 +753,7 @@ int enqueue_error_sums(trixi_hip_ctx *ctx, double t, const double *tdev, double 
     for (int i = 0; i < NA * (N + 1); ++i) ac.vdm[i] = ctx->a_vdm[i];
     for (int i = 0; i < NA; ++i) ac.aw[i] = ctx->a_weights[i];
     for (int i = 0; i <= N; ++i) ac.nodes[i] = ctx->nodes[i];
-    hipLaunchKernelGGL((error_norms_kernel<double, D, N, NA>), dim3(E), dim3(256), 0, ctx->stream, ctx->d_u,
+    hipLaunchKernelGGL((error_norms_kernel<double, D, N, NA>), dim3(E), dim3(256), 0, ctx->stream, cur_u64(ctx),
                        ctx->d_invj, ctx->d_centers, E, t, tdev, ctx->exact, ac, ctx->d_partial);
   });
   if (!ok) return TRIXI_HIP_EINVAL;
@@ -686,7 +770,7 @@ int enqueue_integral_sums(trixi_hip_ctx *ctx, double *out) {
   bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
     constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
     hipLaunchKernelGGL((integrals_kernel<double, D, N>), dim3((E + 255) / 256), dim3(256), 0, ctx->stream,
-                       ctx->d_u, ctx->d_invj, E, ctx->cfg.gamma, ctx->d_scalars + kWeights, ctx->d_partial);
+                       cur_u64(ctx), ctx->d_invj, E, ctx->cfg.gamma, ctx->d_scalars + kWeights, ctx->d_partial);
   });
   if (!ok) return TRIXI_HIP_EINVAL;
   HIP_TRY(hipGetLastError());
@@ -777,7 +861,8 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
   hipLaunchKernelGGL(solve_init_kernel, dim3(1), dim3(1), 0, ctx->stream, tdt, ctl, t0, dt, t_end, cfl, last,
                      (double)max_steps);
   HIP_TRY(hipGetLastError());
-  // one step = the stages + the step control, captured once per (scheme, stream)
+  // one step = the stages + the wave-speed reduction + the step control, one
+  // graph per (scheme, stream, start parity of the state buffers)
   std::vector<double> key{-1.0, (double)nstages};
   for (int s = 0; s < nstages; ++s) {
     key.push_back(a[s]);
@@ -785,31 +870,36 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
     key.push_back(c[s]);
   }
   key.push_back((double)(uintptr_t)ctx->stream);
-  if (!ctx->graph_exec || key != ctx->graph_key) {
-    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->graph_exec = nullptr;
-    ctx->graph = nullptr;
-    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rr = TRIXI_HIP_OK;
-    for (int s = 0; s < nstages && rr == TRIXI_HIP_OK; ++s) {
-      rr = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), c[s], tdt);
-      if (rr == TRIXI_HIP_OK) rr = launch_update(ctx, a[s], b[s], tdt, s == 0);
+  auto record = [&]() -> int {
+    for (int s = 0; s < nstages; ++s) {
+      int rr = launch_stage(ctx, a[s], b[s], c[s], tdt, s == 0);
+      if (rr) return rr;
     }
-    int nb = rr == TRIXI_HIP_OK ? launch_max_speed(ctx) : -1;
-    if (nb > 0)
-      hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, nb, ctx->n, tdt,
-                         ctl);
-    hipGraph_t g = nullptr;
-    hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
-    if (rr != TRIXI_HIP_OK || nb <= 0) {
-      if (g) (void)hipGraphDestroy(g);
-      return rr != TRIXI_HIP_OK ? rr : nb;
+    const int nb = launch_max_speed(ctx);
+    if (nb < 0) return nb;
+    hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, nb, ctx->n, tdt,
+                       ctl);
+    HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+  };
+  // with the fused stage an odd stage count ends a step in the other buffer:
+  // consecutive replays alternate between the graphs of the two parities
+  const int flip = ctx->elem_stage ? (nstages & 1) : 0;
+  auto step_graph = [&](hipGraphExec_t *x) -> int {
+    std::vector<double> k = key;
+    k.push_back((double)ctx->par);
+    *x = find_graph(ctx, k);
+    return *x ? TRIXI_HIP_OK : capture_graph(ctx, k, record, x);
+  };
+  {
+    hipGraphExec_t x = nullptr;
+    if ((r = step_graph(&x))) return r;
+    if (flip) {
+      ctx->par ^= 1;
+      r = step_graph(&x);
+      ctx->par ^= 1;
+      if (r) return r;
     }
-    if (ec != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("end capture: ") + hipGetErrorString(ec));
-    ctx->graph = g;
-    HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
-    ctx->graph_key = key;
   }
   // Replay without host synchronisation, reading the control block between
   // batches. A batch is at most poll_every steps and, near the end, no more than
@@ -828,7 +918,12 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
     int batch = left >= poll_every ? poll_every : (left >= 1.0 ? (int)left : 1);
     batch = std::min(batch, std::max(1, max_steps - (int)h[4]));
     if (every > 0) batch = std::min(batch, every - (int)h[4] % every);
-    for (int i = 0; i < batch; ++i) HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    for (int i = 0; i < batch; ++i) {
+      hipGraphExec_t x = nullptr;
+      if ((r = step_graph(&x))) return r;
+      HIP_TRY(hipGraphLaunch(x, ctx->stream));
+      ctx->par ^= flip;
+    }
     launched += batch;
     recorded = every > 0 && ((int)h[4] + batch) % every == 0;
     if (recorded && (r = enqueue_record(ctx))) return r;
@@ -856,7 +951,7 @@ int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host) {
     return set_err(ctx, TRIXI_HIP_ESTATE, "the learned indicator has no parameters (trixi_hip_mlp_set_params)");
   const int saved = ctx->ind_mode;
   ctx->ind_mode = mode;
-  int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, ctx->d_u32) : launch_indicator_raw<double>(ctx, ctx->d_u);
+  int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, cur_u32(ctx)) : launch_indicator_raw<double>(ctx, cur_u64(ctx));
   ctx->ind_mode = saved;
   if (r) return r;
   bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto) {
@@ -918,12 +1013,9 @@ int trixi_hip_set_indicator(trixi_hip_ctx *ctx, int mode) {
   if (!ctx || mode < TRIXI_HIP_IND_HG || mode > TRIXI_HIP_IND_MLP_WITH_FALLBACK) return TRIXI_HIP_EINVAL;
   if (mode != TRIXI_HIP_IND_HG && !ctx->mlp_ready)
     return set_err(ctx, TRIXI_HIP_ESTATE, "the learned indicator has no parameters (trixi_hip_mlp_set_params)");
-  if (mode != ctx->ind_mode && ctx->graph_exec) {  // the captured step launches the indicator kernels
-    (void)hipGraphExecDestroy(ctx->graph_exec);
-    ctx->graph_exec = nullptr;
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->graph = nullptr;
-    ctx->graph_key.clear();
+  if (mode != ctx->ind_mode) {  // the captured steps launch the indicator kernels
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
   }
   ctx->ind_mode = mode;
   return TRIXI_HIP_OK;
@@ -933,8 +1025,8 @@ int trixi_hip_mlp_features(trixi_hip_ctx *ctx, float *features_host, double *alp
   if (!ctx) return TRIXI_HIP_EINVAL;
   int r = mlp_check(ctx);
   if (r) return r;
-  r = ctx->fp32 ? launch_hg<float>(ctx, ctx->d_u32, ctx->d_alpha_hg, ctx->d_feat)
-                : launch_hg<double>(ctx, ctx->d_u, ctx->d_alpha_hg, ctx->d_feat);
+  r = ctx->fp32 ? launch_hg<float>(ctx, cur_u32(ctx), ctx->d_alpha_hg, ctx->d_feat)
+                : launch_hg<double>(ctx, cur_u64(ctx), ctx->d_alpha_hg, ctx->d_feat);
   if (r) return r;
   if (features_host)
     HIP_TRY(hipMemcpyAsync(features_host, ctx->d_feat, (size_t)ctx->E * kMlpFeatureStride * 4,
@@ -1003,14 +1095,9 @@ int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float
 int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *params, int nparams,
                                  const double *modes) {
   if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_FOURIER_MODES) return TRIXI_HIP_EINVAL;
-  if (ctx->graph_exec) {  // a captured step holds the boundary-state kernel with the old function
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    (void)hipGraphExecDestroy(ctx->graph_exec);
-    ctx->graph_exec = nullptr;
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->graph = nullptr;
-    ctx->graph_key.clear();
-  }
+  // captured steps hold the boundary-state kernel with the old function
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
   ctx->exact.kind = ic_kind;
   ctx->exact.gamma = ctx->cfg.gamma;
   for (int i = 0; i < 4; ++i) ctx->exact.params[i] = (params && i < nparams) ? params[i] : 0.0;
