@@ -52,6 +52,9 @@ def parse():
                    help="also run one plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE)")
     p.add_argument("--no-bounded", action="store_true",
                    help="skip the same RHS on the non-periodic mesh with Dirichlet boundaries")
+    p.add_argument("--profile-step", choices=["fp32", "fp64"], default=None,
+                   help="profiling mode: only run K device-controlled CK2N54 steps (fused stages) of the "
+                        "workload in one precision and print the time per step (no JSON line)")
     return p.parse_args()
 
 
@@ -223,6 +226,15 @@ def main():
     import torch
     import trixi_hip as th
     torch.cuda.set_device(0)
+    if args.profile_step:
+        semi = make_semi(th, args.level, args.polydeg, args.profile_step)
+        ode = th.semidiscretize(semi, (0.0, 10.0))
+        sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=1.0)),
+                       maxiters=args.steps, save_final=False)
+        print(f"[bench] {args.profile_step}: {sol.nsteps} CK2N54 steps, {1e6 * sol.wall_time / sol.nsteps:.1f} us "
+              "per step (device step control, fused stages)", flush=True)
+        semi.close()
+        return
 
     res = {}
     semis = {}
