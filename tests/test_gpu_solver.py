@@ -246,3 +246,42 @@ def test_fused_stage_equals_rhs_plus_update(case, precision, monkeypatch):
     assert ref[:2] == fused[:2] and ref[0] == 13
     for k in range(2, len(ref)):
         assert np.array_equal(ref[k], fused[k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 5e-6)])
+def test_rk_steps_with_time_independent_boundaries(precision, tol):
+    """A time-independent Dirichlet boundary function (weak blast wave) is
+    evaluated once per context: eager steps, graph replays and a
+    device-controlled solve agree bitwise, and a step equals the oracle's."""
+    import trixi_hip as th
+    spec = dict(ndims=3, polydeg=3, level=2, cmin=(-2, -2, -2), cmax=(2, 2, 2), ic="weak_blast_wave",
+                volume_integral="fd", surface_flux="llf", periodicity=(True, False, False))
+    o = oracle_for(spec)
+    alg = th.CarpenterKennedy2N54()
+    na = 5
+    a = (ctypes.c_double * na)(*alg.a)
+    b = (ctypes.c_double * na)(*alg.b)
+    c = (ctypes.c_double * na)(*alg.c)
+    res = {}
+    for graph in (0, 1):
+        semi, _ = build(spec, precision)
+        lib, ctx = semi._lib, semi.ctx
+        u0 = semi.compute_coefficients(0.0)
+        upload(semi, u0)
+        outs = []
+        for rep in range(3):
+            assert lib.trixi_hip_rk_step(ctx, a, b, c, na, 0.1 + rep * 2e-3, 2e-3, graph) == 0
+            outs.append(download(semi))
+        res[graph] = outs
+        if graph == 1:
+            assert relerr(outs[0], o.rk_step(u0.copy(), 0.1, 2e-3)) < tol
+            sols = []
+            for dev in (False, True):
+                ode = th.semidiscretize(semi, (0.0, 0.05))
+                sols.append(th.solve(ode, th.CarpenterKennedy2N54(),
+                                     callback=th.CallbackSet(th.StepsizeCallback(cfl=0.8)), device_steps=dev))
+            assert sols[0].nsteps == sols[1].nsteps and np.array_equal(sols[0].u, sols[1].u)
+        semi.close()
+    for rep in range(3):
+        assert np.array_equal(res[0][rep], res[1][rep]), rep

@@ -202,7 +202,8 @@ __device__ __forceinline__ double nanmax(double a, double b) {
 // max_dt (stepsize_dg3d.jl:8-32, stepsize_dg2d.jl:8-30) in two passes: per
 // block, max over its elements of inv_jac * sum_d max_nodes lambda_d with
 // lambda_d = |v_d| + c (max_abs_speeds, compressible_euler_3d.jl:1770-1775);
-// one thread per node (EB = 256 / NN elements per block, coalesced loads). The
+// one thread per node (groups of EB = 256 / NN elements, coalesced loads; a
+// block walks groups with a grid stride and keeps its running maximum). The
 // node maxima are reduced through LDS in parallel: S threads per (element,
 // direction) each take every S-th node, then one thread per pair combines
 // the S partials. Maxima are exact, so the result does not depend on the
@@ -219,48 +220,68 @@ __global__ __launch_bounds__(256) void max_speed_nodes_kernel(const T *__restric
   __shared__ double s_p[NPAIR * S];
   __shared__ double s_e[EB];
   const int tid = threadIdx.x;
-  const int e0 = blockIdx.x * EB;
-  const int ne = min(EB, E - e0);
-  if (tid < ne * NN) {
-    const T *un = u + ((size_t)e0 * NN + tid) * V;
-    const double rho = un[0];
-    double ke2 = 0, v[D];
+  const int ngroups = (E + EB - 1) / EB;
+  double bmax = 4.9406564584124654e-324;  // nextfloat(0.0), as the reference starts
+  // groups of EB elements, grid-stride (block-uniform loop: the barriers are safe)
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int e0 = grp * EB;
+    const int ne = min(EB, E - e0);
+    if (tid < ne * NN) {
+      const T *un = u + ((size_t)e0 * NN + tid) * V;
+      const double rho = un[0];
+      double ke2 = 0, v[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      v[d] = double(un[1 + d]) / rho;
-      ke2 += double(un[1 + d]) * v[d];
+      for (int d = 0; d < D; ++d) {
+        v[d] = double(un[1 + d]) / rho;
+        ke2 += double(un[1 + d]) * v[d];
+      }
+      const double p = (gamma - 1) * (double(un[D + 1]) - 0.5 * ke2);
+      const double c = sqrt(gamma * p / rho);
+#pragma unroll
+      for (int d = 0; d < D; ++d) s_l[d][tid] = fabs(v[d]) + c;
     }
-    const double p = (gamma - 1) * (double(un[D + 1]) - 0.5 * ke2);
-    const double c = sqrt(gamma * p / rho);
-#pragma unroll
-    for (int d = 0; d < D; ++d) s_l[d][tid] = fabs(v[d]) + c;
-  }
-  __syncthreads();
-  for (int k = tid; k < NPAIR * S; k += 256) {
-    const int pair = k / S, sub = k - pair * S;
-    const int el = pair / D, d = pair - el * D;
-    double m = 0.0;
-    if (el < ne)
-      for (int q = sub; q < NN; q += S) m = nanmax(m, s_l[d][el * NN + q]);
-    s_p[k] = m;
-  }
-  __syncthreads();
-  if (tid < ne) {
-    double s = 0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
+    __syncthreads();
+    for (int k = tid; k < NPAIR * S; k += 256) {
+      const int pair = k / S, sub = k - pair * S;
+      const int el = pair / D, d = pair - el * D;
       double m = 0.0;
-#pragma unroll
-      for (int j = 0; j < S; ++j) m = nanmax(m, s_p[(tid * D + d) * S + j]);
-      s += m;
+      if (el < ne)
+        for (int q = sub; q < NN; q += S) m = nanmax(m, s_l[d][el * NN + q]);
+      s_p[k] = m;
     }
-    s_e[tid] = inv_jac[e0 + tid] * s;
+    __syncthreads();
+    if (tid < ne) {
+      double sum = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double m = 0.0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) m = nanmax(m, s_p[(tid * D + d) * S + j]);
+        sum += m;
+      }
+      s_e[tid] = inv_jac[e0 + tid] * sum;
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int i = 0; i < ne; ++i) bmax = nanmax(bmax, s_e[i]);
+    __syncthreads();  // s_l, s_p and s_e are rewritten by the next group
   }
+  if (tid == 0) partial[blockIdx.x] = bmax;
+}
+
+// Max (NaN-propagating) of np partials by one workgroup of kRedThreads
+// threads: independent strided loads, then a tree in LDS; s_red[0] holds the
+// result for every thread after the call.
+constexpr int kRedThreads = 1024;
+__device__ __forceinline__ void block_nanmax(const double *__restrict__ partial, int np, double *s_red) {
+  double m = 4.9406564584124654e-324;
+#pragma unroll 8
+  for (int i = threadIdx.x; i < np; i += kRedThreads) m = nanmax(m, partial[i]);
+  s_red[threadIdx.x] = m;
   __syncthreads();
-  if (tid == 0) {
-    double m = 4.9406564584124654e-324;
-    for (int i = 0; i < ne; ++i) m = nanmax(m, s_e[i]);
-    partial[blockIdx.x] = m;
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s_red[threadIdx.x] = nanmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
+    __syncthreads();
   }
 }
 
@@ -273,17 +294,11 @@ __global__ __launch_bounds__(256) void max_speed_nodes_kernel(const T *__restric
 //   NaN dt -> error, done; steps == max_steps -> done; done -> dt = 0.
 // tdt = {t, dt}; ctl = {t_end, cfl, last, done, steps, error, max_steps}. A
 // step replayed after done leaves u and t unchanged (dt = 0).
-__global__ __launch_bounds__(256) void step_control_kernel(const double *__restrict__ partial, int np, int nnodes,
-                                                           double *__restrict__ tdt, double *__restrict__ ctl) {
-  __shared__ double s_red[256];
-  double m = 4.9406564584124654e-324;
-  for (int i = threadIdx.x; i < np; i += 256) m = nanmax(m, partial[i]);
-  s_red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s_red[threadIdx.x] = nanmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
-    __syncthreads();
-  }
+__global__ __launch_bounds__(kRedThreads) void step_control_kernel(const double *__restrict__ partial, int np,
+                                                                   int nnodes, double *__restrict__ tdt,
+                                                                   double *__restrict__ ctl) {
+  __shared__ double s_red[kRedThreads];
+  block_nanmax(partial, np, s_red);
   if (threadIdx.x != 0) return;
   if (ctl[3] != 0.0) {  // done: this replay changed nothing
     tdt[1] = 0.0;
@@ -313,18 +328,11 @@ __global__ __launch_bounds__(256) void step_control_kernel(const double *__restr
   ctl[4] = steps;
 }
 
-__global__ __launch_bounds__(256) void max_dt_finalize_kernel(const double *__restrict__ partial,
-                                                              int np, int nnodes, double cfl,
-                                                              double *__restrict__ out) {
-  __shared__ double s_red[256];
-  double m = 4.9406564584124654e-324;
-  for (int i = threadIdx.x; i < np; i += 256) m = nanmax(m, partial[i]);
-  s_red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s_red[threadIdx.x] = nanmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
-    __syncthreads();
-  }
+__global__ __launch_bounds__(kRedThreads) void max_dt_finalize_kernel(const double *__restrict__ partial,
+                                                                      int np, int nnodes, double cfl,
+                                                                      double *__restrict__ out) {
+  __shared__ double s_red[kRedThreads];
+  block_nanmax(partial, np, s_red);
   if (threadIdx.x == 0) out[0] = cfl * (2.0 / (nnodes * s_red[0]));
 }
 
@@ -560,41 +568,22 @@ __global__ __launch_bounds__(256) void reduce_columns_kernel(const double *__res
 
 // Outer states of the boundaries for BoundaryConditionDirichlet(initial_condition)
 // (equations.jl:159-182; calc_boundary_flux_by_direction!, dg_2d.jl:826-860):
-// ghost[b][l] = u_exact(x, t) at face node l of boundary b, where x are the
+// ghost[b][l] = u_exact(x, t) at face node l of boundary b. x = bx[b][l] are the
 // element's face node coordinates (calc_boundary_node_coordinates!,
-// containers_2d.jl:394-413) and l orders the face's free indices with the
-// lower dimension fastest. bnd[2b] = element, bnd[2b + 1] = 0-based direction.
-// The time is tdev[0] + t * tdev[1] (a stage of a running step) or t.
-template <int N> struct NodesConst {
-  double nodes[N + 1];
-};
-
+// containers_2d.jl:394-413), computed once at set-up; l orders the face's free
+// indices with the lower dimension fastest. The time is tdev[0] + t * tdev[1]
+// (a stage of a running step) or t.
 template <typename T, int D, int N>
-__global__ __launch_bounds__(256) void boundary_state_kernel(const int32_t *__restrict__ bnd, int B,
-                                                             const double *__restrict__ centers,
-                                                             const double *__restrict__ inv_jac,
-                                                             NodesConst<N> nc, double t,
-                                                             const double *__restrict__ tdev, ExactParams ep,
-                                                             T *__restrict__ ghost) {
-  constexpr int n = N + 1, L = ipow_c(n, D - 1), V = D + 2;
+__global__ __launch_bounds__(256) void boundary_state_kernel(const double *__restrict__ bx, int nodes_total,
+                                                             double t, const double *__restrict__ tdev,
+                                                             ExactParams ep, T *__restrict__ ghost) {
+  constexpr int V = D + 2;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= B * L) return;
-  const int b = k / L, l = k - b * L;
-  const int e = bnd[2 * b], dir = bnd[2 * b + 1], o = dir >> 1;
+  if (k >= nodes_total) return;
   const double tt = tdev ? (tdev[0] + t * tdev[1]) : t;
-  const double jac = 1.0 / inv_jac[e];
-  int idx[3];
-  idx[o] = (dir & 1) ? n - 1 : 0;
-  int r = l;
-#pragma unroll
-  for (int d = 0; d < D; ++d)
-    if (d != o) {
-      idx[d] = r % n;
-      r /= n;
-    }
   double x[3], ub[V];
 #pragma unroll
-  for (int d = 0; d < D; ++d) x[d] = centers[d + D * e] + jac * nc.nodes[idx[d]];
+  for (int d = 0; d < D; ++d) x[d] = bx[(size_t)k * D + d];
   exact_solution<D>(ep, x, tt, ub);
 #pragma unroll
   for (int v = 0; v < V; ++v) ghost[(size_t)k * V + v] = T(ub[v]);

@@ -65,6 +65,7 @@ template <typename F> bool dispatch_dn(int D, int N, F &&f) {
 }
 
 constexpr int kMaxPartials = 1 << 16;
+constexpr int kMaxSpeedBlocks = 1 << 16;  // workgroups (and partials) of the wave-speed reduction
 constexpr int kMlpMaxWaves = 1024;  // partial gradient rows of one training step
 
 }  // namespace
@@ -82,7 +83,8 @@ struct trixi_hip_ctx {
   // mesh / basis on the device
   int32_t *d_nbr = nullptr;
   int B = 0;                     // boundaries (non-periodic mesh)
-  int32_t *d_bnd = nullptr;      // [B] {element, direction}
+  double *d_bx = nullptr;        // [B, n^(D-1), D] coordinates of the boundary face nodes
+  bool ghost_ready = false;      // ghost holds the states of a time-independent boundary function
   void *d_ghost = nullptr;       // [B, n^(D-1), V] outer boundary states, native precision
   double *d_invj = nullptr, *d_centers = nullptr;
   // resident state: FP64 RK registers u (a pair in FP64 mode, one buffer in
@@ -256,15 +258,16 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
     int r = launch_indicator_raw<T>(ctx, u);
     if (r) return r;
   }
-  if (ctx->B > 0) {  // outer states of the Dirichlet boundaries at this stage's time
+  // outer states of the Dirichlet boundaries at this stage's time; a
+  // time-independent boundary function is evaluated once
+  const bool t_dep = ctx->exact.kind == TRIXI_HIP_IC_CONVERGENCE_TEST;
+  if (ctx->B > 0 && (t_dep || !ctx->ghost_ready)) {
+    ctx->ghost_ready = !t_dep;
     bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
       constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
-      NodesConst<N> nc{};
-      for (int i = 0; i <= N; ++i) nc.nodes[i] = ctx->nodes[i];
       const int total = ctx->B * ipow_c(N + 1, D - 1);
       hipLaunchKernelGGL((boundary_state_kernel<T, D, N>), dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
-                         ctx->d_bnd, ctx->B, ctx->d_centers, ctx->d_invj, nc, t, tdev, ctx->exact,
-                         (T *)ctx->d_ghost);
+                         ctx->d_bx, total, t, tdev, ctx->exact, (T *)ctx->d_ghost);
     });
     if (!ok) return TRIXI_HIP_EINVAL;
     HIP_TRY(hipGetLastError());
@@ -393,11 +396,13 @@ template <typename F> int capture_graph(trixi_hip_ctx *ctx, const std::vector<do
   if (g.graph) (void)hipGraphDestroy(g.graph);
   g = trixi_hip_ctx::GraphSlot{};
   const int par0 = ctx->par;
+  const bool ghost0 = ctx->ghost_ready;  // a graph that is never replayed computes no boundary states
   HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int r = record();
   hipGraph_t graph = nullptr;
   hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
   ctx->par = par0;
+  if (r || ec != hipSuccess) ctx->ghost_ready = ghost0;
   if (r) {
     if (graph) (void)hipGraphDestroy(graph);
     return r;
@@ -474,6 +479,25 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     bnd[2 * b] = e;
     bnd[2 * b + 1] = dir;
   }
+  // coordinates of the boundary face nodes: x_d = center_d + J node_{i_d}, with
+  // i_o at the face and the free indices in face-node order (lower dimension fastest)
+  const int Lf = D == 3 ? n * n : n;
+  std::vector<double> bx((size_t)ctx->B * Lf * D);
+  for (int b = 0; b < ctx->B; ++b) {
+    const int e = bnd[2 * b], dir = bnd[2 * b + 1], o = dir / 2;
+    const double jac = 1.0 / mesh->inverse_jacobian[e];
+    for (int l = 0; l < Lf; ++l) {
+      int idx[3], r = l;
+      idx[o] = (dir & 1) ? n - 1 : 0;
+      for (int d = 0; d < D; ++d)
+        if (d != o) {
+          idx[d] = r % n;
+          r /= n;
+        }
+      for (int d = 0; d < D; ++d)
+        bx[((size_t)b * Lf + l) * D + d] = std::fma(jac, basis->nodes[idx[d]], mesh->element_centers[d + D * e]);
+    }
+  }
   for (size_t k = 0; k < nbr.size(); ++k)
     if (nbr[k] == kUnset) return fail(TRIXI_HIP_EINVAL, "element face without interface or boundary (mortars unsupported)");
   // basis
@@ -521,7 +545,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
             hipMalloc(&ctx->d_partial, ctx->partial_cap * 8) == hipSuccess &&
             hipMalloc(&ctx->d_scalars, 64 * 8) == hipSuccess;
   if (ok && ctx->B > 0)
-    ok = hipMalloc(&ctx->d_bnd, bnd.size() * 4) == hipSuccess &&
+    ok = hipMalloc(&ctx->d_bx, bx.size() * 8) == hipSuccess &&
          hipMalloc(&ctx->d_ghost, (size_t)ctx->B * ipow_c(n, D - 1) * ctx->V * tsz) == hipSuccess;
   if (ok && ctx->fp32)
     ok = hipMalloc(&ctx->d_u32b[0], nv * 4) == hipSuccess && hipMalloc(&ctx->d_u32b[1], nv * 4) == hipSuccess &&
@@ -539,7 +563,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
        hipMemcpy(ctx->d_alpha_raw, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(ctx->d_scalars + 32, ctx->weights.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       (ctx->B == 0 || hipMemcpy(ctx->d_bnd, bnd.data(), bnd.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
+       (ctx->B == 0 || hipMemcpy(ctx->d_bx, bx.data(), bx.size() * 8, hipMemcpyHostToDevice) == hipSuccess);
   if (!ok) {
     trixi_hip_destroy(ctx);
     fprintf(stderr, "trixi_hip_create: upload failed\n");
@@ -557,7 +581,7 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
                   ctx->d_u32b[0], ctx->d_u32b[1], ctx->d_du,
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
                   ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
-                  ctx->d_alpha_hg, ctx->d_records, ctx->d_bnd, ctx->d_ghost};
+                  ctx->d_alpha_hg, ctx->d_records, ctx->d_bx, ctx->d_ghost};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -704,7 +728,7 @@ int launch_max_speed(trixi_hip_ctx *ctx) {
     constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
     constexpr int NN = ipow_c(N + 1, D);
     constexpr int EB = NN >= 256 ? 1 : 256 / NN;
-    nb = (ctx->E + EB - 1) / EB;
+    nb = std::min((ctx->E + EB - 1) / EB, kMaxSpeedBlocks);  // beyond that, several groups per block
     if ((size_t)nb > ctx->partial_cap) {
       nb = -1;
       return;
@@ -828,7 +852,7 @@ int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt) {
   if (!ctx || !dt) return TRIXI_HIP_EINVAL;
   const int nb = launch_max_speed(ctx);
   if (nb < 0) return nb;
-  hipLaunchKernelGGL(max_dt_finalize_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, nb, ctx->n,
+  hipLaunchKernelGGL(max_dt_finalize_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream, ctx->d_partial, nb, ctx->n,
                      cfl, ctx->d_scalars + 2);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dt, ctx->d_scalars + 2, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -877,7 +901,7 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
     }
     const int nb = launch_max_speed(ctx);
     if (nb < 0) return nb;
-    hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, nb, ctx->n, tdt,
+    hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream, ctx->d_partial, nb, ctx->n, tdt,
                        ctl);
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
@@ -1098,6 +1122,7 @@ int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *
   // captured steps hold the boundary-state kernel with the old function
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);
+  ctx->ghost_ready = false;
   ctx->exact.kind = ic_kind;
   ctx->exact.gamma = ctx->cfg.gamma;
   for (int i = 0; i < 4; ++i) ctx->exact.params[i] = (params && i < nparams) ? params[i] : 0.0;
