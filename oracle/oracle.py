@@ -21,7 +21,9 @@ class _Config(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("volume_integral", ctypes.c_int), ("volume_flux", ctypes.c_int),
                 ("surface_flux", ctypes.c_int), ("fv_flux", ctypes.c_int), ("alpha_max", ctypes.c_double),
                 ("alpha_min", ctypes.c_double), ("alpha_smooth", ctypes.c_int), ("source_terms", ctypes.c_int),
-                ("periodicity", ctypes.c_int * 3), ("boundary_condition", ctypes.c_int)]
+                ("periodicity", ctypes.c_int * 3), ("boundary_condition", ctypes.c_int),
+                ("npatches", ctypes.c_int), ("patch_min", (ctypes.c_double * 3) * 8),
+                ("patch_max", (ctypes.c_double * 3) * 8)]
 
 
 def build():
@@ -41,13 +43,15 @@ def _load():
         L.oracle_create.restype = P
         L.oracle_create.argtypes = [ctypes.POINTER(_Config)]
         L.oracle_destroy.argtypes = [P]
-        for nm in ("oracle_nelements", "oracle_ninterfaces", "oracle_nboundaries", "oracle_nnodes"):
+        for nm in ("oracle_nelements", "oracle_ninterfaces", "oracle_nboundaries", "oracle_nmortars", "oracle_nnodes"):
             getattr(L, nm).restype = I
             getattr(L, nm).argtypes = [P]
         L.oracle_ndofs.restype = ctypes.c_long
         L.oracle_ndofs.argtypes = [P]
         L.oracle_get_mesh.argtypes = [P] + [P] * 6
         L.oracle_get_boundaries.argtypes = [P] + [P] * 3
+        L.oracle_get_mortars.argtypes = [P] + [P] * 3
+        L.oracle_get_mortar_l2.argtypes = [P] + [P] * 4
         L.oracle_get_basis.argtypes = [P] + [P] * 6
         L.oracle_set_initial_condition.argtypes = [P, I, P, I, P]
         L.oracle_compute_coefficients.argtypes = [P, D, P]
@@ -82,13 +86,15 @@ class Oracle:
     """One oracle discretisation: Oracle(ndims=3, polydeg=3, level=2, cmin=..., cmax=...,
     volume_integral='fd', volume_flux='ranocha', surface_flux='llf', fv_flux='llf',
     alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, source='none'|'convergence_test',
-    gamma=1.4, periodicity=True). Non-periodic boundaries carry
+    gamma=1.4, periodicity=True, refinement_patches=()). A refinement patch is a
+    box (cmin, cmax) whose leaf cells are refined once (2:1 balanced, L2
+    mortars at the level changes). Non-periodic boundaries carry
     BoundaryConditionDirichlet(initial_condition) with the initial condition set
     by set_initial_condition."""
 
     def __init__(self, ndims, polydeg, level, cmin, cmax, volume_integral="fd", volume_flux="ranocha",
                  surface_flux="llf", fv_flux="llf", alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
-                 source="none", gamma=1.4, periodicity=True):
+                 source="none", gamma=1.4, periodicity=True, refinement_patches=()):
         L = _load()
         c = _Config()
         c.ndims, c.polydeg, c.initial_refinement_level = ndims, polydeg, level
@@ -106,6 +112,13 @@ class Oracle:
         for d in range(ndims):
             c.periodicity[d] = int(bool(per[d]))
         c.boundary_condition = 0 if all(per) else 1
+        if len(refinement_patches) > 8:
+            raise ValueError("at most 8 refinement patches")
+        c.npatches = len(refinement_patches)
+        for k, (pmin, pmax) in enumerate(refinement_patches):
+            for d in range(ndims):
+                c.patch_min[k][d] = pmin[d]
+                c.patch_max[k][d] = pmax[d]
         self._cfg = c
         self._c = L.oracle_create(ctypes.byref(c))
         if not self._c:
@@ -117,6 +130,7 @@ class Oracle:
         self.nelements = L.oracle_nelements(self._c)
         self.ninterfaces = L.oracle_ninterfaces(self._c)
         self.nboundaries = L.oracle_nboundaries(self._c)
+        self.nmortars = L.oracle_nmortars(self._c)
         self.ndofs = L.oracle_ndofs(self._c)
         self.nvalues = self.ndofs * self.nvars
 
@@ -140,6 +154,21 @@ class Oracle:
         return dict(neighbor_ids=nid.reshape(I, 2), orientations=ori, inverse_jacobian=ij,
                     centers=cen.reshape(E, D), node_coordinates=xc.reshape(E, NN, D),
                     element_neighbors=en.reshape(E, 2 * D))
+
+    def mortars(self):
+        """(neighbor_ids [M, 2^(D-1) + 1], large_sides [M], orientations [M]) of the
+        mortars container, 0-based ids/orientations."""
+        M, ns = self.nmortars, 1 << (self.ndims - 1)
+        ids, side, ori = np.empty(M * (ns + 1), np.int32), np.empty(M, np.int32), np.empty(M, np.int32)
+        self.L.oracle_get_mortars(self._c, _p(ids), _p(side), _p(ori))
+        return ids.reshape(M, ns + 1), side, ori
+
+    def mortar_l2(self):
+        """forward_upper, forward_lower, reverse_upper, reverse_lower (n x n)."""
+        n = self.n
+        m = [np.empty((n, n)) for _ in range(4)]
+        self.L.oracle_get_mortar_l2(self._c, *[_p(a) for a in m])
+        return m
 
     def boundaries(self):
         """(neighbor_ids, orientations, neighbor_sides) of the boundaries container, 0-based."""

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 #include <algorithm>
 #include <type_traits>
@@ -87,6 +88,72 @@ void gauss_lobatto_nodes_weights(int n_nodes, std::vector<double> &nodes,
         calc_q_and_l(N, 0.0, q, qder, L);
         nodes[N / 2] = 0;
         weights[N / 2] = weights[0] / (L * L);
+    }
+}
+
+// Legendre polynomial and derivative, normalized by sqrt(N + 1/2)
+// (legendre_polynomial_and_derivative, basis_lobatto_legendre.jl:677-708)
+void legendre_and_derivative(int N, double x, double &poly, double &deriv) {
+    if (N == 0) {
+        poly = 1.0;
+        deriv = 0.0;
+    } else if (N == 1) {
+        poly = x;
+        deriv = 1.0;
+    } else {
+        double p2 = 1.0, p1 = x, d2 = 0.0, d1 = 1.0;
+        poly = deriv = 0.0;
+        for (int i = 2; i <= N; ++i) {
+            poly = ((2 * i - 1) * x * p1 - (i - 1) * p2) / i;
+            deriv = d2 + (2 * i - 1) * p1;
+            p2 = p1;
+            p1 = poly;
+            d2 = d1;
+            d1 = deriv;
+        }
+    }
+    poly *= std::sqrt(N + 0.5);
+    deriv *= std::sqrt(N + 0.5);
+}
+
+// Legendre-Gauss nodes and weights by Newton's method from the Chebyshev-like
+// starting guesses, using the symmetry of the roots (gauss_nodes_weights,
+// basis_gauss_legendre.jl:149-205).
+void gauss_nodes_weights(int n_nodes, std::vector<double> &nodes, std::vector<double> &weights) {
+    const int n_iterations = 20;
+    const double tolerance = 2 * 2.220446049250313e-16;
+    nodes.assign(n_nodes, 1.0);
+    weights.assign(n_nodes, 0.0);
+    const int N = n_nodes - 1;
+    if (N == 0) {
+        nodes[0] = 0;
+        weights[0] = 2;
+        return;
+    }
+    if (N == 1) {
+        nodes[0] = -std::sqrt(1.0 / 3);
+        nodes[1] = -nodes[0];
+        weights[0] = weights[1] = 1;
+        return;
+    }
+    double poly, deriv;
+    for (int i = 0; i <= (N + 1) / 2 - 1; ++i) {
+        nodes[i] = -std::cos(M_PI / (2 * N + 2) * (2 * i + 1));
+        for (int k = 0; k <= n_iterations; ++k) {
+            legendre_and_derivative(N + 1, nodes[i], poly, deriv);
+            const double dx = -poly / deriv;
+            nodes[i] += dx;
+            if (std::fabs(dx) < tolerance * std::fabs(nodes[i])) break;
+        }
+        legendre_and_derivative(N + 1, nodes[i], poly, deriv);
+        weights[i] = (2 * N + 3) / ((1 - nodes[i] * nodes[i]) * deriv * deriv);
+        nodes[N - i] = -nodes[i];
+        weights[N - i] = weights[i];
+    }
+    if (n_nodes % 2 == 1) {
+        legendre_and_derivative(N + 1, 0.0, poly, deriv);
+        nodes[N / 2] = 0;
+        weights[N / 2] = (2 * N + 3) / (deriv * deriv);
     }
 }
 
@@ -227,6 +294,41 @@ struct Basis {
         na = 2 * polydeg + 1;
         gauss_lobatto_nodes_weights(na, a_nodes, a_weights);
         a_vdm = interpolation_matrix(nodes, a_nodes);
+        init_mortar_l2();
+    }
+
+    // LobattoLegendreMortarL2 (basis_lobatto_legendre.jl:176-205, l2projection.jl:26-116):
+    // forward_{upper,lower}[j, i] = l_i(0.5 (xi_j +- 1)) (interpolation to the
+    // halves); reverse_{upper,lower} = gauss2lobatto * op * lobatto2gauss with
+    // op[i, j] = 0.5 lg_i(0.5 (g_j +- 1)) wg_j / wg_i on the Gauss nodes g.
+    std::vector<double> fwd_upper, fwd_lower, rev_upper, rev_lower;  // row-major n x n
+    void init_mortar_l2() {
+        std::vector<double> up(n), lo(n);
+        for (int j = 0; j < n; ++j) {
+            up[j] = 0.5 * (nodes[j] + 1);
+            lo[j] = 0.5 * (nodes[j] - 1);
+        }
+        fwd_upper = interpolation_matrix(nodes, up);
+        fwd_lower = interpolation_matrix(nodes, lo);
+        std::vector<double> g, gw;
+        gauss_nodes_weights(n, g, gw);
+        auto g2l = interpolation_matrix(g, nodes);  // [lobatto k][gauss j]
+        auto l2g = interpolation_matrix(nodes, g);  // [gauss k][lobatto j]
+        for (int side = 0; side < 2; ++side) {
+            std::vector<double> pts(n), op(n * n);
+            for (int j = 0; j < n; ++j) pts[j] = 0.5 * (g[j] + (side == 0 ? 1.0 : -1.0));
+            auto P = interpolation_matrix(g, pts);  // P[j * n + i] = lg_i(pts_j)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) op[i * n + j] = 0.5 * P[j * n + i] * gw[j] / gw[i];
+            std::vector<double> t(n * n, 0.0), r(n * n, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j)
+                    for (int k = 0; k < n; ++k) t[i * n + j] += op[i * n + k] * l2g[k * n + j];
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j)
+                    for (int k = 0; k < n; ++k) r[i * n + j] += g2l[i * n + k] * t[k * n + j];
+            (side == 0 ? rev_upper : rev_lower) = r;
+        }
     }
 };
 
@@ -447,6 +549,8 @@ struct oracle_config {
     int source_terms;     // SRC_*
     int periodicity[3];   // TreeMesh(...; periodicity) per dimension (nonzero = periodic)
     int boundary_condition;  // BC_* for every non-periodic boundary
+    int npatches;         // refinement_patches of type "box", applied in order
+    double patch_min[8][3], patch_max[8][3];
 };
 
 struct oracle_ctx;
@@ -466,6 +570,13 @@ struct oracle_ctx {
     std::vector<int32_t> orientations;  // [I] 0-based orientation
     std::vector<int32_t> elem_nbr;      // [2D, E] neighbour element per direction (0-based dir), -1 at a boundary
     std::vector<int32_t> bnd_elem, bnd_dir;  // boundaries: element and its 0-based direction (init_boundaries!)
+    // L2 mortars (init_mortars!, containers_3d.jl:652-788): small elements in
+    // positions q = (first free dim +) + 2 (second free dim +), then the large one
+    std::vector<int32_t> mortar_ids;      // [M][2^(D-1) + 1]
+    std::vector<int32_t> mortar_orient;   // [M] 0-based orientation
+    std::vector<int32_t> mortar_side;     // [M] large_sides: 1 = large element on the left
+    int M = 0;
+    std::vector<int32_t> levels;          // [E] tree level of each element
     std::vector<double> sfv;            // surface_flux_values
     std::vector<double> alpha, alpha_tmp;
     // analysis exact-solution parameters
@@ -830,6 +941,16 @@ void hg_indicator(oracle_ctx &c, const Euler<D> &eq, const double *u) {
             c.alpha[l] = std::max(std::max(c.alpha_tmp[l], 0.5 * c.alpha_tmp[r]), c.alpha[l]);
             c.alpha[r] = std::max(std::max(c.alpha_tmp[r], 0.5 * c.alpha_tmp[l]), c.alpha[r]);
         }
+        // L2 mortars: every small element and the large one (indicators_3d.jl:145-171)
+        const int ns = 1 << (D - 1);
+        for (int m = 0; m < c.M; ++m) {
+            const int32_t *ids = &c.mortar_ids[(size_t)m * (ns + 1)];
+            const int large = ids[ns];
+            for (int q = 0; q < ns; ++q)
+                c.alpha[ids[q]] = std::max(std::max(c.alpha_tmp[ids[q]], 0.5 * c.alpha_tmp[large]), c.alpha[ids[q]]);
+            for (int q = 0; q < ns; ++q)
+                c.alpha[large] = std::max(std::max(c.alpha_tmp[large], 0.5 * c.alpha_tmp[ids[q]]), c.alpha[large]);
+        }
     }
 }
 
@@ -916,6 +1037,89 @@ template <int D, typename R> void rhs(oracle_ctx &c, R *du, const R *u, double t
                 else num_flux<D>(eq, c.cfg.surface_flux, ubr, ui, o, f);
                 for (int v = 0; v < V; ++v) sfv[V * (fn + nf * (dir + 2 * D * e)) + v] = f[v];
             }
+        }
+    }
+
+    // prolong2mortars! + calc_mortar_flux! (dg_3d.jl:770-1335, dg_2d.jl:897-1243):
+    // the large element's face trace is interpolated to each small face
+    // (forward_{lower,upper} per free dimension, first index first), the
+    // numerical flux is evaluated there (the large element is the left state
+    // when large_sides == 1) and given to the small element, and the large
+    // element receives the reverse L2 projection summed over the small faces.
+    {
+        const int ns = 1 << (D - 1);
+        const int n1 = D == 3 ? n : 1;  // extent of the second free index
+        const Basis &b = c.basis;
+#pragma omp parallel for schedule(static)
+        for (int m = 0; m < c.M; ++m) {
+            const int32_t *ids = &c.mortar_ids[(size_t)m * (ns + 1)];
+            const int large = ids[ns], o = c.mortar_orient[m];
+            const bool large_left = c.mortar_side[m] == 1;
+            const int large_dir = large_left ? 2 * o + 1 : 2 * o, small_dir = large_left ? 2 * o : 2 * o + 1;
+            std::vector<R> ul((size_t)nf * V), tmp((size_t)nf * V), up((size_t)nf * V), fs((size_t)ns * nf * V),
+                fl((size_t)nf * V, R(0));
+            for (int fn = 0; fn < nf; ++fn) {
+                const int node = face_to_node<D>(n, o, large_left ? n - 1 : 0, fn);
+                for (int v = 0; v < V; ++v) ul[V * fn + v] = u[V * (node + NN * large) + v];
+            }
+            for (int q = 0; q < ns; ++q) {
+                const std::vector<double> &F0 = (q & 1) ? b.fwd_upper : b.fwd_lower;
+                const std::vector<double> &F1 = (q >> 1) ? b.fwd_upper : b.fwd_lower;
+                for (int j = 0; j < n1; ++j)  // first free index
+                    for (int i = 0; i < n; ++i)
+                        for (int v = 0; v < V; ++v) {
+                            R acc = 0;
+                            for (int a = 0; a < n; ++a) acc += R(F0[i * n + a]) * ul[V * (a + n * j) + v];
+                            tmp[V * (i + n * j) + v] = acc;
+                        }
+                for (int j = 0; j < n1; ++j)  // second free index (3D)
+                    for (int i = 0; i < n; ++i)
+                        for (int v = 0; v < V; ++v) {
+                            R acc = 0;
+                            if (D == 3)
+                                for (int a = 0; a < n; ++a) acc += R(F1[j * n + a]) * tmp[V * (i + n * a) + v];
+                            else
+                                acc = tmp[V * (i + n * j) + v];
+                            up[V * (i + n * j) + v] = acc;
+                        }
+                const int small = ids[q];
+                for (int fn = 0; fn < nf; ++fn) {
+                    const int node = face_to_node<D>(n, o, large_left ? 0 : n - 1, fn);
+                    const R *us = &u[V * (node + NN * small)];
+                    R *f = &fs[(size_t)V * (fn + nf * q)];
+                    if (large_left) num_flux<D>(eq, c.cfg.surface_flux, &up[V * fn], us, o, f);
+                    else num_flux<D>(eq, c.cfg.surface_flux, us, &up[V * fn], o, f);
+                    for (int v = 0; v < V; ++v) sfv[V * (fn + nf * (small_dir + 2 * D * small)) + v] = f[v];
+                }
+            }
+            // mortar_fluxes_to_elements!: reverse projections in the order
+            // upper_left, upper_right, lower_left, lower_right (3D), upper, lower (2D)
+            const int order3[4] = {2, 3, 0, 1}, order2[2] = {1, 0};
+            for (int k = 0; k < ns; ++k) {
+                const int q = D == 3 ? order3[k] : order2[k];
+                const std::vector<double> &R0 = (q & 1) ? b.rev_upper : b.rev_lower;
+                const std::vector<double> &R1 = (q >> 1) ? b.rev_upper : b.rev_lower;
+                const R *fq = &fs[(size_t)V * nf * q];
+                for (int j = 0; j < n1; ++j)
+                    for (int i = 0; i < n; ++i)
+                        for (int v = 0; v < V; ++v) {
+                            R acc = 0;
+                            for (int a = 0; a < n; ++a) acc += R(R0[i * n + a]) * fq[V * (a + n * j) + v];
+                            tmp[V * (i + n * j) + v] = acc;
+                        }
+                for (int j = 0; j < n1; ++j)
+                    for (int i = 0; i < n; ++i)
+                        for (int v = 0; v < V; ++v) {
+                            R acc = 0;
+                            if (D == 3)
+                                for (int a = 0; a < n; ++a) acc += R(R1[j * n + a]) * tmp[V * (i + n * a) + v];
+                            else
+                                acc = tmp[V * (i + n * j) + v];
+                            fl[V * (i + n * j) + v] += acc;
+                        }
+            }
+            for (int fn = 0; fn < nf; ++fn)
+                for (int v = 0; v < V; ++v) sfv[V * (fn + nf * (large_dir + 2 * D * large)) + v] = fl[V * fn + v];
         }
     }
 
@@ -1051,46 +1255,186 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
     c->nf = ipow(c->n, c->D - 1);
     c->V = c->D + 2;
     c->basis.init(cfg->polydeg);
-    // TreeMesh(coordinates_min, coordinates_max; initial_refinement_level, periodicity=true)
-    // tree_mesh.jl:128-185
+    // TreeMesh(coordinates_min, coordinates_max; initial_refinement_level,
+    // refinement_patches, periodicity), tree_mesh.jl:128-215: uniform refinement,
+    // then every box patch refines the leaf cells whose centre lies strictly
+    // inside the box (refine_box!, abstract_tree.jl:405-418), rebalancing to
+    // 2:1 (refine!, rebalance!, :356-490). Children are inserted right behind
+    // their parent, so the leaves stay in depth-first order (:317-353).
     double center[3];
     for (int d = 0; d < c->D; ++d) center[d] = (cfg->coordinates_min[d] + cfg->coordinates_max[d]) / 2;
     double length = cfg->coordinates_max[0] - cfg->coordinates_min[0];
     c->domain_length = length;
-    std::vector<int> icoords;
-    int ic0[3] = {0, 0, 0};
-    build_leaves(c->D, 0, cfg->initial_refinement_level, center, length, ic0, c->centers, c->lengths, icoords);
-    c->E = (int)c->lengths.size();
-    const int L = cfg->initial_refinement_level, nside = 1 << L;
-    // integer coordinate -> element id
-    std::vector<int> lookup(ipow(nside, c->D), -1);
-    auto lin = [&](const int *ic) {
-        int r = 0, s = 1;
-        for (int d = 0; d < c->D; ++d) {
-            r += ic[d] * s;
-            s *= nside;
-        }
-        return r;
+    const int D = c->D;
+    struct Cell {
+        int level, ic[3];
+        double center[3];
     };
-    for (int e = 0; e < c->E; ++e) lookup[lin(&icoords[c->D * e])] = e;
-    // neighbours (init_neighbors!, abstract_tree.jl:252-313): across a periodic
-    // boundary the cell on the opposite side (the level-0 cell is its own
-    // neighbour, serial_tree.jl:119-123); none (-1) across a non-periodic one
-    c->elem_nbr.assign(2 * c->D * c->E, -1);
-    for (int e = 0; e < c->E; ++e)
-        for (int dir = 0; dir < 2 * c->D; ++dir) {
-            int ic[3] = {0, 0, 0};
-            for (int d = 0; d < c->D; ++d) ic[d] = icoords[c->D * e + d];
-            int o = dir / 2;
-            int k = ic[o] + (dir % 2 ? 1 : -1);
-            if ((k < 0 || k >= nside) && !cfg->periodicity[o]) continue;
-            ic[o] = (k + nside) % nside;
-            c->elem_nbr[dir + 2 * c->D * e] = lookup[lin(ic)];
+    std::vector<Cell> leaves;
+    {
+        std::vector<double> cen, len;
+        std::vector<int> icoords;
+        int ic0[3] = {0, 0, 0};
+        build_leaves(D, 0, cfg->initial_refinement_level, center, length, ic0, cen, len, icoords);
+        for (size_t e = 0; e < len.size(); ++e) {
+            Cell cl{cfg->initial_refinement_level, {0, 0, 0}, {0, 0, 0}};
+            for (int d = 0; d < D; ++d) {
+                cl.ic[d] = icoords[D * e + d];
+                cl.center[d] = cen[D * e + d];
+            }
+            leaves.push_back(cl);
+        }
+    }
+    auto key = [&](int level, const int *ic) -> long long {
+        long long k = level;
+        for (int d = 0; d < D; ++d) k |= (long long)ic[d] << (6 + 19 * d);
+        return k;
+    };
+    std::unordered_map<long long, int> cells;  // leaf -> element index, inner cell -> -1
+    auto rebuild = [&]() {
+        cells.clear();
+        for (size_t i = 0; i < leaves.size(); ++i) {
+            const Cell &cl = leaves[i];
+            cells[key(cl.level, cl.ic)] = (int)i;
+            int ic[3] = {cl.ic[0], cl.ic[1], cl.ic[2]};
+            for (int l = cl.level - 1; l >= 0; --l) {
+                for (int d = 0; d < D; ++d) ic[d] >>= 1;
+                if (!cells.emplace(key(l, ic), -1).second) break;
+            }
+        }
+    };
+    auto find = [&](int level, const int *ic) -> const int * {
+        auto it = cells.find(key(level, ic));
+        return it == cells.end() ? nullptr : &it->second;
+    };
+    // same-level neighbour coordinates across face dir, wrapping periodic dimensions
+    auto neighbour = [&](int level, const int *ic, int dir, int *out) -> bool {
+        const int o = dir / 2, nside = 1 << level;
+        for (int d = 0; d < D; ++d) out[d] = ic[d];
+        int k = ic[o] + (dir % 2 ? 1 : -1);
+        if ((k < 0 || k >= nside) && !cfg->periodicity[o]) return false;
+        out[o] = (k + nside) % nside;
+        return true;
+    };
+    auto refine = [&](const std::vector<char> &mark) {
+        std::vector<Cell> out;
+        for (size_t i = 0; i < leaves.size(); ++i) {
+            const Cell &p = leaves[i];
+            if (!mark[i]) {
+                out.push_back(p);
+                continue;
+            }
+            const double child_length = length / (1 << p.level) / 2;
+            for (int child = 0; child < (1 << D); ++child) {
+                Cell ch{p.level + 1, {0, 0, 0}, {0, 0, 0}};
+                for (int d = 0; d < D; ++d) {
+                    const int bit = (child >> d) & 1;  // child_sign, abstract_tree.jl:118-128
+                    ch.center[d] = p.center[d] + (bit ? 1 : -1) * child_length / 2;
+                    ch.ic[d] = 2 * p.ic[d] + bit;
+                }
+                out.push_back(ch);
+            }
+        }
+        leaves.swap(out);
+        rebuild();
+    };
+    rebuild();
+    for (int pt = 0; pt < cfg->npatches; ++pt) {
+        std::vector<char> mark(leaves.size(), 0);
+        std::vector<Cell> refined;
+        for (size_t i = 0; i < leaves.size(); ++i) {
+            bool inside = true;
+            for (int d = 0; d < D; ++d)
+                inside = inside && cfg->patch_min[pt][d] < leaves[i].center[d] && cfg->patch_max[pt][d] > leaves[i].center[d];
+            if (inside) {
+                mark[i] = 1;
+                refined.push_back(leaves[i]);
+            }
+        }
+        refine(mark);
+        while (!refined.empty()) {  // rebalance!: coarse neighbours of refined cells
+            std::vector<long long> want;
+            for (const Cell &p : refined)
+                for (int dir = 0; dir < 2 * D; ++dir) {
+                    int nb[3];
+                    if (neighbour(p.level, p.ic, dir, nb) && find(p.level, nb)) continue;  // has_neighbor
+                    if (p.level == 0) continue;
+                    int par[3], pnb[3];
+                    for (int d = 0; d < D; ++d) par[d] = p.ic[d] >> 1;
+                    if (!neighbour(p.level - 1, par, dir, pnb) || !find(p.level - 1, pnb)) continue;
+                    want.push_back(key(p.level - 1, pnb));
+                }
+            std::vector<char> m2(leaves.size(), 0);
+            refined.clear();
+            for (size_t i = 0; i < leaves.size(); ++i)
+                for (long long k : want)
+                    if (k == key(leaves[i].level, leaves[i].ic) && !m2[i]) {
+                        m2[i] = 1;
+                        refined.push_back(leaves[i]);
+                    }
+            if (!refined.empty()) refine(m2);
+        }
+    }
+    c->E = (int)leaves.size();
+    c->centers.clear();
+    c->lengths.clear();
+    c->levels.clear();
+    for (const Cell &cl : leaves) {
+        for (int d = 0; d < D; ++d) c->centers.push_back(cl.center[d]);
+        c->lengths.push_back(length / (1 << cl.level));
+        c->levels.push_back(cl.level);
+    }
+    // neighbours, mortars, boundaries (init_neighbors!, init_interfaces!,
+    // init_mortars!, init_boundaries!): a same-level leaf across a face is an
+    // interface neighbour; a refined same-level cell makes a mortar with this
+    // element as the large side; no same-level cell means this element is a
+    // small element of the coarser neighbour's mortar; outside a non-periodic
+    // domain the face is a boundary.
+    c->elem_nbr.assign(2 * D * c->E, -1);
+    std::vector<char> is_boundary(2 * D * c->E, 0);
+    const int ns = 1 << (D - 1);
+    for (int e = 0; e < c->E; ++e) {
+        const Cell &cl = leaves[e];
+        for (int dir = 0; dir < 2 * D; ++dir) {
+            int nb[3];
+            if (!neighbour(cl.level, cl.ic, dir, nb)) {
+                is_boundary[dir + 2 * D * e] = 1;
+                continue;
+            }
+            const int *f = find(cl.level, nb);
+            if (!f) continue;  // small side of a mortar
+            if (*f >= 0) {
+                c->elem_nbr[dir + 2 * D * e] = *f;
+                continue;
+            }
+            const int o = dir / 2, plus = dir % 2;
+            int fd[2] = {0, 0}, r = 0;
+            for (int d = 0; d < D; ++d)
+                if (d != o) fd[r++] = d;
+            for (int q = 0; q < ns; ++q) {
+                int ch[3];
+                for (int d = 0; d < D; ++d) ch[d] = 2 * nb[d];
+                ch[o] += plus ? 0 : 1;
+                ch[fd[0]] += q & 1;
+                if (D == 3) ch[fd[1]] += q >> 1;
+                const int *s = find(cl.level + 1, ch);
+                c->mortar_ids.push_back(s ? *s : -1);
+            }
+            c->mortar_ids.push_back(e);
+            c->mortar_orient.push_back(o);
+            c->mortar_side.push_back(plus ? 1 : 2);
+        }
+    }
+    c->M = (int)c->mortar_orient.size();
+    for (int32_t id : c->mortar_ids)
+        if (id < 0) {  // a mortar partner that is not a leaf: the tree is not 2:1 balanced
+            delete c;
+            return nullptr;
         }
     // init_boundaries!: containers_2d.jl:475-543 (direction-major, element order)
-    for (int dir = 0; dir < 2 * c->D; ++dir)
+    for (int dir = 0; dir < 2 * D; ++dir)
         for (int e = 0; e < c->E; ++e)
-            if (c->elem_nbr[dir + 2 * c->D * e] < 0) {
+            if (is_boundary[dir + 2 * D * e]) {
                 c->bnd_elem.push_back(e);
                 c->bnd_dir.push_back(dir);
             }
@@ -1136,6 +1480,21 @@ void oracle_destroy(oracle_ctx *c) { delete c; }
 int oracle_nelements(const oracle_ctx *c) { return c->E; }
 int oracle_ninterfaces(const oracle_ctx *c) { return c->I; }
 int oracle_nboundaries(const oracle_ctx *c) { return (int)c->bnd_elem.size(); }
+int oracle_nmortars(const oracle_ctx *c) { return c->M; }
+// mortars container (0-based): neighbor_ids [M][2^(D-1) + 1], large_sides, orientations
+void oracle_get_mortars(const oracle_ctx *c, int32_t *ids, int32_t *large_sides, int32_t *orientations) {
+    if (ids) std::copy(c->mortar_ids.begin(), c->mortar_ids.end(), ids);
+    if (large_sides) std::copy(c->mortar_side.begin(), c->mortar_side.end(), large_sides);
+    if (orientations) std::copy(c->mortar_orient.begin(), c->mortar_orient.end(), orientations);
+}
+// L2 mortar matrices, row-major n x n each: forward_upper, forward_lower, reverse_upper, reverse_lower
+void oracle_get_mortar_l2(const oracle_ctx *c, double *fu, double *fl, double *ru, double *rl) {
+    const Basis &b = c->basis;
+    std::copy(b.fwd_upper.begin(), b.fwd_upper.end(), fu);
+    std::copy(b.fwd_lower.begin(), b.fwd_lower.end(), fl);
+    std::copy(b.rev_upper.begin(), b.rev_upper.end(), ru);
+    std::copy(b.rev_lower.begin(), b.rev_lower.end(), rl);
+}
 // boundaries container (0-based): element, orientation, neighbor_side (1: element
 // left of the boundary, 2: right), as init_boundaries! fills them
 void oracle_get_boundaries(const oracle_ctx *c, int32_t *elem, int32_t *orientation, int32_t *side) {

@@ -140,3 +140,22 @@ def test_entropy_gradient_is_entropy_variables(D):
             um[k] -= h
             grad[k] = (S(up) - S(um)) / (2 * h)
         np.testing.assert_allclose(grad, cons2entropy(D, u), rtol=1e-7, atol=1e-8)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5])
+def test_mortar_l2_matrices(N):
+    """LobattoLegendreMortarL2 (l2projection.jl): the forward matrices
+    interpolate polynomials of degree N exactly onto the two halves, and the
+    reverse (L2) projections undo the split: R_u F_u + R_l F_l = I. A reverse
+    matrix built from the wrong half or transposed fails the second check."""
+    from oracle import Oracle
+    o = Oracle(2, N, 1, (0, 0), (2, 2))
+    fu, fl, ru, rl = o.mortar_l2()
+    b = o.basis()
+    x = b["nodes"]
+    for k in range(N + 1):
+        p = x ** k
+        np.testing.assert_allclose(fu @ p, (0.5 * (x + 1)) ** k, atol=1e-13)
+        np.testing.assert_allclose(fl @ p, (0.5 * (x - 1)) ** k, atol=1e-13)
+    np.testing.assert_allclose(ru @ fu + rl @ fl, np.eye(N + 1), atol=1e-13)
+    assert np.abs(ru.T @ fu + rl.T @ fl - np.eye(N + 1)).max() > 1e-3 or N == 1
