@@ -108,6 +108,7 @@ typedef struct trixi_hip_config {
   int32_t threads_per_block;  /* 0 = autotuned default */
   int32_t nboundaries;        /* B: boundary faces of a non-periodic mesh (0 if periodic) */
   int32_t boundary_condition; /* TRIXI_HIP_BC_*; must be set when nboundaries > 0 */
+  int32_t nmortars;           /* M: L2 mortars of a refined (non-conforming) mesh, 0 if uniform */
 } trixi_hip_config;
 
 /* Mesh data as held in the reference cache (containers_{2,3}d.jl). */
@@ -122,6 +123,13 @@ typedef struct trixi_hip_mesh {
   const int32_t *boundary_orientations;   /* [B] boundaries.orientations, 0-based */
   const int32_t *boundary_neighbor_sides; /* [B] boundaries.neighbor_sides: 1 = element left of
                                              the boundary (its + face), 2 = right (its - face) */
+  /* L2 mortars container (containers_3d.jl:652-788, containers_2d.jl:700-800), M
+   * entries; neighbor_ids[k + (S + 1) m], S = 2^(ndims-1) small elements (3D: lower
+   * left, lower right, upper left, upper right = first free dimension -/+, second
+   * free dimension -/+; 2D: lower, upper) then the large element, 0-based */
+  const int32_t *mortar_neighbor_ids;  /* [S + 1, M] */
+  const int32_t *mortar_large_sides;   /* [M] 1 = large element left of the mortar, 2 = right */
+  const int32_t *mortar_orientations;  /* [M] 0-based orientation */
 } trixi_hip_mesh;
 
 /* LobattoLegendreBasis fields (basis_lobatto_legendre.jl:17-86), row-major
@@ -136,6 +144,12 @@ typedef struct trixi_hip_basis {
   int32_t analysis_nnodes;                    /* na (2N + 1) */
   const double *analysis_weights;             /* [na] */
   const double *analysis_vandermonde;         /* [na, n] interpolation matrix */
+  /* LobattoLegendreMortarL2 (basis_lobatto_legendre.jl:176-205, l2projection.jl),
+   * [n, n] each; may be NULL when the mesh has no mortars */
+  const double *mortar_forward_upper;
+  const double *mortar_forward_lower;
+  const double *mortar_reverse_upper;
+  const double *mortar_reverse_lower;
 } trixi_hip_basis;
 
 typedef struct trixi_hip_ctx trixi_hip_ctx;

@@ -49,7 +49,8 @@ def build(spec, precision="fp64"):
         VI = th.VolumeIntegralPureLGLFiniteVolume(FLUX[fvflux])
     solver = th.DGSEM(basis, FLUX[spec.get("surface_flux", "llf")], VI)
     mesh = th.TreeMesh(spec["cmin"], spec["cmax"], initial_refinement_level=spec["level"],
-                       periodicity=spec.get("periodicity", True))
+                       periodicity=spec.get("periodicity", True),
+                       refinement_patches=spec.get("refinement_patches", ()))
     ic = make_ic(spec["ic"], spec.get("ic_params", ()))
     src = th.source_terms_convergence_test if spec.get("source") == "convergence_test" else None
     bc = th.boundary_condition_periodic if mesh.nboundaries == 0 else th.BoundaryConditionDirichlet(ic)
@@ -64,7 +65,8 @@ def oracle_for(spec):
                volume_integral=spec.get("volume_integral", "fd"), volume_flux=spec.get("volume_flux", "ranocha"),
                surface_flux=spec.get("surface_flux", "llf"), fv_flux=spec.get("fv_flux", "llf"),
                alpha_max=spec.get("alpha_max", 0.5), alpha_min=spec.get("alpha_min", 0.001),
-               source=spec.get("source", "none"), periodicity=spec.get("periodicity", True))
+               source=spec.get("source", "none"), periodicity=spec.get("periodicity", True),
+               refinement_patches=spec.get("refinement_patches", ()))
     if spec.get("periodicity", True) is not True:  # the boundary values come from the initial condition
         ic = make_ic(spec["ic"], spec.get("ic_params", ()))
         o.set_initial_condition(spec["ic"], tuple(ic.params), ic.modes.reshape(-1) if ic.modes is not None else None)

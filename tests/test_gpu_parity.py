@@ -50,6 +50,22 @@ CASES = {
     "2d_p3_hg_khi_bounded": (dict(ndims=2, polydeg=3, level=4, cmin=(-1, -1), cmax=(1, 1), ic="kelvin_helmholtz",
                                   ic_params=(0.0, 0), volume_integral="hg", surface_flux="llf_naive",
                                   fv_flux="llf_naive", alpha_max=0.002, alpha_min=1e-4, periodicity=False), 2e-4),
+    # refined meshes (box refinement patches): L2 mortars at the level changes
+    "3d_p3_weak_mms_mortar": (dict(MMS3, volume_integral="weak", surface_flux="llf_naive",
+                                   refinement_patches=[((0.5, 0.5, 0.5), (1.5, 1.5, 1.5))]), 1e-4),
+    "3d_p3_fd_mms_mortar": (dict(MMS3, volume_integral="fd", surface_flux="llf",
+                                 refinement_patches=[((0.5, 0.5, 0.5), (1.5, 1.5, 1.5))]), 1e-4),
+    "3d_p4_fd_mms_mortar": (dict(ndims=3, polydeg=4, level=1, cmin=(0,) * 3, cmax=(2,) * 3, ic="convergence_test",
+                                 source="convergence_test", refinement_patches=[((0, 0, 0), (1, 1, 1))]), 1e-4),
+    "3d_p2_fd_ec_mortar_bounded": (dict(BLAST, polydeg=2, volume_integral="fd", surface_flux="ranocha",
+                                        periodicity=False, refinement_patches=[((-1, -1, -1), (1, 1, 1))]), 1e-5),
+    "2d_p3_fd_mms_mortar_nested": (dict(ndims=2, polydeg=3, level=3, cmin=(0, 0), cmax=(2, 2), ic="convergence_test",
+                                        source="convergence_test",
+                                        refinement_patches=[((0.6, 0.6), (1.4, 1.4)), ((0.8, 0.8), (1.2, 1.2))]), 2e-4),
+    "2d_p3_hg_khi_mortar": (dict(ndims=2, polydeg=3, level=3, cmin=(-1, -1), cmax=(1, 1), ic="kelvin_helmholtz",
+                                 ic_params=(1e-3, 0), volume_integral="hg", surface_flux="llf_naive",
+                                 fv_flux="llf_naive", alpha_max=0.002, alpha_min=1e-4,
+                                 refinement_patches=[((-1, -0.5), (1, 0.5))]), 2e-4),
 }
 
 
@@ -133,7 +149,7 @@ def test_free_stream_preservation(D, per):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["3d_p3_hg_blast", "2d_p3_hg_khi", "2d_p3_hg_khi_bounded"])
+@pytest.mark.parametrize("name", ["3d_p3_hg_blast", "2d_p3_hg_khi", "2d_p3_hg_khi_bounded", "2d_p3_hg_khi_mortar"])
 def test_hg_indicator_matches_oracle(name):
     spec = CASES[name][0]
     o = oracle_for(spec)

@@ -95,7 +95,8 @@ def run_kta(case, precision):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d",
-                                  "khi_2d", "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic"])
+                                  "khi_2d", "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic",
+                                  "mms_3d_mortar"])
 def test_kta_trajectory_on_gpu_fp64(case):
     """The whole run (RK stages, dt, analysis) on the device reproduces the
     reference's stored L2/Linf errors."""
@@ -207,7 +208,7 @@ def test_device_step_control_equals_host_loop(precision):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("case", ["khi_2d", "mms_2d_nonperiodic", "shockcapturing_3d", "mms_3d_weak"])
+@pytest.mark.parametrize("case", ["khi_2d", "mms_2d_nonperiodic", "shockcapturing_3d", "mms_3d_weak", "mms_3d_mortar"])
 def test_fused_stage_equals_rhs_plus_update(case, precision, monkeypatch):
     """The element-kernel variant that applies the 2N update in its epilogue
     (the state alternates between two buffers) runs bitwise the same steps as

@@ -173,3 +173,38 @@ def test_boundaries_of_a_bounded_mesh():
     with pytest.raises(ValueError):
         th.SemidiscretizationHyperbolic(th.TreeMesh((0, 0), (2, 2), initial_refinement_level=2, periodicity=True),
                                         eq, ic, solver, boundary_conditions=th.BoundaryConditionDirichlet(ic))
+
+
+@pytest.mark.parametrize("D,N,level,cmin,cmax,per,patches", [
+    (3, 3, 2, (0, 0, 0), (2, 2, 2), True, [((0.5, 0.5, 0.5), (1.5, 1.5, 1.5))]),
+    (2, 3, 3, (-1, -1), (1, 1), True, [((-0.3, -0.3), (0.3, 0.3)), ((-0.1, -0.1), (0.1, 0.1))]),
+    (2, 2, 2, (0, 0), (2, 2), False, [((0.0, 0.0), (1.0, 2.0))]),
+    (3, 2, 1, (-1, -1, -1), (1, 1, 1), (True, False, True), [((-1, -1, -1), (0, 0, 0))]),
+])
+def test_refined_mesh_matches_oracle(D, N, level, cmin, cmax, per, patches):
+    """TreeMesh with box refinement patches (refine_box!, rebalance!): the same
+    depth-first elements, geometry, interfaces, L2 mortars and boundaries as the
+    oracle; every element face is covered exactly once."""
+    from oracle import Oracle
+    o = Oracle(D, N, level, cmin, cmax, periodicity=per, refinement_patches=patches)
+    m = o.mesh()
+    mesh = th.TreeMesh(cmin, cmax, initial_refinement_level=level, periodicity=per, refinement_patches=patches)
+    assert (mesh.nelements, mesh.ninterfaces, mesh.nmortars, mesh.nboundaries) == \
+        (o.nelements, o.ninterfaces, o.nmortars, o.nboundaries)
+    assert mesh.nmortars > 0
+    np.testing.assert_array_equal(mesh.cell_centers, m["centers"])
+    np.testing.assert_array_equal(mesh.neighbor_ids, m["neighbor_ids"])
+    np.testing.assert_array_equal(mesh.orientations, m["orientations"])
+    ids, side, ori = o.mortars()
+    np.testing.assert_array_equal(mesh.mortar_neighbor_ids, ids)
+    np.testing.assert_array_equal(mesh.mortar_large_sides, side)
+    np.testing.assert_array_equal(mesh.mortar_orientations, ori)
+    b_el, b_ori, b_side = o.boundaries()
+    np.testing.assert_array_equal(mesh.boundary_neighbor_ids, b_el)
+    ij, xc = element_geometry(mesh, th.LobattoLegendreBasis(N))
+    np.testing.assert_array_equal(ij, m["inverse_jacobian"])
+    np.testing.assert_array_equal(xc, m["node_coordinates"])
+    ns = 1 << (D - 1)
+    faces = 2 * mesh.ninterfaces + mesh.nboundaries + mesh.nmortars * (ns + 1)
+    assert faces == 2 * D * mesh.nelements
+    assert set(np.diff(np.sort(mesh.levels))) <= {0, 1}

@@ -589,4 +589,134 @@ __global__ __launch_bounds__(256) void boundary_state_kernel(const double *__res
   for (int v = 0; v < V; ++v) ghost[(size_t)k * V + v] = T(ub[v]);
 }
 
+// L2 mortars (prolong2mortars! + calc_mortar_flux! + mortar_fluxes_to_elements!,
+// dg_3d.jl:770-1335, dg_2d.jl:897-1243). Per mortar: the large element's face
+// trace is interpolated to each small face (forward matrices, first free index
+// first), the numerical flux is evaluated at the small-face nodes (the large
+// side is the left state when large_sides == 1) and stored for the small
+// element's face, and the large element's face flux is the sum of the reverse
+// L2 projections of the small-face fluxes (3D order: upper left, upper right,
+// lower left, lower right; 2D: upper, lower). One thread per small-face node
+// (S = 2^(D-1) faces of n^(D-1) nodes), MPB mortars per 256-thread block.
+// mort[m] = {small ids [S], large id, orientation, large_side};
+// mflux[(slot * L + node) * V + v], slot = slot0 + m (S + 1) + face (S = large).
+template <typename T, int N> struct MortarConst {
+  T fu[(N + 1) * (N + 1)], fl[(N + 1) * (N + 1)], ru[(N + 1) * (N + 1)], rl[(N + 1) * (N + 1)];
+};
+
+template <typename T, int D, int N>
+__global__ __launch_bounds__(256) void mortar_flux_kernel(const T *__restrict__ u, const int32_t *__restrict__ mort,
+                                                          int M, int slot0, int surface_flux, T gamma,
+                                                          MortarConst<T, N> mc, T *__restrict__ mflux) {
+  constexpr int n = N + 1, V = D + 2, S = 1 << (D - 1), L = D == 3 ? n * n : n, NN = ipow_c(n, D);
+  constexpr int TPM = S * L, MPB = 256 / TPM, W = S + 3;
+  static_assert(MPB >= 1, "one mortar per block at least");
+  __shared__ T s_ul[MPB][L][V];
+  __shared__ T s_fs[MPB][S][L][V];
+  const int tid = threadIdx.x, mm = tid / TPM, r = tid - mm * TPM;
+  const int m = blockIdx.x * MPB + mm;
+  const bool active = mm < MPB && m < M;
+  const int32_t *md = mort + (size_t)(active ? m : 0) * W;
+  const int large = md[S], o = md[S + 1];
+  const bool large_left = md[S + 2] == 1;
+  const int st[3] = {1, n, n * n};
+  int fd0 = o == 0 ? 1 : 0, fd1 = o == 2 ? 1 : 2;  // free dimensions (3D); 2D: fd0 only
+  if (D == 2) fd0 = 1 - o;
+  // face node (i + n j) of an element's face on side idx_o
+  auto node_of = [&](int fn, int idx_o) {
+    const int i = fn % n, j = fn / n;
+    int node = idx_o * st[o] + i * st[fd0];
+    if (D == 3) node += j * st[fd1];
+    return node;
+  };
+  if (active && r < L) {
+    const T *src = u + ((size_t)large * NN + node_of(r, large_left ? n - 1 : 0)) * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) s_ul[mm][r][v] = src[v];
+  }
+  __syncthreads();
+  if (active) {
+    const int q = r / L, fn = r - q * L, i = fn % n, j = fn / n;
+    const T *F0 = (q & 1) ? mc.fu : mc.fl;
+    const T *F1 = (q >> 1) ? mc.fu : mc.fl;
+    T up[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      if constexpr (D == 3) {
+        T acc = 0;
+        for (int b = 0; b < n; ++b) {
+          T t = 0;
+          for (int a2 = 0; a2 < n; ++a2) t += F0[i * n + a2] * s_ul[mm][a2 + n * b][v];
+          acc += F1[j * n + b] * t;
+        }
+        up[v] = acc;
+      } else {
+        T acc = 0;
+        for (int a2 = 0; a2 < n; ++a2) acc += F0[i * n + a2] * s_ul[mm][a2][v];
+        up[v] = acc;
+      }
+    }
+    const T *us = u + ((size_t)md[q] * NN + node_of(fn, large_left ? 0 : n - 1)) * V;
+    T usv[V], f[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) usv[v] = us[v];
+    const T *ul = large_left ? up : usv;
+    const T *ur = large_left ? usv : up;
+    if (o == 0) num_flux_c<T, D, 0>(surface_flux, ul, ur, gamma, f);
+    else if (o == 1) num_flux_c<T, D, 1>(surface_flux, ul, ur, gamma, f);
+    else if constexpr (D == 3) num_flux_c<T, D, 2>(surface_flux, ul, ur, gamma, f);
+    T *dst = mflux + ((size_t)(slot0 + m * (S + 1) + q) * L + fn) * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      dst[v] = f[v];
+      s_fs[mm][q][fn][v] = f[v];
+    }
+  }
+  __syncthreads();
+  if (active && r < L) {
+    const int i = r % n, j = r / n;
+    T fl[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) fl[v] = 0;
+    for (int k = 0; k < S; ++k) {
+      const int q = D == 3 ? (k < 2 ? k + 2 : k - 2) : 1 - k;
+      const T *R0 = (q & 1) ? mc.ru : mc.rl;
+      const T *R1 = (q >> 1) ? mc.ru : mc.rl;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if constexpr (D == 3) {
+          T acc = 0;
+          for (int b = 0; b < n; ++b) {
+            T t = 0;
+            for (int a2 = 0; a2 < n; ++a2) t += R0[i * n + a2] * s_fs[mm][q][a2 + n * b][v];
+            acc += R1[j * n + b] * t;
+          }
+          fl[v] += acc;
+        } else {
+          T acc = 0;
+          for (int a2 = 0; a2 < n; ++a2) acc += R0[i * n + a2] * s_fs[mm][q][a2][v];
+          fl[v] += acc;
+        }
+      }
+    }
+    T *dst = mflux + ((size_t)(slot0 + m * (S + 1) + S) * L + r) * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) dst[v] = fl[v];
+  }
+}
+
+// HG smoothing over an adjacency list (interfaces and mortars; apply_smoothing!,
+// indicators_3d.jl:125-173): alpha_e = max(alpha_raw_e, 0.5 max_nbr alpha_raw_nbr).
+__global__ void hg_smooth_adj_kernel(const double *__restrict__ alpha_raw, const int32_t *__restrict__ adj_ptr,
+                                     const int32_t *__restrict__ adj, double *__restrict__ alpha, int E) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double m = alpha_raw[e];
+  for (int k = adj_ptr[e]; k < adj_ptr[e + 1]; ++k) {
+    const double an = 0.5 * alpha_raw[adj[k]];
+    m = an > m ? an : m;
+  }
+  alpha[e] = m;
+}
+
 }  // namespace trixi_hip

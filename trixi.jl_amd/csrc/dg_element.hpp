@@ -49,7 +49,9 @@ template <typename T> struct ElemArgs {
   const T *__restrict__ u;               // [V, NN, E]
   T *__restrict__ du;                    // [V, NN, E]
   const int32_t *__restrict__ nbr;       // [2D, E] neighbour element per direction; -(b + 1) at boundary b
-  const T *__restrict__ ghost;           // [B, n^(D-1), V] outer states of the boundaries (null if none)
+  const T *__restrict__ ghost;           // [B + F, n^(D-1), V] outer states of the boundaries (slots < nbnd)
+  const T *__restrict__ mflux;           // [B + F, n^(D-1), V] given face fluxes of mortar faces (slots >= nbnd)
+  int nbnd;                              // B: face slots below are boundaries, the others mortar faces
   const double *__restrict__ inv_jac;    // [E]
   const double *__restrict__ centers;    // [D, E]
   const double *__restrict__ alpha_raw;  // [E] HG indicator before smoothing (HG only)
@@ -151,6 +153,7 @@ template <int VOL> struct VolTraits {
 template <typename T, int V> struct LineCtx {
   int el, line, base;
   bool valid;
+  int fm, fp;  // BND variants: offsets of the given fluxes of mortar faces at the -O / +O face, or -1
   T ul_m[V], ur_p[V], own0[V], own1[V];
 };
 
@@ -191,9 +194,13 @@ __device__ __forceinline__ void line_loads(const ElemArgs<T> &a, LineCtx<T, D + 
   const int st = o == 0 ? 1 : (o == 1 ? n : n * n);
   const T *un_m, *un_p;
   if constexpr (BND) {
+    // a face slot (-nb - 1) below nbnd is a boundary (its ghost state), above a
+    // mortar face (its flux is given; the ghost slot only keeps the load valid)
     un_m = nbm >= 0 ? a.u + ((size_t)nbm * NN + c.base + (n - 1) * st) * V
                     : a.ghost + ((size_t)(-nbm - 1) * L + c.line) * V;
     un_p = nbp >= 0 ? a.u + ((size_t)nbp * NN + c.base) * V : a.ghost + ((size_t)(-nbp - 1) * L + c.line) * V;
+    c.fm = (nbm < 0 && -nbm - 1 >= a.nbnd) ? ((-nbm - 1) * L + c.line) * V : -1;
+    c.fp = (nbp < 0 && -nbp - 1 >= a.nbnd) ? ((-nbp - 1) * L + c.line) * V : -1;
   } else {
     un_m = a.u + ((size_t)nbm * NN + c.base + (n - 1) * st) * V;
     un_p = a.u + ((size_t)nbp * NN + c.base) * V;
@@ -221,7 +228,7 @@ __device__ __forceinline__ void line_own(const T *__restrict__ staged, LineCtx<T
 }
 
 // One line of direction O: volume terms, the two surface terms, LDS write.
-template <typename T, int D, int N, int VOL, int EPB, int O>
+template <typename T, int D, int N, int VOL, int EPB, int O, bool BND>
 __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T, N> &k, const LineCtx<T, D + 2> &c,
                                           const T *__restrict__ s_w, const T *__restrict__ s_c,
                                           T *__restrict__ s_d, const double *__restrict__ s_alpha) {
@@ -263,6 +270,15 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
   // ---------------- interface fluxes + surface integral ----------------
   // (evaluated after the volume integral, see the end of this function)
   const T factor = k.inv_w[0];  // inverse_weights[1] of the reference
+  // mortar faces (BND variants): the flux computed by mortar_flux_kernel
+  auto given_flux = [&](int off, T *F) {
+    if constexpr (BND) {
+      if (off >= 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) F[v] = a.mflux[off + v];
+      }
+    }
+  };
   auto faces = [&]() {
     T F[V];
     bool done = false;
@@ -276,10 +292,12 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
         ConsP<T, D> nb;
         consp_from_cons<T, D>(c.ul_m, gm1, nb);  // -O face: element is the right state
         flux_llf_pc<T, D, O>(nb, own_face[0], Prec<T>::sqrt_(gamma * nb.p * nb.irho), own_c[0], naive, F);
+        given_flux(c.fm, F);
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
         consp_from_cons<T, D>(c.ur_p, gm1, nb);  // +O face: element is the left state
         flux_llf_pc<T, D, O>(own_face[1], nb, own_c[1], Prec<T>::sqrt_(gamma * nb.p * nb.irho), naive, F);
+        given_flux(c.fp, F);
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
         done = true;
@@ -287,9 +305,11 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
     }
     if (!done) {
       num_flux_c<T, D, O>(a.surface_flux, c.ul_m, c.own0, gamma, F);  // -O face: element is the right state
+      given_flux(c.fm, F);
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
       num_flux_c<T, D, O>(a.surface_flux, c.own1, c.ur_p, gamma, F);  // +O face: element is the left state
+      given_flux(c.fp, F);
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
     }
@@ -650,9 +670,9 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
   __syncthreads();
 
   // ---- phase B: one direction per wave group, one line per lane ----
-  if (grp == 0) line_work<T, D, N, VOL, EPB, 0>(a, k, lc, s_w, s_c, s_d, s_alpha);
-  else if (grp == 1) line_work<T, D, N, VOL, EPB, 1>(a, k, lc, s_w, s_c, s_d, s_alpha);
-  else if constexpr (D == 3) line_work<T, D, N, VOL, EPB, 2>(a, k, lc, s_w, s_c, s_d, s_alpha);
+  if (grp == 0) line_work<T, D, N, VOL, EPB, 0, BND>(a, k, lc, s_w, s_c, s_d, s_alpha);
+  else if (grp == 1) line_work<T, D, N, VOL, EPB, 1, BND>(a, k, lc, s_w, s_c, s_d, s_alpha);
+  else if constexpr (D == 3) line_work<T, D, N, VOL, EPB, 2, BND>(a, k, lc, s_w, s_c, s_d, s_alpha);
   __syncthreads();
 
   // ---- phase C: sum directions, Jacobian, sources; stage the output ----

@@ -84,6 +84,13 @@ struct trixi_hip_ctx {
   int32_t *d_nbr = nullptr;
   int B = 0;                     // boundaries (non-periodic mesh)
   double *d_bx = nullptr;        // [B, n^(D-1), D] coordinates of the boundary face nodes
+  // L2 mortars (refined meshes): M mortars, each with S = 2^(D-1) small faces
+  // and one large face; face slots B + m (S + 1) + face in ghost / mflux
+  int M = 0;
+  int32_t *d_mort = nullptr;     // [M][S + 3] {small ids, large id, orientation, large_side}
+  void *d_mflux = nullptr;       // [B + M (S + 1), n^(D-1), V] given fluxes of the mortar faces
+  std::vector<double> mfu, mfl, mru, mrl;  // forward/reverse upper/lower, row-major n x n
+  int32_t *d_adj_ptr = nullptr, *d_adj = nullptr;  // element adjacency (interfaces + mortars) for HG smoothing
   bool ghost_ready = false;      // ghost holds the states of a time-independent boundary function
   void *d_ghost = nullptr;       // [B, n^(D-1), V] outer boundary states, native precision
   double *d_invj = nullptr, *d_centers = nullptr;
@@ -197,13 +204,13 @@ int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
         nt = t.nt;
       }
   }
-  auto it = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B > 0 ? 1 : 0, 0});
+  auto it = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B + ctx->M > 0 ? 1 : 0, 0});
   if (it == elem_registry().end())
     return set_err(ctx, TRIXI_HIP_EINVAL,
                    "no element kernel instantiated for this (precision, ndims, polydeg, volume "
                    "integral, tile, boundaries)");
   ctx->elem = it->second;
-  auto st = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B > 0 ? 1 : 0, 1});
+  auto st = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B + ctx->M > 0 ? 1 : 0, 1});
   const char *unfused = std::getenv("TRIXI_HIP_UNFUSED_STAGE");
   ctx->elem_stage = (st == elem_registry().end() || (unfused && std::atoi(unfused) != 0)) ? nullptr : st->second;
   ctx->epb = epb;
@@ -254,9 +261,15 @@ template <typename T>
 int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *tdev,
                  const StageOut *so = nullptr) {
   // shock capturing: raw indicator first (needs the whole state), smoothing happens inline
-  if (ctx->vol == VOL_HG_RANOCHA || ctx->vol == VOL_HG_CENTRAL) {
+  const bool hg = ctx->vol == VOL_HG_RANOCHA || ctx->vol == VOL_HG_CENTRAL;
+  if (hg) {
     int r = launch_indicator_raw<T>(ctx, u);
     if (r) return r;
+    if (ctx->M > 0 && ctx->cfg.alpha_smooth) {  // smoothing across interfaces and mortars, before the element kernel
+      hipLaunchKernelGGL(hg_smooth_adj_kernel, dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+                         ctx->d_alpha_raw, ctx->d_adj_ptr, ctx->d_adj, ctx->d_alpha, ctx->E);
+      HIP_TRY(hipGetLastError());
+    }
   }
   // outer states of the Dirichlet boundaries at this stage's time; a
   // time-independent boundary function is evaluated once
@@ -272,22 +285,43 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
     if (!ok) return TRIXI_HIP_EINVAL;
     HIP_TRY(hipGetLastError());
   }
+  if (ctx->M > 0) {  // fluxes of the mortar faces
+    bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+      constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+      constexpr int n = N + 1, S = 1 << (D - 1), TPM = S * (D == 3 ? n * n : n), MPB = 256 / TPM;
+      MortarConst<T, N> mc{};
+      for (int i = 0; i < n * n; ++i) {
+        mc.fu[i] = T(ctx->mfu[i]);
+        mc.fl[i] = T(ctx->mfl[i]);
+        mc.ru[i] = T(ctx->mru[i]);
+        mc.rl[i] = T(ctx->mrl[i]);
+      }
+      hipLaunchKernelGGL((mortar_flux_kernel<T, D, N>), dim3((ctx->M + MPB - 1) / MPB), dim3(256), 0, ctx->stream,
+                         u, ctx->d_mort, ctx->M, ctx->B, ctx->cfg.surface_flux, T(ctx->cfg.gamma), mc,
+                         (T *)ctx->d_mflux);
+    });
+    if (!ok) return TRIXI_HIP_EINVAL;
+    HIP_TRY(hipGetLastError());
+  }
   ElemArgs<T> a{};
   a.u = u;
   a.du = du;
   a.nbr = ctx->d_nbr;
   a.ghost = (const T *)ctx->d_ghost;
+  a.mflux = (const T *)ctx->d_mflux;
+  a.nbnd = ctx->B;
   a.inv_jac = ctx->d_invj;
   a.centers = ctx->d_centers;
-  a.alpha_raw = ctx->d_alpha_raw;
-  a.alpha_out = ctx->d_alpha;
+  const bool pre_smoothed = hg && ctx->M > 0 && ctx->cfg.alpha_smooth;  // smoothed across mortars above
+  a.alpha_raw = pre_smoothed ? ctx->d_alpha : ctx->d_alpha_raw;
+  a.alpha_out = pre_smoothed ? nullptr : ctx->d_alpha;
   a.tdev = tdev;
   a.t = t;
   a.E = ctx->E;
   a.surface_flux = ctx->cfg.surface_flux;
   a.fv_flux = ctx->cfg.fv_flux;
   a.source = ctx->cfg.source_terms == TRIXI_HIP_SRC_CONVERGENCE_TEST;
-  a.alpha_smooth = ctx->cfg.alpha_smooth;
+  a.alpha_smooth = ctx->M > 0 ? 0 : ctx->cfg.alpha_smooth;
   a.xcd_remap = ctx->xcd_remap;
   a.gamma = T(ctx->cfg.gamma);
   a.inv_gamma_m1 = T(1) / (a.gamma - T(1));  // the device expression, evaluated once in T
@@ -438,8 +472,17 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   };
   if (ctx->D != 2 && ctx->D != 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 2 or 3");
   ctx->B = cfg->nboundaries;
-  if (ctx->E <= 0 || ctx->B < 0 || 2 * ctx->I + ctx->B != 2 * ctx->D * ctx->E)
-    return fail(TRIXI_HIP_EINVAL, "a conforming TreeMesh has 2 ninterfaces + nboundaries == 2 ndims nelements");
+  ctx->M = cfg->nmortars;
+  const int S = 1 << (ctx->D - 1);
+  if (ctx->E <= 0 || ctx->B < 0 || ctx->M < 0 ||
+      2 * ctx->I + ctx->B + ctx->M * (S + 1) != 2 * ctx->D * ctx->E)
+    return fail(TRIXI_HIP_EINVAL,
+                "every element face must be in one interface, boundary or mortar: "
+                "2 ninterfaces + nboundaries + nmortars (2^(ndims-1) + 1) == 2 ndims nelements");
+  if (ctx->M > 0 && (!mesh->mortar_neighbor_ids || !mesh->mortar_large_sides || !mesh->mortar_orientations ||
+                     !basis->mortar_forward_upper || !basis->mortar_forward_lower || !basis->mortar_reverse_upper ||
+                     !basis->mortar_reverse_lower))
+    return fail(TRIXI_HIP_EINVAL, "mortar arrays or mortar matrices missing");
   if (ctx->B > 0 && cfg->boundary_condition != TRIXI_HIP_BC_DIRICHLET)
     return fail(TRIXI_HIP_EINVAL, "a mesh with boundaries needs boundary_condition = TRIXI_HIP_BC_DIRICHLET");
   if (ctx->B > 0 && (!mesh->boundary_neighbor_ids || !mesh->boundary_orientations || !mesh->boundary_neighbor_sides))
@@ -479,6 +522,46 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     bnd[2 * b] = e;
     bnd[2 * b + 1] = dir;
   }
+  // mortars (init_mortars!, containers_3d.jl:652-788): small element q at its face
+  // towards the large one, the large element at its face towards the smalls
+  std::vector<int32_t> mort((size_t)ctx->M * (S + 3));
+  std::vector<std::vector<int32_t>> adjl(ctx->M > 0 ? E : 0);
+  for (int m = 0; m < ctx->M; ++m) {
+    const int o = mesh->mortar_orientations[m], side = mesh->mortar_large_sides[m];
+    if (o < 0 || o >= D || (side != 1 && side != 2)) return fail(TRIXI_HIP_EINVAL, "bad mortar data");
+    for (int k = 0; k <= S; ++k) {
+      const int e = mesh->mortar_neighbor_ids[k + (S + 1) * m];
+      if (e < 0 || e >= E) return fail(TRIXI_HIP_EINVAL, "bad mortar data");
+      const bool large = k == S;
+      const int dir = (side == 1) == large ? 2 * o + 1 : 2 * o;
+      if (nbr[dir + 2 * D * e] != kUnset) return fail(TRIXI_HIP_EINVAL, "a mortar face is also another face");
+      nbr[dir + 2 * D * e] = -(ctx->B + m * (S + 1) + k + 1);
+      mort[(size_t)m * (S + 3) + k] = e;
+    }
+    mort[(size_t)m * (S + 3) + S + 1] = o;
+    mort[(size_t)m * (S + 3) + S + 2] = side;
+    const int large = mort[(size_t)m * (S + 3) + S];
+    for (int q = 0; q < S; ++q) {
+      adjl[mort[(size_t)m * (S + 3) + q]].push_back(large);
+      adjl[large].push_back(mort[(size_t)m * (S + 3) + q]);
+    }
+  }
+  std::vector<int32_t> adj_ptr, adj;
+  if (ctx->M > 0) {
+    for (int i = 0; i < I; ++i) {
+      adjl[mesh->neighbor_ids[2 * i]].push_back(mesh->neighbor_ids[2 * i + 1]);
+      adjl[mesh->neighbor_ids[2 * i + 1]].push_back(mesh->neighbor_ids[2 * i]);
+    }
+    adj_ptr.push_back(0);
+    for (int e = 0; e < E; ++e) {
+      adj.insert(adj.end(), adjl[e].begin(), adjl[e].end());
+      adj_ptr.push_back((int32_t)adj.size());
+    }
+    ctx->mfu.assign(basis->mortar_forward_upper, basis->mortar_forward_upper + n * n);
+    ctx->mfl.assign(basis->mortar_forward_lower, basis->mortar_forward_lower + n * n);
+    ctx->mru.assign(basis->mortar_reverse_upper, basis->mortar_reverse_upper + n * n);
+    ctx->mrl.assign(basis->mortar_reverse_lower, basis->mortar_reverse_lower + n * n);
+  }
   // coordinates of the boundary face nodes: x_d = center_d + J node_{i_d}, with
   // i_o at the face and the free indices in face-node order (lower dimension fastest)
   const int Lf = D == 3 ? n * n : n;
@@ -499,7 +582,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     }
   }
   for (size_t k = 0; k < nbr.size(); ++k)
-    if (nbr[k] == kUnset) return fail(TRIXI_HIP_EINVAL, "element face without interface or boundary (mortars unsupported)");
+    if (nbr[k] == kUnset) return fail(TRIXI_HIP_EINVAL, "element face without interface, boundary or mortar");
   // basis
   ctx->nodes.assign(basis->nodes, basis->nodes + n);
   ctx->weights.assign(basis->weights, basis->weights + n);
@@ -544,9 +627,17 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
             hipMalloc(&ctx->d_alpha_raw, E * 8) == hipSuccess && hipMalloc(&ctx->d_alpha, E * 8) == hipSuccess &&
             hipMalloc(&ctx->d_partial, ctx->partial_cap * 8) == hipSuccess &&
             hipMalloc(&ctx->d_scalars, 64 * 8) == hipSuccess;
-  if (ok && ctx->B > 0)
-    ok = hipMalloc(&ctx->d_bx, bx.size() * 8) == hipSuccess &&
-         hipMalloc(&ctx->d_ghost, (size_t)ctx->B * ipow_c(n, D - 1) * ctx->V * tsz) == hipSuccess;
+  // face slots: boundaries, then mortar faces; ghost states of the mortar slots
+  // are a valid placeholder state (the element kernel replaces their fluxes)
+  const size_t nslots = (size_t)ctx->B + (size_t)ctx->M * (S + 1);
+  const size_t slot_vals = nslots * Lf * ctx->V;
+  if (ok && ctx->B > 0) ok = hipMalloc(&ctx->d_bx, bx.size() * 8) == hipSuccess;
+  if (ok && nslots > 0) ok = hipMalloc(&ctx->d_ghost, slot_vals * tsz) == hipSuccess;
+  if (ok && ctx->M > 0)
+    ok = hipMalloc(&ctx->d_mflux, slot_vals * tsz) == hipSuccess &&
+         hipMalloc(&ctx->d_mort, mort.size() * 4) == hipSuccess &&
+         hipMalloc(&ctx->d_adj_ptr, adj_ptr.size() * 4) == hipSuccess &&
+         hipMalloc(&ctx->d_adj, std::max<size_t>(adj.size(), 1) * 4) == hipSuccess;
   if (ok && ctx->fp32)
     ok = hipMalloc(&ctx->d_u32b[0], nv * 4) == hipSuccess && hipMalloc(&ctx->d_u32b[1], nv * 4) == hipSuccess &&
          hipMalloc(&ctx->d_su32, nv * 4) == hipSuccess &&
@@ -564,6 +655,23 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
        hipMemcpy(ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(ctx->d_scalars + 32, ctx->weights.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess &&
        (ctx->B == 0 || hipMemcpy(ctx->d_bx, bx.data(), bx.size() * 8, hipMemcpyHostToDevice) == hipSuccess);
+  if (ok && nslots > 0) {
+    std::vector<double> g(slot_vals);
+    for (size_t k = 0; k < slot_vals; ++k) {
+      const int v = (int)(k % ctx->V);
+      g[k] = v == 0 ? 1.0 : (v == ctx->V - 1 ? 2.5 : 0.0);  // rho = 1, v = 0, p = 1 (gamma 1.4)
+    }
+    if (ctx->fp32) {
+      std::vector<float> gf(g.begin(), g.end());
+      ok = hipMemcpy(ctx->d_ghost, gf.data(), slot_vals * 4, hipMemcpyHostToDevice) == hipSuccess;
+    } else {
+      ok = hipMemcpy(ctx->d_ghost, g.data(), slot_vals * 8, hipMemcpyHostToDevice) == hipSuccess;
+    }
+  }
+  if (ok && ctx->M > 0)
+    ok = hipMemcpy(ctx->d_mort, mort.data(), mort.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(ctx->d_adj_ptr, adj_ptr.data(), adj_ptr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         (adj.empty() || hipMemcpy(ctx->d_adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
   if (!ok) {
     trixi_hip_destroy(ctx);
     fprintf(stderr, "trixi_hip_create: upload failed\n");
@@ -581,7 +689,8 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
                   ctx->d_u32b[0], ctx->d_u32b[1], ctx->d_du,
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
                   ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
-                  ctx->d_alpha_hg, ctx->d_records, ctx->d_bx, ctx->d_ghost};
+                  ctx->d_alpha_hg, ctx->d_records, ctx->d_bx, ctx->d_ghost, ctx->d_mort, ctx->d_mflux,
+                  ctx->d_adj_ptr, ctx->d_adj};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -978,11 +1087,17 @@ int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host) {
   int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, cur_u32(ctx)) : launch_indicator_raw<double>(ctx, cur_u64(ctx));
   ctx->ind_mode = saved;
   if (r) return r;
-  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto) {
-    constexpr int D = decltype(Dc)::value;
-    hipLaunchKernelGGL((hg_smooth_kernel<D>), dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
-                       ctx->d_alpha_raw, ctx->d_nbr, ctx->d_alpha, ctx->E, ctx->cfg.alpha_smooth);
-  });
+  bool ok = true;
+  if (ctx->M > 0 && ctx->cfg.alpha_smooth) {  // neighbours across mortars too
+    hipLaunchKernelGGL(hg_smooth_adj_kernel, dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_alpha_raw, ctx->d_adj_ptr, ctx->d_adj, ctx->d_alpha, ctx->E);
+  } else {
+    ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto) {
+      constexpr int D = decltype(Dc)::value;
+      hipLaunchKernelGGL((hg_smooth_kernel<D>), dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+                         ctx->d_alpha_raw, ctx->d_nbr, ctx->d_alpha, ctx->E, ctx->cfg.alpha_smooth);
+    });
+  }
   if (!ok) return TRIXI_HIP_EINVAL;
   HIP_TRY(hipGetLastError());
   if (alpha_host) {

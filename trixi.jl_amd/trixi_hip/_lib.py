@@ -74,6 +74,7 @@ class Config(ctypes.Structure):
         ("threads_per_block", ctypes.c_int32),
         ("nboundaries", ctypes.c_int32),
         ("boundary_condition", ctypes.c_int32),
+        ("nmortars", ctypes.c_int32),
     ]
 
 
@@ -86,6 +87,9 @@ class Mesh(ctypes.Structure):
         ("boundary_neighbor_ids", ctypes.POINTER(ctypes.c_int32)),
         ("boundary_orientations", ctypes.POINTER(ctypes.c_int32)),
         ("boundary_neighbor_sides", ctypes.POINTER(ctypes.c_int32)),
+        ("mortar_neighbor_ids", ctypes.POINTER(ctypes.c_int32)),
+        ("mortar_large_sides", ctypes.POINTER(ctypes.c_int32)),
+        ("mortar_orientations", ctypes.POINTER(ctypes.c_int32)),
     ]
 
 
@@ -99,6 +103,10 @@ class Basis(ctypes.Structure):
         ("analysis_nnodes", ctypes.c_int32),
         ("analysis_weights", ctypes.POINTER(ctypes.c_double)),
         ("analysis_vandermonde", ctypes.POINTER(ctypes.c_double)),
+        ("mortar_forward_upper", ctypes.POINTER(ctypes.c_double)),
+        ("mortar_forward_lower", ctypes.POINTER(ctypes.c_double)),
+        ("mortar_reverse_upper", ctypes.POINTER(ctypes.c_double)),
+        ("mortar_reverse_lower", ctypes.POINTER(ctypes.c_double)),
     ]
 
 

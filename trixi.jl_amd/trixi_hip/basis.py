@@ -139,6 +139,69 @@ def polynomial_interpolation_matrix(nodes_in, nodes_out):
     return Vm
 
 
+def legendre_and_derivative(N, x):
+    """Normalized Legendre polynomial and derivative (basis_lobatto_legendre.jl:677-708)."""
+    if N == 0:
+        poly, deriv = 1.0, 0.0
+    elif N == 1:
+        poly, deriv = float(x), 1.0
+    else:
+        p2, p1, d2, d1 = 1.0, float(x), 0.0, 1.0
+        poly = deriv = 0.0
+        for i in range(2, N + 1):
+            poly = ((2 * i - 1) * x * p1 - (i - 1) * p2) / i
+            deriv = d2 + (2 * i - 1) * p1
+            p2, p1, d2, d1 = p1, poly, d1, deriv
+    s = math.sqrt(N + 0.5)
+    return poly * s, deriv * s
+
+
+def gauss_nodes_weights(n_nodes):
+    """Legendre-Gauss nodes and weights (basis_gauss_legendre.jl:149-205):
+    Newton from cos starting guesses, symmetric roots."""
+    N = n_nodes - 1
+    nodes, weights = np.ones(n_nodes), np.zeros(n_nodes)
+    if N == 0:
+        return np.zeros(1), np.full(1, 2.0)
+    if N == 1:
+        x = math.sqrt(1.0 / 3)
+        return np.array([-x, x]), np.ones(2)
+    for i in range((N + 1) // 2):
+        x = -math.cos(math.pi / (2 * N + 2) * (2 * i + 1))
+        for _ in range(21):
+            poly, deriv = legendre_and_derivative(N + 1, x)
+            dx = -poly / deriv
+            x += dx
+            if abs(dx) < 2 * np.finfo(float).eps * abs(x):
+                break
+        poly, deriv = legendre_and_derivative(N + 1, x)
+        nodes[i], nodes[N - i] = x, -x
+        weights[i] = weights[N - i] = (2 * N + 3) / ((1 - x * x) * deriv * deriv)
+    if n_nodes % 2 == 1:
+        poly, deriv = legendre_and_derivative(N + 1, 0.0)
+        nodes[N // 2] = 0.0
+        weights[N // 2] = (2 * N + 3) / (deriv * deriv)
+    return nodes, weights
+
+
+def mortar_l2_matrices(nodes):
+    """LobattoLegendreMortarL2 (basis_lobatto_legendre.jl:176-205, l2projection.jl:26-116):
+    forward_{upper,lower} interpolate to the two halves; reverse_{upper,lower}
+    are the L2 projections back, evaluated on the Gauss nodes."""
+    n = len(nodes)
+    fu = polynomial_interpolation_matrix(nodes, 0.5 * (nodes + 1))
+    fl = polynomial_interpolation_matrix(nodes, 0.5 * (nodes - 1))
+    g, gw = gauss_nodes_weights(n)
+    g2l = polynomial_interpolation_matrix(g, nodes)
+    l2g = polynomial_interpolation_matrix(nodes, g)
+    rev = []
+    for shift in (1.0, -1.0):
+        P = polynomial_interpolation_matrix(g, 0.5 * (g + shift))  # P[j, i] = lg_i(pts_j)
+        op = 0.5 * P.T * gw[None, :] / gw[:, None]
+        rev.append(g2l @ (op @ l2g))
+    return fu, fl, rev[0], rev[1]
+
+
 class LobattoLegendreBasis:
     """`LobattoLegendreBasis(polydeg)` (basis_lobatto_legendre.jl:55-86)."""
 
@@ -154,6 +217,8 @@ class LobattoLegendreBasis:
         # SolutionAnalyzer(basis; analysis_polydeg = 2 * polydeg)
         self.analysis_nodes, self.analysis_weights = gauss_lobatto_nodes_weights(2 * self.polydeg + 1)
         self.analysis_vandermonde = polynomial_interpolation_matrix(self.nodes, self.analysis_nodes)
+        (self.mortar_forward_upper, self.mortar_forward_lower,
+         self.mortar_reverse_upper, self.mortar_reverse_lower) = mortar_l2_matrices(self.nodes)
 
     def nnodes(self):
         return self.polydeg + 1

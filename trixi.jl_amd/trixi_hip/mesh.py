@@ -15,13 +15,16 @@ Restates the parts of Trixi.jl that fix element order and geometry:
       containers_3d.jl:229-279, containers_2d.jl:228-290
   init_boundaries! (per direction, then element order; neighbor_sides 1 when
       the element is left of the boundary) containers_2d.jl:475-543
+  refinement_patches of type "box": refine_box! / refine! / rebalance!
+      abstract_tree.jl:317-490 (children inserted behind their parent, 2:1
+      balance); init_mortars! containers_3d.jl:652-788, containers_2d.jl:700-800
 """
 import numpy as np
 
 
 class TreeMesh:
     def __init__(self, coordinates_min, coordinates_max, initial_refinement_level,
-                 periodicity=False):
+                 periodicity=False, refinement_patches=()):
         cmin = tuple(float(c) for c in coordinates_min)
         cmax = tuple(float(c) for c in coordinates_max)
         if len(cmin) not in (2, 3) or len(cmin) != len(cmax):
@@ -43,7 +46,24 @@ class TreeMesh:
                 raise ValueError("The TreeMesh domain must be a hypercube")
         self.length_level_0 = length
         self.center_level_0 = tuple((a + b) / 2 for a, b in zip(cmin, cmax))
+        self.refinement_patches = []
+        for patch in refinement_patches:
+            if isinstance(patch, dict):
+                if patch.get("type", "box") != "box":
+                    raise ValueError("only refinement patches of type 'box' are supported")
+                patch = (patch["coordinates_min"], patch["coordinates_max"])
+            pmin, pmax = patch
+            self.refinement_patches.append((tuple(float(v) for v in pmin), tuple(float(v) for v in pmax)))
         self._build()
+        if self.refinement_patches:
+            self._build_refined()
+        else:
+            self.cell_lengths = np.full(self.nelements, self.cell_length)
+            self.levels = np.full(self.nelements, self.initial_refinement_level, dtype=np.int32)
+            self.mortar_neighbor_ids = np.zeros((0, (1 << self.ndims - 1) + 1), dtype=np.int32)
+            self.mortar_large_sides = np.zeros(0, dtype=np.int32)
+            self.mortar_orientations = np.zeros(0, dtype=np.int32)
+            self.nmortars = 0
 
     def _build(self):
         D, L = self.ndims, self.initial_refinement_level
@@ -66,6 +86,7 @@ class TreeMesh:
         rec(0, self.center_level_0, self.length_level_0, (0,) * D)
         self.nelements = len(centers)
         self.cell_centers = np.array(centers, dtype=np.float64)  # [E, D]
+        self._icoords = icoords
         self.cell_length = self.length_level_0 / (1 << L)
         nside = 1 << L
         ic = np.array(icoords, dtype=np.int64)
@@ -114,6 +135,131 @@ class TreeMesh:
         self.boundary_neighbor_sides = np.where(b_dir % 2 == 1, 1, 2).astype(np.int32)
         self.nboundaries = len(b_el)
 
+    def _build_refined(self):
+        """Leaves of the uniform tree refined by the box patches (cells whose
+        centre lies strictly inside), rebalanced 2:1, in depth-first order, and
+        their interfaces, L2 mortars and boundaries."""
+        D, L = self.ndims, self.initial_refinement_level
+        per = self.periodicity
+        leaves = [(L, tuple(int(v) for v in ic), tuple(float(v) for v in c))
+                  for ic, c in zip(self._icoords, self.cell_centers)]
+
+        def rebuild():
+            cells = {}
+            for i, (lv, ic, _) in enumerate(leaves):
+                cells[(lv, ic)] = i
+                a = ic
+                for l2 in range(lv - 1, -1, -1):
+                    a = tuple(v >> 1 for v in a)
+                    if (l2, a) in cells:
+                        break
+                    cells[(l2, a)] = -1
+            return cells
+
+        def neighbour(lv, ic, dirn):
+            o, nside = dirn // 2, 1 << lv
+            k = ic[o] + (1 if dirn % 2 else -1)
+            if (k < 0 or k >= nside) and not per[o]:
+                return None
+            nb = list(ic)
+            nb[o] = k % nside
+            return tuple(nb)
+
+        def refine(marked):
+            out = []
+            for i, leaf in enumerate(leaves):
+                if i not in marked:
+                    out.append(leaf)
+                    continue
+                lv, ic, c = leaf
+                child_length = self.length_level_0 / (1 << lv) / 2
+                for child in range(1 << D):
+                    bits = [(child >> d) & 1 for d in range(D)]
+                    out.append((lv + 1, tuple(2 * ic[d] + bits[d] for d in range(D)),
+                                tuple(c[d] + (1 if bits[d] else -1) * child_length / 2 for d in range(D))))
+            return out
+
+        cells = rebuild()
+        for pmin, pmax in self.refinement_patches:
+            marked = {i for i, (_, _, c) in enumerate(leaves)
+                      if all(pmin[d] < c[d] and pmax[d] > c[d] for d in range(D))}
+            refined = [leaves[i] for i in sorted(marked)]
+            leaves = refine(marked)
+            cells = rebuild()
+            while refined:
+                want = set()
+                for lv, ic, _ in refined:
+                    for dirn in range(2 * D):
+                        nb = neighbour(lv, ic, dirn)
+                        if nb is not None and (lv, nb) in cells:
+                            continue
+                        if lv == 0:
+                            continue
+                        pnb = neighbour(lv - 1, tuple(v >> 1 for v in ic), dirn)
+                        if pnb is None or (lv - 1, pnb) not in cells:
+                            continue
+                        want.add((lv - 1, pnb))
+                marked = {i for i, (lv, ic, _) in enumerate(leaves) if (lv, ic) in want}
+                refined = [leaves[i] for i in sorted(marked)]
+                if refined:
+                    leaves = refine(marked)
+                    cells = rebuild()
+        E = len(leaves)
+        self.nelements = E
+        self.cell_centers = np.array([c for _, _, c in leaves], dtype=np.float64)
+        self.levels = np.array([lv for lv, _, _ in leaves], dtype=np.int32)
+        self.cell_lengths = self.length_level_0 / (1 << self.levels).astype(np.float64)
+        ns = 1 << (D - 1)
+        nbr = np.full((E, 2 * D), -1, dtype=np.int32)
+        boundary = np.zeros((E, 2 * D), dtype=bool)
+        m_ids, m_side, m_ori = [], [], []
+        for e, (lv, ic, _) in enumerate(leaves):
+            for dirn in range(2 * D):
+                nb = neighbour(lv, ic, dirn)
+                if nb is None:
+                    boundary[e, dirn] = True
+                    continue
+                f = cells.get((lv, nb))
+                if f is None:
+                    continue  # small side of a coarser neighbour's mortar
+                if f >= 0:
+                    nbr[e, dirn] = f
+                    continue
+                o, plus = dirn // 2, dirn % 2
+                free = [d for d in range(D) if d != o]
+                ids = []
+                for q in range(ns):
+                    ch = [2 * v for v in nb]
+                    ch[o] += 0 if plus else 1
+                    ch[free[0]] += q & 1
+                    if D == 3:
+                        ch[free[1]] += q >> 1
+                    ids.append(cells[(lv + 1, tuple(ch))])
+                m_ids.append(ids + [e])
+                m_side.append(1 if plus else 2)
+                m_ori.append(o)
+        self.element_neighbors = nbr
+        has = nbr[:, 1::2] >= 0
+        el, o = np.nonzero(has)
+        self.neighbor_ids = np.stack([el, nbr[el, 2 * o + 1]], axis=1).astype(np.int32).reshape(-1, 2)
+        self.orientations = o.astype(np.int32)
+        self.ninterfaces = len(o)
+        self.mortar_neighbor_ids = np.array(m_ids, dtype=np.int32).reshape(-1, ns + 1)
+        self.mortar_large_sides = np.array(m_side, dtype=np.int32)
+        self.mortar_orientations = np.array(m_ori, dtype=np.int32)
+        self.nmortars = len(m_side)
+        b_el, b_dir = [], []
+        for dirn in range(2 * D):
+            ids = np.nonzero(boundary[:, dirn])[0]
+            b_el.append(ids)
+            b_dir.append(np.full(len(ids), dirn))
+        b_el = np.concatenate(b_el).astype(np.int32)
+        b_dir = np.concatenate(b_dir).astype(np.int32)
+        self.boundary_neighbor_ids = b_el
+        self.boundary_orientations = (b_dir // 2).astype(np.int32)
+        self.boundary_neighbor_sides = np.where(b_dir % 2 == 1, 1, 2).astype(np.int32)
+        self.nboundaries = len(b_el)
+
     def ncells(self):
         return self.nelements
 
@@ -128,12 +274,12 @@ def element_geometry(mesh, basis):
     n = basis.nnodes()
     reference_length = float(np.sum(basis.weights))
     reference_offset = (basis.nodes[0] + basis.nodes[-1]) / 2
-    jacobian = mesh.cell_length / reference_length
-    inv_jac = np.full(mesh.nelements, 1.0 / jacobian)
-    rel = jacobian * (basis.nodes - reference_offset)
+    jacobian = mesh.cell_lengths / reference_length  # [E]
+    inv_jac = 1.0 / jacobian
+    rel = jacobian[:, None] * (basis.nodes - reference_offset)[None, :]  # [E, n]
     NN = n ** D
     idx = np.indices((n,) * D).reshape(D, -1)[::-1]  # idx[d, node], node = i + n j (+ n^2 k)
     coords = np.empty((mesh.nelements, NN, D))
     for d in range(D):
-        coords[:, :, d] = mesh.cell_centers[:, d][:, None] + rel[idx[d]][None, :]
+        coords[:, :, d] = mesh.cell_centers[:, d][:, None] + rel[:, idx[d]]
     return inv_jac, coords

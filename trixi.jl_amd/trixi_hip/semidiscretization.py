@@ -107,6 +107,7 @@ class SemidiscretizationHyperbolic:
         cfg.elements_per_block, cfg.threads_per_block = self.tile
         cfg.nboundaries = self.mesh.nboundaries
         cfg.boundary_condition = _lib.BC_DIRICHLET if self.mesh.nboundaries else _lib.BC_NONE
+        cfg.nmortars = self.mesh.nmortars
         return cfg
 
     @property
@@ -123,6 +124,11 @@ class SemidiscretizationHyperbolic:
                 bel=np.ascontiguousarray(self.mesh.boundary_neighbor_ids, dtype=np.int32),
                 bor=np.ascontiguousarray(self.mesh.boundary_orientations, dtype=np.int32),
                 bsd=np.ascontiguousarray(self.mesh.boundary_neighbor_sides, dtype=np.int32),
+                mid=np.ascontiguousarray(self.mesh.mortar_neighbor_ids, dtype=np.int32),
+                msd=np.ascontiguousarray(self.mesh.mortar_large_sides, dtype=np.int32),
+                mor=np.ascontiguousarray(self.mesh.mortar_orientations, dtype=np.int32),
+                mfu=np.ascontiguousarray(b.mortar_forward_upper), mfl=np.ascontiguousarray(b.mortar_forward_lower),
+                mru=np.ascontiguousarray(b.mortar_reverse_upper), mrl=np.ascontiguousarray(b.mortar_reverse_lower),
                 nodes=np.ascontiguousarray(b.nodes), weights=np.ascontiguousarray(b.weights),
                 ds=np.ascontiguousarray(b.derivative_split), dh=np.ascontiguousarray(b.derivative_hat),
                 vinv=np.ascontiguousarray(b.inverse_vandermonde_legendre),
@@ -130,10 +136,13 @@ class SemidiscretizationHyperbolic:
             i32 = ctypes.POINTER(ctypes.c_int32)
             mesh = _lib.Mesh(arrays["nbr"].ctypes.data_as(i32), arrays["ori"].ctypes.data_as(i32),
                              _ptr(arrays["invj"]), _ptr(arrays["cen"]), arrays["bel"].ctypes.data_as(i32),
-                             arrays["bor"].ctypes.data_as(i32), arrays["bsd"].ctypes.data_as(i32))
+                             arrays["bor"].ctypes.data_as(i32), arrays["bsd"].ctypes.data_as(i32),
+                             arrays["mid"].ctypes.data_as(i32), arrays["msd"].ctypes.data_as(i32),
+                             arrays["mor"].ctypes.data_as(i32))
             basis = _lib.Basis(_ptr(arrays["nodes"]), _ptr(arrays["weights"]), _ptr(arrays["ds"]),
                                _ptr(arrays["dh"]), _ptr(arrays["vinv"]), len(b.analysis_nodes),
-                               _ptr(arrays["aw"]), _ptr(arrays["av"]))
+                               _ptr(arrays["aw"]), _ptr(arrays["av"]), _ptr(arrays["mfu"]), _ptr(arrays["mfl"]),
+                               _ptr(arrays["mru"]), _ptr(arrays["mrl"]))
             cfg = self.config()
             out = ctypes.c_void_p()
             rc = lib.trixi_hip_create(ctypes.byref(cfg), ctypes.byref(mesh), ctypes.byref(basis),
