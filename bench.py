@@ -52,14 +52,17 @@ def parse():
                    help="also run one plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE)")
     p.add_argument("--no-bounded", action="store_true",
                    help="skip the same RHS on the non-periodic mesh with Dirichlet boundaries")
+    p.add_argument("--no-refined", action="store_true",
+                   help="skip the RHS on the mesh with the central box refined once (L2 mortars)")
     p.add_argument("--profile-step", choices=["fp32", "fp64"], default=None,
                    help="profiling mode: only run K device-controlled CK2N54 steps (fused stages) of the "
                         "workload in one precision and print the time per step (no JSON line)")
     return p.parse_args()
 
 
-def make_semi(th, level, polydeg, precision, tile=(0, 0), periodic=True):
-    mesh = th.TreeMesh((0.0, 0.0, 0.0), (2.0, 2.0, 2.0), initial_refinement_level=level, periodicity=periodic)
+def make_semi(th, level, polydeg, precision, tile=(0, 0), periodic=True, patches=()):
+    mesh = th.TreeMesh((0.0, 0.0, 0.0), (2.0, 2.0, 2.0), initial_refinement_level=level, periodicity=periodic,
+                       refinement_patches=patches)
     solver = th.DGSEM(polydeg=polydeg, surface_flux=th.flux_lax_friedrichs,
                       volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_ranocha))
     ic = th.initial_condition_convergence_test
@@ -271,6 +274,18 @@ def main():
                                  nboundaries=sb.mesh.nboundaries, finite=rb["finite"])
             sb.close()
         print(f"[bench] bounded mesh: {bounded}", file=sys.stderr, flush=True)
+    refined = {}
+    if not args.no_refined and not args.only:
+        # the same RHS with the leaf cells inside (0.5, 1.5)^3 refined once:
+        # twice the resolution in the centre, L2 mortars around it
+        for prec in ("fp32", "fp64"):
+            sr = make_semi(th, args.level, args.polydeg, prec, patches=[((0.5, 0.5, 0.5), (1.5, 1.5, 1.5))])
+            ur = sr.compute_coefficients(0.0)
+            rr = time_rhs(th, torch, sr, args.steps, args.warmup, ur)
+            refined[prec] = dict(value=sr.ndofs() / (rr["ms_per_step"] * 1e-3), ms_per_step=rr["ms_per_step"],
+                                 nelements=sr.nelements(), nmortars=sr.mesh.nmortars, finite=rr["finite"])
+            sr.close()
+        print(f"[bench] refined mesh: {refined}", file=sys.stderr, flush=True)
 
     ndofs = semis["fp32"].ndofs()
     nv = semis["fp32"].nvariables()
@@ -336,12 +351,17 @@ def main():
         out["bounded_mesh"] = dict(workload="same RHS, periodicity=false, BoundaryConditionDirichlet("
                                             "initial_condition) on all sides (boundary-state kernel included)",
                                    **bounded)
+    if refined:
+        out["refined_mesh"] = dict(workload="same RHS with refinement_patches = box (0.5,0.5,0.5)-(1.5,1.5,1.5): "
+                                            "the central cells refined once, L2 mortars (mortar-flux kernel included)",
+                                   **refined)
     if l2 is not None:
         out["equal_l2"] = l2
     if cpu is not None:
         out["gpu_fp32_over_cpu"] = value / cpu["value"]
         out["cpu_time_per_dof_per_rhs_s"] = cpu["time_per_dof_per_rhs_s"]
-    ok = all(v["finite"] for v in res.values()) and all(v["finite"] for v in bounded.values())
+    ok = all(v["finite"] for v in res.values()) and all(v["finite"] for v in bounded.values()) and \
+        all(v["finite"] for v in refined.values())
     out["outputs_finite"] = ok
     for s in semis.values():
         s.close()

@@ -1,6 +1,7 @@
 /*
  * trixi_hip.h -- C ABI of the MI355X DGSEM right-hand side for the compressible
- * Euler equations on a conforming TreeMesh (2D and 3D), periodic or with Dirichlet boundaries.
+ * Euler equations on a TreeMesh (2D and 3D): uniform or refined (L2 mortars), periodic or with
+ * Dirichlet boundaries.
  *
  * The library replaces the CPU methods that Trixi.jl dispatches from
  *   rhs_hyperbolic!(du_ode, u_ode, semi, t)

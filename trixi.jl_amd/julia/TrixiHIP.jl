@@ -2,8 +2,9 @@
 #
 # Makes the MI355X right-hand side a drop-in for Trixi.jl's
 #   rhs_hyperbolic!(du_ode, u_ode, semi, t)   (src/semidiscretization/semidiscretization_hyperbolic.jl:578-597)
-# for TreeMesh{2,3} + CompressibleEulerEquations{2,3}D + DGSEM with periodic
-# boundaries or BoundaryConditionDirichlet(initial_condition). The ODEProblem keeps `semi` as its parameter, so the 2N
+# for TreeMesh{2,3} (uniform or refined with L2 mortars) +
+# CompressibleEulerEquations{2,3}D + DGSEM with periodic boundaries or
+# BoundaryConditionDirichlet(initial_condition). The ODEProblem keeps `semi` as its parameter, so the 2N
 # integrators and the Stepsize/Analysis callbacks run unchanged; only the RHS
 # function is replaced by a callable that ccalls the library.
 #
@@ -17,7 +18,7 @@ using Trixi: TreeMesh, DGSEM, CompressibleEulerEquations2D, CompressibleEulerEqu
              VolumeIntegralWeakForm, VolumeIntegralFluxDifferencing, VolumeIntegralShockCapturingHGType,
              VolumeIntegralPureLGLFiniteVolume, FluxLaxFriedrichs, flux_ranocha, flux_central,
              max_abs_speed, max_abs_speed_naive, source_terms_convergence_test, SolutionAnalyzer,
-             mesh_equations_solver_cache, nelements, ninterfaces, nboundaries,
+             mesh_equations_solver_cache, nelements, ninterfaces, nboundaries, nmortars,
              BoundaryConditionDirichlet, initial_condition_constant, initial_condition_convergence_test,
              initial_condition_weak_blast_wave
 using SciMLBase: ODEProblem
@@ -44,6 +45,7 @@ struct HipConfig
     threads_per_block::Int32
     nboundaries::Int32
     boundary_condition::Int32
+    nmortars::Int32
 end
 
 struct HipMesh
@@ -54,6 +56,9 @@ struct HipMesh
     boundary_neighbor_ids::Ptr{Int32}
     boundary_orientations::Ptr{Int32}
     boundary_neighbor_sides::Ptr{Int32}
+    mortar_neighbor_ids::Ptr{Int32}
+    mortar_large_sides::Ptr{Int32}
+    mortar_orientations::Ptr{Int32}
 end
 
 struct HipBasis
@@ -65,6 +70,10 @@ struct HipBasis
     analysis_nnodes::Int32
     analysis_weights::Ptr{Float64}
     analysis_vandermonde::Ptr{Float64}
+    mortar_forward_upper::Ptr{Float64}
+    mortar_forward_lower::Ptr{Float64}
+    mortar_reverse_upper::Ptr{Float64}
+    mortar_reverse_lower::Ptr{Float64}
 end
 
 const HOST = Int32(0)
@@ -126,12 +135,13 @@ function HipRHS(semi; precision::Symbol = :fp64)
     I = ninterfaces(solver, cache)
     B = nboundaries(solver, cache)
     B > 0 && check_boundary_conditions(semi)
+    M = nmortars(solver, cache)
     vi, vflux, fvflux, indicator = vi_codes(solver.volume_integral)
     amax, amin, asmooth = indicator === nothing ? (0.5, 0.001, true) :
                           (indicator.alpha_max, indicator.alpha_min, indicator.alpha_smooth)
     cfg = HipConfig(D, Trixi.polydeg(solver), E, I, equations.gamma, precision === :fp32 ? 1 : 0,
                     vi, vflux, flux_code(solver.surface_integral.surface_flux), fvflux, amax, amin,
-                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0, B, B > 0 ? 1 : 0)
+                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0, B, B > 0 ? 1 : 0, M)
     # interface container: 1-based ids and orientations -> 0-based
     nbr = Int32.(cache.interfaces.neighbor_ids .- 1)            # [2, I]
     ori = Int32.(cache.interfaces.orientations .- 1)            # [I]
@@ -141,6 +151,13 @@ function HipRHS(semi; precision::Symbol = :fp64)
     bel = Int32.(cache.boundaries.neighbor_ids[1:B] .- 1)
     bor = Int32.(cache.boundaries.orientations[1:B] .- 1)
     bsd = Int32.(cache.boundaries.neighbor_sides[1:B])
+    # L2 mortars container (containers_3d.jl:652-788): ids [2^(ndims-1) + 1, M], 0-based
+    mid = Int32.(cache.mortars.neighbor_ids[:, 1:M] .- 1)
+    msd = Int32.(cache.mortars.large_sides[1:M])
+    mor = Int32.(cache.mortars.orientations[1:M] .- 1)
+    mortar = solver.mortar  # LobattoLegendreMortarL2
+    mfu, mfl = rowmajor(mortar.forward_upper), rowmajor(mortar.forward_lower)
+    mru, mrl = rowmajor(mortar.reverse_upper), rowmajor(mortar.reverse_lower)
     basis = solver.basis
     analyzer = SolutionAnalyzer(basis)
     nodes, weights = Vector{Float64}(basis.nodes), Vector{Float64}(basis.weights)
@@ -148,11 +165,12 @@ function HipRHS(semi; precision::Symbol = :fp64)
     vinv = rowmajor(basis.inverse_vandermonde_legendre)
     aw, av = Vector{Float64}(analyzer.weights), rowmajor(analyzer.vandermonde)
     out = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve nbr ori invj centers bel bor bsd nodes weights ds dh vinv aw av begin
+    GC.@preserve nbr ori invj centers bel bor bsd mid msd mor nodes weights ds dh vinv aw av mfu mfl mru mrl begin
         hm = Ref(HipMesh(pointer(nbr), pointer(ori), pointer(invj), pointer(centers),
-                         pointer(bel), pointer(bor), pointer(bsd)))
+                         pointer(bel), pointer(bor), pointer(bsd), pointer(mid), pointer(msd), pointer(mor)))
         hb = Ref(HipBasis(pointer(nodes), pointer(weights), pointer(ds), pointer(dh), pointer(vinv),
-                          length(aw), pointer(aw), pointer(av)))
+                          length(aw), pointer(aw), pointer(av), pointer(mfu), pointer(mfl), pointer(mru),
+                          pointer(mrl)))
         rc = ccall((:trixi_hip_create, libtrixi_hip), Cint,
                    (Ref{HipConfig}, Ref{HipMesh}, Ref{HipBasis}, Ref{Ptr{Cvoid}}), Ref(cfg), hm, hb, out)
         rc == 0 || error("trixi_hip_create failed with code $rc")
