@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libtrixi_oracle.so")
 VI = {"weak": 0, "fd": 1, "hg": 2, "fv": 3}
 FLUX = {"llf": 0, "llf_naive": 1, "ranocha": 2, "central": 3}
 IC = {"constant": 0, "convergence_test": 1, "weak_blast_wave": 2, "isentropic_vortex": 3,
-      "kelvin_helmholtz": 4, "fourier_modes": 5}
+      "kelvin_helmholtz": 4, "fourier_modes": 5, "density_pulse": 6}
 
 
 class _Config(ctypes.Structure):
@@ -65,6 +65,12 @@ def _load():
         L.oracle_rk_step.argtypes = [P, P, P, P, D, D]
         L.oracle_solve.restype = I
         L.oracle_solve.argtypes = [P, P, D, D, D, I, P]
+        L.oracle_indicator_max.argtypes = [P, P, I, P]
+        L.oracle_amr_adapt.restype = I
+        L.oracle_amr_adapt.argtypes = [P, P, P, I, I]
+        L.oracle_get_adapted_u.argtypes = [P, P]
+        L.oracle_get_transfer.argtypes = [P, P, P]
+        L.oracle_get_levels.argtypes = [P, P]
         L.oracle_set_threads.argtypes = [I]
         L.oracle_get_threads.restype = I
         L.oracle_ck2n54_coefficients.argtypes = [P, P, P]
@@ -127,6 +133,10 @@ class Oracle:
         self.ndims, self.polydeg = ndims, polydeg
         self.n = polydeg + 1
         self.nvars = ndims + 2
+        self._sizes()
+
+    def _sizes(self):
+        L = self.L
         self.nelements = L.oracle_nelements(self._c)
         self.ninterfaces = L.oracle_ninterfaces(self._c)
         self.nboundaries = L.oracle_nboundaries(self._c)
@@ -245,6 +255,81 @@ class Oracle:
             raise FloatingPointError("time step size `dt` is NaN")
         return u, float(tout[0]), nsteps
 
+    def levels(self):
+        lv = np.empty(self.nelements, np.int32)
+        self.L.oracle_get_levels(self._c, _p(lv))
+        return lv
+
+    def indicator_max(self, u, variable=0):
+        """IndicatorMax(semi; variable): per-element maximum of a conserved variable."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        a = np.empty(self.nelements)
+        self.L.oracle_indicator_max(self._c, _p(u), int(variable), _p(a))
+        return a
+
+    def amr_adapt(self, u, lam, only_refine=False, only_coarsen=False):
+        """Adapt the tree with controller values lam (>0 refine, <0 coarsen);
+        returns (changed, projected u, transfer kind, transfer src)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.int32)
+        rc = self.L.oracle_amr_adapt(self._c, _p(u), _p(lam), int(only_refine), int(only_coarsen))
+        if rc < 0:
+            raise ValueError("oracle_amr_adapt: the adapted tree is not 2:1 balanced")
+        self._sizes()
+        un = np.empty(self.nvalues)
+        kind, src = np.empty(self.nelements, np.int32), np.empty(self.nelements, np.int32)
+        self.L.oracle_get_adapted_u(self._c, _p(un))
+        self.L.oracle_get_transfer(self._c, _p(kind), _p(src))
+        return bool(rc), un, kind, src
+
+    def solve_amr(self, t0, t_end, cfl, controller, interval=5, adapt_initial_condition=True,
+                  adapt_initial_condition_only_refine=True, variable=0, maxiters=0):
+        """solve(ode, CarpenterKennedy2N54; callback = (AMRCallback(controller;
+        interval, adapt_initial_condition), StepsizeCallback(cfl))) on the
+        initial condition set by set_initial_condition. The AMR callback
+        adapts the initial condition until the mesh settles (amr.jl:143-180),
+        then after every `interval`-th step except the final one (amr.jl:52-58);
+        the step size is computed after it (callback order of the elixirs).
+        Returns (u, t, nsteps, history of element counts)."""
+        history = []
+
+        def adapt(u, **kw):
+            lam = controller(self.indicator_max(u, variable), self.levels())
+            changed, un, _, _ = self.amr_adapt(u, lam, **kw)
+            history.append(self.nelements)
+            return changed, un
+
+        u = self.compute_coefficients(t0)
+        if adapt_initial_condition:
+            changed, u = adapt(u, only_refine=adapt_initial_condition_only_refine)
+            iterations = 1
+            while changed:
+                u = self.compute_coefficients(t0)
+                changed, u = adapt(u, only_refine=adapt_initial_condition_only_refine)
+                iterations += 1
+                if iterations > max(10, controller.max_level):
+                    break
+        t = float(t0)
+        dt = self.max_dt(u, cfl)
+        it = 0
+        eps = float(np.sqrt(np.finfo(np.float64).eps))
+        while True:
+            if np.isnan(dt):
+                raise FloatingPointError("time step size `dt` is NaN")
+            te = t + dt
+            final = te > t_end or te == t_end or abs(te - t_end) <= eps * max(abs(te), abs(t_end))
+            if final:
+                dt = t_end - t
+            self.rk_step(u, t, dt)
+            it += 1
+            t += dt
+            if interval > 0 and it % interval == 0 and not final:
+                _, u = adapt(u)
+            dt = self.max_dt(u, cfl)
+            if final or (maxiters and it >= maxiters):
+                break
+        return u, t, it, history
+
     @staticmethod
     def set_threads(n):
         _load().oracle_set_threads(int(n))
@@ -283,3 +368,26 @@ def ck2n54():
     a, b, c = np.empty(5), np.empty(5), np.empty(5)
     _load().oracle_ck2n54_coefficients(_p(a), _p(b), _p(c))
     return a, b, c
+
+
+class ControllerThreeLevel:
+    """ControllerThreeLevel(semi, indicator; base_level, med_level, med_threshold,
+    max_level, max_threshold), amr.jl:1116-1153: target level max_level above
+    max_threshold, med_level above med_threshold (the current level if
+    med_level <= 0), base_level otherwise; +1 / -1 / 0 as the element level is
+    below / above / at the target."""
+
+    def __init__(self, base_level=1, med_level=None, med_threshold=0.0, max_level=None, max_threshold=1.0):
+        self.base_level = int(base_level)
+        self.med_level = int(base_level if med_level is None else med_level)
+        self.max_level = int(base_level if max_level is None else max_level)
+        self.med_threshold = float(med_threshold)
+        self.max_threshold = float(max_threshold)
+
+    def __call__(self, alpha, levels):
+        alpha = np.asarray(alpha, dtype=np.float64)
+        levels = np.asarray(levels)
+        target = np.where(alpha > self.max_threshold, self.max_level,
+                          np.where(alpha > self.med_threshold,
+                                   self.med_level if self.med_level > 0 else levels, self.base_level))
+        return np.sign(target - levels).astype(np.int32)

@@ -22,6 +22,7 @@
 #include <unordered_map>
 #include <vector>
 #include <algorithm>
+#include <array>
 #include <type_traits>
 #include <omp.h>
 
@@ -529,7 +530,8 @@ enum {
     IC_WEAK_BLAST_WAVE = 2,
     IC_ISENTROPIC_VORTEX = 3,
     IC_KELVIN_HELMHOLTZ = 4,
-    IC_FOURIER_MODES = 5
+    IC_FOURIER_MODES = 5,
+    IC_DENSITY_PULSE = 6
 };
 
 struct oracle_config {
@@ -557,6 +559,19 @@ struct oracle_ctx;
 
 }  // extern "C"
 
+// A leaf cell of the TreeMesh: level, integer coordinates at that level, centre.
+struct TreeCell {
+    int level, ic[3];
+    double center[3];
+};
+
+// Where a leaf comes from in an adaptation: kind 0 = the same cell as leaf src
+// before; kind 1 + q = child q of leaf src; kind -1 = the parent of the 2^D
+// consecutive leaves src .. src + 2^D - 1.
+struct Origin {
+    int kind, src;
+};
+
 struct oracle_ctx {
     oracle_config cfg;
     int D, n, NN, nf, V, E, I;
@@ -577,6 +592,9 @@ struct oracle_ctx {
     std::vector<int32_t> mortar_side;     // [M] large_sides: 1 = large element on the left
     int M = 0;
     std::vector<int32_t> levels;          // [E] tree level of each element
+    std::vector<TreeCell> leaves;         // [E] leaf cells in element (depth-first) order
+    std::vector<double> u_adapted;        // solution projected by the last adaptation
+    std::vector<Origin> transfer;         // [E] origin of each element in the last adaptation
     std::vector<double> sfv;            // surface_flux_values
     std::vector<double> alpha, alpha_tmp;
     // analysis exact-solution parameters
@@ -683,6 +701,15 @@ void initial_condition(const oracle_ctx &c, const Euler<D> &eq, const double *x,
             double v2 = 0.1 * std::sin(M_PI * 2 * x[0]);
             double w[4] = {rho, v1, v2, 1.0};
             eq.prim2cons(w, u);
+            break;
+        }
+        case IC_DENSITY_PULSE: {
+            // examples/tree_3d_dgsem/elixir_euler_amr.jl:15-26: a Gaussian density
+            // pulse advected with v = 1 in every direction at p = 1 (3D)
+            double rho = 1 + std::exp(-(x[0] * x[0] + x[1] * x[1] + (D == 3 ? x[2] * x[2] : 0.0))) / 2;
+            u[0] = rho;
+            for (int d = 0; d < D; ++d) u[1 + d] = rho;
+            u[V - 1] = 1 / (eq.gamma - 1) + 1.0 / 2 * rho * D;
             break;
         }
         case IC_FOURIER_MODES: {
@@ -1239,62 +1266,26 @@ template <int D> void error_norms(const oracle_ctx &c, const double *u, double t
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
-int oracle_get_threads(void) { return omp_get_max_threads(); }
-
-oracle_ctx *oracle_create(const oracle_config *cfg) {
-    if (!cfg || (cfg->ndims != 2 && cfg->ndims != 3) || cfg->polydeg < 1 || cfg->polydeg > 15)
-        return nullptr;
-    oracle_ctx *c = new oracle_ctx();
-    c->cfg = *cfg;
-    c->D = cfg->ndims;
-    c->n = cfg->polydeg + 1;
-    c->NN = ipow(c->n, c->D);
-    c->nf = ipow(c->n, c->D - 1);
-    c->V = c->D + 2;
-    c->basis.init(cfg->polydeg);
-    // TreeMesh(coordinates_min, coordinates_max; initial_refinement_level,
-    // refinement_patches, periodicity), tree_mesh.jl:128-215: uniform refinement,
-    // then every box patch refines the leaf cells whose centre lies strictly
-    // inside the box (refine_box!, abstract_tree.jl:405-418), rebalancing to
-    // 2:1 (refine!, rebalance!, :356-490). Children are inserted right behind
-    // their parent, so the leaves stay in depth-first order (:317-353).
-    double center[3];
-    for (int d = 0; d < c->D; ++d) center[d] = (cfg->coordinates_min[d] + cfg->coordinates_max[d]) / 2;
-    double length = cfg->coordinates_max[0] - cfg->coordinates_min[0];
-    c->domain_length = length;
-    const int D = c->D;
-    struct Cell {
-        int level, ic[3];
-        double center[3];
-    };
-    std::vector<Cell> leaves;
-    {
-        std::vector<double> cen, len;
-        std::vector<int> icoords;
-        int ic0[3] = {0, 0, 0};
-        build_leaves(D, 0, cfg->initial_refinement_level, center, length, ic0, cen, len, icoords);
-        for (size_t e = 0; e < len.size(); ++e) {
-            Cell cl{cfg->initial_refinement_level, {0, 0, 0}, {0, 0, 0}};
-            for (int d = 0; d < D; ++d) {
-                cl.ic[d] = icoords[D * e + d];
-                cl.center[d] = cen[D * e + d];
-            }
-            leaves.push_back(cl);
-        }
-    }
-    auto key = [&](int level, const int *ic) -> long long {
+// Cells of the tree addressed by (level, integer coordinates at that level).
+// The map holds every cell of the tree: a leaf maps to its element index, an
+// inner cell (one with children) to -1 (abstract_tree.jl: has_children,
+// has_neighbor, neighbor_ids at the same level).
+struct TreeIndex {
+    int D;
+    const int *periodic;
+    std::unordered_map<long long, int> cells;
+    TreeIndex(int D_, const int *periodic_) : D(D_), periodic(periodic_) {}
+    long long key(int level, const int *ic) const {
         long long k = level;
         for (int d = 0; d < D; ++d) k |= (long long)ic[d] << (6 + 19 * d);
         return k;
-    };
-    std::unordered_map<long long, int> cells;  // leaf -> element index, inner cell -> -1
-    auto rebuild = [&]() {
+    }
+    void rebuild(const std::vector<TreeCell> &leaves) {
         cells.clear();
         for (size_t i = 0; i < leaves.size(); ++i) {
-            const Cell &cl = leaves[i];
+            const TreeCell &cl = leaves[i];
             cells[key(cl.level, cl.ic)] = (int)i;
             int ic[3] = {cl.ic[0], cl.ic[1], cl.ic[2]};
             for (int l = cl.level - 1; l >= 0; --l) {
@@ -1302,86 +1293,187 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
                 if (!cells.emplace(key(l, ic), -1).second) break;
             }
         }
-    };
-    auto find = [&](int level, const int *ic) -> const int * {
+    }
+    const int *find(int level, const int *ic) const {
         auto it = cells.find(key(level, ic));
         return it == cells.end() ? nullptr : &it->second;
-    };
-    // same-level neighbour coordinates across face dir, wrapping periodic dimensions
-    auto neighbour = [&](int level, const int *ic, int dir, int *out) -> bool {
+    }
+    // same-level cell coordinates across face dir (0-based: 2o = -o, 2o+1 = +o),
+    // wrapping periodic dimensions; false outside a non-periodic domain
+    bool neighbour(int level, const int *ic, int dir, int *out) const {
         const int o = dir / 2, nside = 1 << level;
         for (int d = 0; d < D; ++d) out[d] = ic[d];
         int k = ic[o] + (dir % 2 ? 1 : -1);
-        if ((k < 0 || k >= nside) && !cfg->periodicity[o]) return false;
+        if ((k < 0 || k >= nside) && !periodic[o]) return false;
         out[o] = (k + nside) % nside;
         return true;
-    };
-    auto refine = [&](const std::vector<char> &mark) {
-        std::vector<Cell> out;
-        for (size_t i = 0; i < leaves.size(); ++i) {
-            const Cell &p = leaves[i];
-            if (!mark[i]) {
-                out.push_back(p);
-                continue;
-            }
-            const double child_length = length / (1 << p.level) / 2;
-            for (int child = 0; child < (1 << D); ++child) {
-                Cell ch{p.level + 1, {0, 0, 0}, {0, 0, 0}};
-                for (int d = 0; d < D; ++d) {
-                    const int bit = (child >> d) & 1;  // child_sign, abstract_tree.jl:118-128
-                    ch.center[d] = p.center[d] + (bit ? 1 : -1) * child_length / 2;
-                    ch.ic[d] = 2 * p.ic[d] + bit;
-                }
-                out.push_back(ch);
-            }
+    }
+};
+
+// refine_unbalanced!, abstract_tree.jl:317-353: every marked leaf is replaced
+// by its 2^D children (child index bits: x = 1, y = 2, z = 4; child_sign
+// :118-128, child coordinates :668-674), inserted in place (depth-first).
+void refine_marked(int D, double root_length, std::vector<TreeCell> &leaves, std::vector<Origin> &orig,
+                   const std::vector<char> &mark) {
+    std::vector<TreeCell> out;
+    std::vector<Origin> oout;
+    for (size_t i = 0; i < leaves.size(); ++i) {
+        const TreeCell &p = leaves[i];
+        if (!mark[i]) {
+            out.push_back(p);
+            oout.push_back(orig[i]);
+            continue;
         }
-        leaves.swap(out);
-        rebuild();
-    };
-    rebuild();
-    for (int pt = 0; pt < cfg->npatches; ++pt) {
-        std::vector<char> mark(leaves.size(), 0);
-        std::vector<Cell> refined;
-        for (size_t i = 0; i < leaves.size(); ++i) {
-            bool inside = true;
-            for (int d = 0; d < D; ++d)
-                inside = inside && cfg->patch_min[pt][d] < leaves[i].center[d] && cfg->patch_max[pt][d] > leaves[i].center[d];
-            if (inside) {
-                mark[i] = 1;
-                refined.push_back(leaves[i]);
+        const double child_length = root_length / (1 << p.level) / 2;
+        for (int child = 0; child < (1 << D); ++child) {
+            TreeCell ch{p.level + 1, {0, 0, 0}, {0, 0, 0}};
+            for (int d = 0; d < D; ++d) {
+                const int bit = (child >> d) & 1;
+                ch.center[d] = p.center[d] + (bit ? 1 : -1) * child_length / 2;
+                ch.ic[d] = 2 * p.ic[d] + bit;
             }
-        }
-        refine(mark);
-        while (!refined.empty()) {  // rebalance!: coarse neighbours of refined cells
-            std::vector<long long> want;
-            for (const Cell &p : refined)
-                for (int dir = 0; dir < 2 * D; ++dir) {
-                    int nb[3];
-                    if (neighbour(p.level, p.ic, dir, nb) && find(p.level, nb)) continue;  // has_neighbor
-                    if (p.level == 0) continue;
-                    int par[3], pnb[3];
-                    for (int d = 0; d < D; ++d) par[d] = p.ic[d] >> 1;
-                    if (!neighbour(p.level - 1, par, dir, pnb) || !find(p.level - 1, pnb)) continue;
-                    want.push_back(key(p.level - 1, pnb));
-                }
-            std::vector<char> m2(leaves.size(), 0);
-            refined.clear();
-            for (size_t i = 0; i < leaves.size(); ++i)
-                for (long long k : want)
-                    if (k == key(leaves[i].level, leaves[i].ic) && !m2[i]) {
-                        m2[i] = 1;
-                        refined.push_back(leaves[i]);
-                    }
-            if (!refined.empty()) refine(m2);
+            out.push_back(ch);
+            oout.push_back(Origin{1 + child, orig[i].src});
         }
     }
-    c->E = (int)leaves.size();
+    leaves.swap(out);
+    orig.swap(oout);
+}
+
+// refine! + rebalance!, abstract_tree.jl:367-400, 453-490: refine the marked
+// leaves, then, as long as cells were refined, refine every coarse neighbour
+// (the parent's face neighbour) of a refined cell that has no same-level face
+// neighbour (Cartesian neighbours only). Returns the number of refined cells.
+int refine_balanced(int D, double root_length, TreeIndex &idx, std::vector<TreeCell> &leaves,
+                    std::vector<Origin> &orig, std::vector<char> mark) {
+    int count = 0;
+    while (true) {
+        std::vector<TreeCell> refined;
+        for (size_t i = 0; i < leaves.size(); ++i)
+            if (mark[i]) refined.push_back(leaves[i]);
+        if (refined.empty()) break;
+        count += (int)refined.size();
+        refine_marked(D, root_length, leaves, orig, mark);
+        idx.rebuild(leaves);
+        mark.assign(leaves.size(), 0);
+        for (const TreeCell &p : refined)
+            for (int dir = 0; dir < 2 * D; ++dir) {
+                int nb[3];
+                if (idx.neighbour(p.level, p.ic, dir, nb) && idx.find(p.level, nb)) continue;  // has_neighbor
+                if (p.level == 0) continue;
+                int par[3] = {0, 0, 0}, pnb[3];
+                for (int d = 0; d < D; ++d) par[d] = p.ic[d] >> 1;
+                if (!idx.neighbour(p.level - 1, par, dir, pnb)) continue;
+                const int *f = idx.find(p.level - 1, pnb);  // has_coarse_neighbor
+                if (f && *f >= 0) mark[*f] = 1;
+            }
+    }
+    return count;
+}
+
+// AMRCallback coarsening (amr.jl:304-370) and coarsen!(tree, cell_ids)
+// (abstract_tree.jl:516-600): a parent is coarsened when all 2^D children are
+// leaves marked for coarsening, unless a child has a non-sibling same-level
+// face neighbour with children (that would break the 2:1 balance). Parents
+// are taken from the highest level down, each check on the tree as already
+// coarsened. Returns the number of coarsened parents.
+int coarsen_marked(int D, double root_length, TreeIndex &idx, std::vector<TreeCell> &leaves,
+                   std::vector<Origin> &orig, const std::vector<long long> &leaf_keys) {
+    const int nch = 1 << D;
+    std::unordered_map<long long, int> count;
+    std::vector<std::pair<int, std::array<int, 3>>> parents;
+    for (long long k : leaf_keys) {
+        const int level = (int)(k & 63);
+        if (level == 0) continue;  // has_parent
+        int ic[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) ic[d] = (int)((k >> (6 + 19 * d)) & ((1 << 19) - 1));
+        const int *f = idx.find(level, ic);
+        if (!f || *f < 0) continue;  // is_leaf (a cell refined by rebalancing is skipped)
+        std::array<int, 3> pic{{0, 0, 0}};
+        for (int d = 0; d < D; ++d) pic[d] = ic[d] >> 1;
+        if (++count[idx.key(level - 1, pic.data())] == nch) parents.push_back({level - 1, pic});
+    }
+    std::stable_sort(parents.begin(), parents.end(),
+                     [](const std::pair<int, std::array<int, 3>> &a, const std::pair<int, std::array<int, 3>> &b) {
+                         return a.first > b.first;
+                     });
+    std::unordered_map<long long, char> coarsened;
+    for (const auto &pc : parents) {
+        const int level = pc.first;
+        const int *pic = pc.second.data();
+        bool skip = false;
+        for (int q = 0; q < nch && !skip; ++q) {
+            int ch[3] = {0, 0, 0};
+            for (int d = 0; d < D; ++d) ch[d] = 2 * pic[d] + ((q >> d) & 1);
+            for (int dir = 0; dir < 2 * D; ++dir) {
+                const int bit = (q >> (dir / 2)) & 1;
+                if ((dir % 2 == 0 && bit == 1) || (dir % 2 == 1 && bit == 0)) continue;  // has_sibling
+                int nb[3];
+                if (!idx.neighbour(level + 1, ch, dir, nb)) continue;
+                const int *f = idx.find(level + 1, nb);
+                if (f && *f < 0) {  // neighbour with children
+                    skip = true;
+                    break;
+                }
+            }
+        }
+        if (skip) continue;
+        for (int q = 0; q < nch; ++q) {
+            int ch[3] = {0, 0, 0};
+            for (int d = 0; d < D; ++d) ch[d] = 2 * pic[d] + ((q >> d) & 1);
+            idx.cells.erase(idx.key(level + 1, ch));
+        }
+        const long long pk = idx.key(level, pic);
+        idx.cells[pk] = 0;  // now a leaf
+        coarsened[pk] = 1;
+    }
+    if (coarsened.empty()) return 0;
+    std::vector<TreeCell> out;
+    std::vector<Origin> oout;
+    for (size_t i = 0; i < leaves.size(); ++i) {
+        const TreeCell &cl = leaves[i];
+        if (cl.level > 0) {
+            int pic[3] = {0, 0, 0};
+            for (int d = 0; d < D; ++d) pic[d] = cl.ic[d] >> 1;
+            if (coarsened.count(idx.key(cl.level - 1, pic))) {
+                // the children follow their parent in depth-first order, child 0 first
+                const double child_length = root_length / (1 << cl.level);
+                TreeCell p{cl.level - 1, {pic[0], pic[1], pic[2]}, {0, 0, 0}};
+                for (int d = 0; d < D; ++d) p.center[d] = cl.center[d] + child_length / 2;
+                out.push_back(p);
+                oout.push_back(Origin{-1, orig[i].src});
+                i += nch - 1;
+                continue;
+            }
+        }
+        out.push_back(cl);
+        oout.push_back(orig[i]);
+    }
+    leaves.swap(out);
+    orig.swap(oout);
+    idx.rebuild(leaves);
+    return (int)coarsened.size();
+}
+
+// Everything the discretisation needs from the leaves: element geometry,
+// interfaces, L2 mortars and boundaries. False if the tree is not 2:1
+// balanced or has boundaries without a boundary condition.
+bool finalize_mesh(oracle_ctx *c) {
+    const int D = c->D;
+    TreeIndex idx(D, c->cfg.periodicity);
+    idx.rebuild(c->leaves);
+    c->mortar_ids.clear();
+    c->mortar_orient.clear();
+    c->mortar_side.clear();
+    c->bnd_elem.clear();
+    c->bnd_dir.clear();
+    c->E = (int)c->leaves.size();
     c->centers.clear();
     c->lengths.clear();
     c->levels.clear();
-    for (const Cell &cl : leaves) {
+    for (const TreeCell &cl : c->leaves) {
         for (int d = 0; d < D; ++d) c->centers.push_back(cl.center[d]);
-        c->lengths.push_back(length / (1 << cl.level));
+        c->lengths.push_back(c->domain_length / (1 << cl.level));
         c->levels.push_back(cl.level);
     }
     // neighbours, mortars, boundaries (init_neighbors!, init_interfaces!,
@@ -1394,14 +1486,14 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
     std::vector<char> is_boundary(2 * D * c->E, 0);
     const int ns = 1 << (D - 1);
     for (int e = 0; e < c->E; ++e) {
-        const Cell &cl = leaves[e];
+        const TreeCell &cl = c->leaves[e];
         for (int dir = 0; dir < 2 * D; ++dir) {
             int nb[3];
-            if (!neighbour(cl.level, cl.ic, dir, nb)) {
+            if (!idx.neighbour(cl.level, cl.ic, dir, nb)) {
                 is_boundary[dir + 2 * D * e] = 1;
                 continue;
             }
-            const int *f = find(cl.level, nb);
+            const int *f = idx.find(cl.level, nb);
             if (!f) continue;  // small side of a mortar
             if (*f >= 0) {
                 c->elem_nbr[dir + 2 * D * e] = *f;
@@ -1417,7 +1509,7 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
                 ch[o] += plus ? 0 : 1;
                 ch[fd[0]] += q & 1;
                 if (D == 3) ch[fd[1]] += q >> 1;
-                const int *s = find(cl.level + 1, ch);
+                const int *s = idx.find(cl.level + 1, ch);
                 c->mortar_ids.push_back(s ? *s : -1);
             }
             c->mortar_ids.push_back(e);
@@ -1427,10 +1519,7 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
     }
     c->M = (int)c->mortar_orient.size();
     for (int32_t id : c->mortar_ids)
-        if (id < 0) {  // a mortar partner that is not a leaf: the tree is not 2:1 balanced
-            delete c;
-            return nullptr;
-        }
+        if (id < 0) return false;  // a mortar partner that is not a leaf: the tree is not 2:1 balanced
     // init_boundaries!: containers_2d.jl:475-543 (direction-major, element order)
     for (int dir = 0; dir < 2 * D; ++dir)
         for (int e = 0; e < c->E; ++e)
@@ -1438,10 +1527,7 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
                 c->bnd_elem.push_back(e);
                 c->bnd_dir.push_back(dir);
             }
-    if (!c->bnd_elem.empty() && cfg->boundary_condition != BC_DIRICHLET_IC) {
-        delete c;
-        return nullptr;
-    }
+    if (!c->bnd_elem.empty() && c->cfg.boundary_condition != BC_DIRICHLET_IC) return false;
     // init_elements!: containers_3d.jl:87-128
     c->inverse_jacobian.resize(c->E);
     c->node_coords.resize((size_t)c->D * c->NN * c->E);
@@ -1472,10 +1558,177 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
     c->I = (int)c->orientations.size();
     c->sfv.assign((size_t)c->V * c->nf * 2 * c->D * c->E, 0.0);
     c->alpha.assign(c->E, 0.0);
+    return true;
+}
+
+// Solution transfer of the L2 adaptor (LobattoLegendreAdaptorL2,
+// basis_lobatto_legendre.jl:334-360, the mortar matrices): a child q of a
+// refined element is the tensor interpolation forward_{lower,upper} per
+// dimension (bit 0 / 1 of q), refine_element!, amr_dg3d.jl:138-230 (2D:
+// amr_dg2d.jl); a coarsened element is the sum over its 2^D children of the
+// tensor reverse_{lower,upper} projections, coarsen_elements!,
+// amr_dg3d.jl:337-430. Contractions in x, then y, then z
+// (multiply_dimensionwise!, interpolation.jl:297-420).
+void project_solution(const oracle_ctx &c, const double *u_old, const std::vector<Origin> &orig,
+                      std::vector<double> &u_new) {
+    const int D = c.D, n = c.n, NN = c.NN, V = c.V, nch = 1 << D;
+    const size_t per = (size_t)V * NN;
+    u_new.assign(per * orig.size(), 0.0);
+    auto apply = [&](const double *in, double *out, const int *which, bool forward, bool add) {
+        // which[d] = 0 lower, 1 upper
+        std::vector<double> a(in, in + per), b(per);
+        for (int d = 0; d < D; ++d) {
+            const std::vector<double> &M = forward ? (which[d] ? c.basis.fwd_upper : c.basis.fwd_lower)
+                                                   : (which[d] ? c.basis.rev_upper : c.basis.rev_lower);
+            const int stride = ipow(n, d);
+            for (int node = 0; node < NN; ++node) {
+                const int i = (node / stride) % n, base = node - i * stride;
+                for (int v = 0; v < V; ++v) {
+                    double r = 0;
+                    for (int ii = 0; ii < n; ++ii) r += M[i * n + ii] * a[v + V * (base + ii * stride)];
+                    b[v + V * node] = r;
+                }
+            }
+            a.swap(b);
+        }
+        for (size_t k = 0; k < per; ++k) out[k] = add ? out[k] + a[k] : a[k];
+    };
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < (long)orig.size(); ++e) {
+        const Origin o = orig[e];
+        double *out = &u_new[per * e];
+        if (o.kind == 0) {
+            std::copy(u_old + per * o.src, u_old + per * (o.src + 1), out);
+        } else if (o.kind > 0) {
+            const int q = o.kind - 1;
+            int which[3] = {q & 1, (q >> 1) & 1, (q >> 2) & 1};
+            apply(u_old + per * o.src, out, which, true, false);
+        } else {
+            for (int q = 0; q < nch; ++q) {
+                int which[3] = {q & 1, (q >> 1) & 1, (q >> 2) & 1};
+                apply(u_old + per * (o.src + q), out, which, false, q > 0);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+oracle_ctx *oracle_create(const oracle_config *cfg) {
+    if (!cfg || (cfg->ndims != 2 && cfg->ndims != 3) || cfg->polydeg < 1 || cfg->polydeg > 15)
+        return nullptr;
+    oracle_ctx *c = new oracle_ctx();
+    c->cfg = *cfg;
+    c->D = cfg->ndims;
+    c->n = cfg->polydeg + 1;
+    c->NN = ipow(c->n, c->D);
+    c->nf = ipow(c->n, c->D - 1);
+    c->V = c->D + 2;
+    c->basis.init(cfg->polydeg);
+    // TreeMesh(coordinates_min, coordinates_max; initial_refinement_level,
+    // refinement_patches, periodicity), tree_mesh.jl:128-215: uniform refinement,
+    // then every box patch refines the leaf cells whose centre lies strictly
+    // inside the box (refine_box!, abstract_tree.jl:405-418), rebalancing to
+    // 2:1 (refine!, rebalance!, :356-490). Children are inserted right behind
+    // their parent, so the leaves stay in depth-first order (:317-353).
+    const int D = c->D;
+    double center[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) center[d] = (cfg->coordinates_min[d] + cfg->coordinates_max[d]) / 2;
+    const double length = cfg->coordinates_max[0] - cfg->coordinates_min[0];
+    c->domain_length = length;
+    {
+        std::vector<double> cen, len;
+        std::vector<int> icoords;
+        int ic0[3] = {0, 0, 0};
+        build_leaves(D, 0, cfg->initial_refinement_level, center, length, ic0, cen, len, icoords);
+        for (size_t e = 0; e < len.size(); ++e) {
+            TreeCell cl{cfg->initial_refinement_level, {0, 0, 0}, {0, 0, 0}};
+            for (int d = 0; d < D; ++d) {
+                cl.ic[d] = icoords[D * e + d];
+                cl.center[d] = cen[D * e + d];
+            }
+            c->leaves.push_back(cl);
+        }
+    }
+    TreeIndex idx(D, c->cfg.periodicity);
+    idx.rebuild(c->leaves);
+    std::vector<Origin> orig;
+    for (int pt = 0; pt < cfg->npatches; ++pt) {
+        std::vector<char> mark(c->leaves.size(), 0);
+        for (size_t i = 0; i < c->leaves.size(); ++i) {
+            bool inside = true;
+            for (int d = 0; d < D; ++d)
+                inside = inside && cfg->patch_min[pt][d] < c->leaves[i].center[d] &&
+                         cfg->patch_max[pt][d] > c->leaves[i].center[d];
+            mark[i] = inside;
+        }
+        orig.assign(c->leaves.size(), Origin{0, 0});
+        refine_balanced(D, length, idx, c->leaves, orig, mark);
+    }
+    if (!finalize_mesh(c)) {
+        delete c;
+        return nullptr;
+    }
     return c;
 }
 
 void oracle_destroy(oracle_ctx *c) { delete c; }
+
+// IndicatorMax(semi; variable), indicators_3d.jl:239-259 (2D: indicators_2d.jl):
+// alpha[e] = maximum of conserved variable `variable` over the nodes of element e.
+void oracle_indicator_max(const oracle_ctx *c, const double *u, int variable, double *alpha) {
+    for (int e = 0; e < c->E; ++e) {
+        double m = u[variable + c->V * (size_t)c->NN * e];
+        for (int node = 1; node < c->NN; ++node) m = std::max(m, u[variable + c->V * (node + (size_t)c->NN * e)]);
+        alpha[e] = m;
+    }
+}
+
+// One AMRCallback adaptation of a TreeMesh (amr.jl:244-392) with the controller
+// values `lambda` (> 0 refine, < 0 coarsen, per element): refinement with 2:1
+// rebalancing, then coarsening of the parents whose children are all still
+// marked leaves. Returns 1 if the mesh changed, 0 if not, -1 if the adapted
+// tree could not be discretised. The projected solution is read back with
+// oracle_get_adapted_u and the element origins with oracle_get_transfer.
+int oracle_amr_adapt(oracle_ctx *c, const double *u, const int32_t *lambda, int only_refine, int only_coarsen) {
+    const int D = c->D;
+    TreeIndex idx(D, c->cfg.periodicity);
+    idx.rebuild(c->leaves);
+    std::vector<Origin> orig(c->E);
+    for (int e = 0; e < c->E; ++e) orig[e] = Origin{0, e};
+    std::vector<long long> to_coarsen;
+    std::vector<char> to_refine(c->E, 0);
+    bool any_refine = false;
+    for (int e = 0; e < c->E; ++e) {
+        if (lambda[e] > 0) to_refine[e] = 1, any_refine = true;
+        else if (lambda[e] < 0) to_coarsen.push_back(idx.key(c->leaves[e].level, c->leaves[e].ic));
+    }
+    int changed = 0;
+    if (!only_coarsen && any_refine)
+        changed += refine_balanced(D, c->domain_length, idx, c->leaves, orig, to_refine);
+    if (!only_refine && !to_coarsen.empty())
+        changed += coarsen_marked(D, c->domain_length, idx, c->leaves, orig, to_coarsen);
+    project_solution(*c, u, orig, c->u_adapted);
+    c->transfer = orig;
+    if (!changed) return 0;
+    return finalize_mesh(c) ? 1 : -1;
+}
+
+void oracle_get_adapted_u(const oracle_ctx *c, double *u) { std::copy(c->u_adapted.begin(), c->u_adapted.end(), u); }
+
+void oracle_get_transfer(const oracle_ctx *c, int32_t *kind, int32_t *src) {
+    for (size_t e = 0; e < c->transfer.size(); ++e) {
+        kind[e] = c->transfer[e].kind;
+        src[e] = c->transfer[e].src;
+    }
+}
+
+void oracle_get_levels(const oracle_ctx *c, int32_t *levels) { std::copy(c->levels.begin(), c->levels.end(), levels); }
 
 int oracle_nelements(const oracle_ctx *c) { return c->E; }
 int oracle_ninterfaces(const oracle_ctx *c) { return c->I; }

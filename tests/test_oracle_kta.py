@@ -51,3 +51,47 @@ def test_planted_boundary_mistake_is_caught(kta):
     u, t, _ = o.solve(o.compute_coefficients(0.0), 0.0, 2.0, c["cfl"])
     l2, _ = o.error_norms(u, t)
     assert np.max(np.abs(l2 / np.array(c["l2"]) - 1)) > 1e-6
+
+
+def _amr_run(c, controller_overrides=None):
+    from oracle import ControllerThreeLevel
+    a = dict(c["amr"], **(controller_overrides or {}))
+    o = make_oracle(c["oracle"])
+    o.set_initial_condition(c["ic"], c["ic_params"])
+    ctl = ControllerThreeLevel(base_level=a["base_level"], med_level=a["med_level"],
+                               med_threshold=a["med_threshold"], max_level=a["max_level"],
+                               max_threshold=a["max_threshold"])
+    u, t, nsteps, history = o.solve_amr(c["tspan"][0], c["tspan"][1], c["cfl"], ctl, interval=a["interval"],
+                                        adapt_initial_condition=a["adapt_initial_condition"],
+                                        adapt_initial_condition_only_refine=a["adapt_initial_condition_only_refine"],
+                                        variable=a["variable"])
+    return o, u, t, history
+
+
+def test_oracle_reproduces_trixi_amr_kta(kta):
+    """elixir_euler_amr.jl: the initial condition is adapted until the mesh
+    settles, then the mesh is refined and coarsened every 5 steps while the
+    pulse moves. The errors are dominated by the displacement of the pulse
+    (the exact solution is the unmoved initial condition), so mesh decisions
+    move them by as little as 3e-10 relative: rtol 1e-12 here."""
+    c = kta["amr_3d"]
+    o, u, t, history = _amr_run(c)
+    # initial adaptation (4096 -> ... settled), then 3 adaptations during the run
+    assert len(history) >= 5 and history[-1] == o.nelements
+    assert len(set(history[-3:])) > 1  # the run adapts the mesh (refinement and coarsening)
+    assert t == pytest.approx(c["tspan"][1], abs=0, rel=1e-15)
+    l2, linf = o.error_norms(u, t)
+    np.testing.assert_allclose(l2, c["l2"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(linf, c["linf"], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mistake", [{"interval": 0}, {"max_threshold": 2.0}, {"med_threshold": 1.2}])
+def test_planted_amr_mistake_is_caught(kta, mistake):
+    """No adaptation during the run, a controller that never reaches the
+    maximum level, or a shifted medium threshold each move the AMR errors
+    outside the rtol of the KTA test above."""
+    c = kta["amr_3d"]
+    o, u, t, _ = _amr_run(c, mistake)
+    l2, linf = o.error_norms(u, t)
+    worst = max(np.max(np.abs(l2 / np.array(c["l2"]) - 1)), np.max(np.abs(linf / np.array(c["linf"]) - 1)))
+    assert worst > 1e-11

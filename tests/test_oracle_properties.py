@@ -159,3 +159,112 @@ def test_mortar_l2_matrices(N):
         np.testing.assert_allclose(fl @ p, (0.5 * (x - 1)) ** k, atol=1e-13)
     np.testing.assert_allclose(ru @ fu + rl @ fl, np.eye(N + 1), atol=1e-13)
     assert np.abs(ru.T @ fu + rl.T @ fl - np.eye(N + 1)).max() > 1e-3 or N == 1
+
+
+# ---------------------------------------------------------------------------
+# AMR: solution transfer (L2 adaptor) and tree adaptation
+# ---------------------------------------------------------------------------
+def _poly(x, N, seed):
+    """A random polynomial of total degree <= N per variable at points x[..., D]."""
+    rng = np.random.default_rng(seed)
+    D = x.shape[-1]
+    out = np.zeros(x.shape[:-1] + (D + 2,))
+    for v in range(D + 2):
+        for _ in range(6):
+            p = rng.integers(0, N + 1, D)
+            while p.sum() > N:
+                p[np.argmax(p)] -= 1
+            out[..., v] += rng.normal() * np.prod(x ** p, axis=-1)
+    return out
+
+
+def _adapt_oracle(D, level=2, N=3, periodicity=True):
+    return make_oracle(spec(D, N, level, "weak", "ranocha", "llf", periodicity=periodicity))
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_amr_refine_interpolates_polynomials_exactly(D):
+    """refine_element!: the children hold the parent's polynomial, so a field of
+    degree <= N stays equal to the polynomial at the children's nodes (pins the
+    forward matrices and the child placement: bit d of the child index = upper half in d)."""
+    o = _adapt_oracle(D)
+    x = o.mesh()["node_coordinates"]
+    u = _poly(x, o.polydeg, 1).reshape(-1)
+    lam = np.zeros(o.nelements, np.int32)
+    lam[np.random.default_rng(2).choice(o.nelements, o.nelements // 3, replace=False)] = 1
+    changed, un, kind, src = o.amr_adapt(u, lam)
+    assert changed and (kind > 0).any()
+    xn = o.mesh()["node_coordinates"]
+    np.testing.assert_allclose(un, _poly(xn, o.polydeg, 1).reshape(-1), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_amr_coarsen_projects_polynomials_exactly_and_conserves(D):
+    """coarsen_elements!: the L2 projection of 2^D children holding one global
+    polynomial of degree <= N is that polynomial; for arbitrary data the
+    projection conserves the LGL-quadrature integral of every variable."""
+    o = _adapt_oracle(D)
+    u0 = np.zeros(o.nvalues)
+    _, _, _, _ = o.amr_adapt(u0, np.ones(o.nelements, np.int32))  # refine everything once
+    x = o.mesh()["node_coordinates"]
+    u = _poly(x, o.polydeg, 4).reshape(-1)
+    lam = -np.ones(o.nelements, np.int32)
+    lam[: 2 << D] = 0  # keep the first two parents refined
+    changed, un, kind, _ = o.amr_adapt(u, lam)
+    assert changed and (kind < 0).sum() == o.nelements - 2 * (1 << D) and (kind == 0).sum() == 2 * (1 << D)
+    np.testing.assert_allclose(un, _poly(o.mesh()["node_coordinates"], o.polydeg, 4).reshape(-1), rtol=0, atol=1e-12)
+    # conservation for random data: refine all, coarsen all
+    o = _adapt_oracle(D)
+    rng = np.random.default_rng(5)
+    _, ur, _, _ = o.amr_adapt(np.zeros(o.nvalues), np.ones(o.nelements, np.int32))
+    ur = rng.uniform(0.5, 2.0, o.nvalues)
+    before, _ = o.integrals(ur)
+    changed, uc, _, _ = o.amr_adapt(ur, -np.ones(o.nelements, np.int32))
+    after, _ = o.integrals(uc)
+    assert changed and o.nelements == 1 << (2 * D)
+    np.testing.assert_allclose(after, before, rtol=1e-13)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_amr_refine_then_coarsen_is_identity(D):
+    """reverse_{lower,upper} after forward_{lower,upper}, summed over the
+    children, is the identity (L2 projection of the parent's own polynomial)."""
+    o = _adapt_oracle(D)
+    u = np.random.default_rng(7).normal(size=o.nvalues)
+    lv0 = o.levels().copy()
+    _, ur, _, _ = o.amr_adapt(u, np.ones(o.nelements, np.int32))
+    _, uc, kind, src = o.amr_adapt(ur, -np.ones(o.nelements, np.int32))
+    np.testing.assert_array_equal(o.levels(), lv0)
+    np.testing.assert_array_equal(src, np.arange(o.nelements) << D)
+    np.testing.assert_allclose(uc, u, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("D,periodic", [(2, True), (2, False), (3, True), (3, False)])
+def test_amr_random_adaptations_stay_balanced(D, periodic):
+    """Random controller values over many adaptations: the rebalancing and the
+    coarsening guard keep the tree 2:1 balanced (the oracle raises otherwise),
+    the leaves tile the domain, and a coarsened parent is never next to a cell
+    two levels finer."""
+    o = _adapt_oracle(D, level=2, periodicity=periodic)
+    if not periodic:
+        o.set_initial_condition("constant")
+    rng = np.random.default_rng(11 + D)
+    u = np.zeros(o.nvalues)
+    vol0 = 2.0 ** D
+    for it in range(12):
+        lv = o.levels()
+        lam = rng.integers(-1, 2, o.nelements).astype(np.int32)
+        lam[lv >= 5] = np.minimum(lam[lv >= 5], 0)
+        _, u, kind, src = o.amr_adapt(u, lam)
+        lv = o.levels()
+        ij = o.mesh()["inverse_jacobian"]
+        h = 2.0 / ij  # LGL reference length 2
+        assert np.isclose(np.sum(h ** D), vol0)
+        # transfer consistency: copies keep the level, children are one finer, parents one coarser
+        nb = o.mesh()["element_neighbors"]
+        ok = nb >= 0
+        e_idx = np.repeat(np.arange(o.nelements)[:, None], 2 * D, axis=1)
+        assert np.all(np.abs(lv[e_idx[ok]] - lv[nb[ok]]) == 0)  # same-level interfaces
+        ids, _, _ = o.mortars()
+        if len(ids):
+            assert np.all(lv[ids[:, :-1]] == lv[ids[:, -1:]] + 1)
