@@ -73,6 +73,7 @@ extern "C" {
 #define TRIXI_HIP_IC_ISENTROPIC_VORTEX 3
 #define TRIXI_HIP_IC_KELVIN_HELMHOLTZ 4
 #define TRIXI_HIP_IC_FOURIER_MODES 5
+#define TRIXI_HIP_IC_DENSITY_PULSE 6 /* elixir_euler_amr.jl:15-26 */
 
 /* indicator modes for trixi_hip_indicator / trixi_hip_set_indicator */
 #define TRIXI_HIP_IND_HG 0                /* IndicatorHennemannGassner */
@@ -216,6 +217,27 @@ int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt);
 /* Blending factors alpha[E] of IndicatorHennemannGassner (indicators.jl:119-156,
  * indicators_{2,3}d.jl) for the resident state, after smoothing. */
 int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host);
+
+/* IndicatorMax(semi; variable) (indicators_3d.jl:239-259, indicators_2d.jl):
+ * alpha_host[e] = maximum over the nodes of element e of conserved variable
+ * `variable` (0 = rho, as `variable = first`) of the resident state. The AMR
+ * controller (ControllerThreeLevel, amr.jl:1116-1153) reads it on the host. */
+int trixi_hip_indicator_max(trixi_hip_ctx *ctx, int variable, double *alpha_host);
+
+/* Solution transfer of one AMR adaptation (refine!/coarsen! of the DG solver,
+ * amr_dg.jl and amr_dg3d.jl:138-230, 337-430, with LobattoLegendreAdaptorL2,
+ * basis_lobatto_legendre.jl:334-360). `to` is a context created on the adapted
+ * mesh (same ndims and polydeg, same device); its resident state is set from
+ * the resident state of `from`, element by element of `to` (host arrays of
+ * to's nelements):
+ *   kind[e] = 0       copy of element src[e] of `from`;
+ *   kind[e] = 1 + q   child q of element src[e] (bit d of q = upper half in
+ *                     dimension d: forward_upper, else forward_lower);
+ *   kind[e] = -1      L2 projection of the 2^D elements src[e] .. src[e]+2^D-1
+ *                     (children in child order: reverse_upper / reverse_lower).
+ * The matrices are the mortar matrices `to` was created with. Synchronizes
+ * both contexts' streams; `from` may be destroyed afterwards. */
+int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *kind, const int32_t *src);
 
 /* Learned troubled-cell indicator (2D polydeg 3 contexts). No counterpart in
  * Trixi.jl's source tree (its neural-network indicators were moved to a

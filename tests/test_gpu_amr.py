@@ -1,0 +1,189 @@
+"""Adaptive mesh refinement with the device doing the per-node work:
+IndicatorMax and the L2 solution transfer against the oracle, and the
+reference's AMR known-answer run (elixir_euler_amr.jl) end to end."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from gpu_helpers import build, oracle_for, relerr
+
+torch = pytest.importorskip("torch")
+
+
+def upload(semi, u):
+    assert semi._lib.trixi_hip_upload_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p)) == 0
+
+
+def download(semi):
+    u = np.empty(semi.nvalues())
+    assert semi._lib.trixi_hip_download_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p)) == 0
+    return u
+
+
+def _spec(D, patches=True, periodic=True):
+    cmin, cmax = (0.0,) * D, (2.0,) * D
+    box = [((0.5,) * D, (1.5,) * D)] if patches else []
+    return dict(ndims=D, polydeg=3, level=2, cmin=cmin, cmax=cmax, volume_integral="weak", surface_flux="llf",
+                ic="fourier_modes" if D == 3 else "kelvin_helmholtz", refinement_patches=box, periodicity=periodic)
+
+
+def _state(semi, seed):
+    u = semi.compute_coefficients(0.0)
+    return u * (1 + 0.05 * np.random.default_rng(seed).standard_normal(u.shape))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,precision", [(2, "fp64"), (3, "fp64"), (3, "fp32")])
+def test_indicator_max_matches_oracle(D, precision):
+    """IndicatorMax(variable = first) and of the energy: maxima are exact."""
+    import trixi_hip as th
+    spec = _spec(D)
+    print(f"indicator_max D={D} {precision}", flush=True)
+    semi, _ = build(spec, precision)
+    o = oracle_for(spec)
+    u = _state(semi, 3)
+    upload(semi, u)
+    for var in (th.first, D + 1):
+        alpha = th.IndicatorMax(semi, variable=var)(semi)
+        np.testing.assert_array_equal(alpha, o.indicator_max(u, 0 if var is th.first else var))
+    semi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,periodic,precision", [(2, True, "fp64"), (2, False, "fp64"), (3, True, "fp64"),
+                                                  (3, False, "fp32")])
+def test_transfer_matches_oracle(D, periodic, precision):
+    """Random controller values, three adaptations in a row: the device's
+    projected state equals the oracle's (refine: forward interpolation; coarsen:
+    L2 projection of the 2^D children) to FP64 round-off, for FP64 and FP32
+    contexts (the FP64 register is what is projected)."""
+    spec = _spec(D, periodic=periodic)
+    print(f"transfer D={D} periodic={periodic} {precision}", flush=True)
+    semi, _ = build(spec, precision)
+    o = oracle_for(spec)
+    u = _state(semi, 5)
+    upload(semi, u)
+    rng = np.random.default_rng(17 + D)
+    for it in range(3):
+        lam = rng.integers(-1, 2, semi.nelements()).astype(np.int32)
+        lam[semi.mesh.levels >= 4] = np.minimum(lam[semi.mesh.levels >= 4], 0)
+        changed, kind, src = semi.mesh.adapt(lam)
+        assert changed
+        semi.remesh(kind, src)
+        _, u_ref, okind, osrc = o.amr_adapt(u, lam)
+        np.testing.assert_array_equal(kind, okind)
+        np.testing.assert_array_equal(src, osrc)
+        u = download(semi)
+        assert u.size == u_ref.size
+        assert relerr(u, u_ref) < 1e-14
+        u = u_ref
+    semi.close()
+
+
+@pytest.mark.gpu
+def test_transferred_fp32_state_steps_like_an_uploaded_one():
+    """After a transfer into an FP32 context the FP32 shadow matches the FP64
+    register: one RK step from the transferred state equals one RK step from
+    the same state uploaded into a fresh context on the same mesh (bitwise)."""
+    import trixi_hip as th
+    spec = _spec(3)
+    semi, _ = build(spec, "fp32")
+    upload(semi, _state(semi, 9))
+    lam = np.zeros(semi.nelements(), np.int32)
+    lam[::7] = 1
+    _, kind, src = semi.mesh.adapt(lam)
+    semi.remesh(kind, src)
+    u = download(semi)
+    fresh = th.SemidiscretizationHyperbolic(semi.mesh, semi.equations, semi.initial_condition, semi.solver,
+                                            precision="fp32")
+    upload(fresh, u)
+    alg = th.CarpenterKennedy2N54()
+    a = (ctypes.c_double * 5)(*alg.a)
+    b = (ctypes.c_double * 5)(*alg.b)
+    c = (ctypes.c_double * 5)(*alg.c)
+    for s in (semi, fresh):
+        assert s._lib.trixi_hip_rk_step(s.ctx, a, b, c, 5, 0.0, 1e-3, 1) == 0
+        assert s._lib.trixi_hip_synchronize(s.ctx) == 0
+    assert np.array_equal(download(semi), download(fresh))
+    semi.close()
+    fresh.close()
+
+
+@pytest.mark.gpu
+def test_transfer_rejects_out_of_range_maps():
+    """A map entry that would read outside the old context is refused on the
+    host (no launch), as are contexts of another polydeg."""
+    import trixi_hip as th
+    spec = _spec(2, patches=False)
+    semi, _ = build(spec, "fp64")
+    other, _ = build(dict(spec, level=3), "fp64")
+    lib = semi._lib
+    E_new = other.nelements()
+    for kind_val, src_val in ((0, semi.nelements()), (-1, semi.nelements() - 3), (5, 0), (0, -1)):
+        kind = np.zeros(E_new, np.int32)
+        src = np.zeros(E_new, np.int32)
+        kind[-1], src[-1] = kind_val, src_val
+        rc = lib.trixi_hip_transfer(other.ctx, semi.ctx, kind.ctypes.data_as(ctypes.c_void_p),
+                                    src.ctypes.data_as(ctypes.c_void_p))
+        assert rc == th._lib.EINVAL
+    p4, _ = build(dict(spec, polydeg=4), "fp64")
+    kind = np.zeros(p4.nelements(), np.int32)
+    src = np.arange(p4.nelements(), dtype=np.int32)
+    assert lib.trixi_hip_transfer(p4.ctx, semi.ctx, kind.ctypes.data_as(ctypes.c_void_p),
+                                  src.ctypes.data_as(ctypes.c_void_p)) == th._lib.EINVAL
+    for s in (semi, other, p4):
+        s.close()
+
+
+def run_amr_kta(precision):
+    import trixi_hip as th
+    c = golden("kta.json")["amr_3d"]
+    a = c["amr"]
+    spec = dict(c["oracle"], ic=c["ic"], ic_params=tuple(c["ic_params"]))
+    semi, _ = build(spec, precision)
+    ode = th.semidiscretize(semi, tuple(c["tspan"]))
+    ind = th.IndicatorMax(semi, variable=th.first)
+    ctl = th.ControllerThreeLevel(semi, ind, base_level=a["base_level"], med_level=a["med_level"],
+                                  med_threshold=a["med_threshold"], max_level=a["max_level"],
+                                  max_threshold=a["max_threshold"])
+    amr = th.AMRCallback(semi, ctl, interval=a["interval"], adapt_initial_condition=a["adapt_initial_condition"],
+                         adapt_initial_condition_only_refine=a["adapt_initial_condition_only_refine"])
+    ac = th.AnalysisCallback(semi, interval=100)
+    sol = th.solve(ode, th.CarpenterKennedy2N54(),
+                   callback=th.CallbackSet(ac, amr, th.StepsizeCallback(cfl=c["cfl"])))
+    semi.close()
+    return c, sol, amr
+
+
+@pytest.mark.gpu
+def test_amr_kta_on_gpu_fp64():
+    """elixir_euler_amr.jl on the device: the same sequence of adapted meshes
+    as the oracle (which reproduces the reference to 2e-15) and the
+    reference's final errors. The errors are dominated by the displacement of
+    the pulse, so the element-count history is what pins the mesh decisions."""
+    from oracle import ControllerThreeLevel
+    print("amr kta fp64", flush=True)
+    c, sol, amr = run_amr_kta("fp64")
+    a = c["amr"]
+    o = oracle_for(dict(c["oracle"]))
+    o.set_initial_condition(c["ic"])
+    ctl = ControllerThreeLevel(base_level=a["base_level"], med_level=a["med_level"], med_threshold=a["med_threshold"],
+                               max_level=a["max_level"], max_threshold=a["max_threshold"])
+    u_ref, t_ref, n_ref, hist_ref = o.solve_amr(c["tspan"][0], c["tspan"][1], c["cfl"], ctl, interval=a["interval"])
+    assert amr.nelements_history == hist_ref
+    assert sol.nsteps == n_ref and sol.t == pytest.approx(t_ref, rel=1e-15, abs=0)
+    assert relerr(sol.u, u_ref) < 1e-12
+    l2, linf = np.array(sol.analysis[-1]["l2"]), np.array(sol.analysis[-1]["linf"])
+    np.testing.assert_allclose(l2, c["l2"], rtol=1e-11, atol=0)
+    np.testing.assert_allclose(linf, c["linf"], rtol=1e-11, atol=0)
+
+
+@pytest.mark.gpu
+def test_amr_kta_on_gpu_fp32():
+    print("amr kta fp32", flush=True)
+    c, sol, amr = run_amr_kta("fp32")
+    l2, linf = np.array(sol.analysis[-1]["l2"]), np.array(sol.analysis[-1]["linf"])
+    np.testing.assert_allclose(l2, c["l2"], rtol=2e-4)
+    np.testing.assert_allclose(linf, c["linf"], rtol=2e-3)

@@ -208,3 +208,52 @@ def test_refined_mesh_matches_oracle(D, N, level, cmin, cmax, per, patches):
     faces = 2 * mesh.ninterfaces + mesh.nboundaries + mesh.nmortars * (ns + 1)
     assert faces == 2 * D * mesh.nelements
     assert set(np.diff(np.sort(mesh.levels))) <= {0, 1}
+
+
+def _compare_mesh_with_oracle(mesh, o, N):
+    m = o.mesh()
+    assert (mesh.nelements, mesh.ninterfaces, mesh.nmortars, mesh.nboundaries) == \
+        (o.nelements, o.ninterfaces, o.nmortars, o.nboundaries)
+    np.testing.assert_array_equal(mesh.levels, o.levels())
+    np.testing.assert_array_equal(mesh.cell_centers, m["centers"])
+    np.testing.assert_array_equal(mesh.neighbor_ids, m["neighbor_ids"])
+    np.testing.assert_array_equal(mesh.orientations, m["orientations"])
+    ids, side, ori = o.mortars()
+    np.testing.assert_array_equal(mesh.mortar_neighbor_ids, ids)
+    np.testing.assert_array_equal(mesh.mortar_large_sides, side)
+    np.testing.assert_array_equal(mesh.mortar_orientations, ori)
+    b_el, b_ori, b_side = o.boundaries()
+    np.testing.assert_array_equal(mesh.boundary_neighbor_ids, b_el)
+    np.testing.assert_array_equal(mesh.boundary_orientations, b_ori)
+    np.testing.assert_array_equal(mesh.boundary_neighbor_sides, b_side)
+    ij, xc = element_geometry(mesh, th.LobattoLegendreBasis(N))
+    np.testing.assert_array_equal(ij, m["inverse_jacobian"])
+    np.testing.assert_array_equal(xc, m["node_coordinates"])
+
+
+@pytest.mark.parametrize("D,per,patches", [(2, True, ()), (2, False, ()), (3, True, ()), (3, False, ()),
+                                           (3, True, [((0.5, 0.5, 0.5), (1.5, 1.5, 1.5))])])
+def test_amr_adapt_matches_oracle(D, per, patches):
+    """TreeMesh.adapt (refine! with rebalancing, guarded coarsen!) on random
+    controller values: after every adaptation the same elements, interfaces,
+    mortars and boundaries as the oracle's tree, and the same transfer map."""
+    from oracle import Oracle
+    N, level = 3, 2
+    cmin, cmax = (0.0,) * D, (2.0,) * D
+    patches = [(p[0][:D], p[1][:D]) for p in patches]
+    o = Oracle(D, N, level, cmin, cmax, periodicity=per, refinement_patches=patches)
+    if not per:
+        o.set_initial_condition("constant")
+    mesh = th.TreeMesh(cmin, cmax, initial_refinement_level=level, periodicity=per, refinement_patches=patches)
+    rng = np.random.default_rng(100 + D + 10 * per)
+    u = np.zeros(o.nvalues)
+    for it in range(10):
+        lam = rng.integers(-1, 2, mesh.nelements).astype(np.int32)
+        lam[mesh.levels >= 5] = np.minimum(lam[mesh.levels >= 5], 0)
+        only_refine = it == 0
+        changed, kind, src = mesh.adapt(lam, only_refine=only_refine)
+        ochanged, u, okind, osrc = o.amr_adapt(u, lam, only_refine=only_refine)
+        assert changed == ochanged
+        np.testing.assert_array_equal(kind, okind)
+        np.testing.assert_array_equal(src, osrc)
+        _compare_mesh_with_oracle(mesh, o, N)

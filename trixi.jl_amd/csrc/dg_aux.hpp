@@ -412,6 +412,15 @@ __device__ void exact_solution(const ExactParams &ep, const double *x, double t,
       if (D == 2) prim2cons(w);
       break;
     }
+    case 6: {  // initial_condition_density_pulse (elixir_euler_amr.jl:15-26)
+      double r2 = 0;
+      for (int d = 0; d < D; ++d) r2 += x[d] * x[d];
+      const double rho = 1 + exp(-r2) / 2;
+      u[0] = rho;
+      for (int d = 0; d < D; ++d) u[1 + d] = rho;
+      u[V - 1] = igm1 + 0.5 * rho * D;
+      break;
+    }
     default: {  // seeded Fourier modes
       double w[5];
       for (int fld = 0; fld < 1 + D; ++fld) {
@@ -717,6 +726,83 @@ __global__ void hg_smooth_adj_kernel(const double *__restrict__ alpha_raw, const
     m = an > m ? an : m;
   }
   alpha[e] = m;
+}
+
+// IndicatorMax (indicators_3d.jl:239-259, indicators_2d.jl): alpha[e] = max over
+// the nodes of element e of conserved variable var. One thread per element.
+template <int D, int N>
+__global__ __launch_bounds__(256) void indicator_max_kernel(const double *__restrict__ u, int E, int var,
+                                                            double *__restrict__ alpha) {
+  constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const double *ue = u + (size_t)e * NN * V + var;
+  double m = ue[0];
+  for (int node = 1; node < NN; ++node) m = fmax(m, ue[(size_t)node * V]);
+  alpha[e] = m;
+}
+
+// AMR solution transfer with the L2 adaptor (refine_element! / coarsen_elements!,
+// amr_dg3d.jl:138-230, 337-430; amr_dg2d.jl): new element e is
+//   kind 0      a copy of old element src,
+//   kind 1 + q  child q of old element src: per dimension d the interpolation
+//               forward_upper (bit d of q set) or forward_lower,
+//   kind -1     the sum over the 2^D old elements src + q of the projections
+//               reverse_upper / reverse_lower per dimension (bit d of q).
+// mats = [forward_upper, forward_lower, reverse_upper, reverse_lower], each
+// n x n row-major with rows = output nodes. One thread per new node; the
+// tensor contraction runs x innermost, then y, then z.
+template <int D, int N>
+__global__ __launch_bounds__(256) void amr_transfer_kernel(const double *__restrict__ u_old, double *__restrict__ u_new,
+                                                           const int32_t *__restrict__ kind,
+                                                           const int32_t *__restrict__ src, int E_new,
+                                                           const double *__restrict__ mats) {
+  constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
+  __shared__ double s_m[4 * n * n];
+  for (int k = threadIdx.x; k < 4 * n * n; k += blockDim.x) s_m[k] = mats[k];
+  __syncthreads();
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)E_new * NN) return;
+  const int e = (int)(idx / NN), node = (int)(idx % NN);
+  const int i = node % n, j = (node / n) % n, k = D == 3 ? node / (n * n) : 0;
+  const int kd = kind[e], s0 = src[e];
+  double out[V];
+  for (int v = 0; v < V; ++v) out[v] = 0;
+  if (kd == 0) {
+    const double *ue = u_old + ((size_t)s0 * NN + node) * V;
+    for (int v = 0; v < V; ++v) out[v] = ue[v];
+  } else {
+    const bool coarsen = kd < 0;
+    const int nq = coarsen ? (1 << D) : 1;
+    for (int qq = 0; qq < nq; ++qq) {
+      const int q = coarsen ? qq : kd - 1;
+      const double *ue = u_old + (size_t)(coarsen ? s0 + qq : s0) * NN * V;
+      const int base = coarsen ? 2 : 0;
+      const double *mx = &s_m[(base + ((q & 1) ? 0 : 1)) * n * n + i * n];
+      const double *my = &s_m[(base + ((q & 2) ? 0 : 1)) * n * n + j * n];
+      const double *mz = &s_m[(base + ((q & 4) ? 0 : 1)) * n * n + k * n];
+      for (int c = 0; c < (D == 3 ? n : 1); ++c) {
+        double t2[V];
+        for (int v = 0; v < V; ++v) t2[v] = 0;
+        for (int b = 0; b < n; ++b) {
+          double t1[V];
+          for (int v = 0; v < V; ++v) t1[v] = 0;
+          for (int a = 0; a < n; ++a) {
+            const double *un = ue + (size_t)(a + n * b + n * n * c) * V;
+            for (int v = 0; v < V; ++v) t1[v] += mx[a] * un[v];
+          }
+          for (int v = 0; v < V; ++v) t2[v] += my[b] * t1[v];
+        }
+        if (D == 3) {
+          for (int v = 0; v < V; ++v) out[v] += mz[c] * t2[v];
+        } else {
+          for (int v = 0; v < V; ++v) out[v] += t2[v];
+        }
+      }
+    }
+  }
+  double *o = u_new + ((size_t)e * NN + node) * V;
+  for (int v = 0; v < V; ++v) o[v] = out[v];
 }
 
 }  // namespace trixi_hip

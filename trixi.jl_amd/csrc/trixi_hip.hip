@@ -557,6 +557,10 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
       adj.insert(adj.end(), adjl[e].begin(), adjl[e].end());
       adj_ptr.push_back((int32_t)adj.size());
     }
+  }
+  // the L2 mortar matrices serve the mortars and the AMR solution transfer
+  if (basis->mortar_forward_upper && basis->mortar_forward_lower && basis->mortar_reverse_upper &&
+      basis->mortar_reverse_lower) {
     ctx->mfu.assign(basis->mortar_forward_upper, basis->mortar_forward_upper + n * n);
     ctx->mfl.assign(basis->mortar_forward_lower, basis->mortar_forward_lower + n * n);
     ctx->mru.assign(basis->mortar_reverse_upper, basis->mortar_reverse_upper + n * n);
@@ -1231,9 +1235,68 @@ int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float
   return TRIXI_HIP_OK;
 }
 
+int trixi_hip_indicator_max(trixi_hip_ctx *ctx, int variable, double *alpha_host) {
+  if (!ctx || !alpha_host || variable < 0 || variable >= ctx->V) return TRIXI_HIP_EINVAL;
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    hipLaunchKernelGGL((indicator_max_kernel<D, N>), dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+                       cur_u64(ctx), ctx->E, variable, ctx->d_alpha_raw);
+  });
+  if (!ok) return set_err(ctx, TRIXI_HIP_EINVAL, "unsupported ndims/polydeg");
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(alpha_host, ctx->d_alpha_raw, (size_t)ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *kind, const int32_t *src) {
+  if (!to || !from || !kind || !src || to == from) return TRIXI_HIP_EINVAL;
+  if (to->D != from->D || to->N != from->N || to->device != from->device)
+    return set_err(to, TRIXI_HIP_EINVAL, "transfer needs contexts of the same ndims, polydeg and device");
+  const int nch = 1 << to->D;
+  for (int e = 0; e < to->E; ++e) {  // every element reads inside `from`
+    const int k = kind[e], s = src[e];
+    const bool ok = (k == 0 || (k >= 1 && k <= nch)) ? (s >= 0 && s < from->E)
+                    : (k == -1 ? (s >= 0 && s + nch <= from->E) : false);
+    if (!ok) return set_err(to, TRIXI_HIP_EINVAL, "transfer map entry out of range");
+  }
+  const int n = to->n, E = to->E;
+  std::vector<double> mats;
+  for (const std::vector<double> *m : {&to->mfu, &to->mfl, &to->mru, &to->mrl}) {
+    if ((int)m->size() != n * n) return set_err(to, TRIXI_HIP_EINVAL, "mortar matrices missing");
+    mats.insert(mats.end(), m->begin(), m->end());
+  }
+  // staging: kind [E] | src [E] | matrices [4 n n] (8-byte aligned after the ints)
+  const size_t ints = ((size_t)2 * E + 1) & ~(size_t)1;
+  char *d_tmp = nullptr;
+  if (hipMalloc(&d_tmp, ints * 4 + mats.size() * 8) != hipSuccess)
+    return set_err(to, TRIXI_HIP_ENOMEM, "transfer staging allocation failed");
+  int32_t *d_kind = (int32_t *)d_tmp, *d_src = d_kind + E;
+  double *d_mats = (double *)(d_tmp + ints * 4);
+  int r = TRIXI_HIP_OK;
+  if (hipStreamSynchronize(from->stream) != hipSuccess ||
+      hipMemcpyAsync(d_kind, kind, (size_t)E * 4, hipMemcpyHostToDevice, to->stream) != hipSuccess ||
+      hipMemcpyAsync(d_src, src, (size_t)E * 4, hipMemcpyHostToDevice, to->stream) != hipSuccess ||
+      hipMemcpyAsync(d_mats, mats.data(), mats.size() * 8, hipMemcpyHostToDevice, to->stream) != hipSuccess) {
+    r = set_err(to, TRIXI_HIP_EHIP, "transfer staging failed");
+  } else {
+    const long total = (long)E * to->NN;
+    bool ok = dispatch_dn(to->D, to->N, [&](auto Dc, auto Nc) {
+      constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+      hipLaunchKernelGGL((amr_transfer_kernel<D, N>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                         to->stream, cur_u64(from), cur_u64(to), d_kind, d_src, E, d_mats);
+    });
+    if (!ok || hipGetLastError() != hipSuccess) r = set_err(to, TRIXI_HIP_EHIP, "transfer launch failed");
+    if (!r && to->fp32) r = convert<float, double>(to, cur_u32(to), cur_u64(to), to->nvals);
+  }
+  if (hipStreamSynchronize(to->stream) != hipSuccess && !r) r = set_err(to, TRIXI_HIP_EHIP, "transfer failed");
+  (void)hipFree(d_tmp);
+  return r;
+}
+
 int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *params, int nparams,
                                  const double *modes) {
-  if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_FOURIER_MODES) return TRIXI_HIP_EINVAL;
+  if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_DENSITY_PULSE) return TRIXI_HIP_EINVAL;
   // captured steps hold the boundary-state kernel with the old function
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);

@@ -41,6 +41,7 @@ IC_WEAK_BLAST_WAVE = 2
 IC_ISENTROPIC_VORTEX = 3
 IC_KELVIN_HELMHOLTZ = 4
 IC_FOURIER_MODES = 5
+IC_DENSITY_PULSE = 6
 
 IND_HG = 0
 IND_MLP = 1
@@ -134,6 +135,8 @@ SIGNATURES = [
     ("trixi_hip_solve", _I, [_P, _PD, _PD, _PD, _I, _D, _D, _D, _I, _I, _PD, _PI]),
     ("trixi_hip_max_dt", _I, [_P, _D, _PD]),
     ("trixi_hip_indicator", _I, [_P, _I, _P]),
+    ("trixi_hip_indicator_max", _I, [_P, _I, _P]),
+    ("trixi_hip_transfer", _I, [_P, _P, _P, _P]),
     ("trixi_hip_set_exact_solution", _I, [_P, _I, _PD, _I, _PD]),
     ("trixi_hip_error_norms", _I, [_P, _D, _PD, _PD]),
     ("trixi_hip_integrals", _I, [_P, _PD, _PD]),

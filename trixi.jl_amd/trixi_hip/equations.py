@@ -243,6 +243,22 @@ initial_condition_weak_blast_wave = InitialCondition(
     "initial_condition_weak_blast_wave", _ic_weak_blast_wave, _lib.IC_WEAK_BLAST_WAVE)
 
 
+def _ic_density_pulse(x, t, eq):
+    # examples/tree_3d_dgsem/elixir_euler_amr.jl:15-26: a Gaussian density pulse
+    # advected with v = 1 in every direction at p = 1
+    rho = 1 + np.exp(-np.sum(x ** 2, axis=-1)) / 2
+    u = np.empty(x.shape[:-1] + (eq.nvariables(),))
+    u[..., 0] = rho
+    for d in range(eq.ndims):
+        u[..., 1 + d] = rho
+    u[..., -1] = 1 / (eq.gamma - 1) + 1 / 2 * rho * eq.ndims
+    return u
+
+
+initial_condition_density_pulse = InitialCondition(
+    "initial_condition_density_pulse", _ic_density_pulse, _lib.IC_DENSITY_PULSE)
+
+
 def initial_condition_isentropic_vortex(p_background=25.0):
     return InitialCondition("initial_condition_isentropic_vortex", _make_vortex(float(p_background)),
                             _lib.IC_ISENTROPIC_VORTEX, params=(float(p_background),))

@@ -18,6 +18,8 @@ Restates the parts of Trixi.jl that fix element order and geometry:
   refinement_patches of type "box": refine_box! / refine! / rebalance!
       abstract_tree.jl:317-490 (children inserted behind their parent, 2:1
       balance); init_mortars! containers_3d.jl:652-788, containers_2d.jl:700-800
+  AMR adaptation of the tree (refine! / rebalance! / coarsen!,
+      abstract_tree.jl:367-600; the AMRCallback's selection, amr.jl:244-392)
 """
 import numpy as np
 
@@ -54,6 +56,7 @@ class TreeMesh:
                 patch = (patch["coordinates_min"], patch["coordinates_max"])
             pmin, pmax = patch
             self.refinement_patches.append((tuple(float(v) for v in pmin), tuple(float(v) for v in pmax)))
+        self._leaves = None  # (level, integer coordinates, centre) per leaf once the tree is refined
         self._build()
         if self.refinement_patches:
             self._build_refined()
@@ -139,74 +142,29 @@ class TreeMesh:
         """Leaves of the uniform tree refined by the box patches (cells whose
         centre lies strictly inside), rebalanced 2:1, in depth-first order, and
         their interfaces, L2 mortars and boundaries."""
-        D, L = self.ndims, self.initial_refinement_level
-        per = self.periodicity
-        leaves = [(L, tuple(int(v) for v in ic), tuple(float(v) for v in c))
-                  for ic, c in zip(self._icoords, self.cell_centers)]
-
-        def rebuild():
-            cells = {}
-            for i, (lv, ic, _) in enumerate(leaves):
-                cells[(lv, ic)] = i
-                a = ic
-                for l2 in range(lv - 1, -1, -1):
-                    a = tuple(v >> 1 for v in a)
-                    if (l2, a) in cells:
-                        break
-                    cells[(l2, a)] = -1
-            return cells
-
-        def neighbour(lv, ic, dirn):
-            o, nside = dirn // 2, 1 << lv
-            k = ic[o] + (1 if dirn % 2 else -1)
-            if (k < 0 or k >= nside) and not per[o]:
-                return None
-            nb = list(ic)
-            nb[o] = k % nside
-            return tuple(nb)
-
-        def refine(marked):
-            out = []
-            for i, leaf in enumerate(leaves):
-                if i not in marked:
-                    out.append(leaf)
-                    continue
-                lv, ic, c = leaf
-                child_length = self.length_level_0 / (1 << lv) / 2
-                for child in range(1 << D):
-                    bits = [(child >> d) & 1 for d in range(D)]
-                    out.append((lv + 1, tuple(2 * ic[d] + bits[d] for d in range(D)),
-                                tuple(c[d] + (1 if bits[d] else -1) * child_length / 2 for d in range(D))))
-            return out
-
-        cells = rebuild()
+        D = self.ndims
+        leaves = self._uniform_leaves()
         for pmin, pmax in self.refinement_patches:
             marked = {i for i, (_, _, c) in enumerate(leaves)
                       if all(pmin[d] < c[d] and pmax[d] > c[d] for d in range(D))}
-            refined = [leaves[i] for i in sorted(marked)]
-            leaves = refine(marked)
-            cells = rebuild()
-            while refined:
-                want = set()
-                for lv, ic, _ in refined:
-                    for dirn in range(2 * D):
-                        nb = neighbour(lv, ic, dirn)
-                        if nb is not None and (lv, nb) in cells:
-                            continue
-                        if lv == 0:
-                            continue
-                        pnb = neighbour(lv - 1, tuple(v >> 1 for v in ic), dirn)
-                        if pnb is None or (lv - 1, pnb) not in cells:
-                            continue
-                        want.add((lv - 1, pnb))
-                marked = {i for i, (lv, ic, _) in enumerate(leaves) if (lv, ic) in want}
-                refined = [leaves[i] for i in sorted(marked)]
-                if refined:
-                    leaves = refine(marked)
-                    cells = rebuild()
+            leaves, _, _ = _refine_balanced(leaves, [(0, i) for i in range(len(leaves))], marked, D,
+                                            self.length_level_0, self.periodicity)
+        self._leaves = leaves
+        self._connect()
+
+    def _uniform_leaves(self):
+        L = self.initial_refinement_level
+        return [(L, tuple(int(v) for v in ic), tuple(float(v) for v in c))
+                for ic, c in zip(self._icoords, self.cell_centers)]
+
+    def _connect(self):
+        """Element geometry, interfaces, L2 mortars and boundaries of the leaves."""
+        D, per = self.ndims, self.periodicity
+        leaves = self._leaves
+        cells = _tree_cells(leaves)
         E = len(leaves)
         self.nelements = E
-        self.cell_centers = np.array([c for _, _, c in leaves], dtype=np.float64)
+        self.cell_centers = np.array([c for _, _, c in leaves], dtype=np.float64).reshape(E, D)
         self.levels = np.array([lv for lv, _, _ in leaves], dtype=np.int32)
         self.cell_lengths = self.length_level_0 / (1 << self.levels).astype(np.float64)
         ns = 1 << (D - 1)
@@ -215,7 +173,7 @@ class TreeMesh:
         m_ids, m_side, m_ori = [], [], []
         for e, (lv, ic, _) in enumerate(leaves):
             for dirn in range(2 * D):
-                nb = neighbour(lv, ic, dirn)
+                nb = _neighbour(lv, ic, dirn, per)
                 if nb is None:
                     boundary[e, dirn] = True
                     continue
@@ -234,7 +192,10 @@ class TreeMesh:
                     ch[free[0]] += q & 1
                     if D == 3:
                         ch[free[1]] += q >> 1
-                    ids.append(cells[(lv + 1, tuple(ch))])
+                    f2 = cells.get((lv + 1, tuple(ch)))
+                    if f2 is None or f2 < 0:
+                        raise ValueError("the tree is not 2:1 balanced")
+                    ids.append(f2)
                 m_ids.append(ids + [e])
                 m_side.append(1 if plus else 2)
                 m_ori.append(o)
@@ -260,12 +221,180 @@ class TreeMesh:
         self.boundary_neighbor_sides = np.where(b_dir % 2 == 1, 1, 2).astype(np.int32)
         self.nboundaries = len(b_el)
 
+    def adapt(self, lam, only_refine=False, only_coarsen=False):
+        """One AMR adaptation of the tree (AMRCallback on a TreeMesh,
+        amr.jl:244-392): leaves with lam > 0 are refined (refine!, with 2:1
+        rebalancing, abstract_tree.jl:367-490), then parents whose 2^D children
+        are all still leaves with lam < 0 are coarsened unless a child has a
+        non-sibling face neighbour with children (coarsen!,
+        abstract_tree.jl:516-600), highest level first.
+
+        Returns (changed, kind, src): for every element of the adapted mesh
+        kind 0 = copy of old element src, 1 + q = child q of old element src,
+        -1 = parent of old elements src .. src + 2^D - 1 (the solution
+        transfer map of trixi_hip_transfer)."""
+        lam = np.asarray(lam)
+        if lam.shape != (self.nelements,):
+            raise ValueError("one controller value per element")
+        D = self.ndims
+        leaves = self._leaves if self._leaves is not None else self._uniform_leaves()
+        orig = [(0, i) for i in range(len(leaves))]
+        to_refine = set(np.nonzero(lam > 0)[0].tolist())
+        to_coarsen = [(leaves[i][0], leaves[i][1]) for i in np.nonzero(lam < 0)[0].tolist()]
+        changed = 0
+        if not only_coarsen and to_refine:
+            leaves, orig, c = _refine_balanced(leaves, orig, to_refine, D, self.length_level_0, self.periodicity)
+            changed += c
+        if not only_refine and to_coarsen:
+            leaves, orig, c = _coarsen(leaves, orig, to_coarsen, D, self.length_level_0, self.periodicity)
+            changed += c
+        if changed:
+            self._leaves = leaves
+            self._connect()
+        kind = np.array([o[0] for o in orig], dtype=np.int32)
+        src = np.array([o[1] for o in orig], dtype=np.int32)
+        return bool(changed), kind, src
+
     def ncells(self):
         return self.nelements
 
     def __repr__(self):
         return (f"TreeMesh{{{self.ndims}}} with {self.nelements} elements on "
                 f"{self.coordinates_min}..{self.coordinates_max}, periodicity={self.periodicity}")
+
+
+def _tree_cells(leaves):
+    """Every cell of the tree by (level, integer coordinates): a leaf maps to its
+    element index, an inner cell (one with children) to -1."""
+    cells = {}
+    for i, (lv, ic, _) in enumerate(leaves):
+        cells[(lv, ic)] = i
+        a = ic
+        for l2 in range(lv - 1, -1, -1):
+            a = tuple(v >> 1 for v in a)
+            if (l2, a) in cells:
+                break
+            cells[(l2, a)] = -1
+    return cells
+
+
+def _neighbour(lv, ic, dirn, per):
+    """Same-level cell coordinates across face dirn (2o = -o, 2o+1 = +o),
+    wrapping periodic dimensions; None outside a non-periodic domain."""
+    o, nside = dirn // 2, 1 << lv
+    k = ic[o] + (1 if dirn % 2 else -1)
+    if (k < 0 or k >= nside) and not per[o]:
+        return None
+    nb = list(ic)
+    nb[o] = k % nside
+    return tuple(nb)
+
+
+def _refine(leaves, orig, marked, D, length0):
+    """refine_unbalanced! (abstract_tree.jl:317-353): each marked leaf is
+    replaced in place by its children (bit d of the child index = upper half in d)."""
+    out, oout = [], []
+    for i, leaf in enumerate(leaves):
+        if i not in marked:
+            out.append(leaf)
+            oout.append(orig[i])
+            continue
+        lv, ic, c = leaf
+        child_length = length0 / (1 << lv) / 2
+        for child in range(1 << D):
+            bits = [(child >> d) & 1 for d in range(D)]
+            out.append((lv + 1, tuple(2 * ic[d] + bits[d] for d in range(D)),
+                        tuple(c[d] + (1 if bits[d] else -1) * child_length / 2 for d in range(D))))
+            oout.append((1 + child, orig[i][1]))
+    return out, oout
+
+
+def _refine_balanced(leaves, orig, marked, D, length0, per):
+    """refine! + rebalance! (abstract_tree.jl:367-400, 453-490): after each
+    round, the coarse neighbour (the parent's face neighbour) of a refined cell
+    without a same-level face neighbour is refined, until nothing changes."""
+    count = 0
+    while marked:
+        refined = [leaves[i] for i in sorted(marked)]
+        count += len(refined)
+        leaves, orig = _refine(leaves, orig, marked, D, length0)
+        cells = _tree_cells(leaves)
+        marked = set()
+        for lv, ic, _ in refined:
+            for dirn in range(2 * D):
+                nb = _neighbour(lv, ic, dirn, per)
+                if nb is not None and (lv, nb) in cells:
+                    continue
+                if lv == 0:
+                    continue
+                pnb = _neighbour(lv - 1, tuple(v >> 1 for v in ic), dirn, per)
+                if pnb is None:
+                    continue
+                f = cells.get((lv - 1, pnb))
+                if f is not None and f >= 0:
+                    marked.add(f)
+    return leaves, orig, count
+
+
+def _coarsen(leaves, orig, keys, D, length0, per):
+    """Coarsening of the AMR callback (amr.jl:304-370) with the balance guard
+    of coarsen! (abstract_tree.jl:516-600)."""
+    nch = 1 << D
+    cells = _tree_cells(leaves)
+    count, parents = {}, []
+    for lv, ic in keys:
+        if lv == 0:
+            continue
+        f = cells.get((lv, ic))
+        if f is None or f < 0:
+            continue  # refined by rebalancing: no longer a leaf
+        pk = (lv - 1, tuple(v >> 1 for v in ic))
+        count[pk] = count.get(pk, 0) + 1
+        if count[pk] == nch:
+            parents.append(pk)
+    parents.sort(key=lambda p: -p[0])
+    coarsened = set()
+    for lv, pic in parents:
+        children = [tuple(2 * pic[d] + ((q >> d) & 1) for d in range(D)) for q in range(nch)]
+        skip = False
+        for q, ch in enumerate(children):
+            for dirn in range(2 * D):
+                bit = (q >> (dirn // 2)) & 1
+                if bit == (1 if dirn % 2 == 0 else 0):
+                    continue  # the neighbour is a sibling
+                nb = _neighbour(lv + 1, ch, dirn, per)
+                if nb is None:
+                    continue
+                f = cells.get((lv + 1, nb))
+                if f is not None and f < 0:
+                    skip = True
+                    break
+            if skip:
+                break
+        if skip:
+            continue
+        for ch in children:
+            del cells[(lv + 1, ch)]
+        cells[(lv, pic)] = 0
+        coarsened.add((lv, pic))
+    if not coarsened:
+        return leaves, orig, 0
+    out, oout = [], []
+    i = 0
+    while i < len(leaves):
+        lv, ic, c = leaves[i]
+        if lv > 0:
+            pic = tuple(v >> 1 for v in ic)
+            if (lv - 1, pic) in coarsened:
+                h = length0 / (1 << lv)
+                out.append((lv - 1, pic, tuple(c[d] + h / 2 for d in range(D))))
+                oout.append((-1, orig[i][1]))
+                i += nch
+                continue
+        out.append(leaves[i])
+        oout.append(orig[i])
+        i += 1
+    return out, oout, len(coarsened)
 
 
 def element_geometry(mesh, basis):

@@ -157,6 +157,27 @@ class SemidiscretizationHyperbolic:
                                                         _ptr(modes)), out, "set_exact_solution")
         return self._ctx
 
+    def remesh(self, kind, src):
+        """Carry the resident state over to the adapted mesh: a context is
+        created on self.mesh (already adapted) and trixi_hip_transfer projects
+        the old context's state into it (refine!/coarsen! of the DG solver,
+        amr_dg3d.jl:138-430); the old context is destroyed."""
+        old = self._ctx
+        self.inverse_jacobian, self.node_coordinates = element_geometry(self.mesh, self.solver.basis)
+        self._ctx = None
+        new = self.ctx
+        if old is None:
+            return
+        try:
+            k = np.ascontiguousarray(kind, dtype=np.int32)
+            s = np.ascontiguousarray(src, dtype=np.int32)
+            if k.shape != (self.mesh.nelements,) or s.shape != k.shape:
+                raise ValueError("one transfer entry per element of the adapted mesh")
+            _lib.check(self._lib.trixi_hip_transfer(new, old, k.ctypes.data_as(ctypes.c_void_p),
+                                                    s.ctypes.data_as(ctypes.c_void_p)), new, "trixi_hip_transfer")
+        finally:
+            self._lib.trixi_hip_destroy(old)
+
     def tile_used(self):
         e, t = ctypes.c_int(), ctypes.c_int()
         _lib.check(self._lib.trixi_hip_get_tile(self.ctx, ctypes.byref(e), ctypes.byref(t)), self.ctx, "get_tile")

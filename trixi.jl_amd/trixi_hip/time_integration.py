@@ -6,6 +6,7 @@ the StepsizeCallback and AnalysisCallback semantics Trixi.jl uses.
   limit_dt!                   src/time_integration/time_integration.jl:46-55
   StepsizeCallback            src/callbacks_step/stepsize.jl:31-155
   AnalysisCallback (errors)   src/callbacks_step/analysis.jl, analysis_dg3d.jl:123-161
+  AMRCallback                 src/callbacks_step/amr.jl (trixi_hip.amr)
 """
 import ctypes
 import math
@@ -14,6 +15,7 @@ import time
 import numpy as np
 
 from . import _lib
+from .amr import AMRCallback
 
 
 class CarpenterKennedy2N54:
@@ -119,6 +121,9 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
           poll_every=16):
     """`solve(ode, CarpenterKennedy2N54(); dt, callback)` on the device.
 
+    With an AMRCallback the host drives the steps (one graph replay per step);
+    the callbacks after a step run in the elixirs' order: analysis, AMR, step size.
+
     The state lives in the HIP context for the whole run. With a
     StepsizeCallback (interval 1) the whole loop runs on the device
     (trixi_hip_solve: step graph with device-side dt and limit_dt, periodic
@@ -135,6 +140,7 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
     cbs = callback if isinstance(callback, CallbackSet) else CallbackSet(*(callback or []))
     stepsize = cbs.of(StepsizeCallback)
     analysis = cbs.of(AnalysisCallback)
+    amr = cbs.of(AMRCallback)
     t0, t_end = ode.tspan
     t = t0
     na = len(alg.a)
@@ -143,10 +149,13 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
     c = (ctypes.c_double * na)(*alg.c)
     it = 0
     wall0 = time.perf_counter()
-    on_device = bool(device_steps and use_graph and stepsize and stepsize[0].interval == 1)
+    on_device = bool(device_steps and use_graph and stepsize and stepsize[0].interval == 1 and not amr)
     records_on_device = bool(on_device and analysis and analysis[0].interval > 0)
     if analysis and not records_on_device:
         analysis[0].analyze(t0, 0)
+    if amr:  # callback initialisation order of the elixirs: analysis, AMR, step size
+        amr[0].initialize(semi, t0)
+        ctx = semi.ctx
     if on_device:
         _lib.check(lib.trixi_hip_set_analysis(ctx, analysis[0].interval if records_on_device else 0,
                                               analysis[0].capacity if records_on_device else 0), ctx, "set_analysis")
@@ -175,10 +184,13 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
         _lib.check(lib.trixi_hip_rk_step(ctx, a, b, c, na, float(t), float(dt), int(use_graph)), ctx, "rk_step")
         it += 1
         t += dt
-        if stepsize and (it % stepsize[0].interval == 0):
-            dt = stepsize[0].calculate_dt(semi)
         if analysis and analysis[0].interval > 0 and it % analysis[0].interval == 0 and not final:
             analysis[0].analyze(t, it)
+        if amr and amr[0].condition(it, final):
+            amr[0](semi)
+            ctx = semi.ctx
+        if stepsize and (it % stepsize[0].interval == 0):
+            dt = stepsize[0].calculate_dt(semi)
         if final or (maxiters is not None and it >= maxiters):
             break
     _lib.check(lib.trixi_hip_synchronize(ctx), ctx, "synchronize")
