@@ -215,6 +215,77 @@ function semidiscretize_hip(semi, tspan; precision::Symbol = :fp64)
     return ODEProblem{true}(HipRHS(semi; precision = precision), ode.u0, ode.tspan, semi)
 end
 
-export HipRHS, semidiscretize_hip
+# --- AMR with a device-resident state ---------------------------------------
+# Trixi's AMRCallback keeps its tree logic (amr.jl:244-392, abstract_tree.jl);
+# the per-node work runs on the device: IndicatorMax for the controller, and
+# the solution transfer into a context created on the adapted cache.
+
+"""
+    hip_indicator_max(f::HipRHS, variable = 1)
+
+`IndicatorMax(semi; variable)` (indicators_3d.jl:239-259) of the resident state,
+for `variable` = the 1-based index of a conserved variable (`first` = 1).
+"""
+function hip_indicator_max(f::HipRHS, nelements::Integer, variable::Integer = 1)
+    alpha = Vector{Float64}(undef, nelements)
+    check(ccall((:trixi_hip_indicator_max, libtrixi_hip), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
+                f.ctx, variable - 1, alpha), f.ctx)
+    return alpha
+end
+
+"""
+    transfer_map(n_old, elements_to_refine, elements_to_remove, ndims)
+
+The (kind, src) arrays of `trixi_hip_transfer` for one adaptation, from the
+element lists the solver's `refine!` and `coarsen!` receive (amr.jl:288-360):
+`elements_to_refine` are old element ids, `elements_to_remove` are ids after the
+refinement (the 2^ndims consecutive children of every coarsened parent).
+"""
+function transfer_map(n_old::Integer, elements_to_refine, elements_to_remove, ndims::Integer)
+    nch = 2^ndims
+    refine = falses(n_old)
+    refine[elements_to_refine] .= true
+    kind, src = Int32[], Int32[]
+    for e in 1:n_old
+        if refine[e]
+            for q in 0:(nch - 1)
+                push!(kind, Int32(1 + q)); push!(src, Int32(e - 1))
+            end
+        else
+            push!(kind, Int32(0)); push!(src, Int32(e - 1))
+        end
+    end
+    isempty(elements_to_remove) && return kind, src
+    removed = falses(length(kind))
+    removed[elements_to_remove] .= true
+    kind2, src2 = Int32[], Int32[]
+    e = 1
+    while e <= length(kind)
+        if removed[e]  # first child of a coarsened parent; its children are unrefined old elements
+            push!(kind2, Int32(-1)); push!(src2, src[e])
+            e += nch
+        else
+            push!(kind2, kind[e]); push!(src2, src[e])
+            e += 1
+        end
+    end
+    return kind2, src2
+end
+
+"""
+    hip_transfer!(new::HipRHS, old::HipRHS, kind, src)
+
+Project the resident state of `old` into `new` (created with `HipRHS(semi)` on
+the adapted cache): refine!/coarsen! of the DG solver (amr_dg3d.jl:138-430).
+"""
+function hip_transfer!(new::HipRHS, old::HipRHS, kind::Vector{Int32}, src::Vector{Int32})
+    GC.@preserve kind src begin
+        check(ccall((:trixi_hip_transfer, libtrixi_hip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}),
+                    new.ctx, old.ctx, kind, src), new.ctx)
+    end
+    return new
+end
+
+export HipRHS, semidiscretize_hip, hip_indicator_max, transfer_map, hip_transfer!
 
 end # module
