@@ -18,6 +18,7 @@ from . import _lib
 from .amr import AMRCallback, ControllerThreeLevel, IndicatorMax, first
 from .basis import LobattoLegendreBasis
 from .equations import (BoundaryConditionDirichlet, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
+                        cons2cons, cons2prim, varnames,
                         FluxLaxFriedrichs, boundary_condition_periodic,
                         flux_central, flux_lax_friedrichs, flux_ranocha, fourier_modes,
                         initial_condition_constant, initial_condition_convergence_test,
@@ -25,6 +26,9 @@ from .equations import (BoundaryConditionDirichlet, CompressibleEulerEquations2D
                         initial_condition_fourier_modes, initial_condition_isentropic_vortex,
                         initial_condition_kelvin_helmholtz_instability, initial_condition_weak_blast_wave,
                         max_abs_speed, max_abs_speed_naive, source_terms_convergence_test)
+from .io import (SaveRestartCallback, SaveSolutionCallback, load_dt, load_mesh, load_mesh_file, load_restart_file,
+                 load_time, load_timestep, read_solution_header, save_mesh_file, save_restart_file,
+                 save_solution_file)
 from .mesh import TreeMesh
 from .semidiscretization import ODEProblem, SemidiscretizationHyperbolic, rhs_hyperbolic_, semidiscretize
 from .solver import (DGSEM, IndicatorHennemannGassner, VolumeIntegralFluxDifferencing,

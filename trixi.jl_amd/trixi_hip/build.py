@@ -56,5 +56,33 @@ def build_library(verbose=False, extra_flags=(), jobs=None):
     return LIB
 
 
+HDF5_DIR = os.environ.get("TRIXI_HDF5_DIR", "/opt/conda")
+IO_LIB = os.path.join(PKG_ROOT, "libtrixi_hip_io.so")
+
+
+def build_io_library(verbose=False):
+    """libtrixi_hip_io.so (solution/restart/mesh files): host C++ against libhdf5.
+    Returns None (and builds nothing) when the HDF5 headers are absent."""
+    inc = os.path.join(HDF5_DIR, "include")
+    libdir = os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"HDF5 headers not found under {inc}: libtrixi_hip_io.so not built", flush=True)
+        return None
+    repo = os.path.dirname(PKG_ROOT)
+    tmp = IO_LIB + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           "-I" + os.path.join(repo, "include"), "-I" + inc, os.path.join(CSRC, "io", "trixi_hip_io.cpp"),
+           "-L" + libdir, "-lhdf5", "-Wl,-rpath," + libdir, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building libtrixi_hip_io.so failed")
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_io_library(verbose=True))

@@ -57,6 +57,25 @@ class CompressibleEulerEquations:
         return f"CompressibleEulerEquations{self.ndims}D(gamma={self.gamma})"
 
 
+def cons2cons(u, equations):
+    """`cons2cons`: the conserved variables themselves."""
+    return np.asarray(u, dtype=np.float64)
+
+
+def cons2prim(u, equations):
+    """`cons2prim` (compressible_euler_3d.jl:1783-1794): rho, v, p."""
+    return equations.cons2prim(u)
+
+
+def varnames(solution_variables, equations):
+    """`varnames(cons2cons | cons2prim, equations)` (compressible_euler_3d.jl:65-70)."""
+    if solution_variables is cons2cons:
+        return equations.varnames()
+    if solution_variables is cons2prim:
+        return ("rho",) + tuple(f"v{d + 1}" for d in range(equations.ndims)) + ("p",)
+    raise ValueError("solution_variables must be cons2cons or cons2prim")
+
+
 def CompressibleEulerEquations2D(gamma=1.4):
     return CompressibleEulerEquations(2, gamma)
 

@@ -57,6 +57,8 @@ class TreeMesh:
             pmin, pmax = patch
             self.refinement_patches.append((tuple(float(v) for v in pmin), tuple(float(v) for v in pmax)))
         self._leaves = None  # (level, integer coordinates, centre) per leaf once the tree is refined
+        self.unsaved_changes = True  # mesh file out of date (TreeMesh.unsaved_changes)
+        self.current_filename = ""
         self._build()
         if self.refinement_patches:
             self._build_refined()
@@ -251,9 +253,74 @@ class TreeMesh:
         if changed:
             self._leaves = leaves
             self._connect()
+            self.unsaved_changes = True
         kind = np.array([o[0] for o in orig], dtype=np.int32)
         src = np.array([o[1] for o in orig], dtype=np.int32)
         return bool(changed), kind, src
+
+    def tree_arrays(self):
+        """The tree as `save_mesh_file` writes it (mesh_io.jl:15-51): every cell in
+        depth-first pre-order (the order refine! keeps, children right behind
+        their parent), 1-based ids, 0 = none. Returns parent_ids [n],
+        child_ids [n, 2^d], neighbor_ids [n, 2d] (same-level face neighbours,
+        wrapping periodic dimensions; the periodic root is its own neighbour),
+        levels [n], coordinates [n, d] (cell centres: the root's centre, then
+        each child's = parent's +- child length / 2)."""
+        D, per = self.ndims, self.periodicity
+        leaves = self._leaves if self._leaves is not None else self._uniform_leaves()
+        cells = _tree_cells(leaves)
+        nch = 1 << D
+        order, coords = [], []
+        stack = [(0, (0,) * D, self.center_level_0)]
+        while stack:
+            lv, ic, c = stack.pop()
+            order.append((lv, ic))
+            coords.append(c)
+            if cells[(lv, ic)] < 0:
+                child_length = self.length_level_0 / (1 << lv) / 2
+                for q in reversed(range(nch)):
+                    bits = [(q >> d) & 1 for d in range(D)]
+                    stack.append((lv + 1, tuple(2 * ic[d] + bits[d] for d in range(D)),
+                                  tuple(c[d] + (1 if bits[d] else -1) * child_length / 2 for d in range(D))))
+        ids = {key: k + 1 for k, key in enumerate(order)}
+        n = len(order)
+        parent = np.zeros(n, np.int64)
+        child = np.zeros((n, nch), np.int64)
+        nbr = np.zeros((n, 2 * D), np.int64)
+        levels = np.array([lv for lv, _ in order], np.int64)
+        for k, (lv, ic) in enumerate(order):
+            if lv > 0:
+                parent[k] = ids[(lv - 1, tuple(v >> 1 for v in ic))]
+            if cells[(lv, ic)] < 0:
+                for q in range(nch):
+                    child[k, q] = ids[(lv + 1, tuple(2 * ic[d] + ((q >> d) & 1) for d in range(D)))]
+            for dirn in range(2 * D):
+                nb = _neighbour(lv, ic, dirn, per)
+                if nb is not None and (lv, nb) in ids:
+                    nbr[k, dirn] = ids[(lv, nb)]
+        return parent, child, nbr, levels, np.array(coords, np.float64).reshape(n, D)
+
+    @classmethod
+    def from_tree(cls, center_level_0, length_level_0, periodicity, levels, coordinates, child_ids):
+        """The TreeMesh of a tree as `load_mesh!` reads it (mesh_io.jl:603-628):
+        its leaves are the cells without children, in id order."""
+        D = len(center_level_0)
+        leaf = np.all(np.asarray(child_ids) == 0, axis=1)
+        lv = np.asarray(levels)[leaf].astype(int)
+        xc = np.asarray(coordinates, np.float64)[leaf]
+        L0 = float(length_level_0)
+        cmin = tuple(float(c) - L0 / 2 for c in center_level_0)
+        cmax = tuple(float(c) + L0 / 2 for c in center_level_0)
+        mesh = cls(cmin, cmax, initial_refinement_level=int(lv.min()), periodicity=tuple(bool(p) for p in periodicity))
+        leaves = []
+        for level, c in zip(lv.tolist(), xc):
+            h = L0 / (1 << level)
+            ic = tuple(int(round((c[d] - cmin[d]) / h - 0.5)) for d in range(D))
+            leaves.append((level, ic, tuple(float(v) for v in c)))
+        mesh._leaves = leaves
+        mesh._connect()
+        mesh.unsaved_changes = False
+        return mesh
 
     def ncells(self):
         return self.nelements

@@ -252,7 +252,11 @@ class ODEProblem:
         self.p = p
 
 
-def semidiscretize(semi, tspan):
-    """`semidiscretize(semi, tspan)` (semidiscretization.jl:102-153)."""
+def semidiscretize(semi, tspan, restart_file=None):
+    """`semidiscretize(semi, tspan[, restart_file])` (semidiscretization.jl:102-153,
+    156-184): u0 from the initial condition, or read from a restart file."""
+    if restart_file is not None:
+        from .io import load_restart_file
+        return ODEProblem(rhs_hyperbolic_, load_restart_file(semi, restart_file), tspan, semi)
     u0_ode = semi.compute_coefficients(tspan[0])
     return ODEProblem(rhs_hyperbolic_, u0_ode, tspan, semi)

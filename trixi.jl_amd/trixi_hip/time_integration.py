@@ -118,20 +118,22 @@ def _isapprox(a, b):
 
 
 def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_final=True, device_steps=True,
-          poll_every=16):
+          poll_every=16, initial_step=0):
     """`solve(ode, CarpenterKennedy2N54(); dt, callback)` on the device.
 
-    With an AMRCallback the host drives the steps (one graph replay per step);
-    the callbacks after a step run in the elixirs' order: analysis, AMR, step size.
-
     The state lives in the HIP context for the whole run. With a
-    StepsizeCallback (interval 1) the whole loop runs on the device
-    (trixi_hip_solve: step graph with device-side dt and limit_dt, periodic
-    analysis enqueued on the device, host reads the device every `poll_every`
-    steps); otherwise one call of trixi_hip_rk_step (a hipGraph replay of all
-    stages) advances one step and the host computes dt. An AnalysisCallback
-    records the initial state, every `interval`-th step and the final state
-    (analysis.jl:121-122, 143-230)."""
+    StepsizeCallback (interval 1) and no AMR or file-writing callback the whole
+    loop runs on the device (trixi_hip_solve: step graph with device-side dt
+    and limit_dt, periodic analysis enqueued on the device, host reads the
+    device every `poll_every` steps). Otherwise one call of trixi_hip_rk_step (a
+    hipGraph replay of all stages) advances one step and the callbacks run on
+    the host after it, in the order of the CallbackSet (as OrdinaryDiffEq
+    applies them; the elixirs order analysis, restart, solution, AMR, step
+    size), and are initialised in that order before the first step. An
+    AnalysisCallback records the initial state, every `interval`-th step and
+    the final state (analysis.jl:121-122, 143-230). `initial_step` continues the
+    accepted-step count of a restarted run (load_timestep!, save_restart.jl:150-153)."""
+    from .io import SaveRestartCallback, SaveSolutionCallback
     semi = ode.p
     lib = _lib.load()
     ctx = semi.ctx
@@ -141,21 +143,26 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
     stepsize = cbs.of(StepsizeCallback)
     analysis = cbs.of(AnalysisCallback)
     amr = cbs.of(AMRCallback)
+    writers = cbs.of(SaveSolutionCallback) + cbs.of(SaveRestartCallback)
     t0, t_end = ode.tspan
     t = t0
     na = len(alg.a)
     a = (ctypes.c_double * na)(*alg.a)
     b = (ctypes.c_double * na)(*alg.b)
     c = (ctypes.c_double * na)(*alg.c)
-    it = 0
+    it = int(initial_step)
     wall0 = time.perf_counter()
-    on_device = bool(device_steps and use_graph and stepsize and stepsize[0].interval == 1 and not amr)
+    on_device = bool(device_steps and use_graph and stepsize and stepsize[0].interval == 1 and not amr
+                     and not writers and it == 0)
     records_on_device = bool(on_device and analysis and analysis[0].interval > 0)
-    if analysis and not records_on_device:
-        analysis[0].analyze(t0, 0)
-    if amr:  # callback initialisation order of the elixirs: analysis, AMR, step size
-        amr[0].initialize(semi, t0)
-        ctx = semi.ctx
+    for cb in cbs.callbacks:  # initialisation in callback order
+        if isinstance(cb, AnalysisCallback) and not records_on_device:
+            cb.analyze(t0, it)
+        elif isinstance(cb, (SaveSolutionCallback, SaveRestartCallback)):
+            cb.initialize(semi, t0, dt if dt is not None else 0.0, cbs.callbacks)
+        elif isinstance(cb, AMRCallback):
+            cb.initialize(semi, t0)
+            ctx = semi.ctx
     if on_device:
         _lib.check(lib.trixi_hip_set_analysis(ctx, analysis[0].interval if records_on_device else 0,
                                               analysis[0].capacity if records_on_device else 0), ctx, "set_analysis")
@@ -174,6 +181,7 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
             dt = stepsize[0].calculate_dt(semi)
         if dt is None:
             raise ValueError("give dt or a StepsizeCallback")
+    steps = 0
     while not on_device:
         if math.isnan(dt):
             raise FloatingPointError("time step size `dt` is NaN")
@@ -183,15 +191,25 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
             final = True
         _lib.check(lib.trixi_hip_rk_step(ctx, a, b, c, na, float(t), float(dt), int(use_graph)), ctx, "rk_step")
         it += 1
+        steps += 1
         t += dt
-        if analysis and analysis[0].interval > 0 and it % analysis[0].interval == 0 and not final:
-            analysis[0].analyze(t, it)
-        if amr and amr[0].condition(it, final):
-            amr[0](semi)
-            ctx = semi.ctx
-        if stepsize and (it % stepsize[0].interval == 0):
-            dt = stepsize[0].calculate_dt(semi)
-        if final or (maxiters is not None and it >= maxiters):
+        # isfinished (callbacks_step.jl:25-31): t_end reached or maxiters steps taken
+        last = final or (maxiters is not None and steps >= maxiters)
+        for cb in cbs.callbacks:
+            if isinstance(cb, AnalysisCallback):
+                if cb.interval > 0 and it % cb.interval == 0 and not last:
+                    cb.analyze(t, it)
+            elif isinstance(cb, (SaveSolutionCallback, SaveRestartCallback)):
+                if cb.condition(it, last):
+                    cb(semi, t, dt, it, cbs.callbacks)
+            elif isinstance(cb, AMRCallback):
+                if cb.condition(it, last):
+                    cb(semi)
+                    ctx = semi.ctx
+            elif isinstance(cb, StepsizeCallback):
+                if it % cb.interval == 0:
+                    dt = cb.calculate_dt(semi)
+        if last:
             break
     _lib.check(lib.trixi_hip_synchronize(ctx), ctx, "synchronize")
     wall = time.perf_counter() - wall0
