@@ -54,6 +54,12 @@ def parse():
                    help="skip the same RHS on the non-periodic mesh with Dirichlet boundaries")
     p.add_argument("--no-refined", action="store_true",
                    help="skip the RHS on the mesh with the central box refined once (L2 mortars)")
+    p.add_argument("--no-amr", action="store_true",
+                   help="skip the AMR run (elixir_euler_amr.jl to t = 0.1)")
+    p.add_argument("--no-io", action="store_true",
+                   help="skip the restart-file write/read timing")
+    p.add_argument("--profile-amr", action="store_true",
+                   help="only run the FP64 AMR workload once (for rocprofv3)")
     p.add_argument("--profile-step", choices=["fp32", "fp64"], default=None,
                    help="profiling mode: only run K device-controlled CK2N54 steps (fused stages) of the "
                         "workload in one precision and print the time per step (no JSON line)")
@@ -219,6 +225,61 @@ def measured_traffic(level, polydeg, precision):
     return kib * 1024.0, t["summary"]
 
 
+def amr_run(th, precision):
+    """elixir_euler_amr.jl (test_tree_3d_euler.jl:145-165): 3D Euler p3 density
+    pulse on [-5, 5]^3, TreeMesh level 4 adapted by ControllerThreeLevel
+    (IndicatorMax(rho); levels 4/5/6 at 1.05/1.3) with the initial condition
+    adapted first and an AMRCallback every 5 steps, CFL 0.9, t = 0 .. 0.1."""
+    mesh = th.TreeMesh((-5.0, -5.0, -5.0), (5.0, 5.0, 5.0), initial_refinement_level=4, periodicity=True)
+    semi = th.SemidiscretizationHyperbolic(mesh, th.CompressibleEulerEquations3D(1.4),
+                                           th.initial_condition_density_pulse,
+                                           th.DGSEM(polydeg=3, surface_flux=th.FluxLaxFriedrichs(th.max_abs_speed_naive)),
+                                           precision=precision)
+    ctl = th.ControllerThreeLevel(semi, th.IndicatorMax(semi, variable=th.first), base_level=4, med_level=5,
+                                  med_threshold=1.05, max_level=6, max_threshold=1.3)
+    amr = th.AMRCallback(semi, ctl, interval=5, adapt_initial_condition=True, adapt_initial_condition_only_refine=True)
+    ac = th.AnalysisCallback(semi, interval=100)
+    t0 = time.perf_counter()
+    sol = th.solve(th.semidiscretize(semi, (0.0, 0.1)), th.CarpenterKennedy2N54(),
+                   callback=th.CallbackSet(ac, amr, th.StepsizeCallback(cfl=0.9)))
+    wall = time.perf_counter() - t0
+    tm = amr.timer
+    out = dict(wall_s=wall, steps=sol.nsteps, adaptations=tm["calls"], nelements=amr.nelements_history,
+               ms_per_adaptation={k: 1e3 * tm[k] / max(tm["calls"], 1) for k in ("indicator", "mesh", "solver")},
+               ms_per_step_excluding_adaptation=1e3 * (wall - tm["indicator"] - tm["mesh"] - tm["solver"]) / sol.nsteps,
+               l2_rho=sol.analysis[-1]["l2"][0], linf_rho=sol.analysis[-1]["linf"][0])
+    semi.close()
+    return out
+
+
+def amr_cpu_baseline():
+    """The oracle's run of the same AMR workload on the host cores."""
+    from oracle import ControllerThreeLevel, Oracle
+    o = Oracle(3, 3, 4, (-5, -5, -5), (5, 5, 5), volume_integral="weak", surface_flux="llf_naive", periodicity=True)
+    o.set_initial_condition("density_pulse")
+    ctl = ControllerThreeLevel(base_level=4, med_level=5, med_threshold=1.05, max_level=6, max_threshold=1.3)
+    t0 = time.perf_counter()
+    _, _, nsteps, _ = o.solve_amr(0.0, 0.1, 0.9, ctl, interval=5)
+    return dict(wall_s=time.perf_counter() - t0, steps=nsteps, threads=Oracle.get_threads(), kind="port")
+
+
+def io_timing(th, semi, u0, directory):
+    """Write and read one restart file of the state (device download included)."""
+    import numpy as np
+    semi.mesh.current_filename = th.save_mesh_file(semi.mesh, directory)
+    assert semi._lib.trixi_hip_upload_u(semi.ctx, u0.ctypes.data_as(ctypes.c_void_p)) == 0
+    from trixi_hip.io import download_state
+    t0 = time.perf_counter()
+    f = th.save_restart_file(download_state(semi), 0.0, 0.0, 1, semi, directory)
+    t1 = time.perf_counter()
+    u = th.load_restart_file(semi, f)
+    t2 = time.perf_counter()
+    nbytes = os.path.getsize(f)
+    os.remove(f)
+    return dict(file_bytes=nbytes, write_s=t1 - t0, read_s=t2 - t1, write_GBps=nbytes / (t1 - t0) / 1e9,
+                read_GBps=nbytes / (t2 - t1) / 1e9, round_trip_exact=bool(np.array_equal(u, u0)))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -237,6 +298,11 @@ def main():
         print(f"[bench] {args.profile_step}: {sol.nsteps} CK2N54 steps, {1e6 * sol.wall_time / sol.nsteps:.1f} us "
               "per step (device step control, fused stages)", flush=True)
         semi.close()
+        return
+
+    if args.profile_amr:
+        r = amr_run(th, "fp64")
+        print(f"[bench] amr fp64: {r}", flush=True)
         return
 
     res = {}
@@ -286,6 +352,21 @@ def main():
                                  nelements=sr.nelements(), nmortars=sr.mesh.nmortars, finite=rr["finite"])
             sr.close()
         print(f"[bench] refined mesh: {refined}", file=sys.stderr, flush=True)
+
+    amr = {}
+    if not args.no_amr and not args.only:
+        for prec in ("fp64", "fp32"):
+            amr[prec] = amr_run(th, prec)
+            print(f"[bench] amr {prec}: {amr[prec]}", file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            amr["cpu_oracle"] = amr_cpu_baseline()
+            print(f"[bench] amr cpu: {amr['cpu_oracle']}", file=sys.stderr, flush=True)
+    io_res = None
+    if not args.no_io and not args.only:
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            io_res = io_timing(th, semis["fp64"], u0, d)
+        print(f"[bench] io: {io_res}", file=sys.stderr, flush=True)
 
     ndofs = semis["fp32"].ndofs()
     nv = semis["fp32"].nvariables()
@@ -355,6 +436,14 @@ def main():
         out["refined_mesh"] = dict(workload="same RHS with refinement_patches = box (0.5,0.5,0.5)-(1.5,1.5,1.5): "
                                             "the central cells refined once, L2 mortars (mortar-flux kernel included)",
                                    **refined)
+    if amr:
+        out["amr"] = dict(workload="elixir_euler_amr.jl: 3D Euler p3 density pulse, TreeMesh levels 4-6 by "
+                                   "ControllerThreeLevel(IndicatorMax(rho)), AMR every 5 steps, CFL 0.9, t = 0..0.1 "
+                                   "(host-driven steps; adaptation = device indicator, host tree, new context + "
+                                   "device transfer)", **amr)
+    if io_res is not None:
+        out["restart_io"] = dict(workload="restart file (HDF5, FP64 conserved variables) of the state above: "
+                                          "device download + write, read + checks", **io_res)
     if l2 is not None:
         out["equal_l2"] = l2
     if cpu is not None:

@@ -18,6 +18,7 @@ from the device indicator, the adapted mesh gets a new device context, and the
 state is projected from the old context to the new one on the device.
 """
 import ctypes
+import time
 
 import numpy as np
 
@@ -88,13 +89,24 @@ class AMRCallback:
         self.adapt_initial_condition = bool(adapt_initial_condition)
         self.adapt_initial_condition_only_refine = bool(adapt_initial_condition_only_refine)
         self.nelements_history = []
+        # seconds spent per phase over all adaptations (the reference's timer
+        # sections "indicator", "mesh" and "solver" of an AMR step)
+        self.timer = {"indicator": 0.0, "mesh": 0.0, "solver": 0.0, "calls": 0}
 
     def __call__(self, semi, only_refine=False, only_coarsen=False):
         """One adaptation of semi's mesh and resident state; True if the mesh changed."""
+        t0 = time.perf_counter()
         lam = self.controller(semi)
+        t1 = time.perf_counter()
         changed, kind, src = semi.mesh.adapt(lam, only_refine=only_refine, only_coarsen=only_coarsen)
+        t2 = time.perf_counter()
         if changed:
             semi.remesh(kind, src)
+        t3 = time.perf_counter()
+        self.timer["indicator"] += t1 - t0
+        self.timer["mesh"] += t2 - t1
+        self.timer["solver"] += t3 - t2
+        self.timer["calls"] += 1
         self.nelements_history.append(semi.nelements())
         return changed
 
