@@ -56,7 +56,7 @@ class TreeMesh:
                 patch = (patch["coordinates_min"], patch["coordinates_max"])
             pmin, pmax = patch
             self.refinement_patches.append((tuple(float(v) for v in pmin), tuple(float(v) for v in pmax)))
-        self._leaves = None  # (level, integer coordinates, centre) per leaf once the tree is refined
+        self._tree = None  # _Leaves once the tree is refined or adapted
         self.unsaved_changes = True  # mesh file out of date (TreeMesh.unsaved_changes)
         self.current_filename = ""
         self._build()
@@ -92,6 +92,7 @@ class TreeMesh:
         self.nelements = len(centers)
         self.cell_centers = np.array(centers, dtype=np.float64)  # [E, D]
         self._icoords = icoords
+        self._uniform_centers = self.cell_centers.copy()
         self.cell_length = self.length_level_0 / (1 << L)
         nside = 1 << L
         ic = np.array(icoords, dtype=np.int64)
@@ -145,72 +146,79 @@ class TreeMesh:
         centre lies strictly inside), rebalanced 2:1, in depth-first order, and
         their interfaces, L2 mortars and boundaries."""
         D = self.ndims
-        leaves = self._uniform_leaves()
+        t = self._uniform_tree()
         for pmin, pmax in self.refinement_patches:
-            marked = {i for i, (_, _, c) in enumerate(leaves)
-                      if all(pmin[d] < c[d] and pmax[d] > c[d] for d in range(D))}
-            leaves, _, _ = _refine_balanced(leaves, [(0, i) for i in range(len(leaves))], marked, D,
-                                            self.length_level_0, self.periodicity)
-        self._leaves = leaves
+            c = t.centers
+            marked = np.all((np.array(pmin)[None, :] < c) & (np.array(pmax)[None, :] > c), axis=1)
+            t, _ = _refine_balanced(t, marked, D, self.length_level_0, self.periodicity)
+        self._tree = t
         self._connect()
 
-    def _uniform_leaves(self):
-        L = self.initial_refinement_level
-        return [(L, tuple(int(v) for v in ic), tuple(float(v) for v in c))
-                for ic, c in zip(self._icoords, self.cell_centers)]
+    def _uniform_tree(self):
+        ic = np.array(self._icoords, dtype=np.int64).reshape(-1, self.ndims)
+        n = len(ic)
+        return _Leaves(np.full(n, self.initial_refinement_level, np.int64), ic, self._uniform_centers.copy(),
+                       np.zeros(n, np.int32), np.arange(n, dtype=np.int32))
 
     def _connect(self):
-        """Element geometry, interfaces, L2 mortars and boundaries of the leaves."""
+        """Element geometry, interfaces, L2 mortars and boundaries of the leaves
+        (init_interfaces!, init_mortars!, init_boundaries! of containers_{2,3}d.jl):
+        a same-level leaf across a face is an interface neighbour; a same-level
+        cell with children makes a mortar with this element as the large side;
+        no same-level cell means this element is a small element of the coarser
+        neighbour's mortar; outside a non-periodic domain the face is a boundary."""
         D, per = self.ndims, self.periodicity
-        leaves = self._leaves
-        cells = _tree_cells(leaves)
-        E = len(leaves)
+        t = self._tree
+        E = len(t.lv)
+        cells = _Cells(t.lv, t.ic)
         self.nelements = E
-        self.cell_centers = np.array([c for _, _, c in leaves], dtype=np.float64).reshape(E, D)
-        self.levels = np.array([lv for lv, _, _ in leaves], dtype=np.int32)
+        self.cell_centers = t.centers.copy()
+        self.levels = t.lv.astype(np.int32)
         self.cell_lengths = self.length_level_0 / (1 << self.levels).astype(np.float64)
         ns = 1 << (D - 1)
         nbr = np.full((E, 2 * D), -1, dtype=np.int32)
         boundary = np.zeros((E, 2 * D), dtype=bool)
-        m_ids, m_side, m_ori = [], [], []
-        for e, (lv, ic, _) in enumerate(leaves):
-            for dirn in range(2 * D):
-                nb = _neighbour(lv, ic, dirn, per)
-                if nb is None:
-                    boundary[e, dirn] = True
-                    continue
-                f = cells.get((lv, nb))
-                if f is None:
-                    continue  # small side of a coarser neighbour's mortar
-                if f >= 0:
-                    nbr[e, dirn] = f
-                    continue
-                o, plus = dirn // 2, dirn % 2
-                free = [d for d in range(D) if d != o]
-                ids = []
-                for q in range(ns):
-                    ch = [2 * v for v in nb]
-                    ch[o] += 0 if plus else 1
-                    ch[free[0]] += q & 1
-                    if D == 3:
-                        ch[free[1]] += q >> 1
-                    f2 = cells.get((lv + 1, tuple(ch)))
-                    if f2 is None or f2 < 0:
-                        raise ValueError("the tree is not 2:1 balanced")
-                    ids.append(f2)
-                m_ids.append(ids + [e])
-                m_side.append(1 if plus else 2)
-                m_ori.append(o)
+        inner_nb = np.zeros((E, 2 * D), dtype=bool)
+        nb_ic = np.zeros((2 * D, E, D), dtype=np.int64)
+        for dirn in range(2 * D):
+            nb, valid = _shift(t.lv, t.ic, dirn, per)
+            boundary[:, dirn] = ~valid
+            found, val = cells.find(t.lv, nb)
+            leaf = valid & found & (val >= 0)
+            nbr[leaf, dirn] = val[leaf]
+            inner_nb[:, dirn] = valid & found & (val < 0)
+            nb_ic[dirn] = nb
+        # mortars in element order, then direction order
+        m_el, m_dir = np.nonzero(inner_nb)
+        M = len(m_el)
+        m_ids = np.zeros((M, ns + 1), dtype=np.int32)
+        m_ids[:, ns] = m_el
+        o = m_dir // 2
+        plus = m_dir % 2
+        nbm = nb_ic[m_dir, m_el]  # [M, D] same-level neighbour cell (has children)
+        lvm = t.lv[m_el] + 1
+        free = np.array([[d for d in range(D) if d != oo] for oo in range(D)], dtype=np.int64)[o]
+        rows = np.arange(M)
+        for q in range(ns):
+            ch = 2 * nbm
+            ch[rows, o] += 1 - plus
+            ch[rows, free[:, 0]] += q & 1
+            if D == 3:
+                ch[rows, free[:, 1]] += q >> 1
+            found, val = cells.find(lvm, ch)
+            if not np.all(found & (val >= 0)):
+                raise ValueError("the tree is not 2:1 balanced")
+            m_ids[:, q] = val
         self.element_neighbors = nbr
         has = nbr[:, 1::2] >= 0
-        el, o = np.nonzero(has)
-        self.neighbor_ids = np.stack([el, nbr[el, 2 * o + 1]], axis=1).astype(np.int32).reshape(-1, 2)
-        self.orientations = o.astype(np.int32)
-        self.ninterfaces = len(o)
-        self.mortar_neighbor_ids = np.array(m_ids, dtype=np.int32).reshape(-1, ns + 1)
-        self.mortar_large_sides = np.array(m_side, dtype=np.int32)
-        self.mortar_orientations = np.array(m_ori, dtype=np.int32)
-        self.nmortars = len(m_side)
+        el, oi = np.nonzero(has)
+        self.neighbor_ids = np.stack([el, nbr[el, 2 * oi + 1]], axis=1).astype(np.int32).reshape(-1, 2)
+        self.orientations = oi.astype(np.int32)
+        self.ninterfaces = len(oi)
+        self.mortar_neighbor_ids = m_ids
+        self.mortar_large_sides = np.where(plus == 1, 1, 2).astype(np.int32)
+        self.mortar_orientations = o.astype(np.int32)
+        self.nmortars = M
         b_el, b_dir = [], []
         for dirn in range(2 * D):
             ids = np.nonzero(boundary[:, dirn])[0]
@@ -239,24 +247,21 @@ class TreeMesh:
         if lam.shape != (self.nelements,):
             raise ValueError("one controller value per element")
         D = self.ndims
-        leaves = self._leaves if self._leaves is not None else self._uniform_leaves()
-        orig = [(0, i) for i in range(len(leaves))]
-        to_refine = set(np.nonzero(lam > 0)[0].tolist())
-        to_coarsen = [(leaves[i][0], leaves[i][1]) for i in np.nonzero(lam < 0)[0].tolist()]
+        t = self._tree if self._tree is not None else self._uniform_tree()
+        t = _Leaves(t.lv, t.ic, t.centers, np.zeros(len(t.lv), np.int32), np.arange(len(t.lv), dtype=np.int32))
+        coarsen_lv, coarsen_ic = t.lv[lam < 0], t.ic[lam < 0]
         changed = 0
-        if not only_coarsen and to_refine:
-            leaves, orig, c = _refine_balanced(leaves, orig, to_refine, D, self.length_level_0, self.periodicity)
+        if not only_coarsen and np.any(lam > 0):
+            t, c = _refine_balanced(t, lam > 0, D, self.length_level_0, self.periodicity)
             changed += c
-        if not only_refine and to_coarsen:
-            leaves, orig, c = _coarsen(leaves, orig, to_coarsen, D, self.length_level_0, self.periodicity)
+        if not only_refine and len(coarsen_lv):
+            t, c = _coarsen(t, coarsen_lv, coarsen_ic, D, self.length_level_0, self.periodicity)
             changed += c
         if changed:
-            self._leaves = leaves
+            self._tree = t
             self._connect()
             self.unsaved_changes = True
-        kind = np.array([o[0] for o in orig], dtype=np.int32)
-        src = np.array([o[1] for o in orig], dtype=np.int32)
-        return bool(changed), kind, src
+        return bool(changed), t.kind.astype(np.int32), t.src.astype(np.int32)
 
     def tree_arrays(self):
         """The tree as `save_mesh_file` writes it (mesh_io.jl:15-51): every cell in
@@ -267,8 +272,19 @@ class TreeMesh:
         levels [n], coordinates [n, d] (cell centres: the root's centre, then
         each child's = parent's +- child length / 2)."""
         D, per = self.ndims, self.periodicity
-        leaves = self._leaves if self._leaves is not None else self._uniform_leaves()
-        cells = _tree_cells(leaves)
+        t = self._tree if self._tree is not None else self._uniform_tree()
+        cl = _Cells(t.lv, t.ic)
+        lookup = dict(zip(cl.k.tolist(), cl.v.tolist()))
+
+        class _V:  # scalar lookups of the cell table
+            @staticmethod
+            def value(lv, ic):
+                k = lv
+                for d in range(D):
+                    k |= ic[d] << _KEY_SHIFT[d]
+                return lookup.get(k)
+
+        cells = _V
         nch = 1 << D
         order, coords = [], []
         stack = [(0, (0,) * D, self.center_level_0)]
@@ -276,7 +292,7 @@ class TreeMesh:
             lv, ic, c = stack.pop()
             order.append((lv, ic))
             coords.append(c)
-            if cells[(lv, ic)] < 0:
+            if cells.value(lv, ic) < 0:
                 child_length = self.length_level_0 / (1 << lv) / 2
                 for q in reversed(range(nch)):
                     bits = [(q >> d) & 1 for d in range(D)]
@@ -291,12 +307,18 @@ class TreeMesh:
         for k, (lv, ic) in enumerate(order):
             if lv > 0:
                 parent[k] = ids[(lv - 1, tuple(v >> 1 for v in ic))]
-            if cells[(lv, ic)] < 0:
+            if cells.value(lv, ic) < 0:
                 for q in range(nch):
                     child[k, q] = ids[(lv + 1, tuple(2 * ic[d] + ((q >> d) & 1) for d in range(D)))]
             for dirn in range(2 * D):
-                nb = _neighbour(lv, ic, dirn, per)
-                if nb is not None and (lv, nb) in ids:
+                o, nside = dirn // 2, 1 << lv
+                kk = ic[o] + (1 if dirn % 2 else -1)
+                if (kk < 0 or kk >= nside) and not per[o]:
+                    continue
+                nb = list(ic)
+                nb[o] = kk % nside
+                nb = tuple(nb)
+                if (lv, nb) in ids:
                     nbr[k, dirn] = ids[(lv, nb)]
         return parent, child, nbr, levels, np.array(coords, np.float64).reshape(n, D)
 
@@ -306,18 +328,16 @@ class TreeMesh:
         its leaves are the cells without children, in id order."""
         D = len(center_level_0)
         leaf = np.all(np.asarray(child_ids) == 0, axis=1)
-        lv = np.asarray(levels)[leaf].astype(int)
-        xc = np.asarray(coordinates, np.float64)[leaf]
+        lv = np.asarray(levels)[leaf].astype(np.int64)
+        xc = np.asarray(coordinates, np.float64)[leaf].reshape(-1, D)
         L0 = float(length_level_0)
         cmin = tuple(float(c) - L0 / 2 for c in center_level_0)
         cmax = tuple(float(c) + L0 / 2 for c in center_level_0)
         mesh = cls(cmin, cmax, initial_refinement_level=int(lv.min()), periodicity=tuple(bool(p) for p in periodicity))
-        leaves = []
-        for level, c in zip(lv.tolist(), xc):
-            h = L0 / (1 << level)
-            ic = tuple(int(round((c[d] - cmin[d]) / h - 0.5)) for d in range(D))
-            leaves.append((level, ic, tuple(float(v) for v in c)))
-        mesh._leaves = leaves
+        h = L0 / (1 << lv).astype(np.float64)
+        ic = np.rint((xc - np.array(cmin)[None, :]) / h[:, None] - 0.5).astype(np.int64)
+        n = len(lv)
+        mesh._tree = _Leaves(lv, ic, xc.copy(), np.zeros(n, np.int32), np.arange(n, dtype=np.int32))
         mesh._connect()
         mesh.unsaved_changes = False
         return mesh
@@ -330,138 +350,164 @@ class TreeMesh:
                 f"{self.coordinates_min}..{self.coordinates_max}, periodicity={self.periodicity}")
 
 
-def _tree_cells(leaves):
-    """Every cell of the tree by (level, integer coordinates): a leaf maps to its
-    element index, an inner cell (one with children) to -1."""
-    cells = {}
-    for i, (lv, ic, _) in enumerate(leaves):
-        cells[(lv, ic)] = i
-        a = ic
-        for l2 in range(lv - 1, -1, -1):
-            a = tuple(v >> 1 for v in a)
-            if (l2, a) in cells:
-                break
-            cells[(l2, a)] = -1
-    return cells
+class _Leaves:
+    """The leaves of the tree in element (depth-first) order: level, integer
+    coordinates at that level, centre, and the origin of each leaf in the
+    current adaptation (kind 0 = old leaf src, 1 + q = child q of old leaf src,
+    -1 = parent of old leaves src .. src + 2^D - 1)."""
+
+    def __init__(self, lv, ic, centers, kind, src):
+        self.lv, self.ic, self.centers, self.kind, self.src = lv, ic, centers, kind, src
+
+    def take(self, idx):
+        return _Leaves(self.lv[idx], self.ic[idx], self.centers[idx], self.kind[idx], self.src[idx])
 
 
-def _neighbour(lv, ic, dirn, per):
+_KEY_SHIFT = (6, 25, 44)  # level in bits 0-5, coordinate d in 19 bits from _KEY_SHIFT[d]
+
+
+def _keys(lv, ic):
+    k = np.asarray(lv, dtype=np.int64).copy()
+    for d in range(ic.shape[1]):
+        k |= np.asarray(ic[:, d], dtype=np.int64) << _KEY_SHIFT[d]
+    return k
+
+
+class _Cells:
+    """Every cell of the tree by key (level, integer coordinates): a leaf maps
+    to its element index, an inner cell (one with children) to -1."""
+
+    def __init__(self, lv, ic):
+        keys, vals = [_keys(lv, ic)], [np.arange(len(lv), dtype=np.int64)]
+        for level in range(int(lv.max()) if len(lv) else 0):
+            m = lv > level
+            if m.any():
+                a = np.unique(_keys(np.full(int(m.sum()), level), ic[m] >> (lv[m] - level)[:, None]))
+                keys.append(a)
+                vals.append(np.full(len(a), -1, dtype=np.int64))
+        k = np.concatenate(keys)
+        v = np.concatenate(vals)
+        o = np.argsort(k, kind="stable")
+        self.k, self.v = k[o], v[o]
+
+    def find(self, lv, ic):
+        """(found, value) for every query cell; value -2 where absent."""
+        q = _keys(lv, ic)
+        if len(self.k) == 0:
+            return np.zeros(len(q), bool), np.full(len(q), -2, np.int64)
+        pos = np.minimum(np.searchsorted(self.k, q), len(self.k) - 1)
+        found = self.k[pos] == q
+        return found, np.where(found, self.v[pos], -2)
+
+    def value(self, lv, ic):
+        f, v = self.find(np.array([lv]), np.array([ic], dtype=np.int64).reshape(1, -1))
+        return int(v[0]) if f[0] else None
+
+
+def _shift(lv, ic, dirn, per):
     """Same-level cell coordinates across face dirn (2o = -o, 2o+1 = +o),
-    wrapping periodic dimensions; None outside a non-periodic domain."""
-    o, nside = dirn // 2, 1 << lv
-    k = ic[o] + (1 if dirn % 2 else -1)
-    if (k < 0 or k >= nside) and not per[o]:
-        return None
-    nb = list(ic)
-    nb[o] = k % nside
-    return tuple(nb)
+    wrapping periodic dimensions; valid is False outside a non-periodic domain."""
+    o = dirn // 2
+    nside = np.left_shift(1, lv)
+    k = ic[:, o] + (1 if dirn % 2 else -1)
+    outside = (k < 0) | (k >= nside)
+    nb = ic.copy()
+    nb[:, o] = np.mod(k, nside)
+    valid = np.ones(len(lv), bool) if per[o] else ~outside
+    return nb, valid
 
 
-def _refine(leaves, orig, marked, D, length0):
+def _refine(t, marked, D, length0):
     """refine_unbalanced! (abstract_tree.jl:317-353): each marked leaf is
     replaced in place by its children (bit d of the child index = upper half in d)."""
-    out, oout = [], []
-    for i, leaf in enumerate(leaves):
-        if i not in marked:
-            out.append(leaf)
-            oout.append(orig[i])
-            continue
-        lv, ic, c = leaf
-        child_length = length0 / (1 << lv) / 2
-        for child in range(1 << D):
-            bits = [(child >> d) & 1 for d in range(D)]
-            out.append((lv + 1, tuple(2 * ic[d] + bits[d] for d in range(D)),
-                        tuple(c[d] + (1 if bits[d] else -1) * child_length / 2 for d in range(D))))
-            oout.append((1 + child, orig[i][1]))
-    return out, oout
+    nch = 1 << D
+    E = len(t.lv)
+    counts = np.where(marked, nch, 1)
+    idx = np.repeat(np.arange(E), counts)
+    start = np.cumsum(counts) - counts
+    q = np.arange(len(idx)) - start[idx]
+    child = marked[idx]
+    bits = (q[:, None] >> np.arange(D)[None, :]) & 1
+    lv = t.lv[idx] + child
+    ic = np.where(child[:, None], 2 * t.ic[idx] + bits, t.ic[idx])
+    child_length = length0 / (1 << t.lv[idx]).astype(np.float64) / 2
+    c = np.where(child[:, None], t.centers[idx] + np.where(bits == 1, 1.0, -1.0) * (child_length / 2)[:, None],
+                 t.centers[idx])
+    kind = np.where(child, 1 + q, t.kind[idx]).astype(np.int32)
+    return _Leaves(lv, ic, c, kind, t.src[idx].copy())
 
 
-def _refine_balanced(leaves, orig, marked, D, length0, per):
+def _refine_balanced(t, marked, D, length0, per):
     """refine! + rebalance! (abstract_tree.jl:367-400, 453-490): after each
     round, the coarse neighbour (the parent's face neighbour) of a refined cell
     without a same-level face neighbour is refined, until nothing changes."""
     count = 0
-    while marked:
-        refined = [leaves[i] for i in sorted(marked)]
-        count += len(refined)
-        leaves, orig = _refine(leaves, orig, marked, D, length0)
-        cells = _tree_cells(leaves)
-        marked = set()
-        for lv, ic, _ in refined:
-            for dirn in range(2 * D):
-                nb = _neighbour(lv, ic, dirn, per)
-                if nb is not None and (lv, nb) in cells:
-                    continue
-                if lv == 0:
-                    continue
-                pnb = _neighbour(lv - 1, tuple(v >> 1 for v in ic), dirn, per)
-                if pnb is None:
-                    continue
-                f = cells.get((lv - 1, pnb))
-                if f is not None and f >= 0:
-                    marked.add(f)
-    return leaves, orig, count
+    marked = np.asarray(marked, bool)
+    while marked.any():
+        r_lv, r_ic = t.lv[marked], t.ic[marked]
+        count += int(marked.sum())
+        t = _refine(t, marked, D, length0)
+        cells = _Cells(t.lv, t.ic)
+        marked = np.zeros(len(t.lv), bool)
+        for dirn in range(2 * D):
+            nb, valid = _shift(r_lv, r_ic, dirn, per)
+            found, _ = cells.find(r_lv, nb)
+            cand = ~(valid & found) & (r_lv > 0)
+            if not cand.any():
+                continue
+            plv, pic = r_lv[cand] - 1, r_ic[cand] >> 1
+            pnb, pvalid = _shift(plv, pic, dirn, per)
+            f2, v2 = cells.find(plv, pnb)
+            sel = pvalid & f2 & (v2 >= 0)
+            marked[v2[sel]] = True
+    return t, count
 
 
-def _coarsen(leaves, orig, keys, D, length0, per):
+def _coarsen(t, keys_lv, keys_ic, D, length0, per):
     """Coarsening of the AMR callback (amr.jl:304-370) with the balance guard
-    of coarsen! (abstract_tree.jl:516-600)."""
+    of coarsen! (abstract_tree.jl:516-600), highest level first (within one
+    level the decisions do not depend on each other)."""
     nch = 1 << D
-    cells = _tree_cells(leaves)
-    count, parents = {}, []
-    for lv, ic in keys:
-        if lv == 0:
-            continue
-        f = cells.get((lv, ic))
-        if f is None or f < 0:
-            continue  # refined by rebalancing: no longer a leaf
-        pk = (lv - 1, tuple(v >> 1 for v in ic))
-        count[pk] = count.get(pk, 0) + 1
-        if count[pk] == nch:
-            parents.append(pk)
-    parents.sort(key=lambda p: -p[0])
-    coarsened = set()
-    for lv, pic in parents:
-        children = [tuple(2 * pic[d] + ((q >> d) & 1) for d in range(D)) for q in range(nch)]
-        skip = False
-        for q, ch in enumerate(children):
+    cells = _Cells(t.lv, t.ic)
+    found, val = cells.find(keys_lv, keys_ic)
+    ok = (keys_lv > 0) & found & (val >= 0)  # refined by rebalancing: no longer a leaf
+    plv, pic = keys_lv[ok] - 1, keys_ic[ok] >> 1
+    pk = _keys(plv, pic)
+    uniq, first, cnt = np.unique(pk, return_index=True, return_counts=True)
+    full = first[cnt == nch]
+    plv, pic = plv[full], pic[full]
+    total = 0
+    for level in sorted(set(plv.tolist()), reverse=True):
+        at = plv == level
+        P_lv, P_ic = plv[at], pic[at]
+        skip = np.zeros(len(P_lv), bool)
+        for q in range(nch):
+            bits = np.array([(q >> d) & 1 for d in range(D)], dtype=np.int64)
+            ch = 2 * P_ic + bits[None, :]
             for dirn in range(2 * D):
                 bit = (q >> (dirn // 2)) & 1
                 if bit == (1 if dirn % 2 == 0 else 0):
                     continue  # the neighbour is a sibling
-                nb = _neighbour(lv + 1, ch, dirn, per)
-                if nb is None:
-                    continue
-                f = cells.get((lv + 1, nb))
-                if f is not None and f < 0:
-                    skip = True
-                    break
-            if skip:
-                break
-        if skip:
+                nb, valid = _shift(P_lv + 1, ch, dirn, per)
+                f, v = cells.find(P_lv + 1, nb)
+                skip |= valid & f & (v < 0)
+        done = _keys(P_lv[~skip], P_ic[~skip])
+        if len(done) == 0:
             continue
-        for ch in children:
-            del cells[(lv + 1, ch)]
-        cells[(lv, pic)] = 0
-        coarsened.add((lv, pic))
-    if not coarsened:
-        return leaves, orig, 0
-    out, oout = [], []
-    i = 0
-    while i < len(leaves):
-        lv, ic, c = leaves[i]
-        if lv > 0:
-            pic = tuple(v >> 1 for v in ic)
-            if (lv - 1, pic) in coarsened:
-                h = length0 / (1 << lv)
-                out.append((lv - 1, pic, tuple(c[d] + h / 2 for d in range(D))))
-                oout.append((-1, orig[i][1]))
-                i += nch
-                continue
-        out.append(leaves[i])
-        oout.append(orig[i])
-        i += 1
-    return out, oout, len(coarsened)
+        total += len(done)
+        # collapse the children of the coarsened parents (consecutive, child 0 first)
+        par = np.where(t.lv > 0, _keys(t.lv - 1, t.ic >> 1), -1)
+        inpar = (t.lv > 0) & np.isin(par, done)
+        first_child = inpar & np.all((t.ic & 1) == 0, axis=1)
+        keep = ~inpar | first_child
+        h = length0 / (1 << t.lv).astype(np.float64)
+        lv = np.where(first_child, t.lv - 1, t.lv)
+        ic = np.where(first_child[:, None], t.ic >> 1, t.ic)
+        c = np.where(first_child[:, None], t.centers + (h / 2)[:, None], t.centers)
+        kind = np.where(first_child, -1, t.kind).astype(np.int32)
+        t = _Leaves(lv, ic, c, kind, t.src).take(keep)
+        cells = _Cells(t.lv, t.ic)
+    return t, total
 
 
 def element_geometry(mesh, basis):
