@@ -137,7 +137,7 @@ def test_transfer_rejects_out_of_range_maps():
         s.close()
 
 
-def run_amr_kta(precision):
+def run_amr_kta(precision, device_steps=True):
     import trixi_hip as th
     c = golden("kta.json")["amr_3d"]
     a = c["amr"]
@@ -152,7 +152,7 @@ def run_amr_kta(precision):
                          adapt_initial_condition_only_refine=a["adapt_initial_condition_only_refine"])
     ac = th.AnalysisCallback(semi, interval=100)
     sol = th.solve(ode, th.CarpenterKennedy2N54(),
-                   callback=th.CallbackSet(ac, amr, th.StepsizeCallback(cfl=c["cfl"])))
+                   callback=th.CallbackSet(ac, amr, th.StepsizeCallback(cfl=c["cfl"])), device_steps=device_steps)
     semi.close()
     return c, sol, amr
 
@@ -187,3 +187,16 @@ def test_amr_kta_on_gpu_fp32():
     l2, linf = np.array(sol.analysis[-1]["l2"]), np.array(sol.analysis[-1]["linf"])
     np.testing.assert_allclose(l2, c["l2"], rtol=2e-4)
     np.testing.assert_allclose(linf, c["linf"], rtol=2e-3)
+
+
+@pytest.mark.gpu
+def test_amr_device_chunks_equal_host_loop():
+    """With an AMRCallback the steps between adaptations run as device-controlled
+    solves; the run equals the host-driven loop (device_steps=False) bitwise:
+    same adapted meshes, steps, final time and state."""
+    print("amr chunks vs host loop", flush=True)
+    _, sol_dev, amr_dev = run_amr_kta("fp64")
+    _, sol_host, amr_host = run_amr_kta("fp64", device_steps=False)
+    assert amr_dev.nelements_history == amr_host.nelements_history
+    assert sol_dev.nsteps == sol_host.nsteps and sol_dev.t == sol_host.t
+    assert np.array_equal(sol_dev.u, sol_host.u)

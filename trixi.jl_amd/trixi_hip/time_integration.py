@@ -121,18 +121,23 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
           poll_every=16, initial_step=0):
     """`solve(ode, CarpenterKennedy2N54(); dt, callback)` on the device.
 
-    The state lives in the HIP context for the whole run. With a
-    StepsizeCallback (interval 1) and no AMR or file-writing callback the whole
-    loop runs on the device (trixi_hip_solve: step graph with device-side dt
-    and limit_dt, periodic analysis enqueued on the device, host reads the
-    device every `poll_every` steps). Otherwise one call of trixi_hip_rk_step (a
-    hipGraph replay of all stages) advances one step and the callbacks run on
-    the host after it, in the order of the CallbackSet (as OrdinaryDiffEq
-    applies them; the elixirs order analysis, restart, solution, AMR, step
-    size), and are initialised in that order before the first step. An
-    AnalysisCallback records the initial state, every `interval`-th step and
-    the final state (analysis.jl:121-122, 143-230). `initial_step` continues the
-    accepted-step count of a restarted run (load_timestep!, save_restart.jl:150-153)."""
+    The state lives in the HIP context for the whole run. Three loops:
+    * a StepsizeCallback (interval 1) and no AMR or file-writing callback: the
+      whole run is one trixi_hip_solve (step graph with device-side dt and
+      limit_dt, periodic analysis enqueued on the device, the host reads the
+      device every `poll_every` steps);
+    * the same with an AMRCallback: device-controlled solves up to the next
+      adaptation or analysis step, which the host then runs;
+    * otherwise (file writers, a fixed dt, `device_steps=False`): one
+      trixi_hip_rk_step (a hipGraph replay of all stages) per step, the host
+      computes dt and runs the callbacks after each step in the order of the
+      CallbackSet (as OrdinaryDiffEq applies them; the elixirs order analysis,
+      restart, solution, AMR, step size).
+    Callbacks are initialised in CallbackSet order before the first step. All
+    loops take the same steps. An AnalysisCallback records the initial state,
+    every `interval`-th step and the final state (analysis.jl:121-122,
+    143-230). `initial_step` continues the accepted-step count of a restarted
+    run (load_timestep!, save_restart.jl:150-153)."""
     from .io import SaveRestartCallback, SaveSolutionCallback
     semi = ode.p
     lib = _lib.load()
@@ -155,6 +160,9 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
     on_device = bool(device_steps and use_graph and stepsize and stepsize[0].interval == 1 and not amr
                      and not writers and it == 0)
     records_on_device = bool(on_device and analysis and analysis[0].interval > 0)
+    # with AMR (and no file writers) the steps between adaptations and analyses
+    # run as device-controlled solves; the host adapts between them
+    chunked = bool(device_steps and use_graph and stepsize and stepsize[0].interval == 1 and amr and not writers)
     for cb in cbs.callbacks:  # initialisation in callback order
         if isinstance(cb, AnalysisCallback) and not records_on_device:
             cb.analyze(t0, it)
@@ -176,13 +184,41 @@ def solve(ode, alg, dt=None, callback=None, maxiters=None, use_graph=True, save_
         t, it = tout.value, nsteps.value
         if records_on_device:
             analysis[0].device_records()
-    else:
+    elif not chunked:
         if stepsize:
             dt = stepsize[0].calculate_dt(semi)
         if dt is None:
             raise ValueError("give dt or a StepsizeCallback")
     steps = 0
-    while not on_device:
+    while chunked:
+        left = [2 ** 31 - 1]
+        if amr[0].interval > 0:
+            left.append(amr[0].interval - it % amr[0].interval)
+        if analysis and analysis[0].interval > 0:
+            left.append(analysis[0].interval - it % analysis[0].interval)
+        if maxiters is not None:
+            left.append(maxiters - steps)
+        chunk = min(left)
+        _lib.check(lib.trixi_hip_set_analysis(ctx, 0, 0), ctx, "set_analysis")
+        tout, nsteps = ctypes.c_double(), ctypes.c_int()
+        rc = lib.trixi_hip_solve(ctx, a, b, c, na, float(t), float(t_end), stepsize[0].cfl, int(chunk),
+                                 int(poll_every), ctypes.byref(tout), ctypes.byref(nsteps))
+        if rc == _lib.ESTATE and b"NaN" in lib.trixi_hip_last_error(ctx):
+            raise FloatingPointError("time step size `dt` is NaN")
+        _lib.check(rc, ctx, "solve")
+        t, it, steps = tout.value, it + nsteps.value, steps + nsteps.value
+        # a solve stops early only at t_end; a step that is not the last one ends
+        # farther than sqrt(eps) from t_end (limit_dt!)
+        last = nsteps.value < chunk or t >= t_end or _isapprox(t, t_end) or \
+            (maxiters is not None and steps >= maxiters)
+        if analysis and analysis[0].interval > 0 and it % analysis[0].interval == 0 and not last:
+            analysis[0].analyze(t, it)
+        if amr[0].condition(it, last):
+            amr[0](semi)
+            ctx = semi.ctx
+        if last:
+            break
+    while not on_device and not chunked:
         if math.isnan(dt):
             raise FloatingPointError("time step size `dt` is NaN")
         final = False
