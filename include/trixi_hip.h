@@ -74,6 +74,7 @@ extern "C" {
 #define TRIXI_HIP_IC_KELVIN_HELMHOLTZ 4
 #define TRIXI_HIP_IC_FOURIER_MODES 5
 #define TRIXI_HIP_IC_DENSITY_PULSE 6 /* elixir_euler_amr.jl:15-26 */
+#define TRIXI_HIP_IC_BLAST_WAVE 7    /* 2D, elixir_euler_blast_wave_amr.jl:14-33 */
 
 /* indicator modes for trixi_hip_indicator / trixi_hip_set_indicator */
 #define TRIXI_HIP_IND_HG 0                /* IndicatorHennemannGassner */

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libtrixi_oracle.so")
 VI = {"weak": 0, "fd": 1, "hg": 2, "fv": 3}
 FLUX = {"llf": 0, "llf_naive": 1, "ranocha": 2, "central": 3}
 IC = {"constant": 0, "convergence_test": 1, "weak_blast_wave": 2, "isentropic_vortex": 3,
-      "kelvin_helmholtz": 4, "fourier_modes": 5, "density_pulse": 6}
+      "kelvin_helmholtz": 4, "fourier_modes": 5, "density_pulse": 6, "blast_wave": 7}
 
 
 class _Config(ctypes.Structure):
@@ -283,18 +283,21 @@ class Oracle:
         return bool(rc), un, kind, src
 
     def solve_amr(self, t0, t_end, cfl, controller, interval=5, adapt_initial_condition=True,
-                  adapt_initial_condition_only_refine=True, variable=0, maxiters=0):
+                  adapt_initial_condition_only_refine=True, variable=0, maxiters=0, indicator="max"):
         """solve(ode, CarpenterKennedy2N54; callback = (AMRCallback(controller;
         interval, adapt_initial_condition), StepsizeCallback(cfl))) on the
         initial condition set by set_initial_condition. The AMR callback
         adapts the initial condition until the mesh settles (amr.jl:143-180),
         then after every `interval`-th step except the final one (amr.jl:52-58);
         the step size is computed after it (callback order of the elixirs).
+        indicator: "max" (IndicatorMax of conserved variable `variable`) or "hg"
+        (IndicatorHennemannGassner with the configured alpha_max/min/smooth).
         Returns (u, t, nsteps, history of element counts)."""
         history = []
 
         def adapt(u, **kw):
-            lam = controller(self.indicator_max(u, variable), self.levels())
+            alpha = self.indicator_max(u, variable) if indicator == "max" else self.indicator_hg(u)
+            lam = controller(alpha, self.levels())
             changed, un, _, _ = self.amr_adapt(u, lam, **kw)
             history.append(self.nelements)
             return changed, un

@@ -531,7 +531,8 @@ enum {
     IC_ISENTROPIC_VORTEX = 3,
     IC_KELVIN_HELMHOLTZ = 4,
     IC_FOURIER_MODES = 5,
-    IC_DENSITY_PULSE = 6
+    IC_DENSITY_PULSE = 6,
+    IC_BLAST_WAVE = 7
 };
 
 struct oracle_config {
@@ -710,6 +711,22 @@ void initial_condition(const oracle_ctx &c, const Euler<D> &eq, const double *x,
             u[0] = rho;
             for (int d = 0; d < D; ++d) u[1 + d] = rho;
             u[V - 1] = 1 / (eq.gamma - 1) + 1.0 / 2 * rho * D;
+            break;
+        }
+        case IC_BLAST_WAVE: {
+            // examples/tree_2d_dgsem/elixir_euler_blast_wave_amr.jl:14-33: the medium
+            // blast wave (2D; p = 1e-3 outside r = 0.5)
+            double r = std::sqrt(x[0] * x[0] + x[1] * x[1]);
+            double phi = std::atan2(x[1], x[0]);
+            double w[4];
+            w[0] = r > 0.5 ? 1.0 : 1.1691;
+            w[1] = r > 0.5 ? 0.0 : 0.1882 * std::cos(phi);
+            w[2] = r > 0.5 ? 0.0 : 0.1882 * std::sin(phi);
+            w[3] = r > 0.5 ? 1.0e-3 : 1.245;
+            if (D == 2) {
+                double wp[4] = {w[0], w[1], w[2], w[3]};
+                eq.prim2cons(wp, u);
+            }
             break;
         }
         case IC_FOURIER_MODES: {

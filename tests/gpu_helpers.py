@@ -9,7 +9,7 @@ FLUX = {"llf": th.flux_lax_friedrichs, "llf_naive": th.FluxLaxFriedrichs(th.max_
         "ranocha": th.flux_ranocha, "central": th.flux_central}
 IC_ORACLE = {"constant": "constant", "convergence_test": "convergence_test", "weak_blast_wave": "weak_blast_wave",
              "isentropic_vortex": "isentropic_vortex", "kelvin_helmholtz": "kelvin_helmholtz",
-             "fourier_modes": "fourier_modes", "density_pulse": "density_pulse"}
+             "fourier_modes": "fourier_modes", "density_pulse": "density_pulse", "blast_wave": "blast_wave"}
 
 
 def make_ic(name, params=()):
@@ -27,6 +27,8 @@ def make_ic(name, params=()):
         return th.initial_condition_fourier_modes(*(params or (0,)))
     if name == "density_pulse":
         return th.initial_condition_density_pulse
+    if name == "blast_wave":
+        return th.initial_condition_blast_wave
     raise KeyError(name)
 
 

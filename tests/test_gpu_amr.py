@@ -200,3 +200,39 @@ def test_amr_device_chunks_equal_host_loop():
     assert amr_dev.nelements_history == amr_host.nelements_history
     assert sol_dev.nsteps == sol_host.nsteps and sol_dev.t == sol_host.t
     assert np.array_equal(sol_dev.u, sol_host.u)
+
+
+@pytest.mark.gpu
+def test_blast_wave_amr_kta_on_gpu_fp64():
+    """elixir_euler_blast_wave_amr.jl on the device: HG shock capturing on the
+    adapted meshes and the device HG indicator driving ControllerThreeLevel
+    (206 steps, 42 adaptations): the same element-count history as the oracle
+    (which reproduces the reference to 7e-15) and the reference's errors."""
+    import trixi_hip as th
+    from oracle import ControllerThreeLevel
+    print("blast wave amr kta fp64", flush=True)
+    c = golden("kta.json")["blast_wave_amr_2d"]
+    a = c["amr"]
+    spec = dict(c["oracle"], ic=c["ic"], ic_params=())
+    semi, _ = build(spec, "fp64")
+    vi = semi.solver.volume_integral
+    ind = th.IndicatorHennemannGassner(semi, alpha_max=vi.indicator.alpha_max, alpha_min=vi.indicator.alpha_min,
+                                       alpha_smooth=True, variable=th.density_pressure)
+    ctl = th.ControllerThreeLevel(semi, ind, base_level=a["base_level"], max_level=a["max_level"],
+                                  max_threshold=a["max_threshold"])
+    amr = th.AMRCallback(semi, ctl, interval=a["interval"], adapt_initial_condition=True,
+                         adapt_initial_condition_only_refine=True)
+    sol = th.solve(th.semidiscretize(semi, tuple(c["tspan"])), th.CarpenterKennedy2N54(),
+                   callback=th.CallbackSet(th.AnalysisCallback(semi, interval=100), amr,
+                                           th.StepsizeCallback(cfl=c["cfl"])))
+    o = oracle_for(dict(c["oracle"]))
+    o.set_initial_condition(c["ic"])
+    octl = ControllerThreeLevel(base_level=a["base_level"], max_level=a["max_level"], max_threshold=a["max_threshold"])
+    u_ref, t_ref, n_ref, hist_ref = o.solve_amr(c["tspan"][0], c["tspan"][1], c["cfl"], octl, interval=a["interval"],
+                                                indicator="hg")
+    assert amr.nelements_history == hist_ref
+    assert sol.nsteps == n_ref
+    l2, linf = np.array(sol.analysis[-1]["l2"]), np.array(sol.analysis[-1]["linf"])
+    np.testing.assert_allclose(l2, c["l2"], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(linf, c["linf"], rtol=1e-10, atol=0)
+    semi.close()

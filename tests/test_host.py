@@ -64,6 +64,9 @@ def test_tree_order_follows_child_sign_table():
     ("isentropic_vortex", 2, th.initial_condition_isentropic_vortex(25.0), (25.0,)),
     ("kelvin_helmholtz", 2, th.initial_condition_kelvin_helmholtz_instability(), ()),
     ("fourier_modes", 3, th.initial_condition_fourier_modes(2), (0.1, 0.2)),
+    ("density_pulse", 3, th.initial_condition_density_pulse, ()),
+    ("density_pulse", 2, th.initial_condition_density_pulse, ()),
+    ("blast_wave", 2, th.initial_condition_blast_wave, ()),
 ])
 def test_initial_conditions_match_oracle(name, D, ic, params):
     from oracle import Oracle

@@ -64,7 +64,7 @@ def _amr_run(c, controller_overrides=None):
     u, t, nsteps, history = o.solve_amr(c["tspan"][0], c["tspan"][1], c["cfl"], ctl, interval=a["interval"],
                                         adapt_initial_condition=a["adapt_initial_condition"],
                                         adapt_initial_condition_only_refine=a["adapt_initial_condition_only_refine"],
-                                        variable=a["variable"])
+                                        variable=a["variable"], indicator=a.get("indicator", "max"))
     return o, u, t, history
 
 
@@ -95,3 +95,26 @@ def test_planted_amr_mistake_is_caught(kta, mistake):
     l2, linf = o.error_norms(u, t)
     worst = max(np.max(np.abs(l2 / np.array(c["l2"]) - 1)), np.max(np.abs(linf / np.array(c["linf"]) - 1)))
     assert worst > 1e-11
+
+
+def test_oracle_reproduces_trixi_blast_wave_amr_kta(kta):
+    """elixir_euler_blast_wave_amr.jl: HG-indicator-driven AMR with shock
+    capturing on the adapted meshes (206 steps, 42 adaptations, mostly
+    coarsening from level 6 towards 4 away from the blast)."""
+    c = kta["blast_wave_amr_2d"]
+    o, u, t, history = _amr_run(c)
+    assert len(history) > 30 and min(history) < 1000 < max(history)
+    assert t == pytest.approx(c["tspan"][1], abs=0, rel=1e-15)
+    l2, linf = o.error_norms(u, t)
+    np.testing.assert_allclose(l2, c["l2"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(linf, c["linf"], rtol=1e-12, atol=0)
+
+
+def test_planted_blast_wave_amr_mistake_is_caught(kta):
+    """AMR decisions from IndicatorMax(rho) at the same thresholds instead of the
+    HG indicator move the errors far outside the tolerance."""
+    c = dict(kta["blast_wave_amr_2d"])
+    c["amr"] = dict(c["amr"], indicator="max", max_threshold=1.01)
+    o, u, t, _ = _amr_run(c)
+    l2, _ = o.error_norms(u, t)
+    assert np.max(np.abs(l2 / np.array(kta["blast_wave_amr_2d"]["l2"]) - 1)) > 1e-8

@@ -421,6 +421,14 @@ __device__ void exact_solution(const ExactParams &ep, const double *x, double t,
       u[V - 1] = igm1 + 0.5 * rho * D;
       break;
     }
+    case 7: {  // medium blast wave, 2D (elixir_euler_blast_wave_amr.jl:14-33)
+      const double r = sqrt(x[0] * x[0] + x[1] * x[1]);
+      const double phi = atan2(x[1], x[0]);
+      double w[4] = {r > 0.5 ? 1.0 : 1.1691, r > 0.5 ? 0.0 : 0.1882 * cos(phi), r > 0.5 ? 0.0 : 0.1882 * sin(phi),
+                     r > 0.5 ? 1.0e-3 : 1.245};
+      if (D == 2) prim2cons(w);
+      break;
+    }
     default: {  // seeded Fourier modes
       double w[5];
       for (int fld = 0; fld < 1 + D; ++fld) {

@@ -1296,7 +1296,7 @@ int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *ki
 
 int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *params, int nparams,
                                  const double *modes) {
-  if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_DENSITY_PULSE) return TRIXI_HIP_EINVAL;
+  if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_BLAST_WAVE) return TRIXI_HIP_EINVAL;
   // captured steps hold the boundary-state kernel with the old function
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);

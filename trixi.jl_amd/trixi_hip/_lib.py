@@ -42,6 +42,7 @@ IC_ISENTROPIC_VORTEX = 3
 IC_KELVIN_HELMHOLTZ = 4
 IC_FOURIER_MODES = 5
 IC_DENSITY_PULSE = 6
+IC_BLAST_WAVE = 7
 
 IND_HG = 0
 IND_MLP = 1

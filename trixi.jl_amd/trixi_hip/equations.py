@@ -274,6 +274,23 @@ def _ic_density_pulse(x, t, eq):
     return u
 
 
+def _ic_blast_wave(x, t, eq):
+    # examples/tree_2d_dgsem/elixir_euler_blast_wave_amr.jl:14-33: the medium blast
+    # wave of Hennemann & Gassner (2D; p = 1e-3 outside r = 0.5)
+    r = np.sqrt(x[..., 0] ** 2 + x[..., 1] ** 2)
+    phi = np.arctan2(x[..., 1], x[..., 0])
+    out = r > 0.5
+    prim = np.empty(x.shape[:-1] + (4,))
+    prim[..., 0] = np.where(out, 1.0, 1.1691)
+    prim[..., 1] = np.where(out, 0.0, 0.1882 * np.cos(phi))
+    prim[..., 2] = np.where(out, 0.0, 0.1882 * np.sin(phi))
+    prim[..., 3] = np.where(out, 1.0e-3, 1.245)
+    return eq.prim2cons(prim)
+
+
+initial_condition_blast_wave = InitialCondition(
+    "initial_condition_blast_wave", _ic_blast_wave, _lib.IC_BLAST_WAVE)
+
 initial_condition_density_pulse = InitialCondition(
     "initial_condition_density_pulse", _ic_density_pulse, _lib.IC_DENSITY_PULSE)
 

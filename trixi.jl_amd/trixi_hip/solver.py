@@ -39,9 +39,14 @@ class VolumeIntegralPureLGLFiniteVolume:
 
 
 class IndicatorHennemannGassner:
-    """`IndicatorHennemannGassner(equations, basis; alpha_max, alpha_min, alpha_smooth, variable)`."""
+    """`IndicatorHennemannGassner(equations, basis; alpha_max, alpha_min, alpha_smooth, variable)`
+    (indicators.jl:50-117), or `IndicatorHennemannGassner(semi; ...)` as the
+    indicator of an AMR controller. Called with a semidiscretization it returns
+    the smoothed blending factors of the resident state, computed on the device
+    (trixi_hip_indicator) with the parameters of semi's context, which must be
+    these (the volume integral's indicator, or its defaults)."""
 
-    def __init__(self, equations, basis, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
+    def __init__(self, equations, basis=None, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
                  variable="density_pressure"):
         if variable != "density_pressure":
             raise ValueError("the HIP indicator uses variable = density_pressure")
@@ -49,6 +54,19 @@ class IndicatorHennemannGassner:
         self.alpha_min = float(alpha_min)
         self.alpha_smooth = bool(alpha_smooth)
         self.variable = variable
+
+    def __call__(self, semi):
+        import ctypes
+        import numpy as np
+        cfg = semi.config()
+        if (cfg.alpha_max, cfg.alpha_min, bool(cfg.alpha_smooth)) != (self.alpha_max, self.alpha_min,
+                                                                       self.alpha_smooth):
+            raise ValueError("the device HG indicator uses the context's alpha_max, alpha_min and alpha_smooth; "
+                             "the AMR indicator must use the same")
+        alpha = np.empty(semi.nelements())
+        _lib.check(semi._lib.trixi_hip_indicator(semi.ctx, _lib.IND_HG, alpha.ctypes.data_as(ctypes.c_void_p)),
+                   semi.ctx, "trixi_hip_indicator")
+        return alpha
 
     def __repr__(self):
         return (f"IndicatorHennemannGassner({self.variable}, alpha_max={self.alpha_max}, "
