@@ -219,6 +219,17 @@ int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt);
  * indicators_{2,3}d.jl) for the resident state, after smoothing. */
 int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host);
 
+/* IndicatorHennemannGassner(semi; alpha_max, alpha_min, alpha_smooth, variable)
+ * (indicators.jl:50-156, indicators_{2,3}d.jl) of the resident FP64 state with
+ * its own parameters, as the reference builds a separate indicator for an AMR
+ * controller (e.g. elixir_euler_kelvin_helmholtz_instability_amr.jl); smoothing
+ * also across mortars. alpha_host[E] after smoothing (if alpha_smooth). */
+#define TRIXI_HIP_VAR_DENSITY_PRESSURE 0 /* density_pressure, euler_2d.jl:2205-2210 */
+#define TRIXI_HIP_VAR_DENSITY 1          /* Trixi.density */
+#define TRIXI_HIP_VAR_PRESSURE 2         /* Trixi.pressure */
+int trixi_hip_indicator_hg(trixi_hip_ctx *ctx, double alpha_max, double alpha_min, int alpha_smooth, int variable,
+                           double *alpha_host);
+
 /* IndicatorMax(semi; variable) (indicators_3d.jl:239-259, indicators_2d.jl):
  * alpha_host[e] = maximum over the nodes of element e of conserved variable
  * `variable` (0 = rho, as `variable = first`) of the resident state. The AMR

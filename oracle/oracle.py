@@ -66,6 +66,7 @@ def _load():
         L.oracle_solve.restype = I
         L.oracle_solve.argtypes = [P, P, D, D, D, I, P]
         L.oracle_indicator_max.argtypes = [P, P, I, P]
+        L.oracle_indicator_hg_params.argtypes = [P, P, D, D, I, I, P]
         L.oracle_amr_adapt.restype = I
         L.oracle_amr_adapt.argtypes = [P, P, P, I, I]
         L.oracle_get_adapted_u.argtypes = [P, P]
@@ -260,6 +261,15 @@ class Oracle:
         self.L.oracle_get_levels(self._c, _p(lv))
         return lv
 
+    def indicator_hg_params(self, u, alpha_max, alpha_min, alpha_smooth, variable="density_pressure"):
+        """IndicatorHennemannGassner with its own parameters (an AMR indicator)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        a = np.empty(self.nelements)
+        var = {"density_pressure": 0, "density": 1, "pressure": 2}[variable]
+        self.L.oracle_indicator_hg_params(self._c, _p(u), float(alpha_max), float(alpha_min), int(alpha_smooth), var,
+                                          _p(a))
+        return a
+
     def indicator_max(self, u, variable=0):
         """IndicatorMax(semi; variable): per-element maximum of a conserved variable."""
         u = np.ascontiguousarray(u, dtype=np.float64)
@@ -290,13 +300,20 @@ class Oracle:
         adapts the initial condition until the mesh settles (amr.jl:143-180),
         then after every `interval`-th step except the final one (amr.jl:52-58);
         the step size is computed after it (callback order of the elixirs).
-        indicator: "max" (IndicatorMax of conserved variable `variable`) or "hg"
-        (IndicatorHennemannGassner with the configured alpha_max/min/smooth).
+        indicator: "max" (IndicatorMax of conserved variable `variable`), "hg"
+        (IndicatorHennemannGassner with the configured alpha_max/min/smooth), or
+        a tuple (alpha_max, alpha_min, alpha_smooth, variable) for an HG
+        indicator with its own parameters.
         Returns (u, t, nsteps, history of element counts)."""
         history = []
 
         def adapt(u, **kw):
-            alpha = self.indicator_max(u, variable) if indicator == "max" else self.indicator_hg(u)
+            if indicator == "max":
+                alpha = self.indicator_max(u, variable)
+            elif indicator == "hg":
+                alpha = self.indicator_hg(u)
+            else:  # (alpha_max, alpha_min, alpha_smooth, variable) of an AMR-only HG indicator
+                alpha = self.indicator_hg_params(u, *indicator)
             lam = controller(alpha, self.levels())
             changed, un, _, _ = self.amr_adapt(u, lam, **kw)
             history.append(self.nelements)

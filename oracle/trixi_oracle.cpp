@@ -454,6 +454,13 @@ template <int D, typename R = double> struct Euler {
         for (int d = 0; d < D; ++d) lam[d] = std::fabs(w[1 + d]) + c;
     }
 
+    // pressure: euler_2d.jl (pressure(u, equations)): (gamma - 1) (rho e - |rho v|^2 / (2 rho))
+    R pressure(const R *u) const {
+        R m2 = 0;
+        for (int d = 0; d < D; ++d) m2 += u[1 + d] * u[1 + d];
+        return (gamma - 1) * (u[V - 1] - R(0.5) * m2 / u[0]);
+    }
+
     // density_pressure: euler_3d.jl:1951-1956, euler_2d.jl:2205-2210
     R density_pressure(const R *u) const {
         R m2 = 0;
@@ -598,6 +605,7 @@ struct oracle_ctx {
     std::vector<Origin> transfer;         // [E] origin of each element in the last adaptation
     std::vector<double> sfv;            // surface_flux_values
     std::vector<double> alpha, alpha_tmp;
+    int hg_variable = 0;  // HG indicator variable: 0 density_pressure, 1 density, 2 pressure
     // analysis exact-solution parameters
     int ic_kind = IC_CONSTANT;
     double ic_params[8] = {0};
@@ -916,7 +924,10 @@ void hg_indicator(oracle_ctx &c, const Euler<D> &eq, const double *u) {
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < E; ++e) {
         std::vector<double> ind(NN), t1(NN), t2(NN), modal(NN);
-        for (int node = 0; node < NN; ++node) ind[node] = eq.density_pressure(&u[V * (node + NN * e)]);
+        for (int node = 0; node < NN; ++node) {
+            const double *un = &u[V * (node + NN * e)];
+            ind[node] = c.hg_variable == 1 ? un[0] : c.hg_variable == 2 ? eq.pressure(un) : eq.density_pressure(un);
+        }
         // dimensionwise modal transform
         std::vector<double> *src = &ind;
         std::vector<double> *bufs[3] = {&t1, &t2, &modal};
@@ -1840,6 +1851,22 @@ void oracle_indicator_hg(oracle_ctx *c, const double *u, double *alpha) {
     if (c->D == 2) hg_indicator<2>(*c, Euler<2>(c->cfg.gamma), u);
     else hg_indicator<3>(*c, Euler<3>(c->cfg.gamma), u);
     std::copy(c->alpha.begin(), c->alpha.end(), alpha);
+}
+
+// IndicatorHennemannGassner with its own parameters (an AMR indicator, e.g.
+// elixir_euler_kelvin_helmholtz_instability_amr.jl): variable 0 density_pressure,
+// 1 density, 2 pressure.
+void oracle_indicator_hg_params(oracle_ctx *c, const double *u, double alpha_max, double alpha_min, int alpha_smooth,
+                                int variable, double *alpha) {
+    const oracle_config saved = c->cfg;
+    const int saved_var = c->hg_variable;
+    c->cfg.alpha_max = alpha_max;
+    c->cfg.alpha_min = alpha_min;
+    c->cfg.alpha_smooth = alpha_smooth;
+    c->hg_variable = variable;
+    oracle_indicator_hg(c, u, alpha);
+    c->cfg = saved;
+    c->hg_variable = saved_var;
 }
 
 double oracle_max_dt(const oracle_ctx *c, const double *u) {

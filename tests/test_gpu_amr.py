@@ -236,3 +236,56 @@ def test_blast_wave_amr_kta_on_gpu_fp64():
     np.testing.assert_allclose(l2, c["l2"], rtol=1e-10, atol=0)
     np.testing.assert_allclose(linf, c["linf"], rtol=1e-10, atol=0)
     semi.close()
+
+
+@pytest.mark.gpu
+def test_khi_amr_kta_on_gpu_fp64():
+    """elixir_euler_kelvin_helmholtz_instability_amr.jl on the device: an AMR-only
+    HG indicator with its own parameters (variable density, alpha_max 1, no
+    smoothing; trixi_hip_indicator_hg), med_level = current level, an
+    adaptation after every step: the oracle's element-count history and the
+    reference's errors."""
+    import trixi_hip as th
+    from oracle import ControllerThreeLevel
+    print("khi amr kta fp64", flush=True)
+    c = golden("kta.json")["khi_amr_2d"]
+    a = c["amr"]
+    spec = dict(c["oracle"], ic=c["ic"], ic_params=())
+    semi, _ = build(spec, "fp64")
+    am, amin, smooth, var = a["indicator"]
+    ind = th.IndicatorHennemannGassner(semi, alpha_max=am, alpha_min=amin, alpha_smooth=smooth, variable=th.density)
+    ctl = th.ControllerThreeLevel(semi, ind, base_level=a["base_level"], med_level=a["med_level"],
+                                  med_threshold=a["med_threshold"], max_level=a["max_level"],
+                                  max_threshold=a["max_threshold"])
+    amr = th.AMRCallback(semi, ctl, interval=a["interval"], adapt_initial_condition=True,
+                         adapt_initial_condition_only_refine=True)
+    sol = th.solve(th.semidiscretize(semi, tuple(c["tspan"])), th.CarpenterKennedy2N54(),
+                   callback=th.CallbackSet(th.AnalysisCallback(semi, interval=100), amr,
+                                           th.StepsizeCallback(cfl=c["cfl"])))
+    o = oracle_for(dict(c["oracle"]))
+    octl = ControllerThreeLevel(base_level=a["base_level"], med_level=a["med_level"], med_threshold=a["med_threshold"],
+                                max_level=a["max_level"], max_threshold=a["max_threshold"])
+    o.set_initial_condition(c["ic"])
+    _, _, n_ref, hist_ref = o.solve_amr(c["tspan"][0], c["tspan"][1], c["cfl"], octl, interval=a["interval"],
+                                        indicator=tuple(a["indicator"]))
+    assert amr.nelements_history == hist_ref and sol.nsteps == n_ref
+    l2, linf = np.array(sol.analysis[-1]["l2"]), np.array(sol.analysis[-1]["linf"])
+    np.testing.assert_allclose(l2, c["l2"], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(linf, c["linf"], rtol=1e-10, atol=0)
+    semi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variable,smooth", [("density", False), ("pressure", True), ("density_pressure", True)])
+def test_indicator_hg_with_own_parameters_matches_oracle(variable, smooth):
+    """trixi_hip_indicator_hg on a refined mesh (smoothing across mortars)."""
+    import trixi_hip as th
+    spec = _spec(2)
+    semi, _ = build(spec, "fp64")
+    o = oracle_for(spec)
+    u = _state(semi, 21)
+    upload(semi, u)
+    ind = th.IndicatorHennemannGassner(semi, alpha_max=0.7, alpha_min=0.002, alpha_smooth=smooth, variable=variable)
+    np.testing.assert_allclose(ind(semi), o.indicator_hg_params(u, 0.7, 0.002, smooth, variable), rtol=1e-12,
+                               atol=1e-14)
+    semi.close()

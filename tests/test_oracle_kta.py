@@ -64,7 +64,9 @@ def _amr_run(c, controller_overrides=None):
     u, t, nsteps, history = o.solve_amr(c["tspan"][0], c["tspan"][1], c["cfl"], ctl, interval=a["interval"],
                                         adapt_initial_condition=a["adapt_initial_condition"],
                                         adapt_initial_condition_only_refine=a["adapt_initial_condition_only_refine"],
-                                        variable=a["variable"], indicator=a.get("indicator", "max"))
+                                        variable=a["variable"],
+                                        indicator=tuple(a["indicator"]) if isinstance(a.get("indicator"), list)
+                                        else a.get("indicator", "max"))
     return o, u, t, history
 
 
@@ -118,3 +120,26 @@ def test_planted_blast_wave_amr_mistake_is_caught(kta):
     o, u, t, _ = _amr_run(c)
     l2, _ = o.error_norms(u, t)
     assert np.max(np.abs(l2 / np.array(kta["blast_wave_amr_2d"]["l2"]) - 1)) > 1e-8
+
+
+def test_oracle_reproduces_trixi_khi_amr_kta(kta):
+    """elixir_euler_kelvin_helmholtz_instability_amr.jl: an AMR-only HG indicator
+    (own parameters, variable density, no smoothing), med_level = current level,
+    an adaptation after every step."""
+    c = kta["khi_amr_2d"]
+    o, u, t, history = _amr_run(c)
+    assert len(history) > 30 and len(set(history)) > 2
+    l2, linf = o.error_norms(u, t)
+    np.testing.assert_allclose(l2, c["l2"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(linf, c["linf"], rtol=1e-12, atol=0)
+
+
+def test_planted_khi_amr_mistake_is_caught(kta):
+    """The volume integral's HG indicator (density_pressure, alpha_max 0.002,
+    smoothing) in place of the AMR indicator's own parameters moves the errors
+    outside the tolerance."""
+    c = dict(kta["khi_amr_2d"])
+    c["amr"] = dict(c["amr"], indicator="hg")
+    o, u, t, _ = _amr_run(c)
+    l2, _ = o.error_norms(u, t)
+    assert np.max(np.abs(l2 / np.array(kta["khi_amr_2d"]["l2"]) - 1)) > 1e-8

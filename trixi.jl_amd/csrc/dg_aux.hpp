@@ -20,6 +20,7 @@ namespace trixi_hip {
 // ---------------------------------------------------------------------------
 struct HGParams {
   double gamma, alpha_max, alpha_min, threshold, parameter_s;
+  int variable;  // indicator variable: 0 density_pressure, 1 density, 2 pressure
 };
 
 template <int N> struct VinvConst {
@@ -52,7 +53,10 @@ __global__ __launch_bounds__(NT) void hg_indicator_kernel(const T *__restrict__ 
     double rho = un[0], m2 = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) m2 += double(un[1 + d]) * double(un[1 + d]);
-    s_a[q] = (hp.gamma - 1) * (rho * double(un[D + 1]) - 0.5 * m2);
+    // density_pressure (euler_2d.jl:2205-2210), density, pressure
+    s_a[q] = hp.variable == 1   ? rho
+             : hp.variable == 2 ? (hp.gamma - 1) * (double(un[D + 1]) - 0.5 * m2 / rho)
+                                : (hp.gamma - 1) * (rho * double(un[D + 1]) - 0.5 * m2);
     if constexpr (FEAT) s_q[q] = s_a[q];
   }
   __syncthreads();

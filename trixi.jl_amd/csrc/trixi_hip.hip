@@ -220,7 +220,9 @@ int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
 
 // HG indicator of state u: raw alpha into alpha_out; with feat != null (2D p3)
 // also the learned indicator's features.
-template <typename T> int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat) {
+template <typename T>
+int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat, const HGParams *hp = nullptr) {
+  const HGParams params = hp ? *hp : ctx->hg;
   bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
     constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
     constexpr int NN = ipow_c(N + 1, D);
@@ -229,7 +231,7 @@ template <typename T> int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alph
     for (int i = 0; i < (N + 1) * (N + 1); ++i) vc.Vinv[i] = ctx->Vinv[i];
     int nb = (ctx->E + IPB - 1) / IPB;
     hipLaunchKernelGGL((hg_indicator_kernel<T, D, N, IPB, 256>), dim3(nb), dim3(256), 0, ctx->stream, u,
-                       alpha_out, ctx->E, ctx->hg, vc, feat);
+                       alpha_out, ctx->E, params, vc, feat);
   });
   if (!ok) return set_err(ctx, TRIXI_HIP_EINVAL, "unsupported (ndims, polydeg) for the indicator");
   HIP_TRY(hipGetLastError());
@@ -603,6 +605,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   ctx->hg.alpha_min = cfg->alpha_min;
   ctx->hg.threshold = 0.5 * std::pow(10.0, -1.8 * std::pow((double)n, 0.25));
   ctx->hg.parameter_s = std::log((1 - 0.0001) / 0.0001);
+  ctx->hg.variable = 0;
   if (const char *env = std::getenv("TRIXI_HIP_XCD_REMAP")) ctx->xcd_remap = std::atoi(env) != 0;
   ctx->exact.kind = TRIXI_HIP_IC_CONSTANT;
   ctx->exact.gamma = cfg->gamma;
@@ -1232,6 +1235,37 @@ int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float
     HIP_TRY(hipMemcpyAsync(loss, mlp_grad_buffer(ctx) + mlp::NP, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
   }
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_indicator_hg(trixi_hip_ctx *ctx, double alpha_max, double alpha_min, int alpha_smooth, int variable,
+                           double *alpha_host) {
+  if (!ctx || !alpha_host || variable < TRIXI_HIP_VAR_DENSITY_PRESSURE || variable > TRIXI_HIP_VAR_PRESSURE ||
+      !(alpha_max >= 0) || !(alpha_min >= 0))
+    return TRIXI_HIP_EINVAL;
+  HGParams hp = ctx->hg;
+  hp.alpha_max = alpha_max;
+  hp.alpha_min = alpha_min;
+  hp.variable = variable;
+  int r = launch_hg<double>(ctx, cur_u64(ctx), ctx->d_alpha_raw, nullptr, &hp);
+  if (r) return r;
+  const double *out = ctx->d_alpha_raw;
+  if (alpha_smooth) {
+    if (ctx->M > 0) {  // neighbours across mortars too
+      hipLaunchKernelGGL(hg_smooth_adj_kernel, dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+                         ctx->d_alpha_raw, ctx->d_adj_ptr, ctx->d_adj, ctx->d_alpha, ctx->E);
+    } else {
+      dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto) {
+        constexpr int D = decltype(Dc)::value;
+        hipLaunchKernelGGL((hg_smooth_kernel<D>), dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+                           ctx->d_alpha_raw, ctx->d_nbr, ctx->d_alpha, ctx->E, 1);
+      });
+    }
+    HIP_TRY(hipGetLastError());
+    out = ctx->d_alpha;
+  }
+  HIP_TRY(hipMemcpyAsync(alpha_host, out, (size_t)ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
   return TRIXI_HIP_OK;
 }
 

@@ -33,7 +33,7 @@ from .mesh import TreeMesh
 from .semidiscretization import ODEProblem, SemidiscretizationHyperbolic, rhs_hyperbolic_, semidiscretize
 from .solver import (DGSEM, IndicatorHennemannGassner, VolumeIntegralFluxDifferencing,
                      VolumeIntegralPureLGLFiniteVolume, VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm,
-                     density_pressure)
+                     density, density_pressure, pressure)
 from .time_integration import AnalysisCallback, CallbackSet, CarpenterKennedy2N54, StepsizeCallback, solve
 
 __all__ = [n for n in dir() if not n.startswith("_")]

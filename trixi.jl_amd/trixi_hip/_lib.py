@@ -44,6 +44,10 @@ IC_FOURIER_MODES = 5
 IC_DENSITY_PULSE = 6
 IC_BLAST_WAVE = 7
 
+VAR_DENSITY_PRESSURE = 0
+VAR_DENSITY = 1
+VAR_PRESSURE = 2
+
 IND_HG = 0
 IND_MLP = 1
 IND_MLP_WITH_FALLBACK = 2
@@ -136,6 +140,7 @@ SIGNATURES = [
     ("trixi_hip_solve", _I, [_P, _PD, _PD, _PD, _I, _D, _D, _D, _I, _I, _PD, _PI]),
     ("trixi_hip_max_dt", _I, [_P, _D, _PD]),
     ("trixi_hip_indicator", _I, [_P, _I, _P]),
+    ("trixi_hip_indicator_hg", _I, [_P, _D, _D, _I, _I, _P]),
     ("trixi_hip_indicator_max", _I, [_P, _I, _P]),
     ("trixi_hip_transfer", _I, [_P, _P, _P, _P]),
     ("trixi_hip_set_exact_solution", _I, [_P, _I, _PD, _I, _PD]),

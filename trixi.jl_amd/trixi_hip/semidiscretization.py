@@ -97,6 +97,8 @@ class SemidiscretizationHyperbolic:
             cfg.volume_flux = vi.volume_flux_dg.code
             cfg.fv_flux = vi.volume_flux_fv.code
             ind = vi.indicator
+            if ind.variable != "density_pressure":
+                raise ValueError("the shock-capturing volume integral's indicator uses variable = density_pressure")
             cfg.alpha_max, cfg.alpha_min, cfg.alpha_smooth = ind.alpha_max, ind.alpha_min, int(ind.alpha_smooth)
         elif isinstance(vi, VolumeIntegralPureLGLFiniteVolume):
             cfg.fv_flux = vi.volume_flux_fv.code

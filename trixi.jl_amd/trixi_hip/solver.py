@@ -42,14 +42,14 @@ class IndicatorHennemannGassner:
     """`IndicatorHennemannGassner(equations, basis; alpha_max, alpha_min, alpha_smooth, variable)`
     (indicators.jl:50-117), or `IndicatorHennemannGassner(semi; ...)` as the
     indicator of an AMR controller. Called with a semidiscretization it returns
-    the smoothed blending factors of the resident state, computed on the device
-    (trixi_hip_indicator) with the parameters of semi's context, which must be
-    these (the volume integral's indicator, or its defaults)."""
+    the blending factors of the resident state with these parameters, computed
+    on the device (trixi_hip_indicator_hg). As the volume integral's indicator
+    the variable must be density_pressure."""
 
     def __init__(self, equations, basis=None, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
                  variable="density_pressure"):
-        if variable != "density_pressure":
-            raise ValueError("the HIP indicator uses variable = density_pressure")
+        if variable not in _HG_VARIABLES:
+            raise ValueError("variable must be density_pressure, density or pressure")
         self.alpha_max = float(alpha_max)
         self.alpha_min = float(alpha_min)
         self.alpha_smooth = bool(alpha_smooth)
@@ -58,14 +58,11 @@ class IndicatorHennemannGassner:
     def __call__(self, semi):
         import ctypes
         import numpy as np
-        cfg = semi.config()
-        if (cfg.alpha_max, cfg.alpha_min, bool(cfg.alpha_smooth)) != (self.alpha_max, self.alpha_min,
-                                                                       self.alpha_smooth):
-            raise ValueError("the device HG indicator uses the context's alpha_max, alpha_min and alpha_smooth; "
-                             "the AMR indicator must use the same")
         alpha = np.empty(semi.nelements())
-        _lib.check(semi._lib.trixi_hip_indicator(semi.ctx, _lib.IND_HG, alpha.ctypes.data_as(ctypes.c_void_p)),
-                   semi.ctx, "trixi_hip_indicator")
+        _lib.check(semi._lib.trixi_hip_indicator_hg(semi.ctx, self.alpha_max, self.alpha_min, int(self.alpha_smooth),
+                                                    _HG_VARIABLES[self.variable],
+                                                    alpha.ctypes.data_as(ctypes.c_void_p)),
+                   semi.ctx, "trixi_hip_indicator_hg")
         return alpha
 
     def __repr__(self):
@@ -74,6 +71,10 @@ class IndicatorHennemannGassner:
 
 
 density_pressure = "density_pressure"
+density = "density"
+pressure = "pressure"
+_HG_VARIABLES = {"density_pressure": _lib.VAR_DENSITY_PRESSURE, "density": _lib.VAR_DENSITY,
+                 "pressure": _lib.VAR_PRESSURE}
 
 
 class VolumeIntegralShockCapturingHG:
