@@ -75,6 +75,13 @@ extern "C" {
 #define TRIXI_HIP_IC_FOURIER_MODES 5
 #define TRIXI_HIP_IC_DENSITY_PULSE 6 /* elixir_euler_amr.jl:15-26 */
 #define TRIXI_HIP_IC_BLAST_WAVE 7    /* 2D, elixir_euler_blast_wave_amr.jl:14-33 */
+#define TRIXI_HIP_IC_BLOB 8          /* 2D, elixir_euler_blob_amr.jl:13-58 */
+
+/* indicator variables of IndicatorHennemannGassner (config hg_variable,
+ * trixi_hip_indicator_hg) */
+#define TRIXI_HIP_VAR_DENSITY_PRESSURE 0 /* density_pressure, euler_2d.jl:2205-2210 */
+#define TRIXI_HIP_VAR_DENSITY 1          /* Trixi.density */
+#define TRIXI_HIP_VAR_PRESSURE 2         /* Trixi.pressure (euler_2d.jl) */
 
 /* indicator modes for trixi_hip_indicator / trixi_hip_set_indicator */
 #define TRIXI_HIP_IND_HG 0                /* IndicatorHennemannGassner */
@@ -112,6 +119,7 @@ typedef struct trixi_hip_config {
   int32_t nboundaries;        /* B: boundary faces of a non-periodic mesh (0 if periodic) */
   int32_t boundary_condition; /* TRIXI_HIP_BC_*; must be set when nboundaries > 0 */
   int32_t nmortars;           /* M: L2 mortars of a refined (non-conforming) mesh, 0 if uniform */
+  int32_t hg_variable;        /* TRIXI_HIP_VAR_*: variable of the shock-capturing indicator */
 } trixi_hip_config;
 
 /* Mesh data as held in the reference cache (containers_{2,3}d.jl). */
@@ -224,9 +232,6 @@ int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host);
  * its own parameters, as the reference builds a separate indicator for an AMR
  * controller (e.g. elixir_euler_kelvin_helmholtz_instability_amr.jl); smoothing
  * also across mortars. alpha_host[E] after smoothing (if alpha_smooth). */
-#define TRIXI_HIP_VAR_DENSITY_PRESSURE 0 /* density_pressure, euler_2d.jl:2205-2210 */
-#define TRIXI_HIP_VAR_DENSITY 1          /* Trixi.density */
-#define TRIXI_HIP_VAR_PRESSURE 2         /* Trixi.pressure */
 int trixi_hip_indicator_hg(trixi_hip_ctx *ctx, double alpha_max, double alpha_min, int alpha_smooth, int variable,
                            double *alpha_host);
 

@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libtrixi_oracle.so")
 VI = {"weak": 0, "fd": 1, "hg": 2, "fv": 3}
 FLUX = {"llf": 0, "llf_naive": 1, "ranocha": 2, "central": 3}
 IC = {"constant": 0, "convergence_test": 1, "weak_blast_wave": 2, "isentropic_vortex": 3,
-      "kelvin_helmholtz": 4, "fourier_modes": 5, "density_pulse": 6, "blast_wave": 7}
+      "kelvin_helmholtz": 4, "fourier_modes": 5, "density_pulse": 6, "blast_wave": 7, "blob": 8}
+HG_VARIABLE = {"density_pressure": 0, "density": 1, "pressure": 2}
 
 
 class _Config(ctypes.Structure):
@@ -23,7 +24,7 @@ class _Config(ctypes.Structure):
                 ("alpha_min", ctypes.c_double), ("alpha_smooth", ctypes.c_int), ("source_terms", ctypes.c_int),
                 ("periodicity", ctypes.c_int * 3), ("boundary_condition", ctypes.c_int),
                 ("npatches", ctypes.c_int), ("patch_min", (ctypes.c_double * 3) * 8),
-                ("patch_max", (ctypes.c_double * 3) * 8)]
+                ("patch_max", (ctypes.c_double * 3) * 8), ("hg_variable", ctypes.c_int)]
 
 
 def build():
@@ -101,7 +102,7 @@ class Oracle:
 
     def __init__(self, ndims, polydeg, level, cmin, cmax, volume_integral="fd", volume_flux="ranocha",
                  surface_flux="llf", fv_flux="llf", alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
-                 source="none", gamma=1.4, periodicity=True, refinement_patches=()):
+                 source="none", gamma=1.4, periodicity=True, refinement_patches=(), hg_variable="density_pressure"):
         L = _load()
         c = _Config()
         c.ndims, c.polydeg, c.initial_refinement_level = ndims, polydeg, level
@@ -126,6 +127,7 @@ class Oracle:
             for d in range(ndims):
                 c.patch_min[k][d] = pmin[d]
                 c.patch_max[k][d] = pmax[d]
+        c.hg_variable = HG_VARIABLE[hg_variable]
         self._cfg = c
         self._c = L.oracle_create(ctypes.byref(c))
         if not self._c:
@@ -265,7 +267,7 @@ class Oracle:
         """IndicatorHennemannGassner with its own parameters (an AMR indicator)."""
         u = np.ascontiguousarray(u, dtype=np.float64)
         a = np.empty(self.nelements)
-        var = {"density_pressure": 0, "density": 1, "pressure": 2}[variable]
+        var = HG_VARIABLE[variable]
         self.L.oracle_indicator_hg_params(self._c, _p(u), float(alpha_max), float(alpha_min), int(alpha_smooth), var,
                                           _p(a))
         return a
@@ -314,7 +316,10 @@ class Oracle:
                 alpha = self.indicator_hg(u)
             else:  # (alpha_max, alpha_min, alpha_smooth, variable) of an AMR-only HG indicator
                 alpha = self.indicator_hg_params(u, *indicator)
-            lam = controller(alpha, self.levels())
+            if getattr(controller, "combined", False):  # the shock-capturing indicator as the secondary one
+                lam = controller(alpha, self.levels(), self.indicator_hg(u))
+            else:
+                lam = controller(alpha, self.levels())
             changed, un, _, _ = self.amr_adapt(u, lam, **kw)
             history.append(self.nelements)
             return changed, un
@@ -410,4 +415,26 @@ class ControllerThreeLevel:
         target = np.where(alpha > self.max_threshold, self.max_level,
                           np.where(alpha > self.med_threshold,
                                    self.med_level if self.med_level > 0 else levels, self.base_level))
+        return np.sign(target - levels).astype(np.int32)
+
+
+class ControllerThreeLevelCombined(ControllerThreeLevel):
+    """ControllerThreeLevelCombined(semi, indicator_primary, indicator_secondary;
+    ..., max_threshold_secondary), amr.jl:1254-1297: the three-level target of
+    the primary indicator, raised to max_level where the secondary indicator is
+    at least max_threshold_secondary."""
+    combined = True
+
+    def __init__(self, base_level=1, med_level=None, med_threshold=0.0, max_level=None, max_threshold=1.0,
+                 max_threshold_secondary=1.0):
+        super().__init__(base_level, med_level, med_threshold, max_level, max_threshold)
+        self.max_threshold_secondary = float(max_threshold_secondary)
+
+    def __call__(self, alpha, levels, alpha_secondary):
+        alpha = np.asarray(alpha, dtype=np.float64)
+        levels = np.asarray(levels)
+        target = np.where(alpha > self.max_threshold, self.max_level,
+                          np.where(alpha > self.med_threshold,
+                                   self.med_level if self.med_level > 0 else levels, self.base_level))
+        target = np.where(np.asarray(alpha_secondary) >= self.max_threshold_secondary, self.max_level, target)
         return np.sign(target - levels).astype(np.int32)

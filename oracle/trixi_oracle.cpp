@@ -539,7 +539,8 @@ enum {
     IC_KELVIN_HELMHOLTZ = 4,
     IC_FOURIER_MODES = 5,
     IC_DENSITY_PULSE = 6,
-    IC_BLAST_WAVE = 7
+    IC_BLAST_WAVE = 7,
+    IC_BLOB = 8
 };
 
 struct oracle_config {
@@ -561,6 +562,7 @@ struct oracle_config {
     int boundary_condition;  // BC_* for every non-periodic boundary
     int npatches;         // refinement_patches of type "box", applied in order
     double patch_min[8][3], patch_max[8][3];
+    int hg_variable;      // shock-capturing indicator variable: 0 density_pressure, 1 density, 2 pressure
 };
 
 struct oracle_ctx;
@@ -735,6 +737,21 @@ void initial_condition(const oracle_ctx &c, const Euler<D> &eq, const double *x,
                 double wp[4] = {w[0], w[1], w[2], w[3]};
                 eq.prim2cons(wp, u);
             }
+            break;
+        }
+        case IC_BLOB: {
+            // examples/tree_2d_dgsem/elixir_euler_blob_amr.jl:13-58: a dense blob in a
+            // supersonic stream (Agertz et al. 2006), tanh transition of slope 2
+            const double R = 1, dens0 = 1, Chi = 10, tau_kh = 1, tau_cr = tau_kh / 1.6;
+            const double velx0 = 2 * R * std::sqrt(Chi) / tau_cr, vely0 = 0, Ma0 = 2.7;
+            const double cs = velx0 / Ma0, p0 = cs * cs * dens0 / eq.gamma;
+            const double xr = x[0] - (-15), yr = x[1] - 0;
+            const double r = std::sqrt(xr * xr + yr * yr), slope = 2;
+            const double shape = 0.5 * (1 + (std::tanh(slope * (r + R)) - (std::tanh(slope * (r - R)) + 1)));
+            const double dens = dens0 + (Chi - 1) * shape;
+            const double velx = velx0 - velx0 * shape;
+            double w[4] = {dens, velx, vely0, p0};
+            if (D == 2) eq.prim2cons(w, u);
             break;
         }
         case IC_FOURIER_MODES: {
@@ -1658,6 +1675,7 @@ oracle_ctx *oracle_create(const oracle_config *cfg) {
     c->nf = ipow(c->n, c->D - 1);
     c->V = c->D + 2;
     c->basis.init(cfg->polydeg);
+    c->hg_variable = cfg->hg_variable;
     // TreeMesh(coordinates_min, coordinates_max; initial_refinement_level,
     // refinement_patches, periodicity), tree_mesh.jl:128-215: uniform refinement,
     // then every box patch refines the leaf cells whose centre lies strictly

@@ -9,7 +9,8 @@ FLUX = {"llf": th.flux_lax_friedrichs, "llf_naive": th.FluxLaxFriedrichs(th.max_
         "ranocha": th.flux_ranocha, "central": th.flux_central}
 IC_ORACLE = {"constant": "constant", "convergence_test": "convergence_test", "weak_blast_wave": "weak_blast_wave",
              "isentropic_vortex": "isentropic_vortex", "kelvin_helmholtz": "kelvin_helmholtz",
-             "fourier_modes": "fourier_modes", "density_pulse": "density_pulse", "blast_wave": "blast_wave"}
+             "fourier_modes": "fourier_modes", "density_pulse": "density_pulse", "blast_wave": "blast_wave",
+             "blob": "blob"}
 
 
 def make_ic(name, params=()):
@@ -29,6 +30,8 @@ def make_ic(name, params=()):
         return th.initial_condition_density_pulse
     if name == "blast_wave":
         return th.initial_condition_blast_wave
+    if name == "blob":
+        return th.initial_condition_blob
     raise KeyError(name)
 
 
@@ -36,7 +39,8 @@ def build(spec, precision="fp64"):
     """spec: ndims, polydeg, level, cmin, cmax, volume_integral, volume_flux, surface_flux,
     fv_flux, alpha_max, alpha_min, source, ic, ic_params."""
     D, N = spec["ndims"], spec["polydeg"]
-    eq = th.CompressibleEulerEquations3D(1.4) if D == 3 else th.CompressibleEulerEquations2D(1.4)
+    gamma = spec.get("gamma", 1.4)
+    eq = th.CompressibleEulerEquations3D(gamma) if D == 3 else th.CompressibleEulerEquations2D(gamma)
     basis = th.LobattoLegendreBasis(N)
     vi = spec.get("volume_integral", "fd")
     vflux = spec.get("volume_flux", "ranocha")
@@ -47,7 +51,9 @@ def build(spec, precision="fp64"):
     elif vi == "fd":
         VI = th.VolumeIntegralFluxDifferencing(FLUX[vflux])
     elif vi == "hg":
-        VI = th.VolumeIntegralShockCapturingHG(th.IndicatorHennemannGassner(eq, basis, alpha_max=amax, alpha_min=amin),
+        VI = th.VolumeIntegralShockCapturingHG(th.IndicatorHennemannGassner(eq, basis, alpha_max=amax, alpha_min=amin,
+                                                                            variable=spec.get("hg_variable",
+                                                                                              "density_pressure")),
                                                volume_flux_dg=FLUX[vflux], volume_flux_fv=FLUX[fvflux])
     else:
         VI = th.VolumeIntegralPureLGLFiniteVolume(FLUX[fvflux])
@@ -70,7 +76,8 @@ def oracle_for(spec):
                surface_flux=spec.get("surface_flux", "llf"), fv_flux=spec.get("fv_flux", "llf"),
                alpha_max=spec.get("alpha_max", 0.5), alpha_min=spec.get("alpha_min", 0.001),
                source=spec.get("source", "none"), periodicity=spec.get("periodicity", True),
-               refinement_patches=spec.get("refinement_patches", ()))
+               refinement_patches=spec.get("refinement_patches", ()), gamma=spec.get("gamma", 1.4),
+               hg_variable=spec.get("hg_variable", "density_pressure"))
     if spec.get("periodicity", True) is not True:  # the boundary values come from the initial condition
         ic = make_ic(spec["ic"], spec.get("ic_params", ()))
         o.set_initial_condition(spec["ic"], tuple(ic.params), ic.modes.reshape(-1) if ic.modes is not None else None)

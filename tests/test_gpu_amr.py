@@ -289,3 +289,43 @@ def test_indicator_hg_with_own_parameters_matches_oracle(variable, smooth):
     np.testing.assert_allclose(ind(semi), o.indicator_hg_params(u, 0.7, 0.002, smooth, variable), rtol=1e-12,
                                atol=1e-14)
     semi.close()
+
+
+@pytest.mark.gpu
+def test_blob_amr_kta_on_gpu_fp64():
+    """elixir_euler_blob_amr.jl on the device: gamma 5/3, polydeg 4, shock
+    capturing on pressure, ControllerThreeLevelCombined with the device HG
+    indicators (AMR-only density indicator + the shock-capturing one), an
+    adaptation after every step: the oracle's element-count history and the
+    reference's errors."""
+    import trixi_hip as th
+    from oracle import ControllerThreeLevelCombined as OracleCombined
+    print("blob amr kta fp64", flush=True)
+    c = golden("kta.json")["blob_amr_2d"]
+    a = c["amr"]
+    spec = dict(c["oracle"], ic=c["ic"], ic_params=())
+    semi, _ = build(spec, "fp64")
+    am, amin, smooth, var = a["indicator"]
+    primary = th.IndicatorHennemannGassner(semi, alpha_max=am, alpha_min=amin, alpha_smooth=smooth, variable=var)
+    secondary = semi.solver.volume_integral.indicator
+    ctl = th.ControllerThreeLevelCombined(semi, primary, secondary, base_level=a["base_level"],
+                                          med_level=a["med_level"], med_threshold=a["med_threshold"],
+                                          max_level=a["max_level"], max_threshold=a["max_threshold"],
+                                          max_threshold_secondary=a["max_threshold_secondary"])
+    amr = th.AMRCallback(semi, ctl, interval=a["interval"], adapt_initial_condition=True,
+                         adapt_initial_condition_only_refine=True)
+    sol = th.solve(th.semidiscretize(semi, tuple(c["tspan"])), th.CarpenterKennedy2N54(),
+                   callback=th.CallbackSet(th.AnalysisCallback(semi, interval=100), amr,
+                                           th.StepsizeCallback(cfl=c["cfl"])))
+    o = oracle_for(dict(c["oracle"]))
+    o.set_initial_condition(c["ic"])
+    octl = OracleCombined(base_level=a["base_level"], med_level=a["med_level"], med_threshold=a["med_threshold"],
+                          max_level=a["max_level"], max_threshold=a["max_threshold"],
+                          max_threshold_secondary=a["max_threshold_secondary"])
+    _, _, n_ref, hist_ref = o.solve_amr(c["tspan"][0], c["tspan"][1], c["cfl"], octl, interval=a["interval"],
+                                        indicator=tuple(a["indicator"]))
+    assert amr.nelements_history == hist_ref and sol.nsteps == n_ref
+    l2, linf = np.array(sol.analysis[-1]["l2"]), np.array(sol.analysis[-1]["linf"])
+    np.testing.assert_allclose(l2, c["l2"], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(linf, c["linf"], rtol=1e-10, atol=0)
+    semi.close()

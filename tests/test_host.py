@@ -67,6 +67,7 @@ def test_tree_order_follows_child_sign_table():
     ("density_pulse", 3, th.initial_condition_density_pulse, ()),
     ("density_pulse", 2, th.initial_condition_density_pulse, ()),
     ("blast_wave", 2, th.initial_condition_blast_wave, ()),
+    ("blob", 2, th.initial_condition_blob, ()),
 ])
 def test_initial_conditions_match_oracle(name, D, ic, params):
     from oracle import Oracle

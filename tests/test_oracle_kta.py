@@ -54,13 +54,14 @@ def test_planted_boundary_mistake_is_caught(kta):
 
 
 def _amr_run(c, controller_overrides=None):
-    from oracle import ControllerThreeLevel
+    from oracle import ControllerThreeLevel, ControllerThreeLevelCombined
     a = dict(c["amr"], **(controller_overrides or {}))
     o = make_oracle(c["oracle"])
     o.set_initial_condition(c["ic"], c["ic_params"])
-    ctl = ControllerThreeLevel(base_level=a["base_level"], med_level=a["med_level"],
-                               med_threshold=a["med_threshold"], max_level=a["max_level"],
-                               max_threshold=a["max_threshold"])
+    kw = dict(base_level=a["base_level"], med_level=a["med_level"], med_threshold=a["med_threshold"],
+              max_level=a["max_level"], max_threshold=a["max_threshold"])
+    ctl = ControllerThreeLevelCombined(max_threshold_secondary=a["max_threshold_secondary"], **kw) \
+        if a.get("combined") else ControllerThreeLevel(**kw)
     u, t, nsteps, history = o.solve_amr(c["tspan"][0], c["tspan"][1], c["cfl"], ctl, interval=a["interval"],
                                         adapt_initial_condition=a["adapt_initial_condition"],
                                         adapt_initial_condition_only_refine=a["adapt_initial_condition_only_refine"],
@@ -143,3 +144,31 @@ def test_planted_khi_amr_mistake_is_caught(kta):
     o, u, t, _ = _amr_run(c)
     l2, _ = o.error_norms(u, t)
     assert np.max(np.abs(l2 / np.array(kta["khi_amr_2d"]["l2"]) - 1)) > 1e-8
+
+
+def test_oracle_reproduces_trixi_blob_amr_kta(kta):
+    """elixir_euler_blob_amr.jl: gamma 5/3, polydeg 4, the shock-capturing
+    indicator on pressure, ControllerThreeLevelCombined (an AMR-only density HG
+    indicator plus the shock-capturing one), an adaptation after every step."""
+    c = kta["blob_amr_2d"]
+    o, u, t, history = _amr_run(c)
+    assert len(history) > 100
+    l2, linf = o.error_norms(u, t)
+    np.testing.assert_allclose(l2, c["l2"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(linf, c["linf"], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mistake", [{"combined": False}, {"hg_variable": "density_pressure"}])
+def test_planted_blob_amr_mistake_is_caught(kta, mistake):
+    """Dropping the secondary indicator (the run then blows up), or the
+    shock-capturing indicator on density_pressure instead of pressure, moves
+    the errors outside the tolerance."""
+    c = dict(kta["blob_amr_2d"])
+    if "hg_variable" in mistake:
+        c["oracle"] = dict(c["oracle"], **mistake)
+        o, u, t, _ = _amr_run(c)
+    else:
+        o, u, t, _ = _amr_run(c, mistake)
+    l2, _ = o.error_norms(u, t)
+    worst = np.max(np.abs(l2 / np.array(kta["blob_amr_2d"]["l2"]) - 1))
+    assert not np.isfinite(worst) or worst > 1e-8

@@ -433,6 +433,16 @@ __device__ void exact_solution(const ExactParams &ep, const double *x, double t,
       if (D == 2) prim2cons(w);
       break;
     }
+    case 8: {  // blob in a supersonic stream, 2D (elixir_euler_blob_amr.jl:13-58)
+      const double R = 1, dens0 = 1, Chi = 10, tau_cr = 1 / 1.6;
+      const double velx0 = 2 * R * sqrt(Chi) / tau_cr, cs = velx0 / 2.7, p0 = cs * cs * dens0 / g;
+      const double xr = x[0] + 15, yr = x[1];
+      const double r = sqrt(xr * xr + yr * yr);
+      const double shape = 0.5 * (1 + (tanh(2 * (r + R)) - (tanh(2 * (r - R)) + 1)));
+      double w[4] = {dens0 + (Chi - 1) * shape, velx0 - velx0 * shape, 0.0, p0};
+      if (D == 2) prim2cons(w);
+      break;
+    }
     default: {  // seeded Fourier modes
       double w[5];
       for (int fld = 0; fld < 1 + D; ++fld) {

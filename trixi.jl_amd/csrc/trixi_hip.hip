@@ -475,6 +475,8 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   if (ctx->D != 2 && ctx->D != 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 2 or 3");
   ctx->B = cfg->nboundaries;
   ctx->M = cfg->nmortars;
+  if (cfg->hg_variable < TRIXI_HIP_VAR_DENSITY_PRESSURE || cfg->hg_variable > TRIXI_HIP_VAR_PRESSURE)
+    return fail(TRIXI_HIP_EINVAL, "hg_variable must be a TRIXI_HIP_VAR_* code");
   const int S = 1 << (ctx->D - 1);
   if (ctx->E <= 0 || ctx->B < 0 || ctx->M < 0 ||
       2 * ctx->I + ctx->B + ctx->M * (S + 1) != 2 * ctx->D * ctx->E)
@@ -605,7 +607,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   ctx->hg.alpha_min = cfg->alpha_min;
   ctx->hg.threshold = 0.5 * std::pow(10.0, -1.8 * std::pow((double)n, 0.25));
   ctx->hg.parameter_s = std::log((1 - 0.0001) / 0.0001);
-  ctx->hg.variable = 0;
+  ctx->hg.variable = cfg->hg_variable;
   if (const char *env = std::getenv("TRIXI_HIP_XCD_REMAP")) ctx->xcd_remap = std::atoi(env) != 0;
   ctx->exact.kind = TRIXI_HIP_IC_CONSTANT;
   ctx->exact.gamma = cfg->gamma;
@@ -1157,6 +1159,8 @@ int trixi_hip_mlp_get_params(trixi_hip_ctx *ctx, float *params) {
 
 int trixi_hip_set_indicator(trixi_hip_ctx *ctx, int mode) {
   if (!ctx || mode < TRIXI_HIP_IND_HG || mode > TRIXI_HIP_IND_MLP_WITH_FALLBACK) return TRIXI_HIP_EINVAL;
+  if (mode != TRIXI_HIP_IND_HG && ctx->hg.variable != TRIXI_HIP_VAR_DENSITY_PRESSURE)
+    return set_err(ctx, TRIXI_HIP_EINVAL, "the learned indicator reads density_pressure");
   if (mode != TRIXI_HIP_IND_HG && !ctx->mlp_ready)
     return set_err(ctx, TRIXI_HIP_ESTATE, "the learned indicator has no parameters (trixi_hip_mlp_set_params)");
   if (mode != ctx->ind_mode) {  // the captured steps launch the indicator kernels
@@ -1330,7 +1334,7 @@ int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *ki
 
 int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *params, int nparams,
                                  const double *modes) {
-  if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_BLAST_WAVE) return TRIXI_HIP_EINVAL;
+  if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_BLOB) return TRIXI_HIP_EINVAL;
   // captured steps hold the boundary-state kernel with the old function
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);

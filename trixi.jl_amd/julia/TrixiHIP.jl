@@ -46,6 +46,7 @@ struct HipConfig
     nboundaries::Int32
     boundary_condition::Int32
     nmortars::Int32
+    hg_variable::Int32
 end
 
 struct HipMesh
@@ -91,6 +92,11 @@ function vi_codes(v::VolumeIntegralShockCapturingHGType)
      flux_code(v.volume_integral_blend_low_order.volume_flux_fv), v.indicator)
 end
 vi_codes(v::VolumeIntegralPureLGLFiniteVolume) = (Int32(3), Int32(2), flux_code(v.volume_flux_fv), nothing)
+
+# TRIXI_HIP_VAR_* of the indicator variable
+indicator_variable_code(::typeof(Trixi.density_pressure)) = Int32(0)
+indicator_variable_code(::typeof(Trixi.density)) = Int32(1)
+indicator_variable_code(::typeof(Trixi.pressure)) = Int32(2)
 
 source_code(::Nothing) = Int32(0)
 source_code(::typeof(source_terms_convergence_test)) = Int32(1)
@@ -139,9 +145,10 @@ function HipRHS(semi; precision::Symbol = :fp64)
     vi, vflux, fvflux, indicator = vi_codes(solver.volume_integral)
     amax, amin, asmooth = indicator === nothing ? (0.5, 0.001, true) :
                           (indicator.alpha_max, indicator.alpha_min, indicator.alpha_smooth)
+    hgvar = indicator === nothing ? Int32(0) : indicator_variable_code(indicator.variable)
     cfg = HipConfig(D, Trixi.polydeg(solver), E, I, equations.gamma, precision === :fp32 ? 1 : 0,
                     vi, vflux, flux_code(solver.surface_integral.surface_flux), fvflux, amax, amin,
-                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0, B, B > 0 ? 1 : 0, M)
+                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0, B, B > 0 ? 1 : 0, M, hgvar)
     # interface container: 1-based ids and orientations -> 0-based
     nbr = Int32.(cache.interfaces.neighbor_ids .- 1)            # [2, I]
     ori = Int32.(cache.interfaces.orientations .- 1)            # [I]

@@ -15,14 +15,14 @@ All numerical work runs in libtrixi_hip.so (HIP, gfx950) through the C ABI of
 include/trixi_hip.h.
 """
 from . import _lib
-from .amr import AMRCallback, ControllerThreeLevel, IndicatorMax, first
+from .amr import AMRCallback, ControllerThreeLevel, ControllerThreeLevelCombined, IndicatorMax, first
 from .basis import LobattoLegendreBasis
 from .equations import (BoundaryConditionDirichlet, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
                         cons2cons, cons2prim, varnames,
                         FluxLaxFriedrichs, boundary_condition_periodic,
                         flux_central, flux_lax_friedrichs, flux_ranocha, fourier_modes,
                         initial_condition_constant, initial_condition_convergence_test,
-                        initial_condition_blast_wave, initial_condition_density_pulse,
+                        initial_condition_blast_wave, initial_condition_blob, initial_condition_density_pulse,
                         initial_condition_fourier_modes, initial_condition_isentropic_vortex,
                         initial_condition_kelvin_helmholtz_instability, initial_condition_weak_blast_wave,
                         max_abs_speed, max_abs_speed_naive, source_terms_convergence_test)

@@ -43,6 +43,7 @@ IC_KELVIN_HELMHOLTZ = 4
 IC_FOURIER_MODES = 5
 IC_DENSITY_PULSE = 6
 IC_BLAST_WAVE = 7
+IC_BLOB = 8
 
 VAR_DENSITY_PRESSURE = 0
 VAR_DENSITY = 1
@@ -81,6 +82,7 @@ class Config(ctypes.Structure):
         ("nboundaries", ctypes.c_int32),
         ("boundary_condition", ctypes.c_int32),
         ("nmortars", ctypes.c_int32),
+        ("hg_variable", ctypes.c_int32),
     ]
 
 

@@ -6,6 +6,7 @@ and AMRCallback, with the per-node work on the device.
   ControllerThreeLevel(semi, ind; base_level, med_level, med_threshold,
                        max_level, max_threshold)
                                     src/callbacks_step/amr.jl:1116-1153 (host, per element)
+  ControllerThreeLevelCombined      amr.jl:1254-1297
   AMRCallback(semi, controller; interval, adapt_initial_condition,
               adapt_initial_condition_only_refine)
                                     amr.jl:24-76 (condition), :143-198 (initialisation),
@@ -71,6 +72,30 @@ class ControllerThreeLevel:
         med = self.med_level if self.med_level > 0 else levels
         target = np.where(alpha > self.max_threshold, self.max_level,
                           np.where(alpha > self.med_threshold, med, self.base_level))
+        return np.sign(target - levels).astype(np.int32)
+
+
+class ControllerThreeLevelCombined(ControllerThreeLevel):
+    """`ControllerThreeLevelCombined(semi, indicator_primary, indicator_secondary;
+    base_level, med_level, med_threshold, max_level, max_threshold,
+    max_threshold_secondary)` (amr.jl:1254-1297): the three-level target of the
+    primary indicator, raised to max_level where the secondary indicator is at
+    least max_threshold_secondary."""
+
+    def __init__(self, semi, indicator_primary, indicator_secondary, base_level=1, med_level=None,
+                 med_threshold=0.0, max_level=None, max_threshold=1.0, max_threshold_secondary=1.0):
+        super().__init__(semi, indicator_primary, base_level, med_level, med_threshold, max_level, max_threshold)
+        self.indicator_secondary = indicator_secondary
+        self.max_threshold_secondary = float(max_threshold_secondary)
+
+    def __call__(self, semi):
+        alpha = self.indicator(semi)
+        alpha_secondary = self.indicator_secondary(semi)
+        levels = semi.mesh.levels
+        med = self.med_level if self.med_level > 0 else levels
+        target = np.where(alpha > self.max_threshold, self.max_level,
+                          np.where(alpha > self.med_threshold, med, self.base_level))
+        target = np.where(alpha_secondary >= self.max_threshold_secondary, self.max_level, target)
         return np.sign(target - levels).astype(np.int32)
 
 

@@ -291,6 +291,26 @@ def _ic_blast_wave(x, t, eq):
 initial_condition_blast_wave = InitialCondition(
     "initial_condition_blast_wave", _ic_blast_wave, _lib.IC_BLAST_WAVE)
 
+def _ic_blob(x, t, eq):
+    # examples/tree_2d_dgsem/elixir_euler_blob_amr.jl:13-58: a dense blob (density
+    # contrast 10, radius 1, centre (-15, 0)) in a Mach 2.7 stream (Agertz et al. 2006)
+    R, dens0, Chi = 1.0, 1.0, 10.0
+    tau_cr = 1.0 / 1.6
+    velx0 = 2 * R * math.sqrt(Chi) / tau_cr
+    c = velx0 / 2.7
+    p0 = c * c * dens0 / eq.gamma
+    r = np.sqrt((x[..., 0] - (-15)) ** 2 + (x[..., 1] - 0) ** 2)
+    shape = 0.5 * (1 + (np.tanh(2 * (r + R)) - (np.tanh(2 * (r - R)) + 1)))
+    prim = np.empty(x.shape[:-1] + (4,))
+    prim[..., 0] = dens0 + (Chi - 1) * shape
+    prim[..., 1] = velx0 - velx0 * shape
+    prim[..., 2] = 0.0
+    prim[..., 3] = p0
+    return eq.prim2cons(prim)
+
+
+initial_condition_blob = InitialCondition("initial_condition_blob", _ic_blob, _lib.IC_BLOB)
+
 initial_condition_density_pulse = InitialCondition(
     "initial_condition_density_pulse", _ic_density_pulse, _lib.IC_DENSITY_PULSE)
 

@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib
 from .equations import BoundaryConditionDirichlet, boundary_condition_periodic
 from .mesh import element_geometry
-from .solver import (VolumeIntegralFluxDifferencing, VolumeIntegralPureLGLFiniteVolume,
+from .solver import (_HG_VARIABLES, VolumeIntegralFluxDifferencing, VolumeIntegralPureLGLFiniteVolume,
                      VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm)
 
 
@@ -97,9 +97,8 @@ class SemidiscretizationHyperbolic:
             cfg.volume_flux = vi.volume_flux_dg.code
             cfg.fv_flux = vi.volume_flux_fv.code
             ind = vi.indicator
-            if ind.variable != "density_pressure":
-                raise ValueError("the shock-capturing volume integral's indicator uses variable = density_pressure")
             cfg.alpha_max, cfg.alpha_min, cfg.alpha_smooth = ind.alpha_max, ind.alpha_min, int(ind.alpha_smooth)
+            cfg.hg_variable = _HG_VARIABLES[ind.variable]
         elif isinstance(vi, VolumeIntegralPureLGLFiniteVolume):
             cfg.fv_flux = vi.volume_flux_fv.code
         elif not isinstance(vi, VolumeIntegralWeakForm):

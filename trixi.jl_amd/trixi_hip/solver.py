@@ -43,8 +43,7 @@ class IndicatorHennemannGassner:
     (indicators.jl:50-117), or `IndicatorHennemannGassner(semi; ...)` as the
     indicator of an AMR controller. Called with a semidiscretization it returns
     the blending factors of the resident state with these parameters, computed
-    on the device (trixi_hip_indicator_hg). As the volume integral's indicator
-    the variable must be density_pressure."""
+    on the device (trixi_hip_indicator_hg)."""
 
     def __init__(self, equations, basis=None, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
                  variable="density_pressure"):
