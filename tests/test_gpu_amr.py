@@ -329,3 +329,33 @@ def test_blob_amr_kta_on_gpu_fp64():
     np.testing.assert_allclose(l2, c["l2"], rtol=1e-10, atol=0)
     np.testing.assert_allclose(linf, c["linf"], rtol=1e-10, atol=0)
     semi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11)])
+def test_amr_with_dirichlet_boundaries_matches_oracle(precision, tol):
+    """AMR on a non-periodic mesh (Dirichlet boundaries from the initial
+    condition), HG shock capturing and the HG indicator driving the controller:
+    30 steps with an adaptation every 2 steps give the oracle's element-count
+    history and final state (no reference KTA covers this combination)."""
+    import trixi_hip as th
+    from oracle import ControllerThreeLevel
+    print(f"amr bounded {precision}", flush=True)
+    spec = dict(ndims=2, polydeg=3, level=5, cmin=(-2, -2), cmax=(2, 2), volume_integral="hg", volume_flux="ranocha",
+                surface_flux="llf_naive", fv_flux="llf_naive", alpha_max=0.5, alpha_min=0.001, ic="blast_wave",
+                periodicity=False)
+    semi, _ = build(spec, precision)
+    vi = semi.solver.volume_integral
+    ctl = th.ControllerThreeLevel(semi, vi.indicator, base_level=3, max_level=6, max_threshold=0.01)
+    amr = th.AMRCallback(semi, ctl, interval=2, adapt_initial_condition=True, adapt_initial_condition_only_refine=True)
+    sol = th.solve(th.semidiscretize(semi, (0.0, 1.0)), th.CarpenterKennedy2N54(),
+                   callback=th.CallbackSet(amr, th.StepsizeCallback(cfl=0.9)), maxiters=30)
+    o = oracle_for(spec)
+    u_ref, t_ref, n_ref, hist_ref = o.solve_amr(0.0, 1.0, 0.9, ControllerThreeLevel(base_level=3, max_level=6,
+                                                                                     max_threshold=0.01),
+                                                interval=2, indicator="hg", maxiters=30)
+    assert semi.mesh.nboundaries > 0 and semi.mesh.nmortars > 0
+    assert amr.nelements_history == hist_ref and sol.nsteps == n_ref == 30
+    assert sol.t == pytest.approx(t_ref, rel=1e-10 if precision == "fp64" else 1e-4)
+    assert relerr(sol.u, u_ref) < tol
+    semi.close()
