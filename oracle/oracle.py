@@ -300,7 +300,8 @@ class Oracle:
         interval, adapt_initial_condition), StepsizeCallback(cfl))) on the
         initial condition set by set_initial_condition. The AMR callback
         adapts the initial condition until the mesh settles (amr.jl:143-180),
-        then after every `interval`-th step except the final one (amr.jl:52-58);
+        then after every `interval`-th step except the last one (t_end or
+        maxiters reached, amr.jl:52-58);
         the step size is computed after it (callback order of the elixirs).
         indicator: "max" (IndicatorMax of conserved variable `variable`), "hg"
         (IndicatorHennemannGassner with the configured alpha_max/min/smooth), or
@@ -348,10 +349,12 @@ class Oracle:
             self.rk_step(u, t, dt)
             it += 1
             t += dt
-            if interval > 0 and it % interval == 0 and not final:
+            # isfinished (callbacks_step.jl:25-31): t_end reached or maxiters steps taken
+            last = final or bool(maxiters and it >= maxiters)
+            if interval > 0 and it % interval == 0 and not last:
                 _, u = adapt(u)
             dt = self.max_dt(u, cfl)
-            if final or (maxiters and it >= maxiters):
+            if last:
                 break
         return u, t, it, history
 
