@@ -172,7 +172,8 @@ def test_device_step_control_equals_host_loop(precision):
     spec = dict(c["oracle"], ic=c["ic"], ic_params=tuple(c["ic_params"]))
     semi, _ = build(spec, precision)
     runs = {}
-    for tspan, maxiters in (((0.0, 0.37), None), ((0.0, 5.0), 23)):
+    # maxiters 7: a short solve, launched without graphs
+    for tspan, maxiters in (((0.0, 0.37), None), ((0.0, 5.0), 23), ((0.0, 5.0), 7)):
         for dev, poll in ((False, 1), (True, 1), (True, 7), (True, 64)):
             ode = th.semidiscretize(semi, tspan)
             ac = th.AnalysisCallback(semi, interval=5)
@@ -182,6 +183,8 @@ def test_device_step_control_equals_host_loop(precision):
             runs[(tspan, dev, poll)] = sol
         host = runs[(tspan, False, 1)]
         assert host.nsteps == (maxiters if maxiters else host.nsteps) and host.nsteps > 5
+        if maxiters == 7:
+            assert all(runs[(tspan, True, p)].nsteps == 7 for p in (1, 7, 64))
         if maxiters is None:
             assert host.t == tspan[1]
         iters = [r["iter"] for r in host.analysis]

@@ -67,6 +67,7 @@ template <typename F> bool dispatch_dn(int D, int N, F &&f) {
 constexpr int kMaxPartials = 1 << 16;
 constexpr int kMaxSpeedBlocks = 1 << 16;  // workgroups (and partials) of the wave-speed reduction
 constexpr int kMlpMaxWaves = 1024;  // partial gradient rows of one training step
+constexpr int kEagerSolveSteps = 8;  // trixi_hip_solve with at most this many steps launches without graphs
 
 }  // namespace
 
@@ -1033,7 +1034,13 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
     *x = find_graph(ctx, k);
     return *x ? TRIXI_HIP_OK : capture_graph(ctx, k, record, x);
   };
-  {
+  // A solve of a few steps (an AMR run solves up to the next adaptation, each
+  // time on a new context) launches the steps directly: capturing and
+  // instantiating the step graphs would cost more than the replays save. The
+  // kernels and their arguments are the same; record() leaves the parity
+  // flipped as a replay does.
+  const bool eager = max_steps <= kEagerSolveSteps;
+  if (!eager) {
     hipGraphExec_t x = nullptr;
     if ((r = step_graph(&x))) return r;
     if (flip) {
@@ -1061,6 +1068,10 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
     batch = std::min(batch, std::max(1, max_steps - (int)h[4]));
     if (every > 0) batch = std::min(batch, every - (int)h[4] % every);
     for (int i = 0; i < batch; ++i) {
+      if (eager) {
+        if ((r = record())) return r;
+        continue;
+      }
       hipGraphExec_t x = nullptr;
       if ((r = step_graph(&x))) return r;
       HIP_TRY(hipGraphLaunch(x, ctx->stream));
