@@ -184,6 +184,30 @@ def test_training_reduces_loss_deterministically():
     np.testing.assert_array_equal(runs[0][2], runs[1][2])  # deterministic reduction order
 
 
+def test_training_data_from_restart_files(tmp_path):
+    """Restart files written during a KHI run (SaveRestartCallback) give the same
+    training samples (features, HG targets) as taken from the running state."""
+    import trixi_hip as th
+    from trixi_hip.learned_indicator import dataset_from_files, features
+    semi = _semi(level=4, seed=3)
+    out = str(tmp_path / "out")
+    rs = th.SaveRestartCallback(interval=2, save_final_restart=False, output_directory=out)
+    ode = th.semidiscretize(semi, (0.0, 1.0))
+    sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(rs, th.StepsizeCallback(cfl=1.3)),
+                   maxiters=6)
+    assert len(rs.files) == 3 and sol.nsteps == 6
+    X, Y = dataset_from_files(semi, rs.files)
+    # the last file holds the final state: its samples equal features of the resident final state
+    u = th.load_restart_file(semi, rs.files[-1])
+    assert np.array_equal(u, sol.u)
+    assert semi._lib.trixi_hip_upload_u(semi.ctx, sol.u.ctypes.data_as(ctypes.c_void_p)) == 0
+    f, a = features(semi)
+    E = semi.nelements()
+    assert X.shape == (3 * E, f.shape[1]) and Y.shape == (3 * E,)
+    assert np.array_equal(X[-E:], f) and np.array_equal(Y[-E:], (a / 0.002).astype(np.float32))
+    semi.close()
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "trixi.jl_amd", "trixi_hip",
                                                     "data", "indicator_nn_khi_p3.npz")),
                     reason="trained weights not present")

@@ -206,6 +206,23 @@ def collect_khi_data(seeds, level=5, t_end=3.0, nsnap=21, precision="fp64", log=
     return np.concatenate(X), np.concatenate(Y)
 
 
+def dataset_from_files(semi, files, alpha_max=0.002):
+    """Training samples from restart files (the reference's (state, alpha) logging
+    path, save_restart_dg.jl, save_solution_dg.jl): each file's conserved state
+    is loaded into semi's context, and the features and HG targets (raw alpha /
+    alpha_max) are computed on the device as during a run."""
+    from .io import load_restart_file
+    X, Y = [], []
+    lib, ctx = semi._lib, semi.ctx
+    for f in files:
+        u = load_restart_file(semi, f)
+        _lib.check(lib.trixi_hip_upload_u(ctx, u.ctypes.data_as(ctypes.c_void_p)), ctx, "upload_u")
+        x, a = features(semi)
+        X.append(x)
+        Y.append((a / alpha_max).astype(np.float32))
+    return np.concatenate(X), np.concatenate(Y)
+
+
 def train(semi, X, Y, steps=4000, batch=8192, lr=2e-3, seed=0, log=None, log_every=500):
     """Adam on the GPU from init_params(seed); minibatches drawn with
     numpy.random.default_rng(seed). Returns the trained IndicatorNeuralNetwork
