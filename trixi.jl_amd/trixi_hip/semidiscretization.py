@@ -55,10 +55,25 @@ class SemidiscretizationHyperbolic:
         self.source_terms = source_terms
         self.precision = precision
         self.tile = (int(elements_per_block), int(threads_per_block))
-        self.inverse_jacobian, self.node_coordinates = element_geometry(mesh, solver.basis)
+        self._set_geometry()
         self._ctx = None
         self._lib = _lib.load()
         self.performance_counter = {"ncalls": 0, "time": 0.0}
+
+    def _set_geometry(self):
+        # inverse_jacobian[E] (init_elements!); the node coordinates [E, NN, D]
+        # only serve host-side evaluations of the initial condition and are
+        # computed when first asked for
+        b = self.solver.basis
+        self.inverse_jacobian = 1.0 / (self.mesh.cell_lengths / float(np.sum(b.weights)))  # as element_geometry
+        self._node_coordinates = None
+
+    @property
+    def node_coordinates(self):
+        if self._node_coordinates is None:
+            inv_j, self._node_coordinates = element_geometry(self.mesh, self.solver.basis)
+            assert np.array_equal(inv_j, self.inverse_jacobian)
+        return self._node_coordinates
 
     # -- sizes ---------------------------------------------------------------
     def nvariables(self):
@@ -164,7 +179,7 @@ class SemidiscretizationHyperbolic:
         the old context's state into it (refine!/coarsen! of the DG solver,
         amr_dg3d.jl:138-430); the old context is destroyed."""
         old = self._ctx
-        self.inverse_jacobian, self.node_coordinates = element_geometry(self.mesh, self.solver.basis)
+        self._set_geometry()
         self._ctx = None
         new = self.ctx
         if old is None:
