@@ -162,3 +162,43 @@ def test_hg_indicator_matches_oracle(name):
     a_ref = o.indicator_hg(u)
     np.testing.assert_allclose(a, a_ref, rtol=1e-12, atol=1e-15)
     semi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,vi,per", [(2, "fd", True), (3, "fd", True), (2, "hg", False), (3, "weak", False)])
+def test_rhs_parity_on_randomly_adapted_meshes(D, vi, per):
+    """The RHS on meshes adapted three times with random controller values
+    (levels 1-4: mortars in every orientation, on both sides, next to
+    boundaries): FP64 to 1e-12 of the oracle, FP32 within the flux-difference
+    round-off bound."""
+    import trixi_hip as th
+    spec = dict(ndims=D, polydeg=3, level=2, cmin=(0,) * D, cmax=(2,) * D, volume_integral=vi,
+                volume_flux="ranocha", surface_flux="llf", ic="convergence_test", periodicity=per)
+    o = oracle_for(spec)
+    rng = np.random.default_rng(40 + D)
+    lams = []
+    u0 = np.zeros(o.nvalues)
+    for _ in range(3):
+        lam = rng.integers(-1, 2, o.nelements).astype(np.int32)
+        lam[o.levels() >= 4] = np.minimum(lam[o.levels() >= 4], 0)
+        lams.append(lam)
+        _, u0, _, _ = o.amr_adapt(u0, lam)
+    assert o.nmortars > 0
+    o.set_initial_condition("convergence_test")
+    u = perturbed(o.compute_coefficients(0.0), 5)
+    t = 0.21
+    du_ref = o.rhs(u, t)
+    for prec in ("fp64", "fp32"):
+        print(f"adapted-mesh RHS D={D} {vi} periodic={per} {prec}", flush=True)
+        semi, _ = build(spec, prec)
+        for lam in lams:
+            changed, kind, src = semi.mesh.adapt(lam)
+        semi.remesh(kind, src)
+        assert semi.nelements() == o.nelements and semi.mesh.nmortars == o.nmortars
+        du = np.empty_like(u)
+        th.rhs_hyperbolic_(du, u, semi, t)
+        if prec == "fp64":
+            assert relerr(du, du_ref) < 1e-12, relerr(du, du_ref)
+        else:
+            assert np.max(np.abs(du - du_ref)) < fp32_abs_bound(semi, u)
+        semi.close()
