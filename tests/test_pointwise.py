@@ -78,3 +78,39 @@ def test_llf_closed_form():
         fc = 0.5 * (O.flux(D, "physical", ul, None, o) + O.flux(D, "physical", ur, None, o))
         np.testing.assert_allclose(O.flux(D, "llf", ul, ur, o), fc - 0.5 * lam * (ur - ul), rtol=1e-14)
         np.testing.assert_allclose(O.flux(D, "llf_naive", ul, ur, o), fc - 0.5 * lam_naive * (ur - ul), rtol=1e-14)
+
+
+def test_fp32_log_mean_series_coefficients():
+    """The FP32 fast flux evaluates ln_mean and 1/ln_mean as
+    sqrt(xy) P(z) and Q(z)/sqrt(xy) with z = (a_y - a_x)^2, a = log2(x)/2
+    (math.jl:196-263 restated without divisions). The coefficients are read from
+    the HIP source and checked against sinh(w)/w and w/sinh(w), w = ln2 sqrt(z),
+    over the range the kernel admits (kFastLogSpan)."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "trixi.jl_amd", "csrc", "dg_common.hpp")).read()
+    body = src[src.index("void flux_ranocha_fast"):]
+    num = r"(-?[0-9.]+(?:e-?[0-9]+)?)f"
+    p = re.search(r"p1 = tfma\(tfma\(" + num + r", z1, " + num + r"\), z1, " + num + r"\)", body)
+    q = re.search(r"q2 = tfma\(tfma\(" + num + r", z2, " + num + r"\), z2, " + num + r"\)", body)
+    span = float(re.search(r"kFastLogSpan = " + num, src).group(1))
+    assert p and q
+    P = [float(x) for x in p.groups()]  # highest degree first
+    Q = [float(x) for x in q.groups()]
+    ln2 = math.log(2.0)
+    assert abs(span * ln2 - 0.125) < 1e-9
+    d = np.linspace(-span, span, 4001)
+    w = ln2 * d
+    z = d * d
+    ws = np.where(w == 0, 1.0, w)
+    sinhc = np.where(w == 0, 1.0, np.sinh(ws) / ws)
+    Pz = (P[0] * z + P[1]) * z + P[2]
+    Qz = (Q[0] * z + Q[1]) * z + Q[2]
+    assert np.max(np.abs(Pz / sinhc - 1)) < 1e-9
+    assert np.max(np.abs(Qz * sinhc - 1)) < 1e-8
+    # the closed form it stands for, on a pair of values: ln_mean(x, y) = (y - x)/ln(y/x)
+    x, y = 1.7, 1.7 * math.exp(0.24)
+    ax, ay = math.log2(x) / 2, math.log2(y) / 2
+    zz = (ay - ax) ** 2
+    assert abs(math.sqrt(x * y) * ((P[0] * zz + P[1]) * zz + P[2]) / ((y - x) / math.log(y / x)) - 1) < 1e-9
+    assert abs((Q[0] * zz + Q[1]) * zz + Q[2] - math.sqrt(x * y) * math.log(y / x) / (y - x)) < 1e-8

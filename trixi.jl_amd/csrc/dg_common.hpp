@@ -197,22 +197,25 @@ __device__ __forceinline__ void flux_ranocha_r(const NodeR<T, D> &l, const NodeR
 // w = ln(y/x)/2 = ln2 (a_y - a_x), the logarithmic mean satisfies exactly
 //   ln_mean(x, y) = sqrt(x y) sinh(w)/w = 2^(a_x + a_y) * P(w^2),
 //   1/ln_mean(x, y) = 2^-(a_x + a_y) * Q(w^2),
-// P(w^2) = sum_k w^(2k)/(2k+1)!,  Q = w/sinh(w) = 1 - w^2/6 + 7 w^4/360 - 31 w^6/15120 + 127 w^8/604800.
-// Truncated after w^8 the relative error is <= 2.2e-8 for |w| <= 0.5 (y/x <= e),
-// below FP32 round-off; callers use this only when every pair of the line is
-// within that range (flux_ranocha_r otherwise). Coefficients below are pre-
-// multiplied by ln2^(2k) so that the series variable is (a_y - a_x)^2. In the
-// square-root representation (SQ) the factors 2^(a_x + a_y) = sqrt(x y) are
-// products of the per-node roots, otherwise one exp2 each.
+// P(w^2) = sum_k w^(2k)/(2k+1)! = 1 + w^2/6 + w^4/120 + w^6/5040 + ...,
+// Q = w/sinh(w) = 1 - w^2/6 + 7 w^4/360 - 31 w^6/15120 + ...
+// Truncated after w^4 the relative error is <= 7.6e-10 (P) and 7.8e-9 (Q) for
+// |w| <= 1/8 (y/x <= e^(1/4)), below FP32 round-off; callers use this only when
+// every pair of the line is within that range (flux_ranocha_r otherwise).
+// Coefficients below are pre-multiplied by ln2^(2k) so that the series variable
+// is (a_y - a_x)^2. In the square-root representation (SQ) the factors
+// 2^(a_x + a_y) = sqrt(x y) are products of the per-node roots, otherwise one
+// exp2 each. The energy component is returned halved, f[D+1] = f_energy / 2,
+// from half_inv_gamma_m1 = 1/(2 (gamma - 1)): callers accumulate it with twice
+// the derivative weights, which is exact (power-of-two scalings) and saves the
+// two doublings of flux_ranocha_r.
 template <int D, int O, bool SQ>
 __device__ __forceinline__ void flux_ranocha_fast(const NodeR<float, D> &l, const NodeR<float, D> &r,
-                                                  float inv_gamma_m1, float *f) {
+                                                  float half_inv_gamma_m1, float *f) {
   const float d1 = r.lrho - l.lrho, z1 = d1 * d1;
-  const float p1 = tfma(tfma(tfma(tfma(1.46838742113e-7f, z1, 2.20050434191e-5f), z1, 0.00192362582153f), z1,
-                            0.0800755023197f), z1, 1.0f);
+  const float p1 = tfma(tfma(0.00192362582153f, z1, 0.0800755023197f), z1, 1.0f);
   const float d2 = r.lbeta - l.lbeta, z2 = d2 * d2;
-  const float q2 = tfma(tfma(tfma(tfma(1.1189112149e-5f, z2, -0.000227385448664f), z2, 0.00448846025023f), z2,
-                            -0.0800755023197f), z2, 1.0f);
+  const float q2 = tfma(tfma(0.00448846025023f, z2, -0.0800755023197f), z2, 1.0f);
   float rho_mean, inv_rho_p_mean;
   if constexpr (SQ) {
     rho_mean = (l.r * r.r) * p1;
@@ -235,12 +238,12 @@ __device__ __forceinline__ void flux_ranocha_fast(const NodeR<float, D> &l, cons
 #pragma unroll
   for (int d = 0; d < D; ++d) f[1 + d] = f1 * vavg[d];
   f[1 + O] += p_avg;
-  const float inner = tfma(inv_rho_p_mean, inv_gamma_m1, hh + hh);
-  f[D + 1] = tfma(f1, inner, cross + cross);
+  const float inner_h = tfma(inv_rho_p_mean, half_inv_gamma_m1, hh);
+  f[D + 1] = tfma(f1, inner_h, cross);  // half the energy flux
 }
 
-// |a_y - a_x| bound of the FP32 fast path: |w| <= 0.5  <=>  |a_y - a_x| <= 0.5/ln2
-constexpr float kFastLogSpan = 0.7213475204f;
+// |a_y - a_x| bound of the FP32 fast path: |w| <= 1/8  <=>  |a_y - a_x| <= 1/(8 ln2)
+constexpr float kFastLogSpan = 0.1803368801f;
 
 // Physical flux of a conservative state in orientation o (runtime), plus the
 // primitive data LLF needs.

@@ -40,6 +40,7 @@ namespace trixi_hip {
 template <typename T, int N> struct DGConst {
   static constexpr int n = N + 1;
   T Dsplit[n * n];  // derivative_split, row-major [a*n + b] = Dsplit[a, b]
+  T Dsplit2[n * n];  // 2 * derivative_split (halved energy flux of flux_ranocha_fast)
   T Dhat[n * n];    // derivative_hat
   T inv_w[n];       // inverse_weights
   double nodes[n];  // LGL nodes (coordinates of sources)
@@ -372,13 +373,15 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
 #pragma unroll
           for (int j = i + 1; j < n; ++j) {
             T f[V];
-            flux_ranocha_fast<D, O, SQ>(w[i], w[j], igm1, f);
+            flux_ranocha_fast<D, O, SQ>(w[i], w[j], T(0.5) * igm1, f);
             const T di = k.Dsplit[i * n + j], dj = k.Dsplit[j * n + i];
 #pragma unroll
-            for (int v = 0; v < V; ++v) {
+            for (int v = 0; v < V - 1; ++v) {
               acc[i][v] = tfma(di, f[v], acc[i][v]);
               acc[j][v] = tfma(dj, f[v], acc[j][v]);
             }
+            acc[i][V - 1] = tfma(k.Dsplit2[i * n + j], f[V - 1], acc[i][V - 1]);
+            acc[j][V - 1] = tfma(k.Dsplit2[j * n + i], f[V - 1], acc[j][V - 1]);
           }
       }
     } else {

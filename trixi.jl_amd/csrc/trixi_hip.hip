@@ -185,6 +185,7 @@ template <typename T> void build_kconst(trixi_hip_ctx *ctx) {
     DGConst<T, N> k{};
     for (int i = 0; i < n * n; ++i) {
       k.Dsplit[i] = T(ctx->Dsplit[i]);
+      k.Dsplit2[i] = T(2.0 * ctx->Dsplit[i]);
       k.Dhat[i] = T(ctx->Dhat[i]);
     }
     for (int i = 0; i < n; ++i) {
