@@ -196,7 +196,7 @@ int trixi_hip_rhs_native(trixi_hip_ctx *ctx, void *du_dev, const void *u_dev, do
 
 /* One 2N low-storage stage on the resident state (methods_2N.jl:146-159):
  *   du = rhs(u, t_stage); u_tmp = du - a_stage*u_tmp; u += b_stage_dt*u_tmp
- * u and u_tmp are FP64. first_stage != 0 zeroes u_tmp first (methods_2N.jl:143). */
+ * u is FP64; u_tmp is stored in the RHS precision (FP32 in FP32 mode) and updated in FP64. first_stage != 0 zeroes u_tmp first (methods_2N.jl:143). */
 int trixi_hip_rk_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage_dt,
                        double t_stage, int first_stage);
 

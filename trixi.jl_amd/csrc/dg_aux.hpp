@@ -164,11 +164,12 @@ __global__ void hg_smooth_kernel(const double *__restrict__ alpha_raw,
 // ---------------------------------------------------------------------------
 // 2N low-storage stage update (methods_2N.jl:153-157):
 //   u_tmp = du - u_tmp * a_stage;  u += u_tmp * b_stage_dt
-// u, u_tmp FP64 registers; du in the RHS precision; optional FP32 shadow of u.
+// u FP64 register; du and u_tmp stored in the RHS precision TD, arithmetic in
+// FP64; optional FP32 shadow of u.
 // b_stage_dt = b_stage * dt with dt taken from tdev[1] when tdev != null.
 // ---------------------------------------------------------------------------
 template <typename TD>
-__global__ void rk2n_update_kernel(const TD *__restrict__ du, double *__restrict__ u_tmp,
+__global__ void rk2n_update_kernel(const TD *__restrict__ du, TD *__restrict__ u_tmp,
                                    double *__restrict__ u, float *__restrict__ u32, size_t count,
                                    double a_stage, double b_stage, const double *__restrict__ tdev,
                                    int first_stage) {
@@ -176,10 +177,10 @@ __global__ void rk2n_update_kernel(const TD *__restrict__ du, double *__restrict
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < count; i += stride) {
-    double ut = first_stage ? 0.0 : u_tmp[i];
+    double ut = first_stage ? 0.0 : double(u_tmp[i]);
     ut = double(du[i]) - ut * a_stage;
     double un = tfma(ut, b_dt, u[i]);
-    u_tmp[i] = ut;
+    u_tmp[i] = TD(ut);
     u[i] = un;
     if (u32) u32[i] = float(un);
   }

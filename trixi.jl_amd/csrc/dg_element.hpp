@@ -67,9 +67,11 @@ template <typename T> struct ElemArgs {
   int xcd_remap;
   // fused 2N stage (STAGE variants, methods_2N.jl:153-157): instead of storing
   // du, u_tmp = du - a_stage u_tmp and u_reg_out = u_reg_in + b_dt u_tmp, with
+  // the arithmetic in FP64 and u_tmp stored in the RHS precision T (it carries
+  // RHS values only, so FP32 storage adds no error beyond the FP32 RHS itself),
   // b_dt = b_stage * tdev[1] (or b_stage without tdev); u32_out receives the
   // FP32 shadow of u_reg_out (FP32 mode). u_reg_in == u_reg_out is allowed.
-  double *u_tmp;
+  T *u_tmp;
   const double *u_reg_in;
   double *u_reg_out;
   float *u32_out;
@@ -693,7 +695,7 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
 #pragma unroll
         for (int w = 0; w < W; ++w) {
           const size_t g = base + (size_t)ic * W + w;
-          pre_t[r * W + w] = a.first_stage ? 0.0 : a.u_tmp[g];
+          pre_t[r * W + w] = a.first_stage ? 0.0 : double(a.u_tmp[g]);
           pre_u[r * W + w] = a.u_reg_in[g];
         }
       }
@@ -744,7 +746,7 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
     auto update = [&](size_t g, T du_v, double ut, double ur) {
       ut = double(du_v) - ut * a.a_stage;
       const double un = tfma(ut, b_dt, ur);
-      a.u_tmp[g] = ut;
+      a.u_tmp[g] = T(ut);
       a.u_reg_out[g] = un;
       if (a.u32_out) a.u32_out[g] = float(un);
     };
@@ -761,7 +763,7 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
     } else {
       for (int i = tid; i < ne * NN * V; i += NT) {
         const size_t g = base + i;
-        update(g, s_o[i], a.first_stage ? 0.0 : a.u_tmp[g], a.u_reg_in[g]);
+        update(g, s_o[i], a.first_stage ? 0.0 : double(a.u_tmp[g]), a.u_reg_in[g]);
       }
     }
   } else if (full) {

@@ -95,11 +95,13 @@ struct trixi_hip_ctx {
   bool ghost_ready = false;      // ghost holds the states of a time-independent boundary function
   void *d_ghost = nullptr;       // [B, n^(D-1), V] outer boundary states, native precision
   double *d_invj = nullptr, *d_centers = nullptr;
-  // resident state: FP64 RK registers u (a pair in FP64 mode, one buffer in
-  // FP32 mode) and u_tmp, the FP32 shadow of u (a pair in FP32 mode). The fused
+  // resident state: the FP64 RK register u (a pair in FP64 mode, one buffer in
+  // FP32 mode), the 2N register u_tmp in the RHS precision, the FP32 shadow of
+  // u (a pair in FP32 mode). The fused
   // stage kernel reads the native state of buffer par and writes the other one
   // (neighbour faces are read while other workgroups write), then par flips.
-  double *d_ub[2] = {nullptr, nullptr}, *d_utmp = nullptr;
+  double *d_ub[2] = {nullptr, nullptr};
+  void *d_utmp = nullptr;
   float *d_u32b[2] = {nullptr, nullptr};
   int par = 0;
   void *d_du = nullptr;  // native precision
@@ -163,7 +165,7 @@ const void *native_u(trixi_hip_ctx *c) { return c->fp32 ? (const void *)cur_u32(
 
 // outputs of the fused stage kernel (ElemArgs fields of the STAGE variants)
 struct StageOut {
-  double *u_tmp = nullptr;
+  void *u_tmp = nullptr;
   const double *u_reg_in = nullptr;
   double *u_reg_out = nullptr;
   float *u32_out = nullptr;
@@ -331,7 +333,7 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   a.inv_gamma_m1 = T(1) / (a.gamma - T(1));  // the device expression, evaluated once in T
   a.sqrt_gamma = T(std::sqrt(ctx->cfg.gamma));
   if (so) {
-    a.u_tmp = so->u_tmp;
+    a.u_tmp = (T *)so->u_tmp;
     a.u_reg_in = so->u_reg_in;
     a.u_reg_out = so->u_reg_out;
     a.u32_out = so->u32_out;
@@ -367,11 +369,11 @@ int launch_update(trixi_hip_ctx *ctx, double a_stage, double b_stage, const doub
   size_t cnt = ctx->nvals;
   if (ctx->fp32)
     hipLaunchKernelGGL((rk2n_update_kernel<float>), dim3(grid_for(cnt)), dim3(256), 0, ctx->stream,
-                       (const float *)ctx->d_du, ctx->d_utmp, cur_u64(ctx), cur_u32(ctx), cnt, a_stage, b_stage,
+                       (const float *)ctx->d_du, (float *)ctx->d_utmp, cur_u64(ctx), cur_u32(ctx), cnt, a_stage, b_stage,
                        tdev, first);
   else
     hipLaunchKernelGGL((rk2n_update_kernel<double>), dim3(grid_for(cnt)), dim3(256), 0, ctx->stream,
-                       (const double *)ctx->d_du, ctx->d_utmp, cur_u64(ctx), (float *)nullptr, cnt, a_stage,
+                       (const double *)ctx->d_du, (double *)ctx->d_utmp, cur_u64(ctx), (float *)nullptr, cnt, a_stage,
                        b_stage, tdev, first);
   HIP_TRY(hipGetLastError());
   return TRIXI_HIP_OK;
@@ -632,7 +634,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   bool ok = hipMalloc(&ctx->d_nbr, nbr.size() * 4) == hipSuccess &&
             hipMalloc(&ctx->d_invj, E * 8) == hipSuccess &&
             hipMalloc(&ctx->d_centers, (size_t)D * E * 8) == hipSuccess &&
-            hipMalloc(&ctx->d_ub[0], nv * 8) == hipSuccess && hipMalloc(&ctx->d_utmp, nv * 8) == hipSuccess &&
+            hipMalloc(&ctx->d_ub[0], nv * 8) == hipSuccess && hipMalloc(&ctx->d_utmp, nv * tsz) == hipSuccess &&
             (ctx->fp32 || hipMalloc(&ctx->d_ub[1], nv * 8) == hipSuccess) &&
             hipMalloc(&ctx->d_du, nv * tsz) == hipSuccess &&
             hipMalloc(&ctx->d_alpha_raw, E * 8) == hipSuccess && hipMalloc(&ctx->d_alpha, E * 8) == hipSuccess &&
