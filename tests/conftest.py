@@ -11,6 +11,12 @@ for p in (ROOT, PKG):
     if p not in sys.path:
         sys.path.insert(0, p)
 
+# Under pytest-xdist every worker would start one OpenMP thread per CPU in the
+# oracle; oversubscribed, its spin-waiting barriers slow the small 2D runs by
+# orders of magnitude. One thread per worker (set before libgomp loads).
+if os.environ.get("PYTEST_XDIST_WORKER"):
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
