@@ -24,8 +24,8 @@ class SemidiscretizationHyperbolic:
     """`SemidiscretizationHyperbolic(mesh, equations, initial_condition, solver;
     source_terms, boundary_conditions)` with a HIP backend.
 
-    precision: "fp64" (plain FP64) or "fp32" (FP32 fluxes, FP64 RK registers
-    and reductions). elements_per_block / threads_per_block override the
+    precision: "fp64" (plain FP64) or "fp32" (FP32 fluxes and 2N register
+    u_tmp; the state u, the stage arithmetic and reductions in FP64). elements_per_block / threads_per_block override the
     element-kernel tile (0 = default). boundary_conditions: periodic (for a
     periodic mesh) or one BoundaryConditionDirichlet(initial_condition) for
     every boundary of a non-periodic mesh."""
