@@ -184,10 +184,21 @@ def equal_l2(th, semi64, semi32, nsteps=100, cfl=0.6):
                 step_wall_ms_fp32=1e3 * out["fp32"]["wall_s"] / out["fp32"]["nsteps"])
 
 
+def host_cores():
+    """Cores this process may run on (its affinity mask), the machine's count,
+    and the OpenMP setting the job was started with."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return dict(available=avail, cpu_count=os.cpu_count(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                omp_proc_bind=os.environ.get("OMP_PROC_BIND"), omp_places=os.environ.get("OMP_PLACES"))
+
+
 def cpu_baseline(level, polydeg, seconds, u0):
+    """The oracle's FP64 RHS on every core this process may use, one OpenMP
+    thread per core, threads pinned (OMP_PROC_BIND/OMP_PLACES are set before
+    the OpenMP runtime starts, see main)."""
     from oracle import Oracle
-    ncpu = os.cpu_count() or 1
-    threads = min(16, ncpu)
+    hc = host_cores()
+    threads = hc["available"]
     Oracle.set_threads(threads)
     o = Oracle(3, polydeg, level, (0, 0, 0), (2, 2, 2), volume_integral="fd", volume_flux="ranocha",
                surface_flux="llf", source="convergence_test")
@@ -203,8 +214,10 @@ def cpu_baseline(level, polydeg, seconds, u0):
     t_rhs = el / n
     return dict(value=o.ndofs / t_rhs, unit="DOF-updates/s", cores=threads, kind="port",
                 sample=f"{n} full RHS calls of the same 3D p3 32^3 workload (FP64 C++ OpenMP restatement "
-                       f"of Trixi rhs_hyperbolic!, not Trixi/Julia), {el:.1f} s",
-                time_per_dof_per_rhs_s=t_rhs * threads / o.ndofs)
+                       f"of Trixi rhs_hyperbolic!, not Trixi/Julia), {el:.1f} s, {threads} threads on the "
+                       f"{hc['available']} cores of the process affinity mask (os.cpu_count() {hc['cpu_count']}, "
+                       f"OMP_PROC_BIND={hc['omp_proc_bind']}, OMP_PLACES={hc['omp_places']})",
+                time_per_dof_per_rhs_s=t_rhs * threads / o.ndofs, host=hc)
 
 
 def measured_traffic(level, polydeg, precision):
@@ -282,6 +295,10 @@ def io_timing(th, semi, u0, directory):
 
 def main():
     args = parse()
+    # pin the CPU baseline's OpenMP threads, one per core; this must happen
+    # before the first OpenMP runtime is loaded (torch loads one)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built "
               "in this round.", file=sys.stderr)

@@ -114,8 +114,9 @@ typedef struct trixi_hip_config {
   double alpha_min;
   int32_t alpha_smooth;
   int32_t source_terms;    /* TRIXI_HIP_SRC_* */
-  int32_t elements_per_block; /* 0 = autotuned default; else workgroup tile (elements) */
-  int32_t threads_per_block;  /* 0 = autotuned default */
+  int32_t elements_per_block; /* 0 = default tile of each element-kernel variant (see
+                                 trixi_hip_autotune); else the workgroup tile (elements) */
+  int32_t threads_per_block;  /* 0 = default; else threads per workgroup of that tile */
   int32_t nboundaries;        /* B: boundary faces of a non-periodic mesh (0 if periodic) */
   int32_t boundary_condition; /* TRIXI_HIP_BC_*; must be set when nboundaries > 0 */
   int32_t nmortars;           /* M: L2 mortars of a refined (non-conforming) mesh, 0 if uniform */
@@ -262,7 +263,9 @@ int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *ki
  * VolumeIntegralShockCapturingHG (indicators.jl:119-156) when selected with
  * trixi_hip_set_indicator. The network reads features computed from the HG
  * indicator's own quantities (indicators_2d.jl:30-102), alpha = alpha_max *
- * sigmoid(output); smoothing (indicators_2d.jl:104-135) is applied as for HG. */
+ * sigmoid(output), clipped as HG clips its factor (indicators_2d.jl:86-99: below
+ * alpha_min 0, above 1 - alpha_min 1, at most alpha_max); smoothing
+ * (indicators_2d.jl:104-135) is applied as for HG. */
 int trixi_hip_mlp_set_params(trixi_hip_ctx *ctx, const float *params, const float *feature_lo,
                              const float *feature_hi); /* params [NPARAMS]; box [NFEATURES] each; resets Adam */
 int trixi_hip_mlp_get_params(trixi_hip_ctx *ctx, float *params);
@@ -302,8 +305,44 @@ int trixi_hip_set_analysis(trixi_hip_ctx *ctx, int interval, int capacity);
  * entropy integral} as trixi_hip_error_norms and trixi_hip_integrals give them. */
 int trixi_hip_get_analysis(trixi_hip_ctx *ctx, double *records, int max_records, int *count);
 
-/* Workgroup tile used by the element kernel: elements per block and threads. */
+/* Device allocations and host synchronisations (stream synchronisations and
+ * synchronous copies) the library has made for this context since it was
+ * created, creation included. The counterpart of the reference's allocation
+ * test of the RHS, @test_allocations(Trixi.rhs_hyperbolic!, semi, sol, 1000)
+ * (test/test_tree_3d_euler.jl:23): after set-up, trixi_hip_rhs_native,
+ * trixi_hip_rk_stage and trixi_hip_rk_step add to neither count, and
+ * trixi_hip_solve synchronises once per poll of its control block. Either
+ * pointer may be NULL. */
+int trixi_hip_get_counters(const trixi_hip_ctx *ctx, int64_t *device_allocations, int64_t *host_synchronizations);
+
+/* Workgroup tile of the element kernel (trixi_hip_get_tile: the RHS variant;
+ * trixi_hip_get_stage_tile: the fused 2N-stage variant, 0 when the stage runs
+ * unfused): elements per block and threads. */
 int trixi_hip_get_tile(const trixi_hip_ctx *ctx, int *elements_per_block, int *threads_per_block);
+int trixi_hip_get_stage_tile(const trixi_hip_ctx *ctx, int *elements_per_block, int *threads_per_block);
+
+/* Tile autotuner. No counterpart in the reference (its GPU launch shapes are
+ * fixed). Times every instantiated workgroup tile of the context's element
+ * kernel, the RHS variant and the fused 2N-stage variant separately (3 rounds
+ * of `repetitions` launches on the context's stream, best round), on the
+ * resident state (upload it first), and makes the fastest tile of each variant
+ * the context's and, for contexts created later in this process with a default
+ * tile, their default. Tiles give bitwise-equal results in FP64; in FP32 the
+ * log-mean series/exact choice is made per wavefront, so states with large
+ * jumps can differ at round-off level between tiles. The resident state u is
+ * unchanged; the 2N register u_tmp is overwritten (call between steps).
+ * timings (up to max_timings entries, may be NULL with max_timings 0) receive
+ * the measured candidates, ntimings their number. Default tiles without
+ * autotuning come from the table of measured sweeps compiled into the
+ * library (tile_table.inc), else one wavefront per direction. */
+typedef struct trixi_hip_tile_timing {
+  int32_t stage;              /* 0: RHS kernel, 1: fused-stage kernel */
+  int32_t elements_per_block;
+  int32_t threads_per_block;
+  double microseconds;        /* time per launch, best of the 3 rounds */
+} trixi_hip_tile_timing;
+int trixi_hip_autotune(trixi_hip_ctx *ctx, int repetitions, trixi_hip_tile_timing *timings, int max_timings,
+                       int *ntimings);
 
 #ifdef __cplusplus
 }

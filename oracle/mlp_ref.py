@@ -7,6 +7,8 @@ of the feature extraction; the product path never imports it.
                      the state with the HG indicator's definitions
                      (indicators.jl:119-156, indicators_2d.jl:30-102,
                      density_pressure compressible_euler_2d.jl).
+* hg_clip, alpha     HG's clipping of a blending factor (indicators_2d.jl:86-99)
+                     and the network's clipped blending factor.
 * forward/backward   the 19 -> 64 -> 64 -> 1 perceptron (ReLU, ReLU, sigmoid)
                      and the gradient of L = sum (y - target)^2 / n, in FP64.
                      With bf16=True the operands that the MFMA kernels round to
@@ -63,6 +65,21 @@ def forward(p, x, bf=False):
     return dict(X=X, W1=W1, W2=W2, w=w, z1=z1, h1=h1, z2=z2, h2=h2, y=y)
 
 
+def hg_clip(alpha, alpha_min, alpha_max):
+    """The clipping of calc_indicator_hennemann_gassner! (indicators_2d.jl:86-99):
+    0 below alpha_min, 1 above 1 - alpha_min, then min(alpha_max, .)."""
+    a = np.asarray(alpha, dtype=np.float64)
+    a = np.where(a < alpha_min, 0.0, a)
+    a = np.where(a > 1 - alpha_min, 1.0, a)
+    return np.minimum(a, alpha_max)
+
+
+def alpha(p, x, alpha_max, alpha_min, bf=True):
+    """Blending factor of the learned indicator without fallback:
+    hg_clip(alpha_max * y)."""
+    return hg_clip(alpha_max * forward(p, x, bf)["y"], alpha_min, alpha_max)
+
+
 def loss_grad(p, x, target, bf=False):
     f = forward(p, x, bf)
     n = len(f["y"])
@@ -107,10 +124,7 @@ def features(u, ndims, polydeg, vinv, gamma, alpha_max, alpha_min):
     T = 0.5 * 10 ** (-1.8 * n ** 0.25)
     s = np.log((1 - 0.0001) / 0.0001)
     energy = np.maximum(ef1, ef2)
-    al = 1 / (1 + np.exp(-s / T * (energy - T)))
-    al = np.where(al < alpha_min, 0.0, al)
-    al = np.where(al > 1 - alpha_min, 1.0, al)
-    alpha = np.minimum(al, alpha_max)
+    alpha = hg_clip(1 / (1 + np.exp(-s / T * (energy - T))), alpha_min, alpha_max)
     feat = np.zeros((E, KF))
     mean = q.mean(axis=1, keepdims=True)
     dev = np.abs(q - mean).max(axis=1, keepdims=True)

@@ -59,6 +59,7 @@ def _load():
         L.oracle_rhs.argtypes = [P, P, P, D]
         L.oracle_rhs_ld.argtypes = [P, P, P, D]
         L.oracle_indicator_hg.argtypes = [P, P, P]
+        L.oracle_set_alpha.argtypes = [P, P]
         L.oracle_max_dt.restype = D
         L.oracle_max_dt.argtypes = [P, P]
         L.oracle_error_norms.argtypes = [P, P, D, P, P]
@@ -226,6 +227,16 @@ class Oracle:
         a = np.empty(self.nelements)
         self.L.oracle_indicator_hg(self._c, _p(u), _p(a))
         return a
+
+    def set_alpha(self, alpha):
+        """Blending factors [E] (after smoothing) for the shock-capturing RHS
+        instead of its HG indicator; None returns to the HG indicator."""
+        if alpha is None:
+            self.L.oracle_set_alpha(self._c, None)
+        else:
+            self._alpha_given = np.ascontiguousarray(alpha, dtype=np.float64)
+            assert self._alpha_given.shape == (self.nelements,)
+            self.L.oracle_set_alpha(self._c, _p(self._alpha_given))
 
     def max_dt(self, u, cfl=1.0):
         u = np.ascontiguousarray(u, dtype=np.float64)

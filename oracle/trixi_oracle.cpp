@@ -607,6 +607,12 @@ struct oracle_ctx {
     std::vector<Origin> transfer;         // [E] origin of each element in the last adaptation
     std::vector<double> sfv;            // surface_flux_values
     std::vector<double> alpha, alpha_tmp;
+    // blending factors given from outside (oracle_set_alpha): the shock-capturing
+    // volume integral uses them instead of running the HG indicator, as
+    // VolumeIntegralShockCapturingHG uses whatever its indicator returns
+    // (calc_volume_integral.jl:231-272)
+    bool alpha_given = false;
+    std::vector<double> alpha_given_v;
     int hg_variable = 0;  // HG indicator variable: 0 density_pressure, 1 density, 2 pressure
     // analysis exact-solution parameters
     int ic_kind = IC_CONSTANT;
@@ -1045,7 +1051,9 @@ template <int D, typename R> void rhs(oracle_ctx &c, R *du, const R *u, double t
 
     // calc_volume_integral!, calc_volume_integral.jl:180-272
     const int vi = c.cfg.volume_integral;
-    if (vi == VI_SHOCK_CAPTURING_HG) {
+    if (vi == VI_SHOCK_CAPTURING_HG && c.alpha_given) {
+        c.alpha = c.alpha_given_v;
+    } else if (vi == VI_SHOCK_CAPTURING_HG) {
         if constexpr (std::is_same<R, double>::value) hg_indicator<D>(c, Euler<D>(c.cfg.gamma), u);
         else {
             std::vector<double> ud(u, u + total);
@@ -1862,6 +1870,14 @@ void oracle_rhs_ld(oracle_ctx *c, double *du, const double *u, double t) {
     if (c->D == 2) rhs<2, long double>(*c, dul.data(), ul.data(), t);
     else rhs<3, long double>(*c, dul.data(), ul.data(), t);
     for (size_t k = 0; k < total; ++k) du[k] = (double)dul[k];
+}
+
+// Blending factors (after smoothing) that the shock-capturing RHS uses instead
+// of its HG indicator; alpha == NULL returns to the HG indicator.
+void oracle_set_alpha(oracle_ctx *c, const double *alpha) {
+    c->alpha_given = alpha != nullptr;
+    if (alpha) c->alpha_given_v.assign(alpha, alpha + c->E);
+    else c->alpha_given_v.clear();
 }
 
 // Indicator only (alpha[E]); runs the HG indicator regardless of volume integral.

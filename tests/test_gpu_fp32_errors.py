@@ -1,0 +1,100 @@
+"""FP32 bulk RHS against the FP64 oracle, per benchmark configuration, at full
+size: the relative L2 (root mean square of the difference over all values,
+divided by that of the oracle's du) and Linf (max |difference| / max |du|)
+errors are printed and held to stated bounds (about 3x the values measured on
+MI355X; the table of measured values is part of the precision design).
+
+C1 2D vortex p3 16^2, C2 3D MMS p3 32^3, C3 3D Fourier modes p4 32^3 (seed 0),
+C4 2D KHI p3 128^2 with HG shock capturing (initial state and the state of an
+FP64 run at t = 1), C5 the C4 operator with the learned indicator's blending
+factors (the oracle takes the FP32 path's own factors, so the comparison is
+of the FP32 element kernel, not of the indicator)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from gpu_helpers import build, oracle_for
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+C1 = dict(ndims=2, polydeg=3, level=4, cmin=(-10, -10), cmax=(10, 10), ic="isentropic_vortex", ic_params=(1.0,))
+C2 = dict(ndims=3, polydeg=3, level=5, cmin=(0, 0, 0), cmax=(2, 2, 2), ic="convergence_test",
+          source="convergence_test", surface_flux="llf")
+C3 = dict(ndims=3, polydeg=4, level=5, cmin=(-1,) * 3, cmax=(1,) * 3, ic="fourier_modes", ic_params=(0,),
+          surface_flux="llf")
+C4 = dict(ndims=2, polydeg=3, level=7, cmin=(-1, -1), cmax=(1, 1), ic="kelvin_helmholtz", ic_params=(1e-3, 0),
+          volume_integral="hg", surface_flux="llf_naive", fv_flux="llf_naive", alpha_max=0.002, alpha_min=1e-4)
+
+# (relative L2, relative Linf) bounds of the FP32 RHS against the FP64 oracle
+BOUNDS = {
+    "C1": (1e-4, 1e-4),
+    "C2": (1e-4, 1e-4),
+    "C3": (1e-4, 1e-4),
+    "C4_t0": (1e-4, 1e-4),
+    "C4_t1": (1e-4, 1e-4),
+    "C5_t1": (1e-4, 1e-4),
+}
+MEASURED = {}
+
+
+def errors(du, ref):
+    d = du - ref
+    return (float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref * ref))),
+            float(np.max(np.abs(d)) / np.max(np.abs(ref))))
+
+
+def rhs32(semi, u, t=0.0):
+    import trixi_hip as th
+    du = np.empty_like(u)
+    th.rhs_hyperbolic_(du, u, semi, t)
+    return du
+
+
+def check(name, du, ref):
+    l2, linf = errors(du, ref)
+    MEASURED[name] = (l2, linf)
+    print(f"FP32 vs FP64 oracle {name}: relative L2 {l2:.3e}, relative Linf {linf:.3e} "
+          f"(bounds {BOUNDS[name][0]:.1e}, {BOUNDS[name][1]:.1e})", flush=True)
+    assert l2 < BOUNDS[name][0] and linf < BOUNDS[name][1], (name, l2, linf)
+
+
+@pytest.mark.parametrize("name,spec", [("C1", C1), ("C2", C2), ("C3", C3), ("C4_t0", C4)])
+def test_fp32_rhs_error_at_the_initial_state(name, spec):
+    o = oracle_for(spec)
+    semi, _ = build(spec, "fp32")
+    u = semi.compute_coefficients(0.0)
+    check(name, rhs32(semi, u), o.rhs(u, 0.0))
+    semi.close()
+
+
+def _khi_state_at_t1():
+    """The FP64 C4 run (HG, CFL 1.3) at t = 1."""
+    import trixi_hip as th
+    semi, _ = build(C4, "fp64")
+    sol = th.solve(th.semidiscretize(semi, (0.0, 1.0)), th.CarpenterKennedy2N54(),
+                   callback=th.CallbackSet(th.StepsizeCallback(cfl=1.3)))
+    semi.close()
+    return sol.u
+
+
+def test_fp32_rhs_error_developed_khi_hg_and_learned():
+    from trixi_hip import _lib
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork, indicator_alpha
+    u = _khi_state_at_t1()
+    o = oracle_for(C4)
+    semi, _ = build(C4, "fp32")
+    check("C4_t1", rhs32(semi, u), o.rhs(u, 0.0))
+    # C5: the learned indicator (committed weights, with fallback) in the FP32
+    # path; the oracle blends with the same factors
+    net = IndicatorNeuralNetwork.load()
+    net.attach(semi)
+    du = rhs32(semi, u)
+    assert semi._lib.trixi_hip_upload_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p)) == 0
+    alpha = indicator_alpha(semi, net.mode)
+    o.set_alpha(alpha)
+    check("C5_t1", du, o.rhs(u, 0.0))
+    o.set_alpha(None)
+    assert _lib.IND_HG == 0
+    semi.close()

@@ -112,11 +112,26 @@ def test_modes_and_fallback():
     run_segments(semi, 0.3, 2)
     f, a_hg_raw = features(semi)
     p = _params(7)
-    # no fallback: alpha = alpha_max * y for every element
+    # shift the output bias so that about half of the outputs fall below the
+    # clipping threshold alpha_min / alpha_max = 0.05
+    from oracle import mlp_ref
+    y0 = mlp_ref.forward(p, f, bf=True)["y"]
+    p[mlp_ref.OFF["b3"][0]] -= np.float32(np.median(np.log(y0 / (1 - y0))) - np.log(0.05 / 0.95))
+    # no fallback: alpha = alpha_max * y clipped as HG clips (0 below alpha_min,
+    # at most alpha_max), then smoothed over the element's neighbours
     net = IndicatorNeuralNetwork(p, fallback=False)
     net.attach(semi)
     a_nn = indicator_alpha(semi, _lib.IND_MLP)
     assert np.all(a_nn >= 0) and np.all(a_nn <= 0.002 + 1e-12)
+    import torch
+    from trixi_hip.learned_indicator import forward
+    y = torch.empty(f.shape[0], dtype=torch.float32, device="cuda")
+    forward(semi, torch.from_numpy(f).cuda(), y)
+    raw = mlp_ref.hg_clip(0.002 * y.cpu().numpy().astype(np.float64), 0.0001, 0.002)
+    assert 0 < np.count_nonzero(raw == 0) < raw.size, "the clip at alpha_min is not exercised"
+    nb = semi.mesh.element_neighbors
+    smooth = np.maximum(raw, 0.5 * np.where(nb >= 0, raw[np.maximum(nb, 0)], 0.0).max(axis=1))
+    np.testing.assert_array_equal(a_nn, smooth)
     # fallback with an empty box: every element takes the HG value
     IndicatorNeuralNetwork(p, np.full(19, 1.0, np.float32), np.full(19, -1.0, np.float32)).upload(semi)
     a_fb = indicator_alpha(semi, _lib.IND_MLP_WITH_FALLBACK)

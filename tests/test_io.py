@@ -173,6 +173,18 @@ def test_load_mesh_follows_the_restart_files_mesh_attribute(tmp_path):
     assert np.array_equal(m2.cell_centers, mesh.cell_centers) and m2.nboundaries == mesh.nboundaries
 
 
+def test_restart_callback_initialize_writes_the_mesh_file(tmp_path):
+    """SaveRestartCallback's initialize! (save_restart.jl:59-72) writes mesh.h5, so
+    the restart files of a run whose only writer is this callback name mesh.h5."""
+    mesh = th.TreeMesh((-1, -1), (1, 1), initial_refinement_level=2, periodicity=True)
+    semi = semi_for(mesh)
+    cb = th.SaveRestartCallback(interval=5, output_directory=str(tmp_path / "out"))
+    cb.initialize(semi, 0.0, 0.01, None)
+    assert os.path.exists(tmp_path / "out" / "mesh.h5") and not mesh.unsaved_changes
+    f = th.save_restart_file(np.zeros(semi.nvalues()), 0.1, 0.01, 5, semi, cb.output_directory)
+    assert dump(f, "/mesh_file", True) == ["mesh.h5"]
+
+
 def test_io_library_exports_every_declared_symbol():
     import ctypes
     from conftest import ROOT

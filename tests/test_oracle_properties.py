@@ -69,6 +69,28 @@ def test_hg_indicator_range():
     assert np.min(a) == pytest.approx(1e-4, rel=1e-9) and np.max(a) == 0.002
 
 
+def test_given_blending_factors():
+    """With the blending factors given (oracle_set_alpha), the shock-capturing
+    RHS is the one its own HG indicator gives for those factors, pure flux
+    differencing at alpha = 0 and the pure LGL finite volume at alpha = 1."""
+    kw = dict(ndims=2, polydeg=3, level=4, cmin=(-1, -1), cmax=(1, 1), volume_flux="ranocha",
+              surface_flux="llf_naive", fv_flux="llf_naive")
+    o = make_oracle(dict(kw, volume_integral="hg", alpha_max=0.002, alpha_min=1e-4))
+    o.set_initial_condition("kelvin_helmholtz", (1e-3, 0))
+    u = o.compute_coefficients(0.0)
+    du_hg = o.rhs(u)
+    o.set_alpha(o.indicator_hg(u))
+    assert np.array_equal(o.rhs(u), du_hg)
+    o.set_alpha(np.zeros(o.nelements))
+    fd = make_oracle(dict(kw, volume_integral="fd"))
+    np.testing.assert_allclose(o.rhs(u), fd.rhs(u), rtol=0, atol=1e-13 * np.max(np.abs(du_hg)))
+    o.set_alpha(np.ones(o.nelements))
+    fv = make_oracle(dict(kw, volume_integral="fv"))
+    np.testing.assert_allclose(o.rhs(u), fv.rhs(u), rtol=0, atol=1e-13 * np.max(np.abs(du_hg)))
+    o.set_alpha(None)
+    assert np.array_equal(o.rhs(u), du_hg)
+
+
 def test_ck2n54_fourth_order():
     """The 2N recurrence u_tmp = du - a u_tmp; u += b dt u_tmp (methods_2N.jl:153-157)
     with the oracle's coefficients is 4th order on y' = lambda y."""

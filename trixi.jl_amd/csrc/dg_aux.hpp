@@ -299,7 +299,7 @@ __device__ __forceinline__ void block_nanmax(const double *__restrict__ partial,
 //   NaN dt -> error, done; steps == max_steps -> done; done -> dt = 0.
 // tdt = {t, dt}; ctl = {t_end, cfl, last, done, steps, error, max_steps}. A
 // step replayed after done leaves u and t unchanged (dt = 0).
-__global__ __launch_bounds__(kRedThreads) void step_control_kernel(const double *__restrict__ partial, int np,
+static __global__ __launch_bounds__(kRedThreads) void step_control_kernel(const double *__restrict__ partial, int np,
                                                                    int nnodes, double *__restrict__ tdt,
                                                                    double *__restrict__ ctl) {
   __shared__ double s_red[kRedThreads];
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kRedThreads) void step_control_kernel(const double 
   ctl[4] = steps;
 }
 
-__global__ __launch_bounds__(kRedThreads) void max_dt_finalize_kernel(const double *__restrict__ partial,
+static __global__ __launch_bounds__(kRedThreads) void max_dt_finalize_kernel(const double *__restrict__ partial,
                                                                       int np, int nnodes, double cfl,
                                                                       double *__restrict__ out) {
   __shared__ double s_red[kRedThreads];
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void integrals_kernel(const T *__restrict__ u,
 }
 
 // Column sums / maxima of a partial table [rows, cols] -> out[cols]; one block.
-__global__ __launch_bounds__(256) void reduce_columns_kernel(const double *__restrict__ partial,
+static __global__ __launch_bounds__(256) void reduce_columns_kernel(const double *__restrict__ partial,
                                                              int rows, int cols, int nsum,
                                                              double *__restrict__ out) {
   __shared__ double s_red[256];
@@ -739,7 +739,7 @@ __global__ __launch_bounds__(256) void mortar_flux_kernel(const T *__restrict__ 
 
 // HG smoothing over an adjacency list (interfaces and mortars; apply_smoothing!,
 // indicators_3d.jl:125-173): alpha_e = max(alpha_raw_e, 0.5 max_nbr alpha_raw_nbr).
-__global__ void hg_smooth_adj_kernel(const double *__restrict__ alpha_raw, const int32_t *__restrict__ adj_ptr,
+static __global__ void hg_smooth_adj_kernel(const double *__restrict__ alpha_raw, const int32_t *__restrict__ adj_ptr,
                                      const int32_t *__restrict__ adj, double *__restrict__ alpha, int E) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;

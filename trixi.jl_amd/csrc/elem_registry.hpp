@@ -71,3 +71,15 @@ template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND, bool ST
   TRIXI_REG_ALLVOL(T, 3, 2, 7, 1, BND, STAGE)      \
   TRIXI_REG_ALLVOL(T, 3, 3, 4, 1, BND, STAGE)      \
   TRIXI_REG_ALLVOL(T, 3, 4, 2, 1, BND, STAGE)
+
+// alternative tiles (two wavefronts per direction) of the headline volume
+// integrals, RHS and fused-stage variants, periodic meshes
+#define TRIXI_REG_ALT_TILES(T)                                        \
+  TRIXI_REG(T, 3, 3, trixi_hip::VOL_FD_RANOCHA, 8, 2, false, false)  \
+  TRIXI_REG(T, 3, 3, trixi_hip::VOL_FD_RANOCHA, 8, 2, false, true)   \
+  TRIXI_REG(T, 3, 4, trixi_hip::VOL_FD_RANOCHA, 5, 2, false, false)  \
+  TRIXI_REG(T, 3, 4, trixi_hip::VOL_FD_RANOCHA, 5, 2, false, true)   \
+  TRIXI_REG(T, 2, 3, trixi_hip::VOL_FD_RANOCHA, 32, 2, false, false) \
+  TRIXI_REG(T, 2, 3, trixi_hip::VOL_FD_RANOCHA, 32, 2, false, true)  \
+  TRIXI_REG(T, 2, 3, trixi_hip::VOL_HG_RANOCHA, 32, 2, false, false) \
+  TRIXI_REG(T, 2, 3, trixi_hip::VOL_HG_RANOCHA, 32, 2, false, true)

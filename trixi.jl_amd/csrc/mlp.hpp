@@ -164,14 +164,25 @@ __device__ __forceinline__ void forward_tile(const float *__restrict__ x, int n,
   t.y = 1.f / (1.f + __expf(-o));
 }
 
-// Inference over E elements: alpha_raw[e] = alpha_max * y(features[e]), or the
-// HG value (alpha_hg) when a feature lies outside [lo, hi] or y is not finite
-// (fallback != 0); with fallback == 0 the network output is always used.
+// The clipping of calc_indicator_hennemann_gassner! (indicators_2d.jl:86-99)
+// applied to a blending factor: below alpha_min it is 0 (pure DG, the element
+// kernel then skips the FV subcells), above 1 - alpha_min 1 (pure FV), and it
+// never exceeds alpha_max.
+__device__ __forceinline__ double hg_clip(double al, double alpha_min, double alpha_max) {
+  if (al < alpha_min) al = 0.0;
+  if (al > 1 - alpha_min) al = 1.0;
+  return al < alpha_max ? al : alpha_max;
+}
+
+// Inference over E elements: alpha_raw[e] = hg_clip(alpha_max * y(features[e])),
+// or the HG value (alpha_hg, clipped by the HG kernel) when a feature lies
+// outside [lo, hi] or y is not finite (fallback != 0); with fallback == 0 the
+// network output is always used.
 __global__ __launch_bounds__(NTHR) void mlp_infer_kernel(const float *__restrict__ w, const float *__restrict__ x,
                                                          const float *__restrict__ lo, const float *__restrict__ hi,
                                                          const double *__restrict__ alpha_hg, double alpha_max,
-                                                         int fallback, int E, double *__restrict__ alpha_raw,
-                                                         float *__restrict__ y_out) {
+                                                         double alpha_min, int fallback, int E,
+                                                         double *__restrict__ alpha_raw, float *__restrict__ y_out) {
   __shared__ __attribute__((aligned(16))) WeightsLDS s;
   load_weights(w, s);
   __syncthreads();
@@ -192,7 +203,7 @@ __global__ __launch_bounds__(NTHR) void mlp_infer_kernel(const float *__restrict
             use_nn = use_nn && v >= lo[f] && v <= hi[f];
           }
         }
-        alpha_raw[e] = use_nn ? alpha_max * (double)t.y : alpha_hg[e];
+        alpha_raw[e] = use_nn ? hg_clip(alpha_max * (double)t.y, alpha_min, alpha_max) : alpha_hg[e];
       }
     }
   }

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -33,11 +34,35 @@ namespace {
 // ---------------------------------------------------------------------------
 // Element kernels: instantiated in elem_*.hip and looked up here by key.
 // ---------------------------------------------------------------------------
+// Tile when the configuration asks for the default (elements_per_block = 0) and
+// neither the process's own autotuning (trixi_hip_autotune) nor the measured
+// table below has an entry for the kernel variant: one wavefront per direction.
 struct DefaultTile {
   int D, N, epb, nt;
 };
 const DefaultTile kDefaultTiles[] = {{2, 2, 21, 128}, {2, 3, 16, 128}, {2, 4, 12, 128},
                                      {3, 2, 7, 192},  {3, 3, 4, 192},  {3, 4, 2, 192}};
+
+// Tiles chosen by on-device sweeps of every instantiated tile per kernel
+// variant (python -m trixi_hip.autotune on MI355X writes tile_table.inc).
+struct TunedTile {
+  int fp32, D, N, vol, bnd, stage, epb, nt;
+};
+const TunedTile kTunedTiles[] = {
+#include "tile_table.inc"
+    {-1, 0, 0, 0, 0, 0, 0, 0},  // end marker (matches no variant)
+};
+
+// (precision, ndims, polydeg, volume integral, boundaries, stage) -> tile
+// measured by trixi_hip_autotune in this process; consulted before kTunedTiles.
+struct TuneKey {
+  int fp32, D, N, vol, bnd, stage;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(fp32, D, N, vol, bnd, stage) < std::tie(o.fp32, o.D, o.N, o.vol, o.bnd, o.stage);
+  }
+};
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+std::mutex g_tuned_mutex;
 
 int vol_kind(const trixi_hip_config &c) {
   switch (c.volume_integral) {
@@ -121,9 +146,10 @@ struct trixi_hip_ctx {
   ExactParams exact{};
 
   // element-kernel launch (elem_stage: the variant with the fused 2N update,
-  // null when not instantiated for the tile or switched off)
+  // null when not instantiated for the tile or switched off); each with its
+  // own tile (elements per workgroup, threads)
   ElemLaunch elem = nullptr, elem_stage = nullptr;
-  int epb = 0, nt = 0;
+  int epb = 0, nt = 0, epb_st = 0, nt_st = 0;
   std::vector<unsigned char> kconst;  // DGConst<T, N> blob
   int xcd_remap = 0;  // measured: the round-robin XCD placement is faster here
 
@@ -149,9 +175,28 @@ struct trixi_hip_ctx {
   double *d_records = nullptr;
   int rec_cap = 0, rec_interval = 0;
   double tv_err = -1, tv_int = -1;  // total volume as the error norms / integrals compute it
+
+  // device allocations and host synchronisations made for this context
+  // (trixi_hip_get_counters)
+  long long n_alloc = 0, n_sync = 0;
 };
 
 namespace {
+
+// Every device allocation and every host synchronisation of the library goes
+// through these, so that trixi_hip_get_counters can report them.
+template <typename P> hipError_t dev_alloc(trixi_hip_ctx *c, P **p, size_t bytes) {
+  ++c->n_alloc;
+  return hipMalloc(reinterpret_cast<void **>(p), bytes);
+}
+hipError_t host_sync(trixi_hip_ctx *c) {
+  ++c->n_sync;
+  return hipStreamSynchronize(c->stream);
+}
+hipError_t sync_copy(trixi_hip_ctx *c, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  ++c->n_sync;
+  return hipMemcpy(dst, src, bytes, kind);
+}
 
 int set_err(trixi_hip_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
@@ -200,25 +245,62 @@ template <typename T> void build_kconst(trixi_hip_ctx *ctx) {
   });
 }
 
-int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
-  if (epb <= 0 || nt <= 0) {
-    for (const auto &t : kDefaultTiles)
-      if (t.D == ctx->D && t.N == ctx->N) {
-        epb = t.epb;
-        nt = t.nt;
-      }
+TuneKey tune_key(const trixi_hip_ctx *ctx, int stage) {
+  return TuneKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, ctx->B + ctx->M > 0 ? 1 : 0, stage};
+}
+
+ElemLaunch find_launch(const trixi_hip_ctx *ctx, int stage, int epb, int nt) {
+  auto it = elem_registry().find(
+      ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B + ctx->M > 0 ? 1 : 0, stage});
+  return it == elem_registry().end() ? nullptr : it->second;
+}
+
+// Default tile of a kernel variant: this process's autotuning, else the
+// measured table, else one wavefront per direction.
+void default_tile(const trixi_hip_ctx *ctx, int stage, int &epb, int &nt) {
+  const TuneKey k = tune_key(ctx, stage);
+  {
+    std::lock_guard<std::mutex> lock(g_tuned_mutex);
+    auto it = g_tuned.find(k);
+    if (it != g_tuned.end() && find_launch(ctx, stage, it->second.first, it->second.second)) {
+      epb = it->second.first;
+      nt = it->second.second;
+      return;
+    }
   }
-  auto it = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B + ctx->M > 0 ? 1 : 0, 0});
-  if (it == elem_registry().end())
+  for (const auto &t : kTunedTiles)
+    if (t.fp32 == k.fp32 && t.D == k.D && t.N == k.N && t.vol == k.vol && t.bnd == k.bnd && t.stage == k.stage &&
+        find_launch(ctx, stage, t.epb, t.nt)) {
+      epb = t.epb;
+      nt = t.nt;
+      return;
+    }
+  for (const auto &t : kDefaultTiles)
+    if (t.D == ctx->D && t.N == ctx->N) {
+      epb = t.epb;
+      nt = t.nt;
+    }
+}
+
+// The RHS and the fused-stage kernels of the context: the given tile for both,
+// or (epb, nt = 0) each variant's default tile.
+int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
+  int e_r = epb, n_r = nt, e_s = epb, n_s = nt;
+  if (epb <= 0 || nt <= 0) {
+    default_tile(ctx, 0, e_r, n_r);
+    default_tile(ctx, 1, e_s, n_s);
+  }
+  ctx->elem = find_launch(ctx, 0, e_r, n_r);
+  if (!ctx->elem)
     return set_err(ctx, TRIXI_HIP_EINVAL,
                    "no element kernel instantiated for this (precision, ndims, polydeg, volume "
                    "integral, tile, boundaries)");
-  ctx->elem = it->second;
-  auto st = elem_registry().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B + ctx->M > 0 ? 1 : 0, 1});
   const char *unfused = std::getenv("TRIXI_HIP_UNFUSED_STAGE");
-  ctx->elem_stage = (st == elem_registry().end() || (unfused && std::atoi(unfused) != 0)) ? nullptr : st->second;
-  ctx->epb = epb;
-  ctx->nt = nt;
+  ctx->elem_stage = (unfused && std::atoi(unfused) != 0) ? nullptr : find_launch(ctx, 1, e_s, n_s);
+  ctx->epb = e_r;
+  ctx->nt = n_r;
+  ctx->epb_st = ctx->elem_stage ? e_s : 0;
+  ctx->nt_st = ctx->elem_stage ? n_s : 0;
   return TRIXI_HIP_OK;
 }
 
@@ -256,7 +338,7 @@ template <typename T> int launch_indicator_raw(trixi_hip_ctx *ctx, const T *u) {
   if (r) return r;
   const float *lo = ctx->d_mlp + 3 * mlp::NP, *hi = lo + mlp::NF;
   hipLaunchKernelGGL(mlp::mlp_infer_kernel, dim3(mlp_blocks(ctx->E)), dim3(mlp::NTHR), 0, ctx->stream, ctx->d_mlp,
-                     ctx->d_feat, lo, hi, ctx->d_alpha_hg, ctx->cfg.alpha_max,
+                     ctx->d_feat, lo, hi, ctx->d_alpha_hg, ctx->cfg.alpha_max, ctx->cfg.alpha_min,
                      ctx->ind_mode == TRIXI_HIP_IND_MLP_WITH_FALLBACK ? 1 : 0, ctx->E, ctx->d_alpha_raw,
                      (float *)nullptr);
   HIP_TRY(hipGetLastError());
@@ -341,7 +423,8 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
     a.b_stage = so->b_stage;
     a.first_stage = so->first_stage;
   }
-  int nblocks = (ctx->E + ctx->epb - 1) / ctx->epb;
+  const int epb = so ? ctx->epb_st : ctx->epb;
+  int nblocks = (ctx->E + epb - 1) / epb;
   hipError_t e = (so ? ctx->elem_stage : ctx->elem)(&a, ctx->kconst.data(), nblocks, ctx->stream);
   if (e != hipSuccess) return set_err(ctx, TRIXI_HIP_EHIP, std::string("element kernel launch: ") + hipGetErrorString(e));
   return TRIXI_HIP_OK;
@@ -631,43 +714,43 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   const size_t nv = ctx->nvals;
   const size_t tsz = ctx->fp32 ? 4 : 8;
   ctx->partial_cap = std::max<size_t>((size_t)E * 2 * ctx->V, kMaxPartials);
-  bool ok = hipMalloc(&ctx->d_nbr, nbr.size() * 4) == hipSuccess &&
-            hipMalloc(&ctx->d_invj, E * 8) == hipSuccess &&
-            hipMalloc(&ctx->d_centers, (size_t)D * E * 8) == hipSuccess &&
-            hipMalloc(&ctx->d_ub[0], nv * 8) == hipSuccess && hipMalloc(&ctx->d_utmp, nv * tsz) == hipSuccess &&
-            (ctx->fp32 || hipMalloc(&ctx->d_ub[1], nv * 8) == hipSuccess) &&
-            hipMalloc(&ctx->d_du, nv * tsz) == hipSuccess &&
-            hipMalloc(&ctx->d_alpha_raw, E * 8) == hipSuccess && hipMalloc(&ctx->d_alpha, E * 8) == hipSuccess &&
-            hipMalloc(&ctx->d_partial, ctx->partial_cap * 8) == hipSuccess &&
-            hipMalloc(&ctx->d_scalars, 64 * 8) == hipSuccess;
+  bool ok = dev_alloc(ctx, &ctx->d_nbr, nbr.size() * 4) == hipSuccess &&
+            dev_alloc(ctx, &ctx->d_invj, E * 8) == hipSuccess &&
+            dev_alloc(ctx, &ctx->d_centers, (size_t)D * E * 8) == hipSuccess &&
+            dev_alloc(ctx, &ctx->d_ub[0], nv * 8) == hipSuccess && dev_alloc(ctx, &ctx->d_utmp, nv * tsz) == hipSuccess &&
+            (ctx->fp32 || dev_alloc(ctx, &ctx->d_ub[1], nv * 8) == hipSuccess) &&
+            dev_alloc(ctx, &ctx->d_du, nv * tsz) == hipSuccess &&
+            dev_alloc(ctx, &ctx->d_alpha_raw, E * 8) == hipSuccess && dev_alloc(ctx, &ctx->d_alpha, E * 8) == hipSuccess &&
+            dev_alloc(ctx, &ctx->d_partial, ctx->partial_cap * 8) == hipSuccess &&
+            dev_alloc(ctx, &ctx->d_scalars, 64 * 8) == hipSuccess;
   // face slots: boundaries, then mortar faces; ghost states of the mortar slots
   // are a valid placeholder state (the element kernel replaces their fluxes)
   const size_t nslots = (size_t)ctx->B + (size_t)ctx->M * (S + 1);
   const size_t slot_vals = nslots * Lf * ctx->V;
-  if (ok && ctx->B > 0) ok = hipMalloc(&ctx->d_bx, bx.size() * 8) == hipSuccess;
-  if (ok && nslots > 0) ok = hipMalloc(&ctx->d_ghost, slot_vals * tsz) == hipSuccess;
+  if (ok && ctx->B > 0) ok = dev_alloc(ctx, &ctx->d_bx, bx.size() * 8) == hipSuccess;
+  if (ok && nslots > 0) ok = dev_alloc(ctx, &ctx->d_ghost, slot_vals * tsz) == hipSuccess;
   if (ok && ctx->M > 0)
-    ok = hipMalloc(&ctx->d_mflux, slot_vals * tsz) == hipSuccess &&
-         hipMalloc(&ctx->d_mort, mort.size() * 4) == hipSuccess &&
-         hipMalloc(&ctx->d_adj_ptr, adj_ptr.size() * 4) == hipSuccess &&
-         hipMalloc(&ctx->d_adj, std::max<size_t>(adj.size(), 1) * 4) == hipSuccess;
+    ok = dev_alloc(ctx, &ctx->d_mflux, slot_vals * tsz) == hipSuccess &&
+         dev_alloc(ctx, &ctx->d_mort, mort.size() * 4) == hipSuccess &&
+         dev_alloc(ctx, &ctx->d_adj_ptr, adj_ptr.size() * 4) == hipSuccess &&
+         dev_alloc(ctx, &ctx->d_adj, std::max<size_t>(adj.size(), 1) * 4) == hipSuccess;
   if (ok && ctx->fp32)
-    ok = hipMalloc(&ctx->d_u32b[0], nv * 4) == hipSuccess && hipMalloc(&ctx->d_u32b[1], nv * 4) == hipSuccess &&
-         hipMalloc(&ctx->d_su32, nv * 4) == hipSuccess &&
-         hipMalloc(&ctx->d_sdu32, nv * 4) == hipSuccess;
+    ok = dev_alloc(ctx, &ctx->d_u32b[0], nv * 4) == hipSuccess && dev_alloc(ctx, &ctx->d_u32b[1], nv * 4) == hipSuccess &&
+         dev_alloc(ctx, &ctx->d_su32, nv * 4) == hipSuccess &&
+         dev_alloc(ctx, &ctx->d_sdu32, nv * 4) == hipSuccess;
   if (!ok) {
     trixi_hip_destroy(ctx);
     fprintf(stderr, "trixi_hip_create: device allocation failed\n");
     return TRIXI_HIP_ENOMEM;
   }
   std::vector<double> zeros(E, 0.0);
-  ok = hipMemcpy(ctx->d_nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(ctx->d_invj, mesh->inverse_jacobian, E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(ctx->d_centers, mesh->element_centers, (size_t)D * E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(ctx->d_alpha_raw, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(ctx->d_scalars + 32, ctx->weights.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       (ctx->B == 0 || hipMemcpy(ctx->d_bx, bx.data(), bx.size() * 8, hipMemcpyHostToDevice) == hipSuccess);
+  ok = sync_copy(ctx, ctx->d_nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+       sync_copy(ctx, ctx->d_invj, mesh->inverse_jacobian, E * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       sync_copy(ctx, ctx->d_centers, mesh->element_centers, (size_t)D * E * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       sync_copy(ctx, ctx->d_alpha_raw, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       sync_copy(ctx, ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       sync_copy(ctx, ctx->d_scalars + 32, ctx->weights.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess &&
+       (ctx->B == 0 || sync_copy(ctx, ctx->d_bx, bx.data(), bx.size() * 8, hipMemcpyHostToDevice) == hipSuccess);
   if (ok && nslots > 0) {
     std::vector<double> g(slot_vals);
     for (size_t k = 0; k < slot_vals; ++k) {
@@ -676,15 +759,15 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     }
     if (ctx->fp32) {
       std::vector<float> gf(g.begin(), g.end());
-      ok = hipMemcpy(ctx->d_ghost, gf.data(), slot_vals * 4, hipMemcpyHostToDevice) == hipSuccess;
+      ok = sync_copy(ctx, ctx->d_ghost, gf.data(), slot_vals * 4, hipMemcpyHostToDevice) == hipSuccess;
     } else {
-      ok = hipMemcpy(ctx->d_ghost, g.data(), slot_vals * 8, hipMemcpyHostToDevice) == hipSuccess;
+      ok = sync_copy(ctx, ctx->d_ghost, g.data(), slot_vals * 8, hipMemcpyHostToDevice) == hipSuccess;
     }
   }
   if (ok && ctx->M > 0)
-    ok = hipMemcpy(ctx->d_mort, mort.data(), mort.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(ctx->d_adj_ptr, adj_ptr.data(), adj_ptr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         (adj.empty() || hipMemcpy(ctx->d_adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
+    ok = sync_copy(ctx, ctx->d_mort, mort.data(), mort.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         sync_copy(ctx, ctx->d_adj_ptr, adj_ptr.data(), adj_ptr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         (adj.empty() || sync_copy(ctx, ctx->d_adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
   if (!ok) {
     trixi_hip_destroy(ctx);
     fprintf(stderr, "trixi_hip_create: upload failed\n");
@@ -696,7 +779,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
 
 int trixi_hip_destroy(trixi_hip_ctx *ctx) {
   if (!ctx) return TRIXI_HIP_OK;
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream) (void)host_sync(ctx);
   drop_graphs(ctx);
   void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_ub[0], ctx->d_ub[1], ctx->d_utmp,
                   ctx->d_u32b[0], ctx->d_u32b[1], ctx->d_du,
@@ -721,7 +804,7 @@ int trixi_hip_set_stream(trixi_hip_ctx *ctx, void *hip_stream) {
 
 int trixi_hip_synchronize(trixi_hip_ctx *ctx) {
   if (!ctx) return TRIXI_HIP_EINVAL;
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
@@ -732,14 +815,14 @@ int trixi_hip_upload_u(trixi_hip_ctx *ctx, const double *u_host) {
     int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
     if (r) return r;
   }
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
 int trixi_hip_download_u(trixi_hip_ctx *ctx, double *u_host) {
   if (!ctx || !u_host) return TRIXI_HIP_EINVAL;
   HIP_TRY(hipMemcpyAsync(u_host, cur_u64(ctx), ctx->nvals * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
@@ -766,8 +849,8 @@ int trixi_hip_rhs(trixi_hip_ctx *ctx, double *du, const double *u, double t, int
   double *du_out = du;
   if (where == TRIXI_HIP_HOST) {
     if (!ctx->d_hu) {
-      HIP_TRY(hipMalloc(&ctx->d_hu, nv * 8));
-      HIP_TRY(hipMalloc(&ctx->d_hdu, nv * 8));
+      HIP_TRY(dev_alloc(ctx, &ctx->d_hu, nv * 8));
+      HIP_TRY(dev_alloc(ctx, &ctx->d_hdu, nv * 8));
     }
     HIP_TRY(hipMemcpyAsync(ctx->d_hu, u, nv * 8, hipMemcpyHostToDevice, ctx->stream));
     du_in = ctx->d_hu;
@@ -787,7 +870,7 @@ int trixi_hip_rhs(trixi_hip_ctx *ctx, double *du, const double *u, double t, int
   }
   if (where == TRIXI_HIP_HOST) {
     HIP_TRY(hipMemcpyAsync(du, du_out, nv * 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(host_sync(ctx));
   }
   return TRIXI_HIP_OK;
 }
@@ -876,7 +959,7 @@ int rec_width(const trixi_hip_ctx *ctx) { return 3 + 3 * ctx->V; }
 int total_volumes(trixi_hip_ctx *ctx) {
   if (ctx->tv_err > 0) return TRIXI_HIP_OK;
   std::vector<double> ij(ctx->E);
-  HIP_TRY(hipMemcpy(ij.data(), ctx->d_invj, ctx->E * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(sync_copy(ctx, ij.data(), ctx->d_invj, ctx->E * 8, hipMemcpyDeviceToHost));
   double a = 0, b = 0;
   for (int e = 0; e < ctx->E; ++e) {
     a += std::pow(1.0 / ij[e], ctx->D) * std::pow(2.0, ctx->D);
@@ -978,7 +1061,7 @@ int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt) {
                      cfl, ctx->d_scalars + 2);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dt, ctx->d_scalars + 2, 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
@@ -1085,14 +1168,14 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
     if (recorded && (r = enqueue_record(ctx))) return r;
     HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(cur, tdt, sizeof(cur), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(host_sync(ctx));
     if (t_out) *t_out = cur[0];
     if (steps_out) *steps_out = (int)h[4];
     if (h[5] != 0.0) return set_err(ctx, TRIXI_HIP_ESTATE, "time step size `dt` is NaN");
     if (h[3] != 0.0) {
       if (every > 0 && !recorded) {  // the final state, if the last batch did not record it
         if ((r = enqueue_record(ctx))) return r;
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(host_sync(ctx));
       }
       break;
     }
@@ -1125,7 +1208,7 @@ int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host) {
   HIP_TRY(hipGetLastError());
   if (alpha_host) {
     HIP_TRY(hipMemcpyAsync(alpha_host, ctx->d_alpha, ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(host_sync(ctx));
   }
   return TRIXI_HIP_OK;
 }
@@ -1138,10 +1221,10 @@ static int mlp_check(trixi_hip_ctx *ctx) {
     return set_err(ctx, TRIXI_HIP_EINVAL, "the learned indicator is defined for 2D elements of polydeg 3");
   if (ctx->d_mlp) return TRIXI_HIP_OK;
   const size_t nmlp = 3 * (size_t)mlp::NP + 2 * mlp::NF + mlp::NPL;
-  HIP_TRY(hipMalloc(&ctx->d_mlp, nmlp * 4));
-  HIP_TRY(hipMalloc(&ctx->d_feat, (size_t)ctx->E * kMlpFeatureStride * 4));
-  HIP_TRY(hipMalloc(&ctx->d_alpha_hg, (size_t)ctx->E * 8));
-  HIP_TRY(hipMalloc(&ctx->d_mlp_partial, (size_t)kMlpMaxWaves * mlp::NPL * 4));
+  HIP_TRY(dev_alloc(ctx, &ctx->d_mlp, nmlp * 4));
+  HIP_TRY(dev_alloc(ctx, &ctx->d_feat, (size_t)ctx->E * kMlpFeatureStride * 4));
+  HIP_TRY(dev_alloc(ctx, &ctx->d_alpha_hg, (size_t)ctx->E * 8));
+  HIP_TRY(dev_alloc(ctx, &ctx->d_mlp_partial, (size_t)kMlpMaxWaves * mlp::NPL * 4));
   return TRIXI_HIP_OK;
 }
 
@@ -1157,7 +1240,7 @@ int trixi_hip_mlp_set_params(trixi_hip_ctx *ctx, const float *params, const floa
   std::memcpy(h.data() + 3 * mlp::NP, feature_lo, mlp::NF * 4);
   std::memcpy(h.data() + 3 * mlp::NP + mlp::NF, feature_hi, mlp::NF * 4);
   HIP_TRY(hipMemcpyAsync(ctx->d_mlp, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   ctx->mlp_ready = true;
   ctx->mlp_step = 0;
   return TRIXI_HIP_OK;
@@ -1167,7 +1250,7 @@ int trixi_hip_mlp_get_params(trixi_hip_ctx *ctx, float *params) {
   if (!ctx || !params) return TRIXI_HIP_EINVAL;
   if (!ctx->mlp_ready) return set_err(ctx, TRIXI_HIP_ESTATE, "no parameters set");
   HIP_TRY(hipMemcpyAsync(params, ctx->d_mlp, mlp::NP * 4, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
@@ -1178,7 +1261,7 @@ int trixi_hip_set_indicator(trixi_hip_ctx *ctx, int mode) {
   if (mode != TRIXI_HIP_IND_HG && !ctx->mlp_ready)
     return set_err(ctx, TRIXI_HIP_ESTATE, "the learned indicator has no parameters (trixi_hip_mlp_set_params)");
   if (mode != ctx->ind_mode) {  // the captured steps launch the indicator kernels
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(host_sync(ctx));
     drop_graphs(ctx);
   }
   ctx->ind_mode = mode;
@@ -1197,7 +1280,7 @@ int trixi_hip_mlp_features(trixi_hip_ctx *ctx, float *features_host, double *alp
                            hipMemcpyDeviceToHost, ctx->stream));
   if (alpha_hg_host)
     HIP_TRY(hipMemcpyAsync(alpha_hg_host, ctx->d_alpha_hg, (size_t)ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
@@ -1207,7 +1290,7 @@ int trixi_hip_mlp_forward(trixi_hip_ctx *ctx, const float *x_dev, int n, float *
   if (((uintptr_t)x_dev & 15) != 0) return set_err(ctx, TRIXI_HIP_EINVAL, "features must be 16-byte aligned");
   const float *lo = ctx->d_mlp + 3 * mlp::NP, *hi = lo + mlp::NF;
   hipLaunchKernelGGL(mlp::mlp_infer_kernel, dim3(mlp_blocks(n)), dim3(mlp::NTHR), 0, ctx->stream, ctx->d_mlp, x_dev,
-                     lo, hi, (const double *)nullptr, 0.0, 0, n, (double *)nullptr, y_dev);
+                     lo, hi, (const double *)nullptr, 0.0, 0.0, 0, n, (double *)nullptr, y_dev);
   HIP_TRY(hipGetLastError());
   return TRIXI_HIP_OK;
 }
@@ -1233,7 +1316,7 @@ int trixi_hip_mlp_loss_grad(trixi_hip_ctx *ctx, const float *x_dev, const float 
   if (grad_host)
     HIP_TRY(hipMemcpyAsync(grad_host, mlp_grad_buffer(ctx), mlp::NP * 4, hipMemcpyDeviceToHost, ctx->stream));
   if (loss) HIP_TRY(hipMemcpyAsync(loss, mlp_grad_buffer(ctx) + mlp::NP, 4, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
@@ -1251,7 +1334,7 @@ int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float
   HIP_TRY(hipGetLastError());
   if (loss) {
     HIP_TRY(hipMemcpyAsync(loss, mlp_grad_buffer(ctx) + mlp::NP, 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(host_sync(ctx));
   }
   return TRIXI_HIP_OK;
 }
@@ -1283,7 +1366,7 @@ int trixi_hip_indicator_hg(trixi_hip_ctx *ctx, double alpha_max, double alpha_mi
     out = ctx->d_alpha;
   }
   HIP_TRY(hipMemcpyAsync(alpha_host, out, (size_t)ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
@@ -1297,7 +1380,7 @@ int trixi_hip_indicator_max(trixi_hip_ctx *ctx, int variable, double *alpha_host
   if (!ok) return set_err(ctx, TRIXI_HIP_EINVAL, "unsupported ndims/polydeg");
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(alpha_host, ctx->d_alpha_raw, (size_t)ctx->E * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
 }
 
@@ -1321,12 +1404,12 @@ int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *ki
   // staging: kind [E] | src [E] | matrices [4 n n] (8-byte aligned after the ints)
   const size_t ints = ((size_t)2 * E + 1) & ~(size_t)1;
   char *d_tmp = nullptr;
-  if (hipMalloc(&d_tmp, ints * 4 + mats.size() * 8) != hipSuccess)
+  if (dev_alloc(to, &d_tmp, ints * 4 + mats.size() * 8) != hipSuccess)
     return set_err(to, TRIXI_HIP_ENOMEM, "transfer staging allocation failed");
   int32_t *d_kind = (int32_t *)d_tmp, *d_src = d_kind + E;
   double *d_mats = (double *)(d_tmp + ints * 4);
   int r = TRIXI_HIP_OK;
-  if (hipStreamSynchronize(from->stream) != hipSuccess ||
+  if (host_sync(from) != hipSuccess ||
       hipMemcpyAsync(d_kind, kind, (size_t)E * 4, hipMemcpyHostToDevice, to->stream) != hipSuccess ||
       hipMemcpyAsync(d_src, src, (size_t)E * 4, hipMemcpyHostToDevice, to->stream) != hipSuccess ||
       hipMemcpyAsync(d_mats, mats.data(), mats.size() * 8, hipMemcpyHostToDevice, to->stream) != hipSuccess) {
@@ -1341,7 +1424,7 @@ int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *ki
     if (!ok || hipGetLastError() != hipSuccess) r = set_err(to, TRIXI_HIP_EHIP, "transfer launch failed");
     if (!r && to->fp32) r = convert<float, double>(to, cur_u32(to), cur_u64(to), to->nvals);
   }
-  if (hipStreamSynchronize(to->stream) != hipSuccess && !r) r = set_err(to, TRIXI_HIP_EHIP, "transfer failed");
+  if (host_sync(to) != hipSuccess && !r) r = set_err(to, TRIXI_HIP_EHIP, "transfer failed");
   (void)hipFree(d_tmp);
   return r;
 }
@@ -1350,7 +1433,7 @@ int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *
                                  const double *modes) {
   if (!ctx || ic_kind < 0 || ic_kind > TRIXI_HIP_IC_BLOB) return TRIXI_HIP_EINVAL;
   // captured steps hold the boundary-state kernel with the old function
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   drop_graphs(ctx);
   ctx->ghost_ready = false;
   ctx->exact.kind = ic_kind;
@@ -1370,7 +1453,7 @@ int trixi_hip_error_norms(trixi_hip_ctx *ctx, double t, double *l2, double *linf
   if (r) return r;
   double res[16];
   HIP_TRY(hipMemcpyAsync(res, ctx->d_scalars + kErrSums, 2 * V * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   if ((r = total_volumes(ctx))) return r;
   for (int v = 0; v < V; ++v) {
     l2[v] = std::sqrt(res[v] / ctx->tv_err);
@@ -1386,7 +1469,7 @@ int trixi_hip_integrals(trixi_hip_ctx *ctx, double *cons_integrals, double *entr
   if (r) return r;
   double res[8];
   HIP_TRY(hipMemcpyAsync(res, ctx->d_scalars + kIntSums, (V + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   if ((r = total_volumes(ctx))) return r;
   for (int v = 0; v < V; ++v) cons_integrals[v] = res[v] / ctx->tv_int;
   *entropy_integral = res[V] / ctx->tv_int;
@@ -1396,13 +1479,13 @@ int trixi_hip_integrals(trixi_hip_ctx *ctx, double *cons_integrals, double *entr
 int trixi_hip_set_analysis(trixi_hip_ctx *ctx, int interval, int capacity) {
   if (!ctx || interval < 0 || (interval > 0 && capacity <= 0)) return TRIXI_HIP_EINVAL;
   if (ctx->d_records) {
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(host_sync(ctx));
     HIP_TRY(hipFree(ctx->d_records));
     ctx->d_records = nullptr;
   }
   ctx->rec_interval = interval;
   ctx->rec_cap = interval > 0 ? capacity : 0;
-  if (interval > 0) HIP_TRY(hipMalloc(&ctx->d_records, (size_t)capacity * rec_width(ctx) * 8));
+  if (interval > 0) HIP_TRY(dev_alloc(ctx, &ctx->d_records, (size_t)capacity * rec_width(ctx) * 8));
   return TRIXI_HIP_OK;
 }
 
@@ -1412,12 +1495,12 @@ int trixi_hip_get_analysis(trixi_hip_ctx *ctx, double *records, int max_records,
   if (!ctx->d_records) return TRIXI_HIP_OK;
   double total = 0;
   HIP_TRY(hipMemcpyAsync(&total, ctx->d_scalars + kCtl + 7, 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(host_sync(ctx));
   const int W = rec_width(ctx), V = ctx->V;
   const long long made = (long long)total;
   const int kept = (int)std::min<long long>(made, ctx->rec_cap);
   std::vector<double> raw((size_t)ctx->rec_cap * W);
-  HIP_TRY(hipMemcpy(raw.data(), ctx->d_records, raw.size() * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(sync_copy(ctx, raw.data(), ctx->d_records, raw.size() * 8, hipMemcpyDeviceToHost));
   int r = total_volumes(ctx);
   if (r) return r;
   const int nout = std::min(kept, max_records);
@@ -1435,6 +1518,114 @@ int trixi_hip_get_analysis(trixi_hip_ctx *ctx, double *records, int max_records,
     dst[2 + 3 * V] = src[2 + 3 * V] / ctx->tv_int;
   }
   *count = nout;
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_get_counters(const trixi_hip_ctx *ctx, int64_t *device_allocations, int64_t *host_synchronizations) {
+  if (!ctx) return TRIXI_HIP_EINVAL;
+  if (device_allocations) *device_allocations = ctx->n_alloc;
+  if (host_synchronizations) *host_synchronizations = ctx->n_sync;
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_autotune(trixi_hip_ctx *ctx, int repetitions, trixi_hip_tile_timing *timings, int max_timings,
+                       int *ntimings) {
+  if (!ctx || repetitions <= 0 || max_timings < 0 || (max_timings > 0 && !timings)) return TRIXI_HIP_EINVAL;
+  if (ntimings) *ntimings = 0;
+  const int reps = repetitions + (repetitions & 1);  // even: the fused stage returns to its buffer parity
+  struct Cand {
+    int stage, epb, nt;
+    ElemLaunch fn;
+    double best_us;
+  };
+  std::vector<Cand> cand;
+  const bool fused = ctx->elem_stage != nullptr;  // the stage variant is in use (not switched off)
+  for (const auto &kv : elem_registry()) {
+    const ElemKey &k = kv.first;
+    if (k.fp32 == (ctx->fp32 ? 1 : 0) && k.D == ctx->D && k.N == ctx->N && k.vol == ctx->vol &&
+        k.bnd == (ctx->B + ctx->M > 0 ? 1 : 0) && (k.stage == 0 || fused))
+      cand.push_back(Cand{k.stage, k.epb, k.nt, kv.second, 1e300});
+  }
+  // captured steps hold the kernels of the old tiles
+  HIP_TRY(host_sync(ctx));
+  drop_graphs(ctx);
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  HIP_TRY(hipEventCreate(&ev0));
+  if (hipEventCreate(&ev1) != hipSuccess) {
+    (void)hipEventDestroy(ev0);
+    return set_err(ctx, TRIXI_HIP_EHIP, "hipEventCreate failed");
+  }
+  const ElemLaunch keep_elem = ctx->elem, keep_stage = ctx->elem_stage;
+  const int keep[4] = {ctx->epb, ctx->nt, ctx->epb_st, ctx->nt_st};
+  // The RHS kernels write d_du from the resident state. The stage kernels run
+  // as first stages with a = b = 0, which leaves the state unchanged (u_tmp =
+  // du, u = u + 0 u_tmp) and, over an even number of launches, in its buffer.
+  auto launch = [&](const Cand &c) -> int {
+    if (c.stage == 0) return launch_rhs_native(ctx, ctx->d_du, native_u(ctx), 0.0, nullptr);
+    return launch_stage(ctx, 0.0, 0.0, 0.0, nullptr, 1);
+  };
+  int r = TRIXI_HIP_OK;
+  for (int round = 0; round < 3 && !r; ++round) {
+    for (auto &c : cand) {
+      if (c.stage == 0) {
+        ctx->elem = c.fn;
+        ctx->epb = c.epb;
+        ctx->nt = c.nt;
+      } else {
+        ctx->elem_stage = c.fn;
+        ctx->epb_st = c.epb;
+        ctx->nt_st = c.nt;
+      }
+      for (int i = 0; i < 2 && !r; ++i) r = launch(c);
+      if (!r && hipEventRecord(ev0, ctx->stream) != hipSuccess) r = TRIXI_HIP_EHIP;
+      for (int i = 0; i < reps && !r; ++i) r = launch(c);
+      if (!r && hipEventRecord(ev1, ctx->stream) != hipSuccess) r = TRIXI_HIP_EHIP;
+      ++ctx->n_sync;
+      float ms = 0.f;
+      if (!r && (hipEventSynchronize(ev1) != hipSuccess || hipEventElapsedTime(&ms, ev0, ev1) != hipSuccess))
+        r = TRIXI_HIP_EHIP;
+      if (!r) c.best_us = std::min(c.best_us, 1e3 * ms / reps);
+      if (r) break;
+    }
+  }
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  ctx->elem = keep_elem;
+  ctx->elem_stage = keep_stage;
+  ctx->epb = keep[0];
+  ctx->nt = keep[1];
+  ctx->epb_st = keep[2];
+  ctx->nt_st = keep[3];
+  if (r) return r == TRIXI_HIP_EHIP ? set_err(ctx, r, "autotune: a timed launch failed") : r;
+  // the fastest tile of each variant becomes the context's and this process's default
+  for (int stage = 0; stage < 2; ++stage) {
+    const Cand *best = nullptr;
+    for (const auto &c : cand)
+      if (c.stage == stage && (!best || c.best_us < best->best_us)) best = &c;
+    if (!best) continue;
+    if (stage == 0) {
+      ctx->elem = best->fn;
+      ctx->epb = best->epb;
+      ctx->nt = best->nt;
+    } else {
+      ctx->elem_stage = best->fn;
+      ctx->epb_st = best->epb;
+      ctx->nt_st = best->nt;
+    }
+    std::lock_guard<std::mutex> lock(g_tuned_mutex);
+    g_tuned[tune_key(ctx, stage)] = {best->epb, best->nt};
+  }
+  const int nout = std::min<int>((int)cand.size(), max_timings);
+  for (int i = 0; i < nout; ++i)
+    timings[i] = trixi_hip_tile_timing{cand[i].stage, cand[i].epb, cand[i].nt, cand[i].best_us};
+  if (ntimings) *ntimings = (int)cand.size();
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_get_stage_tile(const trixi_hip_ctx *ctx, int *elements_per_block, int *threads_per_block) {
+  if (!ctx) return TRIXI_HIP_EINVAL;
+  if (elements_per_block) *elements_per_block = ctx->epb_st;
+  if (threads_per_block) *threads_per_block = ctx->nt_st;
   return TRIXI_HIP_OK;
 }
 

@@ -87,9 +87,21 @@ flux_code(::typeof(flux_central)) = Int32(3)
 
 vi_codes(::VolumeIntegralWeakForm) = (Int32(0), Int32(2), Int32(0), nothing)
 vi_codes(v::VolumeIntegralFluxDifferencing) = (Int32(1), flux_code(v.volume_flux), Int32(0), nothing)
+# The device implements the classic first-order HG blend only
+# (VolumeIntegralShockCapturingHG, dg.jl:208-221): flux differencing everywhere
+# outside the blended region (volume_integral_default, used for dg_only
+# elements, calc_volume_integral.jl:250-255) and as the high-order part, and
+# the first-order LGL finite volume as the low-order part. Other combinations
+# (e.g. VolumeIntegralShockCapturingRRG, whose low-order part is
+# VolumeIntegralPureLGLFiniteVolumeO2) are refused.
 function vi_codes(v::VolumeIntegralShockCapturingHGType)
-    (Int32(2), flux_code(v.volume_integral_blend_high_order.volume_flux),
-     flux_code(v.volume_integral_blend_low_order.volume_flux_fv), v.indicator)
+    hi, lo, default = v.volume_integral_blend_high_order, v.volume_integral_blend_low_order,
+                      v.volume_integral_default
+    hi isa VolumeIntegralFluxDifferencing && lo isa VolumeIntegralPureLGLFiniteVolume &&
+        default isa VolumeIntegralFluxDifferencing && default.volume_flux === hi.volume_flux ||
+        throw(ArgumentError("TrixiHIP supports VolumeIntegralShockCapturingHG (flux differencing blended " *
+                            "with the first-order LGL finite volume) only"))
+    (Int32(2), flux_code(hi.volume_flux), flux_code(lo.volume_flux_fv), v.indicator)
 end
 vi_codes(v::VolumeIntegralPureLGLFiniteVolume) = (Int32(3), Int32(2), flux_code(v.volume_flux_fv), nothing)
 

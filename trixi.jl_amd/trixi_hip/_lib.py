@@ -151,6 +151,9 @@ SIGNATURES = [
     ("trixi_hip_set_analysis", _I, [_P, _I, _I]),
     ("trixi_hip_get_analysis", _I, [_P, _PD, _I, _PI]),
     ("trixi_hip_get_tile", _I, [_P, _PI, _PI]),
+    ("trixi_hip_get_stage_tile", _I, [_P, _PI, _PI]),
+    ("trixi_hip_autotune", _I, [_P, _I, _P, _I, _PI]),
+    ("trixi_hip_get_counters", _I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     ("trixi_hip_mlp_set_params", _I, [_P, _P, _P, _P]),
     ("trixi_hip_mlp_get_params", _I, [_P, _P]),
     ("trixi_hip_set_indicator", _I, [_P, _I]),
@@ -159,6 +162,12 @@ SIGNATURES = [
     ("trixi_hip_mlp_loss_grad", _I, [_P, _P, _P, _I, _P, _P]),
     ("trixi_hip_mlp_train_step", _I, [_P, _P, _P, _I, _F, _P]),
 ]
+
+class TileTiming(ctypes.Structure):
+    """trixi_hip_tile_timing"""
+    _fields_ = [("stage", ctypes.c_int32), ("elements_per_block", ctypes.c_int32),
+                ("threads_per_block", ctypes.c_int32), ("microseconds", ctypes.c_double)]
+
 
 _lib = None
 

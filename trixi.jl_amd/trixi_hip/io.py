@@ -320,7 +320,9 @@ class SaveRestartCallback:
         return self.interval > 0 and (naccept % self.interval == 0 or (self.save_final_restart and final))
 
     def initialize(self, semi, t, dt, callbacks):
+        # initialize! (save_restart.jl:59-72): the directory and the mesh file
         os.makedirs(self.output_directory, exist_ok=True)
+        _save_mesh(semi, self.output_directory)
 
     def __call__(self, semi, t, dt, naccept, callbacks):
         _save_mesh(semi, self.output_directory, naccept)

@@ -12,8 +12,10 @@ Inputs (computed on the device from the state, see the HIP kernel):
   0..15  nodal density*pressure, centred per element and scaled by the largest
          deviation; 16, 17 the HG logits of the two modal energy ratios / 8,
          clipped to [-2, 2]; 18 log10 of the smoothest-modes energy share / 8.
-Output: alpha = alpha_max * sigmoid(net). With fallback (the default) an
-element whose inputs leave the range seen in training uses the HG value.
+Output: alpha = alpha_max * sigmoid(net), clipped as HG clips its factor
+(indicators_2d.jl:86-99): 0 below alpha_min (the element kernel then skips the
+FV subcells of that element), at most alpha_max. With fallback (the default)
+an element whose inputs leave the range seen in training uses the HG value.
 
 Everything numerical runs in libtrixi_hip.so (BF16 MFMA forward and backward,
 Adam on FP32 master weights). Device arrays are torch tensors (memory only).
@@ -282,6 +284,180 @@ def evaluate(net, seed=1000, level=5, t_end=1.0, nsnap=5):
                 rho_max_diff=float(np.abs(drho).max()))
 
 
+# ---------------------------------------------------------------------------
+# Judging the learned indicator by the HG run's solution error (KHI, level 7)
+# ---------------------------------------------------------------------------
+def transfer_uniform(to, frm):
+    """Carry the resident state of `frm` into `to` on the device, for two
+    uniform meshes of the same domain one level apart: refinement (each element
+    of frm interpolated to its 2^D children, which follow each other in child
+    order in the finer mesh's element order) or the L2 projection of the 2^D
+    children onto their parent (amr_dg2d.jl; trixi_hip_transfer)."""
+    D = to.mesh.ndims
+    nch = 1 << D
+    if to.nelements() == nch * frm.nelements():
+        E = frm.nelements()
+        kind = np.tile(np.arange(1, nch + 1, dtype=np.int32), E)
+        src = np.repeat(np.arange(E, dtype=np.int32), nch)
+    elif frm.nelements() == nch * to.nelements():
+        E = to.nelements()
+        kind = np.full(E, -1, dtype=np.int32)
+        src = (nch * np.arange(E)).astype(np.int32)
+    else:
+        raise ValueError("meshes are not one uniform refinement level apart")
+    _lib.check(to._lib.trixi_hip_transfer(to.ctx, frm.ctx, kind.ctypes.data_as(ctypes.c_void_p),
+                                          src.ctypes.data_as(ctypes.c_void_p)), to.ctx, "transfer")
+
+
+def _download(semi):
+    u = np.empty(semi.nvalues())
+    _lib.check(semi._lib.trixi_hip_download_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p)), semi.ctx, "download_u")
+    return u
+
+
+def _upload(semi, u):
+    _lib.check(semi._lib.trixi_hip_upload_u(semi.ctx, np.ascontiguousarray(u).ctypes.data_as(ctypes.c_void_p)),
+               semi.ctx, "upload_u")
+
+
+def l2_density(semi, u_a, u_b):
+    """L2 norm of the density difference over the domain, by the LGL quadrature
+    of the DG solution (the integral calc_error_norms evaluates, here on the
+    solution nodes), normalised by the domain volume."""
+    b = semi.solver.basis
+    D = semi.mesh.ndims
+    w = np.asarray(b.weights)
+    W = w if D == 1 else (np.outer(w, w).reshape(-1) if D == 2 else np.einsum("i,j,k->kji", w, w, w).reshape(-1))
+    V = semi.nvariables()
+    d = (np.asarray(u_a) - np.asarray(u_b)).reshape(semi.nelements(), -1, V)[:, :, 0]
+    jac = (1.0 / semi.inverse_jacobian) ** D  # volume of the reference element's image / 2^D
+    return float(np.sqrt(np.sum(jac[:, None] * W[None, :] * d * d) / np.sum(jac[:, None] * W[None, :])))
+
+
+def rhs_time(semi, u, mode, launches=200):
+    """Time of one RHS (indicator kernels and element kernel) of the shock-
+    capturing operator with blending factors from indicator `mode`, replayed
+    from a HIP graph on device copies of u in the context's precision."""
+    import torch
+    lib, ctx = semi._lib, semi.ctx
+    _lib.check(lib.trixi_hip_set_indicator(ctx, int(mode)), ctx, "set_indicator")
+    dtype = torch.float32 if semi.precision == "fp32" else torch.float64
+    ud = torch.from_numpy(u).to(device="cuda", dtype=dtype)
+    dud = torch.empty_like(ud)
+    stream = torch.cuda.Stream()
+    _lib.check(lib.trixi_hip_set_stream(ctx, ctypes.c_void_p(stream.cuda_stream)), ctx, "set_stream")
+
+    def launch():
+        _lib.check(lib.trixi_hip_rhs_native(ctx, ctypes.c_void_p(dud.data_ptr()), ctypes.c_void_p(ud.data_ptr()),
+                                            0.0), ctx, "rhs_native")
+
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        for _ in range(20):
+            launch()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        for _ in range(10):
+            launch()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    with torch.cuda.stream(stream):
+        g.replay()
+        ev[0].record(stream)
+        for _ in range(launches // 10):
+            g.replay()
+        ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / (10 * (launches // 10))
+    _lib.check(lib.trixi_hip_set_stream(ctx, None), ctx, "set_stream")
+    _lib.check(lib.trixi_hip_set_indicator(ctx, _lib.IND_HG), ctx, "set_indicator")
+    del g
+    return ms
+
+
+def blended_share(semi, alpha, elements_per_tile=16):
+    """Share of elements whose blending factor is not dg_only
+    (calc_volume_integral.jl:240-255: |alpha| > atol of the RHS precision) and
+    share of element-kernel workgroup tiles with at least one such element."""
+    atol = 1.1920928955078125e-5 if semi.precision == "fp32" else 1.8189894035458565e-12
+    b = np.abs(alpha) > atol
+    E = b.size
+    pad = (-E) % elements_per_tile
+    tiles = np.concatenate([b, np.zeros(pad, bool)]).reshape(-1, elements_per_tile).any(axis=1)
+    return float(b.mean()), float(tiles.mean())
+
+
+def evaluate_config5(net, seed=1000, level=7, t_end=3.0, ref_level=8, cfl=1.3, log=None):
+    """The learned indicator judged by the HG run's solution error on a KHI
+    seed not used in training: three FP64 runs from the same seeded initial
+    state to t_end, (1) HG at `level`, (2) the network (with HG fallback) at
+    `level`, (3) the reference, HG at `ref_level` started from the level-`level`
+    initial state interpolated to the finer mesh. The errors are the L2 norms of
+    the density differences to the reference projected (L2) onto the coarser
+    mesh. Also: the direct density difference of runs (1) and (2), the RHS time
+    of both operators (FP64 and FP32) on the final HG state, the blended shares
+    and the share of elements that fell back to HG."""
+    from .semidiscretization import ODEProblem, rhs_hyperbolic_
+    from .time_integration import CallbackSet, CarpenterKennedy2N54, StepsizeCallback, solve
+
+    def say(msg):
+        if log:
+            log(msg)
+
+    def run(semi, u0):
+        sol = solve(ODEProblem(rhs_hyperbolic_, u0, (0.0, t_end), semi), CarpenterKennedy2N54(),
+                    callback=CallbackSet(StepsizeCallback(cfl=cfl)))
+        return sol
+
+    out = dict(seed=seed, level=level, ref_level=ref_level, t_end=t_end, cfl=cfl)
+    hg = khi_semidiscretization(level, 1e-3, seed, "fp64")
+    u0 = hg.compute_coefficients(0.0)
+    s_hg = run(hg, u0)
+    out["steps_hg"] = s_hg.nsteps
+    out["wall_s_hg"] = s_hg.wall_time
+    say(f"HG level {level}: {s_hg.nsteps} steps, {s_hg.wall_time:.2f} s")
+    nn = khi_semidiscretization(level, 1e-3, seed, "fp64")
+    net.attach(nn)
+    s_nn = run(nn, u0)
+    out["steps_nn"] = s_nn.nsteps
+    out["wall_s_nn"] = s_nn.wall_time
+    say(f"NN level {level}: {s_nn.nsteps} steps, {s_nn.wall_time:.2f} s")
+    # reference: the same initial polynomial state on the finer mesh
+    ref = khi_semidiscretization(ref_level, 1e-3, seed, "fp64")
+    _upload(hg, u0)
+    transfer_uniform(ref, hg)
+    s_ref = run(ref, _download(ref))
+    out["steps_ref"] = s_ref.nsteps
+    say(f"HG level {ref_level}: {s_ref.nsteps} steps, {s_ref.wall_time:.2f} s")
+    transfer_uniform(hg, ref)  # the reference projected onto the coarser mesh
+    u_ref = _download(hg)
+    out["l2_rho_error_hg"] = l2_density(hg, s_hg.u, u_ref)
+    out["l2_rho_error_nn"] = l2_density(hg, s_nn.u, u_ref)
+    out["error_ratio_nn_over_hg"] = out["l2_rho_error_nn"] / out["l2_rho_error_hg"]
+    out["l2_rho_nn_minus_hg"] = l2_density(hg, s_nn.u, s_hg.u)
+    out["l2_rho_hg_norm"] = l2_density(hg, s_hg.u, 0 * s_hg.u)
+    # fallback share and blended shares on the final states
+    _upload(nn, s_nn.u)
+    f, _ = features(nn)
+    outside = np.any((f[:, :NFEATURES] < net.feature_lo) | (f[:, :NFEATURES] > net.feature_hi), axis=1)
+    out["fallback_share_final"] = float(np.mean(outside))
+    ref.close()
+    nn.close()
+    # RHS time of both operators on the final HG state, FP64 and FP32
+    for prec in ("fp64", "fp32"):
+        sm = khi_semidiscretization(level, 1e-3, seed, prec)
+        _upload(sm, s_hg.u)
+        net.upload(sm)
+        for name, mode in (("hg", _lib.IND_HG), ("nn", net.mode)):
+            out[f"rhs_ms_{name}_{prec}"] = rhs_time(sm, s_hg.u, mode)
+            el, tl = blended_share(sm, indicator_alpha(sm, mode))
+            out[f"blended_elements_{name}_{prec}"] = el
+            out[f"blended_tiles_{name}_{prec}"] = tl
+        sm.close()
+    hg.close()
+    say(str(out))
+    return out
+
+
 def main(argv=None):
     import argparse
     import json
@@ -296,13 +472,20 @@ def main(argv=None):
     p.add_argument("--batch", type=int, default=8192)
     p.add_argument("--lr", type=float, default=2e-3)
     p.add_argument("--evaluate", action="store_true", help="evaluate the weights at --out instead of training")
+    p.add_argument("--config5", action="store_true",
+                   help="judge the weights at --out by the HG run's density error (evaluate_config5)")
+    p.add_argument("--seed", type=int, default=1000, help="KHI seed of --evaluate / --config5")
     a = p.parse_args(argv)
 
     def log(msg):
         print(f"[learned_indicator] {msg}", file=sys.stderr, flush=True)
 
     if a.evaluate:
-        print(json.dumps(evaluate(IndicatorNeuralNetwork.load(a.out), level=a.level)))
+        print(json.dumps(evaluate(IndicatorNeuralNetwork.load(a.out), seed=a.seed, level=a.level)))
+        return
+    if a.config5:
+        print(json.dumps(evaluate_config5(IndicatorNeuralNetwork.load(a.out), seed=a.seed, level=a.level,
+                                          t_end=a.t_end, ref_level=a.level + 1, log=log)))
         return
 
     X, Y = collect_khi_data(range(a.seeds), a.level, a.t_end, a.snapshots, log=log)

@@ -195,9 +195,17 @@ class SemidiscretizationHyperbolic:
             self._lib.trixi_hip_destroy(old)
 
     def tile_used(self):
+        """(elements per workgroup, threads) of the element kernel's RHS variant."""
         e, t = ctypes.c_int(), ctypes.c_int()
         _lib.check(self._lib.trixi_hip_get_tile(self.ctx, ctypes.byref(e), ctypes.byref(t)), self.ctx, "get_tile")
         return e.value, t.value
+
+    def stage_tile_used(self):
+        """The same for the fused 2N-stage variant (None when the stage runs unfused)."""
+        e, t = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.trixi_hip_get_stage_tile(self.ctx, ctypes.byref(e), ctypes.byref(t)), self.ctx,
+                   "get_stage_tile")
+        return (e.value, t.value) if e.value > 0 else None
 
     def close(self):
         if self._ctx is not None:
