@@ -63,6 +63,11 @@ def parse():
     p.add_argument("--profile-step", choices=["fp32", "fp64"], default=None,
                    help="profiling mode: only run K device-controlled CK2N54 steps (fused stages) of the "
                         "workload in one precision and print the time per step (no JSON line)")
+    p.add_argument("--no-configs", action="store_true",
+                   help="skip the other benchmark configurations (3D p4 Fourier modes, 2D KHI with HG and with "
+                        "the learned indicator)")
+    p.add_argument("--profile-config", choices=["c3", "c4", "c5"], default=None,
+                   help="profiling mode: only time that configuration's RHS (both precisions; no JSON line)")
     return p.parse_args()
 
 
@@ -77,6 +82,94 @@ def make_semi(th, level, polydeg, precision, tile=(0, 0), periodic=True, patches
                                            source_terms=th.source_terms_convergence_test, boundary_conditions=bc,
                                            precision=precision, elements_per_block=tile[0],
                                            threads_per_block=tile[1])
+
+
+def make_semi_c3(th, precision, seed=0):
+    """3D Euler p4 on [-1,1]^3 at level 5, seeded Fourier-mode state, flux_ranocha
+    FD volume integral, flux_lax_friedrichs surface flux, no sources."""
+    mesh = th.TreeMesh((-1.0,) * 3, (1.0,) * 3, initial_refinement_level=5, periodicity=True)
+    solver = th.DGSEM(polydeg=4, surface_flux=th.flux_lax_friedrichs,
+                      volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_ranocha))
+    return th.SemidiscretizationHyperbolic(mesh, th.CompressibleEulerEquations3D(1.4),
+                                           th.initial_condition_fourier_modes(seed), solver, precision=precision)
+
+
+def step_time(th, semi, steps):
+    """Device-controlled CK2N54 steps (fused stages, step graph, device dt) from
+    the initial state: microseconds per step, and the tiles used."""
+    ode = th.semidiscretize(semi, (0.0, 1e9))
+    th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=1.0)),
+             maxiters=20, save_final=False)  # graph capture
+    sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=1.0)),
+                   maxiters=steps, save_final=False)
+    return dict(us_per_step=1e6 * sol.wall_time / sol.nsteps, steps=sol.nsteps, stage_tile=semi.stage_tile_used())
+
+
+def roofline_line(ndofs, nvars, bytes_per_value, ms):
+    alg = 2 * nvars * ndofs * bytes_per_value  # read u once, write du once
+    achieved = alg / (ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                algorithmic_bytes=alg)
+
+
+def config_c3(th, torch, steps, warmup, with_steps=True):
+    """3D p4 32^3: RHS throughput (FP32, FP64) and the fused-stage time."""
+    out = dict(workload="3D Euler, polydeg 4 (125 nodes/element), TreeMesh level 5 (32^3 elements) on [-1,1]^3, "
+                        "seeded Fourier-mode state (seed 0), flux_ranocha FD volume, flux_lax_friedrichs surface")
+    for prec in ("fp32", "fp64"):
+        semi = make_semi_c3(th, prec)
+        u0 = semi.compute_coefficients(0.0)
+        r = time_rhs(th, torch, semi, steps, warmup, u0)
+        nd = semi.ndofs()
+        out[prec] = dict(value=nd / (r["ms_per_step"] * 1e-3), ms_per_step=r["ms_per_step"], tile=semi.tile_used(),
+                         roofline=roofline_line(nd, 5, 4 if prec == "fp32" else 8, r["kernel_avg_ms"]),
+                         finite=r["finite"])
+        if with_steps:
+            out[prec]["step"] = step_time(th, semi, 100)
+        semi.close()
+    out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    out["ndofs"] = nd
+    return out
+
+
+def config_c4_c5(th, torch, steps, t_state=1.0):
+    """2D KHI p3 128^2 with HG shock capturing (C4) and with the learned
+    indicator in its place (C5): RHS time (indicator kernels + element kernel,
+    graph-replayed) on the FP64 HG run's state at t_state, the blended shares
+    and, for C5, the share of elements that fell back to HG."""
+    import numpy as np
+    from trixi_hip import _lib
+    from trixi_hip.learned_indicator import (IndicatorNeuralNetwork, blended_share, features, indicator_alpha,
+                                             khi_semidiscretization, rhs_time, NFEATURES)
+    semi = khi_semidiscretization(7, 1e-3, 0, "fp64")
+    sol = th.solve(th.semidiscretize(semi, (0.0, t_state)), th.CarpenterKennedy2N54(),
+                   callback=th.CallbackSet(th.StepsizeCallback(cfl=1.3)))
+    u = sol.u
+    semi.close()
+    net = IndicatorNeuralNetwork.load()
+    out = dict(workload=f"2D Euler Kelvin-Helmholtz, polydeg 3, TreeMesh level 7 (128^2 elements), HG shock "
+                        f"capturing (alpha_max 0.002, alpha_min 1e-4, density_pressure), flux_ranocha volume, "
+                        f"LLF (naive wave speed) surface and FV fluxes; state of the FP64 HG run at t = "
+                        f"{t_state} ({sol.nsteps} steps, seed 0)", ndofs=None)
+    for prec in ("fp32", "fp64"):
+        sm = khi_semidiscretization(7, 1e-3, 0, prec)
+        sm._lib.trixi_hip_upload_u(sm.ctx, u.ctypes.data_as(ctypes.c_void_p))
+        net.upload(sm)
+        nd = sm.ndofs()
+        out["ndofs"] = nd
+        for name, mode in (("c4_hg", _lib.IND_HG), ("c5_learned", net.mode)):
+            ms = rhs_time(sm, u, mode, launches=max(steps, 100))
+            el, tl = blended_share(sm, indicator_alpha(sm, mode))
+            out.setdefault(name, {})[prec] = dict(value=nd / (ms * 1e-3), ms_per_rhs=ms,
+                                                  blended_elements=el, blended_tiles=tl,
+                                                  roofline=roofline_line(nd, 4, 4 if prec == "fp32" else 8, ms))
+        f, _ = features(sm)
+        out["c5_learned"]["fallback_share"] = float(np.mean(np.any(
+            (f[:, :NFEATURES] < net.feature_lo) | (f[:, :NFEATURES] > net.feature_hi), axis=1)))
+        sm.close()
+    for prec in ("fp32", "fp64"):
+        out[f"learned_over_hg_rhs_time_{prec}"] = out["c5_learned"][prec]["ms_per_rhs"] / out["c4_hg"][prec]["ms_per_rhs"]
+    return out
 
 
 def time_rhs(th, torch, semi, steps, warmup, u0):
@@ -317,6 +410,14 @@ def main():
         semi.close()
         return
 
+    if args.profile_config:
+        if args.profile_config == "c3":
+            r = config_c3(th, torch, args.steps, args.warmup, with_steps=False)
+        else:
+            r = config_c4_c5(th, torch, args.steps)
+        print(f"[bench] {args.profile_config}: {r}", flush=True)
+        return
+
     if args.profile_amr:
         for rep in ("cold", "warm"):  # the second run finds the kernels loaded
             r = amr_run(th, "fp64")
@@ -335,6 +436,7 @@ def main():
             continue
         res[prec] = time_rhs(th, torch, semi, args.steps, args.warmup, u0)
         res[prec]["tile"] = semi.tile_used()
+        res[prec]["stage_tile"] = semi.stage_tile_used()
         print(f"[bench] {prec}: {res[prec]}", file=sys.stderr, flush=True)
 
     if args.copy_calibration:
@@ -386,6 +488,13 @@ def main():
             io_res = io_timing(th, semis["fp64"], u0, d)
         print(f"[bench] io: {io_res}", file=sys.stderr, flush=True)
 
+    configs = {}
+    if not args.no_configs and not args.only:
+        configs["c3_3d_p4_32cubed"] = config_c3(th, torch, args.steps, args.warmup)
+        print(f"[bench] c3: {configs['c3_3d_p4_32cubed']}", file=sys.stderr, flush=True)
+        configs["c4_c5_khi_128sq"] = config_c4_c5(th, torch, args.steps)
+        print(f"[bench] c4/c5: {configs['c4_c5_khi_128sq']}", file=sys.stderr, flush=True)
+
     ndofs = semis["fp32"].ndofs()
     nv = semis["fp32"].nvariables()
     l2 = None
@@ -424,6 +533,7 @@ def main():
             "ndofs": ndofs,
             "nelements": (2 ** args.level) ** 3,
             "element_tile": list(r["tile"]),
+            "stage_tile": list(r["stage_tile"]) if r["stage_tile"] else None,
             "rotating_buffers": r["rotate"],
         },
         "roofline": {
@@ -444,7 +554,8 @@ def main():
         v64 = ndofs / (res["fp64"]["ms_per_step"] * 1e-3)
         out["fp64_plain"] = {"value": v64, "ms_per_step": res["fp64"]["ms_per_step"],
                              "kernel_avg_ms": res["fp64"]["kernel_avg_ms"],
-                             "tile": list(res["fp64"]["tile"])}
+                             "tile": list(res["fp64"]["tile"]),
+                             "roofline": roofline_line(ndofs, nv, 8, res["fp64"]["kernel_avg_ms"])}
         out["fp32_over_fp64"] = value / v64
     if bounded:
         out["bounded_mesh"] = dict(workload="same RHS, periodicity=false, BoundaryConditionDirichlet("
@@ -462,6 +573,8 @@ def main():
     if io_res is not None:
         out["restart_io"] = dict(workload="restart file (HDF5, FP64 conserved variables) of the state above: "
                                           "device download + write, read + checks", **io_res)
+    if configs:
+        out["configs"] = configs
     if l2 is not None:
         out["equal_l2"] = l2
     if cpu is not None:

@@ -27,14 +27,15 @@ C3 = dict(ndims=3, polydeg=4, level=5, cmin=(-1,) * 3, cmax=(1,) * 3, ic="fourie
 C4 = dict(ndims=2, polydeg=3, level=7, cmin=(-1, -1), cmax=(1, 1), ic="kelvin_helmholtz", ic_params=(1e-3, 0),
           volume_integral="hg", surface_flux="llf_naive", fv_flux="llf_naive", alpha_max=0.002, alpha_min=1e-4)
 
-# (relative L2, relative Linf) bounds of the FP32 RHS against the FP64 oracle
+# (relative L2, relative Linf) bounds of the FP32 RHS against the FP64 oracle,
+# about 3x the values measured on MI355X (in brackets)
 BOUNDS = {
-    "C1": (1e-4, 1e-4),
-    "C2": (1e-4, 1e-4),
-    "C3": (1e-4, 1e-4),
-    "C4_t0": (1e-4, 1e-4),
-    "C4_t1": (1e-4, 1e-4),
-    "C5_t1": (1e-4, 1e-4),
+    "C1": (3.5e-5, 2.5e-5),     # (1.1e-5, 7.2e-6)
+    "C2": (6e-4, 1.8e-3),       # (2.0e-4, 6.0e-4): du is an O(h) difference of O(1) fluxes at 32^3
+    "C3": (1e-5, 3e-5),         # (3.1e-6, 1.0e-5)
+    "C4_t0": (3.5e-4, 5e-4),    # (1.1e-4, 1.5e-4)
+    "C4_t1": (7e-4, 1.3e-3),    # (2.2e-4, 4.3e-4)
+    "C5_t1": (7e-4, 1.3e-3),    # the C4_t1 operator with other blending factors
 }
 MEASURED = {}
 

@@ -108,15 +108,18 @@ def test_features_closed_forms():
 def test_committed_network_tracks_hg_on_khi_states():
     """The committed weights, evaluated by the FP64 reference on Kelvin-Helmholtz
     states of the oracle (level 4, t = 0 and t = 0.4; no noise, unlike the
-    training runs): alpha close to the HG indicator's raw alpha."""
+    training runs): the clipped alpha close to the training target made from
+    the HG indicator's raw alpha."""
     from conftest import PKG
     from oracle import Oracle
     path = os.path.join(PKG, "trixi_hip", "data", "indicator_nn_khi_p3.npz")
     if not os.path.exists(path):
         pytest.skip("trained weights not present")
+    from trixi_hip.learned_indicator import training_targets
     with np.load(path, allow_pickle=False) as z:
         params = z["params"].astype(np.float64)
         assert params.shape == (mlp_ref.NP,) and z["feature_lo"].shape == (mlp_ref.NF,)
+        dead_zone = float(z["dead_zone"]) if "dead_zone" in z.files else 1.0
     o = Oracle(2, 3, 4, (-1, -1), (1, 1), volume_integral="hg", volume_flux="ranocha", surface_flux="llf_naive",
                fv_flux="llf_naive", alpha_max=0.002, alpha_min=0.0001, alpha_smooth=True)
     o.set_initial_condition("kelvin_helmholtz")
@@ -124,6 +127,16 @@ def test_committed_network_tracks_hg_on_khi_states():
     vinv = o.basis()["Vinv"]
     for state in (u, o.solve(u, 0.0, 0.4, 1.3)[0]):
         f, a_hg = mlp_ref.features(state, 2, 3, vinv, 1.4, 0.002, 0.0001)
-        a_nn = 0.002 * mlp_ref.forward(params, f)["y"]
-        err = np.abs(a_nn - a_hg) / 0.002
+        a_nn = mlp_ref.alpha(params, f, 0.002, 0.0001, bf=False)
+        err = np.abs(a_nn - 0.002 * training_targets(a_hg, 0.002, 0.0001, dead_zone)) / 0.002
         assert err.mean() < 0.05 and np.quantile(err, 0.95) < 0.2, (err.mean(), np.quantile(err, 0.95))
+
+
+def test_training_targets():
+    """HG factors divided by alpha_max, 0 inside the dead zone below
+    dead_zone * alpha_min; dead_zone 1 keeps HG's clipped factors."""
+    from trixi_hip.learned_indicator import training_targets
+    a = np.array([0.0, 1e-4, 1.2e-4, 1.6e-4, 1e-3, 2e-3])
+    np.testing.assert_array_equal(training_targets(a, 0.002, 1e-4, 1.0), (a / 0.002).astype(np.float32))
+    np.testing.assert_array_equal(training_targets(a, 0.002, 1e-4, 1.5),
+                                  np.array([0, 0, 0, 0.08, 0.5, 1.0], dtype=np.float32))

@@ -27,7 +27,8 @@ template <int N> struct VinvConst {
   double Vinv[(N + 1) * (N + 1)];
 };
 
-// Features of the learned indicator (2D p3 only; feat [E][32], zero padded):
+// Features of the learned indicator (2D p3 only; feat [E][32], zero padded;
+// computed by indicator_2d_p3_kernel):
 //   0..15  nodal rho*p, centred by its element mean and divided by the largest
 //          deviation (+ 1e-7 |mean|), so that they lie in [-1, 1];
 //   16, 17 the HG logits s/T (r - T) of the two energy ratios r, / 8, clipped
@@ -38,12 +39,9 @@ constexpr int kMlpFeatureStride = 32;
 template <typename T, int D, int N, int IPB, int NT>
 __global__ __launch_bounds__(NT) void hg_indicator_kernel(const T *__restrict__ u,
                                                           double *__restrict__ alpha_raw, int E,
-                                                          HGParams hp, VinvConst<N> vc,
-                                                          float *__restrict__ feat) {
+                                                          HGParams hp, VinvConst<N> vc) {
   constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
-  constexpr bool FEAT = (D == 2 && N == 3);
   __shared__ double s_a[IPB * NN], s_b[IPB * NN];
-  __shared__ double s_q[FEAT ? IPB * NN : 1];
   const int tid = threadIdx.x;
   const int e0 = blockIdx.x * IPB;
   if (e0 >= E) return;
@@ -57,7 +55,6 @@ __global__ __launch_bounds__(NT) void hg_indicator_kernel(const T *__restrict__ 
     s_a[q] = hp.variable == 1   ? rho
              : hp.variable == 2 ? (hp.gamma - 1) * (double(un[D + 1]) - 0.5 * m2 / rho)
                                 : (hp.gamma - 1) * (rho * double(un[D + 1]) - 0.5 * m2);
-    if constexpr (FEAT) s_q[q] = s_a[q];
   }
   __syncthreads();
   double *src = s_a, *dst = s_b;
@@ -120,24 +117,6 @@ __global__ __launch_bounds__(NT) void hg_indicator_kernel(const T *__restrict__ 
     if (al < hp.alpha_min) al = 0.0;
     if (al > 1 - hp.alpha_min) al = 1.0;
     alpha_raw[e0 + tid] = al < hp.alpha_max ? al : hp.alpha_max;
-    if constexpr (FEAT) {
-      if (feat) {
-        const double *qv = s_q + tid * NN;
-        double mean = 0, dev = 0;
-        for (int i = 0; i < NN; ++i) mean += qv[i];
-        mean /= NN;
-        for (int i = 0; i < NN; ++i) dev = fmax(dev, fabs(qv[i] - mean));
-        const double scale = dev + 1e-7 * fabs(mean) + 1e-300;
-        float *fo = feat + (size_t)(e0 + tid) * kMlpFeatureStride;
-        for (int i = 0; i < NN; ++i) fo[i] = float((qv[i] - mean) / scale);
-        const double k = hp.parameter_s / hp.threshold;
-        fo[NN] = float(fmin(2.0, fmax(-2.0, k * (ef1 - hp.threshold) / 8)));
-        fo[NN + 1] = float(fmin(2.0, fmax(-2.0, k * (ef2 - hp.threshold) / 8)));
-        const double smooth = te > 0 ? te2 / te : 1.0;
-        fo[NN + 2] = float(fmin(0.0, fmax(-2.0, log10(smooth + 1e-16) / 8)));
-        for (int i = NN + 3; i < kMlpFeatureStride; ++i) fo[i] = 0.f;
-      }
-    }
   }
 }
 

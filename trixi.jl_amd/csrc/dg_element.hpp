@@ -295,17 +295,28 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
         T own_c[2];
         consp_from_node<D>(c.own0, wend[0], a.sqrt_gamma, own_face[0], own_c[0]);
         consp_from_node<D>(c.own1, wend[1], a.sqrt_gamma, own_face[1], own_c[1]);
+        // twice the LLF flux with half the weight (bitwise the same terms);
+        // a given mortar flux is doubled to match
+        const T hfactor = T(0.5) * factor;
+        auto given_flux2 = [&](int off, T *F2) {
+          if constexpr (BND) {
+            if (off >= 0) {
+#pragma unroll
+              for (int v = 0; v < V; ++v) F2[v] = T(2) * a.mflux[off + v];
+            }
+          }
+        };
         ConsP<T, D> nb;
         consp_from_cons<T, D>(c.ul_m, gm1, nb);  // -O face: element is the right state
-        flux_llf_pc<T, D, O>(nb, own_face[0], Prec<T>::sqrt_(gamma * nb.p * nb.irho), own_c[0], naive, F);
-        given_flux(c.fm, F);
+        flux_llf_pc2<T, D, O>(nb, own_face[0], Prec<T>::sqrt_(gamma * nb.p * nb.irho), own_c[0], naive, F);
+        given_flux2(c.fm, F);
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
+        for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], hfactor, acc[0][v]);
         consp_from_cons<T, D>(c.ur_p, gm1, nb);  // +O face: element is the left state
-        flux_llf_pc<T, D, O>(own_face[1], nb, own_c[1], Prec<T>::sqrt_(gamma * nb.p * nb.irho), naive, F);
-        given_flux(c.fp, F);
+        flux_llf_pc2<T, D, O>(own_face[1], nb, own_c[1], Prec<T>::sqrt_(gamma * nb.p * nb.irho), naive, F);
+        given_flux2(c.fp, F);
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
+        for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], hfactor, acc[n - 1][v]);
         done = true;
       }
     }

@@ -1,0 +1,176 @@
+// indicator2d.hpp -- the shock-capturing indicator of 2D polydeg-3 elements
+// in one launch: IndicatorHennemannGassner's blending factor and, with FEAT,
+// the learned indicator's features, the BF16-MFMA network and its clipped
+// output, with the HG factor as fallback.
+//
+//   calc_indicator_hennemann_gassner!  indicators_2d.jl:30-102 (indicators.jl:119-156)
+//   multiply_scalar_dimensionwise!     interpolation.jl:207-235
+//   density_pressure / density / pressure  compressible_euler_2d.jl:2205-2210
+//
+// Work decomposition: one wavefront of 16 elements per workgroup. Lane l owns
+// element l % 16 and the node row j = l / 16 (nodes i + 4 j, i = 0..3), so
+// the modal transform along x runs in registers, the transform along y reads
+// the other rows of the element through LDS, and the modal energies, mean
+// and deviation are sums over the four lanes of the element (xor shuffles).
+// The lane's four normalised nodal values are exactly its part of the
+// network's B operand (features 4 j .. 4 j + 3), and the lane with j = 0 holds
+// the three energy features: the features never leave registers. The network
+// image is read from device memory (L2) where the MFMA operands need it, so
+// that the workgroup has no barrier. Arithmetic up to the features is FP64, as
+// in hg_indicator_kernel; the sums over the four rows run in a different
+// order than the reference's loops (round-off of the energies only).
+#pragma once
+#include "dg_aux.hpp"
+#include "mlp.hpp"
+
+namespace trixi_hip {
+
+struct Vinv4 {
+  double v[16];  // inverse_vandermonde_legendre, row-major [mode][node]
+};
+
+// FEAT = false: alpha_raw[e] = the HG factor. FEAT = true: the features as
+// well (into feat_out [E][32] when not null), and with the network image img
+// (not null) alpha_raw[e] = the network's clipped factor, or the HG factor
+// where an input leaves [lo, hi] or the output is not finite (fallback != 0);
+// without img alpha_raw receives the HG factor. alpha_hg_out (may be null)
+// receives the HG factor in every case.
+template <typename T, bool FEAT>
+__global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
+    const T *__restrict__ u, int E, HGParams hp, Vinv4 vi, const mlp::InferImage *__restrict__ img,
+    const float *__restrict__ lo, const float *__restrict__ hi, int fallback, double *__restrict__ alpha_raw,
+    double *__restrict__ alpha_hg_out, float *__restrict__ feat_out) {
+  constexpr int n = 4, V = 4;
+  __shared__ double s_mx[16 * 16];  // x-transformed rows [element][row j][mode i]
+  const int l = threadIdx.x;
+  const int el = l & 15, j = l >> 4;
+  const int e0 = blockIdx.x * 16;
+  const int e = e0 + el;
+  const bool valid = e < E;
+  const int ec = valid ? e : E - 1;  // lanes past the end compute on the last element, store nothing
+
+  // indicator variable at the lane's four nodes (i + 4 j)
+  double q[n];
+  const T *un = u + ((size_t)ec * 16 + 4 * j) * V;
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    const double rho = un[i * V], m2 = double(un[i * V + 1]) * double(un[i * V + 1]) +
+                                       double(un[i * V + 2]) * double(un[i * V + 2]);
+    q[i] = hp.variable == 1   ? rho
+           : hp.variable == 2 ? (hp.gamma - 1) * (double(un[i * V + 3]) - 0.5 * m2 / rho)
+                              : (hp.gamma - 1) * (rho * double(un[i * V + 3]) - 0.5 * m2);
+  }
+  // modal transform along x (own row), then along y (all rows of the element)
+  double *mx = &s_mx[el * 16];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    double r = 0;
+#pragma unroll
+    for (int i = 0; i < n; ++i) r += vi.v[a * n + i] * q[i];
+    mx[4 * j + a] = r;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  double m[n];  // modes (a, j)
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    double r = 0;
+#pragma unroll
+    for (int b = 0; b < n; ++b) r += vi.v[j * n + b] * mx[4 * b + a];
+    m[a] = r;
+  }
+  // partial energies of this row j: te2 (a, j < 2), te1 adds (a < 3, j = 2)
+  // and (a = 2, j < 2), te adds (a, j = 3) and (a = 3, j < 3)
+  double p2 = 0, p1 = 0, p0 = 0;
+  if (j < 2) {
+    p2 = m[0] * m[0] + m[1] * m[1];
+    p1 = m[2] * m[2];
+    p0 = m[3] * m[3];
+  } else if (j == 2) {
+    p1 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    p0 = m[3] * m[3];
+  } else {
+    p0 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2] + m[3] * m[3];
+  }
+  p2 += __shfl_xor(p2, 16);
+  p2 += __shfl_xor(p2, 32);
+  p1 += __shfl_xor(p1, 16);
+  p1 += __shfl_xor(p1, 32);
+  p0 += __shfl_xor(p0, 16);
+  p0 += __shfl_xor(p0, 32);
+  const double te2 = p2, te1 = te2 + p1, te = te1 + p0;
+  const double ef1 = te != 0 ? (te - te1) / te : 0.0;
+  const double ef2 = te1 != 0 ? (te1 - te2) / te1 : 0.0;
+  const double energy = ef1 > ef2 ? ef1 : ef2;
+  auto hg_alpha = [&]() {
+    return mlp::hg_clip(1.0 / (1.0 + exp(-hp.parameter_s / hp.threshold * (energy - hp.threshold))), hp.alpha_min,
+                        hp.alpha_max);
+  };
+  if constexpr (!FEAT) {
+    const double a_hg = hg_alpha();
+    if (valid && j == 0) {
+      alpha_raw[e] = a_hg;
+      if (alpha_hg_out) alpha_hg_out[e] = a_hg;
+    }
+  } else {
+    const bool net = img != nullptr;
+    // features: nodal values centred by the element mean and divided by the
+    // largest deviation (+ 1e-7 |mean|); the HG logits of the two energy
+    // ratios / 8 in [-2, 2]; log10 of the smoothest-modes share / 8 in [-2, 0]
+    double s = q[0] + q[1] + q[2] + q[3];
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    const double mean = s / 16;
+    double dev = 0;
+#pragma unroll
+    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(q[i] - mean));
+    dev = fmax(dev, __shfl_xor(dev, 16));
+    dev = fmax(dev, __shfl_xor(dev, 32));
+    const double scale = dev + 1e-7 * fabs(mean) + 1e-300;
+    float f[8];
+#pragma unroll
+    for (int i = 0; i < n; ++i) f[i] = float((q[i] - mean) / scale);
+    const double k = hp.parameter_s / hp.threshold;
+    const double smooth = te > 0 ? te2 / te : 1.0;
+    f[4] = float(fmin(2.0, fmax(-2.0, k * (ef1 - hp.threshold) / 8)));
+    f[5] = float(fmin(2.0, fmax(-2.0, k * (ef2 - hp.threshold) / 8)));
+    f[6] = fminf(0.f, fmaxf(-2.f, log10f(float(smooth) + 1e-16f) / 8));
+    f[7] = 0.f;
+    double a = 0.0;
+    bool need_hg = true;  // the HG factor is the result (no network) or the fallback
+    if (net) {
+      mlp::Tile t;
+      t.xb[0] = mlp::pack4(f[0], f[1], f[2], f[3]);
+      t.xb[1] = j == 0 ? mlp::pack4(f[4], f[5], f[6], 0.f) : mlp::pack4(0.f, 0.f, 0.f, 0.f);
+      mlp::forward_layers(*img, t);
+      // the training box: every feature of the element inside [lo, hi]
+      bool in_box = true;
+#pragma unroll
+      for (int i = 0; i < n; ++i) in_box = in_box && f[i] >= lo[4 * j + i] && f[i] <= hi[4 * j + i];
+      if (j == 0)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) in_box = in_box && f[4 + i] >= lo[16 + i] && f[4 + i] <= hi[16 + i];
+      const unsigned long long out = __builtin_amdgcn_ballot_w64(!in_box);
+      const bool element_in_box = ((out >> el) & 0x0001000100010001ull) == 0;
+      const bool use_nn = !fallback || (element_in_box && isfinite(t.y));
+      if (use_nn) a = mlp::hg_clip(hp.alpha_max * (double)t.y, hp.alpha_min, hp.alpha_max);
+      need_hg = !use_nn;
+    }
+    const double a_hg = (need_hg || alpha_hg_out) ? hg_alpha() : 0.0;
+    if (need_hg) a = a_hg;
+    if (valid) {
+      if (j == 0) {
+        if (alpha_raw) alpha_raw[e] = a;
+        if (alpha_hg_out) alpha_hg_out[e] = a_hg;
+      }
+      if (feat_out) {
+        float *fo = feat_out + (size_t)e * kMlpFeatureStride;
+        *reinterpret_cast<float4 *>(fo + 4 * j) = make_float4(f[0], f[1], f[2], f[3]);
+        *reinterpret_cast<float4 *>(fo + 16 + 4 * j) =
+            j == 0 ? make_float4(f[4], f[5], f[6], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+  }
+}
+
+}  // namespace trixi_hip

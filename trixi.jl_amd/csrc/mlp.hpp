@@ -115,18 +115,12 @@ struct Tile {
   float y;
 };
 
-__device__ __forceinline__ void forward_tile(const float *__restrict__ x, int n, int tile, const WeightsLDS &s,
-                                             Tile &t) {
+// The layers of the forward pass with X^T already in t.xb (B-operand layout:
+// lane l holds element l % 16, features kb * 16 + 4 (l / 16) + j). W is any
+// struct with the BF16 images w1 [NH][KF], w2 [NH][NH] and FP32 b1, b2, w3, b3.
+template <typename W> __device__ __forceinline__ void forward_layers(const W &s, Tile &t) {
   const int l = threadIdx.x & 63;
-  const int e = tile * 16 + (l & 15);
   const int g = l >> 4;
-  // X^T as the B operand: element e, features kb*16 + 4g + j
-#pragma unroll
-  for (int kb = 0; kb < 2; ++kb) {
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (e < n) v = *reinterpret_cast<const float4 *>(x + (size_t)e * KF + kb * 16 + 4 * g);
-    t.xb[kb] = pack4(v.x, v.y, v.z, v.w);
-  }
   // layer 1: Z1^T = W1 X^T + b1
 #pragma unroll
   for (int hb = 0; hb < 4; ++hb) {
@@ -162,6 +156,50 @@ __device__ __forceinline__ void forward_tile(const float *__restrict__ x, int n,
   part += __shfl_xor(part, 32);
   const float o = part + s.b3;
   t.y = 1.f / (1.f + __expf(-o));
+}
+
+__device__ __forceinline__ void forward_tile(const float *__restrict__ x, int n, int tile, const WeightsLDS &s,
+                                             Tile &t) {
+  const int l = threadIdx.x & 63;
+  const int e = tile * 16 + (l & 15);
+  const int g = l >> 4;
+  // X^T as the B operand: element e, features kb*16 + 4g + j
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (e < n) v = *reinterpret_cast<const float4 *>(x + (size_t)e * KF + kb * 16 + 4 * g);
+    t.xb[kb] = pack4(v.x, v.y, v.z, v.w);
+  }
+  forward_layers(s, t);
+}
+
+// The network as inference reads it: BF16 images of W1 (features padded with
+// zeros) and W2, FP32 biases and output weights. Kept in device memory
+// (refreshed whenever the parameters change) and copied into LDS with
+// 16-byte loads, so that a launch converts nothing.
+struct InferImage {
+  uint16_t w1[NH * KF];
+  uint16_t w2[NH * NH];
+  float b1[NH], b2[NH], w3[NH], b3, pad[3];
+};
+static_assert(sizeof(InferImage) % 16 == 0, "16-byte copies");
+
+// FP32 parameters -> InferImage (one workgroup).
+__global__ __launch_bounds__(256) void mlp_pack_kernel(const float *__restrict__ w, InferImage *__restrict__ img) {
+  for (int i = threadIdx.x; i < NH * KF; i += blockDim.x) {
+    const int h = i / KF, f = i - h * KF;
+    img->w1[i] = f < NF ? to_bf16(w[OFF_W1 + h * NF + f]) : 0;
+  }
+  for (int i = threadIdx.x; i < NH * NH; i += blockDim.x) img->w2[i] = to_bf16(w[OFF_W2 + i]);
+  for (int i = threadIdx.x; i < NH; i += blockDim.x) {
+    img->b1[i] = w[OFF_B1 + i];
+    img->b2[i] = w[OFF_B2 + i];
+    img->w3[i] = w[OFF_W3 + i];
+  }
+  if (threadIdx.x == 0) {
+    img->b3 = w[OFF_B3];
+    img->pad[0] = img->pad[1] = img->pad[2] = 0.f;
+  }
 }
 
 // The clipping of calc_indicator_hennemann_gassner! (indicators_2d.jl:86-99)

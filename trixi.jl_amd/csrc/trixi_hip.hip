@@ -18,6 +18,7 @@
 #include "../../include/trixi_hip.h"
 #include "dg_aux.hpp"
 #include "elem_registry.hpp"
+#include "indicator2d.hpp"
 #include "mlp.hpp"
 
 using namespace trixi_hip;
@@ -167,6 +168,7 @@ struct trixi_hip_ctx {
   // learned indicator (2D p3): d_mlp = weights [NP] | Adam m [NP] | Adam v [NP]
   // | feature box lo [NF] | hi [NF] | gradient + loss [NPL]
   float *d_mlp = nullptr, *d_feat = nullptr, *d_mlp_partial = nullptr;
+  mlp::InferImage *d_mlp_img = nullptr;  // BF16 inference image of the parameters
   double *d_alpha_hg = nullptr;
   bool mlp_ready = false;
   int mlp_step = 0;
@@ -304,11 +306,32 @@ int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
   return TRIXI_HIP_OK;
 }
 
-// HG indicator of state u: raw alpha into alpha_out; with feat != null (2D p3)
-// also the learned indicator's features.
+// Raw (unsmoothed) blending factors of state u into alpha_out: the HG factor,
+// or (2D p3, net = true) the learned indicator's, with the HG fallback. With
+// feat != null (2D p3) the learned indicator's features [E][32] too, and with
+// alpha_hg != null the HG factor. 2D p3 runs indicator_2d_p3_kernel, the other
+// (ndims, polydeg) hg_indicator_kernel.
 template <typename T>
-int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat, const HGParams *hp = nullptr) {
+int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat, const HGParams *hp = nullptr,
+              bool net = false, double *alpha_hg = nullptr) {
   const HGParams params = hp ? *hp : ctx->hg;
+  if (ctx->D == 2 && ctx->N == 3) {
+    Vinv4 vi{};
+    for (int i = 0; i < 16; ++i) vi.v[i] = ctx->Vinv[i];
+    const int nb = (ctx->E + 15) / 16;  // one wavefront of 16 elements per workgroup
+    const float *lo = net ? ctx->d_mlp + 3 * mlp::NP : nullptr, *hi = net ? lo + mlp::NF : nullptr;
+    const int fallback = ctx->ind_mode == TRIXI_HIP_IND_MLP_WITH_FALLBACK ? 1 : 0;
+    if (feat || net)
+      hipLaunchKernelGGL((indicator_2d_p3_kernel<T, true>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, params,
+                         vi, net ? ctx->d_mlp_img : nullptr, lo, hi, fallback, alpha_out, alpha_hg, feat);
+    else
+      hipLaunchKernelGGL((indicator_2d_p3_kernel<T, false>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, params,
+                         vi, (const mlp::InferImage *)nullptr, (const float *)nullptr, (const float *)nullptr, 0,
+                         alpha_out, alpha_hg, (float *)nullptr);
+    HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+  }
+  if (feat || net) return set_err(ctx, TRIXI_HIP_EINVAL, "the learned indicator is defined for 2D polydeg 3");
   bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
     constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
     constexpr int NN = ipow_c(N + 1, D);
@@ -317,10 +340,11 @@ int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat, co
     for (int i = 0; i < (N + 1) * (N + 1); ++i) vc.Vinv[i] = ctx->Vinv[i];
     int nb = (ctx->E + IPB - 1) / IPB;
     hipLaunchKernelGGL((hg_indicator_kernel<T, D, N, IPB, 256>), dim3(nb), dim3(256), 0, ctx->stream, u,
-                       alpha_out, ctx->E, params, vc, feat);
+                       alpha_out, ctx->E, params, vc);
   });
   if (!ok) return set_err(ctx, TRIXI_HIP_EINVAL, "unsupported (ndims, polydeg) for the indicator");
   HIP_TRY(hipGetLastError());
+  if (alpha_hg) HIP_TRY(hipMemcpyAsync(alpha_hg, alpha_out, (size_t)ctx->E * 8, hipMemcpyDeviceToDevice, ctx->stream));
   return TRIXI_HIP_OK;
 }
 
@@ -332,17 +356,9 @@ int mlp_blocks(int n) {
 
 // Raw (unsmoothed) blending factors of state u into d_alpha_raw, from the
 // indicator selected by ind_mode.
+// (features, network and fallback in one launch: indicator_2d_p3_kernel).
 template <typename T> int launch_indicator_raw(trixi_hip_ctx *ctx, const T *u) {
-  if (ctx->ind_mode == TRIXI_HIP_IND_HG) return launch_hg<T>(ctx, u, ctx->d_alpha_raw, nullptr);
-  int r = launch_hg<T>(ctx, u, ctx->d_alpha_hg, ctx->d_feat);
-  if (r) return r;
-  const float *lo = ctx->d_mlp + 3 * mlp::NP, *hi = lo + mlp::NF;
-  hipLaunchKernelGGL(mlp::mlp_infer_kernel, dim3(mlp_blocks(ctx->E)), dim3(mlp::NTHR), 0, ctx->stream, ctx->d_mlp,
-                     ctx->d_feat, lo, hi, ctx->d_alpha_hg, ctx->cfg.alpha_max, ctx->cfg.alpha_min,
-                     ctx->ind_mode == TRIXI_HIP_IND_MLP_WITH_FALLBACK ? 1 : 0, ctx->E, ctx->d_alpha_raw,
-                     (float *)nullptr);
-  HIP_TRY(hipGetLastError());
-  return TRIXI_HIP_OK;
+  return launch_hg<T>(ctx, u, ctx->d_alpha_raw, nullptr, nullptr, ctx->ind_mode != TRIXI_HIP_IND_HG);
 }
 
 template <typename T>
@@ -785,7 +801,8 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
                   ctx->d_u32b[0], ctx->d_u32b[1], ctx->d_du,
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
                   ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
-                  ctx->d_alpha_hg, ctx->d_records, ctx->d_bx, ctx->d_ghost, ctx->d_mort, ctx->d_mflux,
+                  ctx->d_alpha_hg, ctx->d_mlp_img, ctx->d_records, ctx->d_bx, ctx->d_ghost, ctx->d_mort,
+                  ctx->d_mflux,
                   ctx->d_adj_ptr, ctx->d_adj};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1225,6 +1242,14 @@ static int mlp_check(trixi_hip_ctx *ctx) {
   HIP_TRY(dev_alloc(ctx, &ctx->d_feat, (size_t)ctx->E * kMlpFeatureStride * 4));
   HIP_TRY(dev_alloc(ctx, &ctx->d_alpha_hg, (size_t)ctx->E * 8));
   HIP_TRY(dev_alloc(ctx, &ctx->d_mlp_partial, (size_t)kMlpMaxWaves * mlp::NPL * 4));
+  HIP_TRY(dev_alloc(ctx, &ctx->d_mlp_img, sizeof(mlp::InferImage)));
+  return TRIXI_HIP_OK;
+}
+
+// Refresh the BF16 inference image after the parameters changed.
+static int mlp_pack(trixi_hip_ctx *ctx) {
+  hipLaunchKernelGGL(mlp::mlp_pack_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_mlp, ctx->d_mlp_img);
+  HIP_TRY(hipGetLastError());
   return TRIXI_HIP_OK;
 }
 
@@ -1240,6 +1265,7 @@ int trixi_hip_mlp_set_params(trixi_hip_ctx *ctx, const float *params, const floa
   std::memcpy(h.data() + 3 * mlp::NP, feature_lo, mlp::NF * 4);
   std::memcpy(h.data() + 3 * mlp::NP + mlp::NF, feature_hi, mlp::NF * 4);
   HIP_TRY(hipMemcpyAsync(ctx->d_mlp, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  if ((r = mlp_pack(ctx))) return r;
   HIP_TRY(host_sync(ctx));
   ctx->mlp_ready = true;
   ctx->mlp_step = 0;
@@ -1273,7 +1299,7 @@ int trixi_hip_mlp_features(trixi_hip_ctx *ctx, float *features_host, double *alp
   int r = mlp_check(ctx);
   if (r) return r;
   r = ctx->fp32 ? launch_hg<float>(ctx, cur_u32(ctx), ctx->d_alpha_hg, ctx->d_feat)
-                : launch_hg<double>(ctx, cur_u64(ctx), ctx->d_alpha_hg, ctx->d_feat);
+                : launch_hg<double>(ctx, cur_u64(ctx), ctx->d_alpha_hg, ctx->d_feat);  // no network: HG factors
   if (r) return r;
   if (features_host)
     HIP_TRY(hipMemcpyAsync(features_host, ctx->d_feat, (size_t)ctx->E * kMlpFeatureStride * 4,
@@ -1332,6 +1358,7 @@ int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float
                      ctx->d_mlp + mlp::NP, ctx->d_mlp + 2 * mlp::NP, mlp_grad_buffer(ctx), learning_rate, 0.9f,
                      0.999f, 1e-8f, ctx->mlp_step);
   HIP_TRY(hipGetLastError());
+  if ((r = mlp_pack(ctx))) return r;
   if (loss) {
     HIP_TRY(hipMemcpyAsync(loss, mlp_grad_buffer(ctx) + mlp::NP, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(host_sync(ctx));

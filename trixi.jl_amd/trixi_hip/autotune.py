@@ -127,7 +127,9 @@ def main(argv=None):
 
 
 def summary_markdown(rows, repetitions):
-    out = ["# Element-kernel tile sweep (`python -m trixi_hip.autotune`)", "",
+    cmd = "python3 -m trixi_hip.autotune " + " ".join(sys.argv[1:])
+    out = ["# Element-kernel tile sweep (`python -m trixi_hip.autotune`)", "", "Command (repository root, "
+           "trixi.jl_amd on PYTHONPATH, one MI355X):", "```", cmd, "```", "",
            f"Each candidate: 3 rounds of {repetitions} launches on the workload's resident state, best round, "
            "HIP events on the context's stream. R = RHS kernel (`trixi_hip_rhs_native`), S = fused 2N-stage "
            "kernel (first stage with a = b = 0, which leaves the state unchanged). Tile = elements per "

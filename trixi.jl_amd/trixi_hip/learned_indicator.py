@@ -225,6 +225,19 @@ def dataset_from_files(semi, files, alpha_max=0.002):
     return np.concatenate(X), np.concatenate(Y)
 
 
+def training_targets(alpha_hg, alpha_max=0.002, alpha_min=0.0001, dead_zone=1.5):
+    """Network targets from HG blending factors: alpha / alpha_max, and 0 where
+    alpha < dead_zone * alpha_min. With the KHI parameters HG never reaches its
+    own clipping (its sigmoid is alpha_min at zero modal energy,
+    indicators.jl:130-135 and indicators_2d.jl:80-99), so every element is
+    blended, most of them at alpha ~ alpha_min where the FV part weighs 1e-4.
+    The dead zone maps that floor to 0: the network learns pure DG there, and
+    its clipped output (0 below alpha_min) lets the element kernel skip the FV
+    subcells of those elements. dead_zone = 1 keeps HG's targets unchanged."""
+    a = np.asarray(alpha_hg, dtype=np.float64)
+    return np.where(a < dead_zone * alpha_min, 0.0, a / alpha_max).astype(np.float32)
+
+
 def train(semi, X, Y, steps=4000, batch=8192, lr=2e-3, seed=0, log=None, log_every=500):
     """Adam on the GPU from init_params(seed); minibatches drawn with
     numpy.random.default_rng(seed). Returns the trained IndicatorNeuralNetwork
@@ -471,6 +484,8 @@ def main(argv=None):
     p.add_argument("--steps", type=int, default=4000)
     p.add_argument("--batch", type=int, default=8192)
     p.add_argument("--lr", type=float, default=2e-3)
+    p.add_argument("--dead-zone", type=float, default=1.5,
+                   help="targets are 0 where the HG factor is below this multiple of alpha_min")
     p.add_argument("--evaluate", action="store_true", help="evaluate the weights at --out instead of training")
     p.add_argument("--config5", action="store_true",
                    help="judge the weights at --out by the HG run's density error (evaluate_config5)")
@@ -489,10 +504,12 @@ def main(argv=None):
         return
 
     X, Y = collect_khi_data(range(a.seeds), a.level, a.t_end, a.snapshots, log=log)
-    log(f"{X.shape[0]} samples, {np.mean(Y > 0):.3f} with alpha > 0")
+    Y = training_targets(Y * 0.002, dead_zone=a.dead_zone)
+    log(f"{X.shape[0]} samples, {np.mean(Y > 0):.3f} with target > 0 (dead zone {a.dead_zone} alpha_min)")
     semi = khi_semidiscretization(a.level, 1e-3, 0)
     net, hist = train(semi, X, Y, a.steps, a.batch, a.lr, log=log)
-    net.save(a.out, loss_history=np.array(hist), seeds=np.arange(a.seeds), level=a.level, t_end=a.t_end)
+    net.save(a.out, loss_history=np.array(hist), seeds=np.arange(a.seeds), level=a.level, t_end=a.t_end,
+             snapshots=a.snapshots, dead_zone=a.dead_zone, steps=a.steps, batch=a.batch, lr=a.lr)
     print(json.dumps({"out": a.out, "samples": int(X.shape[0]), "final_loss": hist[-1][1]}))
 
 
