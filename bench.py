@@ -99,7 +99,7 @@ def step_time(th, semi, steps):
     the initial state: microseconds per step, and the tiles used."""
     ode = th.semidiscretize(semi, (0.0, 1e9))
     th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=1.0)),
-             maxiters=20, save_final=False)  # graph capture
+             maxiters=min(20, steps), save_final=False)  # graph capture
     sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=1.0)),
                    maxiters=steps, save_final=False)
     return dict(us_per_step=1e6 * sol.wall_time / sol.nsteps, steps=sol.nsteps, stage_tile=semi.stage_tile_used())
@@ -125,7 +125,10 @@ def config_c3(th, torch, steps, warmup, with_steps=True):
                          roofline=roofline_line(nd, 5, 4 if prec == "fp32" else 8, r["kernel_avg_ms"]),
                          finite=r["finite"])
         if with_steps:
-            out[prec]["step"] = step_time(th, semi, 100)
+            # the seeded velocity field steepens into shocks near t = 0.1, where
+            # the plain flux-differencing solver fails (the oracle too: NaN at
+            # step 19 on the level-3 mesh); 30 steps reach t = 0.05
+            out[prec]["step"] = step_time(th, semi, 30)
         semi.close()
     out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
     out["ndofs"] = nd
@@ -494,6 +497,12 @@ def main():
         print(f"[bench] c3: {configs['c3_3d_p4_32cubed']}", file=sys.stderr, flush=True)
         configs["c4_c5_khi_128sq"] = config_c4_c5(th, torch, args.steps)
         print(f"[bench] c4/c5: {configs['c4_c5_khi_128sq']}", file=sys.stderr, flush=True)
+        from trixi_hip.learned_indicator import IndicatorNeuralNetwork, evaluate_config5
+        # the learned indicator judged by the HG run's density error at t = 3
+        # (seed 1000, not a training seed; reference: HG on the level-8 mesh)
+        configs["c5_vs_hg_level7_t3"] = evaluate_config5(IndicatorNeuralNetwork.load(), seed=1000, level=7,
+                                                         t_end=3.0, ref_level=8)
+        print(f"[bench] c5 vs hg at t = 3: {configs['c5_vs_hg_level7_t3']}", file=sys.stderr, flush=True)
 
     ndofs = semis["fp32"].ndofs()
     nv = semis["fp32"].nvariables()

@@ -65,9 +65,16 @@ template <> struct Prec<double> {
 __device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
-template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
-template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
+// v_max / v_min (one instruction; a compare-and-select takes two plus a
+// hazard wait). Differs from a > b ? a : b only for NaN operands, where the
+// non-NaN one is returned (the select returned b).
+__device__ __forceinline__ float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float tmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double tmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double tmin(double a, double b) { return __builtin_fmin(a, b); }
+// |a| as an operand modifier of the consuming instruction
+__device__ __forceinline__ float tabs(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double tabs(double a) { return __builtin_fabs(a); }
 
 // ---------------------------------------------------------------------------
 // Logarithmic means with one reciprocal each.

@@ -124,6 +124,13 @@ template <int D, int N, int EPB> struct Layout {
       return q + (q >> 5);
     }
   }
+  // slot of node q0 + i n^O of a line that starts at q0 (index 0 along O),
+  // given p0 = phys(q0): in the swizzled layout only the O-th index field
+  // changes, from 0 ^ el to i ^ el
+  template <int O> static __device__ __forceinline__ int phys_line(int p0, int q0, int i) {
+    if constexpr (SWZ) return p0 ^ (i << (2 * O));
+    else return phys(q0 + i * ipow_c(n, O));
+  }
   // line = r0 + n*r1 with (r0, r1) the two free indices in increasing dimension order
   template <int O> static __device__ __forceinline__ void lane_line(int ll, int &el, int &line) {
     if constexpr (SWZ) {
@@ -256,8 +263,9 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
   NodeR<T, D> wend[2];
 
   int pq[n];
+  pq[0] = LY::phys(q0);
 #pragma unroll
-  for (int i = 0; i < n; ++i) pq[i] = LY::phys(q0 + i * st);
+  for (int i = 1; i < n; ++i) pq[i] = LY::template phys_line<O>(pq[0], q0, i);
 
   T acc[n][V];
 #pragma unroll
@@ -608,8 +616,11 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
   C rc[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) rc[r] = csrc[min(tid + r * NT, nch - 1)];
-  // element of this thread's source-table entry (threads beyond the table repeat the last one)
-  const int sel = min(tid / (2 * n), ne - 1);
+  // element of this thread's source-table entry: the table is built by the
+  // last EPB * 2n threads, away from wave 0, which has the most node work of
+  // phases A and C (threads outside the table repeat the nearest entry)
+  const int ts = tid - (NT - EPB * 2 * n);
+  const int sel = min(max(ts, 0) / (2 * n), ne - 1);
   const double ij = a.inv_jac[e0 + sel];
   double cen[D];
 #pragma unroll
@@ -648,8 +659,8 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
       if (a.alpha_out) a.alpha_out[e] = al;
     }
   }
-  if (tid < ne * 2 * n) {
-    const int idx = tid - sel * 2 * n;
+  if (ts >= 0 && ts < ne * 2 * n) {
+    const int idx = ts - sel * 2 * n;
     if (idx == 0) s_fac[sel] = T(-ij);
     if (a.source) {
       double csum = 0;

@@ -16,9 +16,8 @@
 // network's B operand (features 4 j .. 4 j + 3), and the lane with j = 0 holds
 // the three energy features: the features never leave registers. The network
 // image is read from device memory (L2) where the MFMA operands need it, so
-// that the workgroup has no barrier. Arithmetic up to the features is FP64, as
-// in hg_indicator_kernel; the sums over the four rows run in a different
-// order than the reference's loops (round-off of the energies only).
+// that the workgroup has no barrier. Arithmetic up to the features is FP64 and
+// in the operation order of hg_indicator_kernel (the same factors).
 #pragma once
 #include "dg_aux.hpp"
 #include "mlp.hpp"
@@ -42,6 +41,7 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
     double *__restrict__ alpha_hg_out, float *__restrict__ feat_out) {
   constexpr int n = 4, V = 4;
   __shared__ double s_mx[16 * 16];  // x-transformed rows [element][row j][mode i]
+  __shared__ double s_sq[16 * 16];  // squared modes [element][a + 4 j]
   const int l = threadIdx.x;
   const int el = l & 15, j = l >> 4;
   const int e0 = blockIdx.x * 16;
@@ -79,26 +79,31 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
     for (int b = 0; b < n; ++b) r += vi.v[j * n + b] * mx[4 * b + a];
     m[a] = r;
   }
-  // partial energies of this row j: te2 (a, j < 2), te1 adds (a < 3, j = 2)
-  // and (a = 2, j < 2), te adds (a, j = 3) and (a = 3, j < 3)
-  double p2 = 0, p1 = 0, p0 = 0;
-  if (j < 2) {
-    p2 = m[0] * m[0] + m[1] * m[1];
-    p1 = m[2] * m[2];
-    p0 = m[3] * m[3];
-  } else if (j == 2) {
-    p1 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
-    p0 = m[3] * m[3];
-  } else {
-    p0 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2] + m[3] * m[3];
-  }
-  p2 += __shfl_xor(p2, 16);
-  p2 += __shfl_xor(p2, 32);
-  p1 += __shfl_xor(p1, 16);
-  p1 += __shfl_xor(p1, 32);
-  p0 += __shfl_xor(p0, 16);
-  p0 += __shfl_xor(p0, 32);
-  const double te2 = p2, te1 = te2 + p1, te = te1 + p0;
+  // squared modes (a, j) of the element to LDS; every lane of the element
+  // then sums them in the loop order of indicators_2d.jl:59-75 (te2 over
+  // j < 2, i < 2; te1 adds (i < 3, j = 2), then (i = 2, j < 2); te adds
+  // (i, j = 3), then (i = 3, j < 3)), so that the energies and the factor are
+  // those of hg_indicator_kernel bit for bit
+  double *sq = &s_sq[el * 16];
+#pragma unroll
+  for (int a = 0; a < n; ++a) sq[a + 4 * j] = m[a] * m[a];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  double te2 = 0;
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) te2 += sq[i + 4 * jj];
+  double te1 = te2;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) te1 += sq[i + 4 * 2];
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj) te1 += sq[2 + 4 * jj];
+  double te = te1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) te += sq[i + 4 * 3];
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj) te += sq[3 + 4 * jj];
   const double ef1 = te != 0 ? (te - te1) / te : 0.0;
   const double ef2 = te1 != 0 ? (te1 - te2) / te1 : 0.0;
   const double energy = ef1 > ef2 ? ef1 : ef2;
