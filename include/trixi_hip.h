@@ -224,6 +224,22 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
  * (calculate_dt, stepsize.jl:146-155). FP64 reduction. */
 int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt);
 
+/* No counterpart in the reference. trixi_hip_max_dt for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops):
+ * no context, no hipMalloc/hipFree, no host synchronisation. Inputs are the
+ * FP64 state u_dev [V, (polydeg+1)^ndims, nelements] and inverse_jacobian_dev
+ * [nelements] in device memory; cfl * max_dt is written to dt_dev[0] on
+ * hip_stream (NULL: the null stream), enqueued only. The partial maxima go to
+ * a fixed device array of 8192 doubles (64 KiB) that each GPU holds until the
+ * process ends; larger meshes are reduced with several element groups per
+ * workgroup, so every mesh size fits. That array is shared by all calls on one
+ * GPU: calls that overlap on one GPU (other streams, other host threads) race.
+ * Same kernels and result as trixi_hip_max_dt. Errors: TRIXI_HIP_EINVAL for
+ * null pointers, nelements outside [1, 2^31 - 257] or an unsupported
+ * (ndims, polydeg); trixi_hip_last_error(NULL) describes the failure. */
+int trixi_hip_max_dt_noalloc(int ndims, int polydeg, int nelements, double gamma, const double *u_dev,
+                             const double *inverse_jacobian_dev, double cfl, double *dt_dev, void *hip_stream);
+
 /* Blending factors alpha[E] of IndicatorHennemannGassner (indicators.jl:119-156,
  * indicators_{2,3}d.jl) for the resident state, after smoothing. */
 int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host);

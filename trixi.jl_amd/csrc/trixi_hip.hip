@@ -205,6 +205,15 @@ int set_err(trixi_hip_ctx *c, int code, const std::string &msg) {
   return code;
 }
 
+// message of the last failed context-free call of this thread (fixed buffer:
+// those calls allocate nothing, host memory included)
+thread_local char g_free_err[256] = "";
+
+int free_err(int code, const char *msg, const char *detail = nullptr) {
+  std::snprintf(g_free_err, sizeof g_free_err, "%s%s%s", msg, detail ? ": " : "", detail ? detail : "");
+  return code;
+}
+
 // the current resident state: FP64 register u, FP32 shadow (FP32 mode), native
 double *cur_u64(trixi_hip_ctx *c) { return c->d_ub[c->fp32 ? 0 : c->par]; }
 float *cur_u32(trixi_hip_ctx *c) { return c->d_u32b[c->par]; }
@@ -811,7 +820,10 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
   return TRIXI_HIP_OK;
 }
 
-const char *trixi_hip_last_error(const trixi_hip_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *trixi_hip_last_error(const trixi_hip_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_free_err[0] ? g_free_err : "null context";
+}
 
 int trixi_hip_set_stream(trixi_hip_ctx *ctx, void *hip_stream) {
   if (!ctx) return TRIXI_HIP_EINVAL;
@@ -1079,6 +1091,77 @@ int trixi_hip_max_dt(trixi_hip_ctx *ctx, double cfl, double *dt) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dt, ctx->d_scalars + 2, 8, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(host_sync(ctx));
+  return TRIXI_HIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scratch of trixi_hip_max_dt_noalloc: the partial maxima of the wave-speed
+// reduction, one per workgroup (64 KiB). Every device the library runs on
+// holds its own copy until the process ends. In host code `scratch` names a
+// host-side stand-in, never the device memory: kernels get the per-device
+// address that hipGetSymbolAddress returns (scratch_on_current_device).
+constexpr int kNoAllocPartials = 8192;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_on_current_device(double **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return free_err(TRIXI_HIP_EHIP, "hipGetDevice failed", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return free_err(TRIXI_HIP_EINVAL, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return free_err(TRIXI_HIP_EHIP, "hipGetSymbolAddress(scratch) failed", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return TRIXI_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trixi_hip_max_dt_noalloc(int ndims, int polydeg, int nelements, double gamma, const double *u_dev,
+                             const double *inverse_jacobian_dev, double cfl, double *dt_dev, void *hip_stream) {
+  if (!u_dev || !inverse_jacobian_dev || !dt_dev)
+    return free_err(TRIXI_HIP_EINVAL, "max_dt_noalloc: null device pointer");
+  if (nelements <= 0 || nelements > INT_MAX - 256)
+    return free_err(TRIXI_HIP_EINVAL, "max_dt_noalloc: nelements must be in [1, 2^31 - 257]");
+  if (!dispatch_dn(ndims, polydeg, [](auto, auto) {}))
+    return free_err(TRIXI_HIP_EINVAL, "max_dt_noalloc: ndims must be 2 or 3 and polydeg 2, 3 or 4");
+  double *partial = nullptr;
+  int r = scratch_on_current_device(&partial);
+  if (r) return r;
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  int nb = -1, n = polydeg + 1;
+  dispatch_dn(ndims, polydeg, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    constexpr int NN = ipow_c(N + 1, D);
+    constexpr int EB = NN >= 256 ? 1 : 256 / NN;
+    // more element groups than partials: each workgroup walks several groups
+    nb = std::min((nelements + EB - 1) / EB, kNoAllocPartials);
+    hipLaunchKernelGGL((max_speed_nodes_kernel<double, D, N>), dim3(nb), dim3(256), 0, s, u_dev,
+                       inverse_jacobian_dev, nelements, gamma, partial);
+  });
+  if (nb <= 0 || nb > kNoAllocPartials)
+    return free_err(TRIXI_HIP_EINVAL, "max_dt_noalloc: more partials than the 8192 of the scratch buffer");
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return free_err(TRIXI_HIP_EHIP, "max_dt_noalloc: launch failed", hipGetErrorString(e));
+  // reads exactly the nb partials written above: nothing of an earlier call is used
+  hipLaunchKernelGGL(max_dt_finalize_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, nb, n, cfl, dt_dev);
+  e = hipGetLastError();
+  if (e != hipSuccess) return free_err(TRIXI_HIP_EHIP, "max_dt_noalloc: launch failed", hipGetErrorString(e));
+  g_free_err[0] = '\0';
   return TRIXI_HIP_OK;
 }
 

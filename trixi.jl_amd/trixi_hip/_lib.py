@@ -141,6 +141,7 @@ SIGNATURES = [
     ("trixi_hip_rk_step", _I, [_P, _PD, _PD, _PD, _I, _D, _D, _I]),
     ("trixi_hip_solve", _I, [_P, _PD, _PD, _PD, _I, _D, _D, _D, _I, _I, _PD, _PI]),
     ("trixi_hip_max_dt", _I, [_P, _D, _PD]),
+    ("trixi_hip_max_dt_noalloc", _I, [_I, _I, _I, _D, _P, _P, _D, _P, _P]),
     ("trixi_hip_indicator", _I, [_P, _I, _P]),
     ("trixi_hip_indicator_hg", _I, [_P, _D, _D, _I, _I, _P]),
     ("trixi_hip_indicator_max", _I, [_P, _I, _P]),
