@@ -1,7 +1,7 @@
 /*
  * trixi_hip.h -- C ABI of the MI355X DGSEM right-hand side for the compressible
  * Euler equations on a TreeMesh (2D and 3D): uniform or refined (L2 mortars), periodic or with
- * Dirichlet boundaries.
+ * Dirichlet or slip-wall boundaries.
  *
  * The library replaces the CPU methods that Trixi.jl dispatches from
  *   rhs_hyperbolic!(du_ode, u_ode, semi, t)
@@ -61,6 +61,9 @@ extern "C" {
 #define TRIXI_HIP_BC_DIRICHLET 1 /* BoundaryConditionDirichlet(initial_condition) on every boundary
                                     (equations.jl:159-182); the boundary value function is the
                                     one set by trixi_hip_set_exact_solution */
+#define TRIXI_HIP_BC_SLIP_WALL 2 /* boundary_condition_slip_wall on every boundary (TreeMesh methods of
+                                    compressible_euler_{2,3}d.jl:319-420): wall flux from the inner
+                                    state (Toro's pressure Riemann solution), any surface flux */
 
 /* source terms */
 #define TRIXI_HIP_SRC_NONE 0

@@ -82,6 +82,7 @@ def _load():
         L.oracle_inv_ln_mean.restype = D
         L.oracle_inv_ln_mean.argtypes = [D, D]
         L.oracle_flux.argtypes = [I, I, D, P, P, I, P]
+        L.oracle_slip_wall_flux.argtypes = [I, D, P, I, I, P]
         L.oracle_cons2entropy.argtypes = [I, D, P, P]
         _lib = L
     return _lib
@@ -99,11 +100,13 @@ class Oracle:
     box (cmin, cmax) whose leaf cells are refined once (2:1 balanced, L2
     mortars at the level changes). Non-periodic boundaries carry
     BoundaryConditionDirichlet(initial_condition) with the initial condition set
-    by set_initial_condition."""
+    by set_initial_condition (boundary_condition='dirichlet'), or
+    boundary_condition_slip_wall (boundary_condition='slip_wall')."""
 
     def __init__(self, ndims, polydeg, level, cmin, cmax, volume_integral="fd", volume_flux="ranocha",
                  surface_flux="llf", fv_flux="llf", alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
-                 source="none", gamma=1.4, periodicity=True, refinement_patches=(), hg_variable="density_pressure"):
+                 source="none", gamma=1.4, periodicity=True, refinement_patches=(), hg_variable="density_pressure",
+                 boundary_condition="dirichlet"):
         L = _load()
         c = _Config()
         c.ndims, c.polydeg, c.initial_refinement_level = ndims, polydeg, level
@@ -120,7 +123,7 @@ class Oracle:
         per = (periodicity,) * ndims if isinstance(periodicity, bool) else tuple(periodicity)
         for d in range(ndims):
             c.periodicity[d] = int(bool(per[d]))
-        c.boundary_condition = 0 if all(per) else 1
+        c.boundary_condition = 0 if all(per) else {"dirichlet": 1, "slip_wall": 2}[boundary_condition]
         if len(refinement_patches) > 8:
             raise ValueError("at most 8 refinement patches")
         c.npatches = len(refinement_patches)
@@ -393,6 +396,15 @@ def flux(ndims, kind, ul, ur, orientation, gamma=1.4):
     f = np.empty(ndims + 2)
     k = -1 if kind == "physical" else FLUX[kind]
     _load().oracle_flux(ndims, k, gamma, _p(ul), _p(ur), int(orientation), _p(f))
+    return f
+
+
+def slip_wall_flux(ndims, u, orientation, plus, gamma=1.4):
+    """boundary_condition_slip_wall flux of the inner state u at a face of the
+    given orientation (0-based); plus: the element is left of the wall."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    f = np.empty(ndims + 2)
+    _load().oracle_slip_wall_flux(ndims, gamma, _p(u), int(orientation), int(bool(plus)), _p(f))
     return f
 
 

@@ -529,7 +529,9 @@ enum { VI_WEAK = 0, VI_FLUX_DIFFERENCING = 1, VI_SHOCK_CAPTURING_HG = 2, VI_PURE
 // Source-term selector.
 enum { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1 };
 // BC_DIRICHLET_IC: BoundaryConditionDirichlet(initial_condition), equations.jl:159-182
-enum { BC_NONE = 0, BC_DIRICHLET_IC = 1 };
+// BC_SLIP_WALL: boundary_condition_slip_wall (compressible_euler_2d.jl:322-420,
+// compressible_euler_3d.jl:319-415, the TreeMesh methods)
+enum { BC_NONE = 0, BC_DIRICHLET_IC = 1, BC_SLIP_WALL = 2 };
 // Initial-condition selector.
 enum {
     IC_CONSTANT = 0,
@@ -676,7 +678,17 @@ void initial_condition(const oracle_ctx &c, const Euler<D> &eq, const double *x,
             break;
         }
         case IC_WEAK_BLAST_WAVE: {
-            // euler_3d.jl:163-183 (3D only)
+            if (D == 2) {
+                // euler_2d.jl:181-200 (polar coordinates, sincos of phi)
+                const double xn = x[0], yn = x[1];
+                const double r = std::sqrt(xn * xn + yn * yn);
+                const double phi = std::atan2(yn, xn);
+                const double w[4] = {r > 0.5 ? 1.0 : 1.1691, r > 0.5 ? 0.0 : 0.1882 * std::cos(phi),
+                                     r > 0.5 ? 0.0 : 0.1882 * std::sin(phi), r > 0.5 ? 1.0 : 1.245};
+                eq.prim2cons(w, u);
+                break;
+            }
+            // euler_3d.jl:163-183
             double xn = x[0], yn = x[1], zn = D == 3 ? x[2] : 0.0;
             double r = std::sqrt(xn * xn + yn * yn + zn * zn);
             double phi = std::atan2(yn, xn);
@@ -833,6 +845,47 @@ template <int D> inline int face_to_node(int n, int o, int side_index, int fnode
         else idx[d] = rest[r++];
     }
     return node_index<D>(n, idx);
+}
+
+// boundary_condition_slip_wall on a TreeMesh face of orientation o
+// (compressible_euler_2d.jl:322-420, compressible_euler_3d.jl:319-415): the
+// inner state is rotated to the outward normal n = +-e_o (rotate_to_x with the
+// tangents of the reference: 3D o = x: (e_z, e_y), o = y: (e_x, e_z), o = z:
+// (e_y, e_x) up to sign; 2D o = x: e_y, o = y: e_x), the wall pressure p* of
+// the pressure Riemann problem (Toro 2009, sec. 6.3.3, clipped at 0) gives the
+// flux; only the normal momentum component is nonzero, and for either side of
+// the element it is p* in the direction of +e_o.
+template <int D, typename R> void slip_wall_flux(const Euler<D, R> &eq, const R *u, int o, bool plus, R *f) {
+    constexpr int V = D + 2;
+    const R rho = u[0];
+    const R mn = plus ? u[1 + o] : -u[1 + o];
+    R mt[2] = {R(0), R(0)};
+    if (D == 3) {
+        const int t1 = o == 0 ? 2 : (o == 1 ? 0 : 1), t2 = o == 0 ? 1 : (o == 1 ? 2 : 0);
+        mt[0] = u[1 + t1];
+        mt[1] = u[1 + t2];
+    } else {
+        mt[0] = u[1 + (1 - o)];
+    }
+    // cons2prim of the rotated state (component order n, t1, t2); the signs of
+    // the tangential momenta do not change their products m * (m / rho)
+    const R vn = mn / rho;
+    R ke2 = mn * vn;
+    for (int d = 0; d < D - 1; ++d) ke2 += mt[d] * (mt[d] / rho);
+    const R p = (eq.gamma - 1) * (u[V - 1] - R(0.5) * ke2);
+    R p_star;
+    if (vn <= 0) {
+        const R c = std::sqrt(eq.gamma * p / rho);
+        const R base = 1 + R(0.5) * (eq.gamma - 1) * vn / c;
+        if (base >= 0) p_star = p * std::pow(base, 2 * eq.gamma * eq.inv_gamma_minus_one);
+        else p_star = R(0);
+    } else {
+        const R A = 2 / ((eq.gamma + 1) * rho);
+        const R B = p * (eq.gamma - 1) / (eq.gamma + 1);
+        p_star = p + R(0.5) * vn / A * (vn + std::sqrt(vn * vn + 4 * A * (p + B)));
+    }
+    for (int v = 0; v < V; ++v) f[v] = R(0);
+    f[1 + o] = p_star;
 }
 
 // ---------------------------------------------------------------------------
@@ -1099,6 +1152,7 @@ template <int D, typename R> void rhs(oracle_ctx &c, R *du, const R *u, double t
     // BoundaryConditionDirichlet(initial_condition) (equations.jl:164-182): the
     // outer state is the initial condition at the boundary node and time t; the
     // element is the left state at its + faces (even 1-based direction).
+    // boundary_condition_slip_wall: the flux of slip_wall_flux from the inner state.
     {
         const Euler<D> eqd(c.cfg.gamma);
         const int B = (int)c.bnd_elem.size();
@@ -1108,6 +1162,12 @@ template <int D, typename R> void rhs(oracle_ctx &c, R *du, const R *u, double t
             const bool plus = dir % 2 == 1;
             for (int fn = 0; fn < nf; ++fn) {
                 const int node = face_to_node<D>(n, o, plus ? n - 1 : 0, fn);
+                if (c.cfg.boundary_condition == BC_SLIP_WALL) {
+                    R f[V];
+                    slip_wall_flux<D>(eq, &u[V * (node + NN * e)], o, plus, f);
+                    for (int v = 0; v < V; ++v) sfv[V * (fn + nf * (dir + 2 * D * e)) + v] = f[v];
+                    continue;
+                }
                 double ub[D + 2];
                 initial_condition<D>(c, eqd, &c.node_coords[D * (node + NN * e)], t, ub);
                 R ubr[V], f[V];
@@ -1580,7 +1640,9 @@ bool finalize_mesh(oracle_ctx *c) {
                 c->bnd_elem.push_back(e);
                 c->bnd_dir.push_back(dir);
             }
-    if (!c->bnd_elem.empty() && c->cfg.boundary_condition != BC_DIRICHLET_IC) return false;
+    if (!c->bnd_elem.empty() && c->cfg.boundary_condition != BC_DIRICHLET_IC &&
+        c->cfg.boundary_condition != BC_SLIP_WALL)
+        return false;
     // init_elements!: containers_3d.jl:87-128
     c->inverse_jacobian.resize(c->E);
     c->node_coords.resize((size_t)c->D * c->NN * c->E);
@@ -1952,6 +2014,13 @@ void oracle_flux(int ndims, int kind, double gamma, const double *ul, const doub
         if (kind < 0) eq.flux(ul, o, f);
         else num_flux<3>(eq, kind, ul, ur, o, f);
     }
+}
+
+// boundary_condition_slip_wall flux of the inner state u at a face of
+// orientation o (0-based); plus != 0: the element is left of the wall (+ face).
+void oracle_slip_wall_flux(int ndims, double gamma, const double *u, int o, int plus, double *f) {
+    if (ndims == 2) slip_wall_flux<2>(Euler<2>(gamma), u, o, plus != 0, f);
+    else slip_wall_flux<3>(Euler<3>(gamma), u, o, plus != 0, f);
 }
 
 void oracle_cons2entropy(int ndims, double gamma, const double *u, double *w) {

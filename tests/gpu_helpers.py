@@ -63,7 +63,12 @@ def build(spec, precision="fp64"):
                        refinement_patches=spec.get("refinement_patches", ()))
     ic = make_ic(spec["ic"], spec.get("ic_params", ()))
     src = th.source_terms_convergence_test if spec.get("source") == "convergence_test" else None
-    bc = th.boundary_condition_periodic if mesh.nboundaries == 0 else th.BoundaryConditionDirichlet(ic)
+    if mesh.nboundaries == 0:
+        bc = th.boundary_condition_periodic
+    elif spec.get("boundary_condition") == "slip_wall":
+        bc = th.boundary_condition_slip_wall
+    else:
+        bc = th.BoundaryConditionDirichlet(ic)
     semi = th.SemidiscretizationHyperbolic(mesh, eq, ic, solver, source_terms=src, boundary_conditions=bc,
                                            precision=precision, elements_per_block=spec.get("epb", 0),
                                            threads_per_block=spec.get("nt", 0))
@@ -77,7 +82,8 @@ def oracle_for(spec):
                alpha_max=spec.get("alpha_max", 0.5), alpha_min=spec.get("alpha_min", 0.001),
                source=spec.get("source", "none"), periodicity=spec.get("periodicity", True),
                refinement_patches=spec.get("refinement_patches", ()), gamma=spec.get("gamma", 1.4),
-               hg_variable=spec.get("hg_variable", "density_pressure"))
+               hg_variable=spec.get("hg_variable", "density_pressure"),
+               boundary_condition=spec.get("boundary_condition", "dirichlet"))
     if spec.get("periodicity", True) is not True:  # the boundary values come from the initial condition
         ic = make_ic(spec["ic"], spec.get("ic_params", ()))
         o.set_initial_condition(spec["ic"], tuple(ic.params), ic.modes.reshape(-1) if ic.modes is not None else None)

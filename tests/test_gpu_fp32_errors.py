@@ -99,3 +99,41 @@ def test_fp32_rhs_error_developed_khi_hg_and_learned():
     o.set_alpha(None)
     assert _lib.IND_HG == 0
     semi.close()
+
+
+@pytest.mark.parametrize("spread", [1e-6, 1e-3, 0.12, 0.5])
+def test_fp32_log_mean_series_on_near_equal_states(spread):
+    """The FP32 volume flux takes the logarithmic means of density and of
+    rho/p from a series in w = ln(y/x)/2 whenever every pair of a node line has
+    |w| <= 1/8. States whose node values of rho and p differ by factors up to
+    1 + spread drive that branch for spread <= 0.12 (w from ~1e-7 to 0.113,
+    unequal pairs): with the exact quotient (y - x) / ln(y/x) the tiny spreads
+    would lose all digits, and a wrong series coefficient shows at 0.12;
+    spread 0.5 mixes in the exact formulation (waves with a pair out of range).
+    The FP32 RHS stays within the FP32 round-off of the fluxes of the FP64
+    oracle's (flux_ranocha volume and surface fluxes, the entropy-conservative
+    pair). Bound: 60 eps32 J^-1 max|f| absolute, about 3x the measured 15-21
+    (the same at every spread: round-off of the O(1) fluxes, not of the means)."""
+    from gpu_helpers import fp32_abs_bound
+    spec = dict(ndims=2, polydeg=3, level=3, cmin=(-1, -1), cmax=(1, 1), ic="constant", volume_integral="fd",
+                surface_flux="ranocha")
+    o = oracle_for(spec)
+    semi, _ = build(spec, "fp32")
+    rng = np.random.default_rng(17)
+    npts = semi.nvalues() // 4
+    rho = 1.0 + spread * rng.uniform(0.0, 1.0, npts)
+    p = 1.0 + spread * rng.uniform(0.0, 1.0, npts)
+    v = 0.1 * rng.uniform(-1.0, 1.0, (npts, 2))
+    U = np.empty((npts, 4))
+    U[:, 0] = rho
+    U[:, 1:3] = rho[:, None] * v
+    U[:, 3] = p / 0.4 + 0.5 * rho * np.sum(v * v, axis=1)
+    u = U.reshape(-1)
+    du = rhs32(semi, u)
+    ref = o.rhs(u, 0.0)
+    err = float(np.max(np.abs(du - ref)))
+    unit = fp32_abs_bound(semi, u, K=1.0)
+    print(f"near-equal spread {spread:g}: max|du32 - du_oracle| = {err:.3e} = {err / unit:.2f} eps32 J^-1 max|f|",
+          flush=True)
+    assert err < 60 * unit
+    semi.close()

@@ -50,6 +50,21 @@ CASES = {
     "2d_p3_hg_khi_bounded": (dict(ndims=2, polydeg=3, level=4, cmin=(-1, -1), cmax=(1, 1), ic="kelvin_helmholtz",
                                   ic_params=(0.0, 0), volume_integral="hg", surface_flux="llf_naive",
                                   fv_flux="llf_naive", alpha_max=0.002, alpha_min=1e-4, periodicity=False), 2e-4),
+    # boundary_condition_slip_wall: the wall flux from the inner state
+    "2d_p3_ec_blast_slip": (dict(ndims=2, polydeg=3, level=3, cmin=(-1, -1), cmax=(1, 1), ic="weak_blast_wave",
+                                 volume_integral="fd", surface_flux="ranocha", periodicity=False,
+                                 boundary_condition="slip_wall"), 2e-5),
+    "3d_p3_fd_blast_slip": (dict(BLAST, cmin=(-1,) * 3, cmax=(1,) * 3, volume_integral="fd", surface_flux="llf",
+                                 periodicity=False, boundary_condition="slip_wall"), 1e-5),
+    "3d_p3_hg_blast_slip_x": (dict(BLAST, cmin=(-1,) * 3, cmax=(1,) * 3, volume_integral="hg", surface_flux="ranocha",
+                                   fv_flux="ranocha", periodicity=(False, True, True),
+                                   boundary_condition="slip_wall"), 1e-5),
+    "2d_p4_fd_khi_channel_slip": (dict(ndims=2, polydeg=4, level=3, cmin=(-1, -1), cmax=(1, 1), ic="kelvin_helmholtz",
+                                       ic_params=(0.0, 0), surface_flux="llf_naive", periodicity=(True, False),
+                                       boundary_condition="slip_wall"), 2e-4),
+    "3d_p2_weak_mms_slip_mortar": (dict(MMS3, polydeg=2, volume_integral="weak", periodicity=False,
+                                        boundary_condition="slip_wall",
+                                        refinement_patches=[((0.5, 0.5, 0.5), (1.5, 1.5, 1.5))]), 1e-4),
     # refined meshes (box refinement patches): L2 mortars at the level changes
     "3d_p3_weak_mms_mortar": (dict(MMS3, volume_integral="weak", surface_flux="llf_naive",
                                    refinement_patches=[((0.5, 0.5, 0.5), (1.5, 1.5, 1.5))]), 1e-4),

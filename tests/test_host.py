@@ -61,6 +61,7 @@ def test_tree_order_follows_child_sign_table():
     ("convergence_test", 3, th.initial_condition_convergence_test, ()),
     ("convergence_test", 2, th.initial_condition_convergence_test, ()),
     ("weak_blast_wave", 3, th.initial_condition_weak_blast_wave, ()),
+    ("weak_blast_wave", 2, th.initial_condition_weak_blast_wave, ()),
     ("isentropic_vortex", 2, th.initial_condition_isentropic_vortex(25.0), (25.0,)),
     ("kelvin_helmholtz", 2, th.initial_condition_kelvin_helmholtz_instability(), ()),
     ("fourier_modes", 3, th.initial_condition_fourier_modes(2), (0.1, 0.2)),

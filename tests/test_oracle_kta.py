@@ -7,7 +7,7 @@ import pytest
 from conftest import make_oracle
 
 CASES = ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d", "khi_2d",
-         "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic", "mms_3d_mortar"]
+         "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic", "mms_3d_mortar", "ec_2d_slip_wall"]
 
 
 @pytest.mark.parametrize("name", CASES)

@@ -290,3 +290,25 @@ def test_amr_random_adaptations_stay_balanced(D, periodic):
         ids, _, _ = o.mortars()
         if len(ids):
             assert np.all(lv[ids[:, :-1]] == lv[ids[:, -1:]] + 1)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_slip_walls_conserve_mass_and_energy(D):
+    """A weak blast wave reflected by boundary_condition_slip_wall on every side:
+    the wall flux has no mass or energy component, so the totals stay fixed to
+    round-off through the reflection. With Dirichlet(initial condition)
+    boundaries the same run exchanges mass with the outside."""
+    def run(bc):
+        o = make_oracle(dict(ndims=D, polydeg=3, level=3 if D == 2 else 2, cmin=(-1,) * D, cmax=(1,) * D,
+                             volume_integral="fd", volume_flux="ranocha", surface_flux="ranocha",
+                             periodicity=False, boundary_condition=bc))
+        o.set_initial_condition("weak_blast_wave")
+        u0 = o.compute_coefficients(0.0)
+        u, t, _ = o.solve(u0, 0.0, 0.8, 0.5)
+        return o.integrals(u0)[0], o.integrals(u)[0], u
+    m0, m1, u = run("slip_wall")
+    assert np.isfinite(u).all()
+    assert abs(m1[0] - m0[0]) < 1e-13 * abs(m0[0])
+    assert abs(m1[D + 1] - m0[D + 1]) < 1e-13 * abs(m0[D + 1])
+    d0, d1, _ = run("dirichlet")
+    assert abs(d1[0] - d0[0]) > 1e-6 * abs(d0[0])

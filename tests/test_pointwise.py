@@ -114,3 +114,52 @@ def test_fp32_log_mean_series_coefficients():
     zz = (ay - ax) ** 2
     assert abs(math.sqrt(x * y) * ((P[0] * zz + P[1]) * zz + P[2]) / ((y - x) / math.log(y / x)) - 1) < 1e-9
     assert abs((Q[0] * zz + Q[1]) * zz + Q[2] - math.sqrt(x * y) * math.log(y / x) / (y - x)) < 1e-8
+
+
+def _toro_wave_curve(p_star, rho, p, gamma):
+    """Velocity change across the wave that connects a state (rho, p) to the
+    pressure p_star (Toro 2009, eqs. 4.6-4.7): shock for p_star > p,
+    rarefaction otherwise."""
+    c = math.sqrt(gamma * p / rho)
+    if p_star > p:
+        A = 2 / ((gamma + 1) * rho)
+        B = (gamma - 1) / (gamma + 1) * p
+        return (p_star - p) * math.sqrt(A / (p_star + B))
+    return 2 * c / (gamma - 1) * ((p_star / p) ** ((gamma - 1) / (2 * gamma)) - 1)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_slip_wall_flux_solves_the_wall_riemann_problem(D):
+    """boundary_condition_slip_wall: the wall pressure p* is where the wave
+    curve of the inner state meets the mirrored state's, f(p*) = v_n with v_n
+    the velocity towards the wall (Toro's pressure equation with u_L = v_n,
+    u_R = -v_n); only the normal momentum flux is nonzero, p* on either side
+    of the element. Strong rarefactions (1 + (gamma - 1) v_n / (2c) < 0) give
+    vacuum, p* = 0."""
+    rng = np.random.default_rng(11)
+    gamma = 1.4
+    checked = 0
+    for _ in range(200):
+        rho = rng.uniform(0.3, 3.0)
+        v = rng.uniform(-2.0, 2.0, D)
+        p = rng.uniform(0.2, 3.0)
+        u = np.concatenate([[rho], rho * v, [p / (gamma - 1) + 0.5 * rho * v @ v]])
+        for o in range(D):
+            for plus in (True, False):
+                f = O.slip_wall_flux(D, u, o, plus, gamma)
+                others = np.delete(f, 1 + o)
+                assert np.all(others == 0), f
+                p_star = f[1 + o]
+                vn = v[o] if plus else -v[o]
+                c = math.sqrt(gamma * p / rho)
+                if 1 + (gamma - 1) * vn / (2 * c) < 0:
+                    assert p_star == 0
+                    continue
+                assert p_star > 0
+                assert _toro_wave_curve(p_star, rho, p, gamma) == pytest.approx(vn, rel=1e-11, abs=1e-13)
+                assert (p_star > p) == (vn > 0)
+                checked += 1
+    assert checked > 500
+    # at rest against the wall the wall pressure is the state's own
+    u0 = np.array([1.3] + [0.0] * D + [2.0 / (gamma - 1)])
+    assert O.slip_wall_flux(D, u0, 0, True, gamma)[1] == pytest.approx(2.0, rel=1e-15)

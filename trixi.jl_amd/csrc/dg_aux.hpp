@@ -361,7 +361,16 @@ __device__ void exact_solution(const ExactParams &ep, const double *x, double t,
       u[V - 1] = ini * ini;
       break;
     }
-    case 2: {  // initial_condition_weak_blast_wave (3D)
+    case 2: {  // initial_condition_weak_blast_wave
+      if (D == 2) {  // compressible_euler_2d.jl:181-200: polar coordinates
+        const double r = sqrt(x[0] * x[0] + x[1] * x[1]);
+        const double phi = atan2(x[1], x[0]);
+        const double w[4] = {r > 0.5 ? 1.0 : 1.1691, r > 0.5 ? 0.0 : 0.1882 * cos(phi),
+                             r > 0.5 ? 0.0 : 0.1882 * sin(phi), r > 0.5 ? 1.0 : 1.245};
+        prim2cons(w);
+        break;
+      }
+      // compressible_euler_3d.jl:163-183: spherical coordinates
       double xn = x[0], yn = x[1], zn = D == 3 ? x[2] : 0.0;
       double r = sqrt(xn * xn + yn * yn + zn * zn);
       double phi = atan2(yn, xn);
@@ -598,6 +607,74 @@ __global__ __launch_bounds__(256) void boundary_state_kernel(const double *__res
   exact_solution<D>(ep, x, tt, ub);
 #pragma unroll
   for (int v = 0; v < V; ++v) ghost[(size_t)k * V + v] = T(ub[v]);
+}
+
+// boundary_condition_slip_wall, TreeMesh methods (compressible_euler_2d.jl:322-420,
+// compressible_euler_3d.jl:319-415): the flux at a wall face node from the
+// inner state alone. The state is rotated to the outward normal n = +-e_o;
+// the tangents of rotate_to_x fix the order of the kinetic-energy terms (3D
+// o = x: (z, y), y: (x, z), z: (y, x); 2D the other axis). The wall pressure p*
+// solves the pressure Riemann problem (Toro 2009, sec. 6.3.3): a rarefaction
+// for an outward normal velocity v_n <= 0 (clipped at 0 when the base of the
+// power is negative), a shock otherwise. The flux is p* in the normal momentum
+// component and zero elsewhere, for either side of the element. FP64
+// arithmetic in both precisions (a surface set), stored in T. One thread per
+// boundary face node; bnd[b] = {element, 0-based direction}; the flux goes to
+// face slot b of mflux, face node = the element kernel's line index (free
+// indices, lower dimension fastest).
+template <typename T, int D, int N>
+__global__ __launch_bounds__(256) void slip_wall_flux_kernel(const T *__restrict__ u, const int32_t *__restrict__ bnd,
+                                                             int B, double gamma, T *__restrict__ mflux) {
+  constexpr int n = N + 1, V = D + 2, L = D == 3 ? n * n : n, NN = ipow_c(n, D);
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= B * L) return;
+  const int b = k / L, fn = k - b * L;
+  const int e = bnd[2 * b], dir = bnd[2 * b + 1], o = dir >> 1;
+  const bool plus = (dir & 1) != 0;
+  int idx[3] = {0, 0, 0};
+  {
+    int r = fn;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (d == o) {
+        idx[d] = plus ? n - 1 : 0;
+      } else {
+        idx[d] = r % n;
+        r /= n;
+      }
+    }
+  }
+  const int node = D == 3 ? idx[0] + n * (idx[1] + n * idx[2]) : idx[0] + n * idx[1];
+  const T *q = u + ((size_t)e * NN + node) * V;
+  const double rho = double(q[0]);
+  const double mo = double(q[1 + o]);
+  const double mn = plus ? mo : -mo;
+  double mt0, mt1 = 0.0;
+  if constexpr (D == 3) {
+    mt0 = double(q[1 + (o == 0 ? 2 : (o == 1 ? 0 : 1))]);
+    mt1 = double(q[1 + (o == 0 ? 1 : (o == 1 ? 2 : 0))]);
+  } else {
+    mt0 = double(q[1 + (1 - o)]);
+  }
+  const double vn = mn / rho;
+  double ke2 = mn * vn;
+  ke2 += mt0 * (mt0 / rho);
+  if constexpr (D == 3) ke2 += mt1 * (mt1 / rho);
+  const double p = (gamma - 1) * (double(q[V - 1]) - 0.5 * ke2);
+  double p_star;
+  if (vn <= 0) {
+    const double c = sqrt(gamma * p / rho);
+    const double base = 1 + 0.5 * (gamma - 1) * vn / c;
+    p_star = base >= 0 ? p * pow(base, 2 * gamma * (1.0 / (gamma - 1))) : 0.0;
+  } else {
+    const double A = 2 / ((gamma + 1) * rho);
+    const double Bc = p * (gamma - 1) / (gamma + 1);
+    p_star = p + 0.5 * vn / A * (vn + sqrt(vn * vn + 4 * A * (p + Bc)));
+  }
+  T *f = mflux + (size_t)k * V;
+#pragma unroll
+  for (int v = 0; v < V; ++v) f[v] = T(0);
+  f[1 + o] = T(p_star);
 }
 
 // L2 mortars (prolong2mortars! + calc_mortar_flux! + mortar_fluxes_to_elements!,

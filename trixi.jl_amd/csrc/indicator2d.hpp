@@ -14,10 +14,12 @@
 // and deviation are sums over the four lanes of the element (xor shuffles).
 // The lane's four normalised nodal values are exactly its part of the
 // network's B operand (features 4 j .. 4 j + 3), and the lane with j = 0 holds
-// the three energy features: the features never leave registers. The network
-// image is read from device memory (L2) where the MFMA operands need it, so
-// that the workgroup has no barrier. Arithmetic up to the features is FP64 and
-// in the operation order of hg_indicator_kernel (the same factors).
+// the three energy features: the features never leave registers. Each lane
+// loads its operands of the network (MFMA fragments, biases, the training-box
+// bounds) from the device image at kernel entry, next to the state loads, so
+// the kernel makes one memory round trip and the workgroup has no barrier.
+// Arithmetic up to the features is FP64 and in the operation order of
+// hg_indicator_kernel (the same factors).
 #pragma once
 #include "dg_aux.hpp"
 #include "mlp.hpp"
@@ -29,15 +31,17 @@ struct Vinv4 {
 };
 
 // FEAT = false: alpha_raw[e] = the HG factor. FEAT = true: the features as
-// well (into feat_out [E][32] when not null), and with the network image img
-// (not null) alpha_raw[e] = the network's clipped factor, or the HG factor
-// where an input leaves [lo, hi] or the output is not finite (fallback != 0);
-// without img alpha_raw receives the HG factor. alpha_hg_out (may be null)
-// receives the HG factor in every case.
+// well (into feat_out [E][32] when not null), and with net != 0
+// alpha_raw[e] = the network's clipped factor, or the HG factor where an input
+// leaves [lo, hi] or the output is not finite (fallback != 0); with net == 0
+// alpha_raw receives the HG factor. With FEAT, img, lo and hi must point to
+// device memory of their sizes even when net == 0 (they are read
+// unconditionally, so that no branch sits between the loads and their use).
+// alpha_hg_out (may be null) receives the HG factor in every case.
 template <typename T, bool FEAT>
 __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
     const T *__restrict__ u, int E, HGParams hp, Vinv4 vi, const mlp::InferImage *__restrict__ img,
-    const float *__restrict__ lo, const float *__restrict__ hi, int fallback, double *__restrict__ alpha_raw,
+    const float *__restrict__ lo, const float *__restrict__ hi, int net, int fallback, double *__restrict__ alpha_raw,
     double *__restrict__ alpha_hg_out, float *__restrict__ feat_out) {
   constexpr int n = 4, V = 4;
   __shared__ double s_mx[16 * 16];  // x-transformed rows [element][row j][mode i]
@@ -49,9 +53,36 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
   const bool valid = e < E;
   const int ec = valid ? e : E - 1;  // lanes past the end compute on the last element, store nothing
 
-  // indicator variable at the lane's four nodes (i + 4 j)
+  // the lane's four nodes (i + 4 j): 16 contiguous values, loaded at once in
+  // 16-byte pieces before the (wave-uniform) choice of the indicator variable
+  using C16 = float4;
+  constexpr int NCH = n * V * (int)sizeof(T) / 16;
+  C16 raw[NCH];
+  {
+    const C16 *src = reinterpret_cast<const C16 *>(u + ((size_t)ec * 16 + 4 * j) * V);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) raw[c] = src[c];
+  }
+  // with the network: its operands and the lane's training-box bounds, issued
+  // behind the state loads, so that their latency overlaps the modal
+  // transform (one memory round trip for the whole kernel)
+  mlp::WeightRegs wr;
+  float lo_r[7], hi_r[7];
+  if constexpr (FEAT) {
+    mlp::load_weight_regs(*img, wr);
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      lo_r[i] = lo[4 * j + i];
+      hi_r[i] = hi[4 * j + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      lo_r[4 + i] = lo[16 + i];
+      hi_r[4 + i] = hi[16 + i];
+    }
+  }
+  const T *un = reinterpret_cast<const T *>(raw);
   double q[n];
-  const T *un = u + ((size_t)ec * 16 + 4 * j) * V;
 #pragma unroll
   for (int i = 0; i < n; ++i) {
     const double rho = un[i * V], m2 = double(un[i * V + 1]) * double(un[i * V + 1]) +
@@ -71,12 +102,27 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  // row j of the inverse Vandermonde matrix, selected from registers: a
+  // lane-indexed read of the kernel argument would be a memory load (one more
+  // round trip); the empty asm keeps the selects from being folded back
+  double vj[n];
+#pragma unroll
+  for (int b = 0; b < n; ++b) {
+    double x = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < n; ++jj) {
+      double c = vi.v[jj * n + b];
+      asm volatile("" : "+v"(c));
+      x = j == jj ? c : x;
+    }
+    vj[b] = x;
+  }
   double m[n];  // modes (a, j)
 #pragma unroll
   for (int a = 0; a < n; ++a) {
     double r = 0;
 #pragma unroll
-    for (int b = 0; b < n; ++b) r += vi.v[j * n + b] * mx[4 * b + a];
+    for (int b = 0; b < n; ++b) r += vj[b] * mx[4 * b + a];
     m[a] = r;
   }
   // squared modes (a, j) of the element to LDS; every lane of the element
@@ -118,7 +164,6 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
       if (alpha_hg_out) alpha_hg_out[e] = a_hg;
     }
   } else {
-    const bool net = img != nullptr;
     // features: nodal values centred by the element mean and divided by the
     // largest deviation (+ 1e-7 |mean|); the HG logits of the two energy
     // ratios / 8 in [-2, 2]; log10 of the smoothest-modes share / 8 in [-2, 0]
@@ -147,14 +192,14 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
       mlp::Tile t;
       t.xb[0] = mlp::pack4(f[0], f[1], f[2], f[3]);
       t.xb[1] = j == 0 ? mlp::pack4(f[4], f[5], f[6], 0.f) : mlp::pack4(0.f, 0.f, 0.f, 0.f);
-      mlp::forward_layers(*img, t);
+      mlp::forward_layers_regs(wr, t);
       // the training box: every feature of the element inside [lo, hi]
       bool in_box = true;
 #pragma unroll
-      for (int i = 0; i < n; ++i) in_box = in_box && f[i] >= lo[4 * j + i] && f[i] <= hi[4 * j + i];
+      for (int i = 0; i < n; ++i) in_box = in_box && f[i] >= lo_r[i] && f[i] <= hi_r[i];
       if (j == 0)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) in_box = in_box && f[4 + i] >= lo[16 + i] && f[4 + i] <= hi[16 + i];
+        for (int i = 0; i < 3; ++i) in_box = in_box && f[4 + i] >= lo_r[4 + i] && f[4 + i] <= hi_r[4 + i];
       const unsigned long long out = __builtin_amdgcn_ballot_w64(!in_box);
       const bool element_in_box = ((out >> el) & 0x0001000100010001ull) == 0;
       const bool use_nn = !fallback || (element_in_box && isfinite(t.y));

@@ -158,6 +158,73 @@ template <typename W> __device__ __forceinline__ void forward_layers(const W &s,
   t.y = 1.f / (1.f + __expf(-o));
 }
 
+// The operands one lane needs for forward_layers, held in registers: the A
+// fragments of W1 (hb, kb) and W2 (ob, kb), the biases and output weights of
+// the rows the lane's accumulators hold (h = 16 hb + 4 (l / 16) + r). Loading
+// them all at kernel entry makes one memory round trip of the whole network
+// instead of one per layer.
+struct WeightRegs {
+  bf16x4 w1[4][2];
+  bf16x4 w2[4][4];
+  float b1[4][4], b2[4][4], w3[4][4];
+  float b3;
+};
+
+template <typename W> __device__ __forceinline__ void load_weight_regs(const W &s, WeightRegs &r) {
+  const int l = threadIdx.x & 63;
+  const int g = l >> 4;
+#pragma unroll
+  for (int hb = 0; hb < 4; ++hb) {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) r.w1[hb][kb] = lds4(&s.w1[(hb * 16 + (l & 15)) * KF + kb * 16 + 4 * g]);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) r.w2[hb][kb] = lds4(&s.w2[(hb * 16 + (l & 15)) * NH + kb * 16 + 4 * g]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      r.b1[hb][q] = s.b1[hb * 16 + 4 * g + q];
+      r.b2[hb][q] = s.b2[hb * 16 + 4 * g + q];
+      r.w3[hb][q] = s.w3[hb * 16 + 4 * g + q];
+    }
+  }
+  r.b3 = s.b3;
+}
+
+// forward_layers with the operands from registers: the same products, bias
+// additions and activations in the same order, so the same result bit for bit.
+__device__ __forceinline__ void forward_layers_regs(const WeightRegs &w, Tile &t) {
+#pragma unroll
+  for (int hb = 0; hb < 4; ++hb) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) acc = mfma(w.w1[hb][kb], t.xb[kb], acc);
+    float h[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      acc[r] += w.b1[hb][r];
+      h[r] = acc[r] > 0.f ? acc[r] : 0.f;
+    }
+    t.z1[hb] = acc;
+    t.h1b[hb] = pack4(h[0], h[1], h[2], h[3]);
+  }
+  float part = 0.f;
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc = mfma(w.w2[ob][kb], t.h1b[kb], acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      acc[r] += w.b2[ob][r];
+      part += w.w3[ob][r] * (acc[r] > 0.f ? acc[r] : 0.f);
+    }
+    t.z2[ob] = acc;
+  }
+  part += __shfl_xor(part, 16);
+  part += __shfl_xor(part, 32);
+  const float o = part + w.b3;
+  t.y = 1.f / (1.f + __expf(-o));
+}
+
 __device__ __forceinline__ void forward_tile(const float *__restrict__ x, int n, int tile, const WeightsLDS &s,
                                              Tile &t) {
   const int l = threadIdx.x & 63;

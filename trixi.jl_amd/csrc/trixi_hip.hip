@@ -111,11 +111,13 @@ struct trixi_hip_ctx {
   int32_t *d_nbr = nullptr;
   int B = 0;                     // boundaries (non-periodic mesh)
   double *d_bx = nullptr;        // [B, n^(D-1), D] coordinates of the boundary face nodes
+  bool slip = false;             // boundary_condition_slip_wall: boundary slots carry given fluxes
+  int32_t *d_bnd = nullptr;      // [B][2] {element, 0-based direction} (slip walls)
   // L2 mortars (refined meshes): M mortars, each with S = 2^(D-1) small faces
   // and one large face; face slots B + m (S + 1) + face in ghost / mflux
   int M = 0;
   int32_t *d_mort = nullptr;     // [M][S + 3] {small ids, large id, orientation, large_side}
-  void *d_mflux = nullptr;       // [B + M (S + 1), n^(D-1), V] given fluxes of the mortar faces
+  void *d_mflux = nullptr;       // [B + M (S + 1), n^(D-1), V] given fluxes of mortar faces (and slip walls)
   std::vector<double> mfu, mfl, mru, mrl;  // forward/reverse upper/lower, row-major n x n
   int32_t *d_adj_ptr = nullptr, *d_adj = nullptr;  // element adjacency (interfaces + mortars) for HG smoothing
   bool ghost_ready = false;      // ghost holds the states of a time-independent boundary function
@@ -328,15 +330,19 @@ int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat, co
     Vinv4 vi{};
     for (int i = 0; i < 16; ++i) vi.v[i] = ctx->Vinv[i];
     const int nb = (ctx->E + 15) / 16;  // one wavefront of 16 elements per workgroup
-    const float *lo = net ? ctx->d_mlp + 3 * mlp::NP : nullptr, *hi = net ? lo + mlp::NF : nullptr;
     const int fallback = ctx->ind_mode == TRIXI_HIP_IND_MLP_WITH_FALLBACK ? 1 : 0;
-    if (feat || net)
+    if (feat || net) {
+      // the network buffers exist whenever features or the network are asked
+      // for (trixi_hip_mlp_features and trixi_hip_set_indicator allocate them)
+      if (!ctx->d_mlp || !ctx->d_mlp_img) return set_err(ctx, TRIXI_HIP_ESTATE, "learned-indicator buffers missing");
+      const float *lo = ctx->d_mlp + 3 * mlp::NP, *hi = lo + mlp::NF;
       hipLaunchKernelGGL((indicator_2d_p3_kernel<T, true>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, params,
-                         vi, net ? ctx->d_mlp_img : nullptr, lo, hi, fallback, alpha_out, alpha_hg, feat);
-    else
+                         vi, ctx->d_mlp_img, lo, hi, net ? 1 : 0, fallback, alpha_out, alpha_hg, feat);
+    } else {
       hipLaunchKernelGGL((indicator_2d_p3_kernel<T, false>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, params,
-                         vi, (const mlp::InferImage *)nullptr, (const float *)nullptr, (const float *)nullptr, 0,
+                         vi, (const mlp::InferImage *)nullptr, (const float *)nullptr, (const float *)nullptr, 0, 0,
                          alpha_out, alpha_hg, (float *)nullptr);
+    }
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
   }
@@ -387,13 +393,23 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   // outer states of the Dirichlet boundaries at this stage's time; a
   // time-independent boundary function is evaluated once
   const bool t_dep = ctx->exact.kind == TRIXI_HIP_IC_CONVERGENCE_TEST;
-  if (ctx->B > 0 && (t_dep || !ctx->ghost_ready)) {
+  if (ctx->B > 0 && !ctx->slip && (t_dep || !ctx->ghost_ready)) {
     ctx->ghost_ready = !t_dep;
     bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
       constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
       const int total = ctx->B * ipow_c(N + 1, D - 1);
       hipLaunchKernelGGL((boundary_state_kernel<T, D, N>), dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
                          ctx->d_bx, total, t, tdev, ctx->exact, (T *)ctx->d_ghost);
+    });
+    if (!ok) return TRIXI_HIP_EINVAL;
+    HIP_TRY(hipGetLastError());
+  }
+  if (ctx->slip) {  // wall fluxes from the current inner states (given fluxes of the boundary slots)
+    bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+      constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+      const int total = ctx->B * ipow_c(N + 1, D - 1);
+      hipLaunchKernelGGL((slip_wall_flux_kernel<T, D, N>), dim3((total + 255) / 256), dim3(256), 0, ctx->stream, u,
+                         ctx->d_bnd, ctx->B, ctx->cfg.gamma, (T *)ctx->d_mflux);
     });
     if (!ok) return TRIXI_HIP_EINVAL;
     HIP_TRY(hipGetLastError());
@@ -422,7 +438,7 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   a.nbr = ctx->d_nbr;
   a.ghost = (const T *)ctx->d_ghost;
   a.mflux = (const T *)ctx->d_mflux;
-  a.nbnd = ctx->B;
+  a.nbnd = ctx->slip ? 0 : ctx->B;  // slip walls: every face slot carries a given flux
   a.inv_jac = ctx->d_invj;
   a.centers = ctx->d_centers;
   const bool pre_smoothed = hg && ctx->M > 0 && ctx->cfg.alpha_smooth;  // smoothed across mortars above
@@ -599,8 +615,11 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
                      !basis->mortar_forward_upper || !basis->mortar_forward_lower || !basis->mortar_reverse_upper ||
                      !basis->mortar_reverse_lower))
     return fail(TRIXI_HIP_EINVAL, "mortar arrays or mortar matrices missing");
-  if (ctx->B > 0 && cfg->boundary_condition != TRIXI_HIP_BC_DIRICHLET)
-    return fail(TRIXI_HIP_EINVAL, "a mesh with boundaries needs boundary_condition = TRIXI_HIP_BC_DIRICHLET");
+  if (ctx->B > 0 && cfg->boundary_condition != TRIXI_HIP_BC_DIRICHLET &&
+      cfg->boundary_condition != TRIXI_HIP_BC_SLIP_WALL)
+    return fail(TRIXI_HIP_EINVAL,
+                "a mesh with boundaries needs boundary_condition = TRIXI_HIP_BC_DIRICHLET or TRIXI_HIP_BC_SLIP_WALL");
+  ctx->slip = ctx->B > 0 && cfg->boundary_condition == TRIXI_HIP_BC_SLIP_WALL;
   if (ctx->B > 0 && (!mesh->boundary_neighbor_ids || !mesh->boundary_orientations || !mesh->boundary_neighbor_sides))
     return fail(TRIXI_HIP_EINVAL, "boundary arrays missing");
   bool dn_ok = dispatch_dn(ctx->D, ctx->N, [](auto, auto) {});
@@ -754,9 +773,10 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   const size_t slot_vals = nslots * Lf * ctx->V;
   if (ok && ctx->B > 0) ok = dev_alloc(ctx, &ctx->d_bx, bx.size() * 8) == hipSuccess;
   if (ok && nslots > 0) ok = dev_alloc(ctx, &ctx->d_ghost, slot_vals * tsz) == hipSuccess;
+  if (ok && (ctx->M > 0 || ctx->slip)) ok = dev_alloc(ctx, &ctx->d_mflux, slot_vals * tsz) == hipSuccess;
+  if (ok && ctx->slip) ok = dev_alloc(ctx, &ctx->d_bnd, bnd.size() * 4) == hipSuccess;
   if (ok && ctx->M > 0)
-    ok = dev_alloc(ctx, &ctx->d_mflux, slot_vals * tsz) == hipSuccess &&
-         dev_alloc(ctx, &ctx->d_mort, mort.size() * 4) == hipSuccess &&
+    ok = dev_alloc(ctx, &ctx->d_mort, mort.size() * 4) == hipSuccess &&
          dev_alloc(ctx, &ctx->d_adj_ptr, adj_ptr.size() * 4) == hipSuccess &&
          dev_alloc(ctx, &ctx->d_adj, std::max<size_t>(adj.size(), 1) * 4) == hipSuccess;
   if (ok && ctx->fp32)
@@ -789,6 +809,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
       ok = sync_copy(ctx, ctx->d_ghost, g.data(), slot_vals * 8, hipMemcpyHostToDevice) == hipSuccess;
     }
   }
+  if (ok && ctx->slip) ok = sync_copy(ctx, ctx->d_bnd, bnd.data(), bnd.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
   if (ok && ctx->M > 0)
     ok = sync_copy(ctx, ctx->d_mort, mort.data(), mort.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          sync_copy(ctx, ctx->d_adj_ptr, adj_ptr.data(), adj_ptr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
@@ -811,7 +832,7 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
                   ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
                   ctx->d_alpha_hg, ctx->d_mlp_img, ctx->d_records, ctx->d_bx, ctx->d_ghost, ctx->d_mort,
-                  ctx->d_mflux,
+                  ctx->d_mflux, ctx->d_bnd,
                   ctx->d_adj_ptr, ctx->d_adj};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);

@@ -3,8 +3,8 @@
 # Makes the MI355X right-hand side a drop-in for Trixi.jl's
 #   rhs_hyperbolic!(du_ode, u_ode, semi, t)   (src/semidiscretization/semidiscretization_hyperbolic.jl:578-597)
 # for TreeMesh{2,3} (uniform or refined with L2 mortars) +
-# CompressibleEulerEquations{2,3}D + DGSEM with periodic boundaries or
-# BoundaryConditionDirichlet(initial_condition). The ODEProblem keeps `semi` as its parameter, so the 2N
+# CompressibleEulerEquations{2,3}D + DGSEM with periodic boundaries,
+# BoundaryConditionDirichlet(initial_condition) or boundary_condition_slip_wall. The ODEProblem keeps `semi` as its parameter, so the 2N
 # integrators and the Stepsize/Analysis callbacks run unchanged; only the RHS
 # function is replaced by a callable that ccalls the library.
 #
@@ -19,7 +19,8 @@ using Trixi: TreeMesh, DGSEM, CompressibleEulerEquations2D, CompressibleEulerEqu
              VolumeIntegralPureLGLFiniteVolume, FluxLaxFriedrichs, flux_ranocha, flux_central,
              max_abs_speed, max_abs_speed_naive, source_terms_convergence_test, SolutionAnalyzer,
              mesh_equations_solver_cache, nelements, ninterfaces, nboundaries, nmortars,
-             BoundaryConditionDirichlet, initial_condition_constant, initial_condition_convergence_test,
+             BoundaryConditionDirichlet, boundary_condition_slip_wall, initial_condition_constant,
+             initial_condition_convergence_test,
              initial_condition_weak_blast_wave
 using SciMLBase: ODEProblem
 
@@ -119,14 +120,25 @@ ic_code(::typeof(initial_condition_convergence_test)) = Int32(1)
 ic_code(::typeof(initial_condition_weak_blast_wave)) = Int32(2)
 ic_code(f) = throw(ArgumentError("TrixiHIP cannot evaluate $f on the device"))
 
-# Every boundary must carry BoundaryConditionDirichlet(semi.initial_condition):
-# the device evaluates the initial condition as the outer state.
-function check_boundary_conditions(semi)
+# Every boundary must carry the same condition: BoundaryConditionDirichlet(semi.initial_condition)
+# (the device evaluates the initial condition as the outer state, code 1) or
+# boundary_condition_slip_wall (code 2). Returns the TRIXI_HIP_BC_* code.
+function boundary_condition_code(semi)
+    codes = Int32[]
     for bc in values(semi.boundary_conditions)
         bc isa Trixi.BoundaryConditionPeriodic && continue
-        bc isa BoundaryConditionDirichlet && bc.boundary_value_function === semi.initial_condition ||
-            throw(ArgumentError("TrixiHIP supports BoundaryConditionDirichlet(initial_condition) only"))
+        if bc === boundary_condition_slip_wall
+            push!(codes, Int32(2))
+        elseif bc isa BoundaryConditionDirichlet && bc.boundary_value_function === semi.initial_condition
+            push!(codes, Int32(1))
+        else
+            throw(ArgumentError("TrixiHIP supports BoundaryConditionDirichlet(initial_condition) and " *
+                                "boundary_condition_slip_wall only"))
+        end
     end
+    all(==(first(codes)), codes) ||
+        throw(ArgumentError("TrixiHIP needs the same boundary condition on every boundary"))
+    return first(codes)
 end
 
 check(rc, ctx) = rc == 0 || error("libtrixi_hip: ", unsafe_string(ccall((:trixi_hip_last_error, libtrixi_hip),
@@ -152,7 +164,7 @@ function HipRHS(semi; precision::Symbol = :fp64)
     E = nelements(solver, cache)
     I = ninterfaces(solver, cache)
     B = nboundaries(solver, cache)
-    B > 0 && check_boundary_conditions(semi)
+    bc = B > 0 ? boundary_condition_code(semi) : Int32(0)
     M = nmortars(solver, cache)
     vi, vflux, fvflux, indicator = vi_codes(solver.volume_integral)
     amax, amin, asmooth = indicator === nothing ? (0.5, 0.001, true) :
@@ -160,7 +172,7 @@ function HipRHS(semi; precision::Symbol = :fp64)
     hgvar = indicator === nothing ? Int32(0) : indicator_variable_code(indicator.variable)
     cfg = HipConfig(D, Trixi.polydeg(solver), E, I, equations.gamma, precision === :fp32 ? 1 : 0,
                     vi, vflux, flux_code(solver.surface_integral.surface_flux), fvflux, amax, amin,
-                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0, B, B > 0 ? 1 : 0, M, hgvar)
+                    asmooth ? 1 : 0, source_code(semi.source_terms), 0, 0, B, bc, M, hgvar)
     # interface container: 1-based ids and orientations -> 0-based
     nbr = Int32.(cache.interfaces.neighbor_ids .- 1)            # [2, I]
     ori = Int32.(cache.interfaces.orientations .- 1)            # [I]

@@ -31,6 +31,7 @@ FLUX_CENTRAL = 3
 
 BC_NONE = 0
 BC_DIRICHLET = 1
+BC_SLIP_WALL = 2
 
 SRC_NONE = 0
 SRC_CONVERGENCE_TEST = 1

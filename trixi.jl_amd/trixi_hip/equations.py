@@ -171,6 +171,17 @@ def _ic_convergence(x, t, eq):
 
 
 def _ic_weak_blast_wave(x, t, eq):
+    if eq.ndims == 2:
+        # compressible_euler_2d.jl:181-200 (polar coordinates)
+        r = np.sqrt(x[..., 0] ** 2 + x[..., 1] ** 2)
+        phi = np.arctan2(x[..., 1], x[..., 0])
+        inside = r <= 0.5
+        prim = np.empty(x.shape[:-1] + (4,))
+        prim[..., 0] = np.where(inside, 1.1691, 1.0)
+        prim[..., 1] = np.where(inside, 0.1882 * np.cos(phi), 0.0)
+        prim[..., 2] = np.where(inside, 0.1882 * np.sin(phi), 0.0)
+        prim[..., 3] = np.where(inside, 1.245, 1.0)
+        return eq.prim2cons(prim)
     # compressible_euler_3d.jl:163-183
     xn, yn, zn = x[..., 0], x[..., 1], x[..., 2]
     r = np.sqrt(xn ** 2 + yn ** 2 + zn ** 2)
@@ -381,3 +392,18 @@ class BoundaryConditionDirichlet:
 
     def __repr__(self):
         return f"BoundaryConditionDirichlet({self.boundary_value_function})"
+
+
+class _SlipWallBC:
+    """`boundary_condition_slip_wall` (compressible_euler_2d.jl:322-420,
+    compressible_euler_3d.jl:319-415, TreeMesh methods): the wall flux from the
+    inner state through the pressure Riemann problem (Toro 2009, sec. 6.3.3);
+    the surface flux of the solver is not used at the wall."""
+
+    code = _lib.BC_SLIP_WALL
+
+    def __repr__(self):
+        return "boundary_condition_slip_wall"
+
+
+boundary_condition_slip_wall = _SlipWallBC()

@@ -478,10 +478,11 @@ def main(argv=None):
     p = argparse.ArgumentParser(description="train the learned indicator on seeded KHI runs (needs a GPU)")
     p.add_argument("--out", default=DEFAULT_WEIGHTS)
     p.add_argument("--seeds", type=int, default=20)
-    p.add_argument("--level", type=int, default=5)
+    # the defaults reproduce the committed weights (their metadata records them)
+    p.add_argument("--level", type=int, default=7)
     p.add_argument("--t-end", type=float, default=3.0)
     p.add_argument("--snapshots", type=int, default=21)
-    p.add_argument("--steps", type=int, default=4000)
+    p.add_argument("--steps", type=int, default=6000)
     p.add_argument("--batch", type=int, default=8192)
     p.add_argument("--lr", type=float, default=2e-3)
     p.add_argument("--dead-zone", type=float, default=1.5,

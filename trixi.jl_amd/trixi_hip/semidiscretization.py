@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .equations import BoundaryConditionDirichlet, boundary_condition_periodic
+from .equations import BoundaryConditionDirichlet, boundary_condition_periodic, boundary_condition_slip_wall
 from .mesh import element_geometry
 from .solver import (_HG_VARIABLES, VolumeIntegralFluxDifferencing, VolumeIntegralPureLGLFiniteVolume,
                      VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm)
@@ -27,8 +27,8 @@ class SemidiscretizationHyperbolic:
     precision: "fp64" (plain FP64) or "fp32" (FP32 fluxes and 2N register
     u_tmp; the state u, the stage arithmetic and reductions in FP64). elements_per_block / threads_per_block override the
     element-kernel tile (0 = default). boundary_conditions: periodic (for a
-    periodic mesh) or one BoundaryConditionDirichlet(initial_condition) for
-    every boundary of a non-periodic mesh."""
+    periodic mesh) or, for every boundary of a non-periodic mesh, one
+    BoundaryConditionDirichlet(initial_condition) or boundary_condition_slip_wall."""
 
     def __init__(self, mesh, equations, initial_condition, solver, source_terms=None,
                  boundary_conditions=boundary_condition_periodic, precision="fp64", elements_per_block=0,
@@ -37,12 +37,16 @@ class SemidiscretizationHyperbolic:
         if mesh.nboundaries == 0 and not periodic_bc:
             raise ValueError("a periodic mesh takes boundary_condition_periodic")
         if mesh.nboundaries > 0:
-            if not isinstance(boundary_conditions, BoundaryConditionDirichlet):
-                raise ValueError("a non-periodic mesh needs BoundaryConditionDirichlet(initial_condition)")
-            f = boundary_conditions.boundary_value_function
-            if (f.code, tuple(f.params)) != (initial_condition.code, tuple(initial_condition.params)):
-                raise ValueError("the Dirichlet boundary value function must be the initial condition "
-                                 "(the device evaluates one function for both)")
+            if boundary_conditions is boundary_condition_slip_wall:
+                pass
+            elif not isinstance(boundary_conditions, BoundaryConditionDirichlet):
+                raise ValueError("a non-periodic mesh needs BoundaryConditionDirichlet(initial_condition) or "
+                                 "boundary_condition_slip_wall")
+            else:
+                f = boundary_conditions.boundary_value_function
+                if (f.code, tuple(f.params)) != (initial_condition.code, tuple(initial_condition.params)):
+                    raise ValueError("the Dirichlet boundary value function must be the initial condition "
+                                     "(the device evaluates one function for both)")
         self.boundary_conditions = boundary_conditions
         if mesh.ndims != equations.ndims:
             raise ValueError("mesh and equations dimension differ")
@@ -122,7 +126,7 @@ class SemidiscretizationHyperbolic:
         cfg.source_terms = self.source_terms.code if self.source_terms is not None else _lib.SRC_NONE
         cfg.elements_per_block, cfg.threads_per_block = self.tile
         cfg.nboundaries = self.mesh.nboundaries
-        cfg.boundary_condition = _lib.BC_DIRICHLET if self.mesh.nboundaries else _lib.BC_NONE
+        cfg.boundary_condition = self.boundary_conditions.code if self.mesh.nboundaries else _lib.BC_NONE
         cfg.nmortars = self.mesh.nmortars
         return cfg
 
