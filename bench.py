@@ -164,6 +164,7 @@ def config_c4_c5(th, torch, steps, t_state=1.0):
             ms = rhs_time(sm, u, mode, launches=max(steps, 100))
             el, tl = blended_share(sm, indicator_alpha(sm, mode))
             out.setdefault(name, {})[prec] = dict(value=nd / (ms * 1e-3), ms_per_rhs=ms,
+                                                  ms_per_step=solve_step_ms(th, sm, u, max(steps, 100)),
                                                   blended_elements=el, blended_tiles=tl,
                                                   roofline=roofline_line(nd, 4, 4 if prec == "fp32" else 8, ms))
         f, _ = features(sm)
@@ -171,8 +172,24 @@ def config_c4_c5(th, torch, steps, t_state=1.0):
             (f[:, :NFEATURES] < net.feature_lo) | (f[:, :NFEATURES] > net.feature_hi), axis=1)))
         sm.close()
     for prec in ("fp32", "fp64"):
-        out[f"learned_over_hg_rhs_time_{prec}"] = out["c5_learned"][prec]["ms_per_rhs"] / out["c4_hg"][prec]["ms_per_rhs"]
+        for what in ("rhs", "step"):
+            out[f"learned_over_hg_{what}_time_{prec}"] = (out["c5_learned"][prec][f"ms_per_{what}"] /
+                                                          out["c4_hg"][prec][f"ms_per_{what}"])
     return out
+
+
+def solve_step_ms(th, semi, u, steps, reps=3):
+    """Time per CarpenterKennedy2N54 step (fused stages, wave-speed reduction
+    and device step control, replayed step graphs) from state u: the best of
+    reps device-controlled solves of `steps` steps at CFL 1.3."""
+    ode = th.semidiscretize(semi, (0.0, 1.0e3))
+    ode.u0 = u
+    best = math.inf
+    for _ in range(reps):
+        sol = th.solve(ode, th.CarpenterKennedy2N54(), callback=th.CallbackSet(th.StepsizeCallback(cfl=1.3)),
+                       maxiters=steps, save_final=False)
+        best = min(best, 1e3 * sol.wall_time / sol.nsteps)
+    return best
 
 
 def time_rhs(th, torch, semi, steps, warmup, u0):
@@ -280,11 +297,16 @@ def equal_l2(th, semi64, semi32, nsteps=100, cfl=0.6):
                 step_wall_ms_fp32=1e3 * out["fp32"]["wall_s"] / out["fp32"]["nsteps"])
 
 
+# The process's affinity mask, read before any OpenMP runtime starts: with
+# OMP_PROC_BIND set, the runtime binds the calling thread to one place, and the
+# mask read afterwards is that single core.
+START_AFFINITY = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
 def host_cores():
-    """Cores this process may run on (its affinity mask), the machine's count,
-    and the OpenMP setting the job was started with."""
-    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    return dict(available=avail, cpu_count=os.cpu_count(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+    """Cores this process may run on (its affinity mask at start), the
+    machine's count, and the OpenMP setting the job was started with."""
+    return dict(available=START_AFFINITY, cpu_count=os.cpu_count(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
                 omp_proc_bind=os.environ.get("OMP_PROC_BIND"), omp_places=os.environ.get("OMP_PLACES"))
 
 
@@ -311,7 +333,7 @@ def cpu_baseline(level, polydeg, seconds, u0):
     return dict(value=o.ndofs / t_rhs, unit="DOF-updates/s", cores=threads, kind="port",
                 sample=f"{n} full RHS calls of the same 3D p3 32^3 workload (FP64 C++ OpenMP restatement "
                        f"of Trixi rhs_hyperbolic!, not Trixi/Julia), {el:.1f} s, {threads} threads on the "
-                       f"{hc['available']} cores of the process affinity mask (os.cpu_count() {hc['cpu_count']}, "
+                       f"{hc['available']} cores of the process affinity mask at start (os.cpu_count() {hc['cpu_count']}, "
                        f"OMP_PROC_BIND={hc['omp_proc_bind']}, OMP_PLACES={hc['omp_places']})",
                 time_per_dof_per_rhs_s=t_rhs * threads / o.ndofs, host=hc)
 

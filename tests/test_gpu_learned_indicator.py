@@ -249,3 +249,109 @@ def test_learned_indicator_matches_hg_error_on_unseen_khi_level7():
     # the network leaves most elements pure DG where HG blends every one
     assert r["blended_elements_hg_fp64"] == 1.0 and r["blended_elements_nn_fp64"] < 0.5, r
     assert r["fallback_share_final"] < 0.01, r
+
+
+def _stage_runs(semi, make_net, second_params):
+    """The resident-state trajectories the fused-indicator test compares: a
+    device-controlled solve of 13 steps, graph and eager steps, single stages,
+    steps after a new state is uploaded, after the network's parameters change
+    and after the indicator switches to HG."""
+    import trixi_hip as th
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork, use_hg
+    lib, ctx = semi._lib, semi.ctx
+    alg = th.CarpenterKennedy2N54()
+    na = 5
+    a = (ctypes.c_double * na)(*alg.a)
+    b = (ctypes.c_double * na)(*alg.b)
+    c = (ctypes.c_double * na)(*alg.c)
+    net = make_net()
+    if net is not None:
+        net.attach(semi)
+    u0 = _download(semi)
+    up = lambda u: lib.trixi_hip_upload_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p))
+    ode = th.semidiscretize(semi, (0.0, 5.0))
+    ode.u0 = u0
+    sol = th.solve(ode, alg, callback=th.CallbackSet(th.StepsizeCallback(cfl=0.9)), maxiters=13)
+    res = [np.array([sol.nsteps, sol.t]), sol.u]
+    for graph in (1, 0):
+        assert up(u0) == 0
+        for rep in range(3):
+            assert lib.trixi_hip_rk_step(ctx, a, b, c, na, 0.1 + rep * 1e-3, 1e-3, graph) == 0
+        res.append(_download(semi))
+    assert up(u0) == 0
+    assert lib.trixi_hip_rk_stage(ctx, 0.0, 2e-3, 0.05, 1) == 0
+    assert lib.trixi_hip_rk_stage(ctx, alg.a[1], alg.b[1] * 1e-3, 0.06, 0) == 0
+    res.append(_download(semi))
+    # a new state: the factors of the old one must not be reused
+    u1 = u0 * (1.0 + 1e-3 * np.cos(np.arange(u0.size)))
+    assert up(u0) == 0
+    assert lib.trixi_hip_rk_step(ctx, a, b, c, na, 0.2, 1e-3, 1) == 0
+    assert up(u1) == 0
+    for rep in range(2):
+        assert lib.trixi_hip_rk_step(ctx, a, b, c, na, 0.2 + rep * 1e-3, 1e-3, 1) == 0
+    res.append(_download(semi))
+    if net is not None:
+        # new parameters, then HG: each changes the factors of the same state
+        IndicatorNeuralNetwork(second_params, net.feature_lo, net.feature_hi, net.fallback).upload(semi)
+        assert lib.trixi_hip_rk_step(ctx, a, b, c, na, 0.3, 1e-3, 1) == 0
+        res.append(_download(semi))
+        use_hg(semi)
+        assert lib.trixi_hip_rk_step(ctx, a, b, c, na, 0.31, 1e-3, 1) == 0
+        res.append(_download(semi))
+    return res
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode", ["hg", "mlp", "mlp_with_fallback", "hg_mortars", "mlp_mortars", "hg_tile32"])
+def test_fused_next_stage_indicator_equals_standalone(mode, precision, monkeypatch):
+    """The fused 2D p3 shock-capturing stage computes the next stage's raw
+    blending factors of its new state in its epilogue (HG, or the learned
+    indicator with and without fallback); the stages then run bitwise as with
+    the stand-alone indicator kernel before every stage, through solves, graph
+    and eager steps, single stages, mortars (smoothing across them), both 2D p3
+    tiles, and after the state, the network parameters or the indicator change."""
+    import trixi_hip as th
+    from gpu_helpers import build
+    from trixi_hip.learned_indicator import IndicatorNeuralNetwork
+    out = {}
+    for fused in ("0", "1"):
+        monkeypatch.setenv("TRIXI_HIP_FUSED_INDICATOR", fused)
+        if mode in ("hg_mortars", "mlp_mortars"):
+            spec = dict(ndims=2, polydeg=3, level=3, cmin=(-1, -1), cmax=(1, 1), ic="weak_blast_wave",
+                        volume_integral="hg", surface_flux="llf", fv_flux="llf", alpha_max=0.5, alpha_min=0.001,
+                        refinement_patches=[((-0.5, -0.5), (0.5, 0.5))])
+            semi, ic = build(spec, precision)
+            u = semi.compute_coefficients(0.0)
+            assert semi._lib.trixi_hip_upload_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p)) == 0
+        elif mode == "hg_tile32":
+            spec = dict(ndims=2, polydeg=3, level=4, cmin=(-1, -1), cmax=(1, 1), ic="weak_blast_wave",
+                        volume_integral="hg", surface_flux="llf", fv_flux="llf", alpha_max=0.5, alpha_min=0.001,
+                        epb=32, nt=256)
+            semi, ic = build(spec, precision)
+            e, t = ctypes.c_int(), ctypes.c_int()
+            assert semi._lib.trixi_hip_get_stage_tile(semi.ctx, ctypes.byref(e), ctypes.byref(t)) == 0
+            assert (e.value, t.value) == (32, 256)
+            u = semi.compute_coefficients(0.0)
+            assert semi._lib.trixi_hip_upload_u(semi.ctx, u.ctypes.data_as(ctypes.c_void_p)) == 0
+        else:
+            from trixi_hip.learned_indicator import run_segments
+            semi = _semi(level=4, seed=11, precision=precision)
+            run_segments(semi, 0.3, 2)
+        if mode.startswith("mlp"):
+            p = _params(9)
+            if mode == "mlp_with_fallback":
+                w = IndicatorNeuralNetwork.load()
+                make = lambda: IndicatorNeuralNetwork(w.params, w.feature_lo, w.feature_hi, True)
+                second = p
+            else:
+                make = lambda: IndicatorNeuralNetwork(p, fallback=False)
+                second = _params(10)
+        else:
+            make, second = (lambda: None), None
+        out[fused] = _stage_runs(semi, make, second)
+        semi.close()
+    ref, got = out["0"], out["1"]
+    assert len(ref) == len(got)
+    for k, (r, g) in enumerate(zip(ref, got)):
+        assert np.all(np.isfinite(r)), k
+        assert np.array_equal(r, g), (k, float(np.max(np.abs(r - g))))

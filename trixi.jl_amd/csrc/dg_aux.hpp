@@ -18,23 +18,9 @@ namespace trixi_hip {
 // node for the modal transform, one thread per element for the energies
 // (reference loop order).
 // ---------------------------------------------------------------------------
-struct HGParams {
-  double gamma, alpha_max, alpha_min, threshold, parameter_s;
-  int variable;  // indicator variable: 0 density_pressure, 1 density, 2 pressure
-};
-
 template <int N> struct VinvConst {
   double Vinv[(N + 1) * (N + 1)];
 };
-
-// Features of the learned indicator (2D p3 only; feat [E][32], zero padded;
-// computed by indicator_2d_p3_kernel):
-//   0..15  nodal rho*p, centred by its element mean and divided by the largest
-//          deviation (+ 1e-7 |mean|), so that they lie in [-1, 1];
-//   16, 17 the HG logits s/T (r - T) of the two energy ratios r, / 8, clipped
-//          to [-2, 2];
-//   18     log10(E_clip2 / E_total + 1e-16) / 8, clipped to [-2, 0].
-constexpr int kMlpFeatureStride = 32;
 
 template <typename T, int D, int N, int IPB, int NT>
 __global__ __launch_bounds__(NT) void hg_indicator_kernel(const T *__restrict__ u,
@@ -179,9 +165,6 @@ __global__ void convert_kernel(TO *__restrict__ out, const TI *__restrict__ in, 
 //   out[0] = cfl * 2 / (nnodes * max_scaled_speed)
 // NaN propagates (fmax would drop it; the reference uses Base.max).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double nanmax(double a, double b) {
-  return (a != a || b != b) ? __builtin_nan("") : (a > b ? a : b);
-}
 
 // max_dt (stepsize_dg3d.jl:8-32, stepsize_dg2d.jl:8-30) in two passes: per
 // block, max over its elements of inv_jac * sum_d max_nodes lambda_d with
@@ -212,17 +195,10 @@ __global__ __launch_bounds__(256) void max_speed_nodes_kernel(const T *__restric
     const int ne = min(EB, E - e0);
     if (tid < ne * NN) {
       const T *un = u + ((size_t)e0 * NN + tid) * V;
-      const double rho = un[0];
-      double ke2 = 0, v[D];
+      double lam[D];
+      node_wave_speeds<D>([&](int v) { return double(un[v]); }, gamma, lam);
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        v[d] = double(un[1 + d]) / rho;
-        ke2 += double(un[1 + d]) * v[d];
-      }
-      const double p = (gamma - 1) * (double(un[D + 1]) - 0.5 * ke2);
-      const double c = sqrt(gamma * p / rho);
-#pragma unroll
-      for (int d = 0; d < D; ++d) s_l[d][tid] = fabs(v[d]) + c;
+      for (int d = 0; d < D; ++d) s_l[d][tid] = lam[d];
     }
     __syncthreads();
     for (int k = tid; k < NPAIR * S; k += 256) {

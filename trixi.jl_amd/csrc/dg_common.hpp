@@ -395,7 +395,47 @@ __device__ __forceinline__ void source_convergence_sc(T si, T co, T gamma, T *s)
   }
 }
 
+// NaN-propagating maximum of the wave-speed reductions (max_dt).
+__device__ __forceinline__ double nanmax(double a, double b) {
+  return (a != a || b != b) ? __builtin_nan("") : (a > b ? a : b);
+}
+
+// Per-direction wave speeds |v_d| + c of one node (max_abs_speeds,
+// compressible_euler_3d.jl:1770-1775) in FP64 from its conservative values
+// (load(v) returns value v as double). Shared by max_speed_nodes_kernel and the
+// fused last stage of a step, so that both give the same bits.
+template <int D, typename L> __device__ __forceinline__ void node_wave_speeds(L load, double gamma, double *lam) {
+  const double rho = load(0);
+  double ke2 = 0, v[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    v[d] = load(1 + d) / rho;
+    ke2 += load(1 + d) * v[d];
+  }
+  const double p = (gamma - 1) * (load(D + 1) - 0.5 * ke2);
+  const double c = sqrt(gamma * p / rho);
+#pragma unroll
+  for (int d = 0; d < D; ++d) lam[d] = fabs(v[d]) + c;
+}
+
 __device__ __forceinline__ void tsincos(float x, float *s, float *c) { sincosf(x, s, c); }
 __device__ __forceinline__ void tsincos(double x, double *s, double *c) { sincos(x, s, c); }
+
+// Parameters of IndicatorHennemannGassner (indicators.jl:50, 119-131): the
+// threshold 0.5 10^(-1.8 nnodes^0.25) and the sharpness log((1 - 1e-4) / 1e-4)
+// are set by the host from the number of nodes per direction.
+struct HGParams {
+  double gamma, alpha_max, alpha_min, threshold, parameter_s;
+  int variable;  // indicator variable: 0 density_pressure, 1 density, 2 pressure
+};
+
+// Features of the learned indicator (2D p3 only; feat [E][32], zero padded;
+// computed by indicator_2d_p3_kernel):
+//   0..15  nodal rho*p, centred by its element mean and divided by the largest
+//          deviation (+ 1e-7 |mean|), so that they lie in [-1, 1];
+//   16, 17 the HG logits s/T (r - T) of the two energy ratios r, / 8, clipped
+//          to [-2, 2];
+//   18     log10(E_clip2 / E_total + 1e-16) / 8, clipped to [-2, 0].
+constexpr int kMlpFeatureStride = 32;
 
 }  // namespace trixi_hip

@@ -34,6 +34,7 @@
 // accesses of phases A and C, free of bank conflicts for 32 and for 64 banks.
 #pragma once
 #include "dg_common.hpp"
+#include "indicator2d.hpp"
 
 namespace trixi_hip {
 
@@ -83,6 +84,18 @@ template <typename T> struct ElemArgs {
   T gamma;
   T inv_gamma_m1;  // 1 / (gamma - 1), rounded once in T
   T sqrt_gamma;
+  // fused last stage of a step (STAGE variants): with speed_partial != null
+  // the epilogue also reduces the wave speed of the new state (the max_dt input
+  // of the next step, max_speed_nodes_kernel's arithmetic) to one FP64 partial
+  // per workgroup, speed_partial[workgroup]; gamma64 is gamma in FP64
+  double *speed_partial;
+  double gamma64;
+  // fused stage of a 2D polydeg-3 shock-capturing variant: with ind_next != 0
+  // the epilogue also computes the raw blending factors of the new state (the
+  // next stage's indicator: HG, or the learned one with ind.net) into
+  // ind.alpha_raw, with indicator_2d_p3_kernel's arithmetic
+  int ind_next;
+  Ind2DArgs ind;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs. Remap so that each
@@ -564,6 +577,102 @@ __device__ __forceinline__ void source_table_entry(const DGConst<T, N> &k, int i
   tab[2 * idx + 1] = co;
 }
 
+// LDS region A of a variant (node records and/or conservative state, later the
+// output staging), in elements of T
+template <typename T, int D, int N, int VOL, int EPB> constexpr int region_a_size() {
+  constexpr int NW = VolTraits<VOL>::RAN ? nodeR_count(D) : 0;
+  constexpr int NC = VolTraits<VOL>::CONS ? D + 2 : 0;
+  constexpr int ALIGN = 16 / (int)sizeof(T);
+  return ((NW + NC) * Layout<D, N, EPB>::SIZE + ALIGN - 1) / ALIGN * ALIGN;
+}
+// The fused last stage can reduce the wave speed when region A holds the D
+// per-node FP64 speeds of the tile (not so for the weak form, central flux and
+// pure FV in 3D FP32, whose region A is the conservative state only).
+template <typename T, int D, int N, int VOL, int EPB> constexpr bool stage_speed_fits() {
+  return (size_t)region_a_size<T, D, N, VOL, EPB>() * sizeof(T) >=
+         (size_t)D * EPB * ipow_c(N + 1, D) * sizeof(double);
+}
+// The fused stage of a 2D polydeg-3 shock-capturing variant can compute the
+// next stage's indicator when the tile is whole wavefronts of 16 elements, one
+// wavefront each, and region A holds their 512 doubles of scratch apiece.
+template <typename T, int D, int N, int VOL, int EPB, int NT> constexpr bool stage_ind2d_fits() {
+  return D == 2 && N == 3 && VolTraits<VOL>::HG && EPB % 16 == 0 && EPB / 16 <= NT / 64 &&
+         (size_t)region_a_size<T, D, N, VOL, EPB>() * sizeof(T) >= (size_t)(EPB / 16) * 512 * sizeof(double);
+}
+
+// Wave-speed reduction of the fused last stage (max_speed_nodes_kernel,
+// dg_aux.hpp, for the tile's new state): per node the FP64 wave speeds of
+// node_wave_speeds into s_lam [D][EPB NN] (LDS region A, free after the
+// update), per (element, direction) the maximum over the nodes (S threads per
+// pair, then one), per element inv_jac * sum over d, and the NaN-propagating
+// maximum over the tile to speed_partial[e0 / EPB]. Maxima are exact, so the
+// partials combine to the same maximum as the stand-alone kernel's.
+template <typename T, int D, int N, int EPB, int NT>
+__device__ __forceinline__ void stage_wave_speed(const ElemArgs<T> &a, const double *s_un, double *s_lam, int ne,
+                                                 int e0) {
+  constexpr int NN = ipow_c(N + 1, D), V = D + 2, S = 8, NPAIR = EPB * D;
+  static_assert(EPB <= NT, "one thread per element");
+  __shared__ double s_pm[NPAIR * S];
+  __shared__ double s_em[EPB];
+  const int tid = threadIdx.x;
+  __syncthreads();  // s_un complete; region A (s_o) no longer read
+  for (int q = tid; q < ne * NN; q += NT) {
+    double lam[D];
+    node_wave_speeds<D>([&](int v) { return s_un[q * V + v]; }, a.gamma64, lam);
+#pragma unroll
+    for (int d = 0; d < D; ++d) s_lam[d * EPB * NN + q] = lam[d];
+  }
+  __syncthreads();
+  for (int k = tid; k < NPAIR * S; k += NT) {
+    const int pair = k / S, sub = k - pair * S;
+    const int el = pair / D, d = pair - el * D;
+    double m = 0.0;
+    if (el < ne)
+      for (int q = sub; q < NN; q += S) m = nanmax(m, s_lam[d * EPB * NN + el * NN + q]);
+    s_pm[k] = m;
+  }
+  __syncthreads();
+  if (tid < ne) {
+    double sum = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double m = 0.0;
+#pragma unroll
+      for (int j = 0; j < S; ++j) m = nanmax(m, s_pm[(tid * D + d) * S + j]);
+      sum += m;
+    }
+    s_em[tid] = a.inv_jac[e0 + tid] * sum;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double bmax = 4.9406564584124654e-324;  // nextfloat(0.0), as max_speed_nodes_kernel starts
+    for (int i = 0; i < ne; ++i) bmax = nanmax(bmax, s_em[i]);
+    a.speed_partial[e0 / EPB] = bmax;
+  }
+}
+
+// The next stage's indicator of a 2D polydeg-3 tile, from its new state in
+// LDS (s_un, FP64 [q][v]; in FP32 mode rounded to the FP32 state the next
+// stage reads): one wavefront per 16 elements runs indicator_2d_p3_wave with
+// the network operands read where used, s_tmp (region A) holding its 512
+// doubles of scratch per wavefront.
+template <typename T, int EPB>
+__device__ __forceinline__ void stage_indicator_2d(const ElemArgs<T> &a, const double *s_un, double *s_tmp, int ne,
+                                                   int e0) {
+  __syncthreads();  // s_un complete; region A no longer read
+  const int w = threadIdx.x >> 6;
+  if (w < EPB / 16 && w * 16 < ne) {
+    const int l = threadIdx.x & 63, el = l & 15, j = l >> 4;
+    const int elc = min(w * 16 + el, ne - 1);  // lanes past the tile's end repeat its last element
+    T uv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) uv[k] = T(s_un[(elc * 16 + 4 * j) * 4 + k]);
+    double *s_mx = s_tmp + w * 512, *s_sq = s_mx + 256;
+    if (a.ind.net) indicator_2d_p3_wave<T, true, false>(uv, e0 + w * 16, a.E, a.ind, s_mx, s_sq);
+    else indicator_2d_p3_wave<T, false, false>(uv, e0 + w * 16, a.E, a.ind, s_mx, s_sq);
+  }
+}
+
 // One tile of EPB elements per workgroup. Every global load of phase A is
 // issued before the first one is consumed: neighbour ids, the tile (in chunks,
 // into registers), the per-element scalars, then the neighbour face states,
@@ -580,7 +689,8 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
   constexpr int NC = VT::CONS ? V : 0;
   constexpr int ALIGN = 16 / (int)sizeof(T);
   // primitive / conservative data, later output staging; a multiple of 16 bytes
-  constexpr int SIZE_A = ((NW + NC) * SZ + ALIGN - 1) / ALIGN * ALIGN;
+  constexpr int SIZE_A = region_a_size<T, D, N, VOL, EPB>();
+  static_assert(SIZE_A == ((NW + NC) * SZ + ALIGN - 1) / ALIGN * ALIGN, "region A");
   constexpr int SIZE_B = D * V * SZ;  // du per direction, earlier input staging
   static_assert(SIZE_A >= EPB * NN * V && SIZE_B >= EPB * NN * V, "staging does not fit");
   static_assert(EPB * LY::L <= WPD * 64, "one line per lane");
@@ -766,14 +876,23 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
   __syncthreads();
   if constexpr (STAGE) {
     // the 2N update of the tile's values, in the same arithmetic as
-    // rk2n_update_kernel (bitwise equal to the unfused stage)
+    // rk2n_update_kernel (bitwise equal to the unfused stage); with the speed
+    // reduction the new values are kept in LDS too (region B, [q][v], free
+    // after phase C)
     const double b_dt = a.tdev ? a.b_stage * a.tdev[1] : a.b_stage;
+    constexpr bool SPEED_FITS = stage_speed_fits<T, D, N, VOL, EPB>();
+    const bool speed = SPEED_FITS && a.speed_partial != nullptr;
+    constexpr bool IND2D = stage_ind2d_fits<T, D, N, VOL, EPB, NT>();
+    const bool ind = IND2D && a.ind_next != 0;
+    double *s_un = reinterpret_cast<double *>(s_d);
+    static_assert(SIZE_B * sizeof(T) >= (size_t)EPB * NN * V * sizeof(double), "new state staging does not fit");
     auto update = [&](size_t g, T du_v, double ut, double ur) {
       ut = double(du_v) - ut * a.a_stage;
       const double un = tfma(ut, b_dt, ur);
       a.u_tmp[g] = T(ut);
       a.u_reg_out[g] = un;
       if (a.u32_out) a.u32_out[g] = float(un);
+      if (speed || ind) s_un[g - base] = un;
     };
     if (full) {
 #pragma unroll
@@ -790,6 +909,12 @@ __global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, 
         const size_t g = base + i;
         update(g, s_o[i], a.first_stage ? 0.0 : double(a.u_tmp[g]), a.u_reg_in[g]);
       }
+    }
+    if constexpr (SPEED_FITS) {
+      if (speed) stage_wave_speed<T, D, N, EPB, NT>(a, s_un, reinterpret_cast<double *>(smem), ne, e0);
+    }
+    if constexpr (IND2D) {
+      if (ind) stage_indicator_2d<T, EPB>(a, s_un, reinterpret_cast<double *>(smem), ne, e0);
     }
   } else if (full) {
     const C *src = reinterpret_cast<const C *>(s_o);

@@ -29,6 +29,12 @@ struct ElemKey {
 using ElemLaunch = hipError_t (*)(const void *args, const void *kconst, int nblocks, hipStream_t s);
 
 std::map<ElemKey, ElemLaunch> &elem_registry();  // defined in trixi_hip.hip
+// what a fused-stage variant's epilogue can add: the wave-speed reduction
+// (stage_speed_fits) and the next stage's 2D p3 indicator (stage_ind2d_fits)
+struct ElemCaps {
+  bool speed = false, ind2d = false;
+};
+std::map<ElemKey, ElemCaps> &elem_caps();
 
 template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND, bool STAGE>
 hipError_t launch_elem(const void *args, const void *kconst, int nblocks, hipStream_t s) {
@@ -41,8 +47,10 @@ hipError_t launch_elem(const void *args, const void *kconst, int nblocks, hipStr
 
 template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND, bool STAGE> struct Reg {
   Reg() {
-    elem_registry()[ElemKey{sizeof(T) == 4, D, N, VOL, EPB, D * WPD * 64, BND ? 1 : 0, STAGE ? 1 : 0}] =
-        &launch_elem<T, D, N, VOL, EPB, WPD, BND, STAGE>;
+    const ElemKey key{sizeof(T) == 4, D, N, VOL, EPB, D * WPD * 64, BND ? 1 : 0, STAGE ? 1 : 0};
+    elem_registry()[key] = &launch_elem<T, D, N, VOL, EPB, WPD, BND, STAGE>;
+    elem_caps()[key] = ElemCaps{STAGE && stage_speed_fits<T, D, N, VOL, EPB>(),
+                                STAGE && stage_ind2d_fits<T, D, N, VOL, EPB, D * WPD * 64>()};
   }
 };
 

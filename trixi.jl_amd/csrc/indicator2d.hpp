@@ -21,7 +21,7 @@
 // Arithmetic up to the features is FP64 and in the operation order of
 // hg_indicator_kernel (the same factors).
 #pragma once
-#include "dg_aux.hpp"
+#include "dg_common.hpp"
 #include "mlp.hpp"
 
 namespace trixi_hip {
@@ -30,45 +30,58 @@ struct Vinv4 {
   double v[16];  // inverse_vandermonde_legendre, row-major [mode][node]
 };
 
-// FEAT = false: alpha_raw[e] = the HG factor. FEAT = true: the features as
-// well (into feat_out [E][32] when not null), and with net != 0
-// alpha_raw[e] = the network's clipped factor, or the HG factor where an input
-// leaves [lo, hi] or the output is not finite (fallback != 0); with net == 0
-// alpha_raw receives the HG factor. With FEAT, img, lo and hi must point to
-// device memory of their sizes even when net == 0 (they are read
-// unconditionally, so that no branch sits between the loads and their use).
-// alpha_hg_out (may be null) receives the HG factor in every case.
-template <typename T, bool FEAT>
-__global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
-    const T *__restrict__ u, int E, HGParams hp, Vinv4 vi, const mlp::InferImage *__restrict__ img,
-    const float *__restrict__ lo, const float *__restrict__ hi, int net, int fallback, double *__restrict__ alpha_raw,
-    double *__restrict__ alpha_hg_out, float *__restrict__ feat_out) {
+// Everything the 2D p3 indicator needs besides the state, shared by the
+// stand-alone kernel and the epilogue of the fused stage. FEAT = false:
+// alpha_raw[e] = the HG factor. FEAT = true: the features as well (into
+// feat_out [E][32] when not null), and with net != 0 alpha_raw[e] = the
+// network's clipped factor, or the HG factor where an input leaves [lo, hi]
+// or the output is not finite (fallback != 0); with net == 0 alpha_raw
+// receives the HG factor. With FEAT, img, lo and hi must point to device
+// memory of their sizes even when net == 0 (they are read unconditionally, so
+// that no branch sits between the loads and their use). alpha_hg_out (may be
+// null) receives the HG factor in every case.
+struct Ind2DArgs {
+  HGParams hp;
+  Vinv4 vi;
+  const mlp::InferImage *img;
+  const float *lo, *hi;
+  int net, fallback;
+  double *alpha_raw;
+  double *alpha_hg_out;
+  float *feat_out;
+};
+
+// One wavefront computes the indicator of the 16 elements e0 .. e0 + 15 (those
+// below E): lane l owns element l % 16 and node row j = l / 16; un holds the
+// lane's 16 state values (nodes 4 j .. 4 j + 3, four variables each). s_mx,
+// s_sq: 256 doubles each of LDS owned by the wavefront. PRELOAD: the network
+// operands are loaded into registers at entry (the stand-alone kernel, one
+// wavefront per SIMD, where latency is exposed); otherwise they are read where
+// the products use them (inside the element kernel, whose register budget
+// they would otherwise set). Both give the same result bit for bit.
+template <typename T, bool FEAT, bool PRELOAD = true>
+__device__ __forceinline__ void indicator_2d_p3_wave(const T *un, int e0, int E, const Ind2DArgs &ia, double *s_mx,
+                                                     double *s_sq) {
   constexpr int n = 4, V = 4;
-  __shared__ double s_mx[16 * 16];  // x-transformed rows [element][row j][mode i]
-  __shared__ double s_sq[16 * 16];  // squared modes [element][a + 4 j]
-  const int l = threadIdx.x;
+  const HGParams &hp = ia.hp;
+  const Vinv4 &vi = ia.vi;
+  const mlp::InferImage *__restrict__ img = ia.img;
+  const float *__restrict__ lo = ia.lo;
+  const float *__restrict__ hi = ia.hi;
+  const int net = ia.net, fallback = ia.fallback;
+  double *__restrict__ alpha_raw = ia.alpha_raw;
+  double *__restrict__ alpha_hg_out = ia.alpha_hg_out;
+  float *__restrict__ feat_out = ia.feat_out;
+  const int l = threadIdx.x & 63;
   const int el = l & 15, j = l >> 4;
-  const int e0 = blockIdx.x * 16;
   const int e = e0 + el;
   const bool valid = e < E;
-  const int ec = valid ? e : E - 1;  // lanes past the end compute on the last element, store nothing
-
-  // the lane's four nodes (i + 4 j): 16 contiguous values, loaded at once in
-  // 16-byte pieces before the (wave-uniform) choice of the indicator variable
-  using C16 = float4;
-  constexpr int NCH = n * V * (int)sizeof(T) / 16;
-  C16 raw[NCH];
-  {
-    const C16 *src = reinterpret_cast<const C16 *>(u + ((size_t)ec * 16 + 4 * j) * V);
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) raw[c] = src[c];
-  }
   // with the network: its operands and the lane's training-box bounds, issued
   // behind the state loads, so that their latency overlaps the modal
   // transform (one memory round trip for the whole kernel)
   mlp::WeightRegs wr;
   float lo_r[7], hi_r[7];
-  if constexpr (FEAT) {
+  if constexpr (FEAT && PRELOAD) {
     mlp::load_weight_regs(*img, wr);
 #pragma unroll
     for (int i = 0; i < n; ++i) {
@@ -81,7 +94,6 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
       hi_r[4 + i] = hi[16 + i];
     }
   }
-  const T *un = reinterpret_cast<const T *>(raw);
   double q[n];
 #pragma unroll
   for (int i = 0; i < n; ++i) {
@@ -192,7 +204,21 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
       mlp::Tile t;
       t.xb[0] = mlp::pack4(f[0], f[1], f[2], f[3]);
       t.xb[1] = j == 0 ? mlp::pack4(f[4], f[5], f[6], 0.f) : mlp::pack4(0.f, 0.f, 0.f, 0.f);
-      mlp::forward_layers_regs(wr, t);
+      if constexpr (PRELOAD) {
+        mlp::forward_layers_regs(wr, t);
+      } else {
+        mlp::forward_layers(*img, t);
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+          lo_r[i] = lo[4 * j + i];
+          hi_r[i] = hi[4 * j + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          lo_r[4 + i] = lo[16 + i];
+          hi_r[4 + i] = hi[16 + i];
+        }
+      }
       // the training box: every feature of the element inside [lo, hi]
       bool in_box = true;
 #pragma unroll
@@ -221,6 +247,26 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(
       }
     }
   }
+}
+
+template <typename T, bool FEAT>
+__global__ __launch_bounds__(64) void indicator_2d_p3_kernel(const T *__restrict__ u, int E, Ind2DArgs ia) {
+  constexpr int V = 4;
+  __shared__ double s_mx[16 * 16];  // x-transformed rows [element][row j][mode i]
+  __shared__ double s_sq[16 * 16];  // squared modes [element][a + 4 j]
+  const int l = threadIdx.x;
+  const int el = l & 15, j = l >> 4;
+  const int e0 = blockIdx.x * 16;
+  const int e = e0 + el;
+  const int ec = e < E ? e : E - 1;  // lanes past the end compute on the last element, store nothing
+  // the lane's four nodes (i + 4 j): 16 contiguous values, loaded at once in
+  // 16-byte pieces before the (wave-uniform) choice of the indicator variable
+  constexpr int NCH = 4 * V * (int)sizeof(T) / 16;
+  float4 raw[NCH];
+  const float4 *src = reinterpret_cast<const float4 *>(u + ((size_t)ec * 16 + 4 * j) * V);
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) raw[c] = src[c];
+  indicator_2d_p3_wave<T, FEAT>(reinterpret_cast<const T *>(raw), e0, E, ia, s_mx, s_sq);
 }
 
 }  // namespace trixi_hip

@@ -19,13 +19,17 @@
 #include "dg_aux.hpp"
 #include "elem_registry.hpp"
 #include "indicator2d.hpp"
-#include "mlp.hpp"
+#include "mlp_kernels.hpp"
 
 using namespace trixi_hip;
 
 namespace trixi_hip {
 std::map<ElemKey, ElemLaunch> &elem_registry() {
   static std::map<ElemKey, ElemLaunch> r;
+  return r;
+}
+std::map<ElemKey, ElemCaps> &elem_caps() {
+  static std::map<ElemKey, ElemCaps> r;
   return r;
 }
 }  // namespace trixi_hip
@@ -136,8 +140,19 @@ struct trixi_hip_ctx {
   // staging for trixi_hip_rhs (FP64 interface)
   double *d_hu = nullptr, *d_hdu = nullptr;
   float *d_su32 = nullptr, *d_sdu32 = nullptr;
-  // HG
+  // HG: raw factors (scratch of the RHS and indicator entry points) and the
+  // smoothed ones
   double *d_alpha_raw = nullptr, *d_alpha = nullptr;
+  // raw factors of state buffer par for the stages: a fused 2D p3 stage writes
+  // those of its new state into d_alpha_st[par ^ 1] (other workgroups still
+  // read d_alpha_st[par]); alpha_fresh: d_alpha_st[par] holds those of the
+  // resident state, so the next stage launches no indicator
+  double *d_alpha_st[2] = {nullptr, nullptr};
+  bool alpha_fresh = false;
+  // the fused 2D p3 stage computes the next stage's indicator in its epilogue
+  // instead of the stand-alone indicator_2d_p3_kernel reading the state again;
+  // TRIXI_HIP_FUSED_INDICATOR=0 turns it off
+  bool fused_ind = true;
   // reductions
   double *d_partial = nullptr, *d_scalars = nullptr;  // scalars: [0..1] {t, dt}, [2] max_dt, [3..] reductions
   size_t partial_cap = 0;
@@ -154,7 +169,17 @@ struct trixi_hip_ctx {
   ElemLaunch elem = nullptr, elem_stage = nullptr;
   int epb = 0, nt = 0, epb_st = 0, nt_st = 0;
   std::vector<unsigned char> kconst;  // DGConst<T, N> blob
-  int xcd_remap = 0;  // measured: the round-robin XCD placement is faster here
+  // workgroups of the element kernel walk contiguous element ranges per XCD
+  // (neighbour faces then mostly hit the same L2). FP32 default: 3D p3 RHS
+  // 36.7 -> 35.8 us, CK2N54 step 395.6 -> 382.6 us; FP64 is faster without it
+  // (RHS 108.2 vs 109.2 us). A/B on one box; TRIXI_HIP_XCD_REMAP=0/1 overrides.
+  int xcd_remap = 0;
+  // the last stage of a step reduces the wave speed in its epilogue (see
+  // stage_wave_speed) instead of max_speed_nodes_kernel reading the state again.
+  // FP32 default: 3D p3 CK2N54 step 382.6 -> 370.4 us. FP64 is slower with it
+  // (611.6 vs 623.2 us: at two workgroups per CU the epilogue's barriers are not
+  // hidden). A/B on one box; TRIXI_HIP_FUSED_SPEED=0/1 overrides.
+  bool fused_speed = false;
 
   // captured RK steps, keyed by scheme, stream, entry point and start parity
   // (two slots: consecutive steps of a scheme with an odd stage count alternate)
@@ -229,6 +254,8 @@ struct StageOut {
   float *u32_out = nullptr;
   double a_stage = 0, b_stage = 0;
   int first_stage = 0;
+  double *speed_partial = nullptr;  // the last stage of a step: wave-speed partials of the new state
+  bool ind_next = false;            // 2D p3 shock capturing: the next stage's raw factors too
 };
 
 #define HIP_TRY(call)                                                                            \
@@ -267,6 +294,17 @@ ElemLaunch find_launch(const trixi_hip_ctx *ctx, int stage, int epb, int nt) {
       ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, epb, nt, ctx->B + ctx->M > 0 ? 1 : 0, stage});
   return it == elem_registry().end() ? nullptr : it->second;
 }
+
+// What the context's fused-stage kernel can add in its epilogue: the wave
+// speed of the new state (the last stage of a step then replaces the
+// stand-alone max_speed_nodes_kernel) and the next stage's 2D p3 indicator.
+ElemCaps stage_caps(const trixi_hip_ctx *ctx) {
+  if (!ctx->elem_stage) return ElemCaps{};
+  auto it = elem_caps().find(ElemKey{ctx->fp32 ? 1 : 0, ctx->D, ctx->N, ctx->vol, ctx->epb_st, ctx->nt_st,
+                                     ctx->B + ctx->M > 0 ? 1 : 0, 1});
+  return it != elem_caps().end() ? it->second : ElemCaps{};
+}
+bool stage_speed_ok(const trixi_hip_ctx *ctx) { return stage_caps(ctx).speed; }
 
 // Default tile of a kernel variant: this process's autotuning, else the
 // measured table, else one wavefront per direction.
@@ -317,6 +355,25 @@ int select_tile(trixi_hip_ctx *ctx, int epb, int nt) {
   return TRIXI_HIP_OK;
 }
 
+// Arguments of the 2D p3 indicator (stand-alone kernel or fused-stage
+// epilogue): HG parameters, the inverse Vandermonde matrix, and with the
+// network (or features) its device image, box bounds and modes.
+Ind2DArgs ind2d_args(const trixi_hip_ctx *ctx, const HGParams &params, bool net, double *alpha_out, double *alpha_hg,
+                     float *feat) {
+  Ind2DArgs ia{};
+  ia.hp = params;
+  for (int i = 0; i < 16; ++i) ia.vi.v[i] = ctx->Vinv[i];
+  ia.img = ctx->d_mlp_img;
+  ia.lo = ctx->d_mlp ? ctx->d_mlp + 3 * mlp::NP : nullptr;
+  ia.hi = ctx->d_mlp ? ia.lo + mlp::NF : nullptr;
+  ia.net = net ? 1 : 0;
+  ia.fallback = ctx->ind_mode == TRIXI_HIP_IND_MLP_WITH_FALLBACK ? 1 : 0;
+  ia.alpha_raw = alpha_out;
+  ia.alpha_hg_out = alpha_hg;
+  ia.feat_out = feat;
+  return ia;
+}
+
 // Raw (unsmoothed) blending factors of state u into alpha_out: the HG factor,
 // or (2D p3, net = true) the learned indicator's, with the HG fallback. With
 // feat != null (2D p3) the learned indicator's features [E][32] too, and with
@@ -327,21 +384,15 @@ int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat, co
               bool net = false, double *alpha_hg = nullptr) {
   const HGParams params = hp ? *hp : ctx->hg;
   if (ctx->D == 2 && ctx->N == 3) {
-    Vinv4 vi{};
-    for (int i = 0; i < 16; ++i) vi.v[i] = ctx->Vinv[i];
     const int nb = (ctx->E + 15) / 16;  // one wavefront of 16 elements per workgroup
-    const int fallback = ctx->ind_mode == TRIXI_HIP_IND_MLP_WITH_FALLBACK ? 1 : 0;
+    Ind2DArgs ia = ind2d_args(ctx, params, net, alpha_out, alpha_hg, feat);
     if (feat || net) {
       // the network buffers exist whenever features or the network are asked
       // for (trixi_hip_mlp_features and trixi_hip_set_indicator allocate them)
       if (!ctx->d_mlp || !ctx->d_mlp_img) return set_err(ctx, TRIXI_HIP_ESTATE, "learned-indicator buffers missing");
-      const float *lo = ctx->d_mlp + 3 * mlp::NP, *hi = lo + mlp::NF;
-      hipLaunchKernelGGL((indicator_2d_p3_kernel<T, true>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, params,
-                         vi, ctx->d_mlp_img, lo, hi, net ? 1 : 0, fallback, alpha_out, alpha_hg, feat);
+      hipLaunchKernelGGL((indicator_2d_p3_kernel<T, true>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, ia);
     } else {
-      hipLaunchKernelGGL((indicator_2d_p3_kernel<T, false>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, params,
-                         vi, (const mlp::InferImage *)nullptr, (const float *)nullptr, (const float *)nullptr, 0, 0,
-                         alpha_out, alpha_hg, (float *)nullptr);
+      hipLaunchKernelGGL((indicator_2d_p3_kernel<T, false>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, ia);
     }
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
@@ -369,24 +420,37 @@ int mlp_blocks(int n) {
   return (waves + mlp::WAVES - 1) / mlp::WAVES;
 }
 
-// Raw (unsmoothed) blending factors of state u into d_alpha_raw, from the
+// Raw (unsmoothed) blending factors of state u into alpha_out, from the
 // indicator selected by ind_mode.
 // (features, network and fallback in one launch: indicator_2d_p3_kernel).
-template <typename T> int launch_indicator_raw(trixi_hip_ctx *ctx, const T *u) {
-  return launch_hg<T>(ctx, u, ctx->d_alpha_raw, nullptr, nullptr, ctx->ind_mode != TRIXI_HIP_IND_HG);
+template <typename T> int launch_indicator_raw(trixi_hip_ctx *ctx, const T *u, double *alpha_out) {
+  return launch_hg<T>(ctx, u, alpha_out, nullptr, nullptr, ctx->ind_mode != TRIXI_HIP_IND_HG);
+}
+
+bool is_hg(const trixi_hip_ctx *ctx) { return ctx->vol == VOL_HG_RANOCHA || ctx->vol == VOL_HG_CENTRAL; }
+
+// Whether the context's stages compute the next stage's indicator in their
+// epilogue.
+bool stage_fuses_indicator(const trixi_hip_ctx *ctx) {
+  return ctx->fused_ind && is_hg(ctx) && stage_caps(ctx).ind2d;
 }
 
 template <typename T>
 int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *tdev,
                  const StageOut *so = nullptr) {
-  // shock capturing: raw indicator first (needs the whole state), smoothing happens inline
-  const bool hg = ctx->vol == VOL_HG_RANOCHA || ctx->vol == VOL_HG_CENTRAL;
+  // shock capturing: raw indicator first (needs the whole state), smoothing
+  // happens inline. A stage reads the factors of state buffer par, computed by
+  // the previous fused stage when fresh.
+  const bool hg = is_hg(ctx);
+  double *alpha_in = so ? ctx->d_alpha_st[ctx->par] : ctx->d_alpha_raw;
   if (hg) {
-    int r = launch_indicator_raw<T>(ctx, u);
-    if (r) return r;
+    if (!(so && ctx->alpha_fresh)) {
+      int r = launch_indicator_raw<T>(ctx, u, alpha_in);
+      if (r) return r;
+    }
     if (ctx->M > 0 && ctx->cfg.alpha_smooth) {  // smoothing across interfaces and mortars, before the element kernel
       hipLaunchKernelGGL(hg_smooth_adj_kernel, dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
-                         ctx->d_alpha_raw, ctx->d_adj_ptr, ctx->d_adj, ctx->d_alpha, ctx->E);
+                         alpha_in, ctx->d_adj_ptr, ctx->d_adj, ctx->d_alpha, ctx->E);
       HIP_TRY(hipGetLastError());
     }
   }
@@ -442,7 +506,7 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   a.inv_jac = ctx->d_invj;
   a.centers = ctx->d_centers;
   const bool pre_smoothed = hg && ctx->M > 0 && ctx->cfg.alpha_smooth;  // smoothed across mortars above
-  a.alpha_raw = pre_smoothed ? ctx->d_alpha : ctx->d_alpha_raw;
+  a.alpha_raw = pre_smoothed ? ctx->d_alpha : alpha_in;
   a.alpha_out = pre_smoothed ? nullptr : ctx->d_alpha;
   a.tdev = tdev;
   a.t = t;
@@ -463,7 +527,14 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
     a.a_stage = so->a_stage;
     a.b_stage = so->b_stage;
     a.first_stage = so->first_stage;
+    a.speed_partial = so->speed_partial;
+    if (so->ind_next) {  // raw factors of the new state into the other buffer of the pair
+      a.ind_next = 1;
+      a.ind = ind2d_args(ctx, ctx->hg, ctx->ind_mode != TRIXI_HIP_IND_HG, ctx->d_alpha_st[ctx->par ^ 1], nullptr,
+                         nullptr);
+    }
   }
+  a.gamma64 = ctx->cfg.gamma;
   const int epb = so ? ctx->epb_st : ctx->epb;
   int nblocks = (ctx->E + epb - 1) / epb;
   hipError_t e = (so ? ctx->elem_stage : ctx->elem)(&a, ctx->kconst.data(), nblocks, ctx->stream);
@@ -509,9 +580,11 @@ int launch_update(trixi_hip_ctx *ctx, double a_stage, double b_stage, const doub
 // the element kernel's epilogue and the state moves to the other buffer of
 // its pair (par flips); otherwise the RHS writes du and rk2n_update_kernel
 // updates in place.
-int launch_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage, double t, const double *tdev, int first) {
+int launch_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage, double t, const double *tdev, int first,
+                 double *speed_partial = nullptr) {
   if (!ctx->elem_stage) {
     int r = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), t, tdev);
+    ctx->alpha_fresh = false;
     return r ? r : launch_update(ctx, a_stage, b_stage, tdev, first);
   }
   StageOut so;
@@ -522,10 +595,25 @@ int launch_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage, double t, c
   so.a_stage = a_stage;
   so.b_stage = b_stage;
   so.first_stage = first;
+  so.speed_partial = speed_partial;
+  so.ind_next = stage_fuses_indicator(ctx);
   int r = ctx->fp32 ? launch_rhs_T<float>(ctx, nullptr, (const float *)native_u(ctx), t, tdev, &so)
                     : launch_rhs_T<double>(ctx, nullptr, (const double *)native_u(ctx), t, tdev, &so);
   if (r) return r;
   ctx->par ^= 1;
+  ctx->alpha_fresh = so.ind_next;
+  return TRIXI_HIP_OK;
+}
+
+// Before a captured step is recorded or replayed: with the fused indicator the
+// graph's first stage reads the factors of the state it starts from, so they
+// are computed here when no fused stage left them.
+int refresh_alpha(trixi_hip_ctx *ctx) {
+  if (!stage_fuses_indicator(ctx) || ctx->alpha_fresh) return TRIXI_HIP_OK;
+  int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, (const float *)native_u(ctx), ctx->d_alpha_st[ctx->par])
+                    : launch_indicator_raw<double>(ctx, (const double *)native_u(ctx), ctx->d_alpha_st[ctx->par]);
+  if (r) return r;
+  ctx->alpha_fresh = true;
   return TRIXI_HIP_OK;
 }
 
@@ -739,7 +827,11 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   ctx->hg.threshold = 0.5 * std::pow(10.0, -1.8 * std::pow((double)n, 0.25));
   ctx->hg.parameter_s = std::log((1 - 0.0001) / 0.0001);
   ctx->hg.variable = cfg->hg_variable;
+  ctx->xcd_remap = cfg->precision == TRIXI_HIP_FP32 ? 1 : 0;
+  ctx->fused_speed = cfg->precision == TRIXI_HIP_FP32;
+  if (const char *env = std::getenv("TRIXI_HIP_FUSED_INDICATOR")) ctx->fused_ind = std::atoi(env) != 0;
   if (const char *env = std::getenv("TRIXI_HIP_XCD_REMAP")) ctx->xcd_remap = std::atoi(env) != 0;
+  if (const char *env = std::getenv("TRIXI_HIP_FUSED_SPEED")) ctx->fused_speed = std::atoi(env) != 0;
   ctx->exact.kind = TRIXI_HIP_IC_CONSTANT;
   ctx->exact.gamma = cfg->gamma;
 
@@ -765,6 +857,8 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
             (ctx->fp32 || dev_alloc(ctx, &ctx->d_ub[1], nv * 8) == hipSuccess) &&
             dev_alloc(ctx, &ctx->d_du, nv * tsz) == hipSuccess &&
             dev_alloc(ctx, &ctx->d_alpha_raw, E * 8) == hipSuccess && dev_alloc(ctx, &ctx->d_alpha, E * 8) == hipSuccess &&
+            dev_alloc(ctx, &ctx->d_alpha_st[0], E * 8) == hipSuccess &&
+            dev_alloc(ctx, &ctx->d_alpha_st[1], E * 8) == hipSuccess &&
             dev_alloc(ctx, &ctx->d_partial, ctx->partial_cap * 8) == hipSuccess &&
             dev_alloc(ctx, &ctx->d_scalars, 64 * 8) == hipSuccess;
   // face slots: boundaries, then mortar faces; ghost states of the mortar slots
@@ -830,6 +924,7 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
   void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_ub[0], ctx->d_ub[1], ctx->d_utmp,
                   ctx->d_u32b[0], ctx->d_u32b[1], ctx->d_du,
                   ctx->d_hu, ctx->d_hdu, ctx->d_su32, ctx->d_sdu32, ctx->d_alpha_raw, ctx->d_alpha,
+                  ctx->d_alpha_st[0], ctx->d_alpha_st[1],
                   ctx->d_partial, ctx->d_scalars, ctx->d_mlp, ctx->d_feat, ctx->d_mlp_partial,
                   ctx->d_alpha_hg, ctx->d_mlp_img, ctx->d_records, ctx->d_bx, ctx->d_ghost, ctx->d_mort,
                   ctx->d_mflux, ctx->d_bnd,
@@ -860,6 +955,7 @@ int trixi_hip_synchronize(trixi_hip_ctx *ctx) {
 
 int trixi_hip_upload_u(trixi_hip_ctx *ctx, const double *u_host) {
   if (!ctx || !u_host) return TRIXI_HIP_EINVAL;
+  ctx->alpha_fresh = false;
   HIP_TRY(hipMemcpyAsync(cur_u64(ctx), u_host, ctx->nvals * 8, hipMemcpyHostToDevice, ctx->stream));
   if (ctx->fp32) {
     int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
@@ -878,6 +974,7 @@ int trixi_hip_download_u(trixi_hip_ctx *ctx, double *u_host) {
 
 int trixi_hip_set_u_device(trixi_hip_ctx *ctx, const double *u_dev) {
   if (!ctx || !u_dev) return TRIXI_HIP_EINVAL;
+  ctx->alpha_fresh = false;
   HIP_TRY(hipMemcpyAsync(cur_u64(ctx), u_dev, ctx->nvals * 8, hipMemcpyDeviceToDevice, ctx->stream));
   if (ctx->fp32) {
     int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
@@ -953,6 +1050,8 @@ int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, cons
     return TRIXI_HIP_OK;
   };
   if (!use_graph) return record();
+  int r0 = refresh_alpha(ctx);  // the graph's first stage reads the current factors
+  if (r0) return r0;
   std::vector<double> key{-2.0, (double)nstages};
   for (int s = 0; s < nstages; ++s) {
     key.push_back(a[s]);
@@ -1220,12 +1319,18 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
     key.push_back(c[s]);
   }
   key.push_back((double)(uintptr_t)ctx->stream);
+  // the last stage reduces the wave speed of the new state in its epilogue
+  // when its variant can (else max_speed_nodes_kernel after it); both give
+  // the same partial maxima element by element
+  const bool fused_speed = ctx->fused_speed && stage_speed_ok(ctx) &&
+                           (size_t)((ctx->E + ctx->epb_st - 1) / ctx->epb_st) <= ctx->partial_cap;
   auto record = [&]() -> int {
     for (int s = 0; s < nstages; ++s) {
-      int rr = launch_stage(ctx, a[s], b[s], c[s], tdt, s == 0);
+      int rr = launch_stage(ctx, a[s], b[s], c[s], tdt, s == 0,
+                            fused_speed && s == nstages - 1 ? ctx->d_partial : nullptr);
       if (rr) return rr;
     }
-    const int nb = launch_max_speed(ctx);
+    const int nb = fused_speed ? (ctx->E + ctx->epb_st - 1) / ctx->epb_st : launch_max_speed(ctx);
     if (nb < 0) return nb;
     hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream, ctx->d_partial, nb, ctx->n, tdt,
                        ctl);
@@ -1247,6 +1352,7 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
   // kernels and their arguments are the same; record() leaves the parity
   // flipped as a replay does.
   const bool eager = max_steps <= kEagerSolveSteps;
+  if ((r = refresh_alpha(ctx))) return r;  // the step graphs' first stage reads the current factors
   if (!eager) {
     hipGraphExec_t x = nullptr;
     if ((r = step_graph(&x))) return r;
@@ -1311,7 +1417,8 @@ int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host) {
     return set_err(ctx, TRIXI_HIP_ESTATE, "the learned indicator has no parameters (trixi_hip_mlp_set_params)");
   const int saved = ctx->ind_mode;
   ctx->ind_mode = mode;
-  int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, cur_u32(ctx)) : launch_indicator_raw<double>(ctx, cur_u64(ctx));
+  int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, cur_u32(ctx), ctx->d_alpha_raw)
+                    : launch_indicator_raw<double>(ctx, cur_u64(ctx), ctx->d_alpha_raw);
   ctx->ind_mode = saved;
   if (r) return r;
   bool ok = true;
@@ -1352,6 +1459,7 @@ static int mlp_check(trixi_hip_ctx *ctx) {
 
 // Refresh the BF16 inference image after the parameters changed.
 static int mlp_pack(trixi_hip_ctx *ctx) {
+  ctx->alpha_fresh = false;  // the learned factors change with the parameters
   hipLaunchKernelGGL(mlp::mlp_pack_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->d_mlp, ctx->d_mlp_img);
   HIP_TRY(hipGetLastError());
   return TRIXI_HIP_OK;
@@ -1394,6 +1502,7 @@ int trixi_hip_set_indicator(trixi_hip_ctx *ctx, int mode) {
     HIP_TRY(host_sync(ctx));
     drop_graphs(ctx);
   }
+  if (mode != ctx->ind_mode) ctx->alpha_fresh = false;
   ctx->ind_mode = mode;
   return TRIXI_HIP_OK;
 }
@@ -1430,8 +1539,9 @@ static int mlp_gradient(trixi_hip_ctx *ctx, const float *x_dev, const float *tar
   hipLaunchKernelGGL(mlp::mlp_grad_kernel, dim3(nb), dim3(mlp::NTHR), 0, ctx->stream, ctx->d_mlp, x_dev, target_dev,
                      n, ctx->d_mlp_partial);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(mlp::mlp_reduce_kernel, dim3((mlp::NPL + 255) / 256), dim3(256), 0, ctx->stream,
-                     ctx->d_mlp_partial, nb * mlp::WAVES, mlp_grad_buffer(ctx));
+  hipLaunchKernelGGL(mlp::mlp_reduce_kernel, dim3((mlp::NPL + mlp::RED_COLS - 1) / mlp::RED_COLS),
+                     dim3(mlp::RED_COLS * mlp::RED_SPLIT), 0, ctx->stream, ctx->d_mlp_partial, nb * mlp::WAVES,
+                     mlp_grad_buffer(ctx));
   HIP_TRY(hipGetLastError());
   return TRIXI_HIP_OK;
 }
@@ -1527,6 +1637,7 @@ int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *ki
     if (!ok) return set_err(to, TRIXI_HIP_EINVAL, "transfer map entry out of range");
   }
   const int n = to->n, E = to->E;
+  to->alpha_fresh = false;
   std::vector<double> mats;
   for (const std::vector<double> *m : {&to->mfu, &to->mfl, &to->mru, &to->mrl}) {
     if ((int)m->size() != n * n) return set_err(to, TRIXI_HIP_EINVAL, "mortar matrices missing");
