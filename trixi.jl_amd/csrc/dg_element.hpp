@@ -677,8 +677,17 @@ __device__ __forceinline__ void stage_indicator_2d(const ElemArgs<T> &a, const d
 // issued before the first one is consumed: neighbour ids, the tile (in chunks,
 // into registers), the per-element scalars, then the neighbour face states,
 // which stay in flight until phase B.
+// Minimum wavefronts per SIMD the register allocation must allow. The FP64 3D
+// polydeg-3 flux_ranocha variants are held to 168 VGPRs (from 184, some values
+// spilled): 3 instead of 2 waves per SIMD, RHS 108.9 -> 102.4 us and CK2N54
+// step 613.5 -> 574.7 us on C2 (A/B on one box); no constraint elsewhere.
+template <typename T, int D, int N, int VOL> constexpr int min_waves_per_simd() {
+  return sizeof(T) == 8 && D == 3 && N == 3 && VOL == VOL_FD_RANOCHA ? 3 : 1;
+}
+
 template <typename T, int D, int N, int VOL, int EPB, int WPD, bool BND, bool STAGE>
-__global__ __launch_bounds__(D *WPD * 64) void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
+__global__ __launch_bounds__(D *WPD * 64) __attribute__((amdgpu_waves_per_eu(min_waves_per_simd<T, D, N, VOL>())))
+void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
   using LY = Layout<D, N, EPB>;
   using VT = VolTraits<VOL>;
   using TIO = TileIO<T, D, N, EPB>;
