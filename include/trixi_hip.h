@@ -363,6 +363,54 @@ typedef struct trixi_hip_tile_timing {
 int trixi_hip_autotune(trixi_hip_ctx *ctx, int repetitions, trixi_hip_tile_timing *timings, int max_timings,
                        int *ntimings);
 
+/* Host-side TreeMesh adaptation and connectivity (no device work; callable
+ * without a GPU). A tree is held as its leaves in element (depth-first)
+ * order: level, integer coordinates at that level ([n][ndims], 0 <= c <
+ * 2^level, level <= 19) and centre ([n][ndims]).
+ *   trixi_hip_tree_create   the leaves of a TreeMesh (tree_mesh.jl:128-185);
+ *                           each leaf's origin starts as itself (kind 0, src i)
+ *   trixi_hip_tree_refine   refine! of the marked leaves with rebalance!
+ *                           (abstract_tree.jl:367-393, 453-490); *nrefined =
+ *                           cells refined, rebalancing included
+ *   trixi_hip_tree_coarsen  the AMR callback's coarsening (amr.jl:304-370)
+ *                           with the balance guard of coarsen!
+ *                           (abstract_tree.jl:516-600), highest level first:
+ *                           a parent is coarsened when all its 2^ndims
+ *                           children are among the candidate leaves (levels,
+ *                           coordinates as of before this adaptation's
+ *                           refinement) and no child has a non-sibling face
+ *                           neighbour with children
+ *   trixi_hip_tree_sizes    {leaves, interfaces, mortars, boundaries}
+ *   trixi_hip_tree_leaves   the leaves and their origin in this tree's
+ *                           adaptations: kind 0 = old leaf src, 1 + q = child
+ *                           q of old leaf src, -1 = parent of old leaves src ..
+ *                           src + 2^ndims - 1 (trixi_hip_transfer's map)
+ *   trixi_hip_tree_connectivity  init_interfaces!, init_mortars!,
+ *                           init_boundaries! (containers_2d.jl:228-290,
+ *                           475-543, 707-800; containers_3d.jl:229-279,
+ *                           391-480, 652-788), 0-based: element neighbours
+ *                           [n][2 ndims] (-1: none or boundary), interfaces
+ *                           [I][2] + orientations, mortars [M][2^(ndims-1) +
+ *                           1] (small elements, large element) + large sides
+ *                           + orientations, boundaries + orientations +
+ *                           neighbour sides; NULL outputs are skipped.
+ * create returns NULL for invalid arguments. */
+typedef struct trixi_hip_tree trixi_hip_tree;
+trixi_hip_tree *trixi_hip_tree_create(int ndims, const int *periodicity, double length_level_0, int nleaves,
+                                      const int32_t *levels, const int64_t *coordinates, const double *centers);
+void trixi_hip_tree_destroy(trixi_hip_tree *t);
+const char *trixi_hip_tree_last_error(const trixi_hip_tree *t);
+int trixi_hip_tree_refine(trixi_hip_tree *t, const uint8_t *marked, int64_t *nrefined);
+int trixi_hip_tree_coarsen(trixi_hip_tree *t, int ncandidates, const int32_t *levels, const int64_t *coordinates,
+                           int64_t *ncoarsened);
+int trixi_hip_tree_sizes(trixi_hip_tree *t, int64_t *sizes);
+int trixi_hip_tree_leaves(const trixi_hip_tree *t, int32_t *levels, int64_t *coordinates, double *centers,
+                          int32_t *kind, int32_t *src);
+int trixi_hip_tree_connectivity(trixi_hip_tree *t, int32_t *element_neighbors, int32_t *neighbor_ids,
+                                int32_t *orientations, int32_t *mortar_neighbor_ids, int32_t *mortar_large_sides,
+                                int32_t *mortar_orientations, int32_t *boundary_neighbor_ids,
+                                int32_t *boundary_orientations, int32_t *boundary_neighbor_sides);
+
 #ifdef __cplusplus
 }
 #endif

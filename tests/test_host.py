@@ -262,3 +262,35 @@ def test_amr_adapt_matches_oracle(D, per, patches):
         np.testing.assert_array_equal(kind, okind)
         np.testing.assert_array_equal(src, osrc)
         _compare_mesh_with_oracle(mesh, o, N)
+
+
+def test_native_tree_refuses_invalid_input():
+    """trixi_hip_tree_*: leaves outside their level's range and refinement
+    beyond level 19 are refused with an error instead of corrupting the keys."""
+    from trixi_hip.mesh import _Leaves, _NativeTree
+    one = lambda lv, ic: _Leaves(np.array([lv]), np.array([ic], dtype=np.int64), np.zeros((1, 2)),
+                                 np.zeros(1, np.int32), np.zeros(1, np.int32))
+    with pytest.raises(ValueError):
+        _NativeTree(one(2, (4, 0)), 2, 1.0, (True, True))  # coordinate 4 at level 2
+    with pytest.raises(ValueError):
+        _NativeTree(one(20, (0, 0)), 2, 1.0, (True, True))
+    nt = _NativeTree(one(19, (0, 0)), 2, 1.0, (True, True))
+    with pytest.raises(ValueError, match="level 19"):
+        nt.refine(np.array([True]))
+    nt = _NativeTree(one(0, (0, 0)), 2, 1.0, (False, True))
+    assert nt.refine(np.array([True])) == 1
+    t = nt.leaves()
+    np.testing.assert_array_equal(t.lv, [1, 1, 1, 1])
+    np.testing.assert_array_equal(t.ic, [[0, 0], [1, 0], [0, 1], [1, 1]])
+    np.testing.assert_array_equal(t.kind, [1, 2, 3, 4])
+    np.testing.assert_array_equal(t.src, [0, 0, 0, 0])
+    c = nt.connectivity()
+    # x bounded, y periodic: two x interfaces, four y interfaces (wrapping), four x boundaries
+    assert len(c["orientations"]) == 6 and len(c["boundary_neighbor_ids"]) == 4
+    np.testing.assert_array_equal(c["boundary_orientations"], [0, 0, 0, 0])
+    np.testing.assert_array_equal(c["boundary_neighbor_ids"], [0, 2, 1, 3])
+    np.testing.assert_array_equal(c["boundary_neighbor_sides"], [2, 2, 1, 1])
+    assert nt.coarsen(t.lv, t.ic) == 1
+    t = nt.leaves()
+    np.testing.assert_array_equal(t.lv, [0])
+    assert (t.kind[0], t.src[0]) == (-1, 0)

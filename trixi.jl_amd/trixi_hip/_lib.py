@@ -163,6 +163,14 @@ SIGNATURES = [
     ("trixi_hip_mlp_forward", _I, [_P, _P, _I, _P]),
     ("trixi_hip_mlp_loss_grad", _I, [_P, _P, _P, _I, _P, _P]),
     ("trixi_hip_mlp_train_step", _I, [_P, _P, _P, _I, _F, _P]),
+    ("trixi_hip_tree_create", _P, [_I, _P, _D, _I, _P, _P, _P]),
+    ("trixi_hip_tree_destroy", None, [_P]),
+    ("trixi_hip_tree_last_error", ctypes.c_char_p, [_P]),
+    ("trixi_hip_tree_refine", _I, [_P, _P, _P]),
+    ("trixi_hip_tree_coarsen", _I, [_P, _I, _P, _P, _P]),
+    ("trixi_hip_tree_sizes", _I, [_P, _P]),
+    ("trixi_hip_tree_leaves", _I, [_P, _P, _P, _P, _P, _P]),
+    ("trixi_hip_tree_connectivity", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 ]
 
 class TileTiming(ctypes.Structure):

@@ -1,8 +1,9 @@
 """In-tree build of libtrixi_hip.so for gfx950 (hipcc, no JIT cache).
 
 Every csrc/*.hip translation unit is compiled to an object in parallel (the
-element-kernel instantiations are spread over elem_*.hip), then linked into
-one shared library."""
+element-kernel instantiations are spread over elem_*.hip), the host-only
+csrc/*.cpp units with the host C++ compiler, then all are linked into one
+shared library."""
 import os
 import subprocess
 import sys
@@ -20,19 +21,25 @@ HIPCC_FLAGS = [
 ]
 
 
+CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall"]
+
+
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def build_library(verbose=False, extra_flags=(), jobs=None):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     os.makedirs(OBJ_DIR, exist_ok=True)
     srcs = sources()
-    objs = [os.path.join(OBJ_DIR, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    objs = [os.path.join(OBJ_DIR, os.path.splitext(os.path.basename(s))[0] + ".o") for s in srcs]
 
     def compile_one(so):
         src, obj = so
-        cmd = [hipcc, *HIPCC_FLAGS, *extra_flags, "-c", "-o", obj, src]
+        if src.endswith(".cpp"):  # host code only
+            cmd = [os.environ.get("CXX", "g++"), *CXX_FLAGS, "-c", "-o", obj, src]
+        else:
+            cmd = [hipcc, *HIPCC_FLAGS, *extra_flags, "-c", "-o", obj, src]
         if verbose:
             print(" ".join(cmd), flush=True)
         return subprocess.run(cmd, capture_output=True, text=True)
@@ -43,7 +50,7 @@ def build_library(verbose=False, extra_flags=(), jobs=None):
     for src, r in zip(srcs, results):
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"hipcc failed for {os.path.basename(src)}")
+            raise RuntimeError(f"compiling {os.path.basename(src)} failed")
     tmp = LIB + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

@@ -20,8 +20,15 @@ Restates the parts of Trixi.jl that fix element order and geometry:
       balance); init_mortars! containers_3d.jl:652-788, containers_2d.jl:700-800
   AMR adaptation of the tree (refine! / rebalance! / coarsen!,
       abstract_tree.jl:367-600; the AMRCallback's selection, amr.jl:244-392)
+
+Refinement, coarsening and the connectivity of refined trees run in native
+code (trixi_hip_tree_*, csrc/tree_host.cpp).
 """
+import ctypes
+
 import numpy as np
+
+from . import _lib
 
 
 class TreeMesh:
@@ -145,14 +152,12 @@ class TreeMesh:
         """Leaves of the uniform tree refined by the box patches (cells whose
         centre lies strictly inside), rebalanced 2:1, in depth-first order, and
         their interfaces, L2 mortars and boundaries."""
-        D = self.ndims
-        t = self._uniform_tree()
+        nt = _NativeTree(self._uniform_tree(), self.ndims, self.length_level_0, self.periodicity)
         for pmin, pmax in self.refinement_patches:
-            c = t.centers
-            marked = np.all((np.array(pmin)[None, :] < c) & (np.array(pmax)[None, :] > c), axis=1)
-            t, _ = _refine_balanced(t, marked, D, self.length_level_0, self.periodicity)
-        self._tree = t
-        self._connect()
+            c = nt.leaves().centers
+            nt.refine(np.all((np.array(pmin)[None, :] < c) & (np.array(pmax)[None, :] > c), axis=1))
+        self._tree = nt.leaves()
+        self._connect(nt)
 
     def _uniform_tree(self):
         ic = np.array(self._icoords, dtype=np.int64).reshape(-1, self.ndims)
@@ -160,76 +165,35 @@ class TreeMesh:
         return _Leaves(np.full(n, self.initial_refinement_level, np.int64), ic, self._uniform_centers.copy(),
                        np.zeros(n, np.int32), np.arange(n, dtype=np.int32))
 
-    def _connect(self):
+    def _connect(self, nt=None):
         """Element geometry, interfaces, L2 mortars and boundaries of the leaves
         (init_interfaces!, init_mortars!, init_boundaries! of containers_{2,3}d.jl):
         a same-level leaf across a face is an interface neighbour; a same-level
         cell with children makes a mortar with this element as the large side;
         no same-level cell means this element is a small element of the coarser
-        neighbour's mortar; outside a non-periodic domain the face is a boundary."""
-        D, per = self.ndims, self.periodicity
+        neighbour's mortar; outside a non-periodic domain the face is a boundary.
+        nt: a native tree over the same leaves (one is made when None)."""
+        D = self.ndims
         t = self._tree
-        E = len(t.lv)
-        cells = _Cells(t.lv, t.ic)
-        self.nelements = E
+        if nt is None:
+            nt = _NativeTree(t, D, self.length_level_0, self.periodicity)
+        c = nt.connectivity()
+        self.nelements = len(t.lv)
         self.cell_centers = t.centers.copy()
         self.levels = t.lv.astype(np.int32)
         self.cell_lengths = self.length_level_0 / (1 << self.levels).astype(np.float64)
-        ns = 1 << (D - 1)
-        nbr = np.full((E, 2 * D), -1, dtype=np.int32)
-        boundary = np.zeros((E, 2 * D), dtype=bool)
-        inner_nb = np.zeros((E, 2 * D), dtype=bool)
-        nb_ic = np.zeros((2 * D, E, D), dtype=np.int64)
-        for dirn in range(2 * D):
-            nb, valid = _shift(t.lv, t.ic, dirn, per)
-            boundary[:, dirn] = ~valid
-            found, val = cells.find(t.lv, nb)
-            leaf = valid & found & (val >= 0)
-            nbr[leaf, dirn] = val[leaf]
-            inner_nb[:, dirn] = valid & found & (val < 0)
-            nb_ic[dirn] = nb
-        # mortars in element order, then direction order
-        m_el, m_dir = np.nonzero(inner_nb)
-        M = len(m_el)
-        m_ids = np.zeros((M, ns + 1), dtype=np.int32)
-        m_ids[:, ns] = m_el
-        o = m_dir // 2
-        plus = m_dir % 2
-        nbm = nb_ic[m_dir, m_el]  # [M, D] same-level neighbour cell (has children)
-        lvm = t.lv[m_el] + 1
-        free = np.array([[d for d in range(D) if d != oo] for oo in range(D)], dtype=np.int64)[o]
-        rows = np.arange(M)
-        for q in range(ns):
-            ch = 2 * nbm
-            ch[rows, o] += 1 - plus
-            ch[rows, free[:, 0]] += q & 1
-            if D == 3:
-                ch[rows, free[:, 1]] += q >> 1
-            found, val = cells.find(lvm, ch)
-            if not np.all(found & (val >= 0)):
-                raise ValueError("the tree is not 2:1 balanced")
-            m_ids[:, q] = val
-        self.element_neighbors = nbr
-        has = nbr[:, 1::2] >= 0
-        el, oi = np.nonzero(has)
-        self.neighbor_ids = np.stack([el, nbr[el, 2 * oi + 1]], axis=1).astype(np.int32).reshape(-1, 2)
-        self.orientations = oi.astype(np.int32)
-        self.ninterfaces = len(oi)
-        self.mortar_neighbor_ids = m_ids
-        self.mortar_large_sides = np.where(plus == 1, 1, 2).astype(np.int32)
-        self.mortar_orientations = o.astype(np.int32)
-        self.nmortars = M
-        b_el, b_dir = [], []
-        for dirn in range(2 * D):
-            ids = np.nonzero(boundary[:, dirn])[0]
-            b_el.append(ids)
-            b_dir.append(np.full(len(ids), dirn))
-        b_el = np.concatenate(b_el).astype(np.int32)
-        b_dir = np.concatenate(b_dir).astype(np.int32)
-        self.boundary_neighbor_ids = b_el
-        self.boundary_orientations = (b_dir // 2).astype(np.int32)
-        self.boundary_neighbor_sides = np.where(b_dir % 2 == 1, 1, 2).astype(np.int32)
-        self.nboundaries = len(b_el)
+        self.element_neighbors = c["element_neighbors"]
+        self.neighbor_ids = c["neighbor_ids"]
+        self.orientations = c["orientations"]
+        self.ninterfaces = len(self.orientations)
+        self.mortar_neighbor_ids = c["mortar_neighbor_ids"]
+        self.mortar_large_sides = c["mortar_large_sides"]
+        self.mortar_orientations = c["mortar_orientations"]
+        self.nmortars = len(self.mortar_orientations)
+        self.boundary_neighbor_ids = c["boundary_neighbor_ids"]
+        self.boundary_orientations = c["boundary_orientations"]
+        self.boundary_neighbor_sides = c["boundary_neighbor_sides"]
+        self.nboundaries = len(self.boundary_neighbor_ids)
 
     def adapt(self, lam, only_refine=False, only_coarsen=False):
         """One AMR adaptation of the tree (AMRCallback on a TreeMesh,
@@ -246,22 +210,20 @@ class TreeMesh:
         lam = np.asarray(lam)
         if lam.shape != (self.nelements,):
             raise ValueError("one controller value per element")
-        D = self.ndims
         t = self._tree if self._tree is not None else self._uniform_tree()
-        t = _Leaves(t.lv, t.ic, t.centers, np.zeros(len(t.lv), np.int32), np.arange(len(t.lv), dtype=np.int32))
+        nt = _NativeTree(t, self.ndims, self.length_level_0, self.periodicity)  # origins: kind 0, src = index
         coarsen_lv, coarsen_ic = t.lv[lam < 0], t.ic[lam < 0]
         changed = 0
         if not only_coarsen and np.any(lam > 0):
-            t, c = _refine_balanced(t, lam > 0, D, self.length_level_0, self.periodicity)
-            changed += c
+            changed += nt.refine(lam > 0)
         if not only_refine and len(coarsen_lv):
-            t, c = _coarsen(t, coarsen_lv, coarsen_ic, D, self.length_level_0, self.periodicity)
-            changed += c
+            changed += nt.coarsen(coarsen_lv, coarsen_ic)
+        t = nt.leaves()
         if changed:
             self._tree = t
-            self._connect()
+            self._connect(nt)
             self.unsaved_changes = True
-        return bool(changed), t.kind.astype(np.int32), t.src.astype(np.int32)
+        return bool(changed), t.kind, t.src
 
     def tree_arrays(self):
         """The tree as `save_mesh_file` writes it (mesh_io.jl:15-51): every cell in
@@ -359,8 +321,6 @@ class _Leaves:
     def __init__(self, lv, ic, centers, kind, src):
         self.lv, self.ic, self.centers, self.kind, self.src = lv, ic, centers, kind, src
 
-    def take(self, idx):
-        return _Leaves(self.lv[idx], self.ic[idx], self.centers[idx], self.kind[idx], self.src[idx])
 
 
 _KEY_SHIFT = (6, 25, 44)  # level in bits 0-5, coordinate d in 19 bits from _KEY_SHIFT[d]
@@ -374,8 +334,9 @@ def _keys(lv, ic):
 
 
 class _Cells:
-    """Every cell of the tree by key (level, integer coordinates): a leaf maps
-    to its element index, an inner cell (one with children) to -1."""
+    """Every cell of the tree by key (level, integer coordinates), sorted: a
+    leaf maps to its element index, an inner cell (one with children) to -1
+    (the mesh file's tree walk, tree_arrays)."""
 
     def __init__(self, lv, ic):
         keys, vals = [_keys(lv, ic)], [np.arange(len(lv), dtype=np.int64)]
@@ -390,124 +351,78 @@ class _Cells:
         o = np.argsort(k, kind="stable")
         self.k, self.v = k[o], v[o]
 
-    def find(self, lv, ic):
-        """(found, value) for every query cell; value -2 where absent."""
-        q = _keys(lv, ic)
-        if len(self.k) == 0:
-            return np.zeros(len(q), bool), np.full(len(q), -2, np.int64)
-        pos = np.minimum(np.searchsorted(self.k, q), len(self.k) - 1)
-        found = self.k[pos] == q
-        return found, np.where(found, self.v[pos], -2)
 
-    def value(self, lv, ic):
-        f, v = self.find(np.array([lv]), np.array([ic], dtype=np.int64).reshape(1, -1))
-        return int(v[0]) if f[0] else None
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def _shift(lv, ic, dirn, per):
-    """Same-level cell coordinates across face dirn (2o = -o, 2o+1 = +o),
-    wrapping periodic dimensions; valid is False outside a non-periodic domain."""
-    o = dirn // 2
-    nside = np.left_shift(1, lv)
-    k = ic[:, o] + (1 if dirn % 2 else -1)
-    outside = (k < 0) | (k >= nside)
-    nb = ic.copy()
-    nb[:, o] = np.mod(k, nside)
-    valid = np.ones(len(lv), bool) if per[o] else ~outside
-    return nb, valid
+class _NativeTree:
+    """A native tree (trixi_hip_tree) over the leaves of a _Leaves: refine
+    with rebalancing, coarsen, leaves with their origin, connectivity."""
 
+    def __init__(self, t, D, length0, per):
+        self._lib = _lib.load()
+        self.D = D
+        n = len(t.lv)
+        lv = np.ascontiguousarray(t.lv, dtype=np.int32)
+        ic = np.ascontiguousarray(np.asarray(t.ic, dtype=np.int64).reshape(n, D))
+        c = np.ascontiguousarray(np.asarray(t.centers, dtype=np.float64).reshape(n, D))
+        p = (ctypes.c_int * D)(*[int(bool(x)) for x in per])
+        self.h = self._lib.trixi_hip_tree_create(D, p, float(length0), n, _ptr(lv), _ptr(ic), _ptr(c))
+        if not self.h:
+            raise ValueError("invalid tree leaves (levels 0..19, coordinates inside their level)")
 
-def _refine(t, marked, D, length0):
-    """refine_unbalanced! (abstract_tree.jl:317-353): each marked leaf is
-    replaced in place by its children (bit d of the child index = upper half in d)."""
-    nch = 1 << D
-    E = len(t.lv)
-    counts = np.where(marked, nch, 1)
-    idx = np.repeat(np.arange(E), counts)
-    start = np.cumsum(counts) - counts
-    q = np.arange(len(idx)) - start[idx]
-    child = marked[idx]
-    bits = (q[:, None] >> np.arange(D)[None, :]) & 1
-    lv = t.lv[idx] + child
-    ic = np.where(child[:, None], 2 * t.ic[idx] + bits, t.ic[idx])
-    child_length = length0 / (1 << t.lv[idx]).astype(np.float64) / 2
-    c = np.where(child[:, None], t.centers[idx] + np.where(bits == 1, 1.0, -1.0) * (child_length / 2)[:, None],
-                 t.centers[idx])
-    kind = np.where(child, 1 + q, t.kind[idx]).astype(np.int32)
-    return _Leaves(lv, ic, c, kind, t.src[idx].copy())
+    def __del__(self):
+        h, self.h = getattr(self, "h", None), None
+        if h:
+            self._lib.trixi_hip_tree_destroy(h)
 
+    def _check(self, rc, what):
+        if rc != _lib.OK:
+            raise ValueError(f"{what}: {self._lib.trixi_hip_tree_last_error(self.h).decode()}")
 
-def _refine_balanced(t, marked, D, length0, per):
-    """refine! + rebalance! (abstract_tree.jl:367-400, 453-490): after each
-    round, the coarse neighbour (the parent's face neighbour) of a refined cell
-    without a same-level face neighbour is refined, until nothing changes."""
-    count = 0
-    marked = np.asarray(marked, bool)
-    while marked.any():
-        r_lv, r_ic = t.lv[marked], t.ic[marked]
-        count += int(marked.sum())
-        t = _refine(t, marked, D, length0)
-        cells = _Cells(t.lv, t.ic)
-        marked = np.zeros(len(t.lv), bool)
-        for dirn in range(2 * D):
-            nb, valid = _shift(r_lv, r_ic, dirn, per)
-            found, _ = cells.find(r_lv, nb)
-            cand = ~(valid & found) & (r_lv > 0)
-            if not cand.any():
-                continue
-            plv, pic = r_lv[cand] - 1, r_ic[cand] >> 1
-            pnb, pvalid = _shift(plv, pic, dirn, per)
-            f2, v2 = cells.find(plv, pnb)
-            sel = pvalid & f2 & (v2 >= 0)
-            marked[v2[sel]] = True
-    return t, count
+    def refine(self, marked):
+        m = np.ascontiguousarray(np.asarray(marked, dtype=bool).astype(np.uint8))
+        c = ctypes.c_int64()
+        self._check(self._lib.trixi_hip_tree_refine(self.h, _ptr(m), ctypes.byref(c)), "refine")
+        return int(c.value)
 
+    def coarsen(self, lv, ic):
+        lv = np.ascontiguousarray(lv, dtype=np.int32)
+        ic = np.ascontiguousarray(np.asarray(ic, dtype=np.int64).reshape(len(lv), self.D))
+        c = ctypes.c_int64()
+        self._check(self._lib.trixi_hip_tree_coarsen(self.h, len(lv), _ptr(lv), _ptr(ic), ctypes.byref(c)),
+                    "coarsen")
+        return int(c.value)
 
-def _coarsen(t, keys_lv, keys_ic, D, length0, per):
-    """Coarsening of the AMR callback (amr.jl:304-370) with the balance guard
-    of coarsen! (abstract_tree.jl:516-600), highest level first (within one
-    level the decisions do not depend on each other)."""
-    nch = 1 << D
-    cells = _Cells(t.lv, t.ic)
-    found, val = cells.find(keys_lv, keys_ic)
-    ok = (keys_lv > 0) & found & (val >= 0)  # refined by rebalancing: no longer a leaf
-    plv, pic = keys_lv[ok] - 1, keys_ic[ok] >> 1
-    pk = _keys(plv, pic)
-    uniq, first, cnt = np.unique(pk, return_index=True, return_counts=True)
-    full = first[cnt == nch]
-    plv, pic = plv[full], pic[full]
-    total = 0
-    for level in sorted(set(plv.tolist()), reverse=True):
-        at = plv == level
-        P_lv, P_ic = plv[at], pic[at]
-        skip = np.zeros(len(P_lv), bool)
-        for q in range(nch):
-            bits = np.array([(q >> d) & 1 for d in range(D)], dtype=np.int64)
-            ch = 2 * P_ic + bits[None, :]
-            for dirn in range(2 * D):
-                bit = (q >> (dirn // 2)) & 1
-                if bit == (1 if dirn % 2 == 0 else 0):
-                    continue  # the neighbour is a sibling
-                nb, valid = _shift(P_lv + 1, ch, dirn, per)
-                f, v = cells.find(P_lv + 1, nb)
-                skip |= valid & f & (v < 0)
-        done = _keys(P_lv[~skip], P_ic[~skip])
-        if len(done) == 0:
-            continue
-        total += len(done)
-        # collapse the children of the coarsened parents (consecutive, child 0 first)
-        par = np.where(t.lv > 0, _keys(t.lv - 1, t.ic >> 1), -1)
-        inpar = (t.lv > 0) & np.isin(par, done)
-        first_child = inpar & np.all((t.ic & 1) == 0, axis=1)
-        keep = ~inpar | first_child
-        h = length0 / (1 << t.lv).astype(np.float64)
-        lv = np.where(first_child, t.lv - 1, t.lv)
-        ic = np.where(first_child[:, None], t.ic >> 1, t.ic)
-        c = np.where(first_child[:, None], t.centers + (h / 2)[:, None], t.centers)
-        kind = np.where(first_child, -1, t.kind).astype(np.int32)
-        t = _Leaves(lv, ic, c, kind, t.src).take(keep)
-        cells = _Cells(t.lv, t.ic)
-    return t, total
+    def sizes(self):
+        s = np.zeros(4, dtype=np.int64)
+        self._check(self._lib.trixi_hip_tree_sizes(self.h, _ptr(s)), "tree sizes")
+        return [int(v) for v in s]
+
+    def leaves(self):
+        n = self.sizes()[0]
+        lv = np.empty(n, np.int32)
+        ic = np.empty((n, self.D), np.int64)
+        c = np.empty((n, self.D), np.float64)
+        kind = np.empty(n, np.int32)
+        src = np.empty(n, np.int32)
+        self._check(self._lib.trixi_hip_tree_leaves(self.h, _ptr(lv), _ptr(ic), _ptr(c), _ptr(kind), _ptr(src)),
+                    "tree leaves")
+        return _Leaves(lv.astype(np.int64), ic, c, kind, src)
+
+    def connectivity(self):
+        n, ni, nm, nb = self.sizes()
+        D, ns = self.D, 1 << (self.D - 1)
+        out = dict(element_neighbors=np.empty((n, 2 * D), np.int32), neighbor_ids=np.empty((ni, 2), np.int32),
+                   orientations=np.empty(ni, np.int32), mortar_neighbor_ids=np.empty((nm, ns + 1), np.int32),
+                   mortar_large_sides=np.empty(nm, np.int32), mortar_orientations=np.empty(nm, np.int32),
+                   boundary_neighbor_ids=np.empty(nb, np.int32), boundary_orientations=np.empty(nb, np.int32),
+                   boundary_neighbor_sides=np.empty(nb, np.int32))
+        order = ("element_neighbors", "neighbor_ids", "orientations", "mortar_neighbor_ids", "mortar_large_sides",
+                 "mortar_orientations", "boundary_neighbor_ids", "boundary_orientations", "boundary_neighbor_sides")
+        self._check(self._lib.trixi_hip_tree_connectivity(self.h, *[_ptr(out[k]) for k in order]), "connectivity")
+        return out
 
 
 def element_geometry(mesh, basis):
