@@ -356,6 +356,25 @@ def measured_traffic(level, polydeg, precision):
     return kib * 1024.0, t["summary"]
 
 
+def copy_bandwidth(torch, nbytes=1 << 30, reps=20):
+    """On-box HBM bandwidth of a plain device-to-device copy of nbytes (read +
+    write counted), best of reps launches timed with events; far above the
+    256 MB Infinity Cache, so it measures HBM."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        dst.copy_(src)
+        b.record()
+    torch.cuda.synchronize()
+    ms = min(a.elapsed_time(b) for a, b in ev)
+    del src, dst
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def amr_run(th, precision):
     """elixir_euler_amr.jl (test_tree_3d_euler.jl:145-165): 3D Euler p3 density
     pulse on [-5, 5]^3, TreeMesh level 4 adapted by ControllerThreeLevel
@@ -528,6 +547,8 @@ def main():
 
     ndofs = semis["fp32"].ndofs()
     nv = semis["fp32"].nvariables()
+    copy_gbs = copy_bandwidth(torch)
+    print(f"[bench] HBM copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     l2 = None
     if not args.no_l2 and not args.only:
         l2 = equal_l2(th, semis["fp64"], semis["fp32"])
@@ -578,6 +599,9 @@ def main():
             "kernel": "dg_element_kernel (fused volume + interface + surface + Jacobian + source)",
             "kernel_avg_ms": r["kernel_avg_ms"],
             "algorithmic_bytes": alg_bytes,
+            # the same achieved bandwidth against a plain 1 GiB copy measured on this box
+            "measured_copy_gbs": copy_gbs,
+            "frac_of_measured_copy": achieved / copy_gbs,
         },
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }

@@ -307,6 +307,16 @@ int trixi_hip_mlp_train_step(trixi_hip_ctx *ctx, const float *x_dev, const float
 int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *params,
                                  int nparams, const double *modes);
 
+/* compute_coefficients(func, t, semi) into the resident state on the device
+ * (semidiscretization.jl:224-242; compute_coefficients! dg.jl:1293-1324, the
+ * reference's GPU kernel): u at every node = the exact solution set with
+ * trixi_hip_set_exact_solution at node coordinates center + (1 /
+ * inverse_jacobian) * node (init_elements!, containers_3d.jl:87-128), FP64,
+ * plus the FP32 shadow in FP32 mode. Asynchronous on the context's stream.
+ * The device's transcendental functions can differ from the host's libm in
+ * the last bit, so the state can differ from a host evaluation at round-off. */
+int trixi_hip_compute_coefficients(trixi_hip_ctx *ctx, double t);
+
 /* calc_error_norms(cons2cons, ...) on the resident state (analysis_dg3d.jl:123-161). */
 int trixi_hip_error_norms(trixi_hip_ctx *ctx, double t, double *l2, double *linf);
 

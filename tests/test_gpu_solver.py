@@ -322,3 +322,38 @@ def test_rk_steps_with_time_independent_boundaries(precision, tol):
         semi.close()
     for rep in range(3):
         assert np.array_equal(res[0][rep], res[1][rep]), rep
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("D,ic,params", [(3, "convergence_test", ()), (3, "density_pulse", ()),
+                                         (3, "weak_blast_wave", ()), (3, "fourier_modes", (1,)),
+                                         (2, "isentropic_vortex", (25.0,)), (2, "blast_wave", ()),
+                                         (2, "blob", ()), (2, "weak_blast_wave", ()),
+                                         (2, "kelvin_helmholtz", (0.0, 0))])
+def test_device_compute_coefficients_matches_host(D, ic, params, precision):
+    """trixi_hip_compute_coefficients (the reference's GPU compute_coefficients!)
+    gives the host evaluation of the initial condition at every node up to the
+    last bits of the transcendental functions (relative 1e-14 of the state's
+    magnitude), on a refined mesh; the FP32 shadow is the rounded FP64 state."""
+    cmin, cmax = (-1.0,) * D, (1.0,) * D
+    patch = [((-0.5,) * D, (0.5,) * D)]
+    spec = dict(ndims=D, polydeg=3, level=2 if D == 3 else 3, cmin=cmin, cmax=cmax, ic=ic, ic_params=params,
+                volume_integral="fd", surface_flux="llf", refinement_patches=patch)
+    if ic == "blob":
+        spec["gamma"] = 5.0 / 3.0
+    semi, _ = build(spec, precision)
+    u_host = semi.compute_coefficients(0.3)
+    semi.set_initial_state(0.3)
+    u_dev = download(semi)
+    scale = np.max(np.abs(u_host))
+    assert np.max(np.abs(u_dev - u_host)) <= 1e-14 * scale, float(np.max(np.abs(u_dev - u_host)) / scale)
+    # the native state (FP32 shadow in FP32 mode) is the one an upload of the
+    # same FP64 state gives: a stage from either is bitwise the same
+    lib, ctx = semi._lib, semi.ctx
+    assert lib.trixi_hip_rk_stage(ctx, 0.0, 1e-3, 0.3, 1) == 0
+    a = download(semi)
+    upload(semi, u_dev)
+    assert lib.trixi_hip_rk_stage(ctx, 0.0, 1e-3, 0.3, 1) == 0
+    np.testing.assert_array_equal(a, download(semi))
+    semi.close()

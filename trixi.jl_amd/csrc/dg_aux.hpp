@@ -585,6 +585,38 @@ __global__ __launch_bounds__(256) void boundary_state_kernel(const double *__res
   for (int v = 0; v < V; ++v) ghost[(size_t)k * V + v] = T(ub[v]);
 }
 
+// compute_coefficients! on the device (dg.jl:1293-1324, the reference's own
+// GPU kernel, and init_elements! for the node coordinates,
+// containers_3d.jl:87-128): u at every LGL node is the exact solution at
+// x_d = center_d + jacobian * node_{i_d}, with the product and the sum rounded
+// separately as the host computes them; jacobian = 1 / inverse_jacobian. One
+// thread per node, state layout [E][n^D][V] in FP64.
+template <int D, int N> struct NodesConst {
+  double x[N + 1];
+};
+template <int D, int N>
+__global__ __launch_bounds__(256) void initial_state_kernel(const double *__restrict__ centers,
+                                                            const double *__restrict__ inv_jac, int E,
+                                                            NodesConst<D, N> nodes, ExactParams ep, double t,
+                                                            double *__restrict__ u) {
+  constexpr int n = N + 1, NN = ipow_c(n, D), V = D + 2;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)E * NN) return;
+  const int e = (int)(q / NN), l = (int)(q - (long)e * NN);
+  const double jac = 1.0 / inv_jac[e];
+  double x[3], uu[V];
+  int r = l;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int i = r % n;
+    r /= n;
+    x[d] = __dadd_rn(centers[(size_t)D * e + d], __dmul_rn(jac, nodes.x[i]));
+  }
+  exact_solution<D>(ep, x, t, uu);
+#pragma unroll
+  for (int v = 0; v < V; ++v) u[(size_t)q * V + v] = uu[v];
+}
+
 // boundary_condition_slip_wall, TreeMesh methods (compressible_euler_2d.jl:322-420,
 // compressible_euler_3d.jl:319-415): the flux at a wall face node from the
 // inner state alone. The state is rotated to the outward normal n = +-e_o;

@@ -208,15 +208,107 @@ struct trixi_hip_ctx {
   // device allocations and host synchronisations made for this context
   // (trixi_hip_get_counters)
   long long n_alloc = 0, n_sync = 0;
+  // size class of every device block this context holds (block pool)
+  std::map<void *, size_t> blocks;
 };
 
 namespace {
+
+// Device blocks and streams of destroyed contexts are kept per device and
+// handed to later contexts (an AMR run creates a context per adaptation, of
+// slightly different sizes): a block size class is 256-byte granules rounded
+// up to the next of 4/8, 5/8, 6/8, 7/8 or 8/8 of a power of two (at most 25 %
+// unused). At most kPoolBytes per device and kPoolStreams streams are kept;
+// the rest is freed. A block or stream enters the pool only after the work
+// of its context has completed (destroy synchronises the stream first).
+constexpr int kPoolDevices = 64;
+constexpr size_t kPoolBytes = (size_t)1 << 30;
+constexpr size_t kPoolStreams = 8;
+struct DevicePool {
+  std::mutex m;
+  std::multimap<size_t, void *> blocks[kPoolDevices];
+  size_t cached[kPoolDevices] = {};
+  std::vector<hipStream_t> streams[kPoolDevices];
+};
+DevicePool &device_pool() {
+  static DevicePool *p = new DevicePool;  // never destroyed: blocks outlive static destruction
+  return *p;
+}
+size_t size_class(size_t bytes) {
+  const size_t g = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+  size_t p = 256;
+  while (p < g) p <<= 1;
+  if (p <= 1024) return p;
+  size_t c = p / 2;
+  while (c < g) c += p / 8;
+  return c;
+}
+bool pool_device_ok(int dev) { return dev >= 0 && dev < kPoolDevices; }
 
 // Every device allocation and every host synchronisation of the library goes
 // through these, so that trixi_hip_get_counters can report them.
 template <typename P> hipError_t dev_alloc(trixi_hip_ctx *c, P **p, size_t bytes) {
   ++c->n_alloc;
-  return hipMalloc(reinterpret_cast<void **>(p), bytes);
+  const size_t cls = size_class(bytes);
+  void *ptr = nullptr;
+  if (pool_device_ok(c->device)) {
+    DevicePool &pool = device_pool();
+    std::lock_guard<std::mutex> lock(pool.m);
+    auto it = pool.blocks[c->device].find(cls);
+    if (it != pool.blocks[c->device].end()) {
+      ptr = it->second;
+      pool.blocks[c->device].erase(it);
+      pool.cached[c->device] -= cls;
+    }
+  }
+  if (!ptr) {
+    hipError_t e = hipMalloc(&ptr, cls);
+    if (e != hipSuccess) return e;
+  }
+  c->blocks[ptr] = cls;
+  *p = static_cast<P *>(ptr);
+  return hipSuccess;
+}
+// Returns a block of this context to the pool (its context's work on it must
+// have completed).
+void dev_release(trixi_hip_ctx *c, void *ptr) {
+  if (!ptr) return;
+  auto it = c->blocks.find(ptr);
+  const size_t cls = it == c->blocks.end() ? 0 : it->second;
+  if (it != c->blocks.end()) c->blocks.erase(it);
+  if (cls && pool_device_ok(c->device)) {
+    DevicePool &pool = device_pool();
+    std::lock_guard<std::mutex> lock(pool.m);
+    if (pool.cached[c->device] + cls <= kPoolBytes) {
+      pool.blocks[c->device].emplace(cls, ptr);
+      pool.cached[c->device] += cls;
+      return;
+    }
+  }
+  (void)hipFree(ptr);
+}
+hipError_t stream_acquire(int dev, hipStream_t *s) {
+  if (pool_device_ok(dev)) {
+    DevicePool &pool = device_pool();
+    std::lock_guard<std::mutex> lock(pool.m);
+    if (!pool.streams[dev].empty()) {
+      *s = pool.streams[dev].back();
+      pool.streams[dev].pop_back();
+      return hipSuccess;
+    }
+  }
+  return hipStreamCreate(s);
+}
+void stream_release(int dev, hipStream_t s) {
+  if (pool_device_ok(dev)) {
+    DevicePool &pool = device_pool();
+    std::lock_guard<std::mutex> lock(pool.m);
+    if (pool.streams[dev].size() < kPoolStreams) {
+      pool.streams[dev].push_back(s);
+      return;
+    }
+  }
+  (void)hipStreamDestroy(s);
 }
 hipError_t host_sync(trixi_hip_ctx *c) {
   ++c->n_sync;
@@ -748,7 +840,6 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   // mortars (init_mortars!, containers_3d.jl:652-788): small element q at its face
   // towards the large one, the large element at its face towards the smalls
   std::vector<int32_t> mort((size_t)ctx->M * (S + 3));
-  std::vector<std::vector<int32_t>> adjl(ctx->M > 0 ? E : 0);
   for (int m = 0; m < ctx->M; ++m) {
     const int o = mesh->mortar_orientations[m], side = mesh->mortar_large_sides[m];
     if (o < 0 || o >= D || (side != 1 && side != 2)) return fail(TRIXI_HIP_EINVAL, "bad mortar data");
@@ -763,22 +854,38 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     }
     mort[(size_t)m * (S + 3) + S + 1] = o;
     mort[(size_t)m * (S + 3) + S + 2] = side;
-    const int large = mort[(size_t)m * (S + 3) + S];
-    for (int q = 0; q < S; ++q) {
-      adjl[mort[(size_t)m * (S + 3) + q]].push_back(large);
-      adjl[large].push_back(mort[(size_t)m * (S + 3) + q]);
-    }
   }
+  // element adjacency for the smoothing of alpha on refined meshes (CSR): per
+  // element its mortar partners (mortar order), then its interface neighbours
+  // (interface order)
   std::vector<int32_t> adj_ptr, adj;
   if (ctx->M > 0) {
-    for (int i = 0; i < I; ++i) {
-      adjl[mesh->neighbor_ids[2 * i]].push_back(mesh->neighbor_ids[2 * i + 1]);
-      adjl[mesh->neighbor_ids[2 * i + 1]].push_back(mesh->neighbor_ids[2 * i]);
+    adj_ptr.assign((size_t)E + 1, 0);
+    for (int m = 0; m < ctx->M; ++m) {
+      const int32_t *mm = &mort[(size_t)m * (S + 3)];
+      for (int q = 0; q < S; ++q) {
+        ++adj_ptr[mm[q] + 1];
+        ++adj_ptr[mm[S] + 1];
+      }
     }
-    adj_ptr.push_back(0);
-    for (int e = 0; e < E; ++e) {
-      adj.insert(adj.end(), adjl[e].begin(), adjl[e].end());
-      adj_ptr.push_back((int32_t)adj.size());
+    for (int i = 0; i < I; ++i) {
+      ++adj_ptr[mesh->neighbor_ids[2 * i] + 1];
+      ++adj_ptr[mesh->neighbor_ids[2 * i + 1] + 1];
+    }
+    for (int e = 0; e < E; ++e) adj_ptr[e + 1] += adj_ptr[e];
+    adj.resize(adj_ptr[E]);
+    std::vector<int32_t> fill(adj_ptr.begin(), adj_ptr.end() - 1);
+    for (int m = 0; m < ctx->M; ++m) {
+      const int32_t *mm = &mort[(size_t)m * (S + 3)];
+      for (int q = 0; q < S; ++q) {
+        adj[fill[mm[q]]++] = mm[S];
+        adj[fill[mm[S]]++] = mm[q];
+      }
+    }
+    for (int i = 0; i < I; ++i) {
+      const int l = mesh->neighbor_ids[2 * i], r = mesh->neighbor_ids[2 * i + 1];
+      adj[fill[l]++] = r;
+      adj[fill[r]++] = l;
     }
   }
   // the L2 mortar matrices serve the mortars and the AMR solution transfer
@@ -844,7 +951,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     return TRIXI_HIP_EINVAL;
   }
 
-  if (hipStreamCreate(&ctx->own_stream) != hipSuccess)
+  if (stream_acquire(ctx->device, &ctx->own_stream) != hipSuccess)
     return fail(TRIXI_HIP_EHIP, "hipStreamCreate failed");
   ctx->stream = ctx->own_stream;
   const size_t nv = ctx->nvals;
@@ -882,32 +989,36 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
     fprintf(stderr, "trixi_hip_create: device allocation failed\n");
     return TRIXI_HIP_ENOMEM;
   }
-  std::vector<double> zeros(E, 0.0);
-  ok = sync_copy(ctx, ctx->d_nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-       sync_copy(ctx, ctx->d_invj, mesh->inverse_jacobian, E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       sync_copy(ctx, ctx->d_centers, mesh->element_centers, (size_t)D * E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       sync_copy(ctx, ctx->d_alpha_raw, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       sync_copy(ctx, ctx->d_alpha, zeros.data(), E * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       sync_copy(ctx, ctx->d_scalars + 32, ctx->weights.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess &&
-       (ctx->B == 0 || sync_copy(ctx, ctx->d_bx, bx.data(), bx.size() * 8, hipMemcpyHostToDevice) == hipSuccess);
+  // uploads on the context's stream, one synchronisation at the end (the host
+  // arrays live until then)
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
+  };
+  ok = up(ctx->d_nbr, nbr.data(), nbr.size() * 4) && up(ctx->d_invj, mesh->inverse_jacobian, E * 8) &&
+       up(ctx->d_centers, mesh->element_centers, (size_t)D * E * 8) &&
+       hipMemsetAsync(ctx->d_alpha_raw, 0, E * 8, ctx->stream) == hipSuccess &&
+       hipMemsetAsync(ctx->d_alpha, 0, E * 8, ctx->stream) == hipSuccess &&
+       up(ctx->d_scalars + 32, ctx->weights.data(), n * 8) && (ctx->B == 0 || up(ctx->d_bx, bx.data(), bx.size() * 8));
+  std::vector<double> g;
+  std::vector<float> gf;
   if (ok && nslots > 0) {
-    std::vector<double> g(slot_vals);
+    g.resize(slot_vals);
     for (size_t k = 0; k < slot_vals; ++k) {
       const int v = (int)(k % ctx->V);
       g[k] = v == 0 ? 1.0 : (v == ctx->V - 1 ? 2.5 : 0.0);  // rho = 1, v = 0, p = 1 (gamma 1.4)
     }
     if (ctx->fp32) {
-      std::vector<float> gf(g.begin(), g.end());
-      ok = sync_copy(ctx, ctx->d_ghost, gf.data(), slot_vals * 4, hipMemcpyHostToDevice) == hipSuccess;
+      gf.assign(g.begin(), g.end());
+      ok = up(ctx->d_ghost, gf.data(), slot_vals * 4);
     } else {
-      ok = sync_copy(ctx, ctx->d_ghost, g.data(), slot_vals * 8, hipMemcpyHostToDevice) == hipSuccess;
+      ok = up(ctx->d_ghost, g.data(), slot_vals * 8);
     }
   }
-  if (ok && ctx->slip) ok = sync_copy(ctx, ctx->d_bnd, bnd.data(), bnd.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+  if (ok && ctx->slip) ok = up(ctx->d_bnd, bnd.data(), bnd.size() * 4);
   if (ok && ctx->M > 0)
-    ok = sync_copy(ctx, ctx->d_mort, mort.data(), mort.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         sync_copy(ctx, ctx->d_adj_ptr, adj_ptr.data(), adj_ptr.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         (adj.empty() || sync_copy(ctx, ctx->d_adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
+    ok = up(ctx->d_mort, mort.data(), mort.size() * 4) && up(ctx->d_adj_ptr, adj_ptr.data(), adj_ptr.size() * 4) &&
+         (adj.empty() || up(ctx->d_adj, adj.data(), adj.size() * 4));
+  ok = ok && host_sync(ctx) == hipSuccess;
   if (!ok) {
     trixi_hip_destroy(ctx);
     fprintf(stderr, "trixi_hip_create: upload failed\n");
@@ -920,6 +1031,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
 int trixi_hip_destroy(trixi_hip_ctx *ctx) {
   if (!ctx) return TRIXI_HIP_OK;
   if (ctx->stream) (void)host_sync(ctx);
+  if (ctx->own_stream && ctx->own_stream != ctx->stream) (void)hipStreamSynchronize(ctx->own_stream);
   drop_graphs(ctx);
   void *ptrs[] = {ctx->d_nbr, ctx->d_invj, ctx->d_centers, ctx->d_ub[0], ctx->d_ub[1], ctx->d_utmp,
                   ctx->d_u32b[0], ctx->d_u32b[1], ctx->d_du,
@@ -929,9 +1041,9 @@ int trixi_hip_destroy(trixi_hip_ctx *ctx) {
                   ctx->d_alpha_hg, ctx->d_mlp_img, ctx->d_records, ctx->d_bx, ctx->d_ghost, ctx->d_mort,
                   ctx->d_mflux, ctx->d_bnd,
                   ctx->d_adj_ptr, ctx->d_adj};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
-  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  for (void *p : ptrs) dev_release(ctx, p);
+  while (!ctx->blocks.empty()) dev_release(ctx, ctx->blocks.begin()->first);
+  if (ctx->own_stream) stream_release(ctx->device, ctx->own_stream);
   delete ctx;
   return TRIXI_HIP_OK;
 }
@@ -1667,7 +1779,7 @@ int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *ki
     if (!r && to->fp32) r = convert<float, double>(to, cur_u32(to), cur_u64(to), to->nvals);
   }
   if (host_sync(to) != hipSuccess && !r) r = set_err(to, TRIXI_HIP_EHIP, "transfer failed");
-  (void)hipFree(d_tmp);
+  dev_release(to, d_tmp);
   return r;
 }
 
@@ -1685,6 +1797,26 @@ int trixi_hip_set_exact_solution(trixi_hip_ctx *ctx, int ic_kind, const double *
     if (!modes) return set_err(ctx, TRIXI_HIP_EINVAL, "Fourier-mode exact solution needs modes");
     std::memcpy(ctx->exact.modes, modes, sizeof(ctx->exact.modes));
   }
+  return TRIXI_HIP_OK;
+}
+
+int trixi_hip_compute_coefficients(trixi_hip_ctx *ctx, double t) {
+  if (!ctx) return TRIXI_HIP_EINVAL;
+  const long total = (long)ctx->E * ctx->NN;
+  bool ok = dispatch_dn(ctx->D, ctx->N, [&](auto Dc, auto Nc) {
+    constexpr int D = decltype(Dc)::value, N = decltype(Nc)::value;
+    NodesConst<D, N> nc{};
+    for (int i = 0; i <= N; ++i) nc.x[i] = ctx->nodes[i];
+    hipLaunchKernelGGL((initial_state_kernel<D, N>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       ctx->stream, ctx->d_centers, ctx->d_invj, ctx->E, nc, ctx->exact, t, cur_u64(ctx));
+  });
+  if (!ok) return TRIXI_HIP_EINVAL;
+  HIP_TRY(hipGetLastError());
+  if (ctx->fp32) {
+    int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
+    if (r) return r;
+  }
+  ctx->alpha_fresh = false;
   return TRIXI_HIP_OK;
 }
 
@@ -1722,7 +1854,7 @@ int trixi_hip_set_analysis(trixi_hip_ctx *ctx, int interval, int capacity) {
   if (!ctx || interval < 0 || (interval > 0 && capacity <= 0)) return TRIXI_HIP_EINVAL;
   if (ctx->d_records) {
     HIP_TRY(host_sync(ctx));
-    HIP_TRY(hipFree(ctx->d_records));
+    dev_release(ctx, ctx->d_records);
     ctx->d_records = nullptr;
   }
   ctx->rec_interval = interval;

@@ -148,6 +148,7 @@ SIGNATURES = [
     ("trixi_hip_indicator_max", _I, [_P, _I, _P]),
     ("trixi_hip_transfer", _I, [_P, _P, _P, _P]),
     ("trixi_hip_set_exact_solution", _I, [_P, _I, _PD, _I, _PD]),
+    ("trixi_hip_compute_coefficients", _I, [_P, _D]),
     ("trixi_hip_error_norms", _I, [_P, _D, _PD, _PD]),
     ("trixi_hip_integrals", _I, [_P, _PD, _PD]),
     ("trixi_hip_set_analysis", _I, [_P, _I, _I]),

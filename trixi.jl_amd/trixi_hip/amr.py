@@ -147,9 +147,7 @@ class AMRCallback:
         changed = self(semi, only_refine=only_refine)
         iterations = 1
         while changed:
-            u0 = semi.compute_coefficients(t0)
-            _lib.check(semi._lib.trixi_hip_upload_u(semi.ctx, u0.ctypes.data_as(ctypes.c_void_p)), semi.ctx,
-                       "upload_u")
+            semi.set_initial_state(t0)  # the initial condition on the new mesh
             changed = self(semi, only_refine=only_refine)
             iterations += 1
             if iterations > max(10, self.controller.max_level):

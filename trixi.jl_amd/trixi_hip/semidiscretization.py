@@ -229,6 +229,18 @@ class SemidiscretizationHyperbolic:
         u = self.initial_condition(self.node_coordinates, t, self.equations)  # [E, NN, V]
         return np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
 
+    def set_initial_state(self, t):
+        """The resident state := compute_coefficients(t): evaluated on the device
+        (trixi_hip_compute_coefficients, as the reference's GPU backend does,
+        dg.jl:1293-1324) when the device knows the initial condition, else on
+        the host and uploaded."""
+        ctx = self.ctx
+        if getattr(self.initial_condition, "device_exact", False):
+            _lib.check(self._lib.trixi_hip_compute_coefficients(ctx, float(t)), ctx, "compute_coefficients")
+        else:
+            u0 = self.compute_coefficients(t)
+            _lib.check(self._lib.trixi_hip_upload_u(ctx, u0.ctypes.data_as(ctypes.c_void_p)), ctx, "upload_u")
+
     def __repr__(self):
         return (f"SemidiscretizationHyperbolic({self.mesh}, {self.equations}, {self.initial_condition}, "
                 f"{self.solver}, precision={self.precision})")
