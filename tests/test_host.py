@@ -294,3 +294,38 @@ def test_native_tree_refuses_invalid_input():
     t = nt.leaves()
     np.testing.assert_array_equal(t.lv, [0])
     assert (t.kind[0], t.src[0]) == (-1, 0)
+
+
+def test_spec_features_closed_forms():
+    """spec_features (the configuration's input set, used for the feature
+    comparison): a constant state gives rho*p / mean = 1 and energy ratios
+    (0, 0, 1); a state whose rho*p is one highest mode plus a mean gives
+    (E - E1) / E equal to the mode's energy share, and a mode of degree N - 1
+    gives (E1 - E2) / E1 the same share."""
+    from trixi_hip.learned_indicator import khi_semidiscretization, spec_features
+    semi = khi_semidiscretization(level=1)
+    E, n = semi.nelements(), 4
+    gamma = semi.equations.gamma
+    basis = semi.solver.basis
+    x = basis.nodes
+    u = np.zeros((E, n * n, 4))
+    u[..., 0] = 1.0
+    u[..., 3] = 2.0 / (gamma - 1)  # p = 2, v = 0
+    f = spec_features(semi, u.reshape(-1))
+    np.testing.assert_allclose(f[:, :16], 1.0, rtol=1e-6)
+    np.testing.assert_allclose(f[:, 16:19], np.tile([0.0, 0.0, 1.0], (E, 1)), atol=1e-6)
+    from numpy.polynomial import legendre
+    for deg, col in ((3, 16), (2, 17)):
+        # rho = 1, p(x, y) = 2 + 0.5 P_deg(x): modal energies 2^2 (mean) and
+        # (0.5 c)^2 with c the normalised Legendre coefficient of the nodal values
+        px = 2.0 + 0.5 * legendre.legval(x, [0] * deg + [1])
+        pnode = np.tile(px, n)  # node = i + n j, x index fastest
+        w = u.copy()
+        w[..., 3] = pnode[None, :] / (gamma - 1)
+        f = spec_features(semi, w.reshape(-1))
+        vinv = np.asarray(basis.inverse_vandermonde_legendre)
+        m1 = vinv @ px  # modal coefficients along x
+        # rows are constant in y, so only y-mode 0 carries energy: the shares
+        # are those of the x transform
+        e_mode, e_rest = m1[deg] ** 2, (m1[:deg] ** 2).sum()
+        np.testing.assert_allclose(f[:, col], e_mode / (e_mode + e_rest), rtol=1e-5)

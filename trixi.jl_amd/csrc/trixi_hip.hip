@@ -170,10 +170,12 @@ struct trixi_hip_ctx {
   int epb = 0, nt = 0, epb_st = 0, nt_st = 0;
   std::vector<unsigned char> kconst;  // DGConst<T, N> blob
   // workgroups of the element kernel walk contiguous element ranges per XCD
-  // (neighbour faces then mostly hit the same L2). FP32 default: 3D p3 RHS
-  // 36.7 -> 35.8 us, CK2N54 step 395.6 -> 382.6 us; FP64 is faster without it
-  // (RHS 108.2 vs 109.2 us). A/B on one box; TRIXI_HIP_XCD_REMAP=0/1 overrides.
-  int xcd_remap = 0;
+  // (neighbour faces then mostly hit the same L2), separately for the RHS and
+  // the fused-stage variant. FP32: both (3D p3 RHS 36.7 -> 35.8 us, CK2N54 step
+  // 395.6 -> 382.6 us). FP64: the stage only (step 565.2 -> 555.6 us; the RHS is
+  // slower with it, 101.9 vs 102.8 us). A/B on one box; TRIXI_HIP_XCD_REMAP=0/1
+  // overrides both.
+  int xcd_remap = 0, xcd_stage = 0;
   // the last stage of a step reduces the wave speed in its epilogue (see
   // stage_wave_speed) instead of max_speed_nodes_kernel reading the state again.
   // FP32 default: 3D p3 CK2N54 step 382.6 -> 370.4 us. FP64 is slower with it
@@ -607,7 +609,7 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
   a.fv_flux = ctx->cfg.fv_flux;
   a.source = ctx->cfg.source_terms == TRIXI_HIP_SRC_CONVERGENCE_TEST;
   a.alpha_smooth = ctx->M > 0 ? 0 : ctx->cfg.alpha_smooth;
-  a.xcd_remap = ctx->xcd_remap;
+  a.xcd_remap = so ? ctx->xcd_stage : ctx->xcd_remap;
   a.gamma = T(ctx->cfg.gamma);
   a.inv_gamma_m1 = T(1) / (a.gamma - T(1));  // the device expression, evaluated once in T
   a.sqrt_gamma = T(std::sqrt(ctx->cfg.gamma));
@@ -935,9 +937,10 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   ctx->hg.parameter_s = std::log((1 - 0.0001) / 0.0001);
   ctx->hg.variable = cfg->hg_variable;
   ctx->xcd_remap = cfg->precision == TRIXI_HIP_FP32 ? 1 : 0;
+  ctx->xcd_stage = 1;
   ctx->fused_speed = cfg->precision == TRIXI_HIP_FP32;
   if (const char *env = std::getenv("TRIXI_HIP_FUSED_INDICATOR")) ctx->fused_ind = std::atoi(env) != 0;
-  if (const char *env = std::getenv("TRIXI_HIP_XCD_REMAP")) ctx->xcd_remap = std::atoi(env) != 0;
+  if (const char *env = std::getenv("TRIXI_HIP_XCD_REMAP")) ctx->xcd_remap = ctx->xcd_stage = std::atoi(env) != 0;
   if (const char *env = std::getenv("TRIXI_HIP_FUSED_SPEED")) ctx->fused_speed = std::atoi(env) != 0;
   ctx->exact.kind = TRIXI_HIP_IC_CONSTANT;
   ctx->exact.gamma = cfg->gamma;

@@ -471,6 +471,83 @@ def evaluate_config5(net, seed=1000, level=7, t_end=3.0, ref_level=8, cfl=1.3, l
     return out
 
 
+def spec_features(semi, u):
+    """The input set the learned-indicator configuration describes, computed on
+    the host for the feature comparison (compare_feature_sets): the 16 nodal
+    values of rho*p divided by their element mean, and three ratios of the
+    modal energies of rho*p (HG's dimensionwise modal transform,
+    indicators_2d.jl:59-79): (E - E1) / E, (E1 - E2) / E1 and E2 / E, with E, E1,
+    E2 the energies of all modes, of the modes below N and of those below N - 1.
+    u: the FP64 state [E * 16 * 4]. Returns FP32 [E, 32] (zero padded)."""
+    gamma = semi.equations.gamma
+    n = semi.solver.basis.nnodes()
+    w = np.asarray(u, dtype=np.float64).reshape(-1, n * n, 4)
+    rho = w[..., 0]
+    p = (gamma - 1) * (w[..., 3] - 0.5 * (w[..., 1] ** 2 + w[..., 2] ** 2) / rho)
+    q = rho * p  # [E, n*n], node = i + n j
+    vinv = np.asarray(semi.solver.basis.inverse_vandermonde_legendre, dtype=np.float64)
+    m = np.einsum("ai,bj,eji->eba", vinv, vinv, q.reshape(-1, n, n))  # modal [E, j-mode b, i-mode a]
+    m2 = m ** 2
+    e_all = m2.sum(axis=(1, 2))
+    e1 = m2[:, : n - 1, : n - 1].sum(axis=(1, 2))
+    e2 = m2[:, : n - 2, : n - 2].sum(axis=(1, 2))
+    tiny = 1e-300
+    x = np.zeros((q.shape[0], FEATURE_STRIDE), dtype=np.float32)
+    x[:, :16] = q / q.mean(axis=1, keepdims=True)
+    x[:, 16] = (e_all - e1) / np.maximum(e_all, tiny)
+    x[:, 17] = (e1 - e2) / np.maximum(e1, tiny)
+    x[:, 18] = e2 / np.maximum(e_all, tiny)
+    return x
+
+
+def compare_feature_sets(train_seeds=(0, 1, 2, 3), val_seed=4, level=5, t_end=3.0, nsnap=11, steps=3000,
+                         batch=8192, lr=2e-3, dead_zone=1.5, log=None):
+    """The same network trained on the two input sets, same targets, seeds and
+    schedule: the features of indicator_2d_p3_kernel (rho*p centred by the mean
+    and scaled by the largest deviation, HG logits of the energy ratios, log of
+    the low-mode share) and spec_features (rho*p / mean, three energy ratios).
+    Snapshots of HG-driven KHI runs; validation on a seed not trained on: mean
+    squared error against the targets, mean |alpha - alpha_target| / alpha_max
+    of the clipped output, and the share of elements whose blend decision
+    (clipped alpha > 0) differs from the target's."""
+    import torch
+    data = {}
+    for s in list(train_seeds) + [val_seed]:
+        semi = khi_semidiscretization(level, 1e-3, s)
+        rec = {"ours": [], "spec": [], "y": []}
+
+        def grab(sm, t):
+            f, a = features(sm)
+            rec["ours"].append(f)
+            rec["spec"].append(spec_features(sm, _download(sm)))
+            rec["y"].append(training_targets(a, dead_zone=dead_zone))
+
+        run_segments(semi, t_end, nsnap, on_snapshot=grab)
+        semi.close()
+        data[s] = {k: np.concatenate(v) for k, v in rec.items()}
+    semi = khi_semidiscretization(level, 1e-3, 0)
+    out = dict(level=level, t_end=t_end, snapshots=nsnap, train_seeds=list(train_seeds), val_seed=val_seed,
+               steps=steps, batch=batch)
+    yv = data[val_seed]["y"]
+    for name in ("ours", "spec"):
+        X = np.concatenate([data[s][name] for s in train_seeds])
+        Y = np.concatenate([data[s]["y"] for s in train_seeds])
+        net, hist = train(semi, X, Y, steps, batch, lr, log=log)
+        net.upload(semi)
+        xv = torch.from_numpy(np.ascontiguousarray(data[val_seed][name])).cuda()
+        y = torch.empty(xv.shape[0], dtype=torch.float32, device="cuda")
+        forward(semi, xv, y)
+        yn = y.cpu().numpy().astype(np.float64)
+        alpha = np.where(yn * 0.002 < 0.0001, 0.0, np.minimum(yn * 0.002, 0.002))  # hg_clip
+        out[name] = dict(train_loss=float(hist[-1][1]), val_mse=float(np.mean((yn - yv) ** 2)),
+                         val_alpha_mean_abs_diff=float(np.mean(np.abs(alpha / 0.002 - yv))),
+                         val_blend_decision_mismatch=float(np.mean((alpha > 0) != (yv > 0))))
+        if log:
+            log(f"{name}: {out[name]}")
+    semi.close()
+    return out
+
+
 def main(argv=None):
     import argparse
     import json
@@ -491,6 +568,8 @@ def main(argv=None):
     p.add_argument("--config5", action="store_true",
                    help="judge the weights at --out by the HG run's density error (evaluate_config5)")
     p.add_argument("--seed", type=int, default=1000, help="KHI seed of --evaluate / --config5")
+    p.add_argument("--compare-features", action="store_true",
+                   help="train on this kernel's features and on spec_features and compare (compare_feature_sets)")
     a = p.parse_args(argv)
 
     def log(msg):
@@ -498,6 +577,10 @@ def main(argv=None):
 
     if a.evaluate:
         print(json.dumps(evaluate(IndicatorNeuralNetwork.load(a.out), seed=a.seed, level=a.level)))
+        return
+    if a.compare_features:
+        print(json.dumps(compare_feature_sets(level=a.level, t_end=a.t_end, nsnap=a.snapshots, steps=a.steps,
+                                              batch=a.batch, lr=a.lr, dead_zone=a.dead_zone, log=log)))
         return
     if a.config5:
         print(json.dumps(evaluate_config5(IndicatorNeuralNetwork.load(a.out), seed=a.seed, level=a.level,
