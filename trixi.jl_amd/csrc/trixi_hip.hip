@@ -220,9 +220,10 @@ namespace {
 // handed to later contexts (an AMR run creates a context per adaptation, of
 // slightly different sizes): a block size class is 256-byte granules rounded
 // up to the next of 4/8, 5/8, 6/8, 7/8 or 8/8 of a power of two (at most 25 %
-// unused). At most kPoolBytes per device and kPoolStreams streams are kept;
-// the rest is freed. A block or stream enters the pool only after the work
-// of its context has completed (destroy synchronises the stream first).
+// unused); above 64 MiB, 2 MiB granules (at most 3 % unused). At most
+// kPoolBytes per device and kPoolStreams streams are kept; the rest is freed.
+// A block or stream enters the pool only after the work of its context has
+// completed (destroy synchronises the stream first).
 constexpr int kPoolDevices = 64;
 constexpr size_t kPoolBytes = (size_t)1 << 30;
 constexpr size_t kPoolStreams = 8;
@@ -238,6 +239,8 @@ DevicePool &device_pool() {
 }
 size_t size_class(size_t bytes) {
   const size_t g = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+  constexpr size_t kBig = (size_t)64 << 20, kBigGranule = (size_t)2 << 20;
+  if (g > kBig) return (g + kBigGranule - 1) / kBigGranule * kBigGranule;
   size_t p = 256;
   while (p < g) p <<= 1;
   if (p <= 1024) return p;
