@@ -105,6 +105,18 @@ __device__ __forceinline__ bf16x4 lds4(const uint16_t *p) {
   return r;
 }
 
+// The A fragment of W1 for (hb, kb) and of W2 for (ob, kb) held by this lane
+// (row 16 block + l % 16, k = 16 kb + 4 (l / 16) + 0..3), from the row-major
+// LDS images; the inference image's overloads follow its definition.
+__device__ __forceinline__ bf16x4 frag_w1(const WeightsLDS &s, int hb, int kb) {
+  const int l = threadIdx.x & 63;
+  return lds4(&s.w1[(hb * 16 + (l & 15)) * KF + kb * 16 + 4 * (l >> 4)]);
+}
+__device__ __forceinline__ bf16x4 frag_w2(const WeightsLDS &s, int ob, int kb) {
+  const int l = threadIdx.x & 63;
+  return lds4(&s.w2[(ob * 16 + (l & 15)) * NH + kb * 16 + 4 * (l >> 4)]);
+}
+
 // Forward pass of one 16-element tile. x: [n][KF] FP32 features (zero
 // padded). Leaves per lane: z1/z2 pre-activations (C layout), the BF16
 // operands xb/h1b and the output y of element l % 16.
@@ -126,7 +138,7 @@ template <typename W> __device__ __forceinline__ void forward_layers(const W &s,
   for (int hb = 0; hb < 4; ++hb) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) acc = mfma(lds4(&s.w1[(hb * 16 + (l & 15)) * KF + kb * 16 + 4 * g]), t.xb[kb], acc);
+    for (int kb = 0; kb < 2; ++kb) acc = mfma(frag_w1(s, hb, kb), t.xb[kb], acc);
     float h[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -142,7 +154,7 @@ template <typename W> __device__ __forceinline__ void forward_layers(const W &s,
   for (int ob = 0; ob < 4; ++ob) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) acc = mfma(lds4(&s.w2[(ob * 16 + (l & 15)) * NH + kb * 16 + 4 * g]), t.h1b[kb], acc);
+    for (int kb = 0; kb < 4; ++kb) acc = mfma(frag_w2(s, ob, kb), t.h1b[kb], acc);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int h = ob * 16 + 4 * g + r;
@@ -162,32 +174,13 @@ template <typename W> __device__ __forceinline__ void forward_layers(const W &s,
 // fragments of W1 (hb, kb) and W2 (ob, kb), the biases and output weights of
 // the rows the lane's accumulators hold (h = 16 hb + 4 (l / 16) + r). Loading
 // them all at kernel entry makes one memory round trip of the whole network
-// instead of one per layer.
+// instead of one per layer (load_weight_regs, after InferImage).
 struct WeightRegs {
   bf16x4 w1[4][2];
   bf16x4 w2[4][4];
   float b1[4][4], b2[4][4], w3[4][4];
   float b3;
 };
-
-template <typename W> __device__ __forceinline__ void load_weight_regs(const W &s, WeightRegs &r) {
-  const int l = threadIdx.x & 63;
-  const int g = l >> 4;
-#pragma unroll
-  for (int hb = 0; hb < 4; ++hb) {
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) r.w1[hb][kb] = lds4(&s.w1[(hb * 16 + (l & 15)) * KF + kb * 16 + 4 * g]);
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) r.w2[hb][kb] = lds4(&s.w2[(hb * 16 + (l & 15)) * NH + kb * 16 + 4 * g]);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      r.b1[hb][q] = s.b1[hb * 16 + 4 * g + q];
-      r.b2[hb][q] = s.b2[hb * 16 + 4 * g + q];
-      r.w3[hb][q] = s.w3[hb * 16 + 4 * g + q];
-    }
-  }
-  r.b3 = s.b3;
-}
 
 // forward_layers with the operands from registers: the same products, bias
 // additions and activations in the same order, so the same result bit for bit.
@@ -241,15 +234,43 @@ __device__ __forceinline__ void forward_tile(const float *__restrict__ x, int n,
 }
 
 // The network as inference reads it: BF16 images of W1 (features padded with
-// zeros) and W2, FP32 biases and output weights. Kept in device memory
-// (refreshed whenever the parameters change) and copied into LDS with
-// 16-byte loads, so that a launch converts nothing.
+// zeros) and W2 in MFMA fragment order, [block][kb][lane][4] (lane l: row
+// 16 block + l % 16, k = 16 kb + 4 (l / 16) + 0..3), so that the 64 lanes of a
+// wavefront read a fragment as one contiguous 512-byte block; FP32 biases and
+// output weights. Kept in device memory, refreshed whenever the parameters
+// change, so that a launch converts nothing.
 struct InferImage {
-  uint16_t w1[NH * KF];
-  uint16_t w2[NH * NH];
+  uint16_t w1f[4 * 2 * 64 * 4];
+  uint16_t w2f[4 * 4 * 64 * 4];
   float b1[NH], b2[NH], w3[NH], b3, pad[3];
 };
 static_assert(sizeof(InferImage) % 16 == 0, "16-byte copies");
+
+__device__ __forceinline__ bf16x4 frag_w1(const InferImage &s, int hb, int kb) {
+  return lds4(&s.w1f[((hb * 2 + kb) * 64 + (threadIdx.x & 63)) * 4]);
+}
+__device__ __forceinline__ bf16x4 frag_w2(const InferImage &s, int ob, int kb) {
+  return lds4(&s.w2f[((ob * 4 + kb) * 64 + (threadIdx.x & 63)) * 4]);
+}
+
+__device__ __forceinline__ void load_weight_regs(const InferImage &s, WeightRegs &r) {
+  const int l = threadIdx.x & 63;
+  const int g = l >> 4;
+#pragma unroll
+  for (int hb = 0; hb < 4; ++hb) {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) r.w1[hb][kb] = frag_w1(s, hb, kb);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) r.w2[hb][kb] = frag_w2(s, hb, kb);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      r.b1[hb][q] = s.b1[hb * 16 + 4 * g + q];
+      r.b2[hb][q] = s.b2[hb * 16 + 4 * g + q];
+      r.w3[hb][q] = s.w3[hb * 16 + 4 * g + q];
+    }
+  }
+  r.b3 = s.b3;
+}
 
 // The clipping of calc_indicator_hennemann_gassner! (indicators_2d.jl:86-99)
 // applied to a blending factor: below alpha_min it is 0 (pure DG, the element

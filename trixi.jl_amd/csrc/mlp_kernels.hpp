@@ -11,11 +11,6 @@ namespace mlp {
 
 // FP32 parameters -> InferImage (one workgroup).
 __global__ __launch_bounds__(256) void mlp_pack_kernel(const float *__restrict__ w, InferImage *__restrict__ img) {
-  for (int i = threadIdx.x; i < NH * KF; i += blockDim.x) {
-    const int h = i / KF, f = i - h * KF;
-    img->w1[i] = f < NF ? to_bf16(w[OFF_W1 + h * NF + f]) : 0;
-  }
-  for (int i = threadIdx.x; i < NH * NH; i += blockDim.x) img->w2[i] = to_bf16(w[OFF_W2 + i]);
   for (int i = threadIdx.x; i < NH; i += blockDim.x) {
     img->b1[i] = w[OFF_B1 + i];
     img->b2[i] = w[OFF_B2 + i];
@@ -24,6 +19,17 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float *__restrict__
   if (threadIdx.x == 0) {
     img->b3 = w[OFF_B3];
     img->pad[0] = img->pad[1] = img->pad[2] = 0.f;
+  }
+  // W1 and W2 in fragment order: [block][kb][lane][4]
+  for (int i = threadIdx.x; i < 4 * 2 * 64 * 4; i += blockDim.x) {
+    const int jj = i & 3, l = (i >> 2) & 63, kb = (i >> 8) & 1, hb = i >> 9;
+    const int h = hb * 16 + (l & 15), f = kb * 16 + 4 * (l >> 4) + jj;
+    img->w1f[i] = f < NF ? to_bf16(w[OFF_W1 + h * NF + f]) : 0;
+  }
+  for (int i = threadIdx.x; i < 4 * 4 * 64 * 4; i += blockDim.x) {
+    const int jj = i & 3, l = (i >> 2) & 63, kb = (i >> 8) & 3, ob = i >> 10;
+    const int o = ob * 16 + (l & 15), k = kb * 16 + 4 * (l >> 4) + jj;
+    img->w2f[i] = to_bf16(w[OFF_W2 + o * NH + k]);
   }
 }
 
