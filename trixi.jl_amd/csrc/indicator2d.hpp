@@ -56,12 +56,13 @@ struct Ind2DArgs {
 // lane's 16 state values (nodes 4 j .. 4 j + 3, four variables each). s_mx,
 // s_sq: 256 doubles each of LDS owned by the wavefront. PRELOAD: the network
 // operands are loaded into registers at entry (the stand-alone kernel, one
-// wavefront per SIMD, where latency is exposed); otherwise they are read where
-// the products use them (inside the element kernel, whose register budget
-// they would otherwise set). Both give the same result bit for bit.
-template <typename T, bool FEAT, bool PRELOAD = true>
+// wavefront per SIMD, where latency is exposed), from simg (the workgroup's
+// LDS copy of the image) with SHARED_IMG, else from ia.img; otherwise they are
+// read where the products use them (inside the element kernel, whose register
+// budget they would otherwise set). All give the same result bit for bit.
+template <typename T, bool FEAT, bool PRELOAD = true, bool SHARED_IMG = false>
 __device__ __forceinline__ void indicator_2d_p3_wave(const T *un, int e0, int E, const Ind2DArgs &ia, double *s_mx,
-                                                     double *s_sq) {
+                                                     double *s_sq, const mlp::InferImage *simg = nullptr) {
   constexpr int n = 4, V = 4;
   const HGParams &hp = ia.hp;
   const Vinv4 &vi = ia.vi;
@@ -82,7 +83,8 @@ __device__ __forceinline__ void indicator_2d_p3_wave(const T *un, int e0, int E,
   mlp::WeightRegs wr;
   float lo_r[7], hi_r[7];
   if constexpr (FEAT && PRELOAD) {
-    mlp::load_weight_regs(*img, wr);
+    if constexpr (SHARED_IMG) mlp::load_weight_regs(*simg, wr);
+    else mlp::load_weight_regs(*img, wr);
 #pragma unroll
     for (int i = 0; i < n; ++i) {
       lo_r[i] = lo[4 * j + i];
@@ -249,14 +251,22 @@ __device__ __forceinline__ void indicator_2d_p3_wave(const T *un, int e0, int E,
   }
 }
 
+// Stand-alone launch: one wavefront per 16 elements. The HG factor alone
+// (FEAT = false) runs one wavefront per workgroup. With the features and the
+// network, a workgroup is kIndWaves wavefronts that share one LDS copy of the
+// network image, read from global memory once per workgroup instead of once
+// per wavefront (the image is 13 KB; at 128^2 elements that is 3.3 instead of
+// 13 MB of L2 reads per launch).
+constexpr int kIndWaves = 4;
 template <typename T, bool FEAT>
-__global__ __launch_bounds__(64) void indicator_2d_p3_kernel(const T *__restrict__ u, int E, Ind2DArgs ia) {
-  constexpr int V = 4;
-  __shared__ double s_mx[16 * 16];  // x-transformed rows [element][row j][mode i]
-  __shared__ double s_sq[16 * 16];  // squared modes [element][a + 4 j]
-  const int l = threadIdx.x;
+__global__ __launch_bounds__(FEAT ? 64 * kIndWaves : 64) void indicator_2d_p3_kernel(const T *__restrict__ u, int E,
+                                                                                    Ind2DArgs ia) {
+  constexpr int V = 4, W = FEAT ? kIndWaves : 1;
+  __shared__ double s_mx[W][16 * 16];  // x-transformed rows [element][row j][mode i]
+  __shared__ double s_sq[W][16 * 16];  // squared modes [element][a + 4 j]
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int el = l & 15, j = l >> 4;
-  const int e0 = blockIdx.x * 16;
+  const int e0 = (blockIdx.x * W + w) * 16;
   const int e = e0 + el;
   const int ec = e < E ? e : E - 1;  // lanes past the end compute on the last element, store nothing
   // the lane's four nodes (i + 4 j): 16 contiguous values, loaded at once in
@@ -266,7 +276,19 @@ __global__ __launch_bounds__(64) void indicator_2d_p3_kernel(const T *__restrict
   const float4 *src = reinterpret_cast<const float4 *>(u + ((size_t)ec * 16 + 4 * j) * V);
 #pragma unroll
   for (int c = 0; c < NCH; ++c) raw[c] = src[c];
-  indicator_2d_p3_wave<T, FEAT>(reinterpret_cast<const T *>(raw), e0, E, ia, s_mx, s_sq);
+  if constexpr (FEAT) {
+    __shared__ __attribute__((aligned(16))) mlp::InferImage s_img;
+    if (ia.net) {  // uniform: copy the image in 16-byte pieces
+      constexpr int NC = (int)(sizeof(mlp::InferImage) / 16);
+      const float4 *gi = reinterpret_cast<const float4 *>(ia.img);
+      float4 *si = reinterpret_cast<float4 *>(&s_img);
+      for (int c = threadIdx.x; c < NC; c += 64 * W) si[c] = gi[c];
+    }
+    __syncthreads();
+    indicator_2d_p3_wave<T, FEAT, true, true>(reinterpret_cast<const T *>(raw), e0, E, ia, s_mx[w], s_sq[w], &s_img);
+  } else {
+    indicator_2d_p3_wave<T, FEAT>(reinterpret_cast<const T *>(raw), e0, E, ia, s_mx[w], s_sq[w]);
+  }
 }
 
 }  // namespace trixi_hip

@@ -481,13 +481,14 @@ int launch_hg(trixi_hip_ctx *ctx, const T *u, double *alpha_out, float *feat, co
               bool net = false, double *alpha_hg = nullptr) {
   const HGParams params = hp ? *hp : ctx->hg;
   if (ctx->D == 2 && ctx->N == 3) {
-    const int nb = (ctx->E + 15) / 16;  // one wavefront of 16 elements per workgroup
+    const int nb = (ctx->E + 15) / 16;  // one wavefront per 16 elements
     Ind2DArgs ia = ind2d_args(ctx, params, net, alpha_out, alpha_hg, feat);
     if (feat || net) {
       // the network buffers exist whenever features or the network are asked
       // for (trixi_hip_mlp_features and trixi_hip_set_indicator allocate them)
       if (!ctx->d_mlp || !ctx->d_mlp_img) return set_err(ctx, TRIXI_HIP_ESTATE, "learned-indicator buffers missing");
-      hipLaunchKernelGGL((indicator_2d_p3_kernel<T, true>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, ia);
+      hipLaunchKernelGGL((indicator_2d_p3_kernel<T, true>), dim3((nb + kIndWaves - 1) / kIndWaves),
+                         dim3(64 * kIndWaves), 0, ctx->stream, u, ctx->E, ia);
     } else {
       hipLaunchKernelGGL((indicator_2d_p3_kernel<T, false>), dim3(nb), dim3(64), 0, ctx->stream, u, ctx->E, ia);
     }
