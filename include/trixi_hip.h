@@ -55,6 +55,7 @@ extern "C" {
 #define TRIXI_HIP_FLUX_LLF_NAIVE 1 /* FluxLaxFriedrichs(max_abs_speed_naive) */
 #define TRIXI_HIP_FLUX_RANOCHA 2   /* flux_ranocha */
 #define TRIXI_HIP_FLUX_CENTRAL 3   /* flux_central */
+#define TRIXI_HIP_FLUX_HLLC 4      /* flux_hllc (compressible_euler_3d.jl:1417-1541, 2D :1714-1816); surface or FV flux */
 
 /* boundary conditions of a non-periodic TreeMesh (trixi_hip_config.boundary_condition) */
 #define TRIXI_HIP_BC_NONE 0      /* fully periodic mesh: no boundaries */

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrixi_oracle.so")
 
 VI = {"weak": 0, "fd": 1, "hg": 2, "fv": 3}
-FLUX = {"llf": 0, "llf_naive": 1, "ranocha": 2, "central": 3}
+FLUX = {"llf": 0, "llf_naive": 1, "ranocha": 2, "central": 3, "hllc": 4}
 IC = {"constant": 0, "convergence_test": 1, "weak_blast_wave": 2, "isentropic_vortex": 3,
       "kelvin_helmholtz": 4, "fourier_modes": 5, "density_pulse": 6, "blast_wave": 7, "blob": 8}
 HG_VARIABLE = {"density_pressure": 0, "density": 1, "pressure": 2}
@@ -390,7 +390,7 @@ def inv_ln_mean(x, y):
 
 
 def flux(ndims, kind, ul, ur, orientation, gamma=1.4):
-    """kind: 'llf' | 'llf_naive' | 'ranocha' | 'central' | 'physical' (orientation 0-based)."""
+    """kind: 'llf' | 'llf_naive' | 'ranocha' | 'central' | 'hllc' | 'physical' (orientation 0-based)."""
     ul = np.ascontiguousarray(ul, dtype=np.float64)
     ur = np.ascontiguousarray(ur if ur is not None else ul, dtype=np.float64)
     f = np.empty(ndims + 2)

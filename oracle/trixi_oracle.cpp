@@ -446,6 +446,67 @@ template <int D, typename R = double> struct Euler {
         for (int v = 0; v < V; ++v) f[v] = fc[v] + (-R(0.5) * lambda * (ur[v] - ul[v]));
     }
 
+    // flux_hllc: euler_3d.jl:1417-1541, euler_2d.jl:1714-1816 (orientation
+    // form). Toro's HLL with contact: signal speeds from the Roe averages of
+    // the velocity normal to the face and of the enthalpy (Einfeldt), the
+    // star-state speed from the jump conditions; the flux of the upwind side,
+    // or that side's flux plus its signal speed times the jump to its star
+    // state. Operation order as in the reference.
+    void flux_hllc(const R *ul, const R *ur, int o, R *f) const {
+        const R rho_ll = ul[0], rho_rr = ur[0];
+        R v_ll[D], v_rr[D], ke_ll = 0, ke_rr = 0;
+        for (int d = 0; d < D; ++d) {
+            v_ll[d] = ul[1 + d] / rho_ll;
+            v_rr[d] = ur[1 + d] / rho_rr;
+            ke_ll += v_ll[d] * v_ll[d];
+            ke_rr += v_rr[d] * v_rr[d];
+        }
+        const R e_ll = ul[V - 1] / rho_ll, e_rr = ur[V - 1] / rho_rr;
+        const R p_ll = (gamma - 1) * (ul[V - 1] - R(0.5) * rho_ll * ke_ll);
+        const R p_rr = (gamma - 1) * (ur[V - 1] - R(0.5) * rho_rr * ke_rr);
+        const R c_ll = std::sqrt(gamma * p_ll / rho_ll), c_rr = std::sqrt(gamma * p_rr / rho_rr);
+        R f_ll[V], f_rr[V];
+        flux(ul, o, f_ll);
+        flux(ur, o, f_rr);
+        const R sr_ll = std::sqrt(rho_ll), sr_rr = std::sqrt(rho_rr);
+        const R sum_sr = sr_ll + sr_rr;
+        const R vel_L = v_ll[o], vel_R = v_rr[o];
+        const R vel_roe = (sr_ll * vel_L + sr_rr * vel_R) / sum_sr;
+        R mag = 0;
+        for (int d = 0; d < D; ++d) {
+            const R vr = sr_ll * v_ll[d] + sr_rr * v_rr[d];
+            mag += vr * vr;
+        }
+        mag = mag / (sum_sr * sum_sr);
+        const R H_ll = (ul[V - 1] + p_ll) / rho_ll, H_rr = (ur[V - 1] + p_rr) / rho_rr;
+        const R H_roe = (sr_ll * H_ll + sr_rr * H_rr) / sum_sr;
+        const R c_roe = std::sqrt((gamma - 1) * (H_roe - R(0.5) * mag));
+        const R Ssl = std::min(vel_L - c_ll, vel_roe - c_roe);
+        const R Ssr = std::max(vel_R + c_rr, vel_roe + c_roe);
+        const R sMu_L = Ssl - vel_L, sMu_R = Ssr - vel_R;
+        if (Ssl >= 0) {
+            for (int v = 0; v < V; ++v) f[v] = f_ll[v];
+        } else if (Ssr <= 0) {
+            for (int v = 0; v < V; ++v) f[v] = f_rr[v];
+        } else {
+            const R SStar = (p_rr - p_ll + rho_ll * vel_L * sMu_L - rho_rr * vel_R * sMu_R) /
+                            (rho_ll * sMu_L - rho_rr * sMu_R);
+            const bool left = Ssl <= 0 && 0 <= SStar;
+            const R *us = left ? ul : ur;
+            const R *fs = left ? f_ll : f_rr;
+            const R *vs = left ? v_ll : v_rr;
+            const R rho_s = left ? rho_ll : rho_rr, sMu = left ? sMu_L : sMu_R, S = left ? Ssl : Ssr;
+            const R vel = left ? vel_L : vel_R, p_s = left ? p_ll : p_rr, e_s = left ? e_ll : e_rr;
+            const R densStar = rho_s * sMu / (S - SStar);
+            const R enerStar = e_s + (SStar - vel) * (SStar + p_s / (rho_s * sMu));
+            R ustar[V];
+            ustar[0] = densStar;
+            for (int d = 0; d < D; ++d) ustar[1 + d] = densStar * (d == o ? SStar : vs[d]);
+            ustar[V - 1] = densStar * enerStar;
+            for (int v = 0; v < V; ++v) f[v] = fs[v] + S * (ustar[v] - us[v]);
+        }
+    }
+
     // max_abs_speeds: euler_3d.jl:1770-1775, euler_2d.jl:2025-2030
     void max_abs_speeds(const R *u, R *lam) const {
         R w[V];
@@ -504,7 +565,7 @@ template <int D, typename R = double> struct Euler {
 };
 
 // Numerical-flux selector shared by the volume (two-point), interface and FV paths.
-enum FluxKind { FLUX_LLF = 0, FLUX_LLF_NAIVE = 1, FLUX_RANOCHA = 2, FLUX_CENTRAL = 3 };
+enum FluxKind { FLUX_LLF = 0, FLUX_LLF_NAIVE = 1, FLUX_RANOCHA = 2, FLUX_CENTRAL = 3, FLUX_HLLC = 4 };
 
 template <int D, typename R>
 inline void num_flux(const Euler<D, R> &eq, int kind, const R *ul, const R *ur, int o, R *f) {
@@ -512,6 +573,7 @@ inline void num_flux(const Euler<D, R> &eq, int kind, const R *ul, const R *ur, 
         case FLUX_LLF: eq.flux_llf(ul, ur, o, false, f); break;
         case FLUX_LLF_NAIVE: eq.flux_llf(ul, ur, o, true, f); break;
         case FLUX_RANOCHA: eq.flux_ranocha(ul, ur, o, f); break;
+        case FLUX_HLLC: eq.flux_hllc(ul, ur, o, f); break;
         default: eq.flux_central(ul, ur, o, f); break;
     }
 }

@@ -6,7 +6,7 @@ import trixi_hip as th
 from oracle import Oracle
 
 FLUX = {"llf": th.flux_lax_friedrichs, "llf_naive": th.FluxLaxFriedrichs(th.max_abs_speed_naive),
-        "ranocha": th.flux_ranocha, "central": th.flux_central}
+        "ranocha": th.flux_ranocha, "central": th.flux_central, "hllc": th.flux_hllc}
 IC_ORACLE = {"constant": "constant", "convergence_test": "convergence_test", "weak_blast_wave": "weak_blast_wave",
              "isentropic_vortex": "isentropic_vortex", "kelvin_helmholtz": "kelvin_helmholtz",
              "fourier_modes": "fourier_modes", "density_pulse": "density_pulse", "blast_wave": "blast_wave",

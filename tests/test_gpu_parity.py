@@ -81,6 +81,16 @@ CASES = {
                                  ic_params=(1e-3, 0), volume_integral="hg", surface_flux="llf_naive",
                                  fv_flux="llf_naive", alpha_max=0.002, alpha_min=1e-4,
                                  refinement_patches=[((-1, -0.5), (1, 0.5))]), 2e-4),
+    # flux_hllc (pure finite volume): subcell, interface, boundary and mortar fluxes
+    "3d_p3_pure_fv_hllc_mms": (dict(MMS3, volume_integral="fv", surface_flux="hllc", fv_flux="hllc"), 1e-4),
+    "3d_p3_pure_fv_hllc_blast_mortar": (dict(BLAST, volume_integral="fv", surface_flux="hllc", fv_flux="hllc",
+                                             refinement_patches=[((-1, -1, -1), (1, 1, 1))]), 1e-5),
+    "2d_p3_pure_fv_hllc_mms_bounded": (dict(ndims=2, polydeg=3, level=3, cmin=(0, 0), cmax=(2, 2), ic="convergence_test",
+                                            source="convergence_test", volume_integral="fv", surface_flux="hllc",
+                                            fv_flux="hllc", periodicity=False), 2e-4),
+    "2d_p3_pure_fv_hllc_blast_slip": (dict(ndims=2, polydeg=3, level=3, cmin=(-1, -1), cmax=(1, 1),
+                                           ic="weak_blast_wave", volume_integral="fv", surface_flux="hllc",
+                                           fv_flux="hllc", periodicity=False, boundary_condition="slip_wall"), 2e-5),
 }
 
 
@@ -217,3 +227,4 @@ def test_rhs_parity_on_randomly_adapted_meshes(D, vi, per):
         else:
             assert np.max(np.abs(du - du_ref)) < fp32_abs_bound(semi, u)
         semi.close()
+

@@ -130,6 +130,36 @@ def test_create_rejects_bad_configuration_without_gpu():
     assert lib.trixi_hip_create(None, None, None, None) == _lib.EINVAL
 
 
+def test_hllc_only_with_pure_finite_volume(capfd):
+    """flux_hllc is compiled into the pure finite-volume kernels only: the Python
+    solver and trixi_hip_create both refuse it with any other volume integral
+    (before any device call), and the pure-FV configuration carries its code."""
+    eq = th.CompressibleEulerEquations3D(1.4)
+    basis = th.LobattoLegendreBasis(3)
+    with pytest.raises(ValueError, match="flux_hllc"):
+        th.DGSEM(basis, th.flux_hllc, th.VolumeIntegralFluxDifferencing(th.flux_ranocha))
+    ind = th.IndicatorHennemannGassner(eq, basis, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
+                                       variable=th.density_pressure)
+    with pytest.raises(ValueError, match="flux_hllc"):
+        th.DGSEM(basis, th.flux_lax_friedrichs,
+                 th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=th.flux_ranocha, volume_flux_fv=th.flux_hllc))
+    solver = th.DGSEM(basis, th.flux_hllc, th.VolumeIntegralPureLGLFiniteVolume(th.flux_hllc))
+    mesh = th.TreeMesh((0, 0, 0), (2, 2, 2), initial_refinement_level=1, periodicity=True)
+    semi = th.SemidiscretizationHyperbolic(mesh, eq, th.initial_condition_convergence_test, solver)
+    c = semi.config()
+    assert c.volume_integral == _lib.VI_PURE_FV and c.surface_flux == c.fv_flux == _lib.FLUX_HLLC
+    lib = _lib.load()
+    cfg = _lib.Config()
+    cfg.ndims, cfg.polydeg, cfg.nelements, cfg.ninterfaces = 3, 3, 8, 24
+    cfg.volume_integral, cfg.volume_flux, cfg.surface_flux, cfg.fv_flux = \
+        _lib.VI_FLUX_DIFFERENCING, _lib.FLUX_RANOCHA, _lib.FLUX_HLLC, _lib.FLUX_LLF
+    out = ctypes.c_void_p()
+    rc = lib.trixi_hip_create(ctypes.byref(cfg), ctypes.byref(_lib.Mesh()), ctypes.byref(_lib.Basis()),
+                              ctypes.byref(out))
+    assert rc == _lib.EINVAL and not out.value
+    assert "flux_hllc" in capfd.readouterr().err  # create reports its refusal on stderr
+
+
 def test_host_api_config_mapping():
     eq = th.CompressibleEulerEquations2D(1.4)
     basis = th.LobattoLegendreBasis(3)

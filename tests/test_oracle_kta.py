@@ -7,7 +7,8 @@ import pytest
 from conftest import make_oracle
 
 CASES = ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d", "khi_2d",
-         "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic", "mms_3d_mortar", "ec_2d_slip_wall"]
+         "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic", "mms_3d_mortar", "ec_2d_slip_wall",
+         "pure_fv_hllc_3d", "pure_fv_hllc_2d"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -49,6 +50,17 @@ def test_planted_boundary_mistake_is_caught(kta):
     o = make_oracle(spec)
     o.set_initial_condition(c["ic"], c["ic_params"])
     u, t, _ = o.solve(o.compute_coefficients(0.0), 0.0, 2.0, c["cfl"])
+    l2, _ = o.error_norms(u, t)
+    assert np.max(np.abs(l2 / np.array(c["l2"]) - 1)) > 1e-6
+
+
+def test_planted_hllc_mistake_is_caught(kta):
+    """Local Lax-Friedrichs subcell fluxes in place of flux_hllc move the pure
+    finite-volume errors far outside the KTA tolerance."""
+    c = kta["pure_fv_hllc_2d"]
+    o = make_oracle(dict(c["oracle"], fv_flux="llf"))
+    o.set_initial_condition(c["ic"], c["ic_params"])
+    u, t, _ = o.solve(o.compute_coefficients(0.0), 0.0, c["tspan"][1], c["cfl"])
     l2, _ = o.error_norms(u, t)
     assert np.max(np.abs(l2 / np.array(c["l2"]) - 1)) > 1e-6
 

@@ -163,3 +163,57 @@ def test_slip_wall_flux_solves_the_wall_riemann_problem(D):
     # at rest against the wall the wall pressure is the state's own
     u0 = np.array([1.3] + [0.0] * D + [2.0 / (gamma - 1)])
     assert O.slip_wall_flux(D, u0, 0, True, gamma)[1] == pytest.approx(2.0, rel=1e-15)
+
+
+def _mirror(u, o):
+    m = np.array(u, dtype=float)
+    m[1 + o] = -m[1 + o]
+    return m
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_flux_hllc_properties(D):
+    """flux_hllc (compressible_euler_3d.jl:flux_hllc, Einfeldt signal speeds):
+    consistency; upwinding of supersonic states (all signal speeds on one side:
+    the flux is that side's physical flux); exact resolution of a stationary
+    contact and shear wave (equal pressure, zero normal velocity: only the
+    pressure crosses the face); mirror symmetry f(R u_R, R u_L) = -R f(u_L, u_R)
+    with R the reflection of the normal momentum. A transposed operand, a wrong
+    sign of a signal speed or a star state taken from the wrong side fails one
+    of these."""
+    rng = np.random.default_rng(5)
+    gamma = 1.4
+    for _ in range(50):
+        ul, ur = rand_state(rng, D), rand_state(rng, D)
+        for o in range(D):
+            fu = O.flux(D, "physical", ul, None, o)
+            np.testing.assert_allclose(O.flux(D, "hllc", ul, ul, o), fu, rtol=1e-13, atol=1e-14)
+            f = O.flux(D, "hllc", ul, ur, o)
+            fm = O.flux(D, "hllc", _mirror(ur, o), _mirror(ul, o), o)
+            np.testing.assert_allclose(fm, -_mirror(f, o), rtol=1e-12, atol=1e-13)
+            # the two-state flux differs from both one-sided fluxes in general
+            assert not np.allclose(f, fu)
+            # supersonic to the right / left: upwind physical flux
+            for sgn in (1.0, -1.0):
+                a, b = ul.copy(), ur.copy()
+                for s in (a, b):
+                    v = s[1:1 + D] / s[0]
+                    v[o] = sgn * 4.0
+                    p = (gamma - 1) * (s[-1] - 0.5 * s[0] * (s[1:1 + D] / s[0]) @ (s[1:1 + D] / s[0]))
+                    s[1:1 + D] = s[0] * v
+                    s[-1] = p / (gamma - 1) + 0.5 * s[0] * v @ v
+                up = a if sgn > 0 else b
+                np.testing.assert_allclose(O.flux(D, "hllc", a, b, o), O.flux(D, "physical", up, None, o),
+                                           rtol=1e-14, atol=1e-14)
+    # stationary contact and shear wave: rho and tangential velocity jump, v_n = 0, p equal
+    for o in range(D):
+        p = 1.7
+        vl, vr = rng.uniform(-1, 1, D), rng.uniform(-1, 1, D)
+        vl[o] = vr[o] = 0.0
+        a = np.array([0.6, *(0.6 * vl), p / (gamma - 1) + 0.3 * vl @ vl])
+        b = np.array([2.1, *(2.1 * vr), p / (gamma - 1) + 1.05 * vr @ vr])
+        expect = np.zeros(D + 2)
+        expect[1 + o] = p
+        np.testing.assert_allclose(O.flux(D, "hllc", a, b, o), expect, atol=1e-14)
+        # LLF smears the same contact
+        assert abs(O.flux(D, "llf", a, b, o)[0]) > 1e-3

@@ -758,9 +758,9 @@ __global__ __launch_bounds__(256) void mortar_flux_kernel(const T *__restrict__ 
     for (int v = 0; v < V; ++v) usv[v] = us[v];
     const T *ul = large_left ? up : usv;
     const T *ur = large_left ? usv : up;
-    if (o == 0) num_flux_c<T, D, 0>(surface_flux, ul, ur, gamma, f);
-    else if (o == 1) num_flux_c<T, D, 1>(surface_flux, ul, ur, gamma, f);
-    else if constexpr (D == 3) num_flux_c<T, D, 2>(surface_flux, ul, ur, gamma, f);
+    if (o == 0) num_flux_c<T, D, 0, true>(surface_flux, ul, ur, gamma, f);
+    else if (o == 1) num_flux_c<T, D, 1, true>(surface_flux, ul, ur, gamma, f);
+    else if constexpr (D == 3) num_flux_c<T, D, 2, true>(surface_flux, ul, ur, gamma, f);
     T *dst = mflux + ((size_t)(slot0 + m * (S + 1) + q) * L + fn) * V;
 #pragma unroll
     for (int v = 0; v < V; ++v) {

@@ -20,7 +20,7 @@ namespace trixi_hip {
 
 enum VolumeKind { VOL_WEAK = 0, VOL_FD_RANOCHA = 1, VOL_FD_CENTRAL = 2, VOL_HG_RANOCHA = 3,
                   VOL_HG_CENTRAL = 4, VOL_PURE_FV = 5 };
-enum FluxKind { FLUX_LLF = 0, FLUX_LLF_NAIVE = 1, FLUX_RANOCHA = 2, FLUX_CENTRAL = 3 };
+enum FluxKind { FLUX_LLF = 0, FLUX_LLF_NAIVE = 1, FLUX_RANOCHA = 2, FLUX_CENTRAL = 3, FLUX_HLLC = 4 };
 
 constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
 
@@ -349,6 +349,64 @@ __device__ __forceinline__ void flux_llf_c(const ConsP<T, D> &l, const ConsP<T, 
   flux_llf_pc<T, D, O>(l, r, c_ll, c_rr, naive, f);
 }
 
+// flux_hllc (compressible_euler_3d.jl:1417-1541, 2D compressible_euler_2d.jl:
+// 1714-1816, orientation form): Toro's HLL with contact, signal speeds from
+// the Roe averages of the normal velocity and the enthalpy (Einfeldt), the
+// star-state speed from the jump conditions; the upwind side's flux, or that
+// side's flux plus its signal speed times the jump to its star state.
+// Quotients as products with refined reciprocals (FP64 to round-off of the
+// reference's divisions).
+template <typename T, int D, int O>
+__device__ __forceinline__ void flux_hllc_c(const ConsP<T, D> &l, const ConsP<T, D> &r, T gamma, T *f) {
+  constexpr int V = D + 2;
+  const T rho_ll = l.u[0], rho_rr = r.u[0];
+  const T c_ll = Prec<T>::sqrt_(gamma * l.p * l.irho), c_rr = Prec<T>::sqrt_(gamma * r.p * r.irho);
+  T f_ll[V], f_rr[V];
+  phys_flux<T, D, O>(l, f_ll);
+  phys_flux<T, D, O>(r, f_rr);
+  const T sr_ll = Prec<T>::sqrt_(rho_ll), sr_rr = Prec<T>::sqrt_(rho_rr);
+  const T sum_sr = sr_ll + sr_rr, isum = Prec<T>::rcp(sum_sr);
+  const T vel_L = l.v[O], vel_R = r.v[O];
+  const T vel_roe = (sr_ll * vel_L + sr_rr * vel_R) * isum;
+  T mag = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const T vr = sr_ll * l.v[d] + sr_rr * r.v[d];
+    mag += vr * vr;
+  }
+  mag = mag * (isum * isum);
+  const T H_ll = (l.u[V - 1] + l.p) * l.irho, H_rr = (r.u[V - 1] + r.p) * r.irho;
+  const T H_roe = (sr_ll * H_ll + sr_rr * H_rr) * isum;
+  const T c_roe = Prec<T>::sqrt_((gamma - T(1)) * (H_roe - T(0.5) * mag));
+  const T Ssl = tmin(vel_L - c_ll, vel_roe - c_roe);
+  const T Ssr = tmax(vel_R + c_rr, vel_roe + c_roe);
+  if (Ssl >= T(0)) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) f[v] = f_ll[v];
+  } else if (Ssr <= T(0)) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) f[v] = f_rr[v];
+  } else {
+    const T sMu_L = Ssl - vel_L, sMu_R = Ssr - vel_R;
+    const T SStar = (r.p - l.p + rho_ll * vel_L * sMu_L - rho_rr * vel_R * sMu_R) *
+                    Prec<T>::rcp(rho_ll * sMu_L - rho_rr * sMu_R);
+    const bool left = SStar >= T(0);  // Ssl <= 0 holds here
+    const ConsP<T, D> &s = left ? l : r;
+    const T *fs = left ? f_ll : f_rr;
+    const T sMu = left ? sMu_L : sMu_R, S = left ? Ssl : Ssr;
+    const T rho_s = s.u[0];
+    const T densStar = rho_s * sMu * Prec<T>::rcp(S - SStar);
+    const T enerStar = s.u[V - 1] * s.irho + (SStar - s.v[O]) * (SStar + s.p * Prec<T>::rcp(rho_s * sMu));
+    T ustar[V];
+    ustar[0] = densStar;
+#pragma unroll
+    for (int d = 0; d < D; ++d) ustar[1 + d] = densStar * (d == O ? SStar : s.v[d]);
+    ustar[V - 1] = densStar * enerStar;
+#pragma unroll
+    for (int v = 0; v < V; ++v) f[v] = fs[v] + S * (ustar[v] - s.u[v]);
+  }
+}
+
 // flux_ranocha from conservative states (surface / FV use).
 template <typename T, int D, int O>
 __device__ __forceinline__ void flux_ranocha_c(const T *ul, const T *ur, T gamma_m1, T *f) {
@@ -358,8 +416,11 @@ __device__ __forceinline__ void flux_ranocha_c(const T *ul, const T *ur, T gamma
   flux_ranocha_r<T, D, O>(a, b, T(1) / gamma_m1, f);
 }
 
-// Runtime-selected two-point flux of conservative states (surface and FV subcells).
-template <typename T, int D, int O>
+// Runtime-selected two-point flux of conservative states (surface and FV
+// subcells). flux_hllc is compiled in only where HLLC is true (the pure-FV
+// element kernels and the mortar fluxes): as one more run-time branch it would
+// raise the register count, and lower the occupancy, of every other variant.
+template <typename T, int D, int O, bool HLLC = false>
 __device__ __forceinline__ void num_flux_c(int kind, const T *ul, const T *ur, T gamma, T *f) {
   if (kind == FLUX_RANOCHA) {
     flux_ranocha_c<T, D, O>(ul, ur, gamma - T(1), f);
@@ -367,8 +428,17 @@ __device__ __forceinline__ void num_flux_c(int kind, const T *ul, const T *ur, T
     ConsP<T, D> l, r;
     consp_from_cons<T, D>(ul, gamma - T(1), l);
     consp_from_cons<T, D>(ur, gamma - T(1), r);
-    if (kind == FLUX_CENTRAL) flux_central_c<T, D, O>(l, r, f);
-    else flux_llf_c<T, D, O>(l, r, kind == FLUX_LLF_NAIVE, gamma, f);
+    if (kind == FLUX_CENTRAL) {
+      flux_central_c<T, D, O>(l, r, f);
+      return;
+    }
+    if constexpr (HLLC) {
+      if (kind == FLUX_HLLC) {
+        flux_hllc_c<T, D, O>(l, r, gamma, f);
+        return;
+      }
+    }
+    flux_llf_c<T, D, O>(l, r, kind == FLUX_LLF_NAIVE, gamma, f);
   }
 }
 

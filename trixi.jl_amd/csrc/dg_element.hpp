@@ -342,11 +342,11 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
       }
     }
     if (!done) {
-      num_flux_c<T, D, O>(a.surface_flux, c.ul_m, c.own0, gamma, F);  // -O face: element is the right state
+      num_flux_c<T, D, O, VOL == VOL_PURE_FV>(a.surface_flux, c.ul_m, c.own0, gamma, F);  // -O face: element is the right state
       given_flux(c.fm, F);
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[0][v] = tfma(-F[v], factor, acc[0][v]);
-      num_flux_c<T, D, O>(a.surface_flux, c.own1, c.ur_p, gamma, F);  // +O face: element is the left state
+      num_flux_c<T, D, O, VOL == VOL_PURE_FV>(a.surface_flux, c.own1, c.ur_p, gamma, F);  // +O face: element is the left state
       given_flux(c.fp, F);
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[n - 1][v] = tfma(F[v], factor, acc[n - 1][v]);
@@ -516,7 +516,7 @@ __device__ __forceinline__ void line_work(const ElemArgs<T> &a, const DGConst<T,
           T nxt[V];
 #pragma unroll
           for (int v = 0; v < V; ++v) nxt[v] = s_c[v * SZ + pq[i + 1]];
-          num_flux_c<T, D, O>(a.fv_flux, cur, nxt, gamma, fnext);
+          num_flux_c<T, D, O, VOL == VOL_PURE_FV>(a.fv_flux, cur, nxt, gamma, fnext);
 #pragma unroll
           for (int v = 0; v < V; ++v) cur[v] = nxt[v];
         } else {

@@ -815,8 +815,11 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
   ctx->fp32 = cfg->precision == TRIXI_HIP_FP32;
   ctx->vol = vol_kind(*cfg);
   if (ctx->vol < 0) return fail(TRIXI_HIP_EINVAL, "unsupported volume integral / volume flux");
-  if (cfg->surface_flux < 0 || cfg->surface_flux > 3 || cfg->fv_flux < 0 || cfg->fv_flux > 3)
+  if (cfg->surface_flux < 0 || cfg->surface_flux > TRIXI_HIP_FLUX_HLLC || cfg->fv_flux < 0 ||
+      cfg->fv_flux > TRIXI_HIP_FLUX_HLLC)
     return fail(TRIXI_HIP_EINVAL, "unsupported surface or FV flux");
+  if ((cfg->surface_flux == TRIXI_HIP_FLUX_HLLC || cfg->fv_flux == TRIXI_HIP_FLUX_HLLC) && ctx->vol != VOL_PURE_FV)
+    return fail(TRIXI_HIP_EINVAL, "flux_hllc is supported with VolumeIntegralPureLGLFiniteVolume only");
   const int n = ctx->n, D = ctx->D, E = ctx->E, I = ctx->I;
   // element neighbour table from the interface container (containers_3d.jl:229-279):
   // interface i with orientation o joins left (direction 2o+1 of left) and right (2o of right)

@@ -16,7 +16,7 @@ module TrixiHIP
 using Trixi
 using Trixi: TreeMesh, DGSEM, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
              VolumeIntegralWeakForm, VolumeIntegralFluxDifferencing, VolumeIntegralShockCapturingHGType,
-             VolumeIntegralPureLGLFiniteVolume, FluxLaxFriedrichs, flux_ranocha, flux_central,
+             VolumeIntegralPureLGLFiniteVolume, FluxLaxFriedrichs, flux_ranocha, flux_central, flux_hllc,
              max_abs_speed, max_abs_speed_naive, source_terms_convergence_test, SolutionAnalyzer,
              mesh_equations_solver_cache, nelements, ninterfaces, nboundaries, nmortars,
              BoundaryConditionDirichlet, boundary_condition_slip_wall, initial_condition_constant,
@@ -85,6 +85,7 @@ const DEVICE = Int32(1)
 flux_code(f::FluxLaxFriedrichs) = f.dissipation.max_abs_speed === max_abs_speed_naive ? Int32(1) : Int32(0)
 flux_code(::typeof(flux_ranocha)) = Int32(2)
 flux_code(::typeof(flux_central)) = Int32(3)
+flux_code(::typeof(flux_hllc)) = Int32(4)
 
 vi_codes(::VolumeIntegralWeakForm) = (Int32(0), Int32(2), Int32(0), nothing)
 vi_codes(v::VolumeIntegralFluxDifferencing) = (Int32(1), flux_code(v.volume_flux), Int32(0), nothing)

@@ -20,7 +20,7 @@ from .basis import LobattoLegendreBasis
 from .equations import (BoundaryConditionDirichlet, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
                         cons2cons, cons2prim, varnames,
                         FluxLaxFriedrichs, boundary_condition_periodic, boundary_condition_slip_wall,
-                        flux_central, flux_lax_friedrichs, flux_ranocha, fourier_modes,
+                        flux_central, flux_hllc, flux_lax_friedrichs, flux_ranocha, fourier_modes,
                         initial_condition_constant, initial_condition_convergence_test,
                         initial_condition_blast_wave, initial_condition_blob, initial_condition_density_pulse,
                         initial_condition_fourier_modes, initial_condition_isentropic_vortex,

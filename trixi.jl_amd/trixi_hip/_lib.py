@@ -28,6 +28,7 @@ FLUX_LLF = 0
 FLUX_LLF_NAIVE = 1
 FLUX_RANOCHA = 2
 FLUX_CENTRAL = 3
+FLUX_HLLC = 4
 
 BC_NONE = 0
 BC_DIRICHLET = 1

@@ -118,6 +118,7 @@ def FluxLaxFriedrichs(speed=max_abs_speed):
 flux_lax_friedrichs = FluxLaxFriedrichs()
 flux_ranocha = _Flux("flux_ranocha", _lib.FLUX_RANOCHA)
 flux_central = _Flux("flux_central", _lib.FLUX_CENTRAL)
+flux_hllc = _Flux("flux_hllc", _lib.FLUX_HLLC)  # surface and FV flux (compressible_euler_3d.jl:1417-1541)
 
 
 # ---------------------------------------------------------------------------

@@ -102,6 +102,13 @@ class DGSEM:
         self.basis = basis
         self.surface_flux = surface_flux
         self.volume_integral = volume_integral if volume_integral is not None else VolumeIntegralWeakForm()
+        # the HIP element kernels compile flux_hllc into the pure finite-volume
+        # variants only (as one more run-time branch it would cost every other
+        # variant registers and occupancy)
+        fluxes = (surface_flux, getattr(self.volume_integral, "volume_flux_fv", None))
+        if any(getattr(f, "code", None) == _lib.FLUX_HLLC for f in fluxes) and \
+                not isinstance(self.volume_integral, VolumeIntegralPureLGLFiniteVolume):
+            raise ValueError("flux_hllc is supported with VolumeIntegralPureLGLFiniteVolume only")
 
     def polydeg(self):
         return self.basis.polydeg
