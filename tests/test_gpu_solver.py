@@ -96,7 +96,8 @@ def run_kta(case, precision):
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d",
                                   "khi_2d", "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic",
-                                  "mms_3d_mortar", "ec_2d_slip_wall", "pure_fv_hllc_3d", "pure_fv_hllc_2d"])
+                                  "mms_3d_mortar", "ec_2d_slip_wall", "pure_fv_hllc_3d", "pure_fv_hllc_2d",
+                                  "blast_wave_pure_fv_2d", "density_pulse_3d"])
 def test_kta_trajectory_on_gpu_fp64(case):
     """The whole run (RK stages, dt, analysis) on the device reproduces the
     reference's stored L2/Linf errors."""
@@ -109,7 +110,7 @@ def test_kta_trajectory_on_gpu_fp64(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["mms_3d_weak", "ec_3d", "khi_2d"])
+@pytest.mark.parametrize("case", ["mms_3d_weak", "ec_3d", "khi_2d", "density_pulse_3d"])
 def test_kta_trajectory_on_gpu_fp32(case):
     c, l2, linf = run_kta(case, "fp32")
     np.testing.assert_allclose(l2, c["l2"], rtol=2e-4)

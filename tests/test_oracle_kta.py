@@ -8,7 +8,7 @@ from conftest import make_oracle
 
 CASES = ["mms_3d_weak", "mms_3d_split_central", "ec_3d", "ec_3d_constant", "shockcapturing_3d", "khi_2d",
          "vortex_2d", "vortex_2d_constant_one_step", "mms_2d_nonperiodic", "mms_3d_mortar", "ec_2d_slip_wall",
-         "pure_fv_hllc_3d", "pure_fv_hllc_2d"]
+         "pure_fv_hllc_3d", "pure_fv_hllc_2d", "blast_wave_pure_fv_2d", "density_pulse_3d"]
 
 
 @pytest.mark.parametrize("name", CASES)
