@@ -74,11 +74,18 @@ template <typename T> struct ElemArgs {
   // the equal-L2 run of the 3D MMS case bounds the total effect (density L2
   // errors of the FP32 and FP64 runs differ by 3.6e-7 after 100 steps).
   // b_dt = b_stage * tdev[1] (or b_stage without tdev); u32_out receives the
-  // FP32 shadow of u_reg_out (FP32 mode). u_reg_in == u_reg_out is allowed.
+  // FP32 shadow of u_reg_out (FP32 mode, may be null). u_reg_in == u_reg_out
+  // is allowed when the stage reads its state from u. With u64 != null (FP32
+  // stage on a mesh without boundaries and mortars) the stage reads its tile
+  // and the neighbour face states from the FP64 register u64 (= u_reg_in of
+  // every element, so u_reg_out must be another buffer), rounded to FP32 on
+  // load: the values the shadow would hold, bit for bit, without its 4 bytes
+  // of writes and reads per value.
   T *u_tmp;
   const double *u_reg_in;
   double *u_reg_out;
   float *u32_out;
+  const double *u64;
   double a_stage, b_stage;
   int first_stage;
   T gamma;
@@ -214,10 +221,23 @@ __device__ __forceinline__ void line_ids(const ElemArgs<T> &a, int e0, int ne, i
 // the ghost record of that boundary (face node = the lane's line index);
 // periodic meshes use the instantiation without that select.
 template <typename T, int D, int N, int EPB, bool BND>
-__device__ __forceinline__ void line_loads(const ElemArgs<T> &a, LineCtx<T, D + 2> &c, int o, int nbm, int nbp) {
+__device__ __forceinline__ void line_loads(const ElemArgs<T> &a, LineCtx<T, D + 2> &c, int o, int nbm, int nbp,
+                                           const double *u64) {
   using LY = Layout<D, N, EPB>;
   constexpr int n = N + 1, NN = LY::NN, V = D + 2, L = LY::L;
   const int st = o == 0 ? 1 : (o == 1 ? n : n * n);
+  if constexpr (sizeof(T) == 4 && !BND) {
+    if (u64) {  // FP32 stage reading the FP64 register (wave-uniform)
+      const double *vm = u64 + ((size_t)nbm * NN + c.base + (n - 1) * st) * V;
+      const double *vp = u64 + ((size_t)nbp * NN + c.base) * V;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        c.ul_m[v] = T(vm[v]);
+        c.ur_p[v] = T(vp[v]);
+      }
+      return;
+    }
+  }
   const T *un_m, *un_p;
   if constexpr (BND) {
     // a face slot (-nb - 1) below nbnd is a boundary (its ghost state), above a
@@ -731,10 +751,24 @@ void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
   // chunk indices are clamped to the tile's extent, so all lanes load (a
   // partial last tile is staged by the scalar loop below instead)
   const int nch = full ? TIO::NCH : max(1, ne * NN * V * (int)sizeof(T) / TIO::CB);
+  const double *u64 = STAGE && sizeof(T) == 4 ? a.u64 : nullptr;  // FP64 source of the FP32 stage, or null
   const C *csrc = reinterpret_cast<const C *>(a.u + (size_t)e0 * NN * V);
   C rc[RPT];
+  if (STAGE && sizeof(T) == 4 && u64) {
+    constexpr int WC = TIO::CB / (int)sizeof(T);
+    const double *src64 = u64 + (size_t)e0 * NN * V;
 #pragma unroll
-  for (int r = 0; r < RPT; ++r) rc[r] = csrc[min(tid + r * NT, nch - 1)];
+    for (int r = 0; r < RPT; ++r) {
+      const int ic = min(tid + r * NT, nch - 1);
+      T x[WC];
+#pragma unroll
+      for (int w = 0; w < WC; ++w) x[w] = T(src64[(size_t)ic * WC + w]);
+      __builtin_memcpy(&rc[r], x, sizeof(C));
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) rc[r] = csrc[min(tid + r * NT, nch - 1)];
+  }
   // element of this thread's source-table entry: the table is built by the
   // last EPB * 2n threads, away from wave 0, which has the most node work of
   // phases A and C (threads outside the table repeat the nearest entry)
@@ -744,7 +778,7 @@ void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
   double cen[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) cen[d] = a.centers[d + D * (e0 + sel)];
-  line_loads<T, D, N, EPB, BND>(a, lc, grp, nbm, nbp);
+  line_loads<T, D, N, EPB, BND>(a, lc, grp, nbm, nbp, u64);
 
   // ---- phase A: stage the tile and the per-element data into LDS ----
   {
@@ -755,7 +789,9 @@ void dg_element_kernel(ElemArgs<T> a, DGConst<T, N> k) {
     for (int r = 0; r < RPT; ++r) dst[min(tid + r * NT, nch - 1)] = rc[r];
     if (!full) {  // the bytes of a partial tile past its last whole chunk
       const T *src = a.u + (size_t)e0 * NN * V;
-      for (int i = nch * (TIO::CB / (int)sizeof(T)) + tid; i < ne * NN * V; i += NT) s_d[i] = src[i];
+      const double *src64 = u64 ? u64 + (size_t)e0 * NN * V : nullptr;
+      for (int i = nch * (TIO::CB / (int)sizeof(T)) + tid; i < ne * NN * V; i += NT)
+        s_d[i] = src64 ? T(src64[i]) : src[i];
     }
   }
   if constexpr (VT::HG) {

@@ -136,6 +136,10 @@ struct trixi_hip_ctx {
   void *d_utmp = nullptr;
   float *d_u32b[2] = {nullptr, nullptr};
   int par = 0;
+  // FP32 mode: the shadow d_u32b[par] lags the register d_ub[par] (the fused
+  // stages of a mesh without boundaries and mortars do not write it); readers
+  // of the shadow call sync_u32 first
+  bool u32_stale = false;
   void *d_du = nullptr;  // native precision
   // staging for trixi_hip_rhs (FP64 interface)
   double *d_hu = nullptr, *d_hdu = nullptr;
@@ -339,7 +343,7 @@ int free_err(int code, const char *msg, const char *detail = nullptr) {
 }
 
 // the current resident state: FP64 register u, FP32 shadow (FP32 mode), native
-double *cur_u64(trixi_hip_ctx *c) { return c->d_ub[c->fp32 ? 0 : c->par]; }
+double *cur_u64(trixi_hip_ctx *c) { return c->d_ub[c->par]; }
 float *cur_u32(trixi_hip_ctx *c) { return c->d_u32b[c->par]; }
 const void *native_u(trixi_hip_ctx *c) { return c->fp32 ? (const void *)cur_u32(c) : (const void *)cur_u64(c); }
 
@@ -349,6 +353,7 @@ struct StageOut {
   const double *u_reg_in = nullptr;
   double *u_reg_out = nullptr;
   float *u32_out = nullptr;
+  const double *u64 = nullptr;  // FP32 stage: read the state from this FP64 register
   double a_stage = 0, b_stage = 0;
   int first_stage = 0;
   double *speed_partial = nullptr;  // the last stage of a step: wave-speed partials of the new state
@@ -533,6 +538,13 @@ bool stage_fuses_indicator(const trixi_hip_ctx *ctx) {
   return ctx->fused_ind && is_hg(ctx) && stage_caps(ctx).ind2d;
 }
 
+// FP32 fused stages read the FP64 register directly (no shadow traffic) when
+// nothing reads the native state between stages: no boundary-state or mortar
+// kernel, and no stand-alone indicator before each stage
+bool stage_reads_u64(const trixi_hip_ctx *c) {
+  return c->fp32 && c->B + c->M == 0 && (!is_hg(c) || stage_fuses_indicator(c));
+}
+
 template <typename T>
 int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *tdev,
                  const StageOut *so = nullptr) {
@@ -622,6 +634,7 @@ int launch_rhs_T(trixi_hip_ctx *ctx, T *du, const T *u, double t, const double *
     a.u_reg_in = so->u_reg_in;
     a.u_reg_out = so->u_reg_out;
     a.u32_out = so->u32_out;
+    a.u64 = so->u64;
     a.a_stage = so->a_stage;
     a.b_stage = so->b_stage;
     a.first_stage = so->first_stage;
@@ -658,6 +671,14 @@ template <typename TO, typename TI> int convert(trixi_hip_ctx *ctx, TO *out, con
   return TRIXI_HIP_OK;
 }
 
+// the shadow from the register, where fused stages left it behind
+int sync_u32(trixi_hip_ctx *ctx) {
+  if (!ctx->fp32 || !ctx->u32_stale) return TRIXI_HIP_OK;
+  int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
+  if (!r) ctx->u32_stale = false;
+  return r;
+}
+
 int launch_update(trixi_hip_ctx *ctx, double a_stage, double b_stage, const double *tdev, int first) {
   size_t cnt = ctx->nvals;
   if (ctx->fp32)
@@ -681,15 +702,22 @@ int launch_update(trixi_hip_ctx *ctx, double a_stage, double b_stage, const doub
 int launch_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage, double t, const double *tdev, int first,
                  double *speed_partial = nullptr) {
   if (!ctx->elem_stage) {
-    int r = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), t, tdev);
+    int r = sync_u32(ctx);
+    if (!r) r = launch_rhs_native(ctx, ctx->d_du, native_u(ctx), t, tdev);
     ctx->alpha_fresh = false;
     return r ? r : launch_update(ctx, a_stage, b_stage, tdev, first);
+  }
+  if (is_hg(ctx) && !ctx->alpha_fresh) {  // the stage's indicator reads the native state
+    int r = sync_u32(ctx);
+    if (r) return r;
   }
   StageOut so;
   so.u_tmp = ctx->d_utmp;
   so.u_reg_in = cur_u64(ctx);
-  so.u_reg_out = ctx->fp32 ? ctx->d_ub[0] : ctx->d_ub[ctx->par ^ 1];
-  so.u32_out = ctx->fp32 ? ctx->d_u32b[ctx->par ^ 1] : nullptr;
+  const bool from64 = stage_reads_u64(ctx);
+  so.u_reg_out = ctx->d_ub[ctx->par ^ 1];
+  so.u32_out = ctx->fp32 && !from64 ? ctx->d_u32b[ctx->par ^ 1] : nullptr;
+  so.u64 = from64 ? so.u_reg_in : nullptr;
   so.a_stage = a_stage;
   so.b_stage = b_stage;
   so.first_stage = first;
@@ -699,6 +727,7 @@ int launch_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage, double t, c
                     : launch_rhs_T<double>(ctx, nullptr, (const double *)native_u(ctx), t, tdev, &so);
   if (r) return r;
   ctx->par ^= 1;
+  if (from64) ctx->u32_stale = true;
   ctx->alpha_fresh = so.ind_next;
   return TRIXI_HIP_OK;
 }
@@ -708,7 +737,9 @@ int launch_stage(trixi_hip_ctx *ctx, double a_stage, double b_stage, double t, c
 // are computed here when no fused stage left them.
 int refresh_alpha(trixi_hip_ctx *ctx) {
   if (!stage_fuses_indicator(ctx) || ctx->alpha_fresh) return TRIXI_HIP_OK;
-  int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, (const float *)native_u(ctx), ctx->d_alpha_st[ctx->par])
+  int r = sync_u32(ctx);
+  if (r) return r;
+  r = ctx->fp32 ? launch_indicator_raw<float>(ctx, (const float *)native_u(ctx), ctx->d_alpha_st[ctx->par])
                     : launch_indicator_raw<double>(ctx, (const double *)native_u(ctx), ctx->d_alpha_st[ctx->par]);
   if (r) return r;
   ctx->alpha_fresh = true;
@@ -971,7 +1002,7 @@ int trixi_hip_create(const trixi_hip_config *cfg, const trixi_hip_mesh *mesh,
             dev_alloc(ctx, &ctx->d_invj, E * 8) == hipSuccess &&
             dev_alloc(ctx, &ctx->d_centers, (size_t)D * E * 8) == hipSuccess &&
             dev_alloc(ctx, &ctx->d_ub[0], nv * 8) == hipSuccess && dev_alloc(ctx, &ctx->d_utmp, nv * tsz) == hipSuccess &&
-            (ctx->fp32 || dev_alloc(ctx, &ctx->d_ub[1], nv * 8) == hipSuccess) &&
+            dev_alloc(ctx, &ctx->d_ub[1], nv * 8) == hipSuccess &&
             dev_alloc(ctx, &ctx->d_du, nv * tsz) == hipSuccess &&
             dev_alloc(ctx, &ctx->d_alpha_raw, E * 8) == hipSuccess && dev_alloc(ctx, &ctx->d_alpha, E * 8) == hipSuccess &&
             dev_alloc(ctx, &ctx->d_alpha_st[0], E * 8) == hipSuccess &&
@@ -1082,6 +1113,7 @@ int trixi_hip_upload_u(trixi_hip_ctx *ctx, const double *u_host) {
   if (ctx->fp32) {
     int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
     if (r) return r;
+    ctx->u32_stale = false;
   }
   HIP_TRY(host_sync(ctx));
   return TRIXI_HIP_OK;
@@ -1101,6 +1133,7 @@ int trixi_hip_set_u_device(trixi_hip_ctx *ctx, const double *u_dev) {
   if (ctx->fp32) {
     int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
     if (r) return r;
+    ctx->u32_stale = false;
   }
   return TRIXI_HIP_OK;
 }
@@ -1188,7 +1221,10 @@ int trixi_hip_rk_step(trixi_hip_ctx *ctx, const double *a, const double *b, cons
     if (r) return r;
   }
   HIP_TRY(hipGraphLaunch(x, ctx->stream));
-  if (ctx->elem_stage) ctx->par ^= (nstages & 1);
+  if (ctx->elem_stage) {
+    ctx->par ^= (nstages & 1);
+    if (stage_reads_u64(ctx)) ctx->u32_stale = true;
+  }
   return TRIXI_HIP_OK;
 }
 
@@ -1511,6 +1547,7 @@ int trixi_hip_solve(trixi_hip_ctx *ctx, const double *a, const double *b, const 
       if ((r = step_graph(&x))) return r;
       HIP_TRY(hipGraphLaunch(x, ctx->stream));
       ctx->par ^= flip;
+      if (ctx->elem_stage && stage_reads_u64(ctx)) ctx->u32_stale = true;
     }
     launched += batch;
     recorded = every > 0 && ((int)h[4] + batch) % every == 0;
@@ -1537,9 +1574,11 @@ int trixi_hip_indicator(trixi_hip_ctx *ctx, int mode, double *alpha_host) {
   if (!ctx || mode < TRIXI_HIP_IND_HG || mode > TRIXI_HIP_IND_MLP_WITH_FALLBACK) return TRIXI_HIP_EINVAL;
   if (mode != TRIXI_HIP_IND_HG && !ctx->mlp_ready)
     return set_err(ctx, TRIXI_HIP_ESTATE, "the learned indicator has no parameters (trixi_hip_mlp_set_params)");
+  int r = sync_u32(ctx);
+  if (r) return r;
   const int saved = ctx->ind_mode;
   ctx->ind_mode = mode;
-  int r = ctx->fp32 ? launch_indicator_raw<float>(ctx, cur_u32(ctx), ctx->d_alpha_raw)
+  r = ctx->fp32 ? launch_indicator_raw<float>(ctx, cur_u32(ctx), ctx->d_alpha_raw)
                     : launch_indicator_raw<double>(ctx, cur_u64(ctx), ctx->d_alpha_raw);
   ctx->ind_mode = saved;
   if (r) return r;
@@ -1633,6 +1672,7 @@ int trixi_hip_mlp_features(trixi_hip_ctx *ctx, float *features_host, double *alp
   if (!ctx) return TRIXI_HIP_EINVAL;
   int r = mlp_check(ctx);
   if (r) return r;
+  if ((r = sync_u32(ctx))) return r;
   r = ctx->fp32 ? launch_hg<float>(ctx, cur_u32(ctx), ctx->d_alpha_hg, ctx->d_feat)
                 : launch_hg<double>(ctx, cur_u64(ctx), ctx->d_alpha_hg, ctx->d_feat);  // no network: HG factors
   if (r) return r;
@@ -1787,6 +1827,7 @@ int trixi_hip_transfer(trixi_hip_ctx *to, trixi_hip_ctx *from, const int32_t *ki
     });
     if (!ok || hipGetLastError() != hipSuccess) r = set_err(to, TRIXI_HIP_EHIP, "transfer launch failed");
     if (!r && to->fp32) r = convert<float, double>(to, cur_u32(to), cur_u64(to), to->nvals);
+    if (!r) to->u32_stale = false;
   }
   if (host_sync(to) != hipSuccess && !r) r = set_err(to, TRIXI_HIP_EHIP, "transfer failed");
   dev_release(to, d_tmp);
@@ -1825,6 +1866,7 @@ int trixi_hip_compute_coefficients(trixi_hip_ctx *ctx, double t) {
   if (ctx->fp32) {
     int r = convert<float, double>(ctx, cur_u32(ctx), cur_u64(ctx), ctx->nvals);
     if (r) return r;
+    ctx->u32_stale = false;
   }
   ctx->alpha_fresh = false;
   return TRIXI_HIP_OK;
@@ -1945,7 +1987,10 @@ int trixi_hip_autotune(trixi_hip_ctx *ctx, int repetitions, trixi_hip_tile_timin
   // as first stages with a = b = 0, which leaves the state unchanged (u_tmp =
   // du, u = u + 0 u_tmp) and, over an even number of launches, in its buffer.
   auto launch = [&](const Cand &c) -> int {
-    if (c.stage == 0) return launch_rhs_native(ctx, ctx->d_du, native_u(ctx), 0.0, nullptr);
+    if (c.stage == 0) {
+      int r0 = sync_u32(ctx);
+      return r0 ? r0 : launch_rhs_native(ctx, ctx->d_du, native_u(ctx), 0.0, nullptr);
+    }
     return launch_stage(ctx, 0.0, 0.0, 0.0, nullptr, 1);
   };
   int r = TRIXI_HIP_OK;
