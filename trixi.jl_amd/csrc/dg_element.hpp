@@ -76,7 +76,7 @@ template <typename T> struct ElemArgs {
   // b_dt = b_stage * tdev[1] (or b_stage without tdev); u32_out receives the
   // FP32 shadow of u_reg_out (FP32 mode, may be null). u_reg_in == u_reg_out
   // is allowed when the stage reads its state from u. With u64 != null (FP32
-  // stage on a mesh without boundaries and mortars) the stage reads its tile
+  // stage on a mesh without mortars and slip walls) the stage reads its tile
   // and the neighbour face states from the FP64 register u64 (= u_reg_in of
   // every element, so u_reg_out must be another buffer), rounded to FP32 on
   // load: the values the shadow would hold, bit for bit, without its 4 bytes
@@ -226,14 +226,31 @@ __device__ __forceinline__ void line_loads(const ElemArgs<T> &a, LineCtx<T, D + 
   using LY = Layout<D, N, EPB>;
   constexpr int n = N + 1, NN = LY::NN, V = D + 2, L = LY::L;
   const int st = o == 0 ? 1 : (o == 1 ? n : n * n);
-  if constexpr (sizeof(T) == 4 && !BND) {
+  if constexpr (sizeof(T) == 4) {
     if (u64) {  // FP32 stage reading the FP64 register (wave-uniform)
-      const double *vm = u64 + ((size_t)nbm * NN + c.base + (n - 1) * st) * V;
-      const double *vp = u64 + ((size_t)nbp * NN + c.base) * V;
+      // With BND only boundary faces (no mortar or slip-wall slots) occur
+      // here: their outer state is the ghost record, loaded in FP32 beside the
+      // (clamped, in-bounds) register load and selected per lane.
+      const bool im = !BND || nbm >= 0, ip = !BND || nbp >= 0;
+      const double *vm = u64 + ((size_t)(im ? nbm : 0) * NN + c.base + (n - 1) * st) * V;
+      const double *vp = u64 + ((size_t)(ip ? nbp : 0) * NN + c.base) * V;
+      if constexpr (BND) {
+        const T *gm = a.ghost + ((size_t)(im ? 0 : -nbm - 1) * L + c.line) * V;
+        const T *gp = a.ghost + ((size_t)(ip ? 0 : -nbp - 1) * L + c.line) * V;
+        c.fm = -1;
+        c.fp = -1;
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        c.ul_m[v] = T(vm[v]);
-        c.ur_p[v] = T(vp[v]);
+        for (int v = 0; v < V; ++v) {
+          const T xm = T(vm[v]), gmv = gm[v], xp = T(vp[v]), gpv = gp[v];
+          c.ul_m[v] = im ? xm : gmv;
+          c.ur_p[v] = ip ? xp : gpv;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          c.ul_m[v] = T(vm[v]);
+          c.ur_p[v] = T(vp[v]);
+        }
       }
       return;
     }

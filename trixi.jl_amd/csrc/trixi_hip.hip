@@ -539,10 +539,11 @@ bool stage_fuses_indicator(const trixi_hip_ctx *ctx) {
 }
 
 // FP32 fused stages read the FP64 register directly (no shadow traffic) when
-// nothing reads the native state between stages: no boundary-state or mortar
-// kernel, and no stand-alone indicator before each stage
+// nothing reads the native state between stages: no mortar or slip-wall flux
+// kernel (Dirichlet outer states come from the boundary function), and no
+// stand-alone indicator before each stage
 bool stage_reads_u64(const trixi_hip_ctx *c) {
-  return c->fp32 && c->B + c->M == 0 && (!is_hg(c) || stage_fuses_indicator(c));
+  return c->fp32 && c->M == 0 && !c->slip && (!is_hg(c) || stage_fuses_indicator(c));
 }
 
 template <typename T>
